@@ -1,0 +1,187 @@
+"""Benchmark: 2-D decaying turbulence, 1024x1024 RealFourier, RK443, dt=1e-3.
+
+One step = one full RK443 timestep (4 dealiased RHS evaluations + stage updates)
+of the vorticity/streamfunction system on the GPU, state resident in HBM.
+Reports grid-point steps/s for the FP32 path (value) and the FP64 path, the
+FP32-vs-FP64 error, the dominant kernel's roofline figure measured live with HIP
+events, and the FP64 oracle timed on the host cores on a bounded sample.
+
+Usage: python bench.py [--gpus 1] [--steps 1000] [--warmup 20] [--no-cpu]
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "tarang.jl_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK_GBS = 8000.0
+N, NU, DT = 1024, 1e-4, 1e-3
+WORKLOAD = "2-D decaying turbulence, vorticity-streamfunction, 1024x1024 RealFourier, 3/2 dealiasing, nu=1e-4, RK443, dt=1e-3"
+KERNELS = ["k_ybwd (y-backward: multipliers, padding, inverse y FFT M=1536)",
+           "k_xpass (inverse x FFTs, u.grad(q) product, forward real FFT M=1536)",
+           "k_yfwd (forward y FFT M=1536, truncation, Nyquist folds)",
+           "k_rk_stage (IMEX stage / final update)"]
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--profile-steps", type=int, default=20)
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (no collectives); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from tarang_amd import _lib, configs
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    dev = torch.device("cuda", 0)
+    log(f"building initial condition {N}x{N}")
+    q0 = configs.turbulence_2d(N, seed=42, kpeak=10.0, energy=0.5)
+    C0 = np.fft.fft(np.fft.rfft(q0, axis=0), axis=1)  # reference layout (kx, ky), unnormalised
+    E0, Z0 = configs.energy_enstrophy_from_coeff(C0, N)
+
+    ctx = ctypes.c_void_p()
+    _lib.call("tarang_ctx_create", 0, ctypes.byref(ctx))
+
+    def make(dtype_code):
+        d = _lib.FivpDesc()
+        d.ndim = 2
+        d.n[0] = d.n[1] = N
+        d.L[0] = d.L[1] = 2 * math.pi
+        d.dtype = dtype_code
+        d.problem = 1
+        d.timestepper = 1  # RK443
+        d.nu = NU
+        d.dt = DT
+        h = ctypes.c_void_p()
+        _lib.call("tarang_fivp_create", ctx, ctypes.byref(d), ctypes.byref(h))
+        return h
+
+    results = {}
+    finals = {}
+    solvers = {}
+    for name, code, ct in (("f64", _lib.F64, torch.complex128), ("f32", _lib.F32, torch.complex64)):
+        s = make(code)
+        solvers[name] = s
+        Cd = torch.as_tensor(np.ascontiguousarray(C0.T), device=dev).to(ct).contiguous()
+        torch.cuda.synchronize()
+        _lib.call("tarang_fivp_set_state", s, ctypes.c_void_p(Cd.data_ptr()))
+        log(f"{name}: warmup {a.warmup} steps")
+        _lib.call("tarang_fivp_step", s, a.warmup)
+        _lib.call("tarang_fivp_set_state", s, ctypes.c_void_p(Cd.data_ptr()))  # timed run starts from the IC
+        _lib.call("tarang_ctx_sync", ctx)
+        torch.cuda.synchronize()
+        log(f"{name}: timing {a.steps} steps")
+        t0 = time.perf_counter()
+        _lib.call("tarang_fivp_step", s, a.steps)
+        _lib.call("tarang_ctx_sync", ctx)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ms = (t1 - t0) * 1e3 / a.steps
+        results[name] = ms
+        out = torch.zeros_like(Cd)
+        _lib.call("tarang_fivp_get_state", s, ctypes.c_void_p(out.data_ptr()))
+        _lib.call("tarang_ctx_sync", ctx)
+        finals[name] = out.cpu().numpy().T.astype(np.complex128)
+        log(f"{name}: {ms:.4f} ms/step, {N * N / (ms * 1e-3):.4e} grid-point steps/s")
+
+    Ef = {k: configs.energy_enstrophy_from_coeff(v, N) for k, v in finals.items()}
+    err_state = float(np.abs(finals["f32"] - finals["f64"]).max() / np.abs(finals["f64"]).max())
+    err_E = abs(Ef["f32"][0] - Ef["f64"][0]) / Ef["f64"][0]
+    err_Z = abs(Ef["f32"][1] - Ef["f64"][1]) / Ef["f64"][1]
+
+    # live per-kernel durations (HIP events on the library stream) for the FP32 path
+    log(f"profiling {a.profile_steps} eager steps with HIP events")
+    avg = (ctypes.c_double * 4)()
+    cnt = (ctypes.c_long * 4)()
+    s32 = solvers["f32"]
+    _lib.call("tarang_fivp_profile", s32, a.profile_steps, avg, cnt)
+    yb, xp, yf = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    _lib.call("tarang_fivp_kernel_bytes", s32, ctypes.byref(yb), ctypes.byref(xp), ctypes.byref(yf))
+    nbytes = [yb.value, xp.value, yf.value, None]
+    per_step = [avg[k] * cnt[k] / a.profile_steps for k in range(4)]
+    dom = max(range(3), key=lambda k: per_step[k])
+    achieved = nbytes[dom] / (avg[dom] * 1e-3) / 1e9
+    kernels = {KERNELS[k]: {"avg_ms": avg[k], "launches_per_step": cnt[k] / a.profile_steps,
+                            "algorithmic_bytes": nbytes[k],
+                            "GBps": (nbytes[k] / (avg[k] * 1e-3) / 1e9) if nbytes[k] and avg[k] else None}
+               for k in range(4)}
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_turbulence_1024_f32.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            tr = json.load(f)
+        traffic = tr.get(["k_ybwd", "k_xpass", "k_yfwd"][dom])
+
+    for s in solvers.values():
+        _lib.call("tarang_fivp_destroy", s)
+    _lib.call("tarang_ctx_destroy", ctx)
+
+    cpu = None
+    if not a.no_cpu:
+        from oracle import oracle as O
+        threads = min(os.cpu_count() or 1, 16)
+        O.set_threads(threads)
+        d = O.domain([0, 0], [N, N])
+        p = O.FourierIVP(d, O.PROB_TURB2D, "RK443", NU, DT)
+        p.set_coeff(C0)
+        log(f"cpu baseline: oracle, {a.cpu_steps} RK443 steps on {threads} threads")
+        t0 = time.perf_counter()
+        p.step(a.cpu_steps)
+        el = time.perf_counter() - t0
+        cpu = {"value": N * N * a.cpu_steps / el, "unit": "grid-point steps/s", "cores": threads, "kind": "port",
+               "sample": f"{a.cpu_steps} RK443 steps of the same 1024^2 workload from the same initial condition, "
+                         f"FP64 oracle (C, OpenMP over transform lines), {el:.2f} s"}
+
+    v32 = N * N / (results["f32"] * 1e-3)
+    v64 = N * N / (results["f64"] * 1e-3)
+    line = {
+        "metric": "grid-point steps/s, FP32+refine vs FP64 at <=1e-6 rel err; p50/p99 latency",
+        "value": v32,
+        "unit": "grid-point steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": results["f32"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded random-phase vorticity, E(k)~k^4 exp(-2(k/10)^2), E=0.5)",
+        "config": {"workload": WORKLOAD, "grid": [N, N], "timestepper": "RK443", "dt": DT, "nu": NU,
+                   "parallelism": "single GPU"},
+        "fp64": {"value": v64, "ms_per_step": results["f64"]},
+        "speedup_fp32_vs_fp64": v32 / v64,
+        "fp32_vs_fp64_rel_err": {"state_maxnorm": err_state, "energy": err_E, "enstrophy": err_Z,
+                                 "after_steps": a.steps},
+        "diagnostics": {"E0": E0, "Z0": Z0, "E_final_f64": Ef["f64"][0], "Z_final_f64": Ef["f64"][1]},
+        "roofline": {"bound": "hbm", "kernel": KERNELS[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "kernels_f32": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

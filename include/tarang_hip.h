@@ -1,0 +1,137 @@
+/*
+ * tarang_hip.h -- C ABI of libtarang_hip.so, the MI355X (gfx950) backend for
+ * Tarang.jl's pseudo-spectral timestep hot path.
+ *
+ * Every function returns an int status (0 = ok, see TARANG_* codes) and leaves
+ * a thread-local message readable with tarang_last_error().  The library owns
+ * device memory it allocates; field data crosses the boundary as plain device
+ * pointers (void*) and sizes.  One context = one HIP stream; a context is not
+ * re-entrant (one transforming task per device, as the reference requires:
+ * ext/cuda/cheb_deriv.jl:27-29).  Nothing ever falls back to the CPU: an
+ * unsupported case returns TARANG_ERR_INVALID with the reason
+ * (src/core/transforms/transform_gpu.jl:151-175).
+ *
+ * Layout: the reference's column-major layout.  Grid data is real
+ * (n0, n1, n2) with axis 0 fastest; coefficient data is complex interleaved
+ * (n0/2+1, n1, n2) -- rfft on axis 0, FFT order on later RealFourier axes,
+ * Chebyshev degree 0..N-1 on ChebyshevT axes
+ * (src/core/transforms/transform_layout.jl:87-100).
+ */
+#ifndef TARANG_HIP_H
+#define TARANG_HIP_H
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TARANG_OK 0
+#define TARANG_ERR_INVALID 1
+#define TARANG_ERR_HIP 2
+#define TARANG_ERR_SINGULAR 3
+#define TARANG_ERR_CAPACITY 4
+#define TARANG_ERR_STATE 5
+
+#define TARANG_F32 0
+#define TARANG_F64 1
+
+#define TARANG_REALFOURIER 0
+#define TARANG_CHEBYSHEVT 1
+
+typedef struct tarang_ctx_s* tarang_ctx_t;
+typedef struct tarang_plan_s* tarang_plan_t;
+typedef struct tarang_fivp_s* tarang_fivp_t;
+
+/* ------------------------------------------------------------------------ */
+/* Context and memory.  Replaces the device hooks of the architecture layer:
+ * Tarang._gpu_device / device(::GPU) / array_type / on_architecture
+ * (src/core/architectures.jl:107-131). */
+const char* tarang_last_error(void);
+int tarang_version(int* major, int* minor);
+int tarang_ctx_create(int device, tarang_ctx_t* out);
+int tarang_ctx_destroy(tarang_ctx_t ctx);
+/* adopt a caller stream (e.g. the framework's current stream); NULL restores the context's own */
+int tarang_ctx_set_stream(tarang_ctx_t ctx, void* hip_stream);
+int tarang_ctx_sync(tarang_ctx_t ctx);
+int tarang_buf_alloc(tarang_ctx_t ctx, size_t bytes, void** dptr);
+int tarang_buf_free(tarang_ctx_t ctx, void* dptr);
+int tarang_h2d(tarang_ctx_t ctx, void* dst, const void* src, size_t bytes);
+int tarang_d2h(tarang_ctx_t ctx, void* dst, const void* src, size_t bytes);
+
+/* ------------------------------------------------------------------------ */
+/* Domain transform plans.  Replaces the GPU transform backends
+ *   Tarang._gpu_forward_transform_backend!(::GPU, field)   (src/core/transforms/transform_gpu.jl:196-204)
+ *   Tarang._gpu_backward_transform_backend!(::GPU, field)  (ext/cuda/transforms.jl:234-237)
+ * that forward_transform!/backward_transform! call (transform_gpu.jl:220-258). */
+typedef struct {
+    int ndim;          /* 1..3 */
+    int kind[3];       /* TARANG_REALFOURIER / TARANG_CHEBYSHEVT; axis 0 must be RealFourier */
+    int n[3];          /* grid sizes */
+    double lo[3];      /* basis bounds */
+    double hi[3];
+    double dealias[3]; /* per-basis dealias factor (3/2 or <= 1) */
+    int dtype;         /* TARANG_F32 / TARANG_F64 (Distributor dtype) */
+} tarang_domain_desc;
+
+int tarang_plan_create(tarang_ctx_t ctx, const tarang_domain_desc* desc, tarang_plan_t* out);
+int tarang_plan_destroy(tarang_plan_t plan);
+/* coefficient shape (n0/2+1, n1, n2) and whether it is complex (always 1 here) */
+int tarang_plan_coeff_shape(tarang_plan_t plan, int* shape3, int* is_complex);
+/* grid (device, real) -> coefficients (device, complex); forward is unnormalised */
+int tarang_forward_transform(tarang_plan_t plan, const void* grid, void* coeff);
+/* coefficients -> grid; backward is normalised by 1/N per axis; coeff is not modified */
+int tarang_backward_transform(tarang_plan_t plan, const void* coeff, void* grid);
+
+/* 3/2-rule padded product of two grid fields, result on the base grid.
+ * Replaces evaluate_padded_multiply and its GPU pieces _pad_spectral!,
+ * _truncate_spectral!, gpu_multiply_fields! (src/core/nonlinear/nonlinear_padding.jl:359-437,
+ * ext/cuda/utils.jl:560,806,871). */
+int tarang_padded_multiply(tarang_plan_t plan, const void* grid_a, const void* grid_b, void* grid_out);
+
+/* ------------------------------------------------------------------------ */
+/* IVPs on RealFourier domains with a diagonal implicit operator nu*|k|^2.
+ * Replaces InitialValueSolver + step!/run! for these equation families
+ * (src/core/solvers/solver_types.jl:468-553, src/core/solvers/solver_stepping.jl:24-38,115-206)
+ * and the diagonal-IMEX stage loop (src/core/timesteppers/step_diagonal_imex.jl:183-437):
+ *   problem 0  dt(T) - nu*lap(T) = 0
+ *   problem 1  dt(q) - nu*lap(q) = -u@grad(q), lap(psi) + tau_psi - q = 0,
+ *              u - skew(grad(psi)) = 0, integ(psi) = 0          (2-D)
+ *   problem 2  dt(u) - nu*lap(u) = -u*dx(u)                      (1-D)
+ * timestepper 0 RK222, 1 RK443, 2 SBDF2, 3 RK111, 4 SBDF1 (src/core/timesteppers/types.jl). */
+typedef struct {
+    int ndim;       /* 1 or 2 */
+    int n[2];
+    double L[2];    /* periods (bounds hi - lo) */
+    int dtype;      /* TARANG_F32 / TARANG_F64 */
+    int problem;
+    int timestepper;
+    double nu;
+    double dt;
+} tarang_fivp_desc;
+
+int tarang_fivp_create(tarang_ctx_t ctx, const tarang_fivp_desc* desc, tarang_fivp_t* out);
+int tarang_fivp_destroy(tarang_fivp_t s);
+/* state = coefficient array of the evolved field (device pointer, reference layout) */
+int tarang_fivp_set_state(tarang_fivp_t s, const void* coeff);
+int tarang_fivp_get_state(tarang_fivp_t s, void* coeff);
+/* one right-hand-side evaluation F(coeff) (evaluate_rhs, src/core/timesteppers/state_utils.jl:261-274) */
+int tarang_fivp_rhs(tarang_fivp_t s, const void* coeff, void* F);
+/* algebraic refresh of the vorticity problem (src/core/timesteppers/state_utils.jl:843-877,
+ * 1059-1108): psi = -q/|k|^2, u = -dy psi, v = dx psi, as coefficients; any output may be NULL */
+int tarang_fivp_aux(tarang_fivp_t s, const void* q, void* psi, void* u, void* v);
+/* advance nsteps (step!); use_graph != 0 replays each step as one captured hipGraph */
+int tarang_fivp_step(tarang_fivp_t s, int nsteps);
+int tarang_fivp_set_graph(tarang_fivp_t s, int use_graph);
+int tarang_fivp_set_dt(tarang_fivp_t s, double dt);
+int tarang_fivp_time(tarang_fivp_t s, double* t, long* iteration);
+/* advance nsteps eagerly with HIP events around each kernel launch on the context stream;
+ * returns the average milliseconds per launch of [y-backward, x-pass, y-forward, stage update]
+ * and the launch counts (counts4 may be NULL).  Measurement support; no reference counterpart. */
+int tarang_fivp_profile(tarang_fivp_t s, int nsteps, double* avg_ms4, long* counts4);
+/* algorithmic bytes per launch of the three RHS kernels (y-backward, x-pass, y-forward) */
+int tarang_fivp_kernel_bytes(tarang_fivp_t s, double* ybwd, double* xpass, double* yfwd);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TARANG_HIP_H */

@@ -1,0 +1,206 @@
+"""ctypes binding of the FP64 oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.  Arrays follow the
+reference's column-major layout; numpy arrays passed here are Fortran-ordered
+views of shape (n0, n1, ...) so that axis 0 is contiguous.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+REALFOURIER, CHEBYSHEVT = 0, 1
+PROB_DIFFUSION, PROB_TURB2D, PROB_BURGERS1D = 0, 1, 2
+TS = {"RK222": 0, "RK443": 1, "SBDF2": 2, "RK111": 3, "SBDF1": 4}
+
+
+class Domain(ctypes.Structure):
+    _fields_ = [
+        ("ndim", ctypes.c_int),
+        ("kind", ctypes.c_int * 3),
+        ("n", ctypes.c_int * 3),
+        ("lo", ctypes.c_double * 3),
+        ("hi", ctypes.c_double * 3),
+        ("dealias", ctypes.c_double * 3),
+    ]
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        P = ctypes.POINTER
+        dp = P(ctypes.c_double)
+        L.orc_fft.argtypes = [ctypes.c_int, dp, ctypes.c_int]
+        L.orc_rfft.argtypes = [ctypes.c_int, dp, dp]
+        L.orc_irfft.argtypes = [ctypes.c_int, dp, dp]
+        L.orc_dct1.argtypes = [ctypes.c_int, dp, dp]
+        L.orc_coeff_shape.argtypes = [P(Domain), P(ctypes.c_int)]
+        L.orc_forward_transform.argtypes = [P(Domain), dp, dp]
+        L.orc_backward_transform.argtypes = [P(Domain), dp, dp]
+        L.orc_padded_multiply.argtypes = [P(Domain), dp, dp, dp]
+        L.orc_padded_size.argtypes = [ctypes.c_int, ctypes.c_double]
+        L.orc_padded_size.restype = ctypes.c_int
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_get_threads.restype = ctypes.c_int
+        L.orc_fivp_create.argtypes = [P(Domain), ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.orc_fivp_create.restype = ctypes.c_void_p
+        L.orc_fivp_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_fivp_set_coeff.argtypes = [ctypes.c_void_p, dp]
+        L.orc_fivp_get_coeff.argtypes = [ctypes.c_void_p, dp]
+        L.orc_fivp_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_fivp_rhs.argtypes = [ctypes.c_void_p, dp, dp]
+        L.orc_fivp_time.argtypes = [ctypes.c_void_p]
+        L.orc_fivp_time.restype = ctypes.c_double
+        L.orc_fivp_diagnostics.argtypes = [ctypes.c_void_p, dp, dp]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
+
+
+def get_threads():
+    return lib().orc_get_threads()
+
+
+def domain(kinds, sizes, bounds=None, dealias=1.5):
+    d = Domain()
+    d.ndim = len(sizes)
+    for a, (k, n) in enumerate(zip(kinds, sizes)):
+        d.kind[a] = k
+        d.n[a] = n
+        lo, hi = (bounds[a] if bounds is not None else (0.0, 2 * np.pi))
+        d.lo[a], d.hi[a] = lo, hi
+        d.dealias[a] = dealias if np.isscalar(dealias) else dealias[a]
+    return d
+
+
+def coeff_shape(d):
+    sh = (ctypes.c_int * 3)()
+    cplx = lib().orc_coeff_shape(ctypes.byref(d), sh)
+    return tuple(sh[a] for a in range(d.ndim)), bool(cplx)
+
+
+def fft(x, sign=-1):
+    a = np.array(x, dtype=np.complex128)
+    lib().orc_fft(len(a), _dp(a), sign)
+    return a
+
+
+def rfft(x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros(len(x) // 2 + 1, np.complex128)
+    lib().orc_rfft(len(x), _dp(x), _dp(out))
+    return out
+
+
+def irfft(c, n):
+    c = np.ascontiguousarray(c, dtype=np.complex128)
+    out = np.zeros(n, np.float64)
+    lib().orc_irfft(n, _dp(c), _dp(out))
+    return out
+
+
+def dct1(x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros_like(x)
+    lib().orc_dct1(len(x), _dp(x), _dp(out))
+    return out
+
+
+def _fortran(a, dtype):
+    return np.asfortranarray(a, dtype=dtype)
+
+
+def forward_transform(d, grid):
+    shape, cplx = coeff_shape(d)
+    g = _fortran(grid, np.float64)
+    out = np.zeros(shape, np.complex128 if cplx else np.float64, order="F")
+    lib().orc_forward_transform(ctypes.byref(d), _dp(g), _dp(out))
+    return out
+
+
+def backward_transform(d, coeff):
+    shape, cplx = coeff_shape(d)
+    c = _fortran(coeff, np.complex128 if cplx else np.float64)
+    grid = np.zeros(tuple(d.n[a] for a in range(d.ndim)), np.float64, order="F")
+    lib().orc_backward_transform(ctypes.byref(d), _dp(c), _dp(grid))
+    return grid
+
+
+def padded_multiply(d, ga, gb):
+    a = _fortran(ga, np.float64)
+    b = _fortran(gb, np.float64)
+    out = np.zeros_like(a, order="F")
+    lib().orc_padded_multiply(ctypes.byref(d), _dp(a), _dp(b), _dp(out))
+    return out
+
+
+def padded_size(n, factor=1.5):
+    return lib().orc_padded_size(int(n), float(factor))
+
+
+class FourierIVP:
+    """Pure-Fourier IVP with a diagonal implicit operator (oracle)."""
+
+    def __init__(self, d, problem, timestepper, nu, dt):
+        self.d = d
+        self.shape, _ = coeff_shape(d)
+        self.h = lib().orc_fivp_create(ctypes.byref(d), problem, TS[timestepper], nu, dt)
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None) and _lib is not None:
+                _lib.orc_fivp_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def set_coeff(self, c):
+        c = _fortran(c, np.complex128)
+        lib().orc_fivp_set_coeff(self.h, _dp(c))
+
+    def get_coeff(self):
+        out = np.zeros(self.shape, np.complex128, order="F")
+        lib().orc_fivp_get_coeff(self.h, _dp(out))
+        return out
+
+    def rhs(self, c):
+        c = _fortran(c, np.complex128)
+        out = np.zeros(self.shape, np.complex128, order="F")
+        lib().orc_fivp_rhs(self.h, _dp(c), _dp(out))
+        return out
+
+    def step(self, n=1):
+        lib().orc_fivp_step(self.h, int(n))
+
+    @property
+    def time(self):
+        return lib().orc_fivp_time(self.h)
+
+    def diagnostics(self):
+        E = ctypes.c_double()
+        Z = ctypes.c_double()
+        lib().orc_fivp_diagnostics(self.h, ctypes.byref(E), ctypes.byref(Z))
+        return E.value, Z.value

@@ -1,0 +1,307 @@
+// capi.cpp -- extern "C" entry points declared in include/tarang_hip.h.
+#include <exception>
+#include <memory>
+#include <new>
+
+#include "../../include/tarang_hip.h"
+#include "fourier_solver.hpp"
+#include "runtime.hpp"
+#include "transforms.hpp"
+
+struct tarang_ctx_s {
+    tarang::Context ctx;
+};
+struct tarang_plan_s {
+    std::unique_ptr<tarang::TransformPlan> plan;
+};
+struct tarang_fivp_s {
+    std::unique_ptr<tarang::FourierSolver> solver;
+};
+
+namespace {
+template <typename F>
+int guard(F&& f) {
+    try {
+        f();
+        tarang::set_error("");
+        return TARANG_OK;
+    } catch (const tarang::Error& e) {
+        tarang::set_error(e.msg);
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        tarang::set_error("host allocation failed");
+        return TARANG_ERR_CAPACITY;
+    } catch (const std::exception& e) {
+        tarang::set_error(e.what());
+        return TARANG_ERR_INVALID;
+    } catch (...) {
+        tarang::set_error("unknown error");
+        return TARANG_ERR_INVALID;
+    }
+}
+#define NEED(p, what) TG_REQUIRE((p) != nullptr, tarang::kErrInvalid, what " is NULL")
+}  // namespace
+
+extern "C" {
+
+const char* tarang_last_error(void) { return tarang::last_error(); }
+
+int tarang_version(int* major, int* minor) {
+    return guard([&] {
+        NEED(major, "major");
+        NEED(minor, "minor");
+        *major = 0;
+        *minor = 1;
+    });
+}
+
+int tarang_ctx_create(int device, tarang_ctx_t* out) {
+    return guard([&] {
+        NEED(out, "out");
+        int n = 0;
+        TG_HIP(hipGetDeviceCount(&n));
+        TG_REQUIRE(device >= 0 && device < n, tarang::kErrInvalid, "device index out of range");
+        auto c = std::make_unique<tarang_ctx_s>();
+        c->ctx.device = device;
+        TG_HIP(hipSetDevice(device));
+        TG_HIP(hipStreamCreateWithFlags(&c->ctx.stream, hipStreamNonBlocking));
+        c->ctx.own_stream = true;
+        *out = c.release();
+    });
+}
+
+int tarang_ctx_destroy(tarang_ctx_t ctx) {
+    return guard([&] { delete ctx; });
+}
+
+int tarang_ctx_set_stream(tarang_ctx_t ctx, void* stream) {
+    return guard([&] {
+        NEED(ctx, "ctx");
+        ctx->ctx.activate();
+        if (ctx->ctx.own_stream && ctx->ctx.stream && stream) {
+            TG_HIP(hipStreamSynchronize(ctx->ctx.stream));
+            TG_HIP(hipStreamDestroy(ctx->ctx.stream));
+            ctx->ctx.own_stream = false;
+            ctx->ctx.stream = static_cast<hipStream_t>(stream);
+        } else if (stream) {
+            ctx->ctx.stream = static_cast<hipStream_t>(stream);
+        } else if (!ctx->ctx.own_stream) {
+            TG_HIP(hipStreamCreateWithFlags(&ctx->ctx.stream, hipStreamNonBlocking));
+            ctx->ctx.own_stream = true;
+        }
+    });
+}
+
+int tarang_ctx_sync(tarang_ctx_t ctx) {
+    return guard([&] {
+        NEED(ctx, "ctx");
+        ctx->ctx.activate();
+        TG_HIP(hipStreamSynchronize(ctx->ctx.stream));
+    });
+}
+
+int tarang_buf_alloc(tarang_ctx_t ctx, size_t bytes, void** dptr) {
+    return guard([&] {
+        NEED(ctx, "ctx");
+        NEED(dptr, "dptr");
+        ctx->ctx.activate();
+        *dptr = ctx->ctx.alloc(bytes);
+    });
+}
+
+int tarang_buf_free(tarang_ctx_t ctx, void* dptr) {
+    return guard([&] {
+        NEED(ctx, "ctx");
+        ctx->ctx.activate();
+        ctx->ctx.free(dptr);
+    });
+}
+
+int tarang_h2d(tarang_ctx_t ctx, void* dst, const void* src, size_t bytes) {
+    return guard([&] {
+        NEED(ctx, "ctx");
+        ctx->ctx.activate();
+        TG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->ctx.stream));
+        TG_HIP(hipStreamSynchronize(ctx->ctx.stream));
+    });
+}
+
+int tarang_d2h(tarang_ctx_t ctx, void* dst, const void* src, size_t bytes) {
+    return guard([&] {
+        NEED(ctx, "ctx");
+        ctx->ctx.activate();
+        TG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->ctx.stream));
+        TG_HIP(hipStreamSynchronize(ctx->ctx.stream));
+    });
+}
+
+int tarang_plan_create(tarang_ctx_t ctx, const tarang_domain_desc* desc, tarang_plan_t* out) {
+    return guard([&] {
+        NEED(ctx, "ctx");
+        NEED(desc, "desc");
+        NEED(out, "out");
+        tarang::DomainDesc d;
+        d.ndim = desc->ndim;
+        for (int a = 0; a < 3; ++a) {
+            d.kind[a] = desc->kind[a];
+            d.n[a] = desc->n[a];
+            d.lo[a] = desc->lo[a];
+            d.hi[a] = desc->hi[a];
+            d.dealias[a] = desc->dealias[a];
+        }
+        d.dtype = desc->dtype;
+        auto p = std::make_unique<tarang_plan_s>();
+        p->plan = std::make_unique<tarang::TransformPlan>(&ctx->ctx, d);
+        *out = p.release();
+    });
+}
+
+int tarang_plan_destroy(tarang_plan_t plan) {
+    return guard([&] { delete plan; });
+}
+
+int tarang_plan_coeff_shape(tarang_plan_t plan, int* shape3, int* is_complex) {
+    return guard([&] {
+        NEED(plan, "plan");
+        NEED(shape3, "shape3");
+        shape3[0] = shape3[1] = shape3[2] = 1;
+        plan->plan->coeff_shape(shape3, is_complex);
+    });
+}
+
+int tarang_forward_transform(tarang_plan_t plan, const void* grid, void* coeff) {
+    return guard([&] {
+        NEED(plan, "plan");
+        NEED(grid, "grid");
+        NEED(coeff, "coeff");
+        plan->plan->forward(grid, coeff);
+    });
+}
+
+int tarang_backward_transform(tarang_plan_t plan, const void* coeff, void* grid) {
+    return guard([&] {
+        NEED(plan, "plan");
+        NEED(grid, "grid");
+        NEED(coeff, "coeff");
+        plan->plan->backward(coeff, grid);
+    });
+}
+
+int tarang_padded_multiply(tarang_plan_t plan, const void* a, const void* b, void* out) {
+    return guard([&] {
+        NEED(plan, "plan");
+        NEED(a, "grid_a");
+        NEED(b, "grid_b");
+        NEED(out, "grid_out");
+        plan->plan->padded_multiply(a, b, out);
+    });
+}
+
+int tarang_fivp_create(tarang_ctx_t ctx, const tarang_fivp_desc* desc, tarang_fivp_t* out) {
+    return guard([&] {
+        NEED(ctx, "ctx");
+        NEED(desc, "desc");
+        NEED(out, "out");
+        tarang::FourierDesc d;
+        d.ndim = desc->ndim;
+        d.n[0] = desc->n[0];
+        d.n[1] = desc->ndim > 1 ? desc->n[1] : 1;
+        d.L[0] = desc->L[0];
+        d.L[1] = desc->ndim > 1 ? desc->L[1] : 1.0;
+        d.dtype = desc->dtype;
+        d.problem = desc->problem;
+        d.stepper = desc->timestepper;
+        d.nu = desc->nu;
+        d.dt = desc->dt;
+        TG_REQUIRE(d.L[0] > 0 && d.L[1] > 0, tarang::kErrInvalid, "domain length must be positive");
+        auto s = std::make_unique<tarang_fivp_s>();
+        s->solver = std::make_unique<tarang::FourierSolver>(&ctx->ctx, d);
+        *out = s.release();
+    });
+}
+
+int tarang_fivp_destroy(tarang_fivp_t s) {
+    return guard([&] { delete s; });
+}
+
+int tarang_fivp_set_state(tarang_fivp_t s, const void* coeff) {
+    return guard([&] {
+        NEED(s, "solver");
+        NEED(coeff, "coeff");
+        s->solver->set_state(coeff);
+    });
+}
+
+int tarang_fivp_get_state(tarang_fivp_t s, void* coeff) {
+    return guard([&] {
+        NEED(s, "solver");
+        NEED(coeff, "coeff");
+        s->solver->get_state(coeff);
+    });
+}
+
+int tarang_fivp_rhs(tarang_fivp_t s, const void* coeff, void* F) {
+    return guard([&] {
+        NEED(s, "solver");
+        NEED(coeff, "coeff");
+        NEED(F, "F");
+        s->solver->rhs(coeff, F);
+    });
+}
+
+int tarang_fivp_aux(tarang_fivp_t s, const void* q, void* psi, void* u, void* v) {
+    return guard([&] {
+        NEED(s, "solver");
+        NEED(q, "q");
+        s->solver->aux(q, psi, u, v);
+    });
+}
+
+int tarang_fivp_step(tarang_fivp_t s, int nsteps) {
+    return guard([&] {
+        NEED(s, "solver");
+        TG_REQUIRE(nsteps >= 0, tarang::kErrInvalid, "nsteps must be >= 0");
+        s->solver->step(nsteps);
+    });
+}
+
+int tarang_fivp_set_graph(tarang_fivp_t s, int use_graph) {
+    return guard([&] {
+        NEED(s, "solver");
+        s->solver->enable_graph(use_graph != 0);
+    });
+}
+
+int tarang_fivp_set_dt(tarang_fivp_t s, double dt) {
+    return guard([&] {
+        NEED(s, "solver");
+        TG_REQUIRE(dt > 0, tarang::kErrInvalid, "dt must be positive");
+        s->solver->set_dt(dt);
+    });
+}
+
+int tarang_fivp_time(tarang_fivp_t s, double* t, long* iteration) {
+    return guard([&] {
+        NEED(s, "solver");
+        if (t) *t = s->solver->time();
+        if (iteration) *iteration = s->solver->iteration();
+    });
+}
+
+int tarang_fivp_profile(tarang_fivp_t s, int nsteps, double* avg_ms4, long* counts4) {
+    return guard([&] {
+        NEED(s, "solver");
+        NEED(avg_ms4, "avg_ms4");
+        TG_REQUIRE(nsteps > 0, tarang::kErrInvalid, "nsteps must be > 0");
+        s->solver->profile(nsteps, avg_ms4, counts4);
+    });
+}
+
+int tarang_fivp_kernel_bytes(tarang_fivp_t s, double* ybwd, double* xpass, double* yfwd) {
+    return guard([&] {
+        NEED(s, "solver");
+        s->solver->kernel_bytes(ybwd, xpass, yfwd);
+    });
+}
+
+}  // extern "C"

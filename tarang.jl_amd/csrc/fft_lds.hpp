@@ -1,0 +1,174 @@
+// fft_lds.hpp -- batched complex FFTs of compile-time length L staged in LDS.
+//
+// Stockham autosort passes (radix 8, 4, 2, 3, 5).  NLINES lines of length L sit
+// in LDS at stride LDL; NT threads share the butterflies of every pass.  Each
+// pass is done in place: every thread first loads all of its butterflies'
+// inputs into registers, the workgroup synchronises, then the twiddled
+// butterflies are written to their Stockham output slots.
+//
+// Conventions: forward = exp(-2 pi i jk/L), inverse = exp(+2 pi i jk/L), both
+// unnormalised (the FFTW convention the reference relies on,
+// src/core/transforms/transform_fourier.jl:473-590).  Twiddles come from a
+// global table tw[m] = exp(-2 pi i m/L), m < L, built in FP64 on the host.
+#pragma once
+#include "common.hpp"
+
+namespace tarang {
+
+struct RadixPlan {
+    int n;
+    int r[16];
+};
+
+constexpr RadixPlan radix_plan(int L) {
+    RadixPlan p{0, {}};
+    int m = L;
+    while (m % 8 == 0 && m != 16) { p.r[p.n++] = 8; m /= 8; }
+    while (m % 4 == 0) { p.r[p.n++] = 4; m /= 4; }
+    while (m % 2 == 0) { p.r[p.n++] = 2; m /= 2; }
+    while (m % 3 == 0) { p.r[p.n++] = 3; m /= 3; }
+    while (m % 5 == 0) { p.r[p.n++] = 5; m /= 5; }
+    if (m != 1) p.n = -1;  // unsupported prime factor
+    return p;
+}
+
+constexpr bool fft_length_supported(int L) { return L >= 1 && radix_plan(L).n >= 0; }
+
+// ---- small DFT kernels on registers; INV selects exp(+...) ----
+template <bool INV, typename V>
+__device__ inline V rot_mi(V a) {  // multiply by w4 = exp(-+ 2 pi i/4): -i forward, +i inverse
+    return INV ? cmul_i(a) : cmul_mi(a);
+}
+
+template <int R, bool INV, typename T>
+__device__ inline void dft_small(cplx<T>* v) {
+    using V = cplx<T>;
+    if constexpr (R == 2) {
+        V a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if constexpr (R == 4) {
+        V a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
+        V c = cadd(v[1], v[3]), d = rot_mi<INV>(csub(v[1], v[3]));
+        v[0] = cadd(a, c);
+        v[1] = cadd(b, d);
+        v[2] = csub(a, c);
+        v[3] = csub(b, d);
+    } else if constexpr (R == 8) {
+        const T h = T(0.70710678118654752440084436210484903928);
+        V a[4], b[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            a[r] = cadd(v[r], v[r + 4]);
+            b[r] = csub(v[r], v[r + 4]);
+        }
+        // b[r] *= w8^r, w8 = exp(-+ 2 pi i / 8)
+        {
+            V t = b[1];  // * (1 -+ i)/sqrt2
+            b[1] = INV ? V{(t.x - t.y) * h, (t.x + t.y) * h} : V{(t.x + t.y) * h, (t.y - t.x) * h};
+            b[2] = rot_mi<INV>(b[2]);
+            t = b[3];  // * (-1 -+ i)/sqrt2
+            b[3] = INV ? V{(-t.x - t.y) * h, (t.x - t.y) * h} : V{(t.y - t.x) * h, (-t.x - t.y) * h};
+        }
+        // two radix-4 DFTs
+        V a0 = cadd(a[0], a[2]), a1 = csub(a[0], a[2]);
+        V a2 = cadd(a[1], a[3]), a3 = rot_mi<INV>(csub(a[1], a[3]));
+        V b0 = cadd(b[0], b[2]), b1 = csub(b[0], b[2]);
+        V b2 = cadd(b[1], b[3]), b3 = rot_mi<INV>(csub(b[1], b[3]));
+        v[0] = cadd(a0, a2);
+        v[2] = cadd(a1, a3);
+        v[4] = csub(a0, a2);
+        v[6] = csub(a1, a3);
+        v[1] = cadd(b0, b2);
+        v[3] = cadd(b1, b3);
+        v[5] = csub(b0, b2);
+        v[7] = csub(b1, b3);
+    } else if constexpr (R == 3) {
+        const T c = T(-0.5), s = INV ? T(0.86602540378443864676372317075293618) : T(-0.86602540378443864676372317075293618);
+        V a = v[0], t = cadd(v[1], v[2]), u = csub(v[1], v[2]);
+        V m = V{a.x + c * t.x, a.y + c * t.y};
+        V iu = V{-s * u.y, s * u.x};  // (s i) * u
+        v[0] = cadd(a, t);
+        v[1] = cadd(m, iu);
+        v[2] = csub(m, iu);
+    } else if constexpr (R == 5) {
+        const T c1 = T(0.30901699437494742410229341718281906), c2 = T(-0.80901699437494742410229341718281906);
+        const T sg = INV ? T(1) : T(-1);
+        const T s1 = sg * T(0.95105651629515357211643933337938214), s2 = sg * T(0.58778525229247312916870595463907277);
+        V a = v[0];
+        V t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+        V u1 = csub(v[1], v[4]), u2 = csub(v[2], v[3]);
+        V m1 = V{a.x + c1 * t1.x + c2 * t2.x, a.y + c1 * t1.y + c2 * t2.y};
+        V m2 = V{a.x + c2 * t1.x + c1 * t2.x, a.y + c2 * t1.y + c1 * t2.y};
+        V n1 = V{s1 * u1.x + s2 * u2.x, s1 * u1.y + s2 * u2.y};
+        V n2 = V{s2 * u1.x - s1 * u2.x, s2 * u1.y - s1 * u2.y};
+        // i * n
+        V in1 = V{-n1.y, n1.x}, in2 = V{-n2.y, n2.x};
+        v[0] = cadd(a, cadd(t1, t2));
+        v[1] = cadd(m1, in1);
+        v[4] = csub(m1, in1);
+        v[2] = cadd(m2, in2);
+        v[3] = csub(m2, in2);
+    }
+}
+
+// One Stockham pass of radix R after NS points have been combined.
+template <typename T, int L, int R, int NS, int NLINES, int LDL, int NT, bool INV>
+__device__ inline void stockham_pass(cplx<T>* buf, const cplx<T>* __restrict__ tw) {
+    using V = cplx<T>;
+    constexpr int NB = L / R;
+    constexpr int TOT = NB * NLINES;
+    constexpr int PER = (TOT + NT - 1) / NT;
+    constexpr int TSTEP = L / (NS * R);
+    V v[PER][R];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int b = threadIdx.x + i * NT;
+        if (TOT % NT == 0 || b < TOT) {
+            const int line = b / NB, j = b - line * NB;
+            const V* src = buf + line * LDL + j;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[i][r] = src[r * NB];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int b = threadIdx.x + i * NT;
+        if (TOT % NT == 0 || b < TOT) {
+            const int line = b / NB, j = b - line * NB;
+            const int k = (NS == 1) ? 0 : j % NS;
+            if (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    V w = tw[r * k * TSTEP];
+                    v[i][r] = INV ? cmulc(v[i][r], w) : cmul(v[i][r], w);
+                }
+            }
+            dft_small<R, INV, T>(v[i]);
+            V* dst = buf + line * LDL + (j / NS) * NS * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) dst[r * NS] = v[i][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, int L, int NLINES, int LDL, int NT, bool INV, int P = 0, int NS = 1>
+__device__ inline void fft_passes(cplx<T>* buf, const cplx<T>* __restrict__ tw) {
+    constexpr RadixPlan plan = radix_plan(L);
+    static_assert(plan.n >= 0, "FFT length must factor into 2, 3 and 5");
+    if constexpr (P < plan.n) {
+        constexpr int R = plan.r[P];
+        stockham_pass<T, L, R, NS, NLINES, LDL, NT, INV>(buf, tw);
+        fft_passes<T, L, NLINES, LDL, NT, INV, P + 1, NS * R>(buf, tw);
+    }
+}
+
+// In-LDS batched FFT; caller must have synchronised after filling buf.
+template <typename T, int L, int NLINES, int LDL, int NT, bool INV>
+__device__ inline void fft_lds(cplx<T>* buf, const cplx<T>* __restrict__ tw) {
+    if constexpr (L > 1) fft_passes<T, L, NLINES, LDL, NT, INV>(buf, tw);
+}
+
+}  // namespace tarang

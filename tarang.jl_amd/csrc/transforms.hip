@@ -1,0 +1,399 @@
+// transforms.hip -- field transforms (RealFourier rfft/fft, ChebyshevT DCT-I as
+// an exact matrix along z) and the standalone dealiased product.
+//
+// Conventions follow the reference CPU chain:
+//   forward: axis 0 rfft (unnormalised), later RealFourier axes complex FFT,
+//            ChebyshevT axes c_k = (-1)^k w_k/(N-1) REDFT00(f)_k, w_0 = w_{N-1} = 1/2
+//            (src/core/transforms/transform_chebyshev.jl:428-486)
+//   backward: reverse order; ifft/irfft normalised by 1/N; Chebyshev evaluates
+//            f_j = sum_k c_k T_k(-cos(pi j/(N-1))) (transform_chebyshev.jl:535-604)
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "dispatch.hpp"
+#include "fourier_kernels.hpp"
+#include "transforms.hpp"
+
+namespace tarang {
+
+// ---- kernels ----
+template <typename T, int N, int NT>
+__global__ __launch_bounds__(NT) void k_r2c_rows(const T* __restrict__ x, size_t nlines, cplx<T>* __restrict__ y,
+                                                 const cplx<T>* __restrict__ tw) {
+    using V = cplx<T>;
+    constexpr int NH = N / 2 + 1;
+    __shared__ V buf[N];
+    const size_t la = 2 * (size_t)blockIdx.x, lb = la + 1;
+    const bool hb = lb < nlines;
+    for (int j = threadIdx.x; j < N; j += NT) buf[j] = V{x[la * N + j], hb ? x[lb * N + j] : T(0)};
+    __syncthreads();
+    fft_lds<T, N, 1, N, NT, false>(buf, tw);
+    for (int k = threadIdx.x; k < NH; k += NT) {
+        const V zk = buf[k], zm = buf[(N - k) % N];
+        y[la * NH + k] = V{(zk.x + zm.x) * T(0.5), (zk.y - zm.y) * T(0.5)};
+        if (hb) y[lb * NH + k] = V{(zk.y + zm.y) * T(0.5), (zm.x - zk.x) * T(0.5)};
+    }
+}
+
+template <typename T, int N, int NT>
+__global__ __launch_bounds__(NT) void k_c2r_rows(const cplx<T>* __restrict__ y, size_t nlines, T* __restrict__ x,
+                                                 const cplx<T>* __restrict__ tw) {
+    using V = cplx<T>;
+    constexpr int NH = N / 2 + 1;
+    constexpr bool EVEN = (N % 2) == 0;
+    __shared__ V buf[N];
+    const size_t la = 2 * (size_t)blockIdx.x, lb = la + 1;
+    const bool hb = lb < nlines;
+    for (int k = threadIdx.x; k < N; k += NT) {
+        const int kk = k < NH ? k : N - k;
+        V a = y[la * NH + kk];
+        V b = hb ? y[lb * NH + kk] : V{T(0), T(0)};
+        if (kk == 0 || (EVEN && kk == N / 2)) {  // C2R ignores Im of DC and Nyquist
+            a.y = T(0);
+            b.y = T(0);
+        }
+        buf[k] = k < NH ? V{a.x - b.y, a.y + b.x} : V{a.x + b.y, b.x - a.y};
+    }
+    __syncthreads();
+    fft_lds<T, N, 1, N, NT, true>(buf, tw);
+    const T s = T(1) / T(N);
+    for (int j = threadIdx.x; j < N; j += NT) {
+        const V z = buf[j];
+        x[la * N + j] = z.x * s;
+        if (hb) x[lb * N + j] = z.y * s;
+    }
+}
+
+// complex FFT along a strided axis: data viewed as (inner, N, outer)
+template <typename T, int N, int TK, int NT, bool INV>
+__global__ __launch_bounds__(NT) void k_c2c_axis(cplx<T>* __restrict__ a, size_t inner,
+                                                 const cplx<T>* __restrict__ tw) {
+    using V = cplx<T>;
+    __shared__ V buf[TK * N];
+    const size_t i0 = (size_t)blockIdx.x * TK;
+    const size_t base = (size_t)blockIdx.y * N * inner;
+    for (int idx = threadIdx.x; idx < TK * N; idx += NT) {
+        const int c = idx % TK, d = idx / TK;
+        const size_t i = i0 + c;
+        buf[c * N + d] = i < inner ? a[base + (size_t)d * inner + i] : V{T(0), T(0)};
+    }
+    __syncthreads();
+    fft_lds<T, N, TK, N, NT, INV>(buf, tw);
+    const T s = INV ? T(1) / T(N) : T(1);
+    for (int idx = threadIdx.x; idx < TK * N; idx += NT) {
+        const int c = idx % TK, d = idx / TK;
+        const size_t i = i0 + c;
+        if (i < inner) {
+            const V v = buf[c * N + d];
+            a[base + (size_t)d * inner + i] = V{v.x * s, v.y * s};
+        }
+    }
+}
+
+// out[c][k] = sum_j mat[k][j] in[c][j] along a strided axis of complex data (inner, n, outer)
+template <typename T, int TK, int NT>
+__global__ __launch_bounds__(NT) void k_cheb_axis(cplx<T>* __restrict__ a, size_t inner, int n,
+                                                  const T* __restrict__ mat) {
+    using V = cplx<T>;
+    extern __shared__ unsigned char smem_raw[];
+    V* buf = reinterpret_cast<V*>(smem_raw);
+    const size_t i0 = (size_t)blockIdx.x * TK;
+    const size_t base = (size_t)blockIdx.y * n * inner;
+    for (int idx = threadIdx.x; idx < TK * n; idx += NT) {
+        const int c = idx % TK, d = idx / TK;
+        const size_t i = i0 + c;
+        buf[c * n + d] = i < inner ? a[base + (size_t)d * inner + i] : V{T(0), T(0)};
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < TK * n; idx += NT) {
+        const int c = idx % TK, k = idx / TK;
+        const size_t i = i0 + c;
+        if (i >= inner) continue;
+        const T* row = mat + (size_t)k * n;
+        const V* col = buf + c * n;
+        T sr = T(0), si = T(0);
+        for (int j = 0; j < n; ++j) {
+            const T m = row[j];
+            sr += m * col[j].x;
+            si += m * col[j].y;
+        }
+        a[base + (size_t)k * inner + i] = V{sr, si};
+    }
+}
+
+template <typename T>
+__global__ void k_pointwise_mul(const T* a, const T* b, T* out, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = a[i] * b[i];
+}
+
+// ---- plan ----
+TransformPlan::TransformPlan(Context* ctx, const DomainDesc& d) : ctx_(ctx), d_(d) {
+    TG_REQUIRE(d.ndim >= 1 && d.ndim <= 3, kErrInvalid, "domain must be 1-D, 2-D or 3-D");
+    TG_REQUIRE(d.dtype == kF32 || d.dtype == kF64, kErrInvalid, "dtype must be f32 or f64");
+    TG_REQUIRE(d.kind[0] == kRealFourier, kErrInvalid,
+               "GPU transforms need a RealFourier first axis (PeriodicDomain / ChannelDomain layouts)");
+    for (int a = 0; a < d.ndim; ++a) {
+        TG_REQUIRE(d.kind[a] == kRealFourier || d.kind[a] == kChebyshevT, kErrInvalid,
+                   "only RealFourier and ChebyshevT bases are supported");
+        TG_REQUIRE(d.n[a] >= 2, kErrInvalid, "axis size must be >= 2");
+        bool ok = d.kind[a] == kChebyshevT;
+        if (!ok) dispatch_size(d.n[a], [&](auto) { ok = true; }, Sizes1D{});
+        TG_REQUIRE(ok, kErrInvalid,
+                   "RealFourier size " + std::to_string(d.n[a]) + " not supported on the GPU (powers of 2, 8..4096)");
+        if (d.kind[a] == kChebyshevT)
+            TG_REQUIRE(d.n[a] <= 1024, kErrInvalid, "ChebyshevT size above 1024 is not supported");
+        cshape_[a] = a == 0 ? d.n[a] / 2 + 1 : d.n[a];
+    }
+    for (int a = d.ndim; a < 3; ++a) cshape_[a] = 1;
+}
+
+void TransformPlan::coeff_shape(int* shape, int* is_complex) const {
+    for (int a = 0; a < d_.ndim; ++a) shape[a] = cshape_[a];
+    if (is_complex) *is_complex = 1;
+}
+
+size_t TransformPlan::grid_elems() const {
+    size_t t = 1;
+    for (int a = 0; a < d_.ndim; ++a) t *= (size_t)d_.n[a];
+    return t;
+}
+
+size_t TransformPlan::coeff_elems() const {
+    size_t t = 1;
+    for (int a = 0; a < d_.ndim; ++a) t *= (size_t)cshape_[a];
+    return t;
+}
+
+const void* TransformPlan::cheb_matrix(int n, bool forward) {
+    const bool f32 = d_.dtype == kF32;
+    const std::string key = std::string("cheb_") + (forward ? "fwd_" : "bwd_") + std::to_string(n) + (f32 ? "_f32" : "_f64");
+    std::vector<double> m((size_t)n * n);
+    const long period = 2L * (n - 1);
+    auto cospi = [&](long num) {  // cos(pi num / (n-1)) with exact range reduction
+        const long r = ((num % period) + period) % period;
+        return (double)cosl(3.14159265358979323846264338327950288L * (long double)r / (long double)(n - 1));
+    };
+    for (int k = 0; k < n; ++k) {
+        for (int j = 0; j < n; ++j) {
+            if (forward) {
+                const double sgn = (k & 1) ? -1.0 : 1.0;
+                const double wk = (k == 0 || k == n - 1) ? 0.5 : 1.0;
+                double e;
+                if (j == 0) e = 1.0;
+                else if (j == n - 1) e = sgn;
+                else e = 2.0 * cospi((long)j * k);
+                m[(size_t)k * n + j] = sgn * wk / (n - 1) * e;
+            } else {  // row k = grid point, column j = Chebyshev degree: (-1)^j cos(pi jk/(n-1))
+                const double sgn = (j & 1) ? -1.0 : 1.0;
+                m[(size_t)k * n + j] = sgn * cospi((long)j * k);
+            }
+        }
+    }
+    if (f32) {
+        std::vector<float> h(m.begin(), m.end());
+        return ctx_->table(key, h.data(), h.size() * sizeof(float));
+    }
+    return ctx_->table(key, m.data(), m.size() * sizeof(double));
+}
+
+template <typename T>
+void TransformPlan::axis_fft(void* data, int axis, bool inverse) {
+    using V = cplx<T>;
+    size_t inner = 1, outer = 1;
+    for (int a = 0; a < axis; ++a) inner *= (size_t)cshape_[a];
+    for (int a = axis + 1; a < d_.ndim; ++a) outer *= (size_t)cshape_[a];
+    const DType dt = std::is_same<T, float>::value ? kF32 : kF64;
+    constexpr int NT = 256;
+    dispatch_size(d_.n[axis], [&](auto NC) {
+        constexpr int N = decltype(NC)::value;
+        constexpr int TK = tile_cols<T, N>();
+        const V* tw = static_cast<const V*>(ctx_->twiddle(N, dt));
+        dim3 grid((unsigned)((inner + TK - 1) / TK), (unsigned)outer);
+        if (inverse)
+            hipLaunchKernelGGL((k_c2c_axis<T, N, TK, NT, true>), grid, dim3(NT), 0, ctx_->stream,
+                               static_cast<V*>(data), inner, tw);
+        else
+            hipLaunchKernelGGL((k_c2c_axis<T, N, TK, NT, false>), grid, dim3(NT), 0, ctx_->stream,
+                               static_cast<V*>(data), inner, tw);
+    }, Sizes1D{});
+    TG_HIP(hipGetLastError());
+}
+
+template <typename T>
+void TransformPlan::axis_cheb(void* data, int axis, bool forward) {
+    using V = cplx<T>;
+    size_t inner = 1, outer = 1;
+    for (int a = 0; a < axis; ++a) inner *= (size_t)cshape_[a];
+    for (int a = axis + 1; a < d_.ndim; ++a) outer *= (size_t)cshape_[a];
+    const int n = d_.n[axis];
+    const T* mat = static_cast<const T*>(cheb_matrix(n, forward));
+    constexpr int NT = 256, TK = 4;
+    const size_t lds = (size_t)TK * n * sizeof(V);
+    dim3 grid((unsigned)((inner + TK - 1) / TK), (unsigned)outer);
+    hipLaunchKernelGGL((k_cheb_axis<T, TK, NT>), grid, dim3(NT), lds, ctx_->stream, static_cast<V*>(data), inner, n,
+                       mat);
+    TG_HIP(hipGetLastError());
+}
+
+template <typename T>
+void TransformPlan::forward_t(const void* grid, void* coeff) {
+    using V = cplx<T>;
+    const DType dt = std::is_same<T, float>::value ? kF32 : kF64;
+    constexpr int NT = 256;
+    size_t nlines = 1;
+    for (int a = 1; a < d_.ndim; ++a) nlines *= (size_t)d_.n[a];
+    dispatch_size(d_.n[0], [&](auto NC) {
+        constexpr int N = decltype(NC)::value;
+        const V* tw = static_cast<const V*>(ctx_->twiddle(N, dt));
+        hipLaunchKernelGGL((k_r2c_rows<T, N, NT>), dim3((unsigned)((nlines + 1) / 2)), dim3(NT), 0, ctx_->stream,
+                           static_cast<const T*>(grid), nlines, static_cast<V*>(coeff), tw);
+    }, Sizes1D{});
+    TG_HIP(hipGetLastError());
+    for (int a = 1; a < d_.ndim; ++a) {
+        if (d_.kind[a] == kRealFourier) axis_fft<T>(coeff, a, false);
+        else axis_cheb<T>(coeff, a, true);
+    }
+}
+
+template <typename T>
+void TransformPlan::backward_t(const void* coeff, void* grid) {
+    using V = cplx<T>;
+    const DType dt = std::is_same<T, float>::value ? kF32 : kF64;
+    constexpr int NT = 256;
+    const size_t cb = coeff_elems() * sizeof(V);
+    if (!scratch_[0].ptr || scratch_[0].bytes < cb) scratch_[0] = DevBuf(ctx_, cb);
+    void* work = scratch_[0].ptr;
+    TG_HIP(hipMemcpyAsync(work, coeff, cb, hipMemcpyDeviceToDevice, ctx_->stream));
+    for (int a = d_.ndim - 1; a >= 1; --a) {
+        if (d_.kind[a] == kRealFourier) axis_fft<T>(work, a, true);
+        else axis_cheb<T>(work, a, false);
+    }
+    size_t nlines = 1;
+    for (int a = 1; a < d_.ndim; ++a) nlines *= (size_t)d_.n[a];
+    dispatch_size(d_.n[0], [&](auto NC) {
+        constexpr int N = decltype(NC)::value;
+        const V* tw = static_cast<const V*>(ctx_->twiddle(N, dt));
+        hipLaunchKernelGGL((k_c2r_rows<T, N, NT>), dim3((unsigned)((nlines + 1) / 2)), dim3(NT), 0, ctx_->stream,
+                           static_cast<const V*>(work), nlines, static_cast<T*>(grid), tw);
+    }, Sizes1D{});
+    TG_HIP(hipGetLastError());
+}
+
+template <typename T>
+void TransformPlan::product_t(const void* ga, const void* gb, void* out) {
+    using V = cplx<T>;
+    const DType dt = std::is_same<T, float>::value ? kF32 : kF64;
+    constexpr int NT = 256;
+    bool fourier_only = true, any_dealias = false, all_32 = true;
+    int minN = 1 << 30;
+    for (int a = 0; a < d_.ndim; ++a) {
+        if (d_.kind[a] != kRealFourier) fourier_only = false;
+        else {
+            if (d_.dealias[a] > 1.0) any_dealias = true;
+            if (d_.dealias[a] != 1.5) all_32 = false;
+            minN = d_.n[a] < minN ? d_.n[a] : minN;
+        }
+    }
+    if (!any_dealias || minN <= 4) {  // nonlinear_padding.jl:90-96 and the dealias <= 1 branch
+        const size_t n = grid_elems();
+        hipLaunchKernelGGL((k_pointwise_mul<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx_->stream,
+                           static_cast<const T*>(ga), static_cast<const T*>(gb), static_cast<T*>(out), n);
+        TG_HIP(hipGetLastError());
+        return;
+    }
+    TG_REQUIRE(fourier_only && d_.ndim <= 2, kErrInvalid,
+               "GPU padded product: only RealFourier domains of 1 or 2 dimensions are supported by this entry point");
+    TG_REQUIRE(all_32, kErrInvalid, "GPU padded product supports dealias = 3/2 only");
+    const size_t cb = coeff_elems() * sizeof(V);
+    for (int i = 1; i <= 3; ++i)
+        if (!scratch_[i].ptr || scratch_[i].bytes < cb) scratch_[i] = DevBuf(ctx_, cb);
+    V* A = scratch_[1].as<V>();
+    V* B = scratch_[2].as<V>();
+    V* P = scratch_[3].as<V>();
+    forward_t<T>(ga, A);
+    forward_t<T>(gb, B);
+    FourierParams<T> p;
+    p.in0 = A;
+    p.in1 = B;
+    p.nxh = cshape_[0];
+    p.nx = d_.n[0];
+    p.ny = d_.ndim > 1 ? d_.n[1] : 1;
+    p.k0x = T(1);
+    p.k0y = T(1);
+    const int mx = padded_len(d_.n[0]);
+    if (d_.ndim == 1) {
+        const T scale = T(1.0 / ((double)d_.n[0] * mx));
+        dispatch_size(d_.n[0], [&](auto NC) {
+            constexpr int N = decltype(NC)::value;
+            constexpr int M = padded_len(N);
+            const V* tw = static_cast<const V*>(ctx_->twiddle(M, dt));
+            hipLaunchKernelGGL((k_line1d<T, N, M, NT, ProductSpec>), dim3(1), dim3(NT), 0, ctx_->stream, p, (size_t)0,
+                               P, (size_t)0, scale, tw);
+        }, Sizes1D{});
+        TG_HIP(hipGetLastError());
+    } else {
+        const int ny = d_.n[1], my = padded_len(ny), nxh = cshape_[0];
+        bool oky = false, okx = false;
+        dispatch_size(ny, [&](auto) { oky = true; }, Sizes2D{});
+        dispatch_size(d_.n[0], [&](auto) { okx = true; }, Sizes2D{});
+        const bool ok = okx && oky && !(d_.dtype == kF64 && (d_.n[0] > 1024 || ny > 1024));
+        TG_REQUIRE(ok, kErrInvalid, "padded product: size not supported for 2-D fused kernels");
+        const size_t gstride = (size_t)nxh * my;
+        DevBuf G(ctx_, 2 * gstride * sizeof(V)), H(ctx_, gstride * sizeof(V));
+        const T scale = T(1.0 / ((double)d_.n[0] * ny * (double)mx * my));
+        dispatch_size(ny, [&](auto NC) {
+            constexpr int N = decltype(NC)::value;
+            constexpr int M = padded_len(N);
+            constexpr int TK = tile_cols<T, M>();
+            const V* tw = static_cast<const V*>(ctx_->twiddle(M, dt));
+            hipLaunchKernelGGL((k_ybwd<T, N, M, TK, NT, ProductSpec>), dim3((nxh + TK - 1) / TK), dim3(NT), 0,
+                               ctx_->stream, p, G.as<V>(), gstride, tw);
+        }, Sizes2D{});
+        TG_HIP(hipGetLastError());
+        dispatch_size(d_.n[0], [&](auto NC) {
+            constexpr int N = decltype(NC)::value;
+            constexpr int M = padded_len(N);
+            if constexpr ((size_t)2 * M * sizeof(V) <= 160 * 1024) {
+                const V* tw = static_cast<const V*>(ctx_->twiddle(M, dt));
+                hipLaunchKernelGGL((k_xpass<T, N, M, NT, ProductSpec>), dim3(my / 2), dim3(NT), 0, ctx_->stream,
+                                   G.as<V>(), gstride, nxh, scale, H.as<V>(), tw);
+            } else {
+                throw Error{kErrInvalid, "x-pass does not fit in LDS at this size/precision"};
+            }
+        }, Sizes2D{});
+        TG_HIP(hipGetLastError());
+        dispatch_size(ny, [&](auto NC) {
+            constexpr int N = decltype(NC)::value;
+            constexpr int M = padded_len(N);
+            constexpr int TK = tile_cols<T, M>();
+            const V* tw = static_cast<const V*>(ctx_->twiddle(M, dt));
+            hipLaunchKernelGGL((k_yfwd<T, N, M, TK, NT>), dim3((nxh + TK - 1) / TK), dim3(NT), 0, ctx_->stream,
+                               H.as<V>(), nxh, d_.n[0], P, tw);
+        }, Sizes2D{});
+        TG_HIP(hipGetLastError());
+        TG_HIP(hipStreamSynchronize(ctx_->stream));  // G and H are released on return
+    }
+    backward_t<T>(P, out);
+}
+
+void TransformPlan::forward(const void* grid, void* coeff) {
+    ctx_->activate();
+    if (d_.dtype == kF32) forward_t<float>(grid, coeff);
+    else forward_t<double>(grid, coeff);
+}
+
+void TransformPlan::backward(const void* coeff, void* grid) {
+    ctx_->activate();
+    if (d_.dtype == kF32) backward_t<float>(coeff, grid);
+    else backward_t<double>(coeff, grid);
+}
+
+void TransformPlan::padded_multiply(const void* ga, const void* gb, void* out) {
+    ctx_->activate();
+    if (d_.dtype == kF32) product_t<float>(ga, gb, out);
+    else product_t<double>(ga, gb, out);
+}
+
+}  // namespace tarang

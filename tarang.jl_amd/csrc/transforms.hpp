@@ -1,0 +1,46 @@
+// transforms.hpp -- forward/backward spectral transforms of one field on a
+// RealFourier / ChebyshevT domain, in the reference's layout and conventions.
+#pragma once
+#include "runtime.hpp"
+
+namespace tarang {
+
+enum BasisKind : int { kRealFourier = 0, kChebyshevT = 1 };
+
+struct DomainDesc {
+    int ndim;
+    int kind[3];
+    int n[3];
+    double lo[3], hi[3];
+    double dealias[3];
+    int dtype;
+};
+
+class TransformPlan {
+  public:
+    TransformPlan(Context* ctx, const DomainDesc& d);
+    const DomainDesc& desc() const { return d_; }
+    void coeff_shape(int* shape, int* is_complex) const;
+    size_t grid_elems() const;
+    size_t coeff_elems() const;  // complex elements
+    // device pointers; grid is real (n0, n1, n2), coeff complex (n0/2+1, n1, n2)
+    void forward(const void* grid, void* coeff);
+    void backward(const void* coeff, void* grid);
+    // 3/2-dealiased product of two grid fields (RealFourier-only domains)
+    void padded_multiply(const void* ga, const void* gb, void* out);
+
+  private:
+    template <typename T> void forward_t(const void* grid, void* coeff);
+    template <typename T> void backward_t(const void* coeff, void* grid);
+    template <typename T> void product_t(const void* ga, const void* gb, void* out);
+    template <typename T> void axis_fft(void* data, int axis, bool inverse);
+    template <typename T> void axis_cheb(void* data, int axis, bool forward);
+    const void* cheb_matrix(int n, bool forward);
+
+    Context* ctx_;
+    DomainDesc d_;
+    int cshape_[3];
+    DevBuf scratch_[4];
+};
+
+}  // namespace tarang

@@ -1,0 +1,21 @@
+"""tarang_amd -- MI355X (gfx950) backend for Tarang.jl's pseudo-spectral timestep.
+
+The public names mirror Tarang.jl (src/public_api.jl:19-48); Julia's `!` suffix
+is spelled with a trailing underscore.
+"""
+from ._lib import TarangError, LIB_PATH, load as load_library
+from .core import (CPU, GPU, CartesianCoordinates, ChannelDomain, ChannelDomain3D, ChebyshevT, Coordinate, Distributor,
+                   Domain, GPUTransformUnsupported, PeriodicDomain, RealFourier, ScalarField, VectorField,
+                   backward_transform_, ensure_layout_, forward_transform_, get_grid_coordinates, padded_multiply, set_)
+from .problems import IVP, add_bc_, add_equation_, add_parameters_
+from .solvers import (RK111, RK222, RK443, SBDF1, SBDF2, DiagonalIMEX_RK222, DiagonalIMEX_RK443, DiagonalIMEX_SBDF2,
+                      InitialValueSolver, run_, step_)
+
+__all__ = [
+    "CPU", "GPU", "CartesianCoordinates", "ChannelDomain", "ChannelDomain3D", "ChebyshevT", "Coordinate",
+    "Distributor", "Domain", "GPUTransformUnsupported", "PeriodicDomain", "RealFourier", "ScalarField", "VectorField",
+    "backward_transform_", "ensure_layout_", "forward_transform_", "get_grid_coordinates", "padded_multiply", "set_",
+    "IVP", "add_bc_", "add_equation_", "add_parameters_", "RK111", "RK222", "RK443", "SBDF1", "SBDF2",
+    "DiagonalIMEX_RK222", "DiagonalIMEX_RK443", "DiagonalIMEX_SBDF2", "InitialValueSolver", "run_", "step_",
+    "TarangError", "LIB_PATH", "load_library",
+]

@@ -1,0 +1,116 @@
+"""ctypes binding of libtarang_hip.so (declared in include/tarang_hip.h).
+
+The library is loaded from tarang.jl_amd/lib/.  There is no fallback: if the
+library is missing or fails to load, every GPU entry point raises.
+"""
+import ctypes
+import os
+
+_PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG_ROOT, "lib", "libtarang_hip.so")
+
+OK, ERR_INVALID, ERR_HIP, ERR_SINGULAR, ERR_CAPACITY, ERR_STATE = range(6)
+F32, F64 = 0, 1
+REALFOURIER, CHEBYSHEVT = 0, 1
+
+
+class TarangError(RuntimeError):
+    """Raised when a library call returns a non-zero status."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"[tarang_hip status {code}] {msg}")
+        self.code = code
+
+
+class DomainDesc(ctypes.Structure):
+    _fields_ = [
+        ("ndim", ctypes.c_int),
+        ("kind", ctypes.c_int * 3),
+        ("n", ctypes.c_int * 3),
+        ("lo", ctypes.c_double * 3),
+        ("hi", ctypes.c_double * 3),
+        ("dealias", ctypes.c_double * 3),
+        ("dtype", ctypes.c_int),
+    ]
+
+
+class FivpDesc(ctypes.Structure):
+    _fields_ = [
+        ("ndim", ctypes.c_int),
+        ("n", ctypes.c_int * 2),
+        ("L", ctypes.c_double * 2),
+        ("dtype", ctypes.c_int),
+        ("problem", ctypes.c_int),
+        ("timestepper", ctypes.c_int),
+        ("nu", ctypes.c_double),
+        ("dt", ctypes.c_double),
+    ]
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_PD = ctypes.POINTER(ctypes.c_double)
+_PI = ctypes.POINTER(ctypes.c_int)
+_PP = ctypes.POINTER(ctypes.c_void_p)
+
+# name -> argtypes (every function returns int status)
+SIGNATURES = {
+    "tarang_version": [_PI, _PI],
+    "tarang_ctx_create": [_I, _PP],
+    "tarang_ctx_destroy": [_P],
+    "tarang_ctx_set_stream": [_P, _P],
+    "tarang_ctx_sync": [_P],
+    "tarang_buf_alloc": [_P, ctypes.c_size_t, _PP],
+    "tarang_buf_free": [_P, _P],
+    "tarang_h2d": [_P, _P, _P, ctypes.c_size_t],
+    "tarang_d2h": [_P, _P, _P, ctypes.c_size_t],
+    "tarang_plan_create": [_P, ctypes.POINTER(DomainDesc), _PP],
+    "tarang_plan_destroy": [_P],
+    "tarang_plan_coeff_shape": [_P, _PI, _PI],
+    "tarang_forward_transform": [_P, _P, _P],
+    "tarang_backward_transform": [_P, _P, _P],
+    "tarang_padded_multiply": [_P, _P, _P, _P],
+    "tarang_fivp_create": [_P, ctypes.POINTER(FivpDesc), _PP],
+    "tarang_fivp_destroy": [_P],
+    "tarang_fivp_set_state": [_P, _P],
+    "tarang_fivp_get_state": [_P, _P],
+    "tarang_fivp_rhs": [_P, _P, _P],
+    "tarang_fivp_aux": [_P, _P, _P, _P, _P],
+    "tarang_fivp_step": [_P, _I],
+    "tarang_fivp_set_graph": [_P, _I],
+    "tarang_fivp_set_dt": [_P, _D],
+    "tarang_fivp_time": [_P, _PD, ctypes.POINTER(ctypes.c_long)],
+    "tarang_fivp_profile": [_P, _I, _PD, ctypes.POINTER(ctypes.c_long)],
+    "tarang_fivp_kernel_bytes": [_P, _PD, _PD, _PD],
+}
+
+_lib = None
+
+
+def load():
+    """Load the library (raises OSError if it is missing)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OSError(f"libtarang_hip.so not found at {LIB_PATH}; build it with "
+                      f"`python tarang.jl_amd/build.py` (or __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    L.tarang_last_error.restype = ctypes.c_char_p
+    L.tarang_last_error.argtypes = []
+    for name, args in SIGNATURES.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def call(name, *args):
+    L = load()
+    rc = getattr(L, name)(*args)
+    if rc != OK:
+        msg = L.tarang_last_error()
+        raise TarangError(rc, msg.decode() if msg else "")
+    return rc

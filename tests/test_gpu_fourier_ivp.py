@@ -1,0 +1,145 @@
+"""Pure-Fourier IVPs on the GPU vs the FP64 oracle (RHS, single steps, trajectories)."""
+import numpy as np
+import pytest
+
+from tarang_amd import configs
+from tests.conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+DIFF, TURB, BURG = 0, 1, 2
+TS = {"RK222": 0, "RK443": 1, "SBDF2": 2}
+TOL = {np.float64: 1e-12, np.float32: 1e-5}
+
+
+def turb_ic(N, seed=7, kpeak=4.0):
+    """Seeded random-phase vorticity, E(k) ~ k^4 exp(-2 (k/kpeak)^2), E = 0.5."""
+    return configs.turbulence_2d(N, seed=seed, kpeak=kpeak)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("N", [16, 64, 256])
+def test_turbulence_rhs_parity(gpu, oracle, dtype, N):
+    print(f"turbulence rhs N={N} {dtype.__name__}", flush=True)
+    d = oracle.domain([0, 0], [N, N])
+    q = turb_ic(N)
+    C = oracle.forward_transform(d, q)
+    ref = oracle.FourierIVP(d, oracle.PROB_TURB2D, "RK443", 1e-4, 1e-3).rhs(C)
+    s = gpu.fivp([N, N], [2 * np.pi, 2 * np.pi], TURB, TS["RK443"], 1e-4, 1e-3, dtype=dtype)
+    F = s.rhs(C)
+    assert relerr(F, ref) < TOL[dtype] * (10 if dtype == np.float32 else 1)
+    s.close()
+
+
+def test_turbulence_rhs_closed_form(gpu, oracle, golden):
+    g = golden("ivp.npz")
+    d = oracle.domain([0, 0], [16, 16])
+    s = gpu.fivp([16, 16], [2 * np.pi, 2 * np.pi], TURB, TS["RK443"], 0.0, 1e-3)
+    F = s.rhs(oracle.forward_transform(d, g["turb_q"]))
+    assert np.abs(oracle.backward_transform(d, F) - g["turb_F"]).max() < 1e-12
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("N", [32, 4096])
+def test_burgers_rhs_parity(gpu, oracle, dtype, N):
+    d = oracle.domain([0], [N])
+    x = 2 * np.pi * np.arange(N) / N
+    u = np.sin(x) + 0.1 * np.cos(3 * x + 0.3) + 0.05 * np.sin(7 * x)
+    C = oracle.forward_transform(d, u)
+    ref = oracle.FourierIVP(d, oracle.PROB_BURGERS1D, "RK222", 1e-2, 1e-3).rhs(C)
+    s = gpu.fivp([N], [2 * np.pi], BURG, TS["RK222"], 1e-2, 1e-3, dtype=dtype)
+    assert relerr(s.rhs(C), ref) < TOL[dtype] * (10 if dtype == np.float32 else 1)
+    s.close()
+
+
+@pytest.mark.parametrize("ts", ["RK222", "RK443", "SBDF2"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_turbulence_steps_parity(gpu, oracle, ts, dtype):
+    N, nu, dt, nsteps = 64, 1e-3, 2e-3, 5
+    print(f"turbulence steps {ts} {dtype.__name__}", flush=True)
+    d = oracle.domain([0, 0], [N, N])
+    C = oracle.forward_transform(d, turb_ic(N))
+    ref = oracle.FourierIVP(d, oracle.PROB_TURB2D, ts, nu, dt)
+    ref.set_coeff(C)
+    s = gpu.fivp([N, N], [2 * np.pi, 2 * np.pi], TURB, TS[ts], nu, dt, dtype=dtype)
+    s.set_state(C)
+    ref.step(1)
+    s.step(1)
+    assert relerr(s.get_state(), ref.get_coeff()) < TOL[dtype], "single step"
+    ref.step(nsteps - 1)
+    s.step(nsteps - 1)
+    assert relerr(s.get_state(), ref.get_coeff()) < TOL[dtype] * 10, "multi step"
+    s.close()
+
+
+def test_graph_replay_matches_eager(gpu, oracle):
+    N = 64
+    d = oracle.domain([0, 0], [N, N])
+    C = oracle.forward_transform(d, turb_ic(N))
+    out = []
+    for graph in (False, True):
+        s = gpu.fivp([N, N], [2 * np.pi, 2 * np.pi], TURB, TS["RK443"], 1e-3, 2e-3)
+        s.set_graph(graph)
+        s.set_state(C)
+        s.step(3)
+        out.append(s.get_state())
+        s.close()
+    assert np.array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("ts", ["RK222", "RK443", "SBDF2"])
+def test_diffusion_exact_decay(gpu, oracle, golden, ts):
+    g = golden("ivp.npz")
+    d = oracle.domain([0], [64])
+    s = gpu.fivp([64], [2 * np.pi], DIFF, TS[ts], 0.01, 0.01)
+    s.set_state(oracle.forward_transform(d, g["diff_T0"]))
+    s.step(100)
+    C = s.get_state()
+    ref = oracle.FourierIVP(d, oracle.PROB_DIFFUSION, ts, 0.01, 0.01)
+    ref.set_coeff(oracle.forward_transform(d, g["diff_T0"]))
+    ref.step(100)
+    assert relerr(C, ref.get_coeff()) < 1e-12
+    assert relerr(oracle.backward_transform(d, C), g["diff_T1"]) < 2e-5
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_burgers_trajectory(gpu, oracle, dtype):
+    N, nu, dt = 256, 1e-2, 1e-3
+    d = oracle.domain([0], [N])
+    x = 2 * np.pi * np.arange(N) / N
+    r = np.random.default_rng(9)
+    u0 = np.sin(x) + 0.1 * sum(r.standard_normal() * np.cos(k * x + r.uniform(0, 6.3)) for k in range(1, 5))
+    C = oracle.forward_transform(d, u0)
+    ref = oracle.FourierIVP(d, oracle.PROB_BURGERS1D, "RK222", nu, dt)
+    ref.set_coeff(C)
+    ref.step(200)
+    s = gpu.fivp([N], [2 * np.pi], BURG, TS["RK222"], nu, dt, dtype=dtype)
+    s.set_state(C)
+    s.step(200)
+    assert relerr(s.get_state(), ref.get_coeff()) < (1e-11 if dtype == np.float64 else 1e-4)
+    s.close()
+
+
+def test_full_size_turbulence_fp32_vs_fp64(gpu, oracle):
+    """1024^2 (the benchmark grid): size-independent checks -- FP32 tracks FP64,
+    energy and enstrophy decay, one RHS matches the oracle."""
+    N = 1024
+    q = turb_ic(N, kpeak=10.0)
+    d = oracle.domain([0, 0], [N, N])
+    C = oracle.forward_transform(d, q)
+    ref_F = oracle.FourierIVP(d, oracle.PROB_TURB2D, "RK443", 1e-4, 1e-3).rhs(C)
+    states = {}
+    for dtype in (np.float64, np.float32):
+        s = gpu.fivp([N, N], [2 * np.pi, 2 * np.pi], TURB, TS["RK443"], 1e-4, 1e-3, dtype=dtype)
+        if dtype == np.float64:
+            assert relerr(s.rhs(C), ref_F) < 1e-12
+        s.set_state(C)
+        s.step(20)
+        states[dtype] = s.get_state()
+        s.close()
+    assert relerr(states[np.float32], states[np.float64]) < 1e-4
+    w = np.full(C.shape[0], 2.0)
+    w[0] = w[-1] = 1.0  # half-spectrum weights of a real field
+    Z0 = np.sum(w[:, None] * np.abs(C) ** 2)
+    Z1 = np.sum(w[:, None] * np.abs(states[np.float64]) ** 2)
+    assert Z1 < Z0

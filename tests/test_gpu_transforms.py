@@ -1,0 +1,76 @@
+"""GPU transforms and the padded product through the C ABI vs the FP64 oracle.
+Tolerances: 1e-12 relative on the FP64 path, 1e-5 on the FP32 path."""
+import numpy as np
+import pytest
+
+from tests.conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+RF, CH = 0, 1
+TOL = {np.float64: 1e-12, np.float32: 1e-5}
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("kinds,sizes,bounds", [
+    ([RF], [64], None),
+    ([RF], [4096], None),
+    ([RF, RF], [64, 32], None),
+    ([RF, RF], [1024, 1024], None),
+    ([RF, RF, RF], [32, 16, 8], None),
+    ([RF, CH], [64, 17], [(0.0, 4.0), (0.0, 1.0)]),
+    ([RF, CH], [256, 64], [(0.0, 4.0), (0.0, 1.0)]),
+    ([RF, RF, CH], [32, 16, 9], [(0.0, 4.0), (0.0, 4.0), (0.0, 1.0)]),
+])
+def test_transform_parity(gpu, oracle, dtype, kinds, sizes, bounds):
+    print(f"transform parity {kinds} {sizes} {dtype.__name__}", flush=True)
+    r = np.random.default_rng(11)
+    g = r.standard_normal(sizes)
+    d = oracle.domain(kinds, sizes, bounds=bounds or [(0.0, 2 * np.pi)] * len(sizes))
+    C_ref = oracle.forward_transform(d, g)
+    p = gpu.plan(kinds, sizes, bounds=bounds, dtype=dtype)
+    C = p.forward(g)
+    assert C.shape == C_ref.shape
+    assert relerr(C, C_ref) < TOL[dtype]
+    G = p.backward(C_ref)
+    assert relerr(G, oracle.backward_transform(d, C_ref)) < TOL[dtype]
+    assert relerr(p.backward(C), g) < TOL[dtype] * 4
+    p.close()
+
+
+def test_transform_mode_ordering_bit_exact(gpu):
+    # a single Fourier mode lands on exactly one coefficient index (FFT order on y)
+    nx, ny = 16, 8
+    x = 2 * np.pi * np.arange(nx) / nx
+    y = 2 * np.pi * np.arange(ny) / ny
+    p = gpu.plan([RF, RF], [nx, ny])
+    for kx, ky in [(3, 2), (3, -2), (0, -3), (8, 0)]:
+        g = np.cos(kx * x[:, None] + ky * y[None, :])
+        C = p.forward(g)
+        big = np.argwhere(np.abs(C) > 1e-9 * np.abs(C).max())
+        iy = ky % ny
+        expected = {(kx, iy)} if kx not in (0, nx // 2) else {(kx, iy), (kx, (-ky) % ny)}
+        assert {tuple(b) for b in big} == expected, (kx, ky, big)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("sizes", [[8], [64], [4096], [8, 8], [64, 32], [256, 256]])
+def test_padded_product_parity(gpu, oracle, golden, dtype, sizes):
+    print(f"padded product parity {sizes} {dtype.__name__}", flush=True)
+    r = np.random.default_rng(12)
+    a, b = r.standard_normal(sizes), r.standard_normal(sizes)
+    d = oracle.domain([RF] * len(sizes), sizes)
+    ref = oracle.padded_multiply(d, a, b)
+    p = gpu.plan([RF] * len(sizes), sizes, dtype=dtype)
+    out = p.multiply(a, b)
+    assert relerr(out, ref) < TOL[dtype] * (10 if dtype == np.float32 else 1)
+    p.close()
+
+
+def test_padded_product_golden(gpu, golden):
+    g = golden("product.npz")
+    p = gpu.plan([RF, RF], [8, 8])
+    out = p.multiply(g["p2_8x8_a"], g["p2_8x8_b"])
+    assert relerr(out, g["p2_8x8_out"]) < 1e-12
+    p1 = gpu.plan([RF], [16])
+    assert relerr(p1.multiply(g["p1_16_a"], g["p1_16_b"]), g["p1_16_out"]) < 1e-12
