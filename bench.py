@@ -42,6 +42,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="first run one plain device-to-device copy of this many MiB (counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (no collectives); run with --gpus 1",
@@ -55,6 +57,15 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     dev = torch.device("cuda", 0)
+    if a.copy_mib > 0:
+        n = a.copy_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {a.copy_mib} MiB read + {a.copy_mib} MiB written")
+        del src, dst
     log(f"building initial condition {N}x{N}")
     q0 = configs.turbulence_2d(N, seed=42, kpeak=10.0, energy=0.5)
     C0 = np.fft.fft(np.fft.rfft(q0, axis=0), axis=1)  # reference layout (kx, ky), unnormalised
@@ -80,9 +91,13 @@ def main():
     results = {}
     finals = {}
     solvers = {}
+    tuning = {}
     for name, code, ct in (("f64", _lib.F64, torch.complex128), ("f32", _lib.F32, torch.complex64)):
         s = make(code)
         solvers[name] = s
+        tv = (ctypes.c_int * 3)()
+        _lib.call("tarang_fivp_get_tuning", s, tv)
+        tuning[name] = {"ybwd": tv[0], "xpass": tv[1], "yfwd": tv[2]}
         Cd = torch.as_tensor(np.ascontiguousarray(C0.T), device=dev).to(ct).contiguous()
         torch.cuda.synchronize()
         _lib.call("tarang_fivp_set_state", s, ctypes.c_void_p(Cd.data_ptr()))
@@ -178,6 +193,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": KERNELS[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "kernels_f32": kernels,
+        "autotuned_variants": tuning,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

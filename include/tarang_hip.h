@@ -128,6 +128,11 @@ int tarang_fivp_time(tarang_fivp_t s, double* t, long* iteration);
  * returns the average milliseconds per launch of [y-backward, x-pass, y-forward, stage update]
  * and the launch counts (counts4 may be NULL).  Measurement support; no reference counterpart. */
 int tarang_fivp_profile(tarang_fivp_t s, int nsteps, double* avg_ms4, long* counts4);
+/* kernel variants of [y-backward, x-pass, y-forward] picked by the autotuner at creation
+ * (tile width / workgroup size, timed on the solver's own buffers; TARANG_AUTOTUNE=0 keeps the
+ * defaults); set_tuning forces a choice (0..3, 0..1, 0..3).  No reference counterpart. */
+int tarang_fivp_get_tuning(tarang_fivp_t s, int* variants3);
+int tarang_fivp_set_tuning(tarang_fivp_t s, const int* variants3);
 /* algorithmic bytes per launch of the three RHS kernels (y-backward, x-pass, y-forward) */
 int tarang_fivp_kernel_bytes(tarang_fivp_t s, double* ybwd, double* xpass, double* yfwd);
 

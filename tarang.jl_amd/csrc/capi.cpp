@@ -297,6 +297,32 @@ int tarang_fivp_profile(tarang_fivp_t s, int nsteps, double* avg_ms4, long* coun
     });
 }
 
+int tarang_fivp_get_tuning(tarang_fivp_t s, int* variants3) {
+    return guard([&] {
+        NEED(s, "solver");
+        NEED(variants3, "variants3");
+        const auto t = s->solver->tuning();
+        variants3[0] = t.ybwd;
+        variants3[1] = t.xpass;
+        variants3[2] = t.yfwd;
+    });
+}
+
+int tarang_fivp_set_tuning(tarang_fivp_t s, const int* variants3) {
+    return guard([&] {
+        NEED(s, "solver");
+        NEED(variants3, "variants3");
+        TG_REQUIRE(variants3[0] >= 0 && variants3[0] < 4 && variants3[1] >= 0 && variants3[1] < 2 &&
+                       variants3[2] >= 0 && variants3[2] < 4,
+                   tarang::kErrInvalid, "variant index out of range");
+        tarang::FourierTuning t;
+        t.ybwd = variants3[0];
+        t.xpass = variants3[1];
+        t.yfwd = variants3[2];
+        s->solver->set_tuning(t);
+    });
+}
+
 int tarang_fivp_kernel_bytes(tarang_fivp_t s, double* ybwd, double* xpass, double* yfwd) {
     return guard([&] {
         NEED(s, "solver");

@@ -1,15 +1,19 @@
 // fft_lds.hpp -- batched complex FFTs of compile-time length L staged in LDS.
 //
 // Stockham autosort passes (radix 8, 4, 2, 3, 5).  NLINES lines of length L sit
-// in LDS at stride LDL; NT threads share the butterflies of every pass.  Each
-// pass is done in place: every thread first loads all of its butterflies'
-// inputs into registers, the workgroup synchronises, then the twiddled
-// butterflies are written to their Stockham output slots.
+// in LDS; NT threads share the butterflies of every pass.  Each pass is done in
+// place: every thread first loads all of its butterflies' inputs into
+// registers, the workgroup synchronises, then the twiddled butterflies are
+// written to their Stockham output slots.
+//
+// LDS layout: element i of a line lives at pidx(i) = i + i/16 (one pad slot
+// per 16 complex values), so the stride-R writes of the early passes do not
+// collide on LDS banks.  Lines are LINE_STRIDE apart.  Twiddles are read from
+// an LDS copy of tw[m] = exp(-2 pi i m/L) (built in FP64 on the host).
 //
 // Conventions: forward = exp(-2 pi i jk/L), inverse = exp(+2 pi i jk/L), both
 // unnormalised (the FFTW convention the reference relies on,
-// src/core/transforms/transform_fourier.jl:473-590).  Twiddles come from a
-// global table tw[m] = exp(-2 pi i m/L), m < L, built in FP64 on the host.
+// src/core/transforms/transform_fourier.jl:473-590).
 #pragma once
 #include "common.hpp"
 
@@ -33,6 +37,17 @@ constexpr RadixPlan radix_plan(int L) {
 }
 
 constexpr bool fft_length_supported(int L) { return L >= 1 && radix_plan(L).n >= 0; }
+
+// padded LDS index of element i within a line
+__host__ __device__ constexpr int pidx(int i) { return i + (i >> 4); }
+// distance between consecutive lines of length L
+__host__ __device__ constexpr int line_stride(int L) { return pidx(L - 1) + 1 + ((pidx(L - 1) + 1) & 1); }
+
+// copy the global twiddle table into LDS (caller synchronises before use)
+template <typename T, int L, int NT>
+__device__ inline void load_twiddles(cplx<T>* twl, const cplx<T>* __restrict__ tw) {
+    for (int m = threadIdx.x; m < L; m += NT) twl[m] = tw[m];
+}
 
 // ---- small DFT kernels on registers; INV selects exp(+...) ----
 template <bool INV, typename V>
@@ -70,7 +85,6 @@ __device__ inline void dft_small(cplx<T>* v) {
             t = b[3];  // * (-1 -+ i)/sqrt2
             b[3] = INV ? V{(-t.x - t.y) * h, (t.x - t.y) * h} : V{(t.y - t.x) * h, (-t.x - t.y) * h};
         }
-        // two radix-4 DFTs
         V a0 = cadd(a[0], a[2]), a1 = csub(a[0], a[2]);
         V a2 = cadd(a[1], a[3]), a3 = rot_mi<INV>(csub(a[1], a[3]));
         V b0 = cadd(b[0], b[2]), b1 = csub(b[0], b[2]);
@@ -102,7 +116,6 @@ __device__ inline void dft_small(cplx<T>* v) {
         V m2 = V{a.x + c2 * t1.x + c1 * t2.x, a.y + c2 * t1.y + c1 * t2.y};
         V n1 = V{s1 * u1.x + s2 * u2.x, s1 * u1.y + s2 * u2.y};
         V n2 = V{s2 * u1.x - s1 * u2.x, s2 * u1.y - s1 * u2.y};
-        // i * n
         V in1 = V{-n1.y, n1.x}, in2 = V{-n2.y, n2.x};
         v[0] = cadd(a, cadd(t1, t2));
         v[1] = cadd(m1, in1);
@@ -113,22 +126,23 @@ __device__ inline void dft_small(cplx<T>* v) {
 }
 
 // One Stockham pass of radix R after NS points have been combined.
-template <typename T, int L, int R, int NS, int NLINES, int LDL, int NT, bool INV>
-__device__ inline void stockham_pass(cplx<T>* buf, const cplx<T>* __restrict__ tw) {
+template <typename T, int L, int R, int NS, int NLINES, int NT, bool INV>
+__device__ inline void stockham_pass(cplx<T>* buf, const cplx<T>* twl) {
     using V = cplx<T>;
     constexpr int NB = L / R;
     constexpr int TOT = NB * NLINES;
     constexpr int PER = (TOT + NT - 1) / NT;
     constexpr int TSTEP = L / (NS * R);
+    constexpr int LS = line_stride(L);
     V v[PER][R];
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
         const int b = threadIdx.x + i * NT;
         if (TOT % NT == 0 || b < TOT) {
             const int line = b / NB, j = b - line * NB;
-            const V* src = buf + line * LDL + j;
+            const V* src = buf + line * LS;
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[i][r] = src[r * NB];
+            for (int r = 0; r < R; ++r) v[i][r] = src[pidx(j + r * NB)];
         }
     }
     __syncthreads();
@@ -141,34 +155,42 @@ __device__ inline void stockham_pass(cplx<T>* buf, const cplx<T>* __restrict__ t
             if (NS > 1) {
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
-                    V w = tw[r * k * TSTEP];
+                    V w = twl[r * k * TSTEP];
                     v[i][r] = INV ? cmulc(v[i][r], w) : cmul(v[i][r], w);
                 }
             }
             dft_small<R, INV, T>(v[i]);
-            V* dst = buf + line * LDL + (j / NS) * NS * R + k;
+            V* dst = buf + line * LS;
+            const int base = (j / NS) * NS * R + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) dst[r * NS] = v[i][r];
+            for (int r = 0; r < R; ++r) dst[pidx(base + r * NS)] = v[i][r];
         }
     }
     __syncthreads();
 }
 
-template <typename T, int L, int NLINES, int LDL, int NT, bool INV, int P = 0, int NS = 1>
-__device__ inline void fft_passes(cplx<T>* buf, const cplx<T>* __restrict__ tw) {
+template <typename T, int L, int NLINES, int NT, bool INV, int P = 0, int NS = 1>
+__device__ inline void fft_passes(cplx<T>* buf, const cplx<T>* twl) {
     constexpr RadixPlan plan = radix_plan(L);
     static_assert(plan.n >= 0, "FFT length must factor into 2, 3 and 5");
     if constexpr (P < plan.n) {
         constexpr int R = plan.r[P];
-        stockham_pass<T, L, R, NS, NLINES, LDL, NT, INV>(buf, tw);
-        fft_passes<T, L, NLINES, LDL, NT, INV, P + 1, NS * R>(buf, tw);
+        stockham_pass<T, L, R, NS, NLINES, NT, INV>(buf, twl);
+        fft_passes<T, L, NLINES, NT, INV, P + 1, NS * R>(buf, twl);
     }
 }
 
-// In-LDS batched FFT; caller must have synchronised after filling buf.
-template <typename T, int L, int NLINES, int LDL, int NT, bool INV>
-__device__ inline void fft_lds(cplx<T>* buf, const cplx<T>* __restrict__ tw) {
-    if constexpr (L > 1) fft_passes<T, L, NLINES, LDL, NT, INV>(buf, tw);
+// In-LDS batched FFT of NLINES padded lines; twl is the LDS twiddle table.
+// The caller must have synchronised after filling buf and twl.
+template <typename T, int L, int NLINES, int NT, bool INV>
+__device__ inline void fft_lds(cplx<T>* buf, const cplx<T>* twl) {
+    if constexpr (L > 1) fft_passes<T, L, NLINES, NT, INV>(buf, twl);
+}
+
+// LDS bytes of NLINES padded lines plus the twiddle table
+template <typename T, int L, int NLINES>
+constexpr size_t fft_lds_bytes() {
+    return sizeof(cplx<T>) * ((size_t)NLINES * line_stride(L) + L);
 }
 
 }  // namespace tarang

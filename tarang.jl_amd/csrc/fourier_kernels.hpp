@@ -95,60 +95,83 @@ struct BurgersSpec {
 };
 
 // ---------------------------------------------------------------------------
-// k_ybwd: per tile of TK kx columns; for each field: build the padded column
-// (multiplier, Hermitian projection at kx=0 / kx=Nx/2, Nyquist split), inverse
-// FFT of length MY, store to g[f] (nxh x MY, kx fastest).
+// k_ybwd: one field and one tile of TK kx columns per workgroup; build the
+// padded column (multiplier, Hermitian projection at kx=0 / kx=Nx/2, Nyquist
+// split), inverse FFT of length MY, store to g[f] (nxh x MY, kx fastest).
+// Grid: (ceil(nxh/TK), NF); the launch id is remapped so that tiles sharing
+// cache lines run on the same XCD.
 // ---------------------------------------------------------------------------
 template <typename T, int NY, int MY, int TK, int NT, typename Spec>
 __global__ __launch_bounds__(NT) void k_ybwd(FourierParams<T> p, cplx<T>* __restrict__ g, size_t gstride,
                                              const cplx<T>* __restrict__ tw) {
     using V = cplx<T>;
-    constexpr int LDL = MY;
+    constexpr int LS = line_stride(MY);
     constexpr bool EVEN = (NY % 2) == 0;
     constexpr int NN = NY - NY / 2 - 1;  // negative frequencies kept
-    __shared__ V buf[TK * LDL];
-    const int tile = (int)xcd_remap(blockIdx.x, gridDim.x);
-    const int kx0 = tile * TK;
+    __shared__ V buf[TK * LS];
+    __shared__ V twl[MY];
+    const unsigned ntiles = gridDim.x;
+    const unsigned lin = xcd_remap(blockIdx.x + blockIdx.y * ntiles, ntiles * gridDim.y);
+    const int f = (int)(lin / ntiles);
+    const int kx0 = (int)(lin % ntiles) * TK;
     const bool xeven = (p.nx % 2) == 0;
-    for (int f = 0; f < Spec::NF; ++f) {
-        for (int idx = threadIdx.x; idx < TK * MY; idx += NT) {
-            const int c = idx % TK, d = idx / TK;
+    load_twiddles<T, MY, NT>(twl, tw);
+    // 1) issue every source load of this thread before touching LDS
+    constexpr int NSRC = TK * NY;
+    constexpr int PS = (NSRC + NT - 1) / NT;
+    V va[PS], vb[PS];
+#pragma unroll
+    for (int e = 0; e < PS; ++e) {
+        const int idx = threadIdx.x + e * NT;
+        va[e] = vb[e] = V{T(0), T(0)};
+        if (NSRC % NT == 0 || idx < NSRC) {
+            const int c = idx % TK, s = idx / TK;
             const int kx = kx0 + c;
-            V val{T(0), T(0)};
             if (kx < p.nxh) {
-                int s = -1;
-                T w = T(1);
-                if (d <= NY / 2) {
-                    s = d;
-                    if (EVEN && d == NY / 2) w = T(0.5);
-                } else if (EVEN && d == MY - NY / 2) {
-                    s = NY / 2;
-                    w = T(0.5);
-                } else if (d >= MY - NN) {
-                    s = NY - (MY - d);
-                }
-                if (s >= 0) {
-                    V a = Spec::value(f, p, kx, s);
-                    const bool xnyq = xeven && kx == p.nx / 2;
-                    if (kx == 0 || xnyq) {
-                        const V b = Spec::value(f, p, kx, (NY - s) % NY);
-                        a = V{(a.x + b.x) * T(0.5), (a.y - b.y) * T(0.5)};
-                        if (xnyq) w *= T(0.5);
-                    }
-                    val = V{a.x * w, a.y * w};
-                }
+                va[e] = Spec::value(f, p, kx, s);
+                if (kx == 0 || (xeven && kx == p.nx / 2)) vb[e] = Spec::value(f, p, kx, (NY - s) % NY);
             }
-            buf[c * LDL + d] = val;
         }
-        __syncthreads();
-        fft_lds<T, MY, TK, LDL, NT, true>(buf, tw);
-        V* out = g + (size_t)f * gstride;
-        for (int idx = threadIdx.x; idx < TK * MY; idx += NT) {
-            const int c = idx % TK, d = idx / TK;
+    }
+    // 2) zero the padding gap between +NY/2 and the negative frequencies
+    constexpr int GAP0 = NY / 2 + 1, GAP1 = EVEN ? MY - NY / 2 : MY - NN;  // [GAP0, GAP1)
+    for (int idx = threadIdx.x; idx < TK * (GAP1 - GAP0); idx += NT) {
+        const int c = idx % TK, d = GAP0 + idx / TK;
+        buf[c * LS + pidx(d)] = V{T(0), T(0)};
+    }
+    // 3) projection (kx = 0, Nx/2), Nyquist split, placement
+#pragma unroll
+    for (int e = 0; e < PS; ++e) {
+        const int idx = threadIdx.x + e * NT;
+        if (NSRC % NT == 0 || idx < NSRC) {
+            const int c = idx % TK, s = idx / TK;
             const int kx = kx0 + c;
-            if (kx < p.nxh) out[(size_t)d * p.nxh + kx] = buf[c * LDL + d];
+            V a = va[e];
+            T w = T(1);
+            const bool xnyq = xeven && kx == p.nx / 2;
+            if (kx == 0 || xnyq) {
+                a = V{(a.x + vb[e].x) * T(0.5), (a.y - vb[e].y) * T(0.5)};
+                if (xnyq) w = T(0.5);
+            }
+            if (kx >= p.nxh) a = V{T(0), T(0)};
+            if (EVEN && s == NY / 2) {
+                w *= T(0.5);
+                const V h{a.x * w, a.y * w};
+                buf[c * LS + pidx(NY / 2)] = h;
+                buf[c * LS + pidx(MY - NY / 2)] = h;
+            } else {
+                const int d = s <= NY / 2 ? s : MY - (NY - s);
+                buf[c * LS + pidx(d)] = V{a.x * w, a.y * w};
+            }
         }
-        __syncthreads();
+    }
+    __syncthreads();
+    fft_lds<T, MY, TK, NT, true>(buf, twl);
+    V* out = g + (size_t)f * gstride;
+    for (int idx = threadIdx.x; idx < TK * MY; idx += NT) {
+        const int c = idx % TK, d = idx / TK;
+        const int kx = kx0 + c;
+        if (kx < p.nxh) out[(size_t)d * p.nxh + kx] = buf[c * LS + pidx(d)];
     }
 }
 
@@ -164,29 +187,50 @@ __global__ __launch_bounds__(NT) void k_xpass(const cplx<T>* __restrict__ g, siz
     using V = cplx<T>;
     constexpr int NP = Spec::NF / 2;
     constexpr int NL = 2 * NP;
-    constexpr int LDL = MX;
-    __shared__ V buf[NL * LDL];
+    constexpr int LS = line_stride(MX);
+    __shared__ V buf[NL * LS];
+    __shared__ V twl[MX];
     const int y0 = 2 * (int)xcd_remap(blockIdx.x, gridDim.x);
-    for (int idx = threadIdx.x; idx < NL * MX; idx += NT) {
-        const int line = idx / MX, k = idx - line * MX;
-        const int row = line / NP, pp = line - row * NP;
-        const size_t roff = (size_t)(y0 + row) * nxh;
-        const V* A = g + (size_t)(2 * pp) * gstride + roff;
-        const V* B = g + (size_t)(2 * pp + 1) * gstride + roff;
-        V z{T(0), T(0)};
-        if (k == 0) {
-            z = V{A[0].x, B[0].x};
-        } else if (k < nxh) {
-            const V a = A[k], b = B[k];
-            z = V{a.x - b.y, a.y + b.x};
-        } else if (k > MX - nxh) {
-            const V a = A[MX - k], b = B[MX - k];
-            z = V{a.x + b.y, b.x - a.y};
+    load_twiddles<T, MX, NT>(twl, tw);
+    // the stored half spectrum (k < nxh) of every packed line, loaded before any LDS write
+    constexpr int NXH = NX / 2 + 1;
+    constexpr int NSRC = NL * NXH;
+    constexpr int PS = (NSRC + NT - 1) / NT;
+    V va[PS], vb[PS];
+#pragma unroll
+    for (int e = 0; e < PS; ++e) {
+        const int idx = threadIdx.x + e * NT;
+        if (NSRC % NT == 0 || idx < NSRC) {
+            const int line = idx / NXH, k = idx - line * NXH;
+            const int row = line / NP, pp = line - row * NP;
+            const size_t off = (size_t)(y0 + row) * nxh + k;
+            va[e] = g[(size_t)(2 * pp) * gstride + off];
+            vb[e] = g[(size_t)(2 * pp + 1) * gstride + off];
         }
-        buf[line * LDL + k] = z;
+    }
+    // zero the band between the stored modes and their mirrors: [NXH, MX - NXH]
+    constexpr int GAPN = MX - 2 * NXH + 1;
+    for (int idx = threadIdx.x; idx < NL * GAPN; idx += NT) {
+        const int line = idx / GAPN, k = NXH + (idx - line * GAPN);
+        buf[line * LS + pidx(k)] = V{T(0), T(0)};
+    }
+#pragma unroll
+    for (int e = 0; e < PS; ++e) {
+        const int idx = threadIdx.x + e * NT;
+        if (NSRC % NT == 0 || idx < NSRC) {
+            const int line = idx / NXH, k = idx - line * NXH;
+            const V a = va[e], b = vb[e];
+            V* L = buf + line * LS;
+            if (k == 0) {
+                L[pidx(0)] = V{a.x, b.x};  // C2R keeps only the real part of the DC term
+            } else {
+                L[pidx(k)] = V{a.x - b.y, a.y + b.x};           // a + i b
+                L[pidx(MX - k)] = V{a.x + b.y, b.x - a.y};      // conj(a) + i conj(b)
+            }
+        }
     }
     __syncthreads();
-    fft_lds<T, MX, NL, LDL, NT, true>(buf, tw);
+    fft_lds<T, MX, NL, NT, true>(buf, twl);
     for (int x = threadIdx.x; x < MX; x += NT) {
         T pr[2];
 #pragma unroll
@@ -194,31 +238,31 @@ __global__ __launch_bounds__(NT) void k_xpass(const cplx<T>* __restrict__ g, siz
             T v[Spec::NF];
 #pragma unroll
             for (int pp = 0; pp < NP; ++pp) {
-                const V z = buf[(row * NP + pp) * LDL + x];
+                const V z = buf[(row * NP + pp) * LS + pidx(x)];
                 v[2 * pp] = z.x;
                 v[2 * pp + 1] = z.y;
             }
             pr[row] = Spec::combine(v) * scale;
         }
-        buf[x] = V{pr[0], pr[1]};
+        buf[pidx(x)] = V{pr[0], pr[1]};
     }
     __syncthreads();
-    fft_lds<T, MX, 1, LDL, NT, false>(buf, tw);
+    fft_lds<T, MX, 1, NT, false>(buf, twl);
     for (int k = threadIdx.x; k < nxh; k += NT) {
-        const V zk = buf[k], zm = buf[(MX - k) % MX];
+        const V zk = buf[pidx(k)], zm = buf[pidx((MX - k) % MX)];
         h[(size_t)y0 * nxh + k] = V{(zk.x + zm.x) * T(0.5), (zk.y - zm.y) * T(0.5)};
         h[(size_t)(y0 + 1) * nxh + k] = V{(zk.y + zm.y) * T(0.5), (zm.x - zk.x) * T(0.5)};
     }
 }
 
 // y truncation with the +-Ny/2 fold, reading the forward-transformed column c
-template <typename T, int NY, int MY, int LDL>
+template <typename T, int NY, int MY, int LS>
 __device__ inline cplx<T> ytrunc(const cplx<T>* buf, int c, int ky) {
     constexpr bool EVEN = (NY % 2) == 0;
-    const cplx<T>* col = buf + c * LDL;
-    if (ky < (NY + 1) / 2) return col[ky];
-    if (EVEN && ky == NY / 2) return cadd(col[NY / 2], col[MY - NY / 2]);
-    return col[MY - (NY - ky)];
+    const cplx<T>* col = buf + c * LS;
+    if (ky < (NY + 1) / 2) return col[pidx(ky)];
+    if (EVEN && ky == NY / 2) return cadd(col[pidx(NY / 2)], col[pidx(MY - NY / 2)]);
+    return col[pidx(MY - (NY - ky))];
 }
 
 // ---------------------------------------------------------------------------
@@ -229,24 +273,36 @@ template <typename T, int NY, int MY, int TK, int NT>
 __global__ __launch_bounds__(NT) void k_yfwd(const cplx<T>* __restrict__ h, int nxh, int nx,
                                              cplx<T>* __restrict__ F, const cplx<T>* __restrict__ tw) {
     using V = cplx<T>;
-    constexpr int LDL = MY;
-    __shared__ V buf[TK * LDL];
+    constexpr int LS = line_stride(MY);
+    __shared__ V buf[TK * LS];
+    __shared__ V twl[MY];
     const int tile = (int)xcd_remap(blockIdx.x, gridDim.x);
     const int kx0 = tile * TK;
-    for (int idx = threadIdx.x; idx < TK * MY; idx += NT) {
+    load_twiddles<T, MY, NT>(twl, tw);
+    constexpr int NSRC = TK * MY;
+    constexpr int PS = (NSRC + NT - 1) / NT;
+    V va[PS];
+#pragma unroll
+    for (int e = 0; e < PS; ++e) {
+        const int idx = threadIdx.x + e * NT;
         const int c = idx % TK, d = idx / TK;
         const int kx = kx0 + c;
-        buf[c * LDL + d] = kx < nxh ? h[(size_t)d * nxh + kx] : V{T(0), T(0)};
+        va[e] = ((NSRC % NT == 0 || idx < NSRC) && kx < nxh) ? h[(size_t)d * nxh + kx] : V{T(0), T(0)};
+    }
+#pragma unroll
+    for (int e = 0; e < PS; ++e) {
+        const int idx = threadIdx.x + e * NT;
+        if (NSRC % NT == 0 || idx < NSRC) buf[(idx % TK) * LS + pidx(idx / TK)] = va[e];
     }
     __syncthreads();
-    fft_lds<T, MY, TK, LDL, NT, false>(buf, tw);
+    fft_lds<T, MY, TK, NT, false>(buf, twl);
     const bool xeven = (nx % 2) == 0;
     for (int idx = threadIdx.x; idx < TK * NY; idx += NT) {
         const int c = idx % TK, ky = idx / TK;
         const int kx = kx0 + c;
         if (kx >= nxh) continue;
-        V v = ytrunc<T, NY, MY, LDL>(buf, c, ky);
-        if (xeven && kx == nx / 2) v = cadd(v, cconj(ytrunc<T, NY, MY, LDL>(buf, c, (NY - ky) % NY)));
+        V v = ytrunc<T, NY, MY, LS>(buf, c, ky);
+        if (xeven && kx == nx / 2) v = cadd(v, cconj(ytrunc<T, NY, MY, LS>(buf, c, (NY - ky) % NY)));
         F[(size_t)ky * nxh + kx] = v;
     }
 }
@@ -262,10 +318,15 @@ __global__ __launch_bounds__(NT) void k_line1d(FourierParams<T> p, size_t in_str
     using V = cplx<T>;
     constexpr int NH = N / 2 + 1;
     constexpr bool EVEN = (N % 2) == 0;
-    __shared__ V buf[M];
+    // long FP64 lines keep the twiddles in global memory (LDS holds the line only)
+    constexpr bool TW_LDS = (size_t)(line_stride(M) + M) * sizeof(V) <= 160 * 1024;
+    __shared__ V buf[line_stride(M)];
+    __shared__ V twl_lds[TW_LDS ? M : 1];
+    const V* twl = TW_LDS ? twl_lds : tw;
     const int line = blockIdx.x;
     p.in0 += (size_t)line * in_stride;
     if (p.in1) p.in1 += (size_t)line * in_stride;
+    if constexpr (TW_LDS) load_twiddles<T, M, NT>(twl_lds, tw);
     auto fetch = [&](int f, int k) -> V {
         V a = Spec::value(f, p, k, 0);
         if (k == 0) a.y = T(0);
@@ -281,20 +342,20 @@ __global__ __launch_bounds__(NT) void k_line1d(FourierParams<T> p, size_t in_str
             const V a = fetch(0, M - k), b = fetch(1, M - k);
             z = V{a.x + b.y, b.x - a.y};
         }
-        buf[k] = z;
+        buf[pidx(k)] = z;
     }
     __syncthreads();
-    fft_lds<T, M, 1, M, NT, true>(buf, tw);
+    fft_lds<T, M, 1, NT, true>(buf, twl);
     for (int x = threadIdx.x; x < M; x += NT) {
-        const V z = buf[x];
+        const V z = buf[pidx(x)];
         T v[2] = {z.x, z.y};
-        buf[x] = V{Spec::combine(v) * scale, T(0)};
+        buf[pidx(x)] = V{Spec::combine(v) * scale, T(0)};
     }
     __syncthreads();
-    fft_lds<T, M, 1, M, NT, false>(buf, tw);
+    fft_lds<T, M, 1, NT, false>(buf, twl);
     V* out = F + (size_t)line * out_stride;
     for (int k = threadIdx.x; k < NH; k += NT) {
-        V v = buf[k];
+        V v = buf[pidx(k)];
         if (EVEN && k == N / 2) v = V{T(2) * v.x, T(0)};
         out[k] = v;
     }

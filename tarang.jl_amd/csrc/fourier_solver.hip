@@ -1,8 +1,14 @@
 // fourier_solver.hip -- RealFourier IVP stepping: fused dealiased RHS, diagonal
 // IMEX RK / SBDF stage updates, whole-step hipGraph replay.
+#include <algorithm>
+#include <array>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <vector>
 
 #include "dispatch.hpp"
 #include "fourier_kernels.hpp"
@@ -117,6 +123,7 @@ FourierSolver::FourierSolver(Context* ctx, const FourierDesc& d) : ctx_(ctx), d_
         if (d.ndim == 2) ctx_->twiddle(my_, (DType)d.dtype);
     }
     TG_HIP(hipStreamSynchronize(ctx_->stream));
+    autotune();
 }
 
 FourierSolver::~FourierSolver() {
@@ -192,42 +199,190 @@ void FourierSolver::rhs_t(const void* Yv, void* Fv) {
     V* H = H_.as<V>();
     const size_t gstride = (size_t)nxh_ * my_;
     const T scale = T(1.0 / ((double)d_.n[0] * ny_ * (double)mx_ * my_));
+    launch_ybwd<T, TurbSpec>(tune_.ybwd, p, G, gstride);
+    launch_xpass<T, TurbSpec>(tune_.xpass, G, gstride, scale, H);
+    launch_yfwd<T>(tune_.yfwd, H, F);
+}
+
+// ---- kernel variants (tile width TK, workgroup size NT) chosen by the autotuner ----
+template <typename T, int M>
+struct YTiles {
+    static constexpr int TK0 = tile_cols<T, M>();
+    static constexpr int TK1 = TK0 > 1 ? TK0 / 2 : 1;
+    static constexpr int TK2 = TK0 < 8 ? TK0 * 2 : 8;
+    static constexpr bool fits(int tk) {
+        return (size_t)(tk * line_stride(M) + M) * sizeof(cplx<T>) <= 160 * 1024;
+    }
+};
+
+template <typename T, typename Spec>
+void FourierSolver::launch_ybwd(int variant, const FourierParams<T>& p, cplx<T>* G, size_t gstride) {
+    using V = cplx<T>;
+    const DType dt = std::is_same<T, float>::value ? kF32 : kF64;
+    hipStream_t s = ctx_->stream;
     dispatch_size(ny_, [&](auto NC) {
         constexpr int N = decltype(NC)::value;
         constexpr int M = padded_len(N);
-        constexpr int TK = tile_cols<T, M>();
+        using Y = YTiles<T, M>;
         const V* tw = static_cast<const V*>(ctx_->twiddle(M, dt));
-        const int nb = (nxh_ + TK - 1) / TK;
-        timed(0, [&] {
-            hipLaunchKernelGGL((k_ybwd<T, N, M, TK, NT, TurbSpec>), dim3(nb), dim3(NT), 0, s, p, G, gstride, tw);
-        });
+        auto go = [&](auto TKC, auto NTC) {
+            constexpr int TK = decltype(TKC)::value, NT = decltype(NTC)::value;
+            const int nb = (nxh_ + TK - 1) / TK;
+            timed(0, [&] {
+                hipLaunchKernelGGL((k_ybwd<T, N, M, TK, NT, Spec>), dim3(nb, Spec::NF), dim3(NT), 0, s, p, G, gstride,
+                                   tw);
+            });
+        };
+        if constexpr (N >= 64) {
+            if (variant == 1) return go(std::integral_constant<int, Y::TK1>{}, std::integral_constant<int, 256>{});
+            if (variant == 2) {
+                if constexpr (Y::fits(Y::TK2))
+                    return go(std::integral_constant<int, Y::TK2>{}, std::integral_constant<int, 512>{});
+            }
+            if (variant == 3) return go(std::integral_constant<int, Y::TK0>{}, std::integral_constant<int, 512>{});
+        }
+        go(std::integral_constant<int, Y::TK0>{}, std::integral_constant<int, 256>{});
     }, Sizes2D{});
     TG_HIP(hipGetLastError());
+}
+
+template <typename T, typename Spec>
+void FourierSolver::launch_xpass(int variant, const cplx<T>* G, size_t gstride, T scale, cplx<T>* H) {
+    using V = cplx<T>;
+    const DType dt = std::is_same<T, float>::value ? kF32 : kF64;
+    hipStream_t s = ctx_->stream;
     dispatch_size(d_.n[0], [&](auto NC) {
         constexpr int N = decltype(NC)::value;
         constexpr int M = padded_len(N);
-        if constexpr ((size_t)4 * M * sizeof(V) <= 160 * 1024) {
+        if constexpr ((size_t)(Spec::NF * line_stride(M) + M) * sizeof(V) <= 160 * 1024) {
             const V* tw = static_cast<const V*>(ctx_->twiddle(M, dt));
-            timed(1, [&] {
-                hipLaunchKernelGGL((k_xpass<T, N, M, NT, TurbSpec>), dim3(my_ / 2), dim3(NT), 0, s, G, gstride, nxh_,
-                                   scale, H, tw);
-            });
+            auto go = [&](auto NTC) {
+                constexpr int NT = decltype(NTC)::value;
+                timed(1, [&] {
+                    hipLaunchKernelGGL((k_xpass<T, N, M, NT, Spec>), dim3(my_ / 2), dim3(NT), 0, s, G, gstride, nxh_,
+                                       scale, H, tw);
+                });
+            };
+            if constexpr (N >= 64) {
+                if (variant == 1) return go(std::integral_constant<int, 512>{});
+            }
+            go(std::integral_constant<int, 256>{});
         } else {
             throw Error{kErrInvalid, "x-pass does not fit in LDS at this size/precision"};
         }
     }, Sizes2D{});
     TG_HIP(hipGetLastError());
+}
+
+template <typename T>
+void FourierSolver::launch_yfwd(int variant, const cplx<T>* H, cplx<T>* F) {
+    using V = cplx<T>;
+    const DType dt = std::is_same<T, float>::value ? kF32 : kF64;
+    hipStream_t s = ctx_->stream;
     dispatch_size(ny_, [&](auto NC) {
         constexpr int N = decltype(NC)::value;
         constexpr int M = padded_len(N);
-        constexpr int TK = tile_cols<T, M>();
+        using Y = YTiles<T, M>;
         const V* tw = static_cast<const V*>(ctx_->twiddle(M, dt));
-        const int nb = (nxh_ + TK - 1) / TK;
-        timed(2, [&] {
-            hipLaunchKernelGGL((k_yfwd<T, N, M, TK, NT>), dim3(nb), dim3(NT), 0, s, H, nxh_, d_.n[0], F, tw);
-        });
+        auto go = [&](auto TKC, auto NTC) {
+            constexpr int TK = decltype(TKC)::value, NT = decltype(NTC)::value;
+            const int nb = (nxh_ + TK - 1) / TK;
+            timed(2, [&] {
+                hipLaunchKernelGGL((k_yfwd<T, N, M, TK, NT>), dim3(nb), dim3(NT), 0, s, H, nxh_, d_.n[0], F, tw);
+            });
+        };
+        if constexpr (N >= 64) {
+            if (variant == 1) return go(std::integral_constant<int, Y::TK1>{}, std::integral_constant<int, 256>{});
+            if (variant == 2) {
+                if constexpr (Y::fits(Y::TK2))
+                    return go(std::integral_constant<int, Y::TK2>{}, std::integral_constant<int, 512>{});
+            }
+            if (variant == 3) return go(std::integral_constant<int, Y::TK0>{}, std::integral_constant<int, 512>{});
+        }
+        go(std::integral_constant<int, Y::TK0>{}, std::integral_constant<int, 256>{});
     }, Sizes2D{});
     TG_HIP(hipGetLastError());
+}
+
+// Times every variant of the three RHS kernels on this solver's own buffers and
+// keeps the fastest (HIP events, median of 5 launches); results are cached per
+// (problem, Nx, Ny, dtype) for the life of the process.
+static std::map<std::array<int, 4>, FourierTuning>& tuning_cache() {
+    static std::map<std::array<int, 4>, FourierTuning> m;
+    return m;
+}
+
+void FourierSolver::autotune() {
+    if (d_.ndim != 2 || d_.problem == kDiffusion) return;
+    const std::array<int, 4> key{d_.problem, d_.n[0], ny_, d_.dtype};
+    static std::mutex mu;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = tuning_cache().find(key);
+        if (it != tuning_cache().end()) {
+            tune_ = it->second;
+            return;
+        }
+    }
+    const char* env = std::getenv("TARANG_AUTOTUNE");
+    if (env && std::string(env) == "0") return;
+    ctx_->activate();
+    hipEvent_t a, b;
+    TG_HIP(hipEventCreate(&a));
+    TG_HIP(hipEventCreate(&b));
+    auto measure = [&](auto&& launch) {
+        std::vector<float> v;
+        for (int r = 0; r < 6; ++r) {
+            TG_HIP(hipEventRecord(a, ctx_->stream));
+            launch();
+            TG_HIP(hipEventRecord(b, ctx_->stream));
+            TG_HIP(hipEventSynchronize(b));
+            float ms = 0;
+            TG_HIP(hipEventElapsedTime(&ms, a, b));
+            if (r > 0) v.push_back(ms);
+        }
+        std::sort(v.begin(), v.end());
+        return v[v.size() / 2];
+    };
+    auto run = [&](auto tag) {
+        using T = decltype(tag);
+        using V = cplx<T>;
+        FourierParams<T> p;
+        p.in0 = X_.as<V>();
+        p.in1 = nullptr;
+        p.nxh = nxh_;
+        p.nx = d_.n[0];
+        p.ny = ny_;
+        p.k0x = T(2.0 * M_PI / d_.L[0]);
+        p.k0y = T(2.0 * M_PI / d_.L[1]);
+        const size_t gstride = (size_t)nxh_ * my_;
+        V* G = G_.as<V>();
+        V* H = H_.as<V>();
+        V* F = F_[0].as<V>();
+        float best = 1e30f;
+        for (int v = 0; v < 4; ++v) {
+            const float t = measure([&] { launch_ybwd<T, TurbSpec>(v, p, G, gstride); });
+            if (t < best) { best = t; tune_.ybwd = v; }
+        }
+        best = 1e30f;
+        for (int v = 0; v < 2; ++v) {
+            const float t = measure([&] { launch_xpass<T, TurbSpec>(v, G, gstride, T(1), H); });
+            if (t < best) { best = t; tune_.xpass = v; }
+        }
+        best = 1e30f;
+        for (int v = 0; v < 4; ++v) {
+            const float t = measure([&] { launch_yfwd<T>(v, H, F); });
+            if (t < best) { best = t; tune_.yfwd = v; }
+        }
+    };
+    if (d_.dtype == kF32) run(float{});
+    else run(double{});
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    TG_HIP(hipMemsetAsync(F_[0].ptr, 0, coeff_bytes(), ctx_->stream));
+    TG_HIP(hipStreamSynchronize(ctx_->stream));
+    std::lock_guard<std::mutex> lk(mu);
+    tuning_cache()[key] = tune_;
 }
 
 void FourierSolver::rhs(const void* Y, void* F) {

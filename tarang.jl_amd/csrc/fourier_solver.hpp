@@ -18,6 +18,13 @@ struct Tableau {
 Tableau make_tableau(int stepper);
 bool is_multistep(int stepper);
 
+// kernel variant indices picked by the autotuner
+struct FourierTuning {
+    int ybwd = 0, xpass = 0, yfwd = 0;
+};
+
+template <typename T> struct FourierParams;
+
 struct FourierDesc {
     int ndim;        // 1 or 2
     int n[2];        // grid sizes
@@ -51,12 +58,23 @@ class FourierSolver {
     // run nsteps eagerly with HIP events around every kernel; average ms per launch
     // for [y-backward, x-pass, y-forward, stage update] and launch counts
     void profile(int nsteps, double* avg_ms, long* counts);
+    FourierTuning tuning() const { return tune_; }
+    void set_tuning(const FourierTuning& t) { tune_ = t; invalidate_graph(); }
 
   private:
     template <typename T> void rhs_t(const void* Y, void* F);
     template <typename T> void stage_t(int s);
     template <typename T> void final_t();
     template <typename T> void sbdf_t(bool first);
+    template <typename T, typename Spec> void launch_ybwd(int variant, const FourierParams<T>& p, cplx<T>* G, size_t gstride);
+    template <typename T, typename Spec> void launch_xpass(int variant, const cplx<T>* G, size_t gstride, T scale, cplx<T>* H);
+    template <typename T> void launch_yfwd(int variant, const cplx<T>* H, cplx<T>* F);
+    void autotune();
+    void invalidate_graph() {
+        if (graph_) (void)hipGraphExecDestroy(graph_);
+        graph_ = nullptr;
+    }
+    FourierTuning tune_;
     template <typename F> void timed(int kind, F&& launch);
     struct Mark { int kind; hipEvent_t a, b; };
     bool profiling_ = false;

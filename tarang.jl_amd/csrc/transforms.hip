@@ -23,14 +23,16 @@ __global__ __launch_bounds__(NT) void k_r2c_rows(const T* __restrict__ x, size_t
                                                  const cplx<T>* __restrict__ tw) {
     using V = cplx<T>;
     constexpr int NH = N / 2 + 1;
-    __shared__ V buf[N];
+    __shared__ V buf[line_stride(N)];
+    __shared__ V twl[N];
     const size_t la = 2 * (size_t)blockIdx.x, lb = la + 1;
     const bool hb = lb < nlines;
-    for (int j = threadIdx.x; j < N; j += NT) buf[j] = V{x[la * N + j], hb ? x[lb * N + j] : T(0)};
+    load_twiddles<T, N, NT>(twl, tw);
+    for (int j = threadIdx.x; j < N; j += NT) buf[pidx(j)] = V{x[la * N + j], hb ? x[lb * N + j] : T(0)};
     __syncthreads();
-    fft_lds<T, N, 1, N, NT, false>(buf, tw);
+    fft_lds<T, N, 1, NT, false>(buf, twl);
     for (int k = threadIdx.x; k < NH; k += NT) {
-        const V zk = buf[k], zm = buf[(N - k) % N];
+        const V zk = buf[pidx(k)], zm = buf[pidx((N - k) % N)];
         y[la * NH + k] = V{(zk.x + zm.x) * T(0.5), (zk.y - zm.y) * T(0.5)};
         if (hb) y[lb * NH + k] = V{(zk.y + zm.y) * T(0.5), (zm.x - zk.x) * T(0.5)};
     }
@@ -42,9 +44,11 @@ __global__ __launch_bounds__(NT) void k_c2r_rows(const cplx<T>* __restrict__ y, 
     using V = cplx<T>;
     constexpr int NH = N / 2 + 1;
     constexpr bool EVEN = (N % 2) == 0;
-    __shared__ V buf[N];
+    __shared__ V buf[line_stride(N)];
+    __shared__ V twl[N];
     const size_t la = 2 * (size_t)blockIdx.x, lb = la + 1;
     const bool hb = lb < nlines;
+    load_twiddles<T, N, NT>(twl, tw);
     for (int k = threadIdx.x; k < N; k += NT) {
         const int kk = k < NH ? k : N - k;
         V a = y[la * NH + kk];
@@ -53,13 +57,13 @@ __global__ __launch_bounds__(NT) void k_c2r_rows(const cplx<T>* __restrict__ y, 
             a.y = T(0);
             b.y = T(0);
         }
-        buf[k] = k < NH ? V{a.x - b.y, a.y + b.x} : V{a.x + b.y, b.x - a.y};
+        buf[pidx(k)] = k < NH ? V{a.x - b.y, a.y + b.x} : V{a.x + b.y, b.x - a.y};
     }
     __syncthreads();
-    fft_lds<T, N, 1, N, NT, true>(buf, tw);
+    fft_lds<T, N, 1, NT, true>(buf, twl);
     const T s = T(1) / T(N);
     for (int j = threadIdx.x; j < N; j += NT) {
-        const V z = buf[j];
+        const V z = buf[pidx(j)];
         x[la * N + j] = z.x * s;
         if (hb) x[lb * N + j] = z.y * s;
     }
@@ -70,22 +74,25 @@ template <typename T, int N, int TK, int NT, bool INV>
 __global__ __launch_bounds__(NT) void k_c2c_axis(cplx<T>* __restrict__ a, size_t inner,
                                                  const cplx<T>* __restrict__ tw) {
     using V = cplx<T>;
-    __shared__ V buf[TK * N];
+    constexpr int LS = line_stride(N);
+    __shared__ V buf[TK * LS];
+    __shared__ V twl[N];
     const size_t i0 = (size_t)blockIdx.x * TK;
     const size_t base = (size_t)blockIdx.y * N * inner;
+    load_twiddles<T, N, NT>(twl, tw);
     for (int idx = threadIdx.x; idx < TK * N; idx += NT) {
         const int c = idx % TK, d = idx / TK;
         const size_t i = i0 + c;
-        buf[c * N + d] = i < inner ? a[base + (size_t)d * inner + i] : V{T(0), T(0)};
+        buf[c * LS + pidx(d)] = i < inner ? a[base + (size_t)d * inner + i] : V{T(0), T(0)};
     }
     __syncthreads();
-    fft_lds<T, N, TK, N, NT, INV>(buf, tw);
+    fft_lds<T, N, TK, NT, INV>(buf, twl);
     const T s = INV ? T(1) / T(N) : T(1);
     for (int idx = threadIdx.x; idx < TK * N; idx += NT) {
         const int c = idx % TK, d = idx / TK;
         const size_t i = i0 + c;
         if (i < inner) {
-            const V v = buf[c * N + d];
+            const V v = buf[c * LS + pidx(d)];
             a[base + (size_t)d * inner + i] = V{v.x * s, v.y * s};
         }
     }
@@ -348,7 +355,7 @@ void TransformPlan::product_t(const void* ga, const void* gb, void* out) {
             constexpr int M = padded_len(N);
             constexpr int TK = tile_cols<T, M>();
             const V* tw = static_cast<const V*>(ctx_->twiddle(M, dt));
-            hipLaunchKernelGGL((k_ybwd<T, N, M, TK, NT, ProductSpec>), dim3((nxh + TK - 1) / TK), dim3(NT), 0,
+            hipLaunchKernelGGL((k_ybwd<T, N, M, TK, NT, ProductSpec>), dim3((nxh + TK - 1) / TK, ProductSpec::NF), dim3(NT), 0,
                                ctx_->stream, p, G.as<V>(), gstride, tw);
         }, Sizes2D{});
         TG_HIP(hipGetLastError());

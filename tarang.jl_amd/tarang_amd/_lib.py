@@ -82,6 +82,8 @@ SIGNATURES = {
     "tarang_fivp_set_dt": [_P, _D],
     "tarang_fivp_time": [_P, _PD, ctypes.POINTER(ctypes.c_long)],
     "tarang_fivp_profile": [_P, _I, _PD, ctypes.POINTER(ctypes.c_long)],
+    "tarang_fivp_get_tuning": [_P, _PI],
+    "tarang_fivp_set_tuning": [_P, _PI],
     "tarang_fivp_kernel_bytes": [_P, _PD, _PD, _PD],
 }
 

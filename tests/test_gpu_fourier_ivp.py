@@ -143,3 +143,29 @@ def test_full_size_turbulence_fp32_vs_fp64(gpu, oracle):
     Z0 = np.sum(w[:, None] * np.abs(C) ** 2)
     Z1 = np.sum(w[:, None] * np.abs(states[np.float64]) ** 2)
     assert Z1 < Z0
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_every_tuning_variant_is_bit_identical(gpu, oracle, dtype):
+    """The autotuner only changes tile width / workgroup size: every variant must
+    produce the same bits as the default."""
+    import ctypes
+    from tarang_amd import _lib
+    N = 256
+    d = oracle.domain([0, 0], [N, N])
+    C = oracle.forward_transform(d, turb_ic(N))
+    s = gpu.fivp([N, N], [2 * np.pi, 2 * np.pi], TURB, TS["RK443"], 1e-4, 1e-3, dtype=dtype)
+    chosen = (ctypes.c_int * 3)()
+    _lib.call("tarang_fivp_get_tuning", s.h, chosen)
+    print("autotuned variants", list(chosen), flush=True)
+    base = None
+    for yb in range(4):
+        for xp in range(2):
+            for yf in range(4):
+                _lib.call("tarang_fivp_set_tuning", s.h, (ctypes.c_int * 3)(yb, xp, yf))
+                F = s.rhs(C)
+                if base is None:
+                    base = F
+                else:
+                    assert np.array_equal(F, base), (yb, xp, yf)
+    s.close()
