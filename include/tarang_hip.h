@@ -41,6 +41,7 @@ extern "C" {
 typedef struct tarang_ctx_s* tarang_ctx_t;
 typedef struct tarang_plan_s* tarang_plan_t;
 typedef struct tarang_fivp_s* tarang_fivp_t;
+typedef struct tarang_chan_s* tarang_chan_t;
 
 /* ------------------------------------------------------------------------ */
 /* Context and memory.  Replaces the device hooks of the architecture layer:
@@ -135,6 +136,60 @@ int tarang_fivp_get_tuning(tarang_fivp_t s, int* variants3);
 int tarang_fivp_set_tuning(tarang_fivp_t s, const int* variants3);
 /* algorithmic bytes per launch of the three RHS kernels (y-backward, x-pass, y-forward) */
 int tarang_fivp_kernel_bytes(tarang_fivp_t s, double* ybwd, double* xpass, double* yfwd);
+
+/* ------------------------------------------------------------------------ */
+/* 2-D Rayleigh-Benard channel, RealFourier(x, [0, Lx)) x ChebyshevT(z, [0, Lz]),
+ * Chebyshev-tau method in the first-order form of test/test_subproblem_rk.jl:27-51:
+ *   grad_u = grad(u) + ez*lift(tau_u1),  grad_b = grad(b) + ez*lift(tau_b1)
+ *   trace(grad_u) + tau_p = 0
+ *   dt(b) - kappa*div(grad_b) + lift(tau_b2) = -u@grad(b)
+ *   dt(u) - nu*div(grad_u) + grad(p) - b*ez + lift(tau_u2) = -u@grad(u)
+ *   b(z=0) = b_bottom, u(z=0) = 0, b(z=Lz) = b_top, u(z=Lz) = 0, integ(p) = 0
+ * Replaces, for this equation family, InitialValueSolver's per-mode subproblem
+ * build (src/core/subsystems/subproblem_matrix_build.jl:11-309), the LHS
+ * factorisation (src/tools/batched_matsolvers.jl:57-123) and step!/run! of the
+ * per-mode IMEX RK loop (src/core/timesteppers/step_subproblem_rk.jl:393-739,
+ * step_subproblem_rk_batched.jl:838-1005).  The per-mode systems are solved in
+ * FP64 whatever dtype is; dtype sets the state, the RHS and the stage arithmetic.
+ * timestepper: 0 RK222, 1 RK443.  nx: power of 2 in 8..2048 (FP64: ..1024); nz: 4..512. */
+typedef struct {
+    int nx, nz;
+    double Lx, Lz;
+    double kappa, nu;
+    double dt;
+    int dtype;
+    int timestepper;
+    double b_bottom, b_top;  /* Dirichlet values b(z=0), b(z=Lz) (the test problem: Lz, 0) */
+} tarang_chan_desc;
+
+#define TARANG_CHAN_P 0
+#define TARANG_CHAN_B 1
+#define TARANG_CHAN_UX 2
+#define TARANG_CHAN_UZ 3
+
+/* builds and factors the per-mode systems for dt; TARANG_ERR_SINGULAR names singular modes */
+int tarang_chan_create(tarang_ctx_t ctx, const tarang_chan_desc* desc, tarang_chan_t* out);
+int tarang_chan_destroy(tarang_chan_t s);
+/* coefficient array of p, b, ux or uz: complex (nx/2+1, nz), reference layout, device pointer
+ * (scatter_inputs / gather_inputs!, src/core/subsystems/subproblem_io.jl:1143-1152) */
+int tarang_chan_set_field(tarang_chan_t s, int var, const void* coeff);
+int tarang_chan_get_field(tarang_chan_t s, int var, void* coeff);
+/* tau variable t (0 tau_p, 1 tau_b1, 2 tau_b2, 3 tau_u1.x, 4 tau_u1.z, 5 tau_u2.x, 6 tau_u2.z):
+ * nx/2+1 complex values (tau_p: entry 0 only) */
+int tarang_chan_get_tau(tarang_chan_t s, int t, void* vals);
+/* F_b, F_ux, F_uz of the current state as the stage loop evaluates them (grid round trip of
+ * the state, 3/2-padded products; src/core/timesteppers/state_utils.jl:261-274); NULL skips one */
+int tarang_chan_rhs(tarang_chan_t s, void* Fb, void* Fux, void* Fuz);
+int tarang_chan_step(tarang_chan_t s, int nsteps);
+int tarang_chan_set_graph(tarang_chan_t s, int use_graph);
+/* a new dt refactors the per-mode systems at the next step (factor pairing on dt change) */
+int tarang_chan_set_dt(tarang_chan_t s, double dt);
+int tarang_chan_time(tarang_chan_t s, double* t, long* iteration);
+/* eager steps with HIP events; average ms per launch and counts of
+ * [z-backward, x-pass, z-forward, L x, per-mode solve, stage/copy].  No reference counterpart. */
+int tarang_chan_profile(tarang_chan_t s, int nsteps, double* avg_ms6, long* counts6);
+/* algorithmic bytes per launch of the same six kernel kinds */
+int tarang_chan_kernel_bytes(tarang_chan_t s, double* bytes6);
 
 #ifdef __cplusplus
 }

@@ -84,6 +84,25 @@ double orc_fivp_time(const orc_fivp *p);
    E = 1/2 <u^2> for scalar problems (Z unused) */
 void orc_fivp_diagnostics(orc_fivp *p, double *E, double *Z);
 
+/* ---- 2-D Rayleigh-Benard channel (RealFourier x ChebyshevT) with the Chebyshev-tau
+ *      method: per-mode dense systems, RK222/RK443 (step_subproblem_rk.jl:393-739) ---- */
+enum { ORC_RBC_P = 0, ORC_RBC_B = 1, ORC_RBC_UX = 2, ORC_RBC_UZ = 3 };
+typedef struct orc_rbc orc_rbc;
+orc_rbc *orc_rbc_create(int nx, int nz, double Lx, double Lz, double kappa, double nu, double dt, int timestepper);
+void orc_rbc_destroy(orc_rbc *s);
+int orc_rbc_size(const orc_rbc *s, int mode);
+/* Dirichlet values of b (default b(0) = Lz, b(Lz) = 0) */
+void orc_rbc_set_bc(orc_rbc *s, double b_bottom, double b_top);
+/* coefficient arrays (nx/2+1, nz) complex, reference layout */
+void orc_rbc_set_field(orc_rbc *s, int var, const double *coeff);
+void orc_rbc_get_field(const orc_rbc *s, int var, double *coeff);
+/* tau values per mode: 0 tau_p, 1 tau_b1, 2 tau_b2, 3 tau_u1.x, 4 tau_u1.z, 5 tau_u2.x, 6 tau_u2.z */
+void orc_rbc_get_tau(const orc_rbc *s, int t, double *vals);
+int orc_rbc_step(orc_rbc *s, int nsteps);
+double orc_rbc_time(const orc_rbc *s);
+void orc_rbc_rhs_fields(orc_rbc *s, double *Fb, double *Fux, double *Fuz);
+void orc_rbc_matrices(const orc_rbc *s, int mode, double *M, double *L);
+
 #ifdef __cplusplus
 }
 #endif

@@ -68,6 +68,21 @@ def lib():
         L.orc_fivp_time.argtypes = [ctypes.c_void_p]
         L.orc_fivp_time.restype = ctypes.c_double
         L.orc_fivp_diagnostics.argtypes = [ctypes.c_void_p, dp, dp]
+        vp = ctypes.c_void_p
+        L.orc_rbc_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+        L.orc_rbc_create.restype = vp
+        L.orc_rbc_destroy.argtypes = [vp]
+        L.orc_rbc_size.argtypes = [vp, ctypes.c_int]
+        L.orc_rbc_set_bc.argtypes = [vp, ctypes.c_double, ctypes.c_double]
+        L.orc_rbc_set_field.argtypes = [vp, ctypes.c_int, dp]
+        L.orc_rbc_get_field.argtypes = [vp, ctypes.c_int, dp]
+        L.orc_rbc_get_tau.argtypes = [vp, ctypes.c_int, dp]
+        L.orc_rbc_step.argtypes = [vp, ctypes.c_int]
+        L.orc_rbc_time.argtypes = [vp]
+        L.orc_rbc_time.restype = ctypes.c_double
+        L.orc_rbc_rhs_fields.argtypes = [vp, dp, dp, dp]
+        L.orc_rbc_matrices.argtypes = [vp, ctypes.c_int, dp, dp]
         _lib = L
     return _lib
 
@@ -204,3 +219,68 @@ class FourierIVP:
         Z = ctypes.c_double()
         lib().orc_fivp_diagnostics(self.h, ctypes.byref(E), ctypes.byref(Z))
         return E.value, Z.value
+
+
+RBC_P, RBC_B, RBC_UX, RBC_UZ = 0, 1, 2, 3
+
+
+class RBC2D:
+    """2-D Rayleigh-Benard channel, Chebyshev-tau per-mode solves (oracle)."""
+
+    def __init__(self, nx, nz, Lx, Lz, kappa, nu, dt, timestepper="RK222"):
+        self.nx, self.nz = nx, nz
+        self.shape = (nx // 2 + 1, nz)
+        self.h = lib().orc_rbc_create(nx, nz, Lx, Lz, kappa, nu, dt, TS[timestepper])
+        if not self.h:
+            raise ValueError("per-mode system is not square after filtering")
+        self.d = domain([REALFOURIER, CHEBYSHEVT], [nx, nz], bounds=[(0.0, Lx), (0.0, Lz)])
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None) and _lib is not None:
+                _lib.orc_rbc_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def n(self, mode=1):
+        return lib().orc_rbc_size(self.h, int(mode))
+
+    def set_bc(self, b_bottom, b_top):
+        """Dirichlet values of b at z = 0 and z = Lz."""
+        lib().orc_rbc_set_bc(self.h, float(b_bottom), float(b_top))
+
+    def set(self, var, coeff):
+        c = _fortran(coeff, np.complex128)
+        lib().orc_rbc_set_field(self.h, var, _dp(c))
+
+    def get(self, var):
+        out = np.zeros(self.shape, np.complex128, order="F")
+        lib().orc_rbc_get_field(self.h, var, _dp(out))
+        return out
+
+    def tau(self, t):
+        out = np.zeros(self.shape[0], np.complex128)
+        lib().orc_rbc_get_tau(self.h, t, _dp(out))
+        return out
+
+    def step(self, n=1):
+        rc = lib().orc_rbc_step(self.h, int(n))
+        if rc != 0:
+            raise RuntimeError("singular per-mode system")
+
+    @property
+    def time(self):
+        return lib().orc_rbc_time(self.h)
+
+    def rhs(self):
+        outs = [np.zeros(self.shape, np.complex128, order="F") for _ in range(3)]
+        lib().orc_rbc_rhs_fields(self.h, *[_dp(o) for o in outs])
+        return outs
+
+    def matrices(self, mode):
+        n = self.n(mode)
+        M = np.zeros((n, n), np.complex128, order="F")
+        L = np.zeros((n, n), np.complex128, order="F")
+        lib().orc_rbc_matrices(self.h, mode, _dp(M), _dp(L))
+        return M, L

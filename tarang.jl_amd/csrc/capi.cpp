@@ -4,6 +4,7 @@
 #include <new>
 
 #include "../../include/tarang_hip.h"
+#include "channel_solver.hpp"
 #include "fourier_solver.hpp"
 #include "runtime.hpp"
 #include "transforms.hpp"
@@ -16,6 +17,9 @@ struct tarang_plan_s {
 };
 struct tarang_fivp_s {
     std::unique_ptr<tarang::FourierSolver> solver;
+};
+struct tarang_chan_s {
+    std::unique_ptr<tarang::ChannelSolver> solver;
 };
 
 namespace {
@@ -327,6 +331,112 @@ int tarang_fivp_kernel_bytes(tarang_fivp_t s, double* ybwd, double* xpass, doubl
     return guard([&] {
         NEED(s, "solver");
         s->solver->kernel_bytes(ybwd, xpass, yfwd);
+    });
+}
+
+// ---- 2-D Rayleigh-Benard channel ----
+int tarang_chan_create(tarang_ctx_t ctx, const tarang_chan_desc* desc, tarang_chan_t* out) {
+    return guard([&] {
+        NEED(ctx, "ctx");
+        NEED(desc, "desc");
+        NEED(out, "out");
+        *out = nullptr;
+        tarang::ChannelDesc d;
+        d.nx = desc->nx;
+        d.nz = desc->nz;
+        d.Lx = desc->Lx;
+        d.Lz = desc->Lz;
+        d.kappa = desc->kappa;
+        d.nu = desc->nu;
+        d.dt = desc->dt;
+        d.dtype = desc->dtype;
+        d.b_bottom = desc->b_bottom;
+        d.b_top = desc->b_top;
+        d.stepper = desc->timestepper == 1 ? tarang::kRK443 : (desc->timestepper == 0 ? tarang::kRK222 : -1);
+        auto h = std::make_unique<tarang_chan_s>();
+        h->solver = std::make_unique<tarang::ChannelSolver>(&ctx->ctx, d);
+        *out = h.release();
+    });
+}
+
+int tarang_chan_destroy(tarang_chan_t s) {
+    return guard([&] { delete s; });
+}
+
+int tarang_chan_set_field(tarang_chan_t s, int var, const void* coeff) {
+    return guard([&] {
+        NEED(s, "solver");
+        NEED(coeff, "coeff");
+        s->solver->set_field(var, coeff);
+    });
+}
+
+int tarang_chan_get_field(tarang_chan_t s, int var, void* coeff) {
+    return guard([&] {
+        NEED(s, "solver");
+        NEED(coeff, "coeff");
+        s->solver->get_field(var, coeff);
+    });
+}
+
+int tarang_chan_get_tau(tarang_chan_t s, int t, void* vals) {
+    return guard([&] {
+        NEED(s, "solver");
+        NEED(vals, "vals");
+        s->solver->get_tau(t, vals);
+    });
+}
+
+int tarang_chan_rhs(tarang_chan_t s, void* Fb, void* Fux, void* Fuz) {
+    return guard([&] {
+        NEED(s, "solver");
+        s->solver->rhs(Fb, Fux, Fuz);
+    });
+}
+
+int tarang_chan_step(tarang_chan_t s, int nsteps) {
+    return guard([&] {
+        NEED(s, "solver");
+        s->solver->step(nsteps);
+    });
+}
+
+int tarang_chan_set_graph(tarang_chan_t s, int use_graph) {
+    return guard([&] {
+        NEED(s, "solver");
+        s->solver->enable_graph(use_graph != 0);
+    });
+}
+
+int tarang_chan_set_dt(tarang_chan_t s, double dt) {
+    return guard([&] {
+        NEED(s, "solver");
+        s->solver->set_dt(dt);
+    });
+}
+
+int tarang_chan_time(tarang_chan_t s, double* t, long* iteration) {
+    return guard([&] {
+        NEED(s, "solver");
+        if (t) *t = s->solver->time();
+        if (iteration) *iteration = s->solver->iteration();
+    });
+}
+
+int tarang_chan_profile(tarang_chan_t s, int nsteps, double* avg_ms6, long* counts6) {
+    return guard([&] {
+        NEED(s, "solver");
+        NEED(avg_ms6, "avg_ms6");
+        TG_REQUIRE(nsteps > 0, tarang::kErrInvalid, "nsteps must be > 0");
+        s->solver->profile(nsteps, avg_ms6, counts6);
+    });
+}
+
+int tarang_chan_kernel_bytes(tarang_chan_t s, double* bytes6) {
+    return guard([&] {
+        NEED(s, "solver");
+        NEED(bytes6, "bytes6");
+        s->solver->kernel_bytes(bytes6);
     });
 }
 

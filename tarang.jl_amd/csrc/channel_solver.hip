@@ -1,0 +1,481 @@
+// channel_solver.hip -- 2-D Rayleigh-Benard channel: dealiased RHS through the
+// grid (z matrices, padded x FFT), structured L x, per-mode FP64 tau solves,
+// IMEX RK stage loop replayed as one hipGraph per step.
+//
+// Step (src/core/timesteppers/step_subproblem_rk.jl:393-739, mode_batch.jl:575-668):
+//   X0 = (b, ux, uz);  stage 0 (a_00 = 0): x_0 = M^+ M X0, i.e. p and the taus are 0
+//   stage i: rhs = M X0 + dt sum_{j<i} (AE_ij F_j - AI_ij L P(x_j)); BC rows dt a_ii ALG
+//            x_i = (M + dt a_ii L)^-1 rhs;  F_i = F(P(x_i))
+//   final:   (b, ux, uz) = X0 + dt sum_j (bE_j F_j - bI_j L P(x_j));  p = taus = 0
+// P is the grid round trip of the state (real part at kx = 0 and kx = Nx/2).
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "channel_kernels.hpp"
+#include "channel_solver.hpp"
+#include "dispatch.hpp"
+
+namespace tarang {
+
+namespace {
+constexpr int kTK = 4;
+
+// Chebyshev matrices on the Gauss-Lobatto grid z_j = (1 - cos(pi j/(N-1))) Lz/2:
+//   Bm[z][k] = T_k at point z = (-1)^k cos(pi k z/(N-1))             (backward)
+//   Fm[k][z] = (-1)^k w_k/(N-1) * (DCT-I kernel)                       (forward, its inverse)
+//   BD = Bm * Dz                                                        (backward of the z derivative)
+void chebyshev_tables(int N, double Lz, std::vector<double>& BT, std::vector<double>& BDT, std::vector<double>& FT) {
+    const long period = 2L * (N - 1);
+    auto cospi = [&](long num) {
+        const long r = ((num % period) + period) % period;
+        return (double)cosl(3.14159265358979323846264338327950288L * (long double)r / (long double)(N - 1));
+    };
+    std::vector<double> Bm((size_t)N * N), Fm((size_t)N * N);
+    for (int z = 0; z < N; ++z)
+        for (int k = 0; k < N; ++k) Bm[(size_t)z * N + k] = ((k & 1) ? -1.0 : 1.0) * cospi((long)k * z);
+    for (int k = 0; k < N; ++k)
+        for (int z = 0; z < N; ++z) {
+            const double sgn = (k & 1) ? -1.0 : 1.0;
+            const double wk = (k == 0 || k == N - 1) ? 0.5 : 1.0;
+            double e;
+            if (z == 0) e = 1.0;
+            else if (z == N - 1) e = sgn;
+            else e = 2.0 * cospi((long)z * k);
+            Fm[(size_t)k * N + z] = sgn * wk / (N - 1) * e;
+        }
+    const double s = 2.0 / Lz;
+    BT.assign((size_t)N * N, 0.0);
+    BDT.assign((size_t)N * N, 0.0);
+    FT.assign((size_t)N * N, 0.0);
+    for (int z = 0; z < N; ++z)
+        for (int k = 0; k < N; ++k) {
+            BT[(size_t)k * N + z] = Bm[(size_t)z * N + k];
+            FT[(size_t)z * N + k] = Fm[(size_t)k * N + z];
+            long double acc = 0;  // (Bm Dz)[z][k] = sum_{q<k} Bm[z][q] Dz[q][k]
+            for (int q = 0; q < k; ++q) acc += (long double)Bm[(size_t)z * N + q] * cheb_dz(q, k, s);
+            BDT[(size_t)k * N + z] = (double)acc;
+        }
+}
+
+bool channel_nx_ok(int nx, bool f64) {
+    bool ok = false;
+    dispatch_size(nx, [&](auto) { ok = true; }, Sizes2D{});
+    return ok && !(f64 && nx > 1024);
+}
+}  // namespace
+
+ChannelSolver::ChannelSolver(Context* ctx, const ChannelDesc& d) : ctx_(ctx), d_(d) {
+    TG_REQUIRE(d.dtype == kF32 || d.dtype == kF64, kErrInvalid, "channel solver: dtype must be f32 or f64");
+    TG_REQUIRE(d.stepper == kRK222 || d.stepper == kRK443, kErrInvalid,
+               "channel solver: timestepper must be RK222 or RK443 (explicit first stage)");
+    TG_REQUIRE(channel_nx_ok(d.nx, d.dtype == kF64), kErrInvalid,
+               "channel solver: RealFourier size " + std::to_string(d.nx) +
+                   " unsupported (powers of 2, 8..2048; FP64 up to 1024)");
+    TG_REQUIRE(d.nz >= 4 && d.nz <= 512, kErrInvalid,
+               "channel solver: ChebyshevT size " + std::to_string(d.nz) + " unsupported (4..512)");
+    TG_REQUIRE(d.Lx > 0 && d.Lz > 0 && d.dt > 0, kErrInvalid, "channel solver: Lx, Lz and dt must be positive");
+    nxh_ = d.nx / 2 + 1;
+    mx_ = padded_len(d.nx);
+    nb_ = d.nz + 2;
+    nu_max_ = 3 * d.nz + 5;
+    tab_ = make_tableau(d.stepper);
+    gamma_ = tab_.AI[1][1];
+    ctx_->activate();
+    const size_t fb = field_bytes(), vb = 2 * dtype_size((DType)d.dtype);
+    hipStream_t s = ctx_->stream;
+    auto zeroed = [&](DevBuf& b, size_t bytes) {
+        b = DevBuf(ctx_, bytes);
+        TG_HIP(hipMemsetAsync(b.ptr, 0, bytes, s));
+    };
+    zeroed(P_, fb);
+    zeroed(B_, fb);
+    zeroed(UX_, fb);
+    zeroed(UZ_, fb);
+    zeroed(TAU_, (size_t)nxh_ * 8 * vb);
+    for (int i = 0; i < 3; ++i) zeroed(X0_[i], fb);
+    for (int st = 0; st < tab_.stages; ++st) {
+        for (int r = 0; r < 3; ++r) zeroed(F_[st][r], fb);
+        for (int r = 0; r < 4; ++r) zeroed(LX_[st][r], fb);
+    }
+    for (int r = 0; r < 4; ++r) zeroed(R_[r], fb);
+    zeroed(G_, 8 * fb);
+    zeroed(H_, 3 * fb);
+    std::vector<double> BT, BDT, FT;
+    chebyshev_tables(d.nz, d.Lz, BT, BDT, FT);
+    const size_t nn = (size_t)d.nz * d.nz;
+    auto upload = [&](DevBuf& b, const std::vector<double>& h) {
+        if (d.dtype == kF64) {
+            b = DevBuf(ctx_, nn * sizeof(double));
+            TG_HIP(hipMemcpyAsync(b.ptr, h.data(), nn * sizeof(double), hipMemcpyHostToDevice, s));
+        } else {
+            std::vector<float> f(h.begin(), h.end());
+            b = DevBuf(ctx_, nn * sizeof(float));
+            TG_HIP(hipMemcpyAsync(b.ptr, f.data(), nn * sizeof(float), hipMemcpyHostToDevice, s));
+            TG_HIP(hipStreamSynchronize(s));  // f is released at scope exit
+        }
+    };
+    upload(BT_, BT);
+    upload(BDT_, BDT);
+    upload(FT_, FT);
+    Ab_ = DevBuf(ctx_, (size_t)nxh_ * nb_ * nb_ * sizeof(double));
+    Aup_ = DevBuf(ctx_, (size_t)nxh_ * nu_max_ * nu_max_ * sizeof(double));
+    pivb_ = DevBuf(ctx_, (size_t)nxh_ * nb_ * sizeof(int));
+    pivu_ = DevBuf(ctx_, (size_t)nxh_ * nu_max_ * sizeof(int));
+    info_ = DevBuf(ctx_, (size_t)nxh_ * 2 * sizeof(int));
+    ctx_->twiddle(mx_, (DType)d.dtype);
+    TG_HIP(hipStreamSynchronize(s));
+    factor();
+}
+
+ChannelSolver::~ChannelSolver() {
+    if (graph_) (void)hipGraphExecDestroy(graph_);
+}
+
+void* ChannelSolver::field_ptr(int var) const {
+    switch (var) {
+        case kChP: return P_.ptr;
+        case kChB: return B_.ptr;
+        case kChUX: return UX_.ptr;
+        case kChUZ: return UZ_.ptr;
+        default: break;
+    }
+    TG_REQUIRE(false, kErrInvalid, "channel solver: variable must be 0 (p), 1 (b), 2 (ux) or 3 (uz)");
+    return nullptr;
+}
+
+// ---- layout changes at the ABI ----
+template <typename T>
+void ChannelSolver::to_modes_t(const void* in, void* out) {
+    const int N = d_.nz;  // in: (nxh, N) kx fastest = N rows of nxh
+    dim3 grid((unsigned)((nxh_ + 31) / 32), (unsigned)((N + 31) / 32));
+    hipLaunchKernelGGL((k_ch_transpose<T>), grid, dim3(32, 8), 0, ctx_->stream, static_cast<const cplx<T>*>(in),
+                       static_cast<cplx<T>*>(out), N, nxh_);
+    TG_HIP(hipGetLastError());
+}
+
+template <typename T>
+void ChannelSolver::from_modes_t(const void* in, void* out) {
+    const int N = d_.nz;  // in: nxh rows of N
+    dim3 grid((unsigned)((N + 31) / 32), (unsigned)((nxh_ + 31) / 32));
+    hipLaunchKernelGGL((k_ch_transpose<T>), grid, dim3(32, 8), 0, ctx_->stream, static_cast<const cplx<T>*>(in),
+                       static_cast<cplx<T>*>(out), nxh_, N);
+    TG_HIP(hipGetLastError());
+}
+
+void ChannelSolver::set_field(int var, const void* dev) {
+    ctx_->activate();
+    void* dst = field_ptr(var);
+    if (d_.dtype == kF32) to_modes_t<float>(dev, dst); else to_modes_t<double>(dev, dst);
+    t_ = 0;
+    iter_ = 0;
+}
+
+void ChannelSolver::get_field(int var, void* dev) {
+    ctx_->activate();
+    const void* src = field_ptr(var);
+    if (d_.dtype == kF32) from_modes_t<float>(src, dev); else from_modes_t<double>(src, dev);
+}
+
+void ChannelSolver::get_tau(int t, void* dev) {
+    TG_REQUIRE(t >= 0 && t < 7, kErrInvalid, "channel solver: tau index must be 0..6");
+    ctx_->activate();
+    const size_t vb = 2 * dtype_size((DType)d_.dtype);
+    TG_HIP(hipMemcpy2DAsync(dev, vb, static_cast<const char*>(TAU_.ptr) + t * vb, 8 * vb, vb, nxh_,
+                            hipMemcpyDeviceToDevice, ctx_->stream));
+}
+
+void ChannelSolver::set_dt(double dt) {
+    TG_REQUIRE(dt > 0, kErrInvalid, "channel solver: dt must be positive");
+    d_.dt = dt;
+}
+
+// ---- implicit systems ----
+void ChannelSolver::factor() {
+    const double c = d_.dt * gamma_;
+    if (c == factored_c_) return;
+    const int N = d_.nz;
+    hipStream_t s = ctx_->stream;
+    ChSys sys{N, nxh_, nu_max_, 2.0 * M_PI / d_.Lx, 2.0 / d_.Lz, d_.kappa, d_.nu, c, d_.Lx, d_.Lz};
+    hipLaunchKernelGGL(k_ch_assemble, dim3(nxh_), dim3(256), 0, s, sys, Ab_.as<double>(), Aup_.as<double>());
+    TG_HIP(hipGetLastError());
+    int* info = info_.as<int>();
+    constexpr int NT = 256;
+    hipLaunchKernelGGL((k_lu_factor<NT>), dim3(nxh_), dim3(NT), (size_t)2 * nb_ * sizeof(double), s,
+                       Ab_.as<double>(), pivb_.as<int>(), info, nb_, 0, (size_t)nb_ * nb_, (size_t)nb_);
+    TG_HIP(hipGetLastError());
+    hipLaunchKernelGGL((k_lu_factor<NT>), dim3(nxh_), dim3(NT), (size_t)2 * nu_max_ * sizeof(double), s,
+                       Aup_.as<double>(), pivu_.as<int>(), info + nxh_, 3 * N + 4, 3 * N + 5,
+                       (size_t)nu_max_ * nu_max_, (size_t)nu_max_);
+    TG_HIP(hipGetLastError());
+    std::vector<int> h((size_t)2 * nxh_);
+    TG_HIP(hipMemcpyAsync(h.data(), info, h.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+    TG_HIP(hipStreamSynchronize(s));
+    std::string bad;
+    for (int m = 0; m < nxh_; ++m) {
+        if (h[m]) bad += " b-block(kx mode " + std::to_string(m) + ")";
+        if (h[nxh_ + m]) bad += " u-p-block(kx mode " + std::to_string(m) + ")";
+        if (bad.size() > 200) break;
+    }
+    TG_REQUIRE(bad.empty(), kErrSingular, "channel solver: singular per-mode system:" + bad);
+    factored_c_ = c;
+    invalidate_graph();
+}
+
+// ---- kernels of one step ----
+template <typename T>
+void ChannelSolver::rhs_t(const void* b, const void* ux, const void* uz, void* F0, void* F1, void* F2) {
+    using V = cplx<T>;
+    const int N = d_.nz;
+    hipStream_t s = ctx_->stream;
+    const DType dt = std::is_same<T, float>::value ? kF32 : kF64;
+    V* G = G_.as<V>();
+    V* H = H_.as<V>();
+    const size_t lds = (size_t)kTK * N * sizeof(V);
+    const unsigned ntile = (unsigned)((nxh_ + kTK - 1) / kTK);
+    auto zb = [&](auto NTC) {
+        constexpr int NT = decltype(NTC)::value;
+        timed(0, [&] {
+            hipLaunchKernelGGL((k_ch_zbwd<T, kTK, NT>), dim3(ntile, 6), dim3(NT), lds, s, static_cast<const V*>(b),
+                               static_cast<const V*>(ux), static_cast<const V*>(uz), nxh_, d_.nx, N,
+                               BT_.as<const T>(), BDT_.as<const T>(), T(2.0 * M_PI / d_.Lx), G);
+        });
+    };
+    if (N <= 64) zb(std::integral_constant<int, 64>{}); else zb(std::integral_constant<int, 256>{});
+    TG_HIP(hipGetLastError());
+    const T scale = T(1.0 / ((double)d_.nx * (double)mx_));
+    dispatch_size(d_.nx, [&](auto NC) {
+        constexpr int NX = decltype(NC)::value;
+        constexpr int MX = padded_len(NX);
+        if constexpr ((size_t)4 * line_stride(MX) * sizeof(V) <= 128 * 1024) {
+            const V* tw = static_cast<const V*>(ctx_->twiddle(MX, dt));
+            constexpr int NT = NX >= 512 ? 512 : 256;
+            timed(1, [&] {
+                hipLaunchKernelGGL((k_ch_xpass<T, NX, MX, NT>), dim3(N), dim3(NT), 0, s, G, N, scale, H, tw);
+            });
+        } else {
+            TG_REQUIRE(false, kErrInvalid, "channel solver: x size too large for FP64");
+        }
+    }, Sizes2D{});
+    TG_HIP(hipGetLastError());
+    auto zf = [&](auto NTC) {
+        constexpr int NT = decltype(NTC)::value;
+        timed(2, [&] {
+            hipLaunchKernelGGL((k_ch_zfwd<T, kTK, NT>), dim3(ntile, 3), dim3(NT), lds, s, H, nxh_, d_.nx, N,
+                               FT_.as<const T>(), static_cast<V*>(F0), static_cast<V*>(F1), static_cast<V*>(F2));
+        });
+    };
+    if (N <= 64) zf(std::integral_constant<int, 64>{}); else zf(std::integral_constant<int, 256>{});
+    TG_HIP(hipGetLastError());
+}
+
+template <typename T>
+void ChannelSolver::lx_t(int st) {
+    using V = cplx<T>;
+    const int N = d_.nz;
+    constexpr int NT = 256;
+    timed(3, [&] {
+        hipLaunchKernelGGL((k_ch_lx<T, NT>), dim3(nxh_), dim3(NT), (size_t)8 * N * sizeof(double2), ctx_->stream,
+                           P_.as<const V>(), B_.as<const V>(), UX_.as<const V>(), UZ_.as<const V>(),
+                           TAU_.as<const V>(), nxh_, d_.nx, N, 2.0 * M_PI / d_.Lx, 2.0 / d_.Lz, d_.kappa, d_.nu,
+                           LX_[st][0].as<V>(), LX_[st][1].as<V>(), LX_[st][2].as<V>(), LX_[st][3].as<V>());
+    });
+    TG_HIP(hipGetLastError());
+}
+
+template <typename T>
+void ChannelSolver::stage_rhs_t(int st, bool final_update) {
+    using V = cplx<T>;
+    ChStageArgs<T> a;
+    std::memset(&a, 0, sizeof(a));
+    for (int r = 0; r < 3; ++r) a.X0[r] = X0_[r].as<const V>();
+    const int nterms = final_update ? tab_.stages : st;
+    for (int j = 0; j < nterms; ++j) {
+        for (int r = 0; r < 3; ++r) a.F[j][r] = F_[j][r].as<const V>();
+        for (int r = 0; r < 4; ++r) a.LX[j][r] = LX_[j][r].as<const V>();
+        a.ae[j] = T(final_update ? tab_.bE[j] : tab_.AE[st][j]);
+        a.ai[j] = T(final_update ? tab_.bI[j] : tab_.AI[st][j]);
+    }
+    a.nterms = nterms;
+    a.dt = T(d_.dt);
+    a.final_update = final_update ? 1 : 0;
+    if (final_update) {
+        a.out[0] = B_.as<V>();
+        a.out[1] = UX_.as<V>();
+        a.out[2] = UZ_.as<V>();
+    } else {
+        for (int r = 0; r < 4; ++r) a.out[r] = R_[r].as<V>();
+    }
+    const size_t n = field_elems();
+    const unsigned nblk = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+    timed(5, [&] { hipLaunchKernelGGL((k_ch_stage<T>), dim3(nblk), dim3(256), 0, ctx_->stream, a, n); });
+    TG_HIP(hipGetLastError());
+}
+
+template <typename T>
+void ChannelSolver::solve_t() {
+    using V = cplx<T>;
+    const int N = d_.nz;
+    ChSolveArgs sa{Ab_.as<const double>(), pivb_.as<const int>(), Aup_.as<const double>(), pivu_.as<const int>(),
+                   N, nxh_, nu_max_, d_.dt * gamma_, (double)d_.nx * d_.b_bottom, (double)d_.nx * d_.b_top};
+    constexpr int NT = 256;
+    const size_t lds = ((size_t)2 * (nu_max_ + 1) + 2 * (size_t)N) * sizeof(double);
+    timed(4, [&] {
+        hipLaunchKernelGGL((k_ch_solve<T, NT>), dim3(nxh_), dim3(NT), lds, ctx_->stream, sa, R_[0].as<const V>(),
+                           R_[1].as<const V>(), R_[2].as<const V>(), R_[3].as<const V>(), P_.as<V>(), B_.as<V>(),
+                           UX_.as<V>(), UZ_.as<V>(), TAU_.as<V>());
+    });
+    TG_HIP(hipGetLastError());
+}
+
+template <typename T>
+void ChannelSolver::zero_t(void* p, size_t n) {
+    const unsigned nblk = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+    timed(5, [&] {
+        hipLaunchKernelGGL((k_ch_zero<T>), dim3(nblk), dim3(256), 0, ctx_->stream, static_cast<cplx<T>*>(p), n);
+    });
+    TG_HIP(hipGetLastError());
+}
+
+void ChannelSolver::launch_step_body() {
+    const bool f32 = d_.dtype == kF32;
+    const size_t n = field_elems();
+    auto run = [&](auto tag) {
+        using T = decltype(tag);
+        using V = cplx<T>;
+        const unsigned nblk = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+        timed(5, [&] {
+            hipLaunchKernelGGL((k_ch_copy<T>), dim3(nblk), dim3(256), 0, ctx_->stream, B_.as<const V>(),
+                               UX_.as<const V>(), UZ_.as<const V>(), X0_[0].as<V>(), X0_[1].as<V>(), X0_[2].as<V>(),
+                               n);
+        });
+        TG_HIP(hipGetLastError());
+        // stage 0: x_0 = M^+ M X0 keeps b, u and clears p and the taus
+        zero_t<T>(P_.ptr, n);
+        zero_t<T>(TAU_.ptr, (size_t)nxh_ * 8);
+        rhs_t<T>(B_.ptr, UX_.ptr, UZ_.ptr, F_[0][0].ptr, F_[0][1].ptr, F_[0][2].ptr);
+        lx_t<T>(0);
+        for (int st = 1; st < tab_.stages; ++st) {
+            stage_rhs_t<T>(st, false);
+            solve_t<T>();
+            rhs_t<T>(B_.ptr, UX_.ptr, UZ_.ptr, F_[st][0].ptr, F_[st][1].ptr, F_[st][2].ptr);
+            lx_t<T>(st);
+        }
+        stage_rhs_t<T>(0, true);
+        zero_t<T>(P_.ptr, n);
+        zero_t<T>(TAU_.ptr, (size_t)nxh_ * 8);
+    };
+    if (f32) run(float{}); else run(double{});
+}
+
+void ChannelSolver::step_once(bool allow_graph) {
+    factor();  // no-op unless dt changed
+    if (!(allow_graph && graph_on_)) {
+        launch_step_body();
+    } else {
+        if (graph_ && graph_dt_ != d_.dt) invalidate_graph();
+        if (!graph_) {
+            hipGraph_t g = nullptr;
+            TG_HIP(hipStreamBeginCapture(ctx_->stream, hipStreamCaptureModeThreadLocal));
+            try {
+                launch_step_body();
+            } catch (...) {
+                (void)hipStreamEndCapture(ctx_->stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            TG_HIP(hipStreamEndCapture(ctx_->stream, &g));
+            TG_HIP(hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0));
+            TG_HIP(hipGraphDestroy(g));
+            graph_dt_ = d_.dt;
+        }
+        TG_HIP(hipGraphLaunch(graph_, ctx_->stream));
+    }
+    t_ += d_.dt;
+    ++iter_;
+}
+
+void ChannelSolver::step(int nsteps) {
+    TG_REQUIRE(nsteps >= 0, kErrInvalid, "channel solver: nsteps must be >= 0");
+    ctx_->activate();
+    for (int i = 0; i < nsteps; ++i) step_once(true);
+}
+
+void ChannelSolver::rhs(void* Fb, void* Fux, void* Fuz) {
+    ctx_->activate();
+    const bool f32 = d_.dtype == kF32;
+    void* outs[3] = {Fb, Fux, Fuz};
+    // mode-major products into the last stage's F slots, then back to the reference layout
+    const int st = tab_.stages - 1;
+    if (f32) rhs_t<float>(B_.ptr, UX_.ptr, UZ_.ptr, F_[st][0].ptr, F_[st][1].ptr, F_[st][2].ptr);
+    else rhs_t<double>(B_.ptr, UX_.ptr, UZ_.ptr, F_[st][0].ptr, F_[st][1].ptr, F_[st][2].ptr);
+    for (int r = 0; r < 3; ++r) {
+        if (!outs[r]) continue;
+        if (f32) from_modes_t<float>(F_[st][r].ptr, outs[r]); else from_modes_t<double>(F_[st][r].ptr, outs[r]);
+    }
+}
+
+template <typename F>
+void ChannelSolver::timed(int kind, F&& launch) {
+    if (!profiling_) {
+        launch();
+        return;
+    }
+    Mark m{kind, nullptr, nullptr};
+    TG_HIP(hipEventCreate(&m.a));
+    TG_HIP(hipEventCreate(&m.b));
+    TG_HIP(hipEventRecord(m.a, ctx_->stream));
+    launch();
+    TG_HIP(hipEventRecord(m.b, ctx_->stream));
+    marks_.push_back(m);
+}
+
+void ChannelSolver::profile(int nsteps, double* avg_ms, long* counts) {
+    ctx_->activate();
+    factor();
+    marks_.clear();
+    profiling_ = true;
+    try {
+        for (int i = 0; i < nsteps; ++i) step_once(false);
+    } catch (...) {
+        profiling_ = false;
+        throw;
+    }
+    profiling_ = false;
+    TG_HIP(hipStreamSynchronize(ctx_->stream));
+    double sum[kKinds] = {};
+    long cnt[kKinds] = {};
+    for (auto& m : marks_) {
+        float ms = 0.f;
+        TG_HIP(hipEventElapsedTime(&ms, m.a, m.b));
+        sum[m.kind] += ms;
+        cnt[m.kind] += 1;
+        (void)hipEventDestroy(m.a);
+        (void)hipEventDestroy(m.b);
+    }
+    marks_.clear();
+    for (int k = 0; k < kKinds; ++k) {
+        if (avg_ms) avg_ms[k] = cnt[k] ? sum[k] / cnt[k] : 0.0;
+        if (counts) counts[k] = cnt[k];
+    }
+}
+
+void ChannelSolver::kernel_bytes(double* out) const {
+    const double vb = 2.0 * dtype_size((DType)d_.dtype), rb = dtype_size((DType)d_.dtype);
+    const double nc = (double)nxh_ * d_.nz, N = d_.nz;
+    out[0] = (6 * nc + 8 * nc) * vb + 2 * N * N * rb;  // read b, ux, uz per job, write 8 planes, matrices
+    out[1] = (8 * nc + 3 * nc) * vb;                    // read 8 planes, write 3
+    out[2] = (3 * nc + 3 * nc) * vb + N * N * rb;       // read 3 planes, write 3 F
+    out[3] = (4 * nc + 4 * nc) * vb;                    // read p, b, ux, uz; write 4 rows
+    // solve: read the two factor blocks per mode (FP64) and 4 rhs rows, write 4 fields
+    double fac = 0;
+    for (int m = 0; m < nxh_; ++m) {
+        const double nu = 3.0 * d_.nz + 4 + (m == 0 ? 1 : 0);
+        fac += ((double)nb_ * nb_ + nu * nu) * 8.0;
+    }
+    out[4] = fac + 8 * nc * vb;
+    out[5] = 0;
+}
+
+}  // namespace tarang

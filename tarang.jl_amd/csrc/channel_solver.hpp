@@ -1,0 +1,87 @@
+// channel_solver.hpp -- 2-D Rayleigh-Benard convection on RealFourier(x) x
+// ChebyshevT(z) with the Chebyshev-tau method, stepped on the GPU with IMEX RK.
+// Per Fourier mode the implicit system splits into a buoyancy block and a
+// velocity-pressure block; both are factored once per (dt * a_ii) in FP64.
+#pragma once
+#include <vector>
+
+#include "fourier_solver.hpp"
+#include "runtime.hpp"
+
+namespace tarang {
+
+struct ChannelDesc {
+    int nx, nz;          // RealFourier size (x), ChebyshevT size (z)
+    double Lx, Lz;       // x period, z interval [0, Lz]
+    double kappa, nu;    // diffusivities of b and u
+    double dt;
+    int dtype;           // kF32 / kF64 (state and RHS; the implicit solves are FP64)
+    int stepper;         // kRK222 / kRK443
+    double b_bottom, b_top;  // Dirichlet values of b at z = 0 and z = Lz
+};
+
+enum ChannelVar : int { kChP = 0, kChB = 1, kChUX = 2, kChUZ = 3 };
+
+class ChannelSolver {
+  public:
+    ChannelSolver(Context* ctx, const ChannelDesc& d);
+    ~ChannelSolver();
+    size_t field_elems() const { return (size_t)nxh_ * d_.nz; }
+    size_t field_bytes() const { return field_elems() * 2 * dtype_size((DType)d_.dtype); }
+    // coefficient arrays in the reference layout (nx/2+1, nz), kx fastest, device pointers
+    void set_field(int var, const void* dev);
+    void get_field(int var, void* dev);
+    // tau variable t (0 tau_p, 1 tau_b1, 2 tau_b2, 3 tau_u1.x, 4 tau_u1.z, 5 tau_u2.x, 6 tau_u2.z), nx/2+1 values
+    void get_tau(int t, void* dev);
+    // F_b, F_ux, F_uz of the current state (as the timestepper evaluates it), reference layout
+    void rhs(void* Fb, void* Fux, void* Fuz);
+    void step(int nsteps);
+    void set_dt(double dt);
+    void enable_graph(bool on) { graph_on_ = on; }
+    double time() const { return t_; }
+    long iteration() const { return iter_; }
+    // eager steps with HIP events per kernel kind: [z-backward, x-pass, z-forward, L x, solve, stage]
+    static constexpr int kKinds = 6;
+    void profile(int nsteps, double* avg_ms, long* counts);
+    void kernel_bytes(double* out) const;  // algorithmic bytes per launch, same kinds
+
+  private:
+    template <typename T> void rhs_t(const void* b, const void* ux, const void* uz, void* F0, void* F1, void* F2);
+    template <typename T> void lx_t(int s);
+    template <typename T> void stage_rhs_t(int s, bool final_update);
+    template <typename T> void solve_t();
+    template <typename T> void zero_t(void* p, size_t n);
+    template <typename T> void to_modes_t(const void* in, void* out);
+    template <typename T> void from_modes_t(const void* in, void* out);
+    void factor();
+    void launch_step_body();
+    void step_once(bool allow_graph);
+    template <typename F> void timed(int kind, F&& launch);
+    void invalidate_graph() {
+        if (graph_) (void)hipGraphExecDestroy(graph_);
+        graph_ = nullptr;
+    }
+    void* field_ptr(int var) const;
+
+    Context* ctx_;
+    ChannelDesc d_;
+    int nxh_, mx_, nb_, nu_max_;
+    Tableau tab_;
+    double gamma_ = 0, factored_c_ = -1;
+    double t_ = 0;
+    long iter_ = 0;
+    DevBuf P_, B_, UX_, UZ_, TAU_;
+    DevBuf X0_[3];
+    DevBuf F_[4][3], LX_[4][4], R_[4];
+    DevBuf G_, H_;
+    DevBuf BT_, BDT_, FT_;
+    DevBuf Ab_, Aup_, pivb_, pivu_, info_;
+    bool graph_on_ = true;
+    hipGraphExec_t graph_ = nullptr;
+    double graph_dt_ = 0;
+    bool profiling_ = false;
+    struct Mark { int kind; hipEvent_t a, b; };
+    std::vector<Mark> marks_;
+};
+
+}  // namespace tarang

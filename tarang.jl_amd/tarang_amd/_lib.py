@@ -47,6 +47,24 @@ class FivpDesc(ctypes.Structure):
     ]
 
 
+class ChanDesc(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int),
+        ("nz", ctypes.c_int),
+        ("Lx", ctypes.c_double),
+        ("Lz", ctypes.c_double),
+        ("kappa", ctypes.c_double),
+        ("nu", ctypes.c_double),
+        ("dt", ctypes.c_double),
+        ("dtype", ctypes.c_int),
+        ("timestepper", ctypes.c_int),
+        ("b_bottom", ctypes.c_double),
+        ("b_top", ctypes.c_double),
+    ]
+
+
+CHAN_P, CHAN_B, CHAN_UX, CHAN_UZ = range(4)
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int
 _D = ctypes.c_double
@@ -85,6 +103,18 @@ SIGNATURES = {
     "tarang_fivp_get_tuning": [_P, _PI],
     "tarang_fivp_set_tuning": [_P, _PI],
     "tarang_fivp_kernel_bytes": [_P, _PD, _PD, _PD],
+    "tarang_chan_create": [_P, ctypes.POINTER(ChanDesc), _PP],
+    "tarang_chan_destroy": [_P],
+    "tarang_chan_set_field": [_P, _I, _P],
+    "tarang_chan_get_field": [_P, _I, _P],
+    "tarang_chan_get_tau": [_P, _I, _P],
+    "tarang_chan_rhs": [_P, _P, _P, _P],
+    "tarang_chan_step": [_P, _I],
+    "tarang_chan_set_graph": [_P, _I],
+    "tarang_chan_set_dt": [_P, _D],
+    "tarang_chan_time": [_P, _PD, ctypes.POINTER(ctypes.c_long)],
+    "tarang_chan_profile": [_P, _I, _PD, ctypes.POINTER(ctypes.c_long)],
+    "tarang_chan_kernel_bytes": [_P, _PD],
 }
 
 _lib = None

@@ -69,6 +69,19 @@ class Device:
         return Fivp(self, h, sizes, dtype)
 
 
+    def chan(self, nx, nz, Lx, Lz, kappa, nu, dt, stepper=0, dtype=np.float64, b_bottom=None, b_top=0.0):
+        d = _lib.ChanDesc()
+        d.nx, d.nz, d.Lx, d.Lz = nx, nz, Lx, Lz
+        d.kappa, d.nu, d.dt = kappa, nu, dt
+        d.dtype = _lib.F32 if dtype == np.float32 else _lib.F64
+        d.timestepper = stepper
+        d.b_bottom = Lz if b_bottom is None else b_bottom
+        d.b_top = b_top
+        h = ctypes.c_void_p()
+        _lib.call("tarang_chan_create", self.ctx, ctypes.byref(d), ctypes.byref(h))
+        return Chan(self, h, nx, nz, dtype)
+
+
 class Plan:
     def __init__(self, dev, h, sizes, dtype):
         self.dev, self.h, self.sizes, self.dtype = dev, h, tuple(sizes), dtype
@@ -143,3 +156,61 @@ class Fivp:
 
     def close(self):
         _lib.call("tarang_fivp_destroy", self.h)
+
+
+class Chan:
+    """2-D Rayleigh-Benard channel solver handle (tarang_chan_*)."""
+
+    def __init__(self, dev, h, nx, nz, dtype):
+        self.dev, self.h = dev, h
+        self.ct = torch.complex64 if dtype == np.float32 else torch.complex128
+        self.cshape = (nx // 2 + 1, nz)
+
+    def _empty(self):
+        return torch.zeros(tuple(reversed(self.cshape)), dtype=self.ct, device=self.dev.torch_device)
+
+    def set(self, var, c):
+        cd = to_dev(c, self.ct, self.dev.torch_device)
+        self.dev.sync()
+        _lib.call("tarang_chan_set_field", self.h, int(var), ptr(cd))
+        self.dev.sync()
+
+    def get(self, var):
+        c = self._empty()
+        self.dev.sync()
+        _lib.call("tarang_chan_get_field", self.h, int(var), ptr(c))
+        self.dev.sync()
+        return from_dev(c)
+
+    def tau(self, t):
+        v = torch.zeros(self.cshape[0], dtype=self.ct, device=self.dev.torch_device)
+        self.dev.sync()
+        _lib.call("tarang_chan_get_tau", self.h, int(t), ptr(v))
+        self.dev.sync()
+        return v.cpu().numpy()
+
+    def rhs(self):
+        outs = [self._empty() for _ in range(3)]
+        self.dev.sync()
+        _lib.call("tarang_chan_rhs", self.h, *[ptr(o) for o in outs])
+        self.dev.sync()
+        return [from_dev(o) for o in outs]
+
+    def step(self, n):
+        _lib.call("tarang_chan_step", self.h, int(n))
+        self.dev.sync()
+
+    def set_graph(self, on):
+        _lib.call("tarang_chan_set_graph", self.h, 1 if on else 0)
+
+    def set_dt(self, dt):
+        _lib.call("tarang_chan_set_dt", self.h, float(dt))
+
+    def profile(self, n):
+        ms = (ctypes.c_double * 6)()
+        cnt = (ctypes.c_long * 6)()
+        _lib.call("tarang_chan_profile", self.h, int(n), ms, cnt)
+        return list(ms), list(cnt)
+
+    def close(self):
+        _lib.call("tarang_chan_destroy", self.h)

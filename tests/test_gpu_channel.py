@@ -1,0 +1,212 @@
+"""2-D Rayleigh-Benard channel on the GPU (tarang_chan_*) vs the FP64 oracle.
+
+Tolerances (max-norm, relative to the largest oracle entry): FP64 1e-12 for a
+right-hand side and single steps, FP32 1e-5 (x10 where a Chebyshev series of
+length N is summed in FP32).  At the 1024 x 256 workload the oracle cannot hold
+its dense per-mode matrices, so the check there is mode-by-mode: a small-amplitude
+state confined to modes {0, 1, 3} evolves under the same per-mode systems and the
+same (alias-free) product interactions on a 16 x 256 oracle grid.
+"""
+import numpy as np
+import pytest
+
+from tarang_amd import _lib
+from tests.conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+TS = {"RK222": 0, "RK443": 1}
+TOL = {np.float64: 1e-12, np.float32: 1e-5}
+
+
+def params(Ra=1e5, Pr=1.0):
+    return (Ra * Pr) ** -0.5, (Ra / Pr) ** -0.5
+
+
+def zgrid(N, Lz):
+    return Lz * (1 - np.cos(np.pi * np.arange(N) / (N - 1))) / 2
+
+
+def random_state(O, nx, N, Lx, Lz, seed=3, amp=0.05, raw=1e-3):
+    """Conduction profile plus seeded noise vanishing at the walls; `raw` adds unprojected
+    coefficient noise (imaginary parts at kx = 0 and kx = Nx/2 included)."""
+    rng = np.random.default_rng(seed)
+    d = O.domain([O.REALFOURIER, O.CHEBYSHEVT], [nx, N], bounds=[(0.0, Lx), (0.0, Lz)])
+    z = zgrid(N, Lz)
+    wall = (z * (Lz - z))[None, :]
+    grids = {O.RBC_B: (Lz - z)[None, :] + amp * rng.standard_normal((nx, N)) * wall,
+             O.RBC_UX: amp * rng.standard_normal((nx, N)) * wall,
+             O.RBC_UZ: amp * rng.standard_normal((nx, N)) * wall}
+    out = {}
+    for var, g in grids.items():
+        c = O.forward_transform(d, np.broadcast_to(g, (nx, N)).copy())
+        if raw:
+            c = c + raw * (rng.standard_normal(c.shape) + 1j * rng.standard_normal(c.shape))
+        out[var] = np.asarray(c)
+    return out
+
+
+def projected(c, nx):
+    c = c.copy()
+    c[0] = c[0].real
+    c[nx // 2] = c[nx // 2].real
+    return c
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("nx,N", [(16, 12), (64, 33), (128, 64)])
+def test_channel_rhs_parity(gpu, oracle, dtype, nx, N):
+    print(f"channel rhs nx={nx} N={N} {dtype.__name__}", flush=True)
+    Lx, Lz = 4.0, 1.0
+    kappa, nu = params()
+    st = random_state(oracle, nx, N, Lx, Lz)
+    ref = oracle.RBC2D(nx, N, Lx, Lz, kappa, nu, 1e-2)
+    for var, c in st.items():
+        ref.set(var, projected(c, nx))
+    Fr = ref.rhs()
+    s = gpu.chan(nx, N, Lx, Lz, kappa, nu, 1e-2, dtype=dtype)
+    for var, c in st.items():
+        s.set(var, c)
+    Fg = s.rhs()
+    tol = TOL[dtype] * (10 if dtype == np.float32 else 1)
+    for a, b in zip(Fg, Fr):
+        assert relerr(a, b) < tol
+    s.close()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("ts", ["RK222", "RK443"])
+def test_channel_steps_parity(gpu, oracle, ts, dtype):
+    nx, N, Lx, Lz, dt = 32, 24, 4.0, 1.0, 5e-3
+    print(f"channel steps {ts} {dtype.__name__}", flush=True)
+    kappa, nu = params()
+    st = random_state(oracle, nx, N, Lx, Lz, seed=11)
+    ref = oracle.RBC2D(nx, N, Lx, Lz, kappa, nu, dt, ts)
+    s = gpu.chan(nx, N, Lx, Lz, kappa, nu, dt, stepper=TS[ts], dtype=dtype)
+    for var, c in st.items():
+        ref.set(var, c)
+        s.set(var, c)
+    tol = TOL[dtype] * (10 if dtype == np.float32 else 1)
+    for nsteps in (1, 5):
+        ref.step(nsteps)
+        s.step(nsteps)
+        for var in (oracle.RBC_B, oracle.RBC_UX, oracle.RBC_UZ):
+            assert relerr(s.get(var), ref.get(var)) < tol, (nsteps, var)
+        assert np.abs(s.get(oracle.RBC_P)).max() == 0.0
+        for t in range(7):
+            assert np.abs(s.tau(t)).max() == 0.0
+    s.close()
+
+
+def test_channel_graph_matches_eager(gpu, oracle):
+    nx, N, Lx, Lz, dt = 32, 16, 4.0, 1.0, 5e-3
+    kappa, nu = params()
+    st = random_state(oracle, nx, N, Lx, Lz, seed=5)
+    out = []
+    for graph in (False, True):
+        s = gpu.chan(nx, N, Lx, Lz, kappa, nu, dt)
+        s.set_graph(graph)
+        for var, c in st.items():
+            s.set(var, c)
+        s.step(3)
+        out.append([s.get(v) for v in (oracle.RBC_B, oracle.RBC_UX, oracle.RBC_UZ)])
+        s.close()
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
+def test_channel_dt_change_refactors(gpu, oracle):
+    """A new dt refactors the per-mode systems (factor pairing on dt change)."""
+    nx, N, Lx, Lz = 16, 16, 4.0, 1.0
+    kappa, nu = params()
+    st = random_state(oracle, nx, N, Lx, Lz, seed=8)
+    s = gpu.chan(nx, N, Lx, Lz, kappa, nu, 1e-2)
+    r1 = oracle.RBC2D(nx, N, Lx, Lz, kappa, nu, 1e-2)
+    for var, c in st.items():
+        s.set(var, c)
+        r1.set(var, c)
+    s.step(2)
+    r1.step(2)
+    r2 = oracle.RBC2D(nx, N, Lx, Lz, kappa, nu, 4e-3)
+    for var in (oracle.RBC_B, oracle.RBC_UX, oracle.RBC_UZ):
+        r2.set(var, r1.get(var))
+    s.set_dt(4e-3)
+    s.step(3)
+    r2.step(3)
+    for var in (oracle.RBC_B, oracle.RBC_UX, oracle.RBC_UZ):
+        assert relerr(s.get(var), r2.get(var)) < 1e-12
+    s.close()
+
+
+def test_channel_conduction_steady_and_bc(gpu, oracle):
+    nx, N, Lx, Lz = 64, 32, 4.0, 1.0
+    kappa, nu = params()
+    d = oracle.domain([oracle.REALFOURIER, oracle.CHEBYSHEVT], [nx, N], bounds=[(0.0, Lx), (0.0, Lz)])
+    z = zgrid(N, Lz)
+    prof = 2.0 - 1.5 * z
+    s = gpu.chan(nx, N, Lx, Lz, kappa, nu, 1e-2, b_bottom=2.0, b_top=0.5)
+    s.set(oracle.RBC_B, oracle.forward_transform(d, np.tile(prof, (nx, 1))))
+    s.step(4)
+    b = oracle.backward_transform(d, s.get(oracle.RBC_B))
+    assert np.abs(b - prof[None, :]).max() < 1e-13
+    assert np.abs(s.get(oracle.RBC_UZ)).max() < 1e-13
+    s.close()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_channel_full_size_modewise(gpu, oracle, dtype):
+    """1024 x 256 (the channel workload) vs a 16 x 256 oracle on modes {0, 1, 3}."""
+    nx, N, Lx, Lz, dt, ts = 1024, 256, 4.0, 1.0, 2e-3, "RK222"
+    print(f"channel full size {dtype.__name__}", flush=True)
+    kappa, nu = params(1e6)
+    small = 16
+    rng = np.random.default_rng(21)
+    ds = oracle.domain([oracle.REALFOURIER, oracle.CHEBYSHEVT], [small, N], bounds=[(0.0, Lx), (0.0, Lz)])
+    z = zgrid(N, Lz)
+    wall = z * (Lz - z)
+    st = {}
+    for var in (oracle.RBC_B, oracle.RBC_UX, oracle.RBC_UZ):
+        c = np.zeros((small // 2 + 1, N), complex)
+        if var == oracle.RBC_B:
+            c = np.asarray(oracle.forward_transform(ds, np.tile(Lz - z, (small, 1))))
+        for m in (1, 3):
+            prof = np.polynomial.chebyshev.chebfit(2 * z / Lz - 1, wall * np.sin((m + 1) * np.pi * z / Lz), 40)
+            c[m, :41] += 1e-6 * small * prof * np.exp(1j * rng.uniform(0, 2 * np.pi))
+        st[var] = c
+    ref = oracle.RBC2D(small, N, Lx, Lz, kappa, nu, dt, ts)
+    for var, c in st.items():
+        ref.set(var, c)
+    s = gpu.chan(nx, N, Lx, Lz, kappa, nu, dt, stepper=TS[ts], dtype=dtype)
+    scale = nx / small   # unnormalised rfft: coefficients scale with Nx
+    for var, c in st.items():
+        big = np.zeros((nx // 2 + 1, N), complex)
+        big[:small // 2 + 1] = c * scale
+        big[small // 2] = 0   # the small grid's Nyquist column is empty
+        s.set(var, big)
+    ref.step(3)
+    s.step(3)
+    tol = {np.float64: 1e-11, np.float32: 1e-4}[dtype]
+    for var in (oracle.RBC_B, oracle.RBC_UX, oracle.RBC_UZ):
+        g = s.get(var) / scale
+        r = ref.get(var)
+        for m in (0, 1, 3):
+            # a mode that is roundoff (uz at kx = 0) is measured against the perturbation size
+            den = max(np.abs(r[m]).max(), np.abs(r[1]).max())
+            assert np.abs(g[m] - r[m]).max() / den < tol, (var, m)
+        assert np.abs(g[small // 2 + 1:]).max() < 1e-9 * np.abs(r[1]).max()
+    s.close()
+
+
+def test_channel_rejects_unsupported(gpu):
+    kappa, nu = params()
+    with pytest.raises(_lib.TarangError) as e:
+        gpu.chan(24, 16, 4.0, 1.0, kappa, nu, 1e-2)
+    assert e.value.code == _lib.ERR_INVALID and "RealFourier size 24" in str(e.value)
+    with pytest.raises(_lib.TarangError) as e:
+        gpu.chan(32, 600, 4.0, 1.0, kappa, nu, 1e-2)
+    assert e.value.code == _lib.ERR_INVALID
+    with pytest.raises(_lib.TarangError) as e:
+        gpu.chan(32, 16, 4.0, 1.0, kappa, nu, 1e-2, stepper=2)
+    assert e.value.code == _lib.ERR_INVALID
+    with pytest.raises(_lib.TarangError):
+        gpu.chan(2048, 16, 4.0, 1.0, kappa, nu, 1e-2, dtype=np.float64)
