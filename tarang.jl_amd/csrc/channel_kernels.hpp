@@ -354,92 +354,138 @@ __global__ __launch_bounds__(NT) void k_ch_lx(const cplx<T>* __restrict__ p, con
 
 // ---------------------------------------------------------------------------
 // Per-mode real systems (FP64), column-major, assembled for c = dt * a_ii.
-// b block (nb = N+2): unknowns [b_0..b_{N-1}, tau_b1, tau_b2]
-//   rows k < N : (1 + c kappa kx^2) b_k - c kappa (Dzz b)_k - c kappa Dz[k][N-1] tau_b1 + c [k=N-1] tau_b2
-//   row N      : c sum (-1)^j b_j   (b(z=0)),   row N+1: c sum b_j   (b(z=Lz))
-// u-p block (nu = 3N+4, +1 at kx = 0), with ux = i ux~, tau_u.x = i tau~ (real system):
-//   unknowns [p (N), ux~ (N), uz (N), tau~_u1x, tau_u1z, tau~_u2x, tau_u2z, (tau_p)]
-//   E1 rows  : c (-kx ux~_k + (Dz uz)_k) + c [k=N-1] tau_u1z + c [k=0, kx=0] tau_p
-//   E3x rows : (1 + c nu kx^2) ux~_k - c nu (Dzz ux~)_k - c nu Dz[k][N-1] tau~_u1x + c kx p_k + c [k=N-1] tau~_u2x
-//              (the equation row multiplied by -i)
-//   E3z rows : (1 + c nu kx^2) uz_k - c nu (Dzz uz)_k + c (Dz p)_k - c nu Dz[k][N-1] tau_u1z + c [k=N-1] tau_u2z
-//              (the buoyancy -c b_k moves to the right-hand side)
-//   BC rows  : ux(0), uz(0), ux(Lz), uz(Lz), then integ(p) at kx = 0
+//
+// With ux = i ux~ and tau_u.x = i tau~ (E3x rows multiplied by -i) every mode's
+// system is real.  Dz and Dzz flip / keep the parity of the Chebyshev index, the
+// lifts sit at index N-1, and the wall rows are replaced by their half-sum
+// (even part) and half-difference (odd part), so each mode splits exactly into
+// four independent systems, two per block, labelled by a parity g:
+//   b block, group g   : unknowns b_k (k = g mod 2), one tau;  rows E2_k, one wall row
+//   u-p block, group g : unknowns ux~_k, p_k (k = g), uz_k (k = g^1), two taus
+//                        (+ tau_p at kx = 0, g = 0);  rows E3x_k, E1_k (k = g),
+//                        E3z_k (k = g^1), ux wall row, uz wall row (+ integ(p))
+// Tau of each group (PL = parity of N-1):
+//   b:   g = PL^1 -> tau_b1 (column -c kappa Dz[:,N-1]);  g = PL -> tau_b2 (c e_{N-1})
+//   u-p: g = PL   -> tau~_u2x (c e_{N-1} in E3x), tau_u1z (-c nu Dz[:,N-1] in E3z, c e_{N-1} in E1)
+//        g = PL^1 -> tau~_u1x (-c nu Dz[:,N-1] in E3x), tau_u2z (c e_{N-1} in E3z)
+// Row equations (c = dt a_ii, kx = k0x m):
+//   E2  : (1 + c kappa kx^2) b_k - c kappa (Dzz b)_k + tau terms
+//   E3x : (1 + c nu kx^2) ux~_k - c nu (Dzz ux~)_k + c kx p_k + tau terms
+//   E1  : -c kx ux~_k + c (Dz uz)_k + tau terms
+//   E3z : (1 + c nu kx^2) uz_k - c nu (Dzz uz)_k + c (Dz p)_k + tau terms  (buoyancy -c b_k on the rhs)
+//   wall: c sum_{k = parity} f_k ;  integ(p): c Lx (Lz/2) sum_{k even} 2/(1-k^2) p_k
 // ---------------------------------------------------------------------------
+__host__ __device__ inline int npar(int N, int g) { return (N + 1 - g) / 2; }  // #k in [0,N) with k%2 == g
+__host__ __device__ inline int chb_size(int N, int g) { return npar(N, g) + 1; }
+__host__ __device__ inline int chu_size(int N, int g, int m) {
+    return 2 * npar(N, g) + npar(N, g ^ 1) + 2 + ((m == 0 && g == 0) ? 1 : 0);
+}
+
 struct ChSys {
-    int N, nxh, nu_max;
+    int N, nxh;
+    size_t sb, su;       // per-system strides (elements) of the b and u-p factor arrays
     double k0x, sz, kappa, nu, c, Lx, Lz;
 };
 
-__global__ void k_ch_assemble(ChSys s, double* Ab, double* Aup) {
-    const int m = blockIdx.x;
-    const int N = s.N, nb = N + 2;
-    const int nu = 3 * N + 4 + (m == 0 ? 1 : 0);
+__host__ __device__ inline double chb_entry(const ChSys& s, int g, int m, int row, int col) {
+    const int N = s.N, PL = (N - 1) & 1, ng = npar(N, g);
     const double kx = s.k0x * m, c = s.c;
-    double* A = Ab + (size_t)m * nb * nb;
-    for (size_t idx = threadIdx.x; idx < (size_t)nb * nb; idx += blockDim.x) {
-        const int col = (int)(idx / nb), row = (int)(idx % nb);
-        double v = 0;
-        if (row < N) {
-            if (col < N) v = (row == col ? 1.0 + c * s.kappa * kx * kx : 0.0) - c * s.kappa * cheb_dzz(row, col, s.sz);
-            else if (col == N) v = -c * s.kappa * cheb_dz(row, N - 1, s.sz);
-            else v = (row == N - 1) ? c : 0.0;
-        } else if (col < N) {
-            v = row == N ? c * ((col & 1) ? -1.0 : 1.0) : c;
+    if (row < ng) {
+        const int k = 2 * row + g;
+        if (col < ng) {
+            const int j = 2 * col + g;
+            return (j == k ? 1.0 + c * s.kappa * kx * kx : 0.0) - c * s.kappa * cheb_dzz(k, j, s.sz);
         }
-        A[idx] = v;
+        if (g == (PL ^ 1)) return -c * s.kappa * cheb_dz(k, N - 1, s.sz);
+        return k == N - 1 ? c : 0.0;
     }
-    double* U = Aup + (size_t)m * s.nu_max * s.nu_max;
-    for (size_t idx = threadIdx.x; idx < (size_t)nu * nu; idx += blockDim.x) {
-        const int col = (int)(idx / nu), row = (int)(idx % nu);
-        const int cb = col / N, cj = col % N;  // column block: 0 p, 1 ux~, 2 uz, 3 taus
-        const int rb = row / N, rk = row % N;  // row block: 0 E1, 1 E3x, 2 E3z, 3 BC
-        double v = 0;
-        const int tcol = col - 3 * N;  // 0 tau~u1x, 1 tau_u1z, 2 tau~u2x, 3 tau_u2z, 4 tau_p
-        if (rb == 0) {
-            if (cb == 1 && cj == rk) v = -c * kx;
-            else if (cb == 2) v = c * cheb_dz(rk, cj, s.sz);
-            else if (col >= 3 * N) v = (tcol == 1 && rk == N - 1) ? c : ((tcol == 4 && rk == 0) ? c : 0.0);
-        } else if (rb == 1) {
-            if (cb == 1) v = (cj == rk ? 1.0 + c * s.nu * kx * kx : 0.0) - c * s.nu * cheb_dzz(rk, cj, s.sz);
-            else if (cb == 0 && cj == rk) v = c * kx;
-            else if (col >= 3 * N) v = tcol == 0 ? -c * s.nu * cheb_dz(rk, N - 1, s.sz) : ((tcol == 2 && rk == N - 1) ? c : 0.0);
-        } else if (rb == 2) {
-            if (cb == 2) v = (cj == rk ? 1.0 + c * s.nu * kx * kx : 0.0) - c * s.nu * cheb_dzz(rk, cj, s.sz);
-            else if (cb == 0) v = c * cheb_dz(rk, cj, s.sz);
-            else if (col >= 3 * N) v = tcol == 1 ? -c * s.nu * cheb_dz(rk, N - 1, s.sz) : ((tcol == 3 && rk == N - 1) ? c : 0.0);
-        } else {
-            const int bcr = row - 3 * N;  // 0 ux(0), 1 uz(0), 2 ux(Lz), 3 uz(Lz), 4 integ(p)
-            if (bcr < 4 && col < 3 * N) {
-                const int want = (bcr % 2 == 0) ? 1 : 2;
-                if (cb == want) v = (bcr < 2) ? c * ((cj & 1) ? -1.0 : 1.0) : c;
-            } else if (bcr == 4 && cb == 0 && (cj % 2) == 0) {
-                v = c * s.Lx * (s.Lz / 2.0) * 2.0 / (1.0 - (double)cj * cj);
-            }
-        }
-        U[idx] = v;
+    return col < ng ? c : 0.0;
+}
+
+// tau slots of a u-p group in local order; state tau index: 0 tau_p, 3 tau_u1x, 4 tau_u1z, 5 tau_u2x, 6 tau_u2z
+__host__ __device__ inline int chu_tau_slot(int N, int g, int t) {
+    const int PL = (N - 1) & 1;
+    if (t == 2) return 0;
+    if (g == PL) return t == 0 ? 5 : 4;
+    return t == 0 ? 3 : 6;
+}
+
+__host__ __device__ inline double chu_entry(const ChSys& s, int g, int m, int row, int col) {
+    const int N = s.N, ng = npar(N, g), nh = npar(N, g ^ 1);
+    const double kx = s.k0x * m, c = s.c;
+    // column: 0 ux~, 1 p, 2 uz, 3 tau (j = its state slot)
+    int ct, j;
+    if (col < ng) { ct = 0; j = 2 * col + g; }
+    else if (col < 2 * ng) { ct = 1; j = 2 * (col - ng) + g; }
+    else if (col < 2 * ng + nh) { ct = 2; j = 2 * (col - 2 * ng) + (g ^ 1); }
+    else { ct = 3; j = chu_tau_slot(N, g, col - 2 * ng - nh); }
+    if (row < ng) {  // E3x
+        const int k = 2 * row + g;
+        if (ct == 0) return (j == k ? 1.0 + c * s.nu * kx * kx : 0.0) - c * s.nu * cheb_dzz(k, j, s.sz);
+        if (ct == 1) return j == k ? c * kx : 0.0;
+        if (ct == 3 && j == 3) return -c * s.nu * cheb_dz(k, N - 1, s.sz);
+        if (ct == 3 && j == 5) return k == N - 1 ? c : 0.0;
+        return 0.0;
+    }
+    if (row < 2 * ng) {  // E1
+        const int k = 2 * (row - ng) + g;
+        if (ct == 0) return j == k ? -c * kx : 0.0;
+        if (ct == 2) return c * cheb_dz(k, j, s.sz);
+        if (ct == 3 && j == 4) return k == N - 1 ? c : 0.0;
+        if (ct == 3 && j == 0) return k == 0 ? c : 0.0;
+        return 0.0;
+    }
+    if (row < 2 * ng + nh) {  // E3z
+        const int k = 2 * (row - 2 * ng) + (g ^ 1);
+        if (ct == 2) return (j == k ? 1.0 + c * s.nu * kx * kx : 0.0) - c * s.nu * cheb_dzz(k, j, s.sz);
+        if (ct == 1) return c * cheb_dz(k, j, s.sz);
+        if (ct == 3 && j == 4) return -c * s.nu * cheb_dz(k, N - 1, s.sz);
+        if (ct == 3 && j == 6) return k == N - 1 ? c : 0.0;
+        return 0.0;
+    }
+    const int w = row - 2 * ng - nh;  // 0 ux wall part, 1 uz wall part, 2 integ(p)
+    if (w == 0) return ct == 0 ? c : 0.0;
+    if (w == 1) return ct == 2 ? c : 0.0;
+    if (ct == 1 && (j % 2) == 0) return c * s.Lx * (s.Lz / 2.0) * 2.0 / (1.0 - (double)j * j);
+    return 0.0;
+}
+
+// grid (2 nxh, 2): blockIdx.x = 2 m + g, blockIdx.y = 0 b system / 1 u-p system
+__global__ void k_ch_assemble(ChSys s, double* Ab, double* Aup) {
+    const int w = blockIdx.x, m = w >> 1, g = w & 1;
+    if (blockIdx.y == 0) {
+        const int n = chb_size(s.N, g);
+        double* A = Ab + (size_t)w * s.sb;
+        for (size_t idx = threadIdx.x; idx < (size_t)n * n; idx += blockDim.x)
+            A[idx] = chb_entry(s, g, m, (int)(idx % n), (int)(idx / n));
+    } else {
+        const int n = chu_size(s.N, g, m);
+        double* A = Aup + (size_t)w * s.su;
+        for (size_t idx = threadIdx.x; idx < (size_t)n * n; idx += blockDim.x)
+            A[idx] = chu_entry(s, g, m, (int)(idx % n), (int)(idx / n));
     }
 }
 
 // Right-looking LU with partial pivoting of one n x n column-major FP64 matrix
-// per workgroup; piv[k] = row swapped with k.  info[m] = 0, or k+1 at the first
-// zero pivot.  The pivot column and row are staged in LDS.
+// per workgroup (workgroup w = 2 m + g).  perm receives the row permutation as a
+// gather map ((P b)_i = b_perm[i]); info[w] = 0, or k+1 at the first zero pivot.
+// The pivot column and row and the swap sequence are staged in LDS
+// (3 n doubles of dynamic LDS).  kind 0: b systems, 1: u-p systems.
 template <int NT>
-__global__ __launch_bounds__(NT) void k_lu_factor(double* A, int* piv, int* info, int n_common, int n0, size_t stride,
-                                                  size_t piv_stride) {
+__global__ __launch_bounds__(NT) void k_lu_factor(double* A, int* perm, int* info, int N, int kind, size_t stride,
+                                                  size_t pstride) {
     extern __shared__ unsigned char smem_raw[];
-    const int nmax = n0 > n_common ? n0 : n_common;
-    double* col = reinterpret_cast<double*>(smem_raw);  // [nmax]
-    double* row = col + nmax;                            // [nmax]
+    const int w = blockIdx.x, m = w >> 1, g = w & 1;
+    const int n = kind == 0 ? chb_size(N, g) : chu_size(N, g, m);
+    double* col = reinterpret_cast<double*>(smem_raw);  // [n]
+    double* row = col + n;                               // [n]
+    int* piv = reinterpret_cast<int*>(row + n);          // [n]
+    int* pm = piv + n;                                   // [n]
     __shared__ double red_v[NT];
     __shared__ int red_i[NT];
-    const int m = blockIdx.x;
-    const int n = (m == 0 && n0 > 0) ? n0 : n_common;
-    double* a = A + (size_t)m * stride;
-    int* pv = piv + (size_t)m * piv_stride;
+    double* a = A + (size_t)w * stride;
     int bad = 0;
     for (int k = 0; k < n; ++k) {
-        // pivot search
         double best = -1.0;
         int bi = k;
         for (int i = k + threadIdx.x; i < n; i += NT) {
@@ -463,12 +509,11 @@ __global__ __launch_bounds__(NT) void k_lu_factor(double* A, int* piv, int* info
         const int p = red_i[0];
         const double pval = red_v[0];
         __syncthreads();
-        if (threadIdx.x == 0) pv[k] = p;
+        if (threadIdx.x == 0) piv[k] = p;
         if (pval == 0.0) {
             if (!bad) bad = k + 1;
             continue;
         }
-        // swap rows k and p
         if (p != k)
             for (int c = threadIdx.x; c < n; c += NT) {
                 const double t = a[(size_t)c * n + k];
@@ -484,76 +529,158 @@ __global__ __launch_bounds__(NT) void k_lu_factor(double* A, int* piv, int* info
         }
         for (int c = k + 1 + threadIdx.x; c < n; c += NT) row[c] = a[(size_t)c * n + k];
         __syncthreads();
-        // rank-1 update of the trailing block; each thread sweeps columns, rows coalesced
-        const int w = n - k - 1;
-        for (size_t idx = threadIdx.x; idx < (size_t)w * w; idx += NT) {
-            const int cc = k + 1 + (int)(idx / w), i = k + 1 + (int)(idx % w);
+        const int wdt = n - k - 1;
+        for (size_t idx = threadIdx.x; idx < (size_t)wdt * wdt; idx += NT) {
+            const int cc = k + 1 + (int)(idx / wdt), i = k + 1 + (int)(idx % wdt);
             a[(size_t)cc * n + i] -= col[i] * row[cc];
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) info[m] = bad;
-}
-
-}  // namespace tarang
-
-namespace tarang {
-
-// forward/backward substitution with the LU factors of one mode for two real
-// right-hand sides held in LDS (y[2][n]); pivots applied first.
-template <int NT>
-__device__ inline void lu_solve_lds(const double* __restrict__ a, const int* __restrict__ pv, int n, double* y0,
-                                    double* y1) {
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0) {  // swap sequence -> gather map
+        for (int i = 0; i < n; ++i) pm[i] = i;
         for (int k = 0; k < n; ++k) {
-            const int p = pv[k];
-            if (p != k) {
-                double t = y0[k]; y0[k] = y0[p]; y0[p] = t;
-                t = y1[k]; y1[k] = y1[p]; y1[p] = t;
-            }
+            const int q = piv[k];
+            const int t = pm[k];
+            pm[k] = pm[q];
+            pm[q] = t;
         }
+        info[w] = bad;
     }
     __syncthreads();
-    for (int j = 0; j < n; ++j) {  // unit lower triangle
-        const double x0 = y0[j], x1 = y1[j];
-        const double* colj = a + (size_t)j * n;
-        for (int i = j + 1 + threadIdx.x; i < n; i += NT) {
-            const double l = colj[i];
-            y0[i] -= l * x0;
-            y1[i] -= l * x1;
+    for (int i = threadIdx.x; i < n; i += NT) perm[(size_t)w * pstride + i] = pm[i];
+}
+
+// value of v in lane `lane` (wave-uniform index), through scalar registers
+__device__ inline double readlane_f64(double v, int lane) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
+// Solve with LU factors for NR real right-hand sides held in LDS (y[r*ld + i]).
+// Blocked by 64 columns: the diagonal triangle of each block runs in wavefront 0
+// (lane i owns row i, values broadcast with readlane), the rows outside the
+// block are updated by all threads as a coalesced column-panel product.
+// tmp: NR*ld doubles of LDS scratch (may alias blk); blk: 64*64 doubles of LDS.
+template <int NT, int NR>
+__device__ void lu_solve_blocked(const double* __restrict__ a, const int* __restrict__ perm, int n, double* y, int ld,
+                                 double* tmp, double* blk) {
+    for (int i = threadIdx.x; i < n; i += NT) {
+        const int q = perm[i];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) tmp[r * ld + i] = y[r * ld + q];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += NT)
+#pragma unroll
+        for (int r = 0; r < NR; ++r) y[r * ld + i] = tmp[r * ld + i];
+    __syncthreads();
+    // L y = P b (unit lower triangle)
+    for (int j0 = 0; j0 < n; j0 += 64) {
+        const int nb = min(64, n - j0);
+        for (int idx = threadIdx.x; idx < nb * 64; idx += NT) {
+            const int c = idx >> 6, rr = idx & 63;
+            if (rr < nb) blk[c * 64 + rr] = a[(size_t)(j0 + c) * n + j0 + rr];
+        }
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            const int i = threadIdx.x;
+            double v[NR];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) v[r] = i < nb ? y[r * ld + j0 + i] : 0.0;
+            for (int j = 0; j < nb - 1; ++j) {
+                const double l = (i > j && i < nb) ? blk[j * 64 + i] : 0.0;
+#pragma unroll
+                for (int r = 0; r < NR; ++r) v[r] -= l * readlane_f64(v[r], j);
+            }
+            if (i < nb)
+#pragma unroll
+                for (int r = 0; r < NR; ++r) y[r * ld + j0 + i] = v[r];
+        }
+        __syncthreads();
+        for (int i = j0 + nb + threadIdx.x; i < n; i += NT) {
+            double acc[NR];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) acc[r] = 0.0;
+            const double* colp = a + (size_t)j0 * n + i;
+#pragma unroll 16
+            for (int j = 0; j < nb; ++j) {
+                const double l = colp[(size_t)j * n];
+#pragma unroll
+                for (int r = 0; r < NR; ++r) acc[r] += l * y[r * ld + j0 + j];
+            }
+#pragma unroll
+            for (int r = 0; r < NR; ++r) y[r * ld + i] -= acc[r];
         }
         __syncthreads();
     }
-    for (int j = n - 1; j >= 0; --j) {  // upper triangle
-        const double* colj = a + (size_t)j * n;
-        const double d = colj[j];
-        const double x0 = y0[j] / d, x1 = y1[j] / d;
-        for (int i = threadIdx.x; i < j; i += NT) {
-            const double u = colj[i];
-            y0[i] -= u * x0;
-            y1[i] -= u * x1;
+    // U x = y (upper triangle with diagonal)
+    for (int j1 = n; j1 > 0; j1 -= 64) {
+        const int j0 = j1 > 64 ? j1 - 64 : 0, nb = j1 - j0;
+        for (int idx = threadIdx.x; idx < nb * 64; idx += NT) {
+            const int c = idx >> 6, rr = idx & 63;
+            if (rr < nb) blk[c * 64 + rr] = a[(size_t)(j0 + c) * n + j0 + rr];
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            y0[j] = x0;
-            y1[j] = x1;
+        if (threadIdx.x < 64) {
+            const int i = threadIdx.x;
+            double v[NR];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) v[r] = i < nb ? y[r * ld + j0 + i] : 0.0;
+            const double rd = i < nb ? 1.0 / blk[i * 64 + i] : 0.0;  // reciprocals off the dependency chain
+            for (int j = nb - 1; j >= 0; --j) {
+                const double u = i < j ? blk[j * 64 + i] : 0.0;
+                const double dj = readlane_f64(rd, j);
+#pragma unroll
+                for (int r = 0; r < NR; ++r) {
+                    const double xj = readlane_f64(v[r], j) * dj;
+                    v[r] = i == j ? xj : v[r] - u * xj;
+                }
+            }
+            if (i < nb)
+#pragma unroll
+                for (int r = 0; r < NR; ++r) y[r * ld + j0 + i] = v[r];
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < j0; i += NT) {
+            double acc[NR];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) acc[r] = 0.0;
+            const double* colp = a + (size_t)j0 * n + i;
+#pragma unroll 16
+            for (int j = 0; j < nb; ++j) {
+                const double u = colp[(size_t)j * n];
+#pragma unroll
+                for (int r = 0; r < NR; ++r) acc[r] += u * y[r * ld + j0 + j];
+            }
+#pragma unroll
+            for (int r = 0; r < NR; ++r) y[r * ld + i] -= acc[r];
         }
         __syncthreads();
     }
 }
 
 struct ChSolveArgs {
-    const double* Ab;   // b-block factors, stride nb*nb per mode
-    const int* pivb;    // stride nb
-    const double* Aup;  // u-p factors, stride nu_max^2 per mode
-    const int* pivu;    // stride nu_max
-    int N, nxh, nu_max;
+    const double* Ab;   // b-system factors, stride sb
+    const int* permb;   // stride pb
+    const double* Aup;  // u-p factors, stride su
+    const int* permu;   // stride pu
+    int N, nxh;
+    size_t sb, su, pb, pu;
+    int ld;             // LDS row length of one right-hand side (>= largest system)
     double c;           // dt * a_ii
     double bc_b0;       // b(z=0) and b(z=Lz) in coefficient space at kx = 0 (Nx * value)
     double bc_b1;
 };
 
-// One workgroup per mode: solve the b block, then the rotated u-p block.
+// LDS doubles of k_ch_solve: y (2 x ld), the 64 x 64 block (also the permutation
+// scratch, 2 ld <= 4096), buoyancy (2 x (N/2+1))
+__host__ __device__ inline size_t ch_solve_lds_doubles(int ld, int N) {
+    return (size_t)2 * ld + 64 * 64 + 2 * (size_t)(N / 2 + 1);
+}
+
+// grid 2 nxh: workgroup (m, g) solves the b system of parity h = g^1 (whose values
+// enter the E3z rows of u-p group g as buoyancy) and then u-p group g.
 // rhs rows E1, E2, E3x, E3z (mode-major) in; state p, b, ux, uz, taus out.
 template <typename T, int NT>
 __global__ __launch_bounds__(NT) void k_ch_solve(ChSolveArgs s, const cplx<T>* __restrict__ R1,
@@ -561,67 +688,73 @@ __global__ __launch_bounds__(NT) void k_ch_solve(ChSolveArgs s, const cplx<T>* _
                                                  const cplx<T>* __restrict__ R3z, cplx<T>* P, cplx<T>* B, cplx<T>* UX,
                                                  cplx<T>* UZ, cplx<T>* taus) {
     extern __shared__ unsigned char smem_raw[];
-    const int N = s.N, m = blockIdx.x;
-    const int nb = N + 2;
-    const int nu = 3 * N + 4 + (m == 0 ? 1 : 0);
-    double* y0 = reinterpret_cast<double*>(smem_raw);  // re
-    double* y1 = y0 + s.nu_max + 1;                     // im
-    double* bre = y1 + s.nu_max + 1;                    // solved b (re, im) for the buoyancy coupling
-    double* bim = bre + N;
+    const int N = s.N, ld = s.ld;
+    const int wg = blockIdx.x, m = wg >> 1, g = wg & 1, h = g ^ 1;
+    const int PL = (N - 1) & 1;
+    double* y = reinterpret_cast<double*>(smem_raw);  // [2][ld]
+    double* blk = y + 2 * ld;                           // [64*64], also the permutation scratch
+    double* tmp = blk;
+    double* bre = blk + 64 * 64;                        // [N/2+1] buoyancy of parity h (re)
+    double* bim = bre + (N / 2 + 1);                    // [N/2+1] (im)
     const size_t base = (size_t)m * N;
-    // ---- b block ----
-    for (int k = threadIdx.x; k < N; k += NT) {
-        const cplx<T> r = R2[base + k];
-        y0[k] = (double)r.x;
-        y1[k] = (double)r.y;
+    // ---- b system of parity h ----
+    const int nh = npar(N, h), nbh = nh + 1;
+    for (int i = threadIdx.x; i < nh; i += NT) {
+        const cplx<T> r = R2[base + 2 * i + h];
+        y[i] = (double)r.x;
+        y[ld + i] = (double)r.y;
     }
     if (threadIdx.x == 0) {
-        y0[N] = (m == 0) ? s.c * s.bc_b0 : 0.0;  // dt a_ii ALG_F on the BC rows
-        y1[N] = 0.0;
-        y0[N + 1] = (m == 0) ? s.c * s.bc_b1 : 0.0;
-        y1[N + 1] = 0.0;
+        // wall rows: half-sum (even part) / half-difference (odd part) of the two Dirichlet values
+        const double gv = h == 0 ? 0.5 * (s.bc_b1 + s.bc_b0) : 0.5 * (s.bc_b1 - s.bc_b0);
+        y[nh] = m == 0 ? s.c * gv : 0.0;
+        y[ld + nh] = 0.0;
     }
     __syncthreads();
-    lu_solve_lds<NT>(s.Ab + (size_t)m * nb * nb, s.pivb + (size_t)m * nb, nb, y0, y1);
-    for (int k = threadIdx.x; k < N; k += NT) {
-        bre[k] = y0[k];
-        bim[k] = y1[k];
-        B[base + k] = cplx<T>{T(y0[k]), T(y1[k])};
+    const int wb = 2 * m + h;
+    lu_solve_blocked<NT, 2>(s.Ab + (size_t)wb * s.sb, s.permb + (size_t)wb * s.pb, nbh, y, ld, tmp, blk);
+    for (int i = threadIdx.x; i < nh; i += NT) {
+        bre[i] = y[i];
+        bim[i] = y[ld + i];
+        B[base + 2 * i + h] = cplx<T>{T(y[i]), T(y[ld + i])};
     }
+    if (threadIdx.x == 0) taus[(size_t)m * 8 + (h == (PL ^ 1) ? 1 : 2)] = cplx<T>{T(y[nh]), T(y[ld + nh])};
+    __syncthreads();
+    // ---- u-p system of parity g ----
+    const int ng = npar(N, g);
+    const int n = chu_size(N, g, m);
+    for (int i = threadIdx.x; i < ng; i += NT) {
+        const cplx<T> rx = R3x[base + 2 * i + g], r1 = R1[base + 2 * i + g];
+        y[i] = (double)rx.y;  // -i * rx
+        y[ld + i] = -(double)rx.x;
+        y[ng + i] = (double)r1.x;
+        y[ld + ng + i] = (double)r1.y;
+    }
+    for (int i = threadIdx.x; i < nh; i += NT) {
+        const cplx<T> rz = R3z[base + 2 * i + h];
+        y[2 * ng + i] = (double)rz.x + s.c * bre[i];  // buoyancy moved to the right-hand side
+        y[ld + 2 * ng + i] = (double)rz.y + s.c * bim[i];
+    }
+    for (int r = 2 * ng + nh + threadIdx.x; r < n; r += NT) {
+        y[r] = 0.0;
+        y[ld + r] = 0.0;
+    }
+    __syncthreads();
+    lu_solve_blocked<NT, 2>(s.Aup + (size_t)wg * s.su, s.permu + (size_t)wg * s.pu, n, y, ld, tmp, blk);
+    for (int i = threadIdx.x; i < ng; i += NT) {
+        UX[base + 2 * i + g] = cplx<T>{T(-y[ld + i]), T(y[i])};  // ux = i ux~
+        P[base + 2 * i + g] = cplx<T>{T(y[ng + i]), T(y[ld + ng + i])};
+    }
+    for (int i = threadIdx.x; i < nh; i += NT) UZ[base + 2 * i + h] = cplx<T>{T(y[2 * ng + i]), T(y[ld + 2 * ng + i])};
     if (threadIdx.x == 0) {
-        taus[(size_t)m * 8 + 1] = cplx<T>{T(y0[N]), T(y1[N])};
-        taus[(size_t)m * 8 + 2] = cplx<T>{T(y0[N + 1]), T(y1[N + 1])};
-    }
-    __syncthreads();
-    // ---- u-p block ----
-    for (int k = threadIdx.x; k < N; k += NT) {
-        const cplx<T> r1 = R1[base + k], rx = R3x[base + k], rz = R3z[base + k];
-        y0[k] = (double)r1.x;
-        y1[k] = (double)r1.y;
-        y0[N + k] = (double)rx.y;  // -i * rx
-        y1[N + k] = -(double)rx.x;
-        y0[2 * N + k] = (double)rz.x + s.c * bre[k];  // buoyancy moved to the right-hand side
-        y1[2 * N + k] = (double)rz.y + s.c * bim[k];
-    }
-    for (int r = 3 * N + threadIdx.x; r < nu; r += NT) {
-        y0[r] = 0.0;
-        y1[r] = 0.0;
-    }
-    __syncthreads();
-    lu_solve_lds<NT>(s.Aup + (size_t)m * s.nu_max * s.nu_max, s.pivu + (size_t)m * s.nu_max, nu, y0, y1);
-    for (int k = threadIdx.x; k < N; k += NT) {
-        P[base + k] = cplx<T>{T(y0[k]), T(y1[k])};
-        UX[base + k] = cplx<T>{T(-y1[N + k]), T(y0[N + k])};  // ux = i ux~
-        UZ[base + k] = cplx<T>{T(y0[2 * N + k]), T(y1[2 * N + k])};
-    }
-    if (threadIdx.x == 0) {
-        const int t0 = 3 * N;
         cplx<T>* tm = taus + (size_t)m * 8;
-        tm[3] = cplx<T>{T(-y1[t0 + 0]), T(y0[t0 + 0])};  // tau_u1x = i tau~
-        tm[4] = cplx<T>{T(y0[t0 + 1]), T(y1[t0 + 1])};
-        tm[5] = cplx<T>{T(-y1[t0 + 2]), T(y0[t0 + 2])};
-        tm[6] = cplx<T>{T(y0[t0 + 3]), T(y1[t0 + 3])};
-        tm[0] = m == 0 ? cplx<T>{T(y0[t0 + 4]), T(y1[t0 + 4])} : cplx<T>{T(0), T(0)};
+        const int t0 = 2 * ng + nh;
+        for (int t = 0; t < n - t0; ++t) {
+            const int slot = chu_tau_slot(N, g, t);
+            const double re = y[t0 + t], im = y[ld + t0 + t];
+            tm[slot] = (slot == 3 || slot == 5) ? cplx<T>{T(-im), T(re)} : cplx<T>{T(re), T(im)};  // tau_u.x = i tau~
+        }
+        if (m != 0 && g == 0) tm[0] = cplx<T>{T(0), T(0)};
     }
 }
 

@@ -78,8 +78,8 @@ ChannelSolver::ChannelSolver(Context* ctx, const ChannelDesc& d) : ctx_(ctx), d_
     TG_REQUIRE(d.Lx > 0 && d.Lz > 0 && d.dt > 0, kErrInvalid, "channel solver: Lx, Lz and dt must be positive");
     nxh_ = d.nx / 2 + 1;
     mx_ = padded_len(d.nx);
-    nb_ = d.nz + 2;
-    nu_max_ = 3 * d.nz + 5;
+    nb_max_ = chb_size(d.nz, 0);
+    nu_max_ = chu_size(d.nz, 0, 0);
     tab_ = make_tableau(d.stepper);
     gamma_ = tab_.AI[1][1];
     ctx_->activate();
@@ -119,11 +119,12 @@ ChannelSolver::ChannelSolver(Context* ctx, const ChannelDesc& d) : ctx_(ctx), d_
     upload(BT_, BT);
     upload(BDT_, BDT);
     upload(FT_, FT);
-    Ab_ = DevBuf(ctx_, (size_t)nxh_ * nb_ * nb_ * sizeof(double));
-    Aup_ = DevBuf(ctx_, (size_t)nxh_ * nu_max_ * nu_max_ * sizeof(double));
-    pivb_ = DevBuf(ctx_, (size_t)nxh_ * nb_ * sizeof(int));
-    pivu_ = DevBuf(ctx_, (size_t)nxh_ * nu_max_ * sizeof(int));
-    info_ = DevBuf(ctx_, (size_t)nxh_ * 2 * sizeof(int));
+    // two parity systems per mode and block
+    Ab_ = DevBuf(ctx_, (size_t)2 * nxh_ * nb_max_ * nb_max_ * sizeof(double));
+    Aup_ = DevBuf(ctx_, (size_t)2 * nxh_ * nu_max_ * nu_max_ * sizeof(double));
+    permb_ = DevBuf(ctx_, (size_t)2 * nxh_ * nb_max_ * sizeof(int));
+    permu_ = DevBuf(ctx_, (size_t)2 * nxh_ * nu_max_ * sizeof(int));
+    info_ = DevBuf(ctx_, (size_t)4 * nxh_ * sizeof(int));
     ctx_->twiddle(mx_, (DType)d.dtype);
     TG_HIP(hipStreamSynchronize(s));
     factor();
@@ -197,25 +198,27 @@ void ChannelSolver::factor() {
     if (c == factored_c_) return;
     const int N = d_.nz;
     hipStream_t s = ctx_->stream;
-    ChSys sys{N, nxh_, nu_max_, 2.0 * M_PI / d_.Lx, 2.0 / d_.Lz, d_.kappa, d_.nu, c, d_.Lx, d_.Lz};
-    hipLaunchKernelGGL(k_ch_assemble, dim3(nxh_), dim3(256), 0, s, sys, Ab_.as<double>(), Aup_.as<double>());
+    const size_t sb = (size_t)nb_max_ * nb_max_, su = (size_t)nu_max_ * nu_max_;
+    ChSys sys{N, nxh_, sb, su, 2.0 * M_PI / d_.Lx, 2.0 / d_.Lz, d_.kappa, d_.nu, c, d_.Lx, d_.Lz};
+    const unsigned nsys = (unsigned)(2 * nxh_);
+    hipLaunchKernelGGL(k_ch_assemble, dim3(nsys, 2), dim3(256), 0, s, sys, Ab_.as<double>(), Aup_.as<double>());
     TG_HIP(hipGetLastError());
     int* info = info_.as<int>();
     constexpr int NT = 256;
-    hipLaunchKernelGGL((k_lu_factor<NT>), dim3(nxh_), dim3(NT), (size_t)2 * nb_ * sizeof(double), s,
-                       Ab_.as<double>(), pivb_.as<int>(), info, nb_, 0, (size_t)nb_ * nb_, (size_t)nb_);
+    hipLaunchKernelGGL((k_lu_factor<NT>), dim3(nsys), dim3(NT), (size_t)nb_max_ * 24, s, Ab_.as<double>(),
+                       permb_.as<int>(), info, N, 0, sb, (size_t)nb_max_);
     TG_HIP(hipGetLastError());
-    hipLaunchKernelGGL((k_lu_factor<NT>), dim3(nxh_), dim3(NT), (size_t)2 * nu_max_ * sizeof(double), s,
-                       Aup_.as<double>(), pivu_.as<int>(), info + nxh_, 3 * N + 4, 3 * N + 5,
-                       (size_t)nu_max_ * nu_max_, (size_t)nu_max_);
+    hipLaunchKernelGGL((k_lu_factor<NT>), dim3(nsys), dim3(NT), (size_t)nu_max_ * 24, s, Aup_.as<double>(),
+                       permu_.as<int>(), info + nsys, N, 1, su, (size_t)nu_max_);
     TG_HIP(hipGetLastError());
-    std::vector<int> h((size_t)2 * nxh_);
+    std::vector<int> h((size_t)2 * nsys);
     TG_HIP(hipMemcpyAsync(h.data(), info, h.size() * sizeof(int), hipMemcpyDeviceToHost, s));
     TG_HIP(hipStreamSynchronize(s));
     std::string bad;
-    for (int m = 0; m < nxh_; ++m) {
-        if (h[m]) bad += " b-block(kx mode " + std::to_string(m) + ")";
-        if (h[nxh_ + m]) bad += " u-p-block(kx mode " + std::to_string(m) + ")";
+    for (unsigned w = 0; w < nsys; ++w) {
+        const std::string mode = "(kx mode " + std::to_string(w / 2) + ", parity " + std::to_string(w % 2) + ")";
+        if (h[w]) bad += " b-system" + mode;
+        if (h[nsys + w]) bad += " u-p-system" + mode;
         if (bad.size() > 200) break;
     }
     TG_REQUIRE(bad.empty(), kErrSingular, "channel solver: singular per-mode system:" + bad);
@@ -317,12 +320,13 @@ template <typename T>
 void ChannelSolver::solve_t() {
     using V = cplx<T>;
     const int N = d_.nz;
-    ChSolveArgs sa{Ab_.as<const double>(), pivb_.as<const int>(), Aup_.as<const double>(), pivu_.as<const int>(),
-                   N, nxh_, nu_max_, d_.dt * gamma_, (double)d_.nx * d_.b_bottom, (double)d_.nx * d_.b_top};
+    ChSolveArgs sa{Ab_.as<const double>(), permb_.as<const int>(), Aup_.as<const double>(), permu_.as<const int>(),
+                   N, nxh_, (size_t)nb_max_ * nb_max_, (size_t)nu_max_ * nu_max_, (size_t)nb_max_, (size_t)nu_max_,
+                   nu_max_, d_.dt * gamma_, (double)d_.nx * d_.b_bottom, (double)d_.nx * d_.b_top};
     constexpr int NT = 256;
-    const size_t lds = ((size_t)2 * (nu_max_ + 1) + 2 * (size_t)N) * sizeof(double);
+    const size_t lds = ch_solve_lds_doubles(nu_max_, N) * sizeof(double);
     timed(4, [&] {
-        hipLaunchKernelGGL((k_ch_solve<T, NT>), dim3(nxh_), dim3(NT), lds, ctx_->stream, sa, R_[0].as<const V>(),
+        hipLaunchKernelGGL((k_ch_solve<T, NT>), dim3(2 * nxh_), dim3(NT), lds, ctx_->stream, sa, R_[0].as<const V>(),
                            R_[1].as<const V>(), R_[2].as<const V>(), R_[3].as<const V>(), P_.as<V>(), B_.as<V>(),
                            UX_.as<V>(), UZ_.as<V>(), TAU_.as<V>());
     });
@@ -468,12 +472,13 @@ void ChannelSolver::kernel_bytes(double* out) const {
     out[1] = (8 * nc + 3 * nc) * vb;                    // read 8 planes, write 3
     out[2] = (3 * nc + 3 * nc) * vb + N * N * rb;       // read 3 planes, write 3 F
     out[3] = (4 * nc + 4 * nc) * vb;                    // read p, b, ux, uz; write 4 rows
-    // solve: read the two factor blocks per mode (FP64) and 4 rhs rows, write 4 fields
+    // solve: read the four parity factor blocks per mode (FP64) and 4 rhs rows, write 4 fields
     double fac = 0;
-    for (int m = 0; m < nxh_; ++m) {
-        const double nu = 3.0 * d_.nz + 4 + (m == 0 ? 1 : 0);
-        fac += ((double)nb_ * nb_ + nu * nu) * 8.0;
-    }
+    for (int m = 0; m < nxh_; ++m)
+        for (int g = 0; g < 2; ++g) {
+            const double nb = chb_size(d_.nz, g), nu = chu_size(d_.nz, g, m);
+            fac += (nb * nb + nu * nu) * 8.0;
+        }
     out[4] = fac + 8 * nc * vb;
     out[5] = 0;
 }

@@ -65,7 +65,7 @@ class ChannelSolver {
 
     Context* ctx_;
     ChannelDesc d_;
-    int nxh_, mx_, nb_, nu_max_;
+    int nxh_, mx_, nb_max_, nu_max_;
     Tableau tab_;
     double gamma_ = 0, factored_c_ = -1;
     double t_ = 0;
@@ -75,7 +75,7 @@ class ChannelSolver {
     DevBuf F_[4][3], LX_[4][4], R_[4];
     DevBuf G_, H_;
     DevBuf BT_, BDT_, FT_;
-    DevBuf Ab_, Aup_, pivb_, pivu_, info_;
+    DevBuf Ab_, Aup_, permb_, permu_, info_;
     bool graph_on_ = true;
     hipGraphExec_t graph_ = nullptr;
     double graph_dt_ = 0;
