@@ -6,7 +6,11 @@ Reports grid-point steps/s for the FP32 path (value) and the FP64 path, the
 FP32-vs-FP64 error, the dominant kernel's roofline figure measured live with HIP
 events, and the FP64 oracle timed on the host cores on a bounded sample.
 
-Usage: python bench.py [--gpus 1] [--steps 1000] [--warmup 20] [--no-cpu]
+`--workload channel` measures the 2-D Rayleigh-Benard channel instead
+(1024x256 RealFourier x ChebyshevT, Ra=2e6, Pr=1, RK222, dt=0.01): one step =
+3 stages, 3 dealiased RHS evaluations, 2 per-mode FP64 tau solves.
+
+Usage: python bench.py [--gpus 1] [--steps 1000] [--warmup 20] [--no-cpu] [--workload turbulence|channel]
 """
 import argparse
 import ctypes
@@ -44,11 +48,14 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run one plain device-to-device copy of this many MiB (counter calibration)")
+    ap.add_argument("--workload", choices=["turbulence", "channel"], default="turbulence")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (no collectives); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
+    if a.workload == "channel":
+        return bench_channel(a)
 
     import numpy as np
     import torch
@@ -194,6 +201,136 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "kernels_f32": kernels,
         "autotuned_variants": tuning,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+CH = dict(nx=1024, nz=256, Lx=4.0, Lz=1.0, Ra=2e6, Pr=1.0, dt=0.01)
+CH_WORKLOAD = ("2-D Rayleigh-Benard, RealFourier x ChebyshevT 1024x256, Lx=4, Lz=1, Ra=2e6, Pr=1, first-order tau "
+               "form (two lifts per bounded variable), 3/2 dealiasing, RK222, dt=0.01")
+CH_KERNELS = ["k_ch_zbwd (Chebyshev evaluation of 8 operand planes)",
+              "k_ch_xpass (inverse x FFTs M=1536, products, forward x FFTs)",
+              "k_ch_zfwd (Chebyshev forward of 3 products)",
+              "k_ch_lx (structured L x per mode)",
+              "k_ch_solve (per-mode FP64 LU solves, parity-split systems)",
+              "stage / copy / zero"]
+
+
+def channel_ic(np, nx, nz, Lz, seed=42):
+    """b0 = Lz - z + 1e-3 N(0,1) z (Lz - z), u = 0 (grid values, x fastest)."""
+    z = Lz * (1 - np.cos(np.pi * np.arange(nz) / (nz - 1))) / 2
+    rng = np.random.default_rng(seed)
+    return (Lz - z)[None, :] + 1e-3 * rng.standard_normal((nx, nz)) * (z * (Lz - z))[None, :], z
+
+
+def bench_channel(a):
+    import numpy as np
+    import torch
+
+    from tarang_amd import _lib, diagnostics
+
+    nx, nz, Lx, Lz, Ra, Pr, dt = (CH[k] for k in ("nx", "nz", "Lx", "Lz", "Ra", "Pr", "dt"))
+    kappa, nu = (Ra * Pr) ** -0.5, (Ra / Pr) ** -0.5
+    steps = a.steps if a.steps != 1000 else 200
+    dev = torch.device("cuda", 0)
+    bg, z = channel_ic(np, nx, nz, Lz)
+    # coefficients: rfft in x, Chebyshev interpolation in z (exact on the Gauss-Lobatto grid)
+    k = np.arange(nz)
+    T = np.cos(np.outer(np.pi * np.arange(nz) / (nz - 1), k)) * ((-1.0) ** k)[None, :]
+    Bc = np.linalg.solve(T, np.fft.rfft(bg, axis=0).T).T
+    ctx = ctypes.c_void_p()
+    _lib.call("tarang_ctx_create", 0, ctypes.byref(ctx))
+    results, finals, solvers = {}, {}, {}
+    for name, code, ct in (("f64", _lib.F64, torch.complex128), ("f32", _lib.F32, torch.complex64)):
+        d = _lib.ChanDesc()
+        d.nx, d.nz, d.Lx, d.Lz, d.kappa, d.nu, d.dt = nx, nz, Lx, Lz, kappa, nu, dt
+        d.dtype, d.timestepper, d.b_bottom, d.b_top = code, 0, Lz, 0.0
+        h = ctypes.c_void_p()
+        t0 = time.perf_counter()
+        _lib.call("tarang_chan_create", ctx, ctypes.byref(d), ctypes.byref(h))
+        tf = time.perf_counter() - t0
+        solvers[name] = h
+        Bd = torch.as_tensor(np.ascontiguousarray(Bc.T), device=dev).to(ct).contiguous()
+        Zd = torch.zeros_like(Bd)
+        torch.cuda.synchronize()
+
+        def reset():
+            _lib.call("tarang_chan_set_field", h, _lib.CHAN_B, ctypes.c_void_p(Bd.data_ptr()))
+            for v in (_lib.CHAN_P, _lib.CHAN_UX, _lib.CHAN_UZ):
+                _lib.call("tarang_chan_set_field", h, v, ctypes.c_void_p(Zd.data_ptr()))
+        reset()
+        log(f"channel {name}: setup (assemble + factor) {tf:.3f} s; warmup {a.warmup} steps")
+        _lib.call("tarang_chan_step", h, a.warmup)
+        reset()
+        _lib.call("tarang_ctx_sync", ctx)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _lib.call("tarang_chan_step", h, steps)
+        _lib.call("tarang_ctx_sync", ctx)
+        t1 = time.perf_counter()
+        results[name] = ((t1 - t0) * 1e3 / steps, tf)
+        outs = []
+        for v in (_lib.CHAN_B, _lib.CHAN_UZ):
+            o = torch.zeros_like(Bd)
+            _lib.call("tarang_chan_get_field", h, v, ctypes.c_void_p(o.data_ptr()))
+            outs.append(o)
+        _lib.call("tarang_ctx_sync", ctx)
+        finals[name] = [o.cpu().numpy().T.astype(np.complex128) for o in outs]
+        log(f"channel {name}: {results[name][0]:.4f} ms/step")
+    nus = {k: diagnostics.nusselt(v[0], v[1], nx, Lz, kappa, Lz) for k, v in finals.items()}
+    err_b = float(np.abs(finals["f32"][0] - finals["f64"][0]).max() / np.abs(finals["f64"][0]).max())
+    err_w = float(np.abs(finals["f32"][1] - finals["f64"][1]).max() / max(np.abs(finals["f64"][1]).max(), 1e-300))
+    # live per-kernel durations, FP32 path
+    avg = (ctypes.c_double * 6)()
+    cnt = (ctypes.c_long * 6)()
+    _lib.call("tarang_chan_profile", solvers["f32"], a.profile_steps, avg, cnt)
+    nb = (ctypes.c_double * 6)()
+    _lib.call("tarang_chan_kernel_bytes", solvers["f32"], nb)
+    per_step = [avg[i] * cnt[i] / a.profile_steps for i in range(6)]
+    dom = max(range(5), key=lambda i: per_step[i])
+    achieved = nb[dom] / (avg[dom] * 1e-3) / 1e9
+    kernels = {CH_KERNELS[i]: {"avg_ms": avg[i], "launches_per_step": cnt[i] / a.profile_steps,
+                               "algorithmic_bytes": nb[i] or None,
+                               "GBps": (nb[i] / (avg[i] * 1e-3) / 1e9) if nb[i] and avg[i] else None}
+               for i in range(6)}
+    for h in solvers.values():
+        _lib.call("tarang_chan_destroy", h)
+    _lib.call("tarang_ctx_destroy", ctx)
+    cpu = None
+    if not a.no_cpu:
+        from oracle import oracle as O
+        threads = min(os.cpu_count() or 1, 16)
+        O.set_threads(threads)
+        sx, cs = 32, 2
+        ref = O.RBC2D(sx, nz, Lx, Lz, kappa, nu, dt, "RK222")
+        bgs, _ = channel_ic(np, sx, nz, Lz)
+        ref.set(O.RBC_B, O.forward_transform(ref.d, bgs))
+        ref.step(1)  # factorisation happens on the first step
+        t0 = time.perf_counter()
+        ref.step(cs)
+        el = time.perf_counter() - t0
+        cpu = {"value": sx * nz * cs / el, "unit": "grid-point steps/s", "cores": threads, "kind": "port",
+               "sample": f"{cs} RK222 steps at Nx={sx}, Nz={nz} (17 of the 513 per-mode systems, same size each), "
+                         f"FP64 oracle with dense complex per-mode LU, factorisation excluded, {el:.2f} s"}
+    v32 = nx * nz / (results["f32"][0] * 1e-3)
+    v64 = nx * nz / (results["f64"][0] * 1e-3)
+    line = {
+        "metric": "grid-point steps/s, FP32+refine vs FP64 at <=1e-6 rel err; p50/p99 latency",
+        "value": v32, "unit": "grid-point steps/s", "n_gpus": 1, "steps": steps, "warmup": a.warmup,
+        "ms_per_step": results["f32"][0], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (b0 = Lz - z + 1e-3 N(0,1) z (Lz - z), u = 0, seed 42)",
+        "config": {"workload": CH_WORKLOAD, "grid": [nx, nz], "timestepper": "RK222", "dt": dt, "Ra": Ra, "Pr": Pr,
+                   "parallelism": "single GPU"},
+        "fp64": {"value": v64, "ms_per_step": results["f64"][0], "setup_s": results["f64"][1]},
+        "setup_s_f32": results["f32"][1],
+        "speedup_fp32_vs_fp64": v32 / v64,
+        "fp32_vs_fp64_rel_err": {"b_maxnorm": err_b, "uz_maxnorm": err_w,
+                                 "nusselt": abs(nus["f32"] - nus["f64"]) / abs(nus["f64"]), "after_steps": steps},
+        "nusselt": nus,
+        "roofline": {"bound": "hbm", "kernel": CH_KERNELS[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "kernels_f32": kernels,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

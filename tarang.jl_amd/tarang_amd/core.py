@@ -314,6 +314,8 @@ class ScalarField:
 
     def __init__(self, domain, name: str = "", dtype=None, bases=None):
         if isinstance(domain, Distributor):  # ScalarField(dist, name, bases, dtype): constant/tau field
+            if not isinstance(name, str):  # ScalarField(dist, bases, name) spelling
+                name, bases = bases, name
             self.domain = None
             self.dist = domain
             self.name = name
@@ -321,6 +323,11 @@ class ScalarField:
             self.dtype = _np_dtype(dtype if dtype is not None else domain.dtype)
             self.current_layout = "c"
             self._g = self._c = None
+            # coefficient storage of a tau variable: one value per x mode (or one value)
+            if _is_gpu(domain.architecture):
+                n = self.bases[0].size // 2 + 1 if self.bases else 1
+                dev = _Device.get(domain.architecture.device)
+                self._c = torch.zeros((n,), dtype=_torch_cplx(self.dtype), device=dev.torch_device)
             return
         self.domain = domain
         self.dist = domain.dist
@@ -368,7 +375,14 @@ class ScalarField:
 class VectorField:
     """Vector field: one ScalarField per coordinate (src/core/field/field_types.jl:152)."""
 
-    def __init__(self, domain: Domain, name: str = "", dtype=None):
+    def __init__(self, domain, name="", dtype=None, bases=None, dtype_=None):
+        if isinstance(domain, Distributor):  # VectorField(dist, coords, name, bases, dtype): tau vector
+            dist, coords = domain, name
+            vname, vbases, vdtype = dtype, bases, dtype_
+            self.domain = None
+            self.name = vname
+            self.components = [ScalarField(dist, f"{vname}_{n}", vbases, vdtype) for n in coords.names]
+            return
         self.domain = domain
         self.name = name
         names = domain.dist.coords.names

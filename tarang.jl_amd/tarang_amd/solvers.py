@@ -83,6 +83,13 @@ class InitialValueSolver:
         self.start_wall_time = time.time()
         rec = recognize(problem)
         self.recognized = rec
+        if rec.family == "rbc2d":
+            self._channel = _ChannelBackend(self, problem, rec, timestepper, use_graph)
+            self.dev = self._channel.dev
+            self.handle = self._channel.handle
+            self.field = self._channel.b
+            return
+        self._channel = None
         self.field = problem.var(rec.evolved)
         dom = self.field.domain
         if dom is None or not _is_gpu(dom.arch):
@@ -140,7 +147,10 @@ class InitialValueSolver:
         self.dev.after()
 
     def evaluate_rhs(self, out: Optional[ScalarField] = None) -> ScalarField:
-        """F(X) for the current state, in coefficient layout (evaluate_rhs)."""
+        """F(X) for the current state, in coefficient layout (evaluate_rhs).  For the
+        channel family the result is the tuple (F_b, F_ux, F_uz)."""
+        if self._channel is not None:
+            return self._channel.rhs()
         out = out or ScalarField(self.field.domain, "F")
         self.dev.before()
         _lib.call("tarang_fivp_rhs", self.handle, _ptr(self.field._c), _ptr(out._c))
@@ -151,6 +161,11 @@ class InitialValueSolver:
     def step_many(self, n: int):
         """Advance n steps in one library call (no per-step host work)."""
         if n <= 0:
+            return
+        if self._channel is not None:
+            self._channel.step(n)
+            self.iteration += n
+            self.sim_time += n * self.dt
             return
         if self.field.current_layout != "c" or getattr(self, "_dirty", False):
             self._push_state()
@@ -170,12 +185,13 @@ class InitialValueSolver:
         return True
 
     def set_dt(self, dt: float):
-        _lib.call("tarang_fivp_set_dt", self.handle, float(dt))
+        _lib.call("tarang_chan_set_dt" if self._channel is not None else "tarang_fivp_set_dt", self.handle, float(dt))
         self.dt = float(dt)
 
     def close(self):
         if getattr(self, "handle", None):
-            _lib.call("tarang_fivp_destroy", self.handle)
+            _lib.call("tarang_chan_destroy" if getattr(self, "_channel", None) is not None else "tarang_fivp_destroy",
+                      self.handle)
             self.handle = None
 
     def __del__(self):
@@ -183,6 +199,86 @@ class InitialValueSolver:
             self.close()
         except Exception:
             pass
+
+
+class _ChannelBackend:
+    """The rbc2d family on ChannelDomain: one tarang_chan_t holds the state
+    (p, b, u, taus) and the factored per-mode systems.  Fields are pushed before
+    and pulled after every batch of steps (scatter_inputs / gather_inputs!,
+    subproblem_io.jl:1143-1152)."""
+
+    def __init__(self, solver, problem, rec, timestepper, use_graph):
+        names = rec.names
+        self.problem = problem
+        self.b = problem.var(names["b"])
+        self.p = problem.var(names["p"])
+        u = problem.var(names["u"])
+        if not isinstance(u, VectorField) or len(u.components) != 2:
+            raise ValueError("the channel family needs a 2-component VectorField u")
+        self.ux, self.uz = u.components
+        dom = self.b.domain
+        if dom is None or not _is_gpu(dom.arch):
+            raise GPUTransformUnsupported("InitialValueSolver here runs on arch=GPU() domains only")
+        if dom.ndim != 2 or dom.bases[0].kind != _lib.REALFOURIER or dom.bases[1].kind != _lib.CHEBYSHEVT:
+            raise GPUTransformUnsupported("the channel family needs a RealFourier(x) x ChebyshevT(z) domain")
+        for bb in dom.bases:
+            if bb.dealias != 1.5:
+                raise GPUTransformUnsupported("dealias must be 3/2 for the nonlinear GPU path")
+        if timestepper.code not in (0, 1):
+            raise GPUTransformUnsupported(f"{timestepper.name} is not available for the channel family "
+                                          "(RK222 and RK443 are)")
+        self.taus = [problem.var(names[k]) for k in ("tau_p", "tau_b1", "tau_b2")]
+        tu1, tu2 = problem.var(names["tau_u1"]), problem.var(names["tau_u2"])
+        self.taus += list(tu1.components) + list(tu2.components)  # slot order 0..6
+        self.dev = dom.device()
+        d = _lib.ChanDesc()
+        d.nx, d.nz = dom.bases[0].size, dom.bases[1].size
+        d.Lx, d.Lz = dom.bases[0].length, dom.bases[1].length
+        d.kappa, d.nu = rec.extra["kappa"], rec.extra["nu"]
+        d.dt = solver.dt
+        d.dtype = _lib.F32 if dom.dtype == np.float32 else _lib.F64
+        d.timestepper = timestepper.code
+        d.b_bottom, d.b_top = rec.extra["b_bottom"], rec.extra["b_top"]
+        h = ctypes.c_void_p()
+        _lib.call("tarang_chan_create", self.dev.handle, ctypes.byref(d), ctypes.byref(h))
+        self.handle = h
+        _lib.call("tarang_chan_set_graph", h, 1 if use_graph else 0)
+        self.fields = ((_lib.CHAN_P, self.p), (_lib.CHAN_B, self.b), (_lib.CHAN_UX, self.ux), (_lib.CHAN_UZ, self.uz))
+        self.push()
+
+    def push(self):
+        for _, f in self.fields:
+            ensure_layout_(f, "c")
+        self.dev.before()
+        for var, f in self.fields:
+            _lib.call("tarang_chan_set_field", self.handle, var, _ptr(f._c))
+        self.dev.after()
+
+    def pull(self):
+        self.dev.before()
+        for var, f in self.fields:
+            _lib.call("tarang_chan_get_field", self.handle, var, _ptr(f._c))
+            f.current_layout = "c"
+        for t, f in enumerate(self.taus):
+            if f._c is not None and f._c.numel() == self.b._c.shape[-1]:
+                _lib.call("tarang_chan_get_tau", self.handle, t, _ptr(f._c))
+        self.dev.after()
+
+    def step(self, n):
+        # the fields are the state: a host-side change (grid or coefficients) is always seen
+        self.push()
+        _lib.call("tarang_chan_step", self.handle, int(n))
+        self.pull()
+
+    def rhs(self):
+        self.push()
+        outs = [ScalarField(self.b.domain, n) for n in ("F_b", "F_ux", "F_uz")]
+        self.dev.before()
+        _lib.call("tarang_chan_rhs", self.handle, *[_ptr(o._c) for o in outs])
+        self.dev.after()
+        for o in outs:
+            o.current_layout = "c"
+        return tuple(outs)
 
 
 def step_(solver: InitialValueSolver, dt: Optional[float] = None) -> InitialValueSolver:

@@ -71,3 +71,71 @@ def test_canonical_forms():
 def test_timestepper_codes_match_header():
     assert (T.RK222.code, T.RK443.code, T.SBDF2.code, T.RK111.code, T.SBDF1.code) == (0, 1, 2, 3, 4)
     assert T.DiagonalIMEX_SBDF2.code == T.SBDF2.code
+
+
+def _rbc_problem(Lz=1.0, bcs=None):
+    """The first-order Rayleigh-Benard statement of test/test_subproblem_rk.jl:27-56,
+    with stand-in variables (recognition needs names and bases only)."""
+    dom = T.ChannelDomain(16, 8, Lx=4.0, Lz=Lz)
+
+    class V:
+        def __init__(self, n):
+            self.name, self.bases = n, dom.bases
+    coords = dom.dist.coords
+    ex, ez = T.unit_vector_fields(coords)
+    lift_basis = T.derivative_basis(dom.bases[1], 1)
+    tau_lift = lambda A: T.lift(A, lift_basis, -1)  # noqa: E731
+    u, b, p = V("u"), V("b"), V("p")
+    taus = [V(n) for n in ("tau_p", "tau_b1", "tau_b2", "tau_u1", "tau_u2")]
+    grad_u = T.grad(u) + ez * tau_lift(taus[3])
+    grad_b = T.grad(b) + ez * tau_lift(taus[1])
+    pb = T.IVP([p, b, u] + taus)
+    T.add_parameters_(pb, kappa=0.01, nu=0.02, Lz=Lz, ez=ez, grad_u=grad_u, grad_b=grad_b, τ_lift=tau_lift)
+    T.add_equation_(pb, "trace(grad_u) + tau_p = 0")
+    T.add_equation_(pb, "∂t(b) - kappa*div(grad_b) + τ_lift(tau_b2) = -u⋅∇(b)")
+    T.add_equation_(pb, "∂t(u) - nu*div(grad_u) + ∇(p) - b*ez + τ_lift(tau_u2) = -u⋅∇(u)")
+    for bc in bcs or ("b(z=0) = Lz", "u(z=0) = 0", "b(z=Lz) = 0", "u(z=Lz) = 0", "integ(p) = 0"):
+        T.add_bc_(pb, bc)
+    return pb
+
+
+def test_rbc_recognition():
+    r = recognize(_rbc_problem(Lz=2.0))
+    assert r.family == "rbc2d"
+    assert r.extra == {"kappa": 0.01, "nu": 0.02, "b_bottom": 2.0, "b_top": 0.0}
+    assert r.names == {"u": "u", "b": "b", "p": "p", "tau_p": "tau_p", "tau_b1": "tau_b1", "tau_b2": "tau_b2",
+                       "tau_u1": "tau_u1", "tau_u2": "tau_u2"}
+    # free-slip walls, a missing gauge and a wrong buoyancy sign are refused
+    with pytest.raises(NotImplementedError):
+        recognize(_rbc_problem(bcs=("b(z=0) = Lz", "u(z=0) = 1", "b(z=Lz) = 0", "u(z=Lz) = 0", "integ(p) = 0")))
+    with pytest.raises(NotImplementedError):
+        recognize(_rbc_problem(bcs=("b(z=0) = Lz", "u(z=0) = 0", "b(z=Lz) = 0", "u(z=Lz) = 0")))
+    pb = _rbc_problem()
+    pb.equations[2] = pb.equations[2].replace("- b*ez", "+ b*ez")
+    with pytest.raises(NotImplementedError):
+        recognize(pb)
+
+
+def test_nusselt_closed_forms():
+    """Nu = 1 for conduction; + <uz b>/(kappa dB/Lz) for a manufactured convective state."""
+    from tarang_amd import diagnostics as D
+    import numpy.polynomial.chebyshev as C
+    nx, nz, Lx, Lz, kappa = 16, 33, 4.0, 2.0, 0.01
+    x = Lx * np.arange(nx) / nx
+    z = Lz * (1 - np.cos(np.pi * np.arange(nz) / (nz - 1))) / 2
+    # Clenshaw-Curtis is exact for polynomials of degree nz-1
+    w = D.clenshaw_curtis_weights(nz)
+    t = -np.cos(np.pi * np.arange(nz) / (nz - 1))
+    assert abs(w @ t ** 10 - 2 / 11) < 1e-14 and abs(w.sum() - 2) < 1e-14
+
+    def coeff(g):  # grid (nx, nz) -> (nx/2+1, nz): rfft in x, Chebyshev interpolation in z
+        F = np.fft.rfft(g, axis=0)
+        V = C.chebvander(t, nz - 1)
+        return np.linalg.solve(V, F.T).T
+    A, Bamp, kx = 0.3, 0.7, 2 * np.pi / Lx
+    b = (Lz - z)[None, :] + A * np.sin(kx * x)[:, None] * np.sin(np.pi * z / Lz)[None, :]
+    uz = Bamp * np.sin(kx * x)[:, None] * np.sin(np.pi * z / Lz)[None, :]
+    cond = D.nusselt(coeff(np.tile(Lz - z, (nx, 1))), np.zeros((nx // 2 + 1, nz)), nx, Lz, kappa, Lz)
+    assert abs(cond - 1.0) < 1e-13
+    nu = D.nusselt(coeff(b), coeff(uz), nx, Lz, kappa, Lz)
+    assert abs(nu - (1.0 + A * Bamp / 4 / (kappa * Lz / Lz))) < 1e-9
