@@ -65,9 +65,11 @@ struct orc_rbc {
     double *D;                 /* nz x nz, row-major, includes 2/Lz */
     int stages;
     double AE[4][4], AI[4][4], bE[4], bI[4];
-    cplx *lu;                  /* cached LU of M + dt*gamma*L per mode */
+    cplx *lu;                  /* cached LU of alpha M + beta L per mode */
     int *piv;
-    double lu_coef;
+    double lu_alpha, lu_beta;
+    double dt_prev;            /* multistep history: previous dt, M P(x) and F of the last step */
+    cplx *MXm1, *Fm1;          /* [nxh][nfull] */
 };
 
 static int fidx(int var, int k, int nz) { return var * nz + k; }
@@ -223,7 +225,7 @@ orc_rbc *orc_rbc_create(int nx, int nz, double Lx, double Lz, double kappa, doub
         s->AI[1][1] = g; s->AI[2][1] = 0.2820667392457805; s->AI[2][2] = g;
         s->AI[3][1] = 1.208496649176010; s->AI[3][2] = -0.6443631706844688; s->AI[3][3] = g;
         s->bI[1] = 1.208496649176010; s->bI[2] = -0.6443631706844688; s->bI[3] = g;
-    } else { /* RK222 */
+    } else if (timestepper == ORC_TS_RK222) {
         const double g = 1.0 - 1.0 / sqrt(2.0);
         s->stages = 3;
         s->AE[1][0] = g; s->AE[2][0] = g; s->AE[2][1] = 1.0 - g;
@@ -240,6 +242,7 @@ void orc_rbc_destroy(orc_rbc *s) {
     for (int t = 0; t < NTAU; ++t) free(s->tau[t]);
     free(s->D); free(s->nm); free(s->rowmap); free(s->colmap);
     free(s->M); free(s->L); free(s->bc); free(s->algf); free(s->bcrow); free(s->lu); free(s->piv);
+    free(s->MXm1); free(s->Fm1);
     free(s);
 }
 
@@ -295,9 +298,9 @@ static void lu_solve(const cplx *A, int n, const int *piv, cplx *b) {
     }
 }
 
-static int ensure_lu(orc_rbc *s, double coef) {
+static int ensure_lu(orc_rbc *s, double alpha, double beta) {
     const int nf = s->nfull, nxh = s->nxh;
-    if (s->lu && s->lu_coef == coef) return 0;
+    if (s->lu && s->lu_alpha == alpha && s->lu_beta == beta) return 0;
     if (!s->lu) {
         s->lu = (cplx *)malloc(sizeof(cplx) * (size_t)nxh * nf * nf);
         s->piv = (int *)malloc(sizeof(int) * (size_t)nxh * nf);
@@ -308,10 +311,11 @@ static int ensure_lu(orc_rbc *s, double coef) {
         const int n = s->nm[m];
         cplx *A = s->lu + (size_t)m * nf * nf;
         const cplx *Mm = s->M + (size_t)m * nf * nf, *Lm = s->L + (size_t)m * nf * nf;
-        for (size_t i = 0; i < (size_t)n * n; ++i) A[i] = Mm[i] + coef * Lm[i];
+        for (size_t i = 0; i < (size_t)n * n; ++i) A[i] = alpha * Mm[i] + beta * Lm[i];
         if (lu_factor(A, n, s->piv + (size_t)m * nf)) bad |= 1;
     }
-    s->lu_coef = coef;
+    s->lu_alpha = alpha;
+    s->lu_beta = beta;
     return bad ? -1 : 0;
 }
 
@@ -435,7 +439,72 @@ static void gather_F(const orc_rbc *s, int m, const cplx *Fb, const cplx *Fux, c
     }
 }
 
+/* Per-mode IMEX multistep step (step_subproblem_multistep.jl:197-420):
+ *   MX_n = M P(x_n), F_n = F(P(x_n))
+ *   rhs = sum_{k>=1} (c_k F_{n+1-k} - a_k MX_{n+1-k});  rhs[bc] = b_0 ALG
+ *   (a_0 M + b_0 L) x_{n+1} = rhs
+ * SBDF1 on the first step (and always for ORC_TS_SBDF1), then variable-step SBDF2
+ * with w = dt/dt_prev (_sbdf2_coefs, :544-552). */
+static int rbc_step_sbdf(orc_rbc *s) {
+    const int nf = s->nfull, nxh = s->nxh;
+    double a[3] = {0, 0, 0}, c[3] = {0, 0, 0};
+    const double dt = s->dt, b0 = 1.0;
+    if (s->iter == 0 || s->ts == ORC_TS_SBDF1) {
+        a[0] = 1.0 / dt; a[1] = -1.0 / dt; c[1] = 1.0;
+    } else {
+        const double w = dt / s->dt_prev;
+        a[0] = (1.0 + 2.0 * w) / ((1.0 + w) * dt);
+        a[1] = -(1.0 + w) / dt;
+        a[2] = w * w / ((1.0 + w) * dt);
+        c[1] = 1.0 + w;
+        c[2] = -w;
+    }
+    if (ensure_lu(s, a[0], b0) != 0) return -1;
+    if (!s->MXm1) {
+        s->MXm1 = (cplx *)calloc((size_t)nxh * nf, sizeof(cplx));
+        s->Fm1 = (cplx *)calloc((size_t)nxh * nf, sizeof(cplx));
+    }
+    const size_t nc = (size_t)nxh * s->nz;
+    cplx *Fb = (cplx *)malloc(sizeof(cplx) * nc), *Fux = (cplx *)malloc(sizeof(cplx) * nc),
+         *Fuz = (cplx *)malloc(sizeof(cplx) * nc);
+    cplx *x = (cplx *)malloc(sizeof(cplx) * nf), *rhs = (cplx *)malloc(sizeof(cplx) * nf);
+    cplx *mx = (cplx *)malloc(sizeof(cplx) * nf), *f = (cplx *)malloc(sizeof(cplx) * nf);
+    project_state(s); /* the state goes to the grid for the RHS and is gathered back */
+    orc_rbc_rhs_fields(s, (double *)Fb, (double *)Fux, (double *)Fuz);
+    for (int m = 0; m < nxh; ++m) {
+        const int n = s->nm[m];
+        cplx *MXo = s->MXm1 + (size_t)m * nf, *Fo = s->Fm1 + (size_t)m * nf;
+        gather(s, m, x);
+        matvec(s->M + (size_t)m * nf * nf, n, x, mx);
+        gather_F(s, m, Fb, Fux, Fuz, f);
+        for (int r = 0; r < n; ++r) {
+            cplx v = 0;
+            if (fabs(c[1]) > 1e-14) v += c[1] * f[r];
+            if (fabs(c[2]) > 1e-14) v += c[2] * Fo[r];
+            if (fabs(a[1]) > 1e-14) v += -a[1] * mx[r];
+            if (fabs(a[2]) > 1e-14) v += -a[2] * MXo[r];
+            rhs[r] = s->bc[(size_t)m * nf + r] ? b0 * s->algf[(size_t)m * nf + r] : v;
+        }
+        memcpy(MXo, mx, sizeof(cplx) * n);
+        memcpy(Fo, f, sizeof(cplx) * n);
+        lu_solve(s->lu + (size_t)m * nf * nf, n, s->piv + (size_t)m * nf, rhs);
+        scatter(s, m, rhs);
+    }
+    free(Fb); free(Fux); free(Fuz); free(x); free(rhs); free(mx); free(f);
+    s->dt_prev = dt;
+    s->t += dt;
+    s->iter++;
+    return 0;
+}
+
+void orc_rbc_set_dt(orc_rbc *s, double dt) { s->dt = dt; }
+
 int orc_rbc_step(orc_rbc *s, int nsteps) {
+    if (s->ts == ORC_TS_SBDF2 || s->ts == ORC_TS_SBDF1) {
+        for (int i = 0; i < nsteps; ++i)
+            if (rbc_step_sbdf(s) != 0) return -1;
+        return 0;
+    }
     const int nf = s->nfull, nxh = s->nxh, N = s->nz, S = s->stages;
     const size_t nc = (size_t)nxh * N;
     cplx *MX0 = (cplx *)calloc((size_t)nxh * nf, sizeof(cplx));
@@ -455,7 +524,7 @@ int orc_rbc_step(orc_rbc *s, int nsteps) {
         }
         for (int i = 0; i < S; ++i) {
             const double aii = s->AI[i][i];
-            if (fabs(aii) > 1e-14 && ensure_lu(s, s->dt * aii) != 0) { rc = -1; break; }
+            if (fabs(aii) > 1e-14 && ensure_lu(s, 1.0, s->dt * aii) != 0) { rc = -1; break; }
             for (int m = 0; m < nxh; ++m) {
                 const int n = s->nm[m];
                 for (int r = 0; r < n; ++r) {
@@ -509,6 +578,8 @@ int orc_rbc_step(orc_rbc *s, int nsteps) {
 
 void orc_rbc_set_field(orc_rbc *s, int var, const double *coeff) {
     memcpy(s->f[var], coeff, sizeof(cplx) * (size_t)s->nxh * s->nz);
+    s->iter = 0; /* a new initial condition restarts the multistep history */
+    s->t = 0;
 }
 void orc_rbc_get_field(const orc_rbc *s, int var, double *coeff) {
     memcpy(coeff, s->f[var], sizeof(cplx) * (size_t)s->nxh * s->nz);

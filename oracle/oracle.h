@@ -85,7 +85,8 @@ double orc_fivp_time(const orc_fivp *p);
 void orc_fivp_diagnostics(orc_fivp *p, double *E, double *Z);
 
 /* ---- 2-D Rayleigh-Benard channel (RealFourier x ChebyshevT) with the Chebyshev-tau
- *      method: per-mode dense systems, RK222/RK443 (step_subproblem_rk.jl:393-739) ---- */
+ *      method: per-mode dense systems, RK222/RK443 (step_subproblem_rk.jl:393-739),
+ *      SBDF2 (step_subproblem_multistep.jl:197-420) ---- */
 enum { ORC_RBC_P = 0, ORC_RBC_B = 1, ORC_RBC_UX = 2, ORC_RBC_UZ = 3 };
 typedef struct orc_rbc orc_rbc;
 orc_rbc *orc_rbc_create(int nx, int nz, double Lx, double Lz, double kappa, double nu, double dt, int timestepper);
@@ -98,7 +99,9 @@ void orc_rbc_set_field(orc_rbc *s, int var, const double *coeff);
 void orc_rbc_get_field(const orc_rbc *s, int var, double *coeff);
 /* tau values per mode: 0 tau_p, 1 tau_b1, 2 tau_b2, 3 tau_u1.x, 4 tau_u1.z, 5 tau_u2.x, 6 tau_u2.z */
 void orc_rbc_get_tau(const orc_rbc *s, int t, double *vals);
+/* RK222 / RK443 (IMEX RK stage loop) or SBDF2 / SBDF1 (per-mode multistep) */
 int orc_rbc_step(orc_rbc *s, int nsteps);
+void orc_rbc_set_dt(orc_rbc *s, double dt);
 double orc_rbc_time(const orc_rbc *s);
 void orc_rbc_rhs_fields(orc_rbc *s, double *Fb, double *Fux, double *Fuz);
 void orc_rbc_matrices(const orc_rbc *s, int mode, double *M, double *L);

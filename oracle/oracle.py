@@ -79,6 +79,7 @@ def lib():
         L.orc_rbc_get_field.argtypes = [vp, ctypes.c_int, dp]
         L.orc_rbc_get_tau.argtypes = [vp, ctypes.c_int, dp]
         L.orc_rbc_step.argtypes = [vp, ctypes.c_int]
+        L.orc_rbc_set_dt.argtypes = [vp, ctypes.c_double]
         L.orc_rbc_time.argtypes = [vp]
         L.orc_rbc_time.restype = ctypes.c_double
         L.orc_rbc_rhs_fields.argtypes = [vp, dp, dp, dp]
@@ -268,6 +269,9 @@ class RBC2D:
         rc = lib().orc_rbc_step(self.h, int(n))
         if rc != 0:
             raise RuntimeError("singular per-mode system")
+
+    def set_dt(self, dt):
+        lib().orc_rbc_set_dt(self.h, float(dt))
 
     @property
     def time(self):

@@ -91,13 +91,15 @@ def test_rbc_critical_rayleigh(oracle):
     assert abs(at.imag) < 1e-10   # exchange of stabilities: real onset
 
 
-def test_rbc_eigenvector_growth(oracle):
-    """The stepped solution from the leading eigenvector grows as exp(sigma T), O(dt^2)."""
+@pytest.mark.parametrize("ts,order", [("RK222", 2), ("RK443", 3), ("SBDF2", 2), ("SBDF1", 1)])
+def test_rbc_eigenvector_growth(oracle, ts, order):
+    """The stepped solution from the leading eigenvector grows as exp(sigma T) with the
+    scheme's order (SBDF2 starts with one SBDF1 step, step_subproblem_multistep.jl)."""
     N, nx, Ra, k = 16, 8, 3000.0, np.pi
     errs = []
-    for dt in (2e-2, 1e-2):
+    for dt in (4e-2, 2e-2):
         sig, v, _ = growth_rate(oracle, Ra, k=k, N=N)
-        s = make(oracle, Ra, nx, N, 2 * np.pi / k, dt=dt)
+        s = make(oracle, Ra, nx, N, 2 * np.pi / k, dt=dt, ts=ts)
         z = zgrid(N, 1.0)
         b = oracle.forward_transform(s.d, np.tile(1.0 - z, (nx, 1)))
         amp = 1e-7 / np.abs(v[N:4 * N]).max()
@@ -113,10 +115,12 @@ def test_rbc_eigenvector_growth(oracle):
         x1 = np.concatenate([s.get(oracle.RBC_B)[1], s.get(oracle.RBC_UX)[1], s.get(oracle.RBC_UZ)[1]])
         r = np.vdot(x0, x1) / np.vdot(x0, x0)
         errs.append(abs(r - np.exp(sig.real * T)) / np.exp(sig.real * T))
-        assert np.abs(x1 - r * x0).max() < 1e-5 * np.abs(x1).max()
+        # off-eigenvector part: time-discretisation error (and parasitic multistep modes)
+        assert np.abs(x1 - r * x0).max() < {1: 5e-3, 2: 1e-4, 3: 1e-5}[order] * np.abs(x1).max()
     assert sig.real > 0.1
-    assert errs[0] < 5e-6 and errs[1] < 1.5e-6
-    assert 3.0 < errs[0] / errs[1] < 5.0   # second order
+    assert errs[0] < {1: 5e-3, 2: 5e-4, 3: 1e-5}[order]
+    ratio = errs[0] / errs[1]
+    assert 0.75 * 2 ** order < ratio < 1.3 * 2 ** order, ratio
 
 
 def test_rbc_conduction_is_steady(oracle):
@@ -159,3 +163,25 @@ def test_rbc_bc_values(oracle):
     s.step(3)
     b = oracle.backward_transform(s.d, s.get(oracle.RBC_B))
     assert np.abs(b - (2.0 - 1.5 * z)[None, :]).max() < 1e-13   # the linear profile is steady
+
+
+def test_rbc_sbdf2_variable_step_order(oracle):
+    """Alternating steps h, h/2 (w = 1/2, 2) keep SBDF2 second order: the w-dependent
+    coefficients of _sbdf2_coefs (step_subproblem_multistep.jl:544-552)."""
+    nx, N, eps, kappa = 8, 24, 1e-3, 1e-2
+    errs = []
+    for h in (4e-2, 2e-2):
+        s = make(oracle, 1e4, nx, N, 4.0, dt=h, ts="SBDF2")
+        z = zgrid(N, 1.0)
+        s.set(oracle.RBC_B, oracle.forward_transform(s.d, np.tile(1 - z + eps * np.sin(np.pi * z), (nx, 1))))
+        t = 0.0
+        for i in range(int(round(1.0 / (1.5 * h))) * 2):
+            dt = h if i % 2 == 0 else h / 2
+            s.set_dt(dt)
+            s.step(1)
+            t += dt
+        assert abs(s.time - t) < 1e-12
+        b = oracle.backward_transform(s.d, s.get(oracle.RBC_B))
+        exact = 1 - z + eps * np.exp(-kappa * np.pi ** 2 * t) * np.sin(np.pi * z)
+        errs.append(np.abs(b - exact[None, :]).max() / eps)
+    assert errs[0] < 2e-5 and 3.3 < errs[0] / errs[1] < 4.7, errs
