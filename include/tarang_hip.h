@@ -151,7 +151,9 @@ int tarang_fivp_kernel_bytes(tarang_fivp_t s, double* ybwd, double* xpass, doubl
  * per-mode IMEX RK loop (src/core/timesteppers/step_subproblem_rk.jl:393-739,
  * step_subproblem_rk_batched.jl:838-1005).  The per-mode systems are solved in
  * FP64 whatever dtype is; dtype sets the state, the RHS and the stage arithmetic.
- * timestepper: 0 RK222, 1 RK443.  nx: power of 2 in 8..2048 (FP64: ..1024); nz: 4..512. */
+ * timestepper: 0 RK222, 1 RK443 (IMEX RK stage loop), 2 SBDF2, 4 SBDF1 (per-mode multistep,
+ * step_subproblem_multistep.jl:197-420; SBDF2 starts with one SBDF1 step after create/set_field).
+ * nx: power of 2 in 8..2048 (FP64: ..1024); nz: 4..512. */
 typedef struct {
     int nx, nz;
     double Lx, Lz;

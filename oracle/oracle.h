@@ -106,6 +106,27 @@ double orc_rbc_time(const orc_rbc *s);
 void orc_rbc_rhs_fields(orc_rbc *s, double *Fb, double *Fux, double *Fuz);
 void orc_rbc_matrices(const orc_rbc *s, int mode, double *M, double *L);
 
+/* ---- 3-D Rayleigh-Benard (RealFourier x RealFourier x ChebyshevT), same tau form and
+ *      steppers; per (kx, ky) mode systems (channel3d_ivp.c) ---- */
+enum { ORC_RBC3_P = 0, ORC_RBC3_B = 1, ORC_RBC3_UX = 2, ORC_RBC3_UY = 3, ORC_RBC3_UZ = 4 };
+typedef struct orc_rbc3 orc_rbc3;
+orc_rbc3 *orc_rbc3_create(int nx, int ny, int nz, double Lx, double Ly, double Lz, double kappa, double nu, double dt,
+                          int timestepper);
+void orc_rbc3_destroy(orc_rbc3 *s);
+/* mode = m + (nx/2+1) n, m the kx index, n the ky index in FFT order */
+int orc_rbc3_size(const orc_rbc3 *s, int mode);
+void orc_rbc3_set_bc(orc_rbc3 *s, double b_bottom, double b_top);
+/* coefficient arrays (nx/2+1, ny, nz) complex, reference layout */
+void orc_rbc3_set_field(orc_rbc3 *s, int var, const double *coeff);
+void orc_rbc3_get_field(const orc_rbc3 *s, int var, double *coeff);
+/* taus: 0 tau_p, 1 tau_b1, 2 tau_b2, 3-5 tau_u1.{x,y,z}, 6-8 tau_u2.{x,y,z}; (nx/2+1, ny) values */
+void orc_rbc3_get_tau(const orc_rbc3 *s, int t, double *vals);
+int orc_rbc3_step(orc_rbc3 *s, int nsteps);
+void orc_rbc3_set_dt(orc_rbc3 *s, double dt);
+double orc_rbc3_time(const orc_rbc3 *s);
+void orc_rbc3_rhs_fields(orc_rbc3 *s, double *Fb, double *Fux, double *Fuy, double *Fuz);
+void orc_rbc3_matrices(const orc_rbc3 *s, int mode, double *M, double *L);
+
 #ifdef __cplusplus
 }
 #endif

@@ -84,6 +84,21 @@ def lib():
         L.orc_rbc_time.restype = ctypes.c_double
         L.orc_rbc_rhs_fields.argtypes = [vp, dp, dp, dp]
         L.orc_rbc_matrices.argtypes = [vp, ctypes.c_int, dp, dp]
+        L.orc_rbc3_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+        L.orc_rbc3_create.restype = vp
+        L.orc_rbc3_destroy.argtypes = [vp]
+        L.orc_rbc3_size.argtypes = [vp, ctypes.c_int]
+        L.orc_rbc3_set_bc.argtypes = [vp, ctypes.c_double, ctypes.c_double]
+        L.orc_rbc3_set_field.argtypes = [vp, ctypes.c_int, dp]
+        L.orc_rbc3_get_field.argtypes = [vp, ctypes.c_int, dp]
+        L.orc_rbc3_get_tau.argtypes = [vp, ctypes.c_int, dp]
+        L.orc_rbc3_step.argtypes = [vp, ctypes.c_int]
+        L.orc_rbc3_set_dt.argtypes = [vp, ctypes.c_double]
+        L.orc_rbc3_time.argtypes = [vp]
+        L.orc_rbc3_time.restype = ctypes.c_double
+        L.orc_rbc3_rhs_fields.argtypes = [vp, dp, dp, dp, dp]
+        L.orc_rbc3_matrices.argtypes = [vp, ctypes.c_int, dp, dp]
         _lib = L
     return _lib
 
@@ -287,4 +302,74 @@ class RBC2D:
         M = np.zeros((n, n), np.complex128, order="F")
         L = np.zeros((n, n), np.complex128, order="F")
         lib().orc_rbc_matrices(self.h, mode, _dp(M), _dp(L))
+        return M, L
+
+
+RBC3_P, RBC3_B, RBC3_UX, RBC3_UY, RBC3_UZ = range(5)
+
+
+class RBC3D:
+    """3-D Rayleigh-Benard channel, Chebyshev-tau per-(kx, ky) solves (oracle)."""
+
+    def __init__(self, nx, ny, nz, Lx, Ly, Lz, kappa, nu, dt, timestepper="SBDF2"):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.shape = (nx // 2 + 1, ny, nz)
+        self.h = lib().orc_rbc3_create(nx, ny, nz, Lx, Ly, Lz, kappa, nu, dt, TS[timestepper])
+        if not self.h:
+            raise ValueError("per-mode system is not square after filtering")
+        self.d = domain([REALFOURIER, REALFOURIER, CHEBYSHEVT], [nx, ny, nz],
+                        bounds=[(0.0, Lx), (0.0, Ly), (0.0, Lz)])
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None) and _lib is not None:
+                _lib.orc_rbc3_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def mode(self, m, n):
+        return m + (self.nx // 2 + 1) * n
+
+    def n(self, mode):
+        return lib().orc_rbc3_size(self.h, int(mode))
+
+    def set_bc(self, b_bottom, b_top):
+        lib().orc_rbc3_set_bc(self.h, float(b_bottom), float(b_top))
+
+    def set(self, var, coeff):
+        c = _fortran(coeff, np.complex128)
+        lib().orc_rbc3_set_field(self.h, var, _dp(c))
+
+    def get(self, var):
+        out = np.zeros(self.shape, np.complex128, order="F")
+        lib().orc_rbc3_get_field(self.h, var, _dp(out))
+        return out
+
+    def tau(self, t):
+        out = np.zeros(self.shape[:2], np.complex128, order="F")
+        lib().orc_rbc3_get_tau(self.h, t, _dp(out))
+        return out
+
+    def step(self, n=1):
+        if lib().orc_rbc3_step(self.h, int(n)) != 0:
+            raise RuntimeError("singular per-mode system")
+
+    def set_dt(self, dt):
+        lib().orc_rbc3_set_dt(self.h, float(dt))
+
+    @property
+    def time(self):
+        return lib().orc_rbc3_time(self.h)
+
+    def rhs(self):
+        outs = [np.zeros(self.shape, np.complex128, order="F") for _ in range(4)]
+        lib().orc_rbc3_rhs_fields(self.h, *[_dp(o) for o in outs])
+        return outs
+
+    def matrices(self, mode):
+        n = self.n(mode)
+        M = np.zeros((n, n), np.complex128, order="F")
+        L = np.zeros((n, n), np.complex128, order="F")
+        lib().orc_rbc3_matrices(self.h, mode, _dp(M), _dp(L))
         return M, L

@@ -352,7 +352,13 @@ int tarang_chan_create(tarang_ctx_t ctx, const tarang_chan_desc* desc, tarang_ch
         d.dtype = desc->dtype;
         d.b_bottom = desc->b_bottom;
         d.b_top = desc->b_top;
-        d.stepper = desc->timestepper == 1 ? tarang::kRK443 : (desc->timestepper == 0 ? tarang::kRK222 : -1);
+        switch (desc->timestepper) {
+            case 0: d.stepper = tarang::kRK222; break;
+            case 1: d.stepper = tarang::kRK443; break;
+            case 2: d.stepper = tarang::kSBDF2; break;
+            case 4: d.stepper = tarang::kSBDF1; break;
+            default: d.stepper = -1; break;
+        }
         auto h = std::make_unique<tarang_chan_s>();
         h->solver = std::make_unique<tarang::ChannelSolver>(&ctx->ctx, d);
         *out = h.release();

@@ -264,6 +264,42 @@ __global__ void k_ch_stage(ChStageArgs<T> a, size_t n) {
     }
 }
 
+// Per-mode multistep right-hand side (step_subproblem_multistep.jl:197-420), scaled by
+// 1/a0 so that the solve uses M + (1/a0) L:
+//   R = (c1 F_n + c2 F_{n-1} - a1 M P(x_n) - a2 M P(x_{n-1})) / a0,   E1 rows 0
+// M P(x) is the evolved part (b, ux, uz) of the grid-round-tripped state.  The
+// histories F_{n-1}, M P(x_{n-1}) are replaced by the current values in the same pass.
+template <typename T>
+struct ChSbdfArgs {
+    const cplx<T>* X[3];   // state b, ux, uz
+    const cplx<T>* F[3];   // F_n rows E2, E3x, E3z
+    cplx<T>* Fm1[3];
+    cplx<T>* MXm1[3];
+    cplx<T>* R[4];         // rhs rows E1, E2, E3x, E3z
+    T c1, c2, a1, a2, inv_a0;
+    int N, nx;
+};
+
+template <typename T>
+__global__ void k_ch_sbdf(ChSbdfArgs<T> a, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int m = (int)(i / a.N);
+        const bool proj = m == 0 || ((a.nx % 2) == 0 && m == a.nx / 2);
+        a.R[0][i] = cplx<T>{T(0), T(0)};
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            cplx<T> x = a.X[r][i];
+            if (proj) x.y = T(0);
+            const cplx<T> f = a.F[r][i], fo = a.Fm1[r][i], mo = a.MXm1[r][i];
+            const T vx = a.c1 * f.x + a.c2 * fo.x - a.a1 * x.x - a.a2 * mo.x;
+            const T vy = a.c1 * f.y + a.c2 * fo.y - a.a1 * x.y - a.a2 * mo.y;
+            a.R[r + 1][i] = cplx<T>{vx * a.inv_a0, vy * a.inv_a0};
+            a.Fm1[r][i] = f;
+            a.MXm1[r][i] = x;
+        }
+    }
+}
+
 template <typename T>
 __global__ void k_ch_copy(const cplx<T>* __restrict__ s0, const cplx<T>* __restrict__ s1,
                           const cplx<T>* __restrict__ s2, cplx<T>* d0, cplx<T>* d1, cplx<T>* d2, size_t n) {

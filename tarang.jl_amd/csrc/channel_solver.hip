@@ -68,8 +68,8 @@ bool channel_nx_ok(int nx, bool f64) {
 
 ChannelSolver::ChannelSolver(Context* ctx, const ChannelDesc& d) : ctx_(ctx), d_(d) {
     TG_REQUIRE(d.dtype == kF32 || d.dtype == kF64, kErrInvalid, "channel solver: dtype must be f32 or f64");
-    TG_REQUIRE(d.stepper == kRK222 || d.stepper == kRK443, kErrInvalid,
-               "channel solver: timestepper must be RK222 or RK443 (explicit first stage)");
+    TG_REQUIRE(d.stepper == kRK222 || d.stepper == kRK443 || d.stepper == kSBDF2 || d.stepper == kSBDF1,
+               kErrInvalid, "channel solver: timestepper must be RK222, RK443, SBDF2 or SBDF1");
     TG_REQUIRE(channel_nx_ok(d.nx, d.dtype == kF64), kErrInvalid,
                "channel solver: RealFourier size " + std::to_string(d.nx) +
                    " unsupported (powers of 2, 8..2048; FP64 up to 1024)");
@@ -81,7 +81,8 @@ ChannelSolver::ChannelSolver(Context* ctx, const ChannelDesc& d) : ctx_(ctx), d_
     nb_max_ = chb_size(d.nz, 0);
     nu_max_ = chu_size(d.nz, 0, 0);
     tab_ = make_tableau(d.stepper);
-    gamma_ = tab_.AI[1][1];
+    gamma_ = multistep() ? 0.0 : tab_.AI[1][1];
+    dt_prev_ = d.dt;
     ctx_->activate();
     const size_t fb = field_bytes(), vb = 2 * dtype_size((DType)d.dtype);
     hipStream_t s = ctx_->stream;
@@ -95,9 +96,17 @@ ChannelSolver::ChannelSolver(Context* ctx, const ChannelDesc& d) : ctx_(ctx), d_
     zeroed(UZ_, fb);
     zeroed(TAU_, (size_t)nxh_ * 8 * vb);
     for (int i = 0; i < 3; ++i) zeroed(X0_[i], fb);
-    for (int st = 0; st < tab_.stages; ++st) {
-        for (int r = 0; r < 3; ++r) zeroed(F_[st][r], fb);
-        for (int r = 0; r < 4; ++r) zeroed(LX_[st][r], fb);
+    if (multistep()) {
+        for (int r = 0; r < 3; ++r) {
+            zeroed(F_[0][r], fb);
+            zeroed(Fm1_[r], fb);
+            zeroed(MXm1_[r], fb);
+        }
+    } else {
+        for (int st = 0; st < tab_.stages; ++st) {
+            for (int r = 0; r < 3; ++r) zeroed(F_[st][r], fb);
+            for (int r = 0; r < 4; ++r) zeroed(LX_[st][r], fb);
+        }
     }
     for (int r = 0; r < 4; ++r) zeroed(R_[r], fb);
     zeroed(G_, 8 * fb);
@@ -127,7 +136,14 @@ ChannelSolver::ChannelSolver(Context* ctx, const ChannelDesc& d) : ctx_(ctx), d_
     info_ = DevBuf(ctx_, (size_t)4 * nxh_ * sizeof(int));
     ctx_->twiddle(mx_, (DType)d.dtype);
     TG_HIP(hipStreamSynchronize(s));
-    factor();
+    factor(implicit_c(first_multistep()));
+}
+
+double ChannelSolver::implicit_c(bool first) const {
+    if (!multistep()) return d_.dt * gamma_;
+    if (first) return d_.dt;  // SBDF1: a0 = 1/dt
+    const double w = d_.dt / dt_prev_;
+    return (1.0 + w) * d_.dt / (1.0 + 2.0 * w);  // 1/a0 of SBDF2 (_sbdf2_coefs)
 }
 
 ChannelSolver::~ChannelSolver() {
@@ -170,7 +186,7 @@ void ChannelSolver::set_field(int var, const void* dev) {
     void* dst = field_ptr(var);
     if (d_.dtype == kF32) to_modes_t<float>(dev, dst); else to_modes_t<double>(dev, dst);
     t_ = 0;
-    iter_ = 0;
+    iter_ = 0;  // a new initial condition restarts the multistep history
 }
 
 void ChannelSolver::get_field(int var, void* dev) {
@@ -189,12 +205,11 @@ void ChannelSolver::get_tau(int t, void* dev) {
 
 void ChannelSolver::set_dt(double dt) {
     TG_REQUIRE(dt > 0, kErrInvalid, "channel solver: dt must be positive");
-    d_.dt = dt;
+    d_.dt = dt;  // dt_prev_ keeps the last step's dt (SBDF2 ratio w = dt / dt_prev)
 }
 
 // ---- implicit systems ----
-void ChannelSolver::factor() {
-    const double c = d_.dt * gamma_;
+void ChannelSolver::factor(double c) {
     if (c == factored_c_) return;
     const int N = d_.nz;
     hipStream_t s = ctx_->stream;
@@ -322,7 +337,7 @@ void ChannelSolver::solve_t() {
     const int N = d_.nz;
     ChSolveArgs sa{Ab_.as<const double>(), permb_.as<const int>(), Aup_.as<const double>(), permu_.as<const int>(),
                    N, nxh_, (size_t)nb_max_ * nb_max_, (size_t)nu_max_ * nu_max_, (size_t)nb_max_, (size_t)nu_max_,
-                   nu_max_, d_.dt * gamma_, (double)d_.nx * d_.b_bottom, (double)d_.nx * d_.b_top};
+                   nu_max_, factored_c_, (double)d_.nx * d_.b_bottom, (double)d_.nx * d_.b_top};
     constexpr int NT = 256;
     const size_t lds = ch_solve_lds_doubles(nu_max_, N) * sizeof(double);
     timed(4, [&] {
@@ -330,6 +345,40 @@ void ChannelSolver::solve_t() {
                            R_[1].as<const V>(), R_[2].as<const V>(), R_[3].as<const V>(), P_.as<V>(), B_.as<V>(),
                            UX_.as<V>(), UZ_.as<V>(), TAU_.as<V>());
     });
+    TG_HIP(hipGetLastError());
+}
+
+template <typename T>
+void ChannelSolver::sbdf_t(bool first) {
+    using V = cplx<T>;
+    ChSbdfArgs<T> a;
+    const double dt = d_.dt;
+    double a0, a1, a2, c1, c2;
+    if (first) {
+        a0 = 1.0 / dt; a1 = -1.0 / dt; a2 = 0.0; c1 = 1.0; c2 = 0.0;
+    } else {
+        const double w = dt / dt_prev_;
+        a0 = (1.0 + 2.0 * w) / ((1.0 + w) * dt);
+        a1 = -(1.0 + w) / dt;
+        a2 = w * w / ((1.0 + w) * dt);
+        c1 = 1.0 + w;
+        c2 = -w;
+    }
+    a.X[0] = B_.as<const V>();
+    a.X[1] = UX_.as<const V>();
+    a.X[2] = UZ_.as<const V>();
+    for (int r = 0; r < 3; ++r) {
+        a.F[r] = F_[0][r].as<const V>();
+        a.Fm1[r] = Fm1_[r].as<V>();
+        a.MXm1[r] = MXm1_[r].as<V>();
+    }
+    for (int r = 0; r < 4; ++r) a.R[r] = R_[r].as<V>();
+    a.c1 = T(c1); a.c2 = T(c2); a.a1 = T(a1); a.a2 = T(a2); a.inv_a0 = T(1.0 / a0);
+    a.N = d_.nz;
+    a.nx = d_.nx;
+    const size_t n = field_elems();
+    const unsigned nblk = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+    timed(5, [&] { hipLaunchKernelGGL((k_ch_sbdf<T>), dim3(nblk), dim3(256), 0, ctx_->stream, a, n); });
     TG_HIP(hipGetLastError());
 }
 
@@ -345,6 +394,20 @@ void ChannelSolver::zero_t(void* p, size_t n) {
 void ChannelSolver::launch_step_body() {
     const bool f32 = d_.dtype == kF32;
     const size_t n = field_elems();
+    if (multistep()) {
+        // F_n and M P(x_n) of the current state, rhs, one solve; p and the taus come from the solve
+        const bool first = first_multistep();
+        if (f32) {
+            rhs_t<float>(B_.ptr, UX_.ptr, UZ_.ptr, F_[0][0].ptr, F_[0][1].ptr, F_[0][2].ptr);
+            sbdf_t<float>(first);
+            solve_t<float>();
+        } else {
+            rhs_t<double>(B_.ptr, UX_.ptr, UZ_.ptr, F_[0][0].ptr, F_[0][1].ptr, F_[0][2].ptr);
+            sbdf_t<double>(first);
+            solve_t<double>();
+        }
+        return;
+    }
     auto run = [&](auto tag) {
         using T = decltype(tag);
         using V = cplx<T>;
@@ -374,11 +437,13 @@ void ChannelSolver::launch_step_body() {
 }
 
 void ChannelSolver::step_once(bool allow_graph) {
-    factor();  // no-op unless dt changed
+    const bool first = first_multistep();
+    factor(implicit_c(first));  // no-op unless the implicit coefficient changed
     if (!(allow_graph && graph_on_)) {
         launch_step_body();
     } else {
-        if (graph_ && graph_dt_ != d_.dt) invalidate_graph();
+        if (graph_ && (graph_dt_ != d_.dt || graph_first_ != first || (multistep() && graph_dtp_ != dt_prev_)))
+            invalidate_graph();
         if (!graph_) {
             hipGraph_t g = nullptr;
             TG_HIP(hipStreamBeginCapture(ctx_->stream, hipStreamCaptureModeThreadLocal));
@@ -393,9 +458,12 @@ void ChannelSolver::step_once(bool allow_graph) {
             TG_HIP(hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0));
             TG_HIP(hipGraphDestroy(g));
             graph_dt_ = d_.dt;
+            graph_dtp_ = dt_prev_;
+            graph_first_ = first;
         }
         TG_HIP(hipGraphLaunch(graph_, ctx_->stream));
     }
+    dt_prev_ = d_.dt;
     t_ += d_.dt;
     ++iter_;
 }
@@ -411,7 +479,7 @@ void ChannelSolver::rhs(void* Fb, void* Fux, void* Fuz) {
     const bool f32 = d_.dtype == kF32;
     void* outs[3] = {Fb, Fux, Fuz};
     // mode-major products into the last stage's F slots, then back to the reference layout
-    const int st = tab_.stages - 1;
+    const int st = multistep() ? 0 : tab_.stages - 1;
     if (f32) rhs_t<float>(B_.ptr, UX_.ptr, UZ_.ptr, F_[st][0].ptr, F_[st][1].ptr, F_[st][2].ptr);
     else rhs_t<double>(B_.ptr, UX_.ptr, UZ_.ptr, F_[st][0].ptr, F_[st][1].ptr, F_[st][2].ptr);
     for (int r = 0; r < 3; ++r) {
@@ -437,7 +505,6 @@ void ChannelSolver::timed(int kind, F&& launch) {
 
 void ChannelSolver::profile(int nsteps, double* avg_ms, long* counts) {
     ctx_->activate();
-    factor();
     marks_.clear();
     profiling_ = true;
     try {

@@ -16,7 +16,7 @@ struct ChannelDesc {
     double kappa, nu;    // diffusivities of b and u
     double dt;
     int dtype;           // kF32 / kF64 (state and RHS; the implicit solves are FP64)
-    int stepper;         // kRK222 / kRK443
+    int stepper;         // kRK222 / kRK443 / kSBDF2 / kSBDF1
     double b_bottom, b_top;  // Dirichlet values of b at z = 0 and z = Lz
 };
 
@@ -50,10 +50,15 @@ class ChannelSolver {
     template <typename T> void lx_t(int s);
     template <typename T> void stage_rhs_t(int s, bool final_update);
     template <typename T> void solve_t();
+    template <typename T> void sbdf_t(bool first);
+    // c of the systems M + c L the next step solves (RK: dt a_ii; SBDF: 1/a0)
+    double implicit_c(bool first) const;
+    bool multistep() const { return d_.stepper == kSBDF2 || d_.stepper == kSBDF1; }
+    bool first_multistep() const { return multistep() && (iter_ == 0 || d_.stepper == kSBDF1); }
     template <typename T> void zero_t(void* p, size_t n);
     template <typename T> void to_modes_t(const void* in, void* out);
     template <typename T> void from_modes_t(const void* in, void* out);
-    void factor();
+    void factor(double c);
     void launch_step_body();
     void step_once(bool allow_graph);
     template <typename F> void timed(int kind, F&& launch);
@@ -73,12 +78,16 @@ class ChannelSolver {
     DevBuf P_, B_, UX_, UZ_, TAU_;
     DevBuf X0_[3];
     DevBuf F_[4][3], LX_[4][4], R_[4];
+    DevBuf Fm1_[3], MXm1_[3];  // multistep history
+    double dt_prev_ = 0;
+    double solve_c_ = 0;       // c of the factored systems used by the solve kernel
     DevBuf G_, H_;
     DevBuf BT_, BDT_, FT_;
     DevBuf Ab_, Aup_, permb_, permu_, info_;
     bool graph_on_ = true;
     hipGraphExec_t graph_ = nullptr;
-    double graph_dt_ = 0;
+    double graph_dt_ = 0, graph_dtp_ = 0;
+    bool graph_first_ = false;
     bool profiling_ = false;
     struct Mark { int kind; hipEvent_t a, b; };
     std::vector<Mark> marks_;

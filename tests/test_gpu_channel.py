@@ -210,3 +210,32 @@ def test_channel_rejects_unsupported(gpu):
     assert e.value.code == _lib.ERR_INVALID
     with pytest.raises(_lib.TarangError):
         gpu.chan(2048, 16, 4.0, 1.0, kappa, nu, 1e-2, dtype=np.float64)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("ts", ["SBDF2", "SBDF1"])
+def test_channel_sbdf_parity(gpu, oracle, ts, dtype):
+    """Per-mode multistep (step_subproblem_multistep.jl:197-420): SBDF1 start, SBDF2 after,
+    a dt change (w != 1) half way; p and the taus come from the solve."""
+    nx, N, Lx, Lz = 32, 24, 4.0, 1.0
+    print(f"channel {ts} {dtype.__name__}", flush=True)
+    kappa, nu = params()
+    st = random_state(oracle, nx, N, Lx, Lz, seed=13)
+    ref = oracle.RBC2D(nx, N, Lx, Lz, kappa, nu, 5e-3, ts)
+    s = gpu.chan(nx, N, Lx, Lz, kappa, nu, 5e-3, stepper={"SBDF2": 2, "SBDF1": 4}[ts], dtype=dtype)
+    for var, c in st.items():
+        ref.set(var, c)
+        s.set(var, c)
+    tol = TOL[dtype] * (10 if dtype == np.float32 else 1)
+    for nsteps, dt in ((1, 5e-3), (3, 5e-3), (2, 2.5e-3), (2, 4e-3)):
+        ref.set_dt(dt)
+        s.set_dt(dt)
+        ref.step(nsteps)
+        s.step(nsteps)
+        for var in (oracle.RBC_B, oracle.RBC_UX, oracle.RBC_UZ, oracle.RBC_P):
+            assert relerr(s.get(var), ref.get(var)) < tol * (100 if var == oracle.RBC_P else 1), (nsteps, dt, var)
+        for t in range(7):
+            r = ref.tau(t)
+            if np.abs(r).max() > 0:
+                assert relerr(s.tau(t), r) < tol * 100, (t,)
+    s.close()
