@@ -42,6 +42,7 @@ typedef struct tarang_ctx_s* tarang_ctx_t;
 typedef struct tarang_plan_s* tarang_plan_t;
 typedef struct tarang_fivp_s* tarang_fivp_t;
 typedef struct tarang_chan_s* tarang_chan_t;
+typedef struct tarang_chan3_s* tarang_chan3_t;
 
 /* ------------------------------------------------------------------------ */
 /* Context and memory.  Replaces the device hooks of the architecture layer:
@@ -192,6 +193,50 @@ int tarang_chan_time(tarang_chan_t s, double* t, long* iteration);
 int tarang_chan_profile(tarang_chan_t s, int nsteps, double* avg_ms6, long* counts6);
 /* algorithmic bytes per launch of the same six kernel kinds */
 int tarang_chan_kernel_bytes(tarang_chan_t s, double* bytes6);
+
+/* ------------------------------------------------------------------------ */
+/* 3-D Rayleigh-Benard, RealFourier(x, [0, Lx)) x RealFourier(y, [0, Ly)) x
+ * ChebyshevT(z, [0, Lz]): the equations above with u = (ux, uy, uz) and taus
+ * tau_u1, tau_u2 of three components.  Replaces, for this family, the per-mode
+ * subproblem build and the per-mode multistep step!/run!
+ * (src/core/timesteppers/step_subproblem_multistep.jl:197-420, SBDF2 coefficients
+ * :544-552, SBDF1 start).  timestepper: 2 SBDF2, 4 SBDF1.
+ * nx: power of 2, 8..1024 (FP64: ..512); ny: power of 2, 8..1024; nz: 4..256. */
+typedef struct {
+    int nx, ny, nz;
+    double Lx, Ly, Lz;
+    double kappa, nu;
+    double dt;
+    int dtype;
+    int timestepper;
+    double b_bottom, b_top;
+} tarang_chan3_desc;
+
+#define TARANG_CHAN3_P 0
+#define TARANG_CHAN3_B 1
+#define TARANG_CHAN3_UX 2
+#define TARANG_CHAN3_UY 3
+#define TARANG_CHAN3_UZ 4
+
+int tarang_chan3_create(tarang_ctx_t ctx, const tarang_chan3_desc* desc, tarang_chan3_t* out);
+int tarang_chan3_destroy(tarang_chan3_t s);
+/* complex (nx/2+1, ny, nz) coefficient arrays, reference layout, device pointers */
+int tarang_chan3_set_field(tarang_chan3_t s, int var, const void* coeff);
+int tarang_chan3_get_field(tarang_chan3_t s, int var, void* coeff);
+/* tau t: 0 tau_p, 1 tau_b1, 2 tau_b2, 3..5 tau_u1.{x,y,z}, 6..8 tau_u2.{x,y,z}; (nx/2+1, ny) values */
+int tarang_chan3_get_tau(tarang_chan3_t s, int t, void* vals);
+/* F_b, F_ux, F_uy, F_uz of the current state (grid round trip, 3/2-padded x-y products) */
+int tarang_chan3_rhs(tarang_chan3_t s, void* Fb, void* Fux, void* Fuy, void* Fuz);
+int tarang_chan3_step(tarang_chan3_t s, int nsteps);
+int tarang_chan3_set_graph(tarang_chan3_t s, int use_graph);
+int tarang_chan3_set_dt(tarang_chan3_t s, double dt);
+int tarang_chan3_time(tarang_chan3_t s, double* t, long* iteration);
+/* number of distinct |k_h|^2 keys whose factors the modes share */
+int tarang_chan3_num_keys(tarang_chan3_t s, int* nkeys);
+/* eager steps with HIP events; [z-backward, y-backward, x-pass, y-forward, z-forward,
+ * multistep rhs, per-mode solve] average ms and counts.  No reference counterpart. */
+int tarang_chan3_profile(tarang_chan3_t s, int nsteps, double* avg_ms7, long* counts7);
+int tarang_chan3_kernel_bytes(tarang_chan3_t s, double* bytes7);
 
 #ifdef __cplusplus
 }

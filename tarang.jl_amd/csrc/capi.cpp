@@ -4,6 +4,7 @@
 #include <new>
 
 #include "../../include/tarang_hip.h"
+#include "channel3d_solver.hpp"
 #include "channel_solver.hpp"
 #include "fourier_solver.hpp"
 #include "runtime.hpp"
@@ -20,6 +21,9 @@ struct tarang_fivp_s {
 };
 struct tarang_chan_s {
     std::unique_ptr<tarang::ChannelSolver> solver;
+};
+struct tarang_chan3_s {
+    std::unique_ptr<tarang::Channel3Solver> solver;
 };
 
 namespace {
@@ -443,6 +447,122 @@ int tarang_chan_kernel_bytes(tarang_chan_t s, double* bytes6) {
         NEED(s, "solver");
         NEED(bytes6, "bytes6");
         s->solver->kernel_bytes(bytes6);
+    });
+}
+
+// ---- 3-D Rayleigh-Benard channel ----
+int tarang_chan3_create(tarang_ctx_t ctx, const tarang_chan3_desc* desc, tarang_chan3_t* out) {
+    return guard([&] {
+        NEED(ctx, "ctx");
+        NEED(desc, "desc");
+        NEED(out, "out");
+        *out = nullptr;
+        tarang::Channel3Desc d;
+        d.nx = desc->nx;
+        d.ny = desc->ny;
+        d.nz = desc->nz;
+        d.Lx = desc->Lx;
+        d.Ly = desc->Ly;
+        d.Lz = desc->Lz;
+        d.kappa = desc->kappa;
+        d.nu = desc->nu;
+        d.dt = desc->dt;
+        d.dtype = desc->dtype;
+        d.b_bottom = desc->b_bottom;
+        d.b_top = desc->b_top;
+        d.stepper = desc->timestepper == 2 ? tarang::kSBDF2 : (desc->timestepper == 4 ? tarang::kSBDF1 : -1);
+        auto h = std::make_unique<tarang_chan3_s>();
+        h->solver = std::make_unique<tarang::Channel3Solver>(&ctx->ctx, d);
+        *out = h.release();
+    });
+}
+
+int tarang_chan3_destroy(tarang_chan3_t s) {
+    return guard([&] { delete s; });
+}
+
+int tarang_chan3_set_field(tarang_chan3_t s, int var, const void* coeff) {
+    return guard([&] {
+        NEED(s, "solver");
+        NEED(coeff, "coeff");
+        s->solver->set_field(var, coeff);
+    });
+}
+
+int tarang_chan3_get_field(tarang_chan3_t s, int var, void* coeff) {
+    return guard([&] {
+        NEED(s, "solver");
+        NEED(coeff, "coeff");
+        s->solver->get_field(var, coeff);
+    });
+}
+
+int tarang_chan3_get_tau(tarang_chan3_t s, int t, void* vals) {
+    return guard([&] {
+        NEED(s, "solver");
+        NEED(vals, "vals");
+        s->solver->get_tau(t, vals);
+    });
+}
+
+int tarang_chan3_rhs(tarang_chan3_t s, void* Fb, void* Fux, void* Fuy, void* Fuz) {
+    return guard([&] {
+        NEED(s, "solver");
+        s->solver->rhs(Fb, Fux, Fuy, Fuz);
+    });
+}
+
+int tarang_chan3_step(tarang_chan3_t s, int nsteps) {
+    return guard([&] {
+        NEED(s, "solver");
+        s->solver->step(nsteps);
+    });
+}
+
+int tarang_chan3_set_graph(tarang_chan3_t s, int use_graph) {
+    return guard([&] {
+        NEED(s, "solver");
+        s->solver->enable_graph(use_graph != 0);
+    });
+}
+
+int tarang_chan3_set_dt(tarang_chan3_t s, double dt) {
+    return guard([&] {
+        NEED(s, "solver");
+        s->solver->set_dt(dt);
+    });
+}
+
+int tarang_chan3_time(tarang_chan3_t s, double* t, long* iteration) {
+    return guard([&] {
+        NEED(s, "solver");
+        if (t) *t = s->solver->time();
+        if (iteration) *iteration = s->solver->iteration();
+    });
+}
+
+int tarang_chan3_num_keys(tarang_chan3_t s, int* nkeys) {
+    return guard([&] {
+        NEED(s, "solver");
+        NEED(nkeys, "nkeys");
+        *nkeys = s->solver->nkeys();
+    });
+}
+
+int tarang_chan3_profile(tarang_chan3_t s, int nsteps, double* avg_ms7, long* counts7) {
+    return guard([&] {
+        NEED(s, "solver");
+        NEED(avg_ms7, "avg_ms7");
+        TG_REQUIRE(nsteps > 0, tarang::kErrInvalid, "nsteps must be > 0");
+        s->solver->profile(nsteps, avg_ms7, counts7);
+    });
+}
+
+int tarang_chan3_kernel_bytes(tarang_chan3_t s, double* bytes7) {
+    return guard([&] {
+        NEED(s, "solver");
+        NEED(bytes7, "bytes7");
+        s->solver->kernel_bytes(bytes7);
     });
 }
 

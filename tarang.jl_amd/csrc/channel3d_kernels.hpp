@@ -1,0 +1,503 @@
+// channel3d_kernels.hpp -- kernels of the RealFourier(x) x RealFourier(y) x
+// ChebyshevT(z) channel solver (3-D Rayleigh-Benard, Chebyshev tau).
+//
+// Layouts (mode = m + (Nx/2+1) n, m the kx index, n the ky index in FFT order,
+// exactly the reference's column-major (Nx/2+1, Ny) flattening):
+//   state / right-hand side rows : mode-major, f[mode*N + k]
+//   z-grid planes                : P[z*nmodes + mode]          (N x Ny x (Nx/2+1), kx fastest)
+//   y-padded planes              : Q[(z*MY + j)*nxh + m]        (N x MY x (Nx/2+1))
+//
+// The dealiased products follow the 2-D padded product of each z level
+// (nonlinear_padding.jl:359-437; the Chebyshev axis is not padded):
+//   y: Nyquist split onto +-Ny/2 of the MY grid, inverse FFT; the kx = 0 and
+//      kx = Nx/2 lines keep the real part (the state's grid round trip)
+//   x: Hermitian extension to MX (kx = Nx/2 halved), inverse FFT, products,
+//      forward FFT, truncation to Nx/2+1
+//   y: forward FFT, truncation with the y Nyquist fold, x Nyquist fold
+//      F(Nx/2, ky) += conj F(Nx/2, -ky); scale 1/(Nx Ny MX MY).
+#pragma once
+#include "channel_kernels.hpp"
+
+namespace tarang {
+
+// operand planes of the 3-D right-hand side
+//   0 ux, 1 uy, 2 uz, then for f in (b, ux, uy, uz): 3+3f dx f, 4+3f dy f, 5+3f dz f
+constexpr int kCh3Planes = 15;
+
+struct Ch3Geom {
+    int nx, ny, nxh, N, nmodes;
+    double k0x, k0y;
+};
+
+__host__ __device__ inline double ch3_ky(const Ch3Geom& g, int n) {
+    const int kk = n < (g.ny + 1) / 2 ? n : n - g.ny;  // FFT order
+    return g.k0y * kk;
+}
+
+// ---------------------------------------------------------------------------
+// k3_zbwd: z-backward transform of the operands.  Work item (tile of TK modes, job):
+// job = 2 f + d, f in (b, ux, uy, uz), d = 0 (B: value, dx, dy) / 1 (BD: dz).
+// On the kx = 0 and kx = Nx/2 lines the projected state's x derivative vanishes,
+// and so does its y derivative at the y Nyquist; the kx = Nx/2 line is halved.
+// ---------------------------------------------------------------------------
+template <typename T, int TK, int NT>
+__global__ __launch_bounds__(NT) void k3_zbwd(const cplx<T>* __restrict__ b, const cplx<T>* __restrict__ ux,
+                                              const cplx<T>* __restrict__ uy, const cplx<T>* __restrict__ uz,
+                                              Ch3Geom geo, const T* __restrict__ BT, const T* __restrict__ BDT,
+                                              cplx<T>* __restrict__ G) {
+    using V = cplx<T>;
+    extern __shared__ unsigned char smem_raw[];
+    V* src = reinterpret_cast<V*>(smem_raw);  // [TK][N]
+    const int N = geo.N, nmodes = geo.nmodes;
+    const int f = blockIdx.y >> 1, d = blockIdx.y & 1;
+    const int q0 = blockIdx.x * TK;
+    const V* in = f == 0 ? b : (f == 1 ? ux : (f == 2 ? uy : uz));
+    const T* mat = d == 0 ? BT : BDT;
+    for (int idx = threadIdx.x; idx < TK * N; idx += NT) {
+        const int c = idx / N, k = idx - c * N;
+        const int q = q0 + c;
+        src[c * N + k] = q < nmodes ? in[(size_t)q * N + k] : V{T(0), T(0)};
+    }
+    __syncthreads();
+    const size_t plane = (size_t)N * nmodes;
+    const bool xeven = (geo.nx % 2) == 0, yeven = (geo.ny % 2) == 0;
+    for (int z = threadIdx.x; z < N; z += NT) {
+        V acc[TK];
+#pragma unroll
+        for (int c = 0; c < TK; ++c) acc[c] = V{T(0), T(0)};
+        for (int k = 0; k < N; ++k) {
+            const T a = mat[(size_t)k * N + z];
+#pragma unroll
+            for (int c = 0; c < TK; ++c) {
+                const V s = src[c * N + k];
+                acc[c].x += a * s.x;
+                acc[c].y += a * s.y;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < TK; ++c) {
+            const int q = q0 + c;
+            if (q >= nmodes) continue;
+            const int m = q % geo.nxh, n = q / geo.nxh;
+            const bool xline = m == 0 || (xeven && m == geo.nx / 2);
+            const T w = (xeven && m == geo.nx / 2) ? T(0.5) : T(1);
+            const V v{acc[c].x * w, acc[c].y * w};
+            const size_t o = (size_t)z * nmodes + q;
+            if (d == 1) {
+                G[(size_t)(5 + 3 * f) * plane + o] = v;  // dz f
+                continue;
+            }
+            const T kx = T(geo.k0x * m), ky = T(ch3_ky(geo, n));
+            V dx{-kx * v.y, kx * v.x}, dy{-ky * v.y, ky * v.x};
+            if (xline) {
+                dx = V{T(0), T(0)};
+                if (yeven && n == geo.ny / 2) dy = V{T(0), T(0)};
+            }
+            if (f > 0) G[(size_t)(f - 1) * plane + o] = v;  // ux, uy, uz
+            G[(size_t)(3 + 3 * f) * plane + o] = dx;
+            G[(size_t)(4 + 3 * f) * plane + o] = dy;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k3_ybwd: padded inverse y transform of TK kx-lines of one z level and plane.
+// grid (ceil(nxh/TK), N, planes).
+// ---------------------------------------------------------------------------
+template <typename T, int NY, int MY, int TK, int NT>
+__global__ __launch_bounds__(NT) void k3_ybwd(const cplx<T>* __restrict__ G, Ch3Geom geo, cplx<T>* __restrict__ Q,
+                                              const cplx<T>* __restrict__ tw) {
+    using V = cplx<T>;
+    constexpr int LS = line_stride(MY);
+    constexpr bool TW_LDS = (size_t)(TK * LS + MY) * sizeof(V) <= 160 * 1024;
+    __shared__ V buf[TK * LS];
+    __shared__ V twl_lds[TW_LDS ? MY : 1];
+    const V* twl = TW_LDS ? twl_lds : tw;
+    if constexpr (TW_LDS) load_twiddles<T, MY, NT>(twl_lds, tw);
+    const int nxh = geo.nxh, z = blockIdx.y, pl = blockIdx.z;
+    const int m0 = blockIdx.x * TK;
+    const size_t plane = (size_t)geo.N * geo.nmodes;
+    const V* in = G + (size_t)pl * plane + (size_t)z * geo.nmodes;
+    constexpr int GAP0 = NY / 2 + 1, GAP1 = MY - NY / 2;  // zero band [GAP0, GAP1)
+    for (int idx = threadIdx.x; idx < TK * (GAP1 - GAP0); idx += NT) {
+        const int c = idx % TK, j = GAP0 + idx / TK;
+        buf[c * LS + pidx(j)] = V{T(0), T(0)};
+    }
+    for (int idx = threadIdx.x; idx < TK * NY; idx += NT) {
+        const int c = idx % TK, n = idx / TK;
+        const int m = m0 + c;
+        V v = m < nxh ? in[(size_t)n * nxh + m] : V{T(0), T(0)};
+        V* L = buf + c * LS;
+        if (n < NY / 2) {
+            L[pidx(n)] = v;
+        } else if (n == NY / 2) {  // Nyquist split
+            v = V{v.x * T(0.5), v.y * T(0.5)};
+            L[pidx(NY / 2)] = v;
+            L[pidx(MY - NY / 2)] = v;
+        } else {
+            L[pidx(MY - NY + n)] = v;
+        }
+    }
+    __syncthreads();
+    fft_lds<T, MY, TK, NT, true>(buf, twl);
+    const bool xeven = (geo.nx % 2) == 0;
+    cplx<T>* out = Q + (size_t)pl * geo.N * MY * nxh + (size_t)z * MY * nxh;
+    for (int idx = threadIdx.x; idx < TK * MY; idx += NT) {
+        const int c = idx % TK, j = idx / TK;
+        const int m = m0 + c;
+        if (m >= nxh) continue;
+        V v = buf[c * LS + pidx(j)];
+        if (m == 0 || (xeven && m == geo.nx / 2)) v.y = T(0);  // real lines of the grid round trip
+        out[(size_t)j * nxh + m] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k3_xpass: one padded row (z, j) per workgroup: 15 operand rows packed two per
+// complex line (8 lines), inverse x FFT, the four advection products, two forward
+// x FFTs.  Output: four rows of nxh values (scaled by `scale`).
+//   F_b  = -(ux dx b  + uy dy b  + uz dz b)   (and likewise F_ux, F_uy, F_uz)
+// ---------------------------------------------------------------------------
+template <typename T, int NX, int MX, int NT>
+__global__ __launch_bounds__(NT) void k3_xpass(const cplx<T>* __restrict__ Q, int N, int MY, T scale,
+                                               cplx<T>* __restrict__ H, const cplx<T>* __restrict__ tw) {
+    using V = cplx<T>;
+    constexpr int NXH = NX / 2 + 1;
+    constexpr int NL = 8;
+    constexpr int LS = line_stride(MX);
+    constexpr bool TW_LDS = (size_t)(NL * LS + MX) * sizeof(V) <= 160 * 1024;
+    __shared__ V buf[NL * LS];
+    __shared__ V twl_lds[TW_LDS ? MX : 1];
+    const V* twl = TW_LDS ? twl_lds : tw;
+    if constexpr (TW_LDS) load_twiddles<T, MX, NT>(twl_lds, tw);
+    const size_t row = (size_t)blockIdx.y * MY + blockIdx.x;  // (z, j)
+    const size_t plane = (size_t)N * MY * NXH;
+    constexpr int GAPN = MX - 2 * NXH + 1;
+    for (int idx = threadIdx.x; idx < NL * GAPN; idx += NT) {
+        const int line = idx / GAPN, k = NXH + (idx - line * GAPN);
+        buf[line * LS + pidx(k)] = V{T(0), T(0)};
+    }
+    for (int idx = threadIdx.x; idx < NL * NXH; idx += NT) {
+        const int line = idx / NXH, k = idx - line * NXH;
+        const size_t off = row * NXH + k;
+        const V a = Q[(size_t)(2 * line) * plane + off];
+        const V bb = (2 * line + 1 < kCh3Planes) ? Q[(size_t)(2 * line + 1) * plane + off] : V{T(0), T(0)};
+        V* L = buf + line * LS;
+        if (k == 0) {
+            L[pidx(0)] = V{a.x, bb.x};
+        } else {
+            L[pidx(k)] = V{a.x - bb.y, a.y + bb.x};
+            L[pidx(MX - k)] = V{a.x + bb.y, bb.x - a.y};
+        }
+    }
+    __syncthreads();
+    fft_lds<T, MX, NL, NT, true>(buf, twl);
+    // line l holds planes (2l, 2l+1): l0 = (ux, uy), l1 = (uz, dx b), l2 = (dy b, dz b),
+    // l3 = (dx ux, dy ux), l4 = (dz ux, dx uy), l5 = (dy uy, dz uy), l6 = (dx uz, dy uz), l7 = (dz uz, 0)
+    for (int x = threadIdx.x; x < MX; x += NT) {
+        const V l0 = buf[0 * LS + pidx(x)], l1 = buf[1 * LS + pidx(x)], l2 = buf[2 * LS + pidx(x)];
+        const V l3 = buf[3 * LS + pidx(x)], l4 = buf[4 * LS + pidx(x)], l5 = buf[5 * LS + pidx(x)];
+        const V l6 = buf[6 * LS + pidx(x)], l7 = buf[7 * LS + pidx(x)];
+        const T vx = l0.x, vy = l0.y, vz = l1.x;
+        const T fb = -(vx * l1.y + vy * l2.x + vz * l2.y) * scale;
+        const T fux = -(vx * l3.x + vy * l3.y + vz * l4.x) * scale;
+        const T fuy = -(vx * l4.y + vy * l5.x + vz * l5.y) * scale;
+        const T fuz = -(vx * l6.x + vy * l6.y + vz * l7.x) * scale;
+        // each thread reads and rewrites only its own x positions: no barrier needed here
+        buf[0 * LS + pidx(x)] = V{fb, fux};
+        buf[1 * LS + pidx(x)] = V{fuy, fuz};
+    }
+    __syncthreads();
+    fft_lds<T, MX, 2, NT, false>(buf, twl);
+    const size_t hplane = (size_t)N * MY * NXH;
+    for (int k = threadIdx.x; k < NXH; k += NT) {
+        const size_t o = row * NXH + k;
+#pragma unroll
+        for (int l = 0; l < 2; ++l) {
+            const V zk = buf[l * LS + pidx(k)], zm = buf[l * LS + pidx((MX - k) % MX)];
+            H[(size_t)(2 * l) * hplane + o] = V{(zk.x + zm.x) * T(0.5), (zk.y - zm.y) * T(0.5)};
+            H[(size_t)(2 * l + 1) * hplane + o] = V{(zk.y + zm.y) * T(0.5), (zm.x - zk.x) * T(0.5)};
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k3_yfwd: forward y transform of TK padded kx-lines, truncation with the y
+// Nyquist fold, the x Nyquist fold on the kx = Nx/2 line.  Output z-grid planes.
+// grid (ceil(nxh/TK), N, 4).
+// ---------------------------------------------------------------------------
+template <typename T, int NY, int MY, int TK, int NT>
+__global__ __launch_bounds__(NT) void k3_yfwd(const cplx<T>* __restrict__ H, Ch3Geom geo, cplx<T>* __restrict__ F,
+                                              const cplx<T>* __restrict__ tw) {
+    using V = cplx<T>;
+    constexpr int LS = line_stride(MY);
+    constexpr bool TW_LDS = (size_t)(TK * LS + MY) * sizeof(V) <= 160 * 1024;
+    __shared__ V buf[TK * LS];
+    __shared__ V twl_lds[TW_LDS ? MY : 1];
+    const V* twl = TW_LDS ? twl_lds : tw;
+    if constexpr (TW_LDS) load_twiddles<T, MY, NT>(twl_lds, tw);
+    const int nxh = geo.nxh, z = blockIdx.y, o = blockIdx.z;
+    const int m0 = blockIdx.x * TK;
+    const V* in = H + (size_t)o * geo.N * MY * nxh + (size_t)z * MY * nxh;
+    for (int idx = threadIdx.x; idx < TK * MY; idx += NT) {
+        const int c = idx % TK, j = idx / TK;
+        const int m = m0 + c;
+        buf[c * LS + pidx(j)] = m < nxh ? in[(size_t)j * nxh + m] : V{T(0), T(0)};
+    }
+    __syncthreads();
+    fft_lds<T, MY, TK, NT, false>(buf, twl);
+    // truncation (in place: entries n < NY/2 stay, n > NY/2 come from MY - NY + n, Nyquist folded)
+    V tr[(TK * NY + NT - 1) / NT];
+    for (int e = 0, idx = threadIdx.x; idx < TK * NY; ++e, idx += NT) {
+        const int c = idx % TK, n = idx / TK;
+        const V* L = buf + c * LS;
+        V v;
+        if (n < NY / 2) v = L[pidx(n)];
+        else if (n == NY / 2) {
+            const V a = L[pidx(NY / 2)], bb = L[pidx(MY - NY / 2)];
+            v = V{a.x + bb.x, a.y + bb.y};
+        } else v = L[pidx(MY - NY + n)];
+        tr[e] = v;
+    }
+    __syncthreads();
+    for (int e = 0, idx = threadIdx.x; idx < TK * NY; ++e, idx += NT) {
+        const int c = idx % TK, n = idx / TK;
+        buf[c * LS + pidx(n)] = tr[e];
+    }
+    __syncthreads();
+    const bool xeven = (geo.nx % 2) == 0;
+    V* out = F + (size_t)o * geo.N * geo.nmodes + (size_t)z * geo.nmodes;
+    for (int idx = threadIdx.x; idx < TK * NY; idx += NT) {
+        const int c = idx % TK, n = idx / TK;
+        const int m = m0 + c;
+        if (m >= nxh) continue;
+        V v = buf[c * LS + pidx(n)];
+        if (xeven && m == geo.nx / 2) {  // F(Nx/2, ky) += conj F(Nx/2, -ky)
+            const V p = buf[c * LS + pidx((NY - n) % NY)];
+            v = V{v.x + p.x, v.y - p.y};
+        }
+        out[(size_t)n * nxh + m] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-mode systems.  With ux = i ux~, uy = i uy~ (E3x, E3y rows times -i) and the
+// horizontal velocity rotated onto k_h = (kx, ky):
+//   u~par = (kx ux~ + ky uy~)/|k_h|,  u~perp = (-ky ux~ + kx uy~)/|k_h|   (x, y at k_h = 0)
+// every mode splits into: the b system (kappa), the perp system (nu, walls 0) and
+// the 2-D u-p system with kx -> |k_h|, each in two parity groups.  The systems
+// depend on |k_h|^2 only; modes share the factors of their key.
+// ---------------------------------------------------------------------------
+struct Ch3Sys {
+    int N, nkeys;
+    size_t sb, su;
+    const double* kh;   // [nkeys], key 0 is kx = ky = 0
+    double sz, kappa, nu, c, area, Lz;
+};
+
+// grid (2 nkeys, 3): blockIdx.x = 2 key + g; y = 0 b, 1 u-p, 2 perp
+static __global__ void k3_assemble(Ch3Sys s, double* Ab, double* Aup, double* Ap) {
+    const int w = blockIdx.x, key = w >> 1, g = w & 1;
+    const double kh = s.kh[key];
+    if (blockIdx.y == 1) {
+        const int n = chu_size(s.N, g, key);
+        double* A = Aup + (size_t)w * s.su;
+        for (size_t idx = threadIdx.x; idx < (size_t)n * n; idx += blockDim.x)
+            A[idx] = chu_entry_kh(s.N, g, key == 0, kh, s.nu, s.c, s.sz, s.area, s.Lz, (int)(idx % n),
+                                  (int)(idx / n));
+    } else {
+        const int n = chb_size(s.N, g);
+        const double diff = blockIdx.y == 0 ? s.kappa : s.nu;
+        double* A = (blockIdx.y == 0 ? Ab : Ap) + (size_t)w * s.sb;
+        for (size_t idx = threadIdx.x; idx < (size_t)n * n; idx += blockDim.x)
+            A[idx] = chb_entry_kh(s.N, g, kh, diff, s.c, s.sz, (int)(idx % n), (int)(idx / n));
+    }
+}
+
+struct Ch3SolveArgs {
+    const double* Ab;   // b factors, stride sb per (key, parity)
+    const int* permb;   // stride pb
+    const double* Ap;   // perp factors, stride sb
+    const int* permp;
+    const double* Aup;  // u-p factors, stride su
+    const int* permu;   // stride pu
+    const int* order;   // [2 nmodes] workgroup -> 2 mode + g, grouped by key
+    const int* key;     // [nmodes]
+    const double* rot;  // [2 nmodes] (cos, sin) of the k_h direction per mode
+    int N, nmodes;
+    size_t sb, su, pb, pu;
+    int ld;
+    double c;
+    double bc_b0, bc_b1;  // Nx Ny b(z=0), Nx Ny b(z=Lz) at kx = ky = 0
+};
+
+// 3-D tau slots: 0 tau_p, 1 tau_b1, 2 tau_b2, 3 tau_u1x, 4 tau_u1y, 5 tau_u1z, 6 tau_u2x, 7 tau_u2y, 8 tau_u2z
+constexpr int kCh3TauStride = 9;
+
+__host__ __device__ inline size_t ch3_solve_lds_doubles(int ld, int N) {
+    return (size_t)2 * ld + 64 * 64 + 4 * (size_t)(N / 2 + 1) + 8;
+}
+
+// grid 2 nmodes: workgroup -> (mode, g): b system of parity h = g^1, perp system of
+// parity g, u-p system of parity g; un-rotates the parity-g horizontal velocity and taus.
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k3_solve(Ch3SolveArgs s, const cplx<T>* __restrict__ R1,
+                                               const cplx<T>* __restrict__ R2, const cplx<T>* __restrict__ R3x,
+                                               const cplx<T>* __restrict__ R3y, const cplx<T>* __restrict__ R3z,
+                                               cplx<T>* P, cplx<T>* B, cplx<T>* UX, cplx<T>* UY, cplx<T>* UZ,
+                                               cplx<T>* taus) {
+    extern __shared__ unsigned char smem_raw[];
+    const int N = s.N, ld = s.ld;
+    const int wm = s.order[blockIdx.x], mode = wm >> 1, g = wm & 1, h = g ^ 1;
+    const int key = s.key[mode];
+    const int PL = (N - 1) & 1;
+    const double cr = s.rot[2 * mode], sr = s.rot[2 * mode + 1];
+    double* y = reinterpret_cast<double*>(smem_raw);  // [2][ld]
+    double* blk = y + 2 * ld;                           // [64*64], also permutation scratch
+    double* bre = blk + 64 * 64;                        // [N/2+1] buoyancy of parity h
+    double* bim = bre + (N / 2 + 1);
+    double* pre = bim + (N / 2 + 1);                    // [N/2+1] u~perp of parity g
+    double* pim = pre + (N / 2 + 1);
+    double* ptau = pim + (N / 2 + 1);                   // [2] tau of the perp system
+    const size_t base = (size_t)mode * N;
+    const int kb = 2 * key;
+    // ---- b system of parity h ----
+    const int nh = npar(N, h), ng = npar(N, g);
+    for (int i = threadIdx.x; i < nh; i += NT) {
+        const cplx<T> r = R2[base + 2 * i + h];
+        y[i] = (double)r.x;
+        y[ld + i] = (double)r.y;
+    }
+    if (threadIdx.x == 0) {
+        const double gv = h == 0 ? 0.5 * (s.bc_b1 + s.bc_b0) : 0.5 * (s.bc_b1 - s.bc_b0);
+        y[nh] = mode == 0 ? s.c * gv : 0.0;
+        y[ld + nh] = 0.0;
+    }
+    __syncthreads();
+    lu_solve_blocked<NT, 2>(s.Ab + (size_t)(kb + h) * s.sb, s.permb + (size_t)(kb + h) * s.pb, nh + 1, y, ld, blk,
+                            blk);
+    for (int i = threadIdx.x; i < nh; i += NT) {
+        bre[i] = y[i];
+        bim[i] = y[ld + i];
+        B[base + 2 * i + h] = cplx<T>{T(y[i]), T(y[ld + i])};
+    }
+    if (threadIdx.x == 0) taus[(size_t)mode * kCh3TauStride + (h == (PL ^ 1) ? 1 : 2)] = cplx<T>{T(y[nh]), T(y[ld + nh])};
+    __syncthreads();
+    // ---- perp system of parity g: rows -i (-sin E3x + cos E3y) ----
+    for (int i = threadIdx.x; i < ng; i += NT) {
+        const cplx<T> rx = R3x[base + 2 * i + g], ry = R3y[base + 2 * i + g];
+        const double ex_re = (double)rx.y, ex_im = -(double)rx.x;  // -i rx
+        const double ey_re = (double)ry.y, ey_im = -(double)ry.x;
+        y[i] = -sr * ex_re + cr * ey_re;
+        y[ld + i] = -sr * ex_im + cr * ey_im;
+    }
+    if (threadIdx.x == 0) {
+        y[ng] = 0.0;
+        y[ld + ng] = 0.0;
+    }
+    __syncthreads();
+    lu_solve_blocked<NT, 2>(s.Ap + (size_t)(kb + g) * s.sb, s.permp + (size_t)(kb + g) * s.pb, ng + 1, y, ld, blk,
+                            blk);
+    for (int i = threadIdx.x; i < ng; i += NT) {
+        pre[i] = y[i];
+        pim[i] = y[ld + i];
+    }
+    if (threadIdx.x == 0) {
+        ptau[0] = y[ng];
+        ptau[1] = y[ld + ng];
+    }
+    __syncthreads();
+    // ---- u-p system of parity g: rows -i (cos E3x + sin E3y), E1, E3z + c b ----
+    const int n = chu_size(N, g, key);
+    for (int i = threadIdx.x; i < ng; i += NT) {
+        const cplx<T> rx = R3x[base + 2 * i + g], ry = R3y[base + 2 * i + g], r1 = R1[base + 2 * i + g];
+        const double ex_re = (double)rx.y, ex_im = -(double)rx.x;
+        const double ey_re = (double)ry.y, ey_im = -(double)ry.x;
+        y[i] = cr * ex_re + sr * ey_re;
+        y[ld + i] = cr * ex_im + sr * ey_im;
+        y[ng + i] = (double)r1.x;
+        y[ld + ng + i] = (double)r1.y;
+    }
+    for (int i = threadIdx.x; i < nh; i += NT) {
+        const cplx<T> rz = R3z[base + 2 * i + h];
+        y[2 * ng + i] = (double)rz.x + s.c * bre[i];
+        y[ld + 2 * ng + i] = (double)rz.y + s.c * bim[i];
+    }
+    for (int r = 2 * ng + nh + threadIdx.x; r < n; r += NT) {
+        y[r] = 0.0;
+        y[ld + r] = 0.0;
+    }
+    __syncthreads();
+    lu_solve_blocked<NT, 2>(s.Aup + (size_t)(kb + g) * s.su, s.permu + (size_t)(kb + g) * s.pu, n, y, ld, blk, blk);
+    // un-rotate: u~x = cos par - sin perp, u~y = sin par + cos perp; u = i u~
+    for (int i = threadIdx.x; i < ng; i += NT) {
+        const double par_re = y[i], par_im = y[ld + i];
+        const double tx_re = cr * par_re - sr * pre[i], tx_im = cr * par_im - sr * pim[i];
+        const double ty_re = sr * par_re + cr * pre[i], ty_im = sr * par_im + cr * pim[i];
+        UX[base + 2 * i + g] = cplx<T>{T(-tx_im), T(tx_re)};
+        UY[base + 2 * i + g] = cplx<T>{T(-ty_im), T(ty_re)};
+        P[base + 2 * i + g] = cplx<T>{T(y[ng + i]), T(y[ld + ng + i])};
+    }
+    for (int i = threadIdx.x; i < nh; i += NT) UZ[base + 2 * i + h] = cplx<T>{T(y[2 * ng + i]), T(y[ld + 2 * ng + i])};
+    if (threadIdx.x == 0) {
+        cplx<T>* tm = taus + (size_t)mode * kCh3TauStride;
+        const int t0 = 2 * ng + nh;
+        for (int t = 0; t < n - t0; ++t) {
+            const int slot = chu_tau_slot(N, g, t);  // 2-D slots: 0 p, 3 u1x, 4 u1z, 5 u2x, 6 u2z
+            const double re = y[t0 + t], im = y[ld + t0 + t];
+            if (slot == 0) {
+                tm[0] = cplx<T>{T(re), T(im)};
+            } else if (slot == 4 || slot == 6) {
+                tm[slot == 4 ? 5 : 8] = cplx<T>{T(re), T(im)};
+            } else {  // horizontal lift tau: par from this system, perp from the perp system (same lift)
+                const double tx_re = cr * re - sr * ptau[0], tx_im = cr * im - sr * ptau[1];
+                const double ty_re = sr * re + cr * ptau[0], ty_im = sr * im + cr * ptau[1];
+                const int sx = slot == 3 ? 3 : 6;
+                tm[sx] = cplx<T>{T(-tx_im), T(tx_re)};
+                tm[sx + 1] = cplx<T>{T(-ty_im), T(ty_re)};
+            }
+        }
+        if (mode != 0 && g == 0) tm[0] = cplx<T>{T(0), T(0)};
+    }
+}
+
+// Multistep right-hand side with the grid round trip of the state (Hermitian part
+// of the kx = 0 and kx = Nx/2 lines), scaled by 1/a0; histories replaced in place.
+template <typename T>
+struct Ch3SbdfArgs {
+    const cplx<T>* X[4];   // b, ux, uy, uz
+    const cplx<T>* F[4];   // F_n rows E2, E3x, E3y, E3z
+    cplx<T>* Fm1[4];
+    cplx<T>* MXm1[4];
+    cplx<T>* R[5];         // E1, E2, E3x, E3y, E3z
+    T c1, c2, a1, a2, inv_a0;
+    Ch3Geom geo;
+};
+
+template <typename T>
+__global__ void k3_sbdf(Ch3SbdfArgs<T> a, size_t n) {
+    const int N = a.geo.N, nxh = a.geo.nxh, ny = a.geo.ny;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int mode = (int)(i / N), k = (int)(i - (size_t)mode * N);
+        const int m = mode % nxh, nn = mode / nxh;
+        const bool xline = m == 0 || ((a.geo.nx % 2) == 0 && m == a.geo.nx / 2);
+        const size_t partner = ((size_t)(m + nxh * ((ny - nn) % ny))) * N + k;
+        a.R[0][i] = cplx<T>{T(0), T(0)};
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            cplx<T> x = a.X[r][i];
+            if (xline) {
+                const cplx<T> q = a.X[r][partner];
+                x = cplx<T>{(x.x + q.x) * T(0.5), (x.y - q.y) * T(0.5)};
+            }
+            const cplx<T> f = a.F[r][i], fo = a.Fm1[r][i], mo = a.MXm1[r][i];
+            const T vx = a.c1 * f.x + a.c2 * fo.x - a.a1 * x.x - a.a2 * mo.x;
+            const T vy = a.c1 * f.y + a.c2 * fo.y - a.a1 * x.y - a.a2 * mo.y;
+            a.R[r + 1][i] = cplx<T>{vx * a.inv_a0, vy * a.inv_a0};
+            a.Fm1[r][i] = f;
+            a.MXm1[r][i] = x;
+        }
+    }
+}
+
+}  // namespace tarang

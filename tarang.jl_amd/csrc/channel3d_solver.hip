@@ -1,0 +1,491 @@
+// channel3d_solver.hip -- 3-D Rayleigh-Benard channel, per-mode multistep on the GPU.
+//
+// Step (src/core/timesteppers/step_subproblem_multistep.jl:197-420, _sbdf2_coefs :544-552):
+//   F_n = F(P(x_n)) (z-backward, padded y and x passes, z-forward), M P(x_n)
+//   rhs = (c1 F_n + c2 F_{n-1} - a1 M P(x_n) - a2 M P(x_{n-1})) / a0, BC rows c ALG
+//   x_{n+1} = (M + L/a0)^-1 rhs  per mode (three parity-split FP64 systems per parity)
+// SBDF1 on the first step after create / set_field, SBDF2 (w = dt/dt_prev) after.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "channel3d_kernels.hpp"
+#include "channel3d_solver.hpp"
+#include "dispatch.hpp"
+
+namespace tarang {
+
+namespace {
+bool size_ok(int n) {
+    bool ok = false;
+    dispatch_size(n, [&](auto) { ok = true; }, Sizes2D{});
+    return ok;
+}
+}  // namespace
+
+Channel3Solver::Channel3Solver(Context* ctx, const Channel3Desc& d) : ctx_(ctx), d_(d) {
+    TG_REQUIRE(d.dtype == kF32 || d.dtype == kF64, kErrInvalid, "3-D channel solver: dtype must be f32 or f64");
+    TG_REQUIRE(d.stepper == kSBDF2 || d.stepper == kSBDF1, kErrInvalid,
+               "3-D channel solver: timestepper must be SBDF2 or SBDF1");
+    const int xmax = d.dtype == kF64 ? 512 : 1024;
+    TG_REQUIRE(size_ok(d.nx) && d.nx <= xmax && size_ok(d.ny) && d.ny <= 1024, kErrInvalid,
+               "3-D channel solver: RealFourier sizes " + std::to_string(d.nx) + "x" + std::to_string(d.ny) +
+                   " unsupported (powers of 2; x 8.." + std::to_string(xmax) + ", y 8..1024)");
+    TG_REQUIRE(d.nz >= 4 && d.nz <= 256, kErrInvalid,
+               "3-D channel solver: ChebyshevT size " + std::to_string(d.nz) + " unsupported (4..256)");
+    TG_REQUIRE(d.Lx > 0 && d.Ly > 0 && d.Lz > 0 && d.dt > 0, kErrInvalid,
+               "3-D channel solver: lengths and dt must be positive");
+    nxh_ = d.nx / 2 + 1;
+    nmodes_ = nxh_ * d.ny;
+    mx_ = padded_len(d.nx);
+    my_ = padded_len(d.ny);
+    nb_max_ = chb_size(d.nz, 0);
+    nu_max_ = chu_size(d.nz, 0, 0);
+    dt_prev_ = d.dt;
+    ctx_->activate();
+    hipStream_t s = ctx_->stream;
+    const size_t fb = field_bytes(), vb = 2 * dtype_size((DType)d.dtype);
+    auto zeroed = [&](DevBuf& b, size_t bytes) {
+        b = DevBuf(ctx_, bytes);
+        TG_HIP(hipMemsetAsync(b.ptr, 0, bytes, s));
+    };
+    zeroed(P_, fb);
+    zeroed(B_, fb);
+    zeroed(UX_, fb);
+    zeroed(UY_, fb);
+    zeroed(UZ_, fb);
+    zeroed(TAU_, (size_t)nmodes_ * kCh3TauStride * vb);
+    for (int r = 0; r < 4; ++r) {
+        zeroed(F_[r], fb);
+        zeroed(Fm1_[r], fb);
+        zeroed(MXm1_[r], fb);
+    }
+    for (int r = 0; r < 5; ++r) zeroed(R_[r], fb);
+    const size_t prow = (size_t)d.nz * my_ * nxh_ * vb;  // one y-padded plane
+    G_ = DevBuf(ctx_, (size_t)kCh3Planes * fb);
+    Q_ = DevBuf(ctx_, (size_t)kCh3Planes * prow);
+    H_ = DevBuf(ctx_, (size_t)4 * prow);
+    // Chebyshev tables
+    std::vector<double> BT, BDT, FT;
+    chebyshev_tables(d.nz, d.Lz, BT, BDT, FT);
+    const size_t nn = (size_t)d.nz * d.nz;
+    auto upload = [&](DevBuf& b, const std::vector<double>& h) {
+        if (d.dtype == kF64) {
+            b = DevBuf(ctx_, nn * sizeof(double));
+            TG_HIP(hipMemcpyAsync(b.ptr, h.data(), nn * sizeof(double), hipMemcpyHostToDevice, s));
+            TG_HIP(hipStreamSynchronize(s));
+        } else {
+            std::vector<float> f(h.begin(), h.end());
+            b = DevBuf(ctx_, nn * sizeof(float));
+            TG_HIP(hipMemcpyAsync(b.ptr, f.data(), nn * sizeof(float), hipMemcpyHostToDevice, s));
+            TG_HIP(hipStreamSynchronize(s));
+        }
+    };
+    upload(BT_, BT);
+    upload(BDT_, BDT);
+    upload(FT_, FT);
+    // |k_h|^2 keys: integer m^2 + kk^2 when Lx == Ly, the pair (m^2, kk^2) otherwise; key 0 = kx = ky = 0
+    const double k0x = 2.0 * M_PI / d.Lx, k0y = 2.0 * M_PI / d.Ly;
+    std::map<std::pair<long, long>, int> keymap;
+    std::vector<int> key(nmodes_);
+    std::vector<double> khv, rot(2 * (size_t)nmodes_);
+    const bool iso = d.Lx == d.Ly;
+    keymap[{0, 0}] = 0;
+    khv.push_back(0.0);
+    for (int q = 0; q < nmodes_; ++q) {
+        const int m = q % nxh_, n = q / nxh_;
+        const long kk = n < (d.ny + 1) / 2 ? n : n - d.ny;
+        const std::pair<long, long> kp = iso ? std::make_pair((long)m * m + kk * kk, 0L) : std::make_pair((long)m * m, kk * kk);
+        auto it = keymap.find(kp);
+        if (it == keymap.end()) {
+            const double kx = k0x * m, ky = k0y * kk;
+            it = keymap.emplace(kp, (int)khv.size()).first;
+            khv.push_back(std::sqrt(kx * kx + ky * ky));
+        }
+        key[q] = it->second;
+        const double kx = k0x * m, ky = k0y * kk, kh = std::sqrt(kx * kx + ky * ky);
+        rot[2 * q] = kh > 0 ? kx / kh : 1.0;
+        rot[2 * q + 1] = kh > 0 ? ky / kh : 0.0;
+    }
+    // the only mode with key 0 is kx = ky = 0 (integer keys), so the tau_p system is key 0
+    nkeys_ = (int)khv.size();
+    std::vector<int> order(2 * (size_t)nmodes_);
+    {
+        std::vector<int> modes(nmodes_);
+        for (int q = 0; q < nmodes_; ++q) modes[q] = q;
+        std::stable_sort(modes.begin(), modes.end(), [&](int a, int b) { return key[a] < key[b]; });
+        for (int i = 0; i < nmodes_; ++i) {
+            order[2 * i] = 2 * modes[i];
+            order[2 * i + 1] = 2 * modes[i] + 1;
+        }
+    }
+    auto up = [&](DevBuf& b, const void* h, size_t bytes) {
+        b = DevBuf(ctx_, bytes);
+        TG_HIP(hipMemcpyAsync(b.ptr, h, bytes, hipMemcpyHostToDevice, s));
+        TG_HIP(hipStreamSynchronize(s));
+    };
+    up(kh_, khv.data(), khv.size() * sizeof(double));
+    up(order_, order.data(), order.size() * sizeof(int));
+    up(key_, key.data(), key.size() * sizeof(int));
+    up(rot_, rot.data(), rot.size() * sizeof(double));
+    Ab_ = DevBuf(ctx_, (size_t)2 * nkeys_ * nb_max_ * nb_max_ * sizeof(double));
+    Ap_ = DevBuf(ctx_, (size_t)2 * nkeys_ * nb_max_ * nb_max_ * sizeof(double));
+    Aup_ = DevBuf(ctx_, (size_t)2 * nkeys_ * nu_max_ * nu_max_ * sizeof(double));
+    permb_ = DevBuf(ctx_, (size_t)2 * nkeys_ * nb_max_ * sizeof(int));
+    permp_ = DevBuf(ctx_, (size_t)2 * nkeys_ * nb_max_ * sizeof(int));
+    permu_ = DevBuf(ctx_, (size_t)2 * nkeys_ * nu_max_ * sizeof(int));
+    info_ = DevBuf(ctx_, (size_t)6 * nkeys_ * sizeof(int));
+    ctx_->twiddle(mx_, (DType)d.dtype);
+    ctx_->twiddle(my_, (DType)d.dtype);
+    TG_HIP(hipStreamSynchronize(s));
+    factor(implicit_c(true));
+}
+
+Channel3Solver::~Channel3Solver() {
+    if (graph_) (void)hipGraphExecDestroy(graph_);
+}
+
+void* Channel3Solver::field_ptr(int var) const {
+    switch (var) {
+        case kC3P: return P_.ptr;
+        case kC3B: return B_.ptr;
+        case kC3UX: return UX_.ptr;
+        case kC3UY: return UY_.ptr;
+        case kC3UZ: return UZ_.ptr;
+        default: break;
+    }
+    TG_REQUIRE(false, kErrInvalid, "3-D channel solver: variable must be 0 (p), 1 (b), 2 (ux), 3 (uy) or 4 (uz)");
+    return nullptr;
+}
+
+template <typename T>
+void Channel3Solver::transpose_t(const void* in, void* out, int rows, int cols) {
+    dim3 grid((unsigned)((cols + 31) / 32), (unsigned)((rows + 31) / 32));
+    hipLaunchKernelGGL((k_ch_transpose<T>), grid, dim3(32, 8), 0, ctx_->stream, static_cast<const cplx<T>*>(in),
+                       static_cast<cplx<T>*>(out), rows, cols);
+    TG_HIP(hipGetLastError());
+}
+
+// reference layout (mode fastest, nz rows of nmodes) <-> mode-major (nmodes rows of nz)
+void Channel3Solver::set_field(int var, const void* dev) {
+    ctx_->activate();
+    void* dst = field_ptr(var);
+    if (d_.dtype == kF32) transpose_t<float>(dev, dst, d_.nz, nmodes_);
+    else transpose_t<double>(dev, dst, d_.nz, nmodes_);
+    t_ = 0;
+    iter_ = 0;  // a new initial condition restarts the multistep history
+}
+
+void Channel3Solver::get_field(int var, void* dev) {
+    ctx_->activate();
+    const void* src = field_ptr(var);
+    if (d_.dtype == kF32) transpose_t<float>(src, dev, nmodes_, d_.nz);
+    else transpose_t<double>(src, dev, nmodes_, d_.nz);
+}
+
+void Channel3Solver::get_tau(int t, void* dev) {
+    TG_REQUIRE(t >= 0 && t < kCh3TauStride, kErrInvalid, "3-D channel solver: tau index must be 0..8");
+    ctx_->activate();
+    const size_t vb = 2 * dtype_size((DType)d_.dtype);
+    TG_HIP(hipMemcpy2DAsync(dev, vb, static_cast<const char*>(TAU_.ptr) + t * vb, kCh3TauStride * vb, vb, nmodes_,
+                            hipMemcpyDeviceToDevice, ctx_->stream));
+}
+
+void Channel3Solver::set_dt(double dt) {
+    TG_REQUIRE(dt > 0, kErrInvalid, "3-D channel solver: dt must be positive");
+    d_.dt = dt;
+}
+
+double Channel3Solver::implicit_c(bool first) const {
+    if (first) return d_.dt;
+    const double w = d_.dt / dt_prev_;
+    return (1.0 + w) * d_.dt / (1.0 + 2.0 * w);
+}
+
+void Channel3Solver::factor(double c) {
+    if (c == factored_c_) return;
+    hipStream_t s = ctx_->stream;
+    const int N = d_.nz;
+    const size_t sb = (size_t)nb_max_ * nb_max_, su = (size_t)nu_max_ * nu_max_;
+    Ch3Sys sys{N, nkeys_, sb, su, kh_.as<const double>(), 2.0 / d_.Lz, d_.kappa, d_.nu, c, d_.Lx * d_.Ly, d_.Lz};
+    const unsigned nsys = (unsigned)(2 * nkeys_);
+    hipLaunchKernelGGL(k3_assemble, dim3(nsys, 3), dim3(256), 0, s, sys, Ab_.as<double>(), Aup_.as<double>(),
+                       Ap_.as<double>());
+    TG_HIP(hipGetLastError());
+    int* info = info_.as<int>();
+    constexpr int NT = 256;
+    hipLaunchKernelGGL((k_lu_factor<NT>), dim3(nsys), dim3(NT), (size_t)nb_max_ * 24, s, Ab_.as<double>(),
+                       permb_.as<int>(), info, N, 0, sb, (size_t)nb_max_);
+    hipLaunchKernelGGL((k_lu_factor<NT>), dim3(nsys), dim3(NT), (size_t)nb_max_ * 24, s, Ap_.as<double>(),
+                       permp_.as<int>(), info + nsys, N, 0, sb, (size_t)nb_max_);
+    hipLaunchKernelGGL((k_lu_factor<NT>), dim3(nsys), dim3(NT), (size_t)nu_max_ * 24, s, Aup_.as<double>(),
+                       permu_.as<int>(), info + 2 * nsys, N, 1, su, (size_t)nu_max_);
+    TG_HIP(hipGetLastError());
+    std::vector<int> h((size_t)3 * nsys);
+    TG_HIP(hipMemcpyAsync(h.data(), info, h.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+    TG_HIP(hipStreamSynchronize(s));
+    std::string bad;
+    const char* names[3] = {" b-system", " perp-system", " u-p-system"};
+    for (int kind = 0; kind < 3; ++kind)
+        for (unsigned w = 0; w < nsys && bad.size() < 200; ++w)
+            if (h[kind * nsys + w])
+                bad += std::string(names[kind]) + "(|k_h| key " + std::to_string(w / 2) + ", parity " +
+                       std::to_string(w % 2) + ")";
+    TG_REQUIRE(bad.empty(), kErrSingular, "3-D channel solver: singular per-mode system:" + bad);
+    factored_c_ = c;
+    invalidate_graph();
+}
+
+template <typename T>
+void Channel3Solver::rhs_t(void* F0, void* F1, void* F2, void* F3) {
+    using V = cplx<T>;
+    const int N = d_.nz;
+    hipStream_t s = ctx_->stream;
+    const DType dt = std::is_same<T, float>::value ? kF32 : kF64;
+    Ch3Geom geo{d_.nx, d_.ny, nxh_, N, nmodes_, 2.0 * M_PI / d_.Lx, 2.0 * M_PI / d_.Ly};
+    V* G = G_.as<V>();
+    V* Q = Q_.as<V>();
+    V* H = H_.as<V>();
+    constexpr int TKZ = 4;
+    const size_t zlds = (size_t)TKZ * N * sizeof(V);
+    const unsigned ztiles = (unsigned)((nmodes_ + TKZ - 1) / TKZ);
+    auto zb = [&](auto NTC) {
+        constexpr int NT = decltype(NTC)::value;
+        timed(0, [&] {
+            hipLaunchKernelGGL((k3_zbwd<T, TKZ, NT>), dim3(ztiles, 8), dim3(NT), zlds, s, B_.as<const V>(),
+                               UX_.as<const V>(), UY_.as<const V>(), UZ_.as<const V>(), geo, BT_.as<const T>(),
+                               BDT_.as<const T>(), G);
+        });
+    };
+    if (N <= 64) zb(std::integral_constant<int, 64>{}); else zb(std::integral_constant<int, 256>{});
+    TG_HIP(hipGetLastError());
+    dispatch_size(d_.ny, [&](auto NC) {
+        constexpr int NY = decltype(NC)::value;
+        constexpr int MY = padded_len(NY);
+        constexpr int TK = tile_cols<T, MY>();
+        const V* tw = static_cast<const V*>(ctx_->twiddle(MY, dt));
+        timed(1, [&] {
+            hipLaunchKernelGGL((k3_ybwd<T, NY, MY, TK, 256>), dim3((nxh_ + TK - 1) / TK, N, kCh3Planes), dim3(256), 0,
+                               s, G, geo, Q, tw);
+        });
+    }, Sizes2D{});
+    TG_HIP(hipGetLastError());
+    const T scale = T(1.0 / ((double)d_.nx * d_.ny * (double)mx_ * my_));
+    dispatch_size(d_.nx, [&](auto NC) {
+        constexpr int NX = decltype(NC)::value;
+        constexpr int MX = padded_len(NX);
+        if constexpr ((size_t)8 * line_stride(MX) * sizeof(V) <= 128 * 1024) {
+            const V* tw = static_cast<const V*>(ctx_->twiddle(MX, dt));
+            constexpr int NT = NX >= 512 ? 512 : 256;
+            timed(2, [&] {
+                hipLaunchKernelGGL((k3_xpass<T, NX, MX, NT>), dim3(my_, N), dim3(NT), 0, s, Q, N, my_, scale, H, tw);
+            });
+        } else {
+            TG_REQUIRE(false, kErrInvalid, "3-D channel solver: x size too large for this dtype");
+        }
+    }, Sizes2D{});
+    TG_HIP(hipGetLastError());
+    V* Fz = G;  // the operand planes are consumed: the four product planes reuse them
+    dispatch_size(d_.ny, [&](auto NC) {
+        constexpr int NY = decltype(NC)::value;
+        constexpr int MY = padded_len(NY);
+        constexpr int TK = tile_cols<T, MY>();
+        const V* tw = static_cast<const V*>(ctx_->twiddle(MY, dt));
+        timed(3, [&] {
+            hipLaunchKernelGGL((k3_yfwd<T, NY, MY, TK, 256>), dim3((nxh_ + TK - 1) / TK, N, 4), dim3(256), 0, s, H,
+                               geo, Fz, tw);
+        });
+    }, Sizes2D{});
+    TG_HIP(hipGetLastError());
+    auto zf = [&](auto NTC) {
+        constexpr int NT = decltype(NTC)::value;
+        timed(4, [&] {
+            hipLaunchKernelGGL((k_ch_zfwd<T, TKZ, NT>), dim3(ztiles, 4), dim3(NT), zlds, s, Fz, nmodes_, 1, N,
+                               FT_.as<const T>(), static_cast<V*>(F0), static_cast<V*>(F1), static_cast<V*>(F2),
+                               static_cast<V*>(F3));
+        });
+    };
+    if (N <= 64) zf(std::integral_constant<int, 64>{}); else zf(std::integral_constant<int, 256>{});
+    TG_HIP(hipGetLastError());
+}
+
+template <typename T>
+void Channel3Solver::sbdf_t(bool first) {
+    using V = cplx<T>;
+    Ch3SbdfArgs<T> a;
+    const double dt = d_.dt;
+    double a0, a1, a2, c1, c2;
+    if (first) {
+        a0 = 1.0 / dt; a1 = -1.0 / dt; a2 = 0.0; c1 = 1.0; c2 = 0.0;
+    } else {
+        const double w = dt / dt_prev_;
+        a0 = (1.0 + 2.0 * w) / ((1.0 + w) * dt);
+        a1 = -(1.0 + w) / dt;
+        a2 = w * w / ((1.0 + w) * dt);
+        c1 = 1.0 + w;
+        c2 = -w;
+    }
+    const DevBuf* X[4] = {&B_, &UX_, &UY_, &UZ_};
+    for (int r = 0; r < 4; ++r) {
+        a.X[r] = X[r]->as<const V>();
+        a.F[r] = F_[r].as<const V>();
+        a.Fm1[r] = Fm1_[r].as<V>();
+        a.MXm1[r] = MXm1_[r].as<V>();
+    }
+    for (int r = 0; r < 5; ++r) a.R[r] = R_[r].as<V>();
+    a.c1 = T(c1); a.c2 = T(c2); a.a1 = T(a1); a.a2 = T(a2); a.inv_a0 = T(1.0 / a0);
+    a.geo = Ch3Geom{d_.nx, d_.ny, nxh_, d_.nz, nmodes_, 2.0 * M_PI / d_.Lx, 2.0 * M_PI / d_.Ly};
+    const size_t n = field_elems();
+    const unsigned nblk = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+    timed(5, [&] { hipLaunchKernelGGL((k3_sbdf<T>), dim3(nblk), dim3(256), 0, ctx_->stream, a, n); });
+    TG_HIP(hipGetLastError());
+}
+
+template <typename T>
+void Channel3Solver::solve_t() {
+    using V = cplx<T>;
+    const int N = d_.nz;
+    const double scale = (double)d_.nx * d_.ny;
+    Ch3SolveArgs sa{Ab_.as<const double>(), permb_.as<const int>(), Ap_.as<const double>(), permp_.as<const int>(),
+                    Aup_.as<const double>(), permu_.as<const int>(), order_.as<const int>(), key_.as<const int>(),
+                    rot_.as<const double>(), N, nmodes_, (size_t)nb_max_ * nb_max_, (size_t)nu_max_ * nu_max_,
+                    (size_t)nb_max_, (size_t)nu_max_, nu_max_, factored_c_, scale * d_.b_bottom, scale * d_.b_top};
+    constexpr int NT = 256;
+    const size_t lds = ch3_solve_lds_doubles(nu_max_, N) * sizeof(double);
+    timed(6, [&] {
+        hipLaunchKernelGGL((k3_solve<T, NT>), dim3(2 * nmodes_), dim3(NT), lds, ctx_->stream, sa, R_[0].as<const V>(),
+                           R_[1].as<const V>(), R_[2].as<const V>(), R_[3].as<const V>(), R_[4].as<const V>(),
+                           P_.as<V>(), B_.as<V>(), UX_.as<V>(), UY_.as<V>(), UZ_.as<V>(), TAU_.as<V>());
+    });
+    TG_HIP(hipGetLastError());
+}
+
+void Channel3Solver::launch_step_body() {
+    const bool first = first_step();
+    if (d_.dtype == kF32) {
+        rhs_t<float>(F_[0].ptr, F_[1].ptr, F_[2].ptr, F_[3].ptr);
+        sbdf_t<float>(first);
+        solve_t<float>();
+    } else {
+        rhs_t<double>(F_[0].ptr, F_[1].ptr, F_[2].ptr, F_[3].ptr);
+        sbdf_t<double>(first);
+        solve_t<double>();
+    }
+}
+
+void Channel3Solver::step_once(bool allow_graph) {
+    const bool first = first_step();
+    factor(implicit_c(first));
+    if (!(allow_graph && graph_on_)) {
+        launch_step_body();
+    } else {
+        if (graph_ && (graph_dt_ != d_.dt || graph_first_ != first || graph_dtp_ != dt_prev_)) invalidate_graph();
+        if (!graph_) {
+            hipGraph_t g = nullptr;
+            TG_HIP(hipStreamBeginCapture(ctx_->stream, hipStreamCaptureModeThreadLocal));
+            try {
+                launch_step_body();
+            } catch (...) {
+                (void)hipStreamEndCapture(ctx_->stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            TG_HIP(hipStreamEndCapture(ctx_->stream, &g));
+            TG_HIP(hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0));
+            TG_HIP(hipGraphDestroy(g));
+            graph_dt_ = d_.dt;
+            graph_dtp_ = dt_prev_;
+            graph_first_ = first;
+        }
+        TG_HIP(hipGraphLaunch(graph_, ctx_->stream));
+    }
+    dt_prev_ = d_.dt;
+    t_ += d_.dt;
+    ++iter_;
+}
+
+void Channel3Solver::step(int nsteps) {
+    TG_REQUIRE(nsteps >= 0, kErrInvalid, "3-D channel solver: nsteps must be >= 0");
+    ctx_->activate();
+    for (int i = 0; i < nsteps; ++i) step_once(true);
+}
+
+void Channel3Solver::rhs(void* Fb, void* Fux, void* Fuy, void* Fuz) {
+    ctx_->activate();
+    void* outs[4] = {Fb, Fux, Fuy, Fuz};
+    if (d_.dtype == kF32) rhs_t<float>(F_[0].ptr, F_[1].ptr, F_[2].ptr, F_[3].ptr);
+    else rhs_t<double>(F_[0].ptr, F_[1].ptr, F_[2].ptr, F_[3].ptr);
+    for (int r = 0; r < 4; ++r) {
+        if (!outs[r]) continue;
+        if (d_.dtype == kF32) transpose_t<float>(F_[r].ptr, outs[r], nmodes_, d_.nz);
+        else transpose_t<double>(F_[r].ptr, outs[r], nmodes_, d_.nz);
+    }
+}
+
+template <typename F>
+void Channel3Solver::timed(int kind, F&& launch) {
+    if (!profiling_) {
+        launch();
+        return;
+    }
+    Mark m{kind, nullptr, nullptr};
+    TG_HIP(hipEventCreate(&m.a));
+    TG_HIP(hipEventCreate(&m.b));
+    TG_HIP(hipEventRecord(m.a, ctx_->stream));
+    launch();
+    TG_HIP(hipEventRecord(m.b, ctx_->stream));
+    marks_.push_back(m);
+}
+
+void Channel3Solver::profile(int nsteps, double* avg_ms, long* counts) {
+    ctx_->activate();
+    marks_.clear();
+    profiling_ = true;
+    try {
+        for (int i = 0; i < nsteps; ++i) step_once(false);
+    } catch (...) {
+        profiling_ = false;
+        throw;
+    }
+    profiling_ = false;
+    TG_HIP(hipStreamSynchronize(ctx_->stream));
+    double sum[kKinds] = {};
+    long cnt[kKinds] = {};
+    for (auto& m : marks_) {
+        float ms = 0.f;
+        TG_HIP(hipEventElapsedTime(&ms, m.a, m.b));
+        sum[m.kind] += ms;
+        cnt[m.kind] += 1;
+        (void)hipEventDestroy(m.a);
+        (void)hipEventDestroy(m.b);
+    }
+    marks_.clear();
+    for (int k = 0; k < kKinds; ++k) {
+        if (avg_ms) avg_ms[k] = cnt[k] ? sum[k] / cnt[k] : 0.0;
+        if (counts) counts[k] = cnt[k];
+    }
+}
+
+void Channel3Solver::kernel_bytes(double* out) const {
+    const double vb = 2.0 * dtype_size((DType)d_.dtype), rb = dtype_size((DType)d_.dtype);
+    const double nc = (double)nmodes_ * d_.nz, N = d_.nz;
+    const double pad = (double)d_.nz * my_ * nxh_;
+    out[0] = (8 * nc + kCh3Planes * nc) * vb + 2 * N * N * rb;  // read 4 fields x 2 jobs, write 15 planes
+    out[1] = (kCh3Planes * nc + kCh3Planes * pad) * vb;         // read 15 planes, write 15 y-padded planes
+    out[2] = (kCh3Planes * pad + 4 * pad) * vb;                 // read 15 padded planes, write 4
+    out[3] = (4 * pad + 4 * nc) * vb;                           // read 4 padded planes, write 4
+    out[4] = (8 * nc) * vb + N * N * rb;                        // read 4 planes, write 4 rows
+    out[5] = (4 * 4 + 4 + 5) * nc * vb;                         // state (x2 on the kx lines), F, histories, rhs
+    double fac = 0;                                             // every mode reads its key's factors
+    for (int q = 0; q < nmodes_; ++q)
+        for (int g = 0; g < 2; ++g) {
+            const double nb = chb_size(d_.nz, g), nu = chu_size(d_.nz, g, q == 0 ? 0 : 1);
+            fac += (2 * nb * nb + nu * nu) * 8.0;
+        }
+    out[6] = fac + 10 * nc * vb;
+}
+
+}  // namespace tarang

@@ -1,0 +1,87 @@
+// channel3d_solver.hpp -- 3-D Rayleigh-Benard convection on RealFourier(x) x
+// RealFourier(y) x ChebyshevT(z) with the Chebyshev-tau method, per-mode
+// multistep (SBDF1/SBDF2) stepping on the GPU.  Per (kx, ky) mode the implicit
+// system is solved through three FP64 systems that depend on |k_h| only (buoyancy,
+// the velocity normal to k_h, and the u-p system along k_h), each split by parity;
+// modes with equal |k_h|^2 share factors.
+#pragma once
+#include <vector>
+
+#include "fourier_solver.hpp"
+#include "runtime.hpp"
+
+namespace tarang {
+
+struct Channel3Desc {
+    int nx, ny, nz;
+    double Lx, Ly, Lz;
+    double kappa, nu;
+    double dt;
+    int dtype;               // kF32 / kF64 (state, RHS, stage arithmetic; solves are FP64)
+    int stepper;             // kSBDF2 / kSBDF1
+    double b_bottom, b_top;  // Dirichlet values of b at z = 0 and z = Lz
+};
+
+enum Channel3Var : int { kC3P = 0, kC3B = 1, kC3UX = 2, kC3UY = 3, kC3UZ = 4 };
+
+class Channel3Solver {
+  public:
+    Channel3Solver(Context* ctx, const Channel3Desc& d);
+    ~Channel3Solver();
+    size_t field_elems() const { return (size_t)nmodes_ * d_.nz; }
+    size_t field_bytes() const { return field_elems() * 2 * dtype_size((DType)d_.dtype); }
+    int nkeys() const { return nkeys_; }
+    // coefficient arrays in the reference layout (nx/2+1, ny, nz), device pointers
+    void set_field(int var, const void* dev);
+    void get_field(int var, void* dev);
+    // tau t (0 tau_p, 1 tau_b1, 2 tau_b2, 3-5 tau_u1.{x,y,z}, 6-8 tau_u2.{x,y,z}): (nx/2+1, ny) values
+    void get_tau(int t, void* dev);
+    void rhs(void* Fb, void* Fux, void* Fuy, void* Fuz);
+    void step(int nsteps);
+    void set_dt(double dt);
+    void enable_graph(bool on) { graph_on_ = on; }
+    double time() const { return t_; }
+    long iteration() const { return iter_; }
+    // [z-backward, y-backward, x-pass, y-forward, z-forward, multistep rhs, solve]
+    static constexpr int kKinds = 7;
+    void profile(int nsteps, double* avg_ms, long* counts);
+    void kernel_bytes(double* out) const;
+
+  private:
+    template <typename T> void rhs_t(void* F0, void* F1, void* F2, void* F3);
+    template <typename T> void sbdf_t(bool first);
+    template <typename T> void solve_t();
+    template <typename T> void transpose_t(const void* in, void* out, int rows, int cols);
+    double implicit_c(bool first) const;
+    bool first_step() const { return iter_ == 0 || d_.stepper == kSBDF1; }
+    void factor(double c);
+    void launch_step_body();
+    void step_once(bool allow_graph);
+    template <typename F> void timed(int kind, F&& launch);
+    void invalidate_graph() {
+        if (graph_) (void)hipGraphExecDestroy(graph_);
+        graph_ = nullptr;
+    }
+    void* field_ptr(int var) const;
+
+    Context* ctx_;
+    Channel3Desc d_;
+    int nxh_, nmodes_, mx_, my_, nb_max_, nu_max_, nkeys_ = 0;
+    double factored_c_ = -1, dt_prev_ = 0, t_ = 0;
+    long iter_ = 0;
+    DevBuf P_, B_, UX_, UY_, UZ_, TAU_;
+    DevBuf F_[4], Fm1_[4], MXm1_[4], R_[5];
+    DevBuf G_, Q_, H_;
+    DevBuf BT_, BDT_, FT_;
+    DevBuf kh_, order_, key_, rot_;
+    DevBuf Ab_, Ap_, Aup_, permb_, permp_, permu_, info_;
+    bool graph_on_ = true;
+    hipGraphExec_t graph_ = nullptr;
+    double graph_dt_ = 0, graph_dtp_ = 0;
+    bool graph_first_ = false;
+    bool profiling_ = false;
+    struct Mark { int kind; hipEvent_t a, b; };
+    std::vector<Mark> marks_;
+};
+
+}  // namespace tarang

@@ -10,6 +10,9 @@
 //   (Dzz c)_k = (1/c_k) sum_{j>=k+2, j-k even} j (j^2 - k^2) c_j * (2/Lz)^2
 // (src/core/basis/basis_operators.jl:826-852; the second is Dz applied twice).
 #pragma once
+#include <cmath>
+#include <vector>
+
 #include "fft_lds.hpp"
 #include "fourier_kernels.hpp"
 
@@ -22,6 +25,43 @@ __host__ __device__ inline double cheb_dz(int k, int j, double s) {  // s = 2/Lz
 __host__ __device__ inline double cheb_dzz(int k, int j, double s) {
     if (j < k + 2 || ((j - k) & 1)) return 0.0;
     return (k == 0 ? 0.5 : 1.0) * (double)j * ((double)j * j - (double)k * k) * s * s;
+}
+
+// Chebyshev matrices on the Gauss-Lobatto grid z_j = (1 - cos(pi j/(N-1))) Lz/2:
+//   Bm[z][k] = T_k at point z = (-1)^k cos(pi k z/(N-1))             (backward)
+//   Fm[k][z] = (-1)^k w_k/(N-1) * (DCT-I kernel)                       (forward, its inverse)
+//   BD = Bm * Dz                                                        (backward of the z derivative)
+inline void chebyshev_tables(int N, double Lz, std::vector<double>& BT, std::vector<double>& BDT, std::vector<double>& FT) {
+    const long period = 2L * (N - 1);
+    auto cospi = [&](long num) {
+        const long r = ((num % period) + period) % period;
+        return (double)cosl(3.14159265358979323846264338327950288L * (long double)r / (long double)(N - 1));
+    };
+    std::vector<double> Bm((size_t)N * N), Fm((size_t)N * N);
+    for (int z = 0; z < N; ++z)
+        for (int k = 0; k < N; ++k) Bm[(size_t)z * N + k] = ((k & 1) ? -1.0 : 1.0) * cospi((long)k * z);
+    for (int k = 0; k < N; ++k)
+        for (int z = 0; z < N; ++z) {
+            const double sgn = (k & 1) ? -1.0 : 1.0;
+            const double wk = (k == 0 || k == N - 1) ? 0.5 : 1.0;
+            double e;
+            if (z == 0) e = 1.0;
+            else if (z == N - 1) e = sgn;
+            else e = 2.0 * cospi((long)z * k);
+            Fm[(size_t)k * N + z] = sgn * wk / (N - 1) * e;
+        }
+    const double s = 2.0 / Lz;
+    BT.assign((size_t)N * N, 0.0);
+    BDT.assign((size_t)N * N, 0.0);
+    FT.assign((size_t)N * N, 0.0);
+    for (int z = 0; z < N; ++z)
+        for (int k = 0; k < N; ++k) {
+            BT[(size_t)k * N + z] = Bm[(size_t)z * N + k];
+            FT[(size_t)z * N + k] = Fm[(size_t)k * N + z];
+            long double acc = 0;  // (Bm Dz)[z][k] = sum_{q<k} Bm[z][q] Dz[q][k]
+            for (int q = 0; q < k; ++q) acc += (long double)Bm[(size_t)z * N + q] * cheb_dz(q, k, s);
+            BDT[(size_t)k * N + z] = (double)acc;
+        }
 }
 
 // ---------------------------------------------------------------------------
@@ -177,14 +217,16 @@ __global__ __launch_bounds__(NT) void k_ch_xpass(const cplx<T>* __restrict__ G, 
 }
 
 // ---------------------------------------------------------------------------
-// k_ch_zfwd: z-forward Chebyshev transform of the three products into
-// mode-major F arrays; the kx = Nx/2 column takes the x Nyquist fold 2 Re(H).
+// k_ch_zfwd: z-forward Chebyshev transform of the products (grid (tiles, 3 or 4))
+// into mode-major F arrays; the kx = Nx/2 column takes the x Nyquist fold 2 Re(H)
+// (nx odd switches the fold off: the 3-D path folds in its y pass).
 // matT = transposed forward matrix (matT[z*N + k]).
 // ---------------------------------------------------------------------------
 template <typename T, int TK, int NT>
 __global__ __launch_bounds__(NT) void k_ch_zfwd(const cplx<T>* __restrict__ H, int nxh, int nx, int N,
                                                 const T* __restrict__ FT, cplx<T>* __restrict__ F0,
-                                                cplx<T>* __restrict__ F1, cplx<T>* __restrict__ F2) {
+                                                cplx<T>* __restrict__ F1, cplx<T>* __restrict__ F2,
+                                                cplx<T>* __restrict__ F3 = nullptr) {
     using V = cplx<T>;
     extern __shared__ unsigned char smem_raw[];
     V* src = reinterpret_cast<V*>(smem_raw);  // [TK][N]
@@ -201,7 +243,7 @@ __global__ __launch_bounds__(NT) void k_ch_zfwd(const cplx<T>* __restrict__ H, i
         src[c * N + z] = v;
     }
     __syncthreads();
-    V* out = f == 0 ? F0 : (f == 1 ? F1 : F2);
+    V* out = f == 0 ? F0 : (f == 1 ? F1 : (f == 2 ? F2 : F3));
     for (int k = threadIdx.x; k < N; k += NT) {
         V acc[TK];
 #pragma unroll
@@ -423,19 +465,25 @@ struct ChSys {
     double k0x, sz, kappa, nu, c, Lx, Lz;
 };
 
-__host__ __device__ inline double chb_entry(const ChSys& s, int g, int m, int row, int col) {
-    const int N = s.N, PL = (N - 1) & 1, ng = npar(N, g);
-    const double kx = s.k0x * m, c = s.c;
+// b-type system (one diffusing scalar with two lifts, Dirichlet walls), parity group g:
+// diff = kappa (buoyancy) or nu (the horizontal velocity normal to k_h in 3-D)
+__host__ __device__ inline double chb_entry_kh(int N, int g, double kh, double diff, double c, double sz, int row,
+                                               int col) {
+    const int PL = (N - 1) & 1, ng = npar(N, g);
     if (row < ng) {
         const int k = 2 * row + g;
         if (col < ng) {
             const int j = 2 * col + g;
-            return (j == k ? 1.0 + c * s.kappa * kx * kx : 0.0) - c * s.kappa * cheb_dzz(k, j, s.sz);
+            return (j == k ? 1.0 + c * diff * kh * kh : 0.0) - c * diff * cheb_dzz(k, j, sz);
         }
-        if (g == (PL ^ 1)) return -c * s.kappa * cheb_dz(k, N - 1, s.sz);
+        if (g == (PL ^ 1)) return -c * diff * cheb_dz(k, N - 1, sz);
         return k == N - 1 ? c : 0.0;
     }
     return col < ng ? c : 0.0;
+}
+
+__host__ __device__ inline double chb_entry(const ChSys& s, int g, int m, int row, int col) {
+    return chb_entry_kh(s.N, g, s.k0x * m, s.kappa, s.c, s.sz, row, col);
 }
 
 // tau slots of a u-p group in local order; state tau index: 0 tau_p, 3 tau_u1x, 4 tau_u1z, 5 tau_u2x, 6 tau_u2z
@@ -446,9 +494,11 @@ __host__ __device__ inline int chu_tau_slot(int N, int g, int t) {
     return t == 0 ? 3 : 6;
 }
 
-__host__ __device__ inline double chu_entry(const ChSys& s, int g, int m, int row, int col) {
-    const int N = s.N, ng = npar(N, g), nh = npar(N, g ^ 1);
-    const double kx = s.k0x * m, c = s.c;
+__host__ // u-p system with horizontal wavenumber kh (kx in 2-D, |k_h| in 3-D after rotating the
+// horizontal velocity onto k_h); area = Lx (2-D) or Lx Ly (3-D) in the integ(p) row
+__host__ __device__ inline double chu_entry_kh(int N, int g, bool mode0, double kh, double nu, double c, double sz,
+                                               double area, double Lz, int row, int col) {
+    const int ng = npar(N, g), nh = npar(N, g ^ 1);
     // column: 0 ux~, 1 p, 2 uz, 3 tau (j = its state slot)
     int ct, j;
     if (col < ng) { ct = 0; j = 2 * col + g; }
@@ -457,37 +507,41 @@ __host__ __device__ inline double chu_entry(const ChSys& s, int g, int m, int ro
     else { ct = 3; j = chu_tau_slot(N, g, col - 2 * ng - nh); }
     if (row < ng) {  // E3x
         const int k = 2 * row + g;
-        if (ct == 0) return (j == k ? 1.0 + c * s.nu * kx * kx : 0.0) - c * s.nu * cheb_dzz(k, j, s.sz);
-        if (ct == 1) return j == k ? c * kx : 0.0;
-        if (ct == 3 && j == 3) return -c * s.nu * cheb_dz(k, N - 1, s.sz);
+        if (ct == 0) return (j == k ? 1.0 + c * nu * kh * kh : 0.0) - c * nu * cheb_dzz(k, j, sz);
+        if (ct == 1) return j == k ? c * kh : 0.0;
+        if (ct == 3 && j == 3) return -c * nu * cheb_dz(k, N - 1, sz);
         if (ct == 3 && j == 5) return k == N - 1 ? c : 0.0;
         return 0.0;
     }
     if (row < 2 * ng) {  // E1
         const int k = 2 * (row - ng) + g;
-        if (ct == 0) return j == k ? -c * kx : 0.0;
-        if (ct == 2) return c * cheb_dz(k, j, s.sz);
+        if (ct == 0) return j == k ? -c * kh : 0.0;
+        if (ct == 2) return c * cheb_dz(k, j, sz);
         if (ct == 3 && j == 4) return k == N - 1 ? c : 0.0;
         if (ct == 3 && j == 0) return k == 0 ? c : 0.0;
         return 0.0;
     }
     if (row < 2 * ng + nh) {  // E3z
         const int k = 2 * (row - 2 * ng) + (g ^ 1);
-        if (ct == 2) return (j == k ? 1.0 + c * s.nu * kx * kx : 0.0) - c * s.nu * cheb_dzz(k, j, s.sz);
-        if (ct == 1) return c * cheb_dz(k, j, s.sz);
-        if (ct == 3 && j == 4) return -c * s.nu * cheb_dz(k, N - 1, s.sz);
+        if (ct == 2) return (j == k ? 1.0 + c * nu * kh * kh : 0.0) - c * nu * cheb_dzz(k, j, sz);
+        if (ct == 1) return c * cheb_dz(k, j, sz);
+        if (ct == 3 && j == 4) return -c * nu * cheb_dz(k, N - 1, sz);
         if (ct == 3 && j == 6) return k == N - 1 ? c : 0.0;
         return 0.0;
     }
     const int w = row - 2 * ng - nh;  // 0 ux wall part, 1 uz wall part, 2 integ(p)
     if (w == 0) return ct == 0 ? c : 0.0;
     if (w == 1) return ct == 2 ? c : 0.0;
-    if (ct == 1 && (j % 2) == 0) return c * s.Lx * (s.Lz / 2.0) * 2.0 / (1.0 - (double)j * j);
+    if (mode0 && ct == 1 && (j % 2) == 0) return c * area * (Lz / 2.0) * 2.0 / (1.0 - (double)j * j);
     return 0.0;
 }
 
+__host__ __device__ inline double chu_entry(const ChSys& s, int g, int m, int row, int col) {
+    return chu_entry_kh(s.N, g, m == 0, s.k0x * m, s.nu, s.c, s.sz, s.Lx, s.Lz, row, col);
+}
+
 // grid (2 nxh, 2): blockIdx.x = 2 m + g, blockIdx.y = 0 b system / 1 u-p system
-__global__ void k_ch_assemble(ChSys s, double* Ab, double* Aup) {
+static __global__ void k_ch_assemble(ChSys s, double* Ab, double* Aup) {
     const int w = blockIdx.x, m = w >> 1, g = w & 1;
     if (blockIdx.y == 0) {
         const int n = chb_size(s.N, g);

@@ -22,43 +22,6 @@ namespace tarang {
 namespace {
 constexpr int kTK = 4;
 
-// Chebyshev matrices on the Gauss-Lobatto grid z_j = (1 - cos(pi j/(N-1))) Lz/2:
-//   Bm[z][k] = T_k at point z = (-1)^k cos(pi k z/(N-1))             (backward)
-//   Fm[k][z] = (-1)^k w_k/(N-1) * (DCT-I kernel)                       (forward, its inverse)
-//   BD = Bm * Dz                                                        (backward of the z derivative)
-void chebyshev_tables(int N, double Lz, std::vector<double>& BT, std::vector<double>& BDT, std::vector<double>& FT) {
-    const long period = 2L * (N - 1);
-    auto cospi = [&](long num) {
-        const long r = ((num % period) + period) % period;
-        return (double)cosl(3.14159265358979323846264338327950288L * (long double)r / (long double)(N - 1));
-    };
-    std::vector<double> Bm((size_t)N * N), Fm((size_t)N * N);
-    for (int z = 0; z < N; ++z)
-        for (int k = 0; k < N; ++k) Bm[(size_t)z * N + k] = ((k & 1) ? -1.0 : 1.0) * cospi((long)k * z);
-    for (int k = 0; k < N; ++k)
-        for (int z = 0; z < N; ++z) {
-            const double sgn = (k & 1) ? -1.0 : 1.0;
-            const double wk = (k == 0 || k == N - 1) ? 0.5 : 1.0;
-            double e;
-            if (z == 0) e = 1.0;
-            else if (z == N - 1) e = sgn;
-            else e = 2.0 * cospi((long)z * k);
-            Fm[(size_t)k * N + z] = sgn * wk / (N - 1) * e;
-        }
-    const double s = 2.0 / Lz;
-    BT.assign((size_t)N * N, 0.0);
-    BDT.assign((size_t)N * N, 0.0);
-    FT.assign((size_t)N * N, 0.0);
-    for (int z = 0; z < N; ++z)
-        for (int k = 0; k < N; ++k) {
-            BT[(size_t)k * N + z] = Bm[(size_t)z * N + k];
-            FT[(size_t)z * N + k] = Fm[(size_t)k * N + z];
-            long double acc = 0;  // (Bm Dz)[z][k] = sum_{q<k} Bm[z][q] Dz[q][k]
-            for (int q = 0; q < k; ++q) acc += (long double)Bm[(size_t)z * N + q] * cheb_dz(q, k, s);
-            BDT[(size_t)k * N + z] = (double)acc;
-        }
-}
-
 bool channel_nx_ok(int nx, bool f64) {
     bool ok = false;
     dispatch_size(nx, [&](auto) { ok = true; }, Sizes2D{});

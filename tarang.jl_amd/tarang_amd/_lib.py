@@ -65,6 +65,27 @@ class ChanDesc(ctypes.Structure):
 
 CHAN_P, CHAN_B, CHAN_UX, CHAN_UZ = range(4)
 
+
+class Chan3Desc(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int),
+        ("ny", ctypes.c_int),
+        ("nz", ctypes.c_int),
+        ("Lx", ctypes.c_double),
+        ("Ly", ctypes.c_double),
+        ("Lz", ctypes.c_double),
+        ("kappa", ctypes.c_double),
+        ("nu", ctypes.c_double),
+        ("dt", ctypes.c_double),
+        ("dtype", ctypes.c_int),
+        ("timestepper", ctypes.c_int),
+        ("b_bottom", ctypes.c_double),
+        ("b_top", ctypes.c_double),
+    ]
+
+
+CHAN3_P, CHAN3_B, CHAN3_UX, CHAN3_UY, CHAN3_UZ = range(5)
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int
 _D = ctypes.c_double
@@ -115,6 +136,19 @@ SIGNATURES = {
     "tarang_chan_time": [_P, _PD, ctypes.POINTER(ctypes.c_long)],
     "tarang_chan_profile": [_P, _I, _PD, ctypes.POINTER(ctypes.c_long)],
     "tarang_chan_kernel_bytes": [_P, _PD],
+    "tarang_chan3_create": [_P, ctypes.POINTER(Chan3Desc), _PP],
+    "tarang_chan3_destroy": [_P],
+    "tarang_chan3_set_field": [_P, _I, _P],
+    "tarang_chan3_get_field": [_P, _I, _P],
+    "tarang_chan3_get_tau": [_P, _I, _P],
+    "tarang_chan3_rhs": [_P, _P, _P, _P, _P],
+    "tarang_chan3_step": [_P, _I],
+    "tarang_chan3_set_graph": [_P, _I],
+    "tarang_chan3_set_dt": [_P, _D],
+    "tarang_chan3_time": [_P, _PD, ctypes.POINTER(ctypes.c_long)],
+    "tarang_chan3_num_keys": [_P, _PI],
+    "tarang_chan3_profile": [_P, _I, _PD, ctypes.POINTER(ctypes.c_long)],
+    "tarang_chan3_kernel_bytes": [_P, _PD],
 }
 
 _lib = None

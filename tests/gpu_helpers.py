@@ -82,6 +82,19 @@ class Device:
         return Chan(self, h, nx, nz, dtype)
 
 
+    def chan3(self, nx, ny, nz, Lx, Ly, Lz, kappa, nu, dt, stepper=2, dtype=np.float64, b_bottom=None, b_top=0.0):
+        d = _lib.Chan3Desc()
+        d.nx, d.ny, d.nz, d.Lx, d.Ly, d.Lz = nx, ny, nz, Lx, Ly, Lz
+        d.kappa, d.nu, d.dt = kappa, nu, dt
+        d.dtype = _lib.F32 if dtype == np.float32 else _lib.F64
+        d.timestepper = stepper
+        d.b_bottom = Lz if b_bottom is None else b_bottom
+        d.b_top = b_top
+        h = ctypes.c_void_p()
+        _lib.call("tarang_chan3_create", self.ctx, ctypes.byref(d), ctypes.byref(h))
+        return Chan3(self, h, nx, ny, nz, dtype)
+
+
 class Plan:
     def __init__(self, dev, h, sizes, dtype):
         self.dev, self.h, self.sizes, self.dtype = dev, h, tuple(sizes), dtype
@@ -214,3 +227,63 @@ class Chan:
 
     def close(self):
         _lib.call("tarang_chan_destroy", self.h)
+
+
+class Chan3(Chan):
+    """3-D Rayleigh-Benard channel solver handle (tarang_chan3_*)."""
+
+    def __init__(self, dev, h, nx, ny, nz, dtype):
+        self.dev, self.h = dev, h
+        self.ct = torch.complex64 if dtype == np.float32 else torch.complex128
+        self.cshape = (nx // 2 + 1, ny, nz)
+
+    def set(self, var, c):
+        cd = to_dev(c, self.ct, self.dev.torch_device)
+        self.dev.sync()
+        _lib.call("tarang_chan3_set_field", self.h, int(var), ptr(cd))
+        self.dev.sync()
+
+    def get(self, var):
+        c = self._empty()
+        self.dev.sync()
+        _lib.call("tarang_chan3_get_field", self.h, int(var), ptr(c))
+        self.dev.sync()
+        return from_dev(c)
+
+    def tau(self, t):
+        v = torch.zeros(tuple(reversed(self.cshape[:2])), dtype=self.ct, device=self.dev.torch_device)
+        self.dev.sync()
+        _lib.call("tarang_chan3_get_tau", self.h, int(t), ptr(v))
+        self.dev.sync()
+        return from_dev(v)
+
+    def rhs(self):
+        outs = [self._empty() for _ in range(4)]
+        self.dev.sync()
+        _lib.call("tarang_chan3_rhs", self.h, *[ptr(o) for o in outs])
+        self.dev.sync()
+        return [from_dev(o) for o in outs]
+
+    def step(self, n):
+        _lib.call("tarang_chan3_step", self.h, int(n))
+        self.dev.sync()
+
+    def set_graph(self, on):
+        _lib.call("tarang_chan3_set_graph", self.h, 1 if on else 0)
+
+    def set_dt(self, dt):
+        _lib.call("tarang_chan3_set_dt", self.h, float(dt))
+
+    def nkeys(self):
+        k = ctypes.c_int()
+        _lib.call("tarang_chan3_num_keys", self.h, ctypes.byref(k))
+        return k.value
+
+    def profile(self, n):
+        ms = (ctypes.c_double * 7)()
+        cnt = (ctypes.c_long * 7)()
+        _lib.call("tarang_chan3_profile", self.h, int(n), ms, cnt)
+        return list(ms), list(cnt)
+
+    def close(self):
+        _lib.call("tarang_chan3_destroy", self.h)

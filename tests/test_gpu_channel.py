@@ -206,7 +206,7 @@ def test_channel_rejects_unsupported(gpu):
         gpu.chan(32, 600, 4.0, 1.0, kappa, nu, 1e-2)
     assert e.value.code == _lib.ERR_INVALID
     with pytest.raises(_lib.TarangError) as e:
-        gpu.chan(32, 16, 4.0, 1.0, kappa, nu, 1e-2, stepper=2)
+        gpu.chan(32, 16, 4.0, 1.0, kappa, nu, 1e-2, stepper=3)  # RK111: no explicit first stage
     assert e.value.code == _lib.ERR_INVALID
     with pytest.raises(_lib.TarangError):
         gpu.chan(2048, 16, 4.0, 1.0, kappa, nu, 1e-2, dtype=np.float64)

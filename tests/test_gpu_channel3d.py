@@ -1,0 +1,169 @@
+"""3-D Rayleigh-Benard channel on the GPU (tarang_chan3_*) vs the FP64 oracle.
+
+Tolerances: FP64 1e-12 relative (max-norm against the largest oracle entry) for the
+right-hand side and steps, FP32 1e-4 (1e-5 x10 for Chebyshev sums in FP32).  At the
+256 x 256 x 128 workload the check is mode-wise against a small oracle grid on a
+small-amplitude state confined to modes kx, ky in {0, 1}, whose products are alias-free
+on both grids.
+"""
+import numpy as np
+import pytest
+
+from tarang_amd import _lib
+from tests.conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+TOL = {np.float64: 1e-12, np.float32: 1e-4}
+
+
+def zgrid(N, Lz):
+    return Lz * (1 - np.cos(np.pi * np.arange(N) / (N - 1))) / 2
+
+
+def random_state3(O, nx, ny, N, Lx, Ly, Lz, seed=2, raw=1e-3):
+    rng = np.random.default_rng(seed)
+    d = O.domain([O.REALFOURIER, O.REALFOURIER, O.CHEBYSHEVT], [nx, ny, N],
+                 bounds=[(0.0, Lx), (0.0, Ly), (0.0, Lz)])
+    z = zgrid(N, Lz)
+    w = (z * (Lz - z))[None, None, :]
+    out = {}
+    for v in (O.RBC3_B, O.RBC3_UX, O.RBC3_UY, O.RBC3_UZ):
+        g = 0.05 * rng.standard_normal((nx, ny, N)) * w + ((Lz - z)[None, None, :] if v == O.RBC3_B else 0.0)
+        c = np.asarray(O.forward_transform(d, g))
+        if raw:
+            c = c + raw * (rng.standard_normal(c.shape) + 1j * rng.standard_normal(c.shape))
+        out[v] = c
+    return out, d
+
+
+def kp():
+    return 1e5 ** -0.5, 1e5 ** -0.5
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("nx,ny,N", [(16, 8, 12), (32, 16, 17)])
+def test_channel3d_rhs_parity(gpu, oracle, dtype, nx, ny, N):
+    print(f"channel3d rhs {nx}x{ny}x{N} {dtype.__name__}", flush=True)
+    Lx, Ly, Lz = 4.0, 3.0, 1.0
+    kappa, nu = kp()
+    st, d = random_state3(oracle, nx, ny, N, Lx, Ly, Lz)
+    ref = oracle.RBC3D(nx, ny, N, Lx, Ly, Lz, kappa, nu, 5e-3, "SBDF2")
+    for v, c in st.items():  # the oracle's rhs takes the grid-round-tripped state
+        ref.set(v, oracle.forward_transform(d, oracle.backward_transform(d, c)))
+    Fr = ref.rhs()
+    s = gpu.chan3(nx, ny, N, Lx, Ly, Lz, kappa, nu, 5e-3, dtype=dtype)
+    for v, c in st.items():
+        s.set(v, c)
+    Fg = s.rhs()
+    for a, b in zip(Fg, Fr):
+        assert relerr(a, b) < TOL[dtype]
+    s.close()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("ts", ["SBDF2", "SBDF1"])
+def test_channel3d_steps_parity(gpu, oracle, ts, dtype):
+    nx, ny, N, Lx, Ly, Lz = 16, 8, 12, 4.0, 3.0, 1.0
+    print(f"channel3d steps {ts} {dtype.__name__}", flush=True)
+    kappa, nu = kp()
+    st, d = random_state3(oracle, nx, ny, N, Lx, Ly, Lz, seed=5)
+    ref = oracle.RBC3D(nx, ny, N, Lx, Ly, Lz, kappa, nu, 5e-3, ts)
+    s = gpu.chan3(nx, ny, N, Lx, Ly, Lz, kappa, nu, 5e-3, stepper={"SBDF2": 2, "SBDF1": 4}[ts], dtype=dtype)
+    for v, c in st.items():
+        ref.set(v, c)
+        s.set(v, c)
+    for nsteps, dt in ((1, 5e-3), (2, 5e-3), (2, 2.5e-3)):
+        ref.set_dt(dt)
+        s.set_dt(dt)
+        ref.step(nsteps)
+        s.step(nsteps)
+        for v in (oracle.RBC3_B, oracle.RBC3_UX, oracle.RBC3_UY, oracle.RBC3_UZ):
+            assert relerr(s.get(v), ref.get(v)) < TOL[dtype], (nsteps, dt, v)
+        assert relerr(s.get(oracle.RBC3_P), ref.get(oracle.RBC3_P)) < TOL[dtype] * 100
+        for t in range(9):
+            r = ref.tau(t)
+            if np.abs(r).max() > 0:
+                assert relerr(s.tau(t), r) < TOL[dtype] * 100, t
+    s.close()
+
+
+def test_channel3d_graph_and_keys(gpu, oracle):
+    nx, ny, N, L = 16, 16, 9, 2 * np.pi
+    kappa, nu = kp()
+    st, _ = random_state3(oracle, nx, ny, N, L, L, 1.0, seed=9)
+    out = []
+    for graph in (False, True):
+        s = gpu.chan3(nx, ny, N, L, L, 1.0, kappa, nu, 5e-3)
+        # Lx == Ly: modes share factors by m^2 + k^2
+        n_sq = {m * m + k * k for m in range(nx // 2 + 1) for k in range(-ny // 2, ny // 2)}
+        assert s.nkeys() == len(n_sq)
+        s.set_graph(graph)
+        for v, c in st.items():
+            s.set(v, c)
+        s.step(3)
+        out.append([s.get(v) for v in (oracle.RBC3_B, oracle.RBC3_UX, oracle.RBC3_UY, oracle.RBC3_UZ)])
+        s.close()
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_channel3d_full_size_modewise(gpu, oracle, dtype):
+    """256 x 256 x 128 (the 3-D workload) vs a 16 x 16 x 128 oracle on modes kx, ky in {0, 1}."""
+    nx = ny = 256
+    N, Lx, Ly, Lz, dt = 128, 4.0, 4.0, 1.0, 2e-3
+    print(f"channel3d full size {dtype.__name__}", flush=True)
+    Ra, Pr = 1e7, 0.7
+    kappa, nu = (Ra * Pr) ** -0.5, (Ra / Pr) ** -0.5
+    small = 16
+    rng = np.random.default_rng(4)
+    ds = oracle.domain([oracle.REALFOURIER, oracle.REALFOURIER, oracle.CHEBYSHEVT], [small, small, N],
+                       bounds=[(0.0, Lx), (0.0, Ly), (0.0, Lz)])
+    z = zgrid(N, Lz)
+    wall = z * (Lz - z)
+    st = {}
+    for v in (oracle.RBC3_B, oracle.RBC3_UX, oracle.RBC3_UY, oracle.RBC3_UZ):
+        c = np.zeros((small // 2 + 1, small, N), complex)
+        if v == oracle.RBC3_B:
+            c = np.asarray(oracle.forward_transform(ds, np.tile((Lz - z)[None, None, :], (small, small, 1))))
+        for (m, n) in ((1, 0), (0, 1), (1, 1), (1, small - 1)):
+            prof = np.polynomial.chebyshev.chebfit(2 * z / Lz - 1, wall * np.sin(2 * np.pi * z / Lz), 30)
+            c[m, n, :31] += 1e-6 * small * small * prof * np.exp(1j * rng.uniform(0, 2 * np.pi))
+        if v != oracle.RBC3_B:
+            c[0, 1] = 0.0   # keep the kx = 0 line Hermitian: (0, 1) and (0, -1) together
+        st[v] = c
+    for v in st:
+        st[v][0, small - 1] = np.conj(st[v][0, 1])
+    ref = oracle.RBC3D(small, small, N, Lx, Ly, Lz, kappa, nu, dt, "SBDF2")
+    for v, c in st.items():
+        ref.set(v, c)
+    s = gpu.chan3(nx, ny, N, Lx, Ly, Lz, kappa, nu, dt, stepper=2, dtype=dtype)
+    scale = (nx / small) * (ny / small)
+    for v, c in st.items():
+        big = np.zeros((nx // 2 + 1, ny, N), complex)
+        big[:small // 2, :small // 2] = c[:small // 2, :small // 2] * scale
+        big[:small // 2, ny - small // 2 + 1:] = c[:small // 2, small // 2 + 1:] * scale
+        s.set(v, big)
+    ref.step(3)
+    s.step(3)
+    tol = {np.float64: 1e-10, np.float32: 1e-4}[dtype]
+    for v in (oracle.RBC3_B, oracle.RBC3_UX, oracle.RBC3_UY, oracle.RBC3_UZ):
+        g = s.get(v) / scale
+        r = ref.get(v)
+        pert = max(np.abs(r[1, 1]).max(), np.abs(r[1, 0]).max())
+        for (m, n, ns) in ((0, 0, 0), (1, 0, 0), (1, 1, 1), (1, ny - 1, small - 1)):
+            den = max(np.abs(r[m, ns]).max(), pert)
+            assert np.abs(g[m, n] - r[m, ns]).max() / den < tol, (v, m, n)
+    s.close()
+
+
+def test_channel3d_rejects_unsupported(gpu):
+    kappa, nu = kp()
+    with pytest.raises(_lib.TarangError) as e:
+        gpu.chan3(24, 16, 9, 4.0, 4.0, 1.0, kappa, nu, 1e-2)
+    assert e.value.code == _lib.ERR_INVALID
+    with pytest.raises(_lib.TarangError):
+        gpu.chan3(16, 16, 300, 4.0, 4.0, 1.0, kappa, nu, 1e-2)
+    with pytest.raises(_lib.TarangError):
+        gpu.chan3(16, 16, 9, 4.0, 4.0, 1.0, kappa, nu, 1e-2, stepper=0)
