@@ -43,6 +43,7 @@ typedef struct tarang_plan_s* tarang_plan_t;
 typedef struct tarang_fivp_s* tarang_fivp_t;
 typedef struct tarang_chan_s* tarang_chan_t;
 typedef struct tarang_chan3_s* tarang_chan3_t;
+typedef struct tarang_server_s* tarang_server_t;
 
 /* ------------------------------------------------------------------------ */
 /* Context and memory.  Replaces the device hooks of the architecture layer:
@@ -237,6 +238,41 @@ int tarang_chan3_num_keys(tarang_chan3_t s, int* nkeys);
  * multistep rhs, per-mode solve] average ms and counts.  No reference counterpart. */
 int tarang_chan3_profile(tarang_chan3_t s, int nsteps, double* avg_ms7, long* counts7);
 int tarang_chan3_kernel_bytes(tarang_chan3_t s, double* bytes7);
+
+/* ------------------------------------------------------------------------ */
+/* Serving loop: continuous batching of IVP solve requests on one GPU.  No reference
+ * counterpart (the reference runs one IVP per process).  Requests arrive at given
+ * times (seconds after tarang_serve_run starts); an arrived request is admitted when
+ * its device footprint fits mem_budget beside the resident ones, is created on the
+ * least-loaded of `lanes` HIP streams, keeps its state and factored systems in HBM
+ * and advances chunk_steps steps at a time, overlapping with the other active
+ * requests.  A finished request's final state is copied to `out` and its memory
+ * released, which admits waiting requests.  A request whose footprint exceeds the
+ * budget is rejected (status -1).  ic / out are device pointers in the reference
+ * layout: periodic: [0] state; 2-D channel: p, b, ux, uz; 3-D channel: p, b, ux, uy, uz
+ * (a NULL ic entry keeps that field zero). */
+#define TARANG_SERVE_PERIODIC 0
+#define TARANG_SERVE_CHANNEL 1
+#define TARANG_SERVE_CHANNEL3 2
+typedef struct {
+    int kind;
+    tarang_fivp_desc fivp;
+    tarang_chan_desc chan;
+    tarang_chan3_desc chan3;
+    int nsteps;
+    double arrival;
+    const void* ic[5];
+    void* out[5];
+} tarang_serve_request;
+
+int tarang_serve_create(int device, int lanes, size_t mem_budget, int chunk_steps, tarang_server_t* out);
+int tarang_serve_destroy(tarang_server_t s);
+int tarang_serve_submit(tarang_server_t s, const tarang_serve_request* r, int* id);
+/* process every submitted request; returns when all are done or rejected */
+int tarang_serve_run(tarang_server_t s);
+/* times3 = (arrival, admitted, done) in seconds (-1 if never admitted); status 2 done, -1 rejected */
+int tarang_serve_record(tarang_server_t s, int id, double* times3, double* grid_point_steps, size_t* footprint,
+                        int* lane, int* status);
 
 #ifdef __cplusplus
 }

@@ -8,6 +8,7 @@
 #include "channel_solver.hpp"
 #include "fourier_solver.hpp"
 #include "runtime.hpp"
+#include "serving.hpp"
 #include "transforms.hpp"
 
 struct tarang_ctx_s {
@@ -24,6 +25,9 @@ struct tarang_chan_s {
 };
 struct tarang_chan3_s {
     std::unique_ptr<tarang::Channel3Solver> solver;
+};
+struct tarang_server_s {
+    std::unique_ptr<tarang::Server> server;
 };
 
 namespace {
@@ -48,6 +52,62 @@ int guard(F&& f) {
     }
 }
 #define NEED(p, what) TG_REQUIRE((p) != nullptr, tarang::kErrInvalid, what " is NULL")
+
+tarang::FourierDesc conv_desc(const tarang_fivp_desc* desc) {
+    tarang::FourierDesc d;
+    d.ndim = desc->ndim;
+    d.n[0] = desc->n[0];
+    d.n[1] = desc->ndim > 1 ? desc->n[1] : 1;
+    d.L[0] = desc->L[0];
+    d.L[1] = desc->ndim > 1 ? desc->L[1] : 1.0;
+    d.dtype = desc->dtype;
+    d.problem = desc->problem;
+    d.stepper = desc->timestepper;
+    d.nu = desc->nu;
+    d.dt = desc->dt;
+    TG_REQUIRE(d.L[0] > 0 && d.L[1] > 0, tarang::kErrInvalid, "domain length must be positive");
+    return d;
+}
+
+tarang::ChannelDesc conv_desc(const tarang_chan_desc* desc) {
+    tarang::ChannelDesc d;
+    d.nx = desc->nx;
+    d.nz = desc->nz;
+    d.Lx = desc->Lx;
+    d.Lz = desc->Lz;
+    d.kappa = desc->kappa;
+    d.nu = desc->nu;
+    d.dt = desc->dt;
+    d.dtype = desc->dtype;
+    d.b_bottom = desc->b_bottom;
+    d.b_top = desc->b_top;
+    switch (desc->timestepper) {
+        case 0: d.stepper = tarang::kRK222; break;
+        case 1: d.stepper = tarang::kRK443; break;
+        case 2: d.stepper = tarang::kSBDF2; break;
+        case 4: d.stepper = tarang::kSBDF1; break;
+        default: d.stepper = -1; break;
+    }
+    return d;
+}
+
+tarang::Channel3Desc conv_desc(const tarang_chan3_desc* desc) {
+    tarang::Channel3Desc d;
+    d.nx = desc->nx;
+    d.ny = desc->ny;
+    d.nz = desc->nz;
+    d.Lx = desc->Lx;
+    d.Ly = desc->Ly;
+    d.Lz = desc->Lz;
+    d.kappa = desc->kappa;
+    d.nu = desc->nu;
+    d.dt = desc->dt;
+    d.dtype = desc->dtype;
+    d.b_bottom = desc->b_bottom;
+    d.b_top = desc->b_top;
+    d.stepper = desc->timestepper == 2 ? tarang::kSBDF2 : (desc->timestepper == 4 ? tarang::kSBDF1 : -1);
+    return d;
+}
 }  // namespace
 
 extern "C" {
@@ -210,20 +270,8 @@ int tarang_fivp_create(tarang_ctx_t ctx, const tarang_fivp_desc* desc, tarang_fi
         NEED(ctx, "ctx");
         NEED(desc, "desc");
         NEED(out, "out");
-        tarang::FourierDesc d;
-        d.ndim = desc->ndim;
-        d.n[0] = desc->n[0];
-        d.n[1] = desc->ndim > 1 ? desc->n[1] : 1;
-        d.L[0] = desc->L[0];
-        d.L[1] = desc->ndim > 1 ? desc->L[1] : 1.0;
-        d.dtype = desc->dtype;
-        d.problem = desc->problem;
-        d.stepper = desc->timestepper;
-        d.nu = desc->nu;
-        d.dt = desc->dt;
-        TG_REQUIRE(d.L[0] > 0 && d.L[1] > 0, tarang::kErrInvalid, "domain length must be positive");
         auto s = std::make_unique<tarang_fivp_s>();
-        s->solver = std::make_unique<tarang::FourierSolver>(&ctx->ctx, d);
+        s->solver = std::make_unique<tarang::FourierSolver>(&ctx->ctx, conv_desc(desc));
         *out = s.release();
     });
 }
@@ -345,26 +393,8 @@ int tarang_chan_create(tarang_ctx_t ctx, const tarang_chan_desc* desc, tarang_ch
         NEED(desc, "desc");
         NEED(out, "out");
         *out = nullptr;
-        tarang::ChannelDesc d;
-        d.nx = desc->nx;
-        d.nz = desc->nz;
-        d.Lx = desc->Lx;
-        d.Lz = desc->Lz;
-        d.kappa = desc->kappa;
-        d.nu = desc->nu;
-        d.dt = desc->dt;
-        d.dtype = desc->dtype;
-        d.b_bottom = desc->b_bottom;
-        d.b_top = desc->b_top;
-        switch (desc->timestepper) {
-            case 0: d.stepper = tarang::kRK222; break;
-            case 1: d.stepper = tarang::kRK443; break;
-            case 2: d.stepper = tarang::kSBDF2; break;
-            case 4: d.stepper = tarang::kSBDF1; break;
-            default: d.stepper = -1; break;
-        }
         auto h = std::make_unique<tarang_chan_s>();
-        h->solver = std::make_unique<tarang::ChannelSolver>(&ctx->ctx, d);
+        h->solver = std::make_unique<tarang::ChannelSolver>(&ctx->ctx, conv_desc(desc));
         *out = h.release();
     });
 }
@@ -457,22 +487,8 @@ int tarang_chan3_create(tarang_ctx_t ctx, const tarang_chan3_desc* desc, tarang_
         NEED(desc, "desc");
         NEED(out, "out");
         *out = nullptr;
-        tarang::Channel3Desc d;
-        d.nx = desc->nx;
-        d.ny = desc->ny;
-        d.nz = desc->nz;
-        d.Lx = desc->Lx;
-        d.Ly = desc->Ly;
-        d.Lz = desc->Lz;
-        d.kappa = desc->kappa;
-        d.nu = desc->nu;
-        d.dt = desc->dt;
-        d.dtype = desc->dtype;
-        d.b_bottom = desc->b_bottom;
-        d.b_top = desc->b_top;
-        d.stepper = desc->timestepper == 2 ? tarang::kSBDF2 : (desc->timestepper == 4 ? tarang::kSBDF1 : -1);
         auto h = std::make_unique<tarang_chan3_s>();
-        h->solver = std::make_unique<tarang::Channel3Solver>(&ctx->ctx, d);
+        h->solver = std::make_unique<tarang::Channel3Solver>(&ctx->ctx, conv_desc(desc));
         *out = h.release();
     });
 }
@@ -563,6 +579,69 @@ int tarang_chan3_kernel_bytes(tarang_chan3_t s, double* bytes7) {
         NEED(s, "solver");
         NEED(bytes7, "bytes7");
         s->solver->kernel_bytes(bytes7);
+    });
+}
+
+// ---- serving loop ----
+int tarang_serve_create(int device, int lanes, size_t mem_budget, int chunk_steps, tarang_server_t* out) {
+    return guard([&] {
+        NEED(out, "out");
+        *out = nullptr;
+        int n = 0;
+        TG_HIP(hipGetDeviceCount(&n));
+        TG_REQUIRE(device >= 0 && device < n, tarang::kErrInvalid, "device index out of range");
+        auto h = std::make_unique<tarang_server_s>();
+        h->server = std::make_unique<tarang::Server>(device, lanes, mem_budget, chunk_steps);
+        *out = h.release();
+    });
+}
+
+int tarang_serve_destroy(tarang_server_t s) {
+    return guard([&] { delete s; });
+}
+
+int tarang_serve_submit(tarang_server_t s, const tarang_serve_request* r, int* id) {
+    return guard([&] {
+        NEED(s, "server");
+        NEED(r, "request");
+        tarang::ServeRequest q;
+        q.kind = r->kind;
+        if (r->kind == TARANG_SERVE_PERIODIC) q.fivp = conv_desc(&r->fivp);
+        else if (r->kind == TARANG_SERVE_CHANNEL) q.chan = conv_desc(&r->chan);
+        else if (r->kind == TARANG_SERVE_CHANNEL3) q.chan3 = conv_desc(&r->chan3);
+        q.nsteps = r->nsteps;
+        q.arrival = r->arrival;
+        for (int i = 0; i < 5; ++i) {
+            q.ic[i] = r->ic[i];
+            q.out[i] = r->out[i];
+        }
+        const int k = s->server->submit(q);
+        if (id) *id = k;
+    });
+}
+
+int tarang_serve_run(tarang_server_t s) {
+    return guard([&] {
+        NEED(s, "server");
+        s->server->run();
+    });
+}
+
+int tarang_serve_record(tarang_server_t s, int id, double* times3, double* grid_point_steps, size_t* footprint,
+                        int* lane, int* status) {
+    return guard([&] {
+        NEED(s, "server");
+        TG_REQUIRE(id >= 0 && id < s->server->size(), tarang::kErrInvalid, "request id out of range");
+        const tarang::ServeRecord& r = s->server->record(id);
+        if (times3) {
+            times3[0] = r.arrival;
+            times3[1] = r.admitted;
+            times3[2] = r.done;
+        }
+        if (grid_point_steps) *grid_point_steps = r.grid_point_steps;
+        if (footprint) *footprint = r.footprint;
+        if (lane) *lane = r.lane;
+        if (status) *status = r.status;
     });
 }
 

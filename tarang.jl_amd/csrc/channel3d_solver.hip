@@ -488,4 +488,15 @@ void Channel3Solver::kernel_bytes(double* out) const {
     out[6] = fac + 10 * nc * vb;
 }
 
+size_t footprint(const Channel3Desc& d) {
+    const size_t nxh = d.nx / 2 + 1, nmodes = nxh * d.ny, N = d.nz, vb = 2 * dtype_size((DType)d.dtype);
+    const size_t fb = nmodes * N * vb, prow = N * (size_t)padded_len(d.ny) * nxh * vb;
+    const size_t fields = (5 + 4 + 4 + 4 + 5 + kCh3Planes + 1) * fb + (kCh3Planes + 4) * prow;
+    const size_t nb = chb_size(d.nz, 0), nu = chu_size(d.nz, 0, 0);
+    // distinct |k_h|^2 keys: at most one per mode (about nmodes / 4 when Lx == Ly)
+    const size_t keys = d.Lx == d.Ly ? nmodes / 3 + 64 : nmodes;
+    const size_t fac = 2 * keys * (2 * nb * nb + nu * nu) * 8 + 2 * keys * (2 * nb + nu) * 4;
+    return fields + fac + 3 * N * N * 8 + (1 << 20);
+}
+
 }  // namespace tarang

@@ -513,4 +513,13 @@ void ChannelSolver::kernel_bytes(double* out) const {
     out[5] = 0;
 }
 
+size_t footprint(const ChannelDesc& d) {
+    const size_t nxh = d.nx / 2 + 1, N = d.nz, fb = nxh * N * 2 * dtype_size((DType)d.dtype);
+    const size_t S = (d.stepper == kSBDF2 || d.stepper == kSBDF1) ? 3 : (size_t)make_tableau(d.stepper).stages;
+    const size_t fields = (4 + 3 + 7 * S + 4 + 8 + 3 + 6) * fb;
+    const size_t nb = chb_size(d.nz, 0), nu = chu_size(d.nz, 0, 0);
+    const size_t fac = 2 * nxh * (nb * nb + nu * nu) * 8 + 2 * nxh * (nb + nu) * 4;
+    return fields + fac + 3 * N * N * 8 + (1 << 20);
+}
+
 }  // namespace tarang

@@ -598,4 +598,13 @@ void FourierSolver::step(int nsteps) {
     for (int i = 0; i < nsteps; ++i) step_once(true);
 }
 
+size_t footprint(const FourierDesc& d) {
+    const size_t nxh = d.n[0] / 2 + 1, ny = d.ndim > 1 ? d.n[1] : 1, my = d.ndim > 1 ? padded_len(d.n[1]) : 1;
+    const size_t vb = 2 * dtype_size((DType)d.dtype), cb = nxh * ny * vb;
+    const Tableau t = make_tableau(d.stepper);
+    const size_t state = is_multistep(d.stepper) ? 4 * cb : (1 + 2 * (size_t)t.stages) * cb;
+    const size_t work = (d.problem != kDiffusion && d.ndim == 2) ? 5 * nxh * my * vb : 0;
+    return state + work + (1 << 20);
+}
+
 }  // namespace tarang

@@ -86,6 +86,21 @@ class Chan3Desc(ctypes.Structure):
 
 CHAN3_P, CHAN3_B, CHAN3_UX, CHAN3_UY, CHAN3_UZ = range(5)
 
+SERVE_PERIODIC, SERVE_CHANNEL, SERVE_CHANNEL3 = range(3)
+
+
+class ServeRequest(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int),
+        ("fivp", FivpDesc),
+        ("chan", ChanDesc),
+        ("chan3", Chan3Desc),
+        ("nsteps", ctypes.c_int),
+        ("arrival", ctypes.c_double),
+        ("ic", ctypes.c_void_p * 5),
+        ("out", ctypes.c_void_p * 5),
+    ]
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int
 _D = ctypes.c_double
@@ -149,6 +164,11 @@ SIGNATURES = {
     "tarang_chan3_num_keys": [_P, _PI],
     "tarang_chan3_profile": [_P, _I, _PD, ctypes.POINTER(ctypes.c_long)],
     "tarang_chan3_kernel_bytes": [_P, _PD],
+    "tarang_serve_create": [_I, _I, ctypes.c_size_t, _I, _PP],
+    "tarang_serve_destroy": [_P],
+    "tarang_serve_submit": [_P, ctypes.POINTER(ServeRequest), _PI],
+    "tarang_serve_run": [_P],
+    "tarang_serve_record": [_P, _I, _PD, _PD, ctypes.POINTER(ctypes.c_size_t), _PI, _PI],
 }
 
 _lib = None

@@ -325,9 +325,9 @@ class ScalarField:
             self._g = self._c = None
             # coefficient storage of a tau variable: one value per x mode (or one value)
             if _is_gpu(domain.architecture):
-                n = self.bases[0].size // 2 + 1 if self.bases else 1
+                shape = [self.bases[0].size // 2 + 1] + [b.size for b in self.bases[1:]] if self.bases else [1]
                 dev = _Device.get(domain.architecture.device)
-                self._c = torch.zeros((n,), dtype=_torch_cplx(self.dtype), device=dev.torch_device)
+                self._c = torch.zeros(tuple(reversed(shape)), dtype=_torch_cplx(self.dtype), device=dev.torch_device)
             return
         self.domain = domain
         self.dist = domain.dist

@@ -10,7 +10,7 @@ families of the GPU timestep path and refuses anything else loudly:
                  u - skew(grad(psi)) = 0
                  integ(psi) = 0                        (boundary condition)
   burgers        dt(u) - nu*lap(u) = -u*dx(u)
-  rbc2d          Rayleigh-Benard on RealFourier(x) x ChebyshevT(z), first-order
+  rbc2d, rbc3d   Rayleigh-Benard on RealFourier(x) [x RealFourier(y)] x ChebyshevT(z), first-order
                  tau form of test/test_subproblem_rk.jl:43-56:
                    trace(grad_u) + tau_p = 0
                    dt(b) - kappa*div(grad_b) + lift(tau_b2) = -u@grad(b)
@@ -82,7 +82,7 @@ _W = r"([A-Za-z_][A-Za-z_0-9]*)"
 
 @dataclass
 class Recognized:
-    family: str                    # "diffusion" | "turb2d" | "burgers1d" | "rbc2d"
+    family: str                    # "diffusion" | "turb2d" | "burgers1d" | "rbc2d" | "rbc3d"
     evolved: str                   # evolved variable name
     coeff: float                   # implicit Laplacian coefficient
     names: Dict[str, str] = dfield(default_factory=dict)
@@ -199,7 +199,8 @@ def _recognize_rbc2d(problem: IVP) -> Optional[Recognized]:
         raise NotImplementedError("the channel GPU path supports no-slip walls u = 0 only")
     if "integ" not in bcs:
         raise NotImplementedError(f"missing gauge condition integ({p}) = 0")
-    rec = Recognized("rbc2d", b, _param(problem, kappa), names)
+    ndim = len(problem.var(b).bases)
+    rec = Recognized("rbc3d" if ndim == 3 else "rbc2d", b, _param(problem, kappa), names)
     rec.extra = {"kappa": _param(problem, kappa), "nu": _param(problem, nu),
                  "b_bottom": bcs[(b, lo)], "b_top": bcs[(b, hi)]}
     return rec

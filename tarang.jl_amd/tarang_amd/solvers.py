@@ -83,7 +83,7 @@ class InitialValueSolver:
         self.start_wall_time = time.time()
         rec = recognize(problem)
         self.recognized = rec
-        if rec.family == "rbc2d":
+        if rec.family in ("rbc2d", "rbc3d"):
             self._channel = _ChannelBackend(self, problem, rec, timestepper, use_graph)
             self.dev = self._channel.dev
             self.handle = self._channel.handle
@@ -185,13 +185,14 @@ class InitialValueSolver:
         return True
 
     def set_dt(self, dt: float):
-        _lib.call("tarang_chan_set_dt" if self._channel is not None else "tarang_fivp_set_dt", self.handle, float(dt))
+        _lib.call(self._channel.pre + "set_dt" if self._channel is not None else "tarang_fivp_set_dt", self.handle,
+                  float(dt))
         self.dt = float(dt)
 
     def close(self):
         if getattr(self, "handle", None):
-            _lib.call("tarang_chan_destroy" if getattr(self, "_channel", None) is not None else "tarang_fivp_destroy",
-                      self.handle)
+            ch = getattr(self, "_channel", None)
+            _lib.call(ch.pre + "destroy" if ch is not None else "tarang_fivp_destroy", self.handle)
             self.handle = None
 
     def __del__(self):
@@ -202,10 +203,10 @@ class InitialValueSolver:
 
 
 class _ChannelBackend:
-    """The rbc2d family on ChannelDomain: one tarang_chan_t holds the state
-    (p, b, u, taus) and the factored per-mode systems.  Fields are pushed before
-    and pulled after every batch of steps (scatter_inputs / gather_inputs!,
-    subproblem_io.jl:1143-1152)."""
+    """The Rayleigh-Benard families on ChannelDomain (2-D) and RealFourier^2 x ChebyshevT
+    (3-D): one tarang_chan_t / tarang_chan3_t holds the state (p, b, u, taus) and the
+    factored per-mode systems.  Fields are pushed before and pulled after every batch
+    of steps (scatter_inputs / gather_inputs!, subproblem_io.jl:1143-1152)."""
 
     def __init__(self, solver, problem, rec, timestepper, use_graph):
         names = rec.names
@@ -213,37 +214,48 @@ class _ChannelBackend:
         self.b = problem.var(names["b"])
         self.p = problem.var(names["p"])
         u = problem.var(names["u"])
-        if not isinstance(u, VectorField) or len(u.components) != 2:
-            raise ValueError("the channel family needs a 2-component VectorField u")
-        self.ux, self.uz = u.components
         dom = self.b.domain
+        self.three = rec.family == "rbc3d"
+        nd = 3 if self.three else 2
+        if not isinstance(u, VectorField) or len(u.components) != nd:
+            raise ValueError(f"the channel family needs a {nd}-component VectorField u")
         if dom is None or not _is_gpu(dom.arch):
             raise GPUTransformUnsupported("InitialValueSolver here runs on arch=GPU() domains only")
-        if dom.ndim != 2 or dom.bases[0].kind != _lib.REALFOURIER or dom.bases[1].kind != _lib.CHEBYSHEVT:
-            raise GPUTransformUnsupported("the channel family needs a RealFourier(x) x ChebyshevT(z) domain")
+        kinds = [bb.kind for bb in dom.bases]
+        if kinds != [_lib.REALFOURIER] * (nd - 1) + [_lib.CHEBYSHEVT]:
+            raise GPUTransformUnsupported("the channel family needs RealFourier horizontal axes and a ChebyshevT z axis")
         for bb in dom.bases:
             if bb.dealias != 1.5:
                 raise GPUTransformUnsupported("dealias must be 3/2 for the nonlinear GPU path")
-        if timestepper.code not in (0, 1):
-            raise GPUTransformUnsupported(f"{timestepper.name} is not available for the channel family "
-                                          "(RK222 and RK443 are)")
+        allowed = (2, 4) if self.three else (0, 1, 2, 4)
+        if timestepper.code not in allowed:
+            raise GPUTransformUnsupported(f"{timestepper.name} is not available for this channel family")
         self.taus = [problem.var(names[k]) for k in ("tau_p", "tau_b1", "tau_b2")]
         tu1, tu2 = problem.var(names["tau_u1"]), problem.var(names["tau_u2"])
-        self.taus += list(tu1.components) + list(tu2.components)  # slot order 0..6
+        self.taus += list(tu1.components) + list(tu2.components)  # slot order of the library
         self.dev = dom.device()
-        d = _lib.ChanDesc()
-        d.nx, d.nz = dom.bases[0].size, dom.bases[1].size
-        d.Lx, d.Lz = dom.bases[0].length, dom.bases[1].length
+        h = ctypes.c_void_p()
+        if self.three:
+            d = _lib.Chan3Desc()
+            d.nx, d.ny, d.nz = (bb.size for bb in dom.bases)
+            d.Lx, d.Ly, d.Lz = (bb.length for bb in dom.bases)
+            self.pre = "tarang_chan3_"
+            vars_ = (_lib.CHAN3_P, _lib.CHAN3_B, _lib.CHAN3_UX, _lib.CHAN3_UY, _lib.CHAN3_UZ)
+        else:
+            d = _lib.ChanDesc()
+            d.nx, d.nz = dom.bases[0].size, dom.bases[1].size
+            d.Lx, d.Lz = dom.bases[0].length, dom.bases[1].length
+            self.pre = "tarang_chan_"
+            vars_ = (_lib.CHAN_P, _lib.CHAN_B, _lib.CHAN_UX, _lib.CHAN_UZ)
         d.kappa, d.nu = rec.extra["kappa"], rec.extra["nu"]
         d.dt = solver.dt
         d.dtype = _lib.F32 if dom.dtype == np.float32 else _lib.F64
         d.timestepper = timestepper.code
         d.b_bottom, d.b_top = rec.extra["b_bottom"], rec.extra["b_top"]
-        h = ctypes.c_void_p()
-        _lib.call("tarang_chan_create", self.dev.handle, ctypes.byref(d), ctypes.byref(h))
+        _lib.call(self.pre + "create", self.dev.handle, ctypes.byref(d), ctypes.byref(h))
         self.handle = h
-        _lib.call("tarang_chan_set_graph", h, 1 if use_graph else 0)
-        self.fields = ((_lib.CHAN_P, self.p), (_lib.CHAN_B, self.b), (_lib.CHAN_UX, self.ux), (_lib.CHAN_UZ, self.uz))
+        _lib.call(self.pre + "set_graph", h, 1 if use_graph else 0)
+        self.fields = tuple(zip(vars_, [self.p, self.b] + list(u.components)))
         self.push()
 
     def push(self):
@@ -251,30 +263,32 @@ class _ChannelBackend:
             ensure_layout_(f, "c")
         self.dev.before()
         for var, f in self.fields:
-            _lib.call("tarang_chan_set_field", self.handle, var, _ptr(f._c))
+            _lib.call(self.pre + "set_field", self.handle, var, _ptr(f._c))
         self.dev.after()
 
     def pull(self):
         self.dev.before()
         for var, f in self.fields:
-            _lib.call("tarang_chan_get_field", self.handle, var, _ptr(f._c))
+            _lib.call(self.pre + "get_field", self.handle, var, _ptr(f._c))
             f.current_layout = "c"
+        nmodes = self.b._c.numel() // self.b._c.shape[0]  # (nx/2+1)[*ny] values per tau
         for t, f in enumerate(self.taus):
-            if f._c is not None and f._c.numel() == self.b._c.shape[-1]:
-                _lib.call("tarang_chan_get_tau", self.handle, t, _ptr(f._c))
+            if f._c is not None and f._c.numel() == nmodes:
+                _lib.call(self.pre + "get_tau", self.handle, t, _ptr(f._c))
         self.dev.after()
 
     def step(self, n):
         # the fields are the state: a host-side change (grid or coefficients) is always seen
         self.push()
-        _lib.call("tarang_chan_step", self.handle, int(n))
+        _lib.call(self.pre + "step", self.handle, int(n))
         self.pull()
 
     def rhs(self):
         self.push()
-        outs = [ScalarField(self.b.domain, n) for n in ("F_b", "F_ux", "F_uz")]
+        names = ("F_b", "F_ux", "F_uy", "F_uz") if self.three else ("F_b", "F_ux", "F_uz")
+        outs = [ScalarField(self.b.domain, n) for n in names]
         self.dev.before()
-        _lib.call("tarang_chan_rhs", self.handle, *[_ptr(o._c) for o in outs])
+        _lib.call(self.pre + "rhs", self.handle, *[_ptr(o._c) for o in outs])
         self.dev.after()
         for o in outs:
             o.current_layout = "c"

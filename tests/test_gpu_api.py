@@ -141,3 +141,45 @@ def test_rbc_example_through_api(oracle, dtype):
     for f in (b, u[0], u[1]):
         assert np.all(np.isfinite(f["g"].cpu().numpy()))
     solver.close()
+
+
+def test_rbc3d_through_api(oracle):
+    """The 3-D form of the tau problem (u = (ux, uy, uz)) with SBDF2 through the API mirror."""
+    Lx, Ly, Lz, Nx, Ny, Nz, Ra, Pr = 4.0, 4.0, 1.0, 16, 16, 9, 1e5, 0.7
+    kappa, nu = (Ra * Pr) ** -0.5, (Ra / Pr) ** -0.5
+    coords = T.CartesianCoordinates("x", "y", "z")
+    dist = T.Distributor(coords, dtype=np.float64, device=T.GPU())
+    xb = T.RealFourier(coords["x"], size=Nx, bounds=(0.0, Lx), dealias=3 / 2)
+    yb = T.RealFourier(coords["y"], size=Ny, bounds=(0.0, Ly), dealias=3 / 2)
+    zb = T.ChebyshevT(coords["z"], size=Nz, bounds=(0.0, Lz), dealias=3 / 2)
+    domain = T.Domain(dist, (xb, yb, zb))
+    p, b, u = T.ScalarField(domain, "p"), T.ScalarField(domain, "b"), T.VectorField(domain, "u")
+    tau_p = T.ScalarField(dist, "tau_p", (), np.float64)
+    tau_b1 = T.ScalarField(dist, "tau_b1", (xb, yb), np.float64)
+    tau_b2 = T.ScalarField(dist, "tau_b2", (xb, yb), np.float64)
+    tau_u1 = T.VectorField(dist, coords, "tau_u1", (xb, yb), np.float64)
+    tau_u2 = T.VectorField(dist, coords, "tau_u2", (xb, yb), np.float64)
+    ex, ey, ez = T.unit_vector_fields(coords, dist)
+    tau_lift = lambda A: T.lift(A, T.derivative_basis(zb, 1), -1)  # noqa: E731
+    problem = T.IVP([p, b, u, tau_p, tau_b1, tau_b2, tau_u1, tau_u2])
+    T.add_parameters_(problem, kappa=kappa, nu=nu, Lz=Lz, ez=ez, τ_lift=tau_lift,
+                      grad_u=T.grad(u) + ez * tau_lift(tau_u1), grad_b=T.grad(b) + ez * tau_lift(tau_b1))
+    T.add_equation_(problem, "trace(grad_u) + tau_p = 0")
+    T.add_equation_(problem, "∂t(b) - kappa*div(grad_b) + τ_lift(tau_b2) = -u⋅∇(b)")
+    T.add_equation_(problem, "∂t(u) - nu*div(grad_u) + ∇(p) - b*ez + τ_lift(tau_u2) = -u⋅∇(u)")
+    for bc in ("b(z=0) = Lz", "u(z=0) = 0", "b(z=Lz) = 0", "u(z=Lz) = 0", "integ(p) = 0"):
+        T.add_bc_(problem, bc)
+    solver = T.InitialValueSolver(problem, T.SBDF2(), dt=2e-3)
+    z = zb.grid()
+    rng = np.random.default_rng(7)
+    bg = (Lz - z)[None, None, :] + 1e-3 * rng.standard_normal((Nx, Ny, Nz)) * (z * (Lz - z))[None, None, :]
+    b["g"] = bg
+    T.run_(solver, stop_iteration=4)
+    ref = oracle.RBC3D(Nx, Ny, Nz, Lx, Ly, Lz, kappa, nu, 2e-3, "SBDF2")
+    ref.set(oracle.RBC3_B, oracle.forward_transform(ref.d, bg))
+    ref.step(4)
+    assert relerr(b["c"].cpu().numpy(), ref.get(oracle.RBC3_B)) < 1e-12
+    for i, v in enumerate((oracle.RBC3_UX, oracle.RBC3_UY, oracle.RBC3_UZ)):
+        assert relerr(u[i]["c"].cpu().numpy(), ref.get(v)) < 1e-11
+    assert relerr(tau_b1._c.cpu().numpy().T, ref.tau(1)) < 1e-10
+    solver.close()
