@@ -48,7 +48,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run one plain device-to-device copy of this many MiB (counter calibration)")
-    ap.add_argument("--workload", choices=["turbulence", "channel"], default="turbulence")
+    ap.add_argument("--workload", choices=["turbulence", "channel", "serving"], default="turbulence")
+    ap.add_argument("--requests", type=int, default=24, help="serving: number of requests")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (no collectives); run with --gpus 1",
@@ -56,6 +57,8 @@ def main():
         sys.exit(2)
     if a.workload == "channel":
         return bench_channel(a)
+    if a.workload == "serving":
+        return bench_serving(a)
 
     import numpy as np
     import torch
@@ -332,6 +335,105 @@ def bench_channel(a):
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "kernels_f32": kernels,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+SERVE_WORKLOAD = ("serving stream: Poisson arrivals at 80% of the measured saturation capacity; 40% 2-D turbulence "
+                  "N in {128,256,512,1024}^2 (RK443, dt=1e-3), 40% 2-D Rayleigh-Benard Nx x Nz in {128x32, 256x64, "
+                  "512x128, 1024x256}, Ra log-uniform in [1e5, 1e8], Pr=1 (RK222, dt=1e-3), 20% 1-D Burgers N in "
+                  "{64..4096}, nu=1e-2 (RK222, dt=1e-3); FP32 state with FP64 tau solves; steps log-uniform in [200, 2000]")
+
+
+def serving_stream(np, torch, _lib, configs, n, seed, dev):
+    """Seeded request list: (kind, desc, nsteps, ic tensors, output shapes, grid points)."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    cdt = torch.complex64
+
+    def to_dev(a):
+        return torch.as_tensor(np.ascontiguousarray(np.asarray(a).T), device=dev).to(cdt).contiguous()
+    for i in range(n):
+        u = rng.uniform()
+        steps = int(np.exp(rng.uniform(np.log(200), np.log(2000))))
+        if u < 0.4:
+            N = int(rng.choice([128, 256, 512, 1024]))
+            d = _lib.FivpDesc()
+            d.ndim, d.n[0], d.n[1], d.L[0], d.L[1] = 2, N, N, 2 * math.pi, 2 * math.pi
+            d.dtype, d.problem, d.timestepper, d.nu, d.dt = _lib.F32, 1, 1, 1e-4, 1e-3
+            q = configs.turbulence_2d(N, seed=int(rng.integers(1 << 30)), kpeak=10.0, energy=0.5)
+            C = np.fft.fft(np.fft.rfft(q, axis=0), axis=1)
+            reqs.append((_lib.SERVE_PERIODIC, d, steps, [to_dev(C)], [(N // 2 + 1, N)], N * N))
+        elif u < 0.8:
+            nx, nz = [(128, 32), (256, 64), (512, 128), (1024, 256)][int(rng.integers(4))]
+            Ra = float(np.exp(rng.uniform(np.log(1e5), np.log(1e8))))
+            d = _lib.ChanDesc()
+            d.nx, d.nz, d.Lx, d.Lz, d.kappa, d.nu, d.dt = nx, nz, 4.0, 1.0, Ra ** -0.5, Ra ** -0.5, 1e-3
+            d.dtype, d.timestepper, d.b_bottom, d.b_top = _lib.F32, 0, 1.0, 0.0
+            bg, _ = channel_ic(np, nx, nz, 1.0, seed=int(rng.integers(1 << 30)))
+            k = np.arange(nz)
+            T = np.cos(np.outer(np.pi * np.arange(nz) / (nz - 1), k)) * ((-1.0) ** k)[None, :]
+            Bc = np.linalg.solve(T, np.fft.rfft(bg, axis=0).T).T
+            reqs.append((_lib.SERVE_CHANNEL, d, steps, [None, to_dev(Bc), None, None], [(nx // 2 + 1, nz)] * 4,
+                         nx * nz))
+        else:
+            N = int(rng.choice([64, 128, 256, 512, 1024, 2048, 4096]))
+            d = _lib.FivpDesc()
+            d.ndim, d.n[0], d.n[1], d.L[0], d.L[1] = 1, N, 1, 2 * math.pi, 1.0
+            d.dtype, d.problem, d.timestepper, d.nu, d.dt = _lib.F32, 2, 0, 1e-2, 1e-3
+            x = 2 * math.pi * np.arange(N) / N
+            u0 = np.sin(x) + 0.1 * sum(rng.standard_normal() * np.sin(j * x + rng.uniform(0, 6.3)) for j in (2, 3, 4))
+            reqs.append((_lib.SERVE_PERIODIC, d, steps, [to_dev(np.fft.rfft(u0))], [(N // 2 + 1,)], N))
+    return reqs
+
+
+def bench_serving(a):
+    import numpy as np
+    import torch
+
+    from tarang_amd import _lib, configs
+    from tarang_amd.serving import Server, latency_stats
+
+    dev = torch.device("cuda", 0)
+    n = a.requests
+    reqs = serving_stream(np, torch, _lib, configs, n, 7, dev)
+    budget = 200 << 30
+
+    def run(arrivals):
+        srv = Server(0, lanes=4, mem_budget=budget, chunk_steps=25)
+        ids = []
+        for (kind, d, steps, ic, shapes, pts), t in zip(reqs, arrivals):
+            o = [torch.zeros(tuple(reversed(sh)), dtype=torch.complex64, device=dev) for sh in shapes]
+            ids.append(srv.submit(kind, d, steps, t, ic, o))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        srv.run()
+        el = time.perf_counter() - t0
+        recs = [srv.record(i) for i in ids]
+        srv.close()
+        return recs, el
+    log(f"serving: saturation pass, {n} requests arriving at t=0")
+    recs0, el0 = run([0.0] * n)
+    cap = n / el0
+    rate = 0.8 * cap
+    rng = np.random.default_rng(11)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n)) - 0.0
+    log(f"serving: capacity {cap:.2f} req/s, Poisson arrivals at {rate:.2f} req/s")
+    recs, el = run(list(arrivals))
+    st = latency_stats(recs)
+    total = sum(r.grid_point_steps for r in recs if r.status == 2)
+    line = {
+        "metric": "grid-point steps/s, FP32+refine vs FP64 at <=1e-6 rel err; p50/p99 latency",
+        "value": st["grid_point_steps_per_s"], "unit": "grid-point steps/s", "n_gpus": 1, "steps": None,
+        "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded initial conditions per request)",
+        "config": {"workload": SERVE_WORKLOAD, "requests": n, "lanes": 4, "chunk_steps": 25,
+                   "parallelism": "single GPU, continuous batching"},
+        "latency_s": {"p50": st["p50_s"], "p99": st["p99_s"], "mean": st["mean_s"]},
+        "saturation": {"makespan_s": el0, "capacity_req_per_s": cap,
+                       "grid_point_steps_per_s": sum(r.grid_point_steps for r in recs0) / el0},
+        "offered_rate_req_per_s": rate, "makespan_s": el, "total_grid_point_steps": total,
+        "rejected": st.get("rejected", 0),
     }
     print(json.dumps(line), flush=True)
 
