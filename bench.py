@@ -48,7 +48,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run one plain device-to-device copy of this many MiB (counter calibration)")
-    ap.add_argument("--workload", choices=["turbulence", "channel", "serving"], default="turbulence")
+    ap.add_argument("--workload", choices=["turbulence", "channel", "channel3d", "serving"], default="turbulence")
     ap.add_argument("--requests", type=int, default=24, help="serving: number of requests")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -59,6 +59,8 @@ def main():
         return bench_channel(a)
     if a.workload == "serving":
         return bench_serving(a)
+    if a.workload == "channel3d":
+        return bench_channel3d(a)
 
     import numpy as np
     import torch
@@ -332,6 +334,130 @@ def bench_channel(a):
                                  "nusselt": abs(nus["f32"] - nus["f64"]) / abs(nus["f64"]), "after_steps": steps},
         "nusselt": nus,
         "roofline": {"bound": "hbm", "kernel": CH_KERNELS[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "kernels_f32": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+CH3 = dict(nx=256, ny=256, nz=128, Lx=4.0, Ly=4.0, Lz=1.0, Ra=1e7, Pr=0.7, dt=2e-3)
+CH3_WORKLOAD = ("3-D Rayleigh-Benard, RealFourier^2 x ChebyshevT 256x256x128, Lx=Ly=4, Lz=1, Ra=1e7, Pr=0.7, "
+                "no-slip walls, b fixed at both walls, 3/2 dealiasing, SBDF2, dt=2e-3")
+CH3_KERNELS = ["k3_zbwd (Chebyshev evaluation of 15 operand planes)", "k3_ybwd (padded inverse y FFT, MY=384)",
+               "k3_xpass (inverse x FFTs MX=384, 12 products, forward x FFTs)",
+               "k3_yfwd (forward y FFT, truncation, Nyquist folds)", "k_ch_zfwd (Chebyshev forward of 4 products)",
+               "k3_sbdf (multistep right-hand side)", "k3_solve (per-mode FP64 LU solves, rotated parity systems)"]
+
+
+def bench_channel3d(a):
+    import numpy as np
+    import torch
+
+    from tarang_amd import _lib, diagnostics
+
+    c = CH3
+    nx, ny, nz, Lz = c["nx"], c["ny"], c["nz"], c["Lz"]
+    kappa, nu = (c["Ra"] * c["Pr"]) ** -0.5, (c["Ra"] / c["Pr"]) ** -0.5
+    steps = a.steps if a.steps != 1000 else 100
+    dev = torch.device("cuda", 0)
+    z = Lz * (1 - np.cos(np.pi * np.arange(nz) / (nz - 1))) / 2
+    rng = np.random.default_rng(42)
+    bg = (Lz - z)[None, None, :] + 1e-3 * rng.standard_normal((nx, ny, nz)) * (z * (Lz - z))[None, None, :]
+    k = np.arange(nz)
+    T = np.cos(np.outer(np.pi * np.arange(nz) / (nz - 1), k)) * ((-1.0) ** k)[None, :]
+    F = np.fft.fft(np.fft.rfft(bg, axis=0), axis=1)
+    Bc = np.einsum("kz,mnz->mnk", np.linalg.inv(T), F)
+    ctx = ctypes.c_void_p()
+    _lib.call("tarang_ctx_create", 0, ctypes.byref(ctx))
+    results, finals, solvers, keys = {}, {}, {}, {}
+    for name, code, ct in (("f64", _lib.F64, torch.complex128), ("f32", _lib.F32, torch.complex64)):
+        d = _lib.Chan3Desc()
+        d.nx, d.ny, d.nz, d.Lx, d.Ly, d.Lz = nx, ny, nz, c["Lx"], c["Ly"], Lz
+        d.kappa, d.nu, d.dt, d.dtype, d.timestepper, d.b_bottom, d.b_top = kappa, nu, c["dt"], code, 2, Lz, 0.0
+        h = ctypes.c_void_p()
+        t0 = time.perf_counter()
+        _lib.call("tarang_chan3_create", ctx, ctypes.byref(d), ctypes.byref(h))
+        tf = time.perf_counter() - t0
+        solvers[name] = h
+        kk = ctypes.c_int()
+        _lib.call("tarang_chan3_num_keys", h, ctypes.byref(kk))
+        keys[name] = kk.value
+        Bd = torch.as_tensor(np.ascontiguousarray(Bc.T), device=dev).to(ct).contiguous()
+        Zd = torch.zeros_like(Bd)
+        torch.cuda.synchronize()
+
+        def reset():
+            _lib.call("tarang_chan3_set_field", h, _lib.CHAN3_B, ctypes.c_void_p(Bd.data_ptr()))
+            for v in (_lib.CHAN3_P, _lib.CHAN3_UX, _lib.CHAN3_UY, _lib.CHAN3_UZ):
+                _lib.call("tarang_chan3_set_field", h, v, ctypes.c_void_p(Zd.data_ptr()))
+        reset()
+        log(f"channel3d {name}: setup {tf:.3f} s ({kk.value} |k_h|^2 keys); warmup {a.warmup} steps")
+        _lib.call("tarang_chan3_step", h, a.warmup)
+        reset()
+        _lib.call("tarang_ctx_sync", ctx)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _lib.call("tarang_chan3_step", h, steps)
+        _lib.call("tarang_ctx_sync", ctx)
+        t1 = time.perf_counter()
+        results[name] = ((t1 - t0) * 1e3 / steps, tf)
+        outs = []
+        for v in (_lib.CHAN3_B, _lib.CHAN3_UZ):
+            o = torch.zeros_like(Bd)
+            _lib.call("tarang_chan3_get_field", h, v, ctypes.c_void_p(o.data_ptr()))
+            outs.append(o)
+        _lib.call("tarang_ctx_sync", ctx)
+        finals[name] = [o.cpu().numpy().T.astype(np.complex128) for o in outs]
+        log(f"channel3d {name}: {results[name][0]:.4f} ms/step")
+    nus = {kx: diagnostics.nusselt3(v[0], v[1], nx, ny, Lz, kappa, Lz) for kx, v in finals.items()}
+    err_b = float(np.abs(finals["f32"][0] - finals["f64"][0]).max() / np.abs(finals["f64"][0]).max())
+    avg = (ctypes.c_double * 7)()
+    cnt = (ctypes.c_long * 7)()
+    _lib.call("tarang_chan3_profile", solvers["f32"], a.profile_steps, avg, cnt)
+    nb = (ctypes.c_double * 7)()
+    _lib.call("tarang_chan3_kernel_bytes", solvers["f32"], nb)
+    per_step = [avg[i] * cnt[i] / a.profile_steps for i in range(7)]
+    dom = max(range(7), key=lambda i: per_step[i])
+    achieved = nb[dom] / (avg[dom] * 1e-3) / 1e9
+    kernels = {CH3_KERNELS[i]: {"avg_ms": avg[i], "launches_per_step": cnt[i] / a.profile_steps,
+                                "algorithmic_bytes": nb[i], "GBps": nb[i] / (avg[i] * 1e-3) / 1e9 if avg[i] else None}
+               for i in range(7)}
+    for h in solvers.values():
+        _lib.call("tarang_chan3_destroy", h)
+    _lib.call("tarang_ctx_destroy", ctx)
+    cpu = None
+    if not a.no_cpu:
+        from oracle import oracle as O
+        threads = min(os.cpu_count() or 1, 16)
+        O.set_threads(threads)
+        sx, cs = 16, 2
+        ref = O.RBC3D(sx, sx, nz, c["Lx"], c["Ly"], Lz, kappa, nu, c["dt"], "SBDF2")
+        zz = Lz * (1 - np.cos(np.pi * np.arange(nz) / (nz - 1))) / 2
+        ref.set(O.RBC3_B, O.forward_transform(ref.d, np.tile((Lz - zz)[None, None, :], (sx, sx, 1))))
+        ref.step(2)  # SBDF1 start and the SBDF2 factorisation
+        t0 = time.perf_counter()
+        ref.step(cs)
+        el = time.perf_counter() - t0
+        cpu = {"value": sx * sx * nz * cs / el, "unit": "grid-point steps/s", "cores": threads, "kind": "port",
+               "sample": f"{cs} SBDF2 steps at 16x16x128 (144 of the 33,024 per-mode systems, same size), FP64 "
+                         f"oracle with dense complex per-mode LU, factorisation excluded, {el:.2f} s"}
+    pts = nx * ny * nz
+    v32, v64 = pts / (results["f32"][0] * 1e-3), pts / (results["f64"][0] * 1e-3)
+    line = {
+        "metric": "grid-point steps/s, FP32+refine vs FP64 at <=1e-6 rel err; p50/p99 latency",
+        "value": v32, "unit": "grid-point steps/s", "n_gpus": 1, "steps": steps, "warmup": a.warmup,
+        "ms_per_step": results["f32"][0], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (b0 = Lz - z + 1e-3 N(0,1) z (Lz - z), u = 0, seed 42)",
+        "config": {"workload": CH3_WORKLOAD, "grid": [nx, ny, nz], "timestepper": "SBDF2", "dt": c["dt"],
+                   "parallelism": "single GPU"},
+        "fp64": {"value": v64, "ms_per_step": results["f64"][0], "setup_s": results["f64"][1]},
+        "setup_s_f32": results["f32"][1], "kh2_keys": keys["f32"],
+        "speedup_fp32_vs_fp64": v32 / v64,
+        "fp32_vs_fp64_rel_err": {"b_maxnorm": err_b, "nusselt": abs(nus["f32"] - nus["f64"]) / abs(nus["f64"]),
+                                 "after_steps": steps},
+        "nusselt": nus,
+        "roofline": {"bound": "hbm", "kernel": CH3_KERNELS[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "kernels_f32": kernels,
         "cpu_baseline": cpu,

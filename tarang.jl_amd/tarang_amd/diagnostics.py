@@ -62,3 +62,25 @@ def nusselt(b: np.ndarray, uz: np.ndarray, nx: int, Lz: float, kappa: float, dB:
     b0 = b[0].real / nx                                       # x-mean profile coefficients
     jump = float(np.sum(b0) - np.sum(b0 * (-1.0) ** np.arange(b0.size)))  # bbar(Lz) - bbar(0)
     return (conv - kappa * jump / Lz) / (kappa * dB / Lz)
+
+
+def xy_mean_of_product(F: np.ndarray, G: np.ndarray, nx: int, ny: int) -> np.ndarray:
+    """Mean over the x-y grid of f g for real f, g with (nx/2+1, ny, ...) coefficients
+    (rfft in x, full FFT in y, unnormalised); trailing axes are kept."""
+    w = np.full(F.shape[0], 2.0)
+    w[0] = 1.0
+    if nx % 2 == 0:
+        w[nx // 2] = 1.0
+    prod = (F * np.conj(G)).real.sum(axis=1)
+    return np.tensordot(w, prod, axes=(0, 0)) / (nx * nx * ny * ny)
+
+
+def nusselt3(b: np.ndarray, uz: np.ndarray, nx: int, ny: int, Lz: float, kappa: float, dB: float) -> float:
+    """Nu = <uz b - kappa dz b>_V / (kappa dB / Lz) from (nx/2+1, ny, nz) coefficient arrays."""
+    bz, wz = chebyshev_to_grid(b), chebyshev_to_grid(uz)
+    flux = xy_mean_of_product(wz, bz, nx, ny)
+    wq = clenshaw_curtis_weights(b.shape[-1])
+    conv = float(np.dot(wq, flux)) / 2.0
+    b0 = b[0, 0].real / (nx * ny)
+    jump = float(np.sum(b0) - np.sum(b0 * (-1.0) ** np.arange(b0.size)))
+    return (conv - kappa * jump / Lz) / (kappa * dB / Lz)

@@ -139,3 +139,23 @@ def test_nusselt_closed_forms():
     assert abs(cond - 1.0) < 1e-13
     nu = D.nusselt(coeff(b), coeff(uz), nx, Lz, kappa, Lz)
     assert abs(nu - (1.0 + A * Bamp / 4 / (kappa * Lz / Lz))) < 1e-9
+
+
+def test_nusselt3_reduces_to_2d():
+    """A y-independent 3-D state has the 2-D Nusselt number."""
+    from tarang_amd import diagnostics as D
+    rng = np.random.default_rng(0)
+    nx, ny, nz = 8, 4, 9
+    b2 = rng.standard_normal((nx // 2 + 1, nz)) + 1j * rng.standard_normal((nx // 2 + 1, nz))
+    w2 = rng.standard_normal((nx // 2 + 1, nz)) + 1j * rng.standard_normal((nx // 2 + 1, nz))
+    b2[0] = b2[0].real
+    w2[0] = w2[0].real
+    b2[nx // 2] = b2[nx // 2].real
+    w2[nx // 2] = w2[nx // 2].real
+    b3 = np.zeros((nx // 2 + 1, ny, nz), complex)
+    w3 = np.zeros_like(b3)
+    b3[:, 0] = b2 * ny
+    w3[:, 0] = w2 * ny
+    a = D.nusselt(b2, w2, nx, 1.0, 0.01, 1.0)
+    c = D.nusselt3(b3, w3, nx, ny, 1.0, 0.01, 1.0)
+    assert abs(a - c) < 1e-12 * abs(a)
