@@ -347,7 +347,8 @@ CH3_WORKLOAD = ("3-D Rayleigh-Benard, RealFourier^2 x ChebyshevT 256x256x128, Lx
 CH3_KERNELS = ["k3_zbwd (Chebyshev evaluation of 15 operand planes)", "k3_ybwd (padded inverse y FFT, MY=384)",
                "k3_xpass (inverse x FFTs MX=384, 12 products, forward x FFTs)",
                "k3_yfwd (forward y FFT, truncation, Nyquist folds)", "k_ch_zfwd (Chebyshev forward of 4 products)",
-               "k3_sbdf (multistep right-hand side)", "k3_solve (per-mode FP64 LU solves, rotated parity systems)"]
+               "k3_sbdf (multistep right-hand side)", "k3_solve (per-mode FP64 LU solves, rotated parity systems)",
+               "k3_lx (structured L x, RK only)"]
 
 
 def bench_channel3d(a):
@@ -412,17 +413,17 @@ def bench_channel3d(a):
         log(f"channel3d {name}: {results[name][0]:.4f} ms/step")
     nus = {kx: diagnostics.nusselt3(v[0], v[1], nx, ny, Lz, kappa, Lz) for kx, v in finals.items()}
     err_b = float(np.abs(finals["f32"][0] - finals["f64"][0]).max() / np.abs(finals["f64"][0]).max())
-    avg = (ctypes.c_double * 7)()
-    cnt = (ctypes.c_long * 7)()
+    avg = (ctypes.c_double * 8)()
+    cnt = (ctypes.c_long * 8)()
     _lib.call("tarang_chan3_profile", solvers["f32"], a.profile_steps, avg, cnt)
-    nb = (ctypes.c_double * 7)()
+    nb = (ctypes.c_double * 8)()
     _lib.call("tarang_chan3_kernel_bytes", solvers["f32"], nb)
-    per_step = [avg[i] * cnt[i] / a.profile_steps for i in range(7)]
-    dom = max(range(7), key=lambda i: per_step[i])
+    per_step = [avg[i] * cnt[i] / a.profile_steps for i in range(8)]
+    dom = max(range(8), key=lambda i: per_step[i])
     achieved = nb[dom] / (avg[dom] * 1e-3) / 1e9
     kernels = {CH3_KERNELS[i]: {"avg_ms": avg[i], "launches_per_step": cnt[i] / a.profile_steps,
                                 "algorithmic_bytes": nb[i], "GBps": nb[i] / (avg[i] * 1e-3) / 1e9 if avg[i] else None}
-               for i in range(7)}
+               for i in range(8)}
     for h in solvers.values():
         _lib.call("tarang_chan3_destroy", h)
     _lib.call("tarang_ctx_destroy", ctx)

@@ -235,9 +235,9 @@ int tarang_chan3_time(tarang_chan3_t s, double* t, long* iteration);
 /* number of distinct |k_h|^2 keys whose factors the modes share */
 int tarang_chan3_num_keys(tarang_chan3_t s, int* nkeys);
 /* eager steps with HIP events; [z-backward, y-backward, x-pass, y-forward, z-forward,
- * multistep rhs, per-mode solve] average ms and counts.  No reference counterpart. */
-int tarang_chan3_profile(tarang_chan3_t s, int nsteps, double* avg_ms7, long* counts7);
-int tarang_chan3_kernel_bytes(tarang_chan3_t s, double* bytes7);
+ * multistep / stage rhs, per-mode solve, L x] average ms and counts.  No reference counterpart. */
+int tarang_chan3_profile(tarang_chan3_t s, int nsteps, double* avg_ms8, long* counts8);
+int tarang_chan3_kernel_bytes(tarang_chan3_t s, double* bytes8);
 
 /* ------------------------------------------------------------------------ */
 /* Serving loop: continuous batching of IVP solve requests on one GPU.  No reference

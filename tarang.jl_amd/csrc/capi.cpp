@@ -105,7 +105,13 @@ tarang::Channel3Desc conv_desc(const tarang_chan3_desc* desc) {
     d.dtype = desc->dtype;
     d.b_bottom = desc->b_bottom;
     d.b_top = desc->b_top;
-    d.stepper = desc->timestepper == 2 ? tarang::kSBDF2 : (desc->timestepper == 4 ? tarang::kSBDF1 : -1);
+    switch (desc->timestepper) {
+        case 0: d.stepper = tarang::kRK222; break;
+        case 1: d.stepper = tarang::kRK443; break;
+        case 2: d.stepper = tarang::kSBDF2; break;
+        case 4: d.stepper = tarang::kSBDF1; break;
+        default: d.stepper = -1; break;
+    }
     return d;
 }
 }  // namespace
@@ -565,20 +571,20 @@ int tarang_chan3_num_keys(tarang_chan3_t s, int* nkeys) {
     });
 }
 
-int tarang_chan3_profile(tarang_chan3_t s, int nsteps, double* avg_ms7, long* counts7) {
+int tarang_chan3_profile(tarang_chan3_t s, int nsteps, double* avg_ms8, long* counts8) {
     return guard([&] {
         NEED(s, "solver");
-        NEED(avg_ms7, "avg_ms7");
+        NEED(avg_ms8, "avg_ms8");
         TG_REQUIRE(nsteps > 0, tarang::kErrInvalid, "nsteps must be > 0");
-        s->solver->profile(nsteps, avg_ms7, counts7);
+        s->solver->profile(nsteps, avg_ms8, counts8);
     });
 }
 
-int tarang_chan3_kernel_bytes(tarang_chan3_t s, double* bytes7) {
+int tarang_chan3_kernel_bytes(tarang_chan3_t s, double* bytes8) {
     return guard([&] {
         NEED(s, "solver");
-        NEED(bytes7, "bytes7");
-        s->solver->kernel_bytes(bytes7);
+        NEED(bytes8, "bytes8");
+        s->solver->kernel_bytes(bytes8);
     });
 }
 

@@ -500,4 +500,133 @@ __global__ void k3_sbdf(Ch3SbdfArgs<T> a, size_t n) {
     }
 }
 
+// L P(x) per mode (one workgroup per mode) for the RK stage loop; P is the grid round
+// trip (Hermitian part of the kx = 0 and kx = Nx/2 lines, through the partner mode
+// (m, -n)); tau_p has no basis and is not projected.
+//   E1  = i kx ux + i ky uy + Dz uz + [N-1] tau_u1z + [k=0, mode 0] tau_p
+//   E2  = kappa (kh^2 b - Dzz b) - kappa Dz[:,N-1] tau_b1 + [N-1] tau_b2
+//   E3a = nu (kh^2 ua - Dzz ua) - nu Dz[:,N-1] tau_u1a + i ka p + [N-1] tau_u2a   (a = x, y)
+//   E3z = nu (kh^2 uz - Dzz uz) - nu Dz[:,N-1] tau_u1z + Dz p - b + [N-1] tau_u2z
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k3_lx(const cplx<T>* __restrict__ p, const cplx<T>* __restrict__ b,
+                                            const cplx<T>* __restrict__ ux, const cplx<T>* __restrict__ uy,
+                                            const cplx<T>* __restrict__ uz, const cplx<T>* __restrict__ taus,
+                                            Ch3Geom geo, double sz, double kappa, double nu, cplx<T>* E1,
+                                            cplx<T>* E2, cplx<T>* E3x, cplx<T>* E3y, cplx<T>* E3z) {
+    extern __shared__ unsigned char smem_raw[];
+    const int N = geo.N, nxh = geo.nxh;
+    double2* v = reinterpret_cast<double2*>(smem_raw);  // [5][N]: p, b, ux, uy, uz
+    double2* d1 = v + 5 * N;                             // Dz of each
+    const int mode = blockIdx.x, m = mode % nxh, n = mode / nxh;
+    const bool xline = m == 0 || ((geo.nx % 2) == 0 && m == geo.nx / 2);
+    const int partner = m + nxh * ((geo.ny - n) % geo.ny);
+    const cplx<T>* src[5] = {p, b, ux, uy, uz};
+    for (int idx = threadIdx.x; idx < 5 * N; idx += NT) {
+        const int f = idx / N, k = idx - f * N;
+        const cplx<T> x = src[f][(size_t)mode * N + k];
+        double2 w{(double)x.x, (double)x.y};
+        if (xline) {
+            const cplx<T> q = src[f][(size_t)partner * N + k];
+            w = double2{0.5 * (w.x + (double)q.x), 0.5 * (w.y - (double)q.y)};
+        }
+        v[idx] = w;
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < 5 * N; idx += NT) {
+        const int f = idx / N, k = idx - f * N;
+        double2 acc{0, 0};
+        for (int j = k + 1; j < N; j += 2) {
+            const double c = cheb_dz(k, j, sz);
+            acc.x += c * v[f * N + j].x;
+            acc.y += c * v[f * N + j].y;
+        }
+        d1[idx] = acc;
+    }
+    __syncthreads();
+    double2 tau[kCh3TauStride];
+    for (int t = 0; t < kCh3TauStride; ++t) {
+        const cplx<T> x = taus[(size_t)mode * kCh3TauStride + t];
+        double2 w{(double)x.x, (double)x.y};
+        if (xline && t > 0) {
+            const cplx<T> q = taus[(size_t)partner * kCh3TauStride + t];
+            w = double2{0.5 * (w.x + (double)q.x), 0.5 * (w.y - (double)q.y)};
+        }
+        tau[t] = w;
+    }
+    const double kx = geo.k0x * m, ky = ch3_ky(geo, n), kh2 = kx * kx + ky * ky;
+    for (int k = threadIdx.x; k < N; k += NT) {
+        double2 dzz[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};  // Dzz of b, ux, uy, uz
+        for (int j = k + 1; j < N; j += 2) {
+            const double c = cheb_dz(k, j, sz);
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                dzz[f].x += c * d1[(f + 1) * N + j].x;
+                dzz[f].y += c * d1[(f + 1) * N + j].y;
+            }
+        }
+        const double dN = cheb_dz(k, N - 1, sz);
+        const double2 P = v[k], Bv = v[N + k], Ux = v[2 * N + k], Uy = v[3 * N + k], Uz = v[4 * N + k];
+        const double2 DP = d1[k], DUz = d1[4 * N + k];
+        const bool last = k == N - 1;
+        double2 e1{-kx * Ux.y - ky * Uy.y + DUz.x, kx * Ux.x + ky * Uy.x + DUz.y};
+        if (last) { e1.x += tau[5].x; e1.y += tau[5].y; }
+        if (k == 0 && mode == 0) { e1.x += tau[0].x; e1.y += tau[0].y; }
+        double2 e2{kappa * (kh2 * Bv.x - dzz[0].x) - kappa * dN * tau[1].x,
+                   kappa * (kh2 * Bv.y - dzz[0].y) - kappa * dN * tau[1].y};
+        double2 ex{nu * (kh2 * Ux.x - dzz[1].x) - nu * dN * tau[3].x - kx * P.y,
+                   nu * (kh2 * Ux.y - dzz[1].y) - nu * dN * tau[3].y + kx * P.x};
+        double2 ey{nu * (kh2 * Uy.x - dzz[2].x) - nu * dN * tau[4].x - ky * P.y,
+                   nu * (kh2 * Uy.y - dzz[2].y) - nu * dN * tau[4].y + ky * P.x};
+        double2 ez{nu * (kh2 * Uz.x - dzz[3].x) - nu * dN * tau[5].x + DP.x - Bv.x,
+                   nu * (kh2 * Uz.y - dzz[3].y) - nu * dN * tau[5].y + DP.y - Bv.y};
+        if (last) {
+            e2.x += tau[2].x; e2.y += tau[2].y;
+            ex.x += tau[6].x; ex.y += tau[6].y;
+            ey.x += tau[7].x; ey.y += tau[7].y;
+            ez.x += tau[8].x; ez.y += tau[8].y;
+        }
+        const size_t o = (size_t)mode * N + k;
+        E1[o] = cplx<T>{T(e1.x), T(e1.y)};
+        E2[o] = cplx<T>{T(e2.x), T(e2.y)};
+        E3x[o] = cplx<T>{T(ex.x), T(ex.y)};
+        E3y[o] = cplx<T>{T(ey.x), T(ey.y)};
+        E3z[o] = cplx<T>{T(ez.x), T(ez.y)};
+    }
+}
+
+// IMEX RK stage right-hand side on the five row blocks (E1, E2, E3x, E3y, E3z):
+//   rhs = M X0 + dt sum_j (AE_j F_j - AI_j LX_j);  final update writes b, ux, uy, uz
+template <typename T>
+struct Ch3StageArgs {
+    const cplx<T>* X0[4];      // b, ux, uy, uz at the start of the step
+    const cplx<T>* F[4][4];    // F_j rows E2, E3x, E3y, E3z
+    const cplx<T>* LX[4][5];   // LX_j rows E1..E3z
+    T ae[4], ai[4];
+    int nterms;
+    T dt;
+    cplx<T>* out[5];
+    int final_update;
+};
+
+template <typename T>
+__global__ void k3_stage(Ch3StageArgs<T> a, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        for (int r = 0; r < 5; ++r) {
+            if (a.final_update && r == 0) continue;
+            cplx<T> v = r == 0 ? cplx<T>{T(0), T(0)} : a.X0[r - 1][i];
+            for (int j = 0; j < a.nterms; ++j) {
+                if (r > 0) {
+                    const cplx<T> f = a.F[j][r - 1][i];
+                    v.x += a.dt * a.ae[j] * f.x;
+                    v.y += a.dt * a.ae[j] * f.y;
+                }
+                const cplx<T> l = a.LX[j][r][i];
+                v.x -= a.dt * a.ai[j] * l.x;
+                v.y -= a.dt * a.ai[j] * l.y;
+            }
+            a.out[a.final_update ? r - 1 : r][i] = v;
+        }
+    }
+}
+
 }  // namespace tarang

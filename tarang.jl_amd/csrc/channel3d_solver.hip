@@ -28,8 +28,8 @@ bool size_ok(int n) {
 
 Channel3Solver::Channel3Solver(Context* ctx, const Channel3Desc& d) : ctx_(ctx), d_(d) {
     TG_REQUIRE(d.dtype == kF32 || d.dtype == kF64, kErrInvalid, "3-D channel solver: dtype must be f32 or f64");
-    TG_REQUIRE(d.stepper == kSBDF2 || d.stepper == kSBDF1, kErrInvalid,
-               "3-D channel solver: timestepper must be SBDF2 or SBDF1");
+    TG_REQUIRE(d.stepper == kSBDF2 || d.stepper == kSBDF1 || d.stepper == kRK222 || d.stepper == kRK443, kErrInvalid,
+               "3-D channel solver: timestepper must be SBDF2, SBDF1, RK222 or RK443");
     const int xmax = d.dtype == kF64 ? 512 : 1024;
     TG_REQUIRE(size_ok(d.nx) && d.nx <= xmax && size_ok(d.ny) && d.ny <= 1024, kErrInvalid,
                "3-D channel solver: RealFourier sizes " + std::to_string(d.nx) + "x" + std::to_string(d.ny) +
@@ -64,6 +64,14 @@ Channel3Solver::Channel3Solver(Context* ctx, const Channel3Desc& d) : ctx_(ctx),
         zeroed(MXm1_[r], fb);
     }
     for (int r = 0; r < 5; ++r) zeroed(R_[r], fb);
+    tab_ = make_tableau(d.stepper);
+    if (!multistep()) {
+        for (int r = 0; r < 4; ++r) zeroed(X0_[r], fb);
+        for (int st = 0; st < tab_.stages; ++st) {
+            for (int r = 0; r < 4; ++r) zeroed(Frk_[st][r], fb);
+            for (int r = 0; r < 5; ++r) zeroed(LX_[st][r], fb);
+        }
+    }
     const size_t prow = (size_t)d.nz * my_ * nxh_ * vb;  // one y-padded plane
     G_ = DevBuf(ctx_, (size_t)kCh3Planes * fb);
     Q_ = DevBuf(ctx_, (size_t)kCh3Planes * prow);
@@ -141,7 +149,7 @@ Channel3Solver::Channel3Solver(Context* ctx, const Channel3Desc& d) : ctx_(ctx),
     ctx_->twiddle(mx_, (DType)d.dtype);
     ctx_->twiddle(my_, (DType)d.dtype);
     TG_HIP(hipStreamSynchronize(s));
-    factor(implicit_c(true));
+    factor(implicit_c(first_step()));
 }
 
 Channel3Solver::~Channel3Solver() {
@@ -200,6 +208,7 @@ void Channel3Solver::set_dt(double dt) {
 }
 
 double Channel3Solver::implicit_c(bool first) const {
+    if (!multistep()) return d_.dt * tab_.AI[1][1];
     if (first) return d_.dt;
     const double w = d_.dt / dt_prev_;
     return (1.0 + w) * d_.dt / (1.0 + 2.0 * w);
@@ -363,7 +372,91 @@ void Channel3Solver::solve_t() {
     TG_HIP(hipGetLastError());
 }
 
+template <typename T>
+void Channel3Solver::lx_t(int st) {
+    using V = cplx<T>;
+    const int N = d_.nz;
+    constexpr int NT = 256;
+    Ch3Geom geo{d_.nx, d_.ny, nxh_, N, nmodes_, 2.0 * M_PI / d_.Lx, 2.0 * M_PI / d_.Ly};
+    timed(7, [&] {
+        hipLaunchKernelGGL((k3_lx<T, NT>), dim3(nmodes_), dim3(NT), (size_t)10 * N * sizeof(double2), ctx_->stream,
+                           P_.as<const V>(), B_.as<const V>(), UX_.as<const V>(), UY_.as<const V>(), UZ_.as<const V>(),
+                           TAU_.as<const V>(), geo, 2.0 / d_.Lz, d_.kappa, d_.nu, LX_[st][0].as<V>(),
+                           LX_[st][1].as<V>(), LX_[st][2].as<V>(), LX_[st][3].as<V>(), LX_[st][4].as<V>());
+    });
+    TG_HIP(hipGetLastError());
+}
+
+template <typename T>
+void Channel3Solver::stage_t(int st, bool final_update) {
+    using V = cplx<T>;
+    Ch3StageArgs<T> a;
+    std::memset(&a, 0, sizeof(a));
+    for (int r = 0; r < 4; ++r) a.X0[r] = X0_[r].as<const V>();
+    const int nterms = final_update ? tab_.stages : st;
+    for (int j = 0; j < nterms; ++j) {
+        for (int r = 0; r < 4; ++r) a.F[j][r] = Frk_[j][r].as<const V>();
+        for (int r = 0; r < 5; ++r) a.LX[j][r] = LX_[j][r].as<const V>();
+        a.ae[j] = T(final_update ? tab_.bE[j] : tab_.AE[st][j]);
+        a.ai[j] = T(final_update ? tab_.bI[j] : tab_.AI[st][j]);
+    }
+    a.nterms = nterms;
+    a.dt = T(d_.dt);
+    a.final_update = final_update ? 1 : 0;
+    if (final_update) {
+        a.out[0] = B_.as<V>();
+        a.out[1] = UX_.as<V>();
+        a.out[2] = UY_.as<V>();
+        a.out[3] = UZ_.as<V>();
+    } else {
+        for (int r = 0; r < 5; ++r) a.out[r] = R_[r].as<V>();
+    }
+    const size_t n = field_elems();
+    const unsigned nblk = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+    timed(5, [&] { hipLaunchKernelGGL((k3_stage<T>), dim3(nblk), dim3(256), 0, ctx_->stream, a, n); });
+    TG_HIP(hipGetLastError());
+}
+
+template <typename T>
+void Channel3Solver::zero_t(void* p, size_t n) {
+    const unsigned nblk = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+    timed(5, [&] {
+        hipLaunchKernelGGL((k_ch_zero<T>), dim3(nblk), dim3(256), 0, ctx_->stream, static_cast<cplx<T>*>(p), n);
+    });
+    TG_HIP(hipGetLastError());
+}
+
 void Channel3Solver::launch_step_body() {
+    if (!multistep()) {  // IMEX RK stage loop, as in the 2-D solver
+        auto run = [&](auto tag) {
+            using T = decltype(tag);
+            using V = cplx<T>;
+            const size_t n = field_elems();
+            const unsigned nblk = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+            const DevBuf* X[4] = {&B_, &UX_, &UY_, &UZ_};
+            for (int r = 0; r < 4; r += 2)
+                timed(5, [&] {
+                    hipLaunchKernelGGL((k_ch_copy<T>), dim3(nblk), dim3(256), 0, ctx_->stream, X[r]->as<const V>(),
+                                       X[r + 1]->as<const V>(), X[r + 1]->as<const V>(), X0_[r].as<V>(),
+                                       X0_[r + 1].as<V>(), X0_[r + 1].as<V>(), n);
+                });
+            zero_t<T>(P_.ptr, n);
+            zero_t<T>(TAU_.ptr, (size_t)nmodes_ * kCh3TauStride);
+            rhs_t<T>(Frk_[0][0].ptr, Frk_[0][1].ptr, Frk_[0][2].ptr, Frk_[0][3].ptr);
+            lx_t<T>(0);
+            for (int st = 1; st < tab_.stages; ++st) {
+                stage_t<T>(st, false);
+                solve_t<T>();
+                rhs_t<T>(Frk_[st][0].ptr, Frk_[st][1].ptr, Frk_[st][2].ptr, Frk_[st][3].ptr);
+                lx_t<T>(st);
+            }
+            stage_t<T>(0, true);
+            zero_t<T>(P_.ptr, n);
+            zero_t<T>(TAU_.ptr, (size_t)nmodes_ * kCh3TauStride);
+        };
+        if (d_.dtype == kF32) run(float{}); else run(double{});
+        return;
+    }
     const bool first = first_step();
     if (d_.dtype == kF32) {
         rhs_t<float>(F_[0].ptr, F_[1].ptr, F_[2].ptr, F_[3].ptr);
@@ -382,7 +475,8 @@ void Channel3Solver::step_once(bool allow_graph) {
     if (!(allow_graph && graph_on_)) {
         launch_step_body();
     } else {
-        if (graph_ && (graph_dt_ != d_.dt || graph_first_ != first || graph_dtp_ != dt_prev_)) invalidate_graph();
+        if (graph_ && (graph_dt_ != d_.dt || graph_first_ != first || (multistep() && graph_dtp_ != dt_prev_)))
+            invalidate_graph();
         if (!graph_) {
             hipGraph_t g = nullptr;
             TG_HIP(hipStreamBeginCapture(ctx_->stream, hipStreamCaptureModeThreadLocal));
@@ -486,6 +580,7 @@ void Channel3Solver::kernel_bytes(double* out) const {
             fac += (2 * nb * nb + nu * nu) * 8.0;
         }
     out[6] = fac + 10 * nc * vb;
+    out[7] = 10 * nc * vb;  // L x: read p, b, u (and partners on two lines), write five rows
 }
 
 size_t footprint(const Channel3Desc& d) {

@@ -18,7 +18,7 @@ struct Channel3Desc {
     double kappa, nu;
     double dt;
     int dtype;               // kF32 / kF64 (state, RHS, stage arithmetic; solves are FP64)
-    int stepper;             // kSBDF2 / kSBDF1
+    int stepper;             // kSBDF2 / kSBDF1 (per-mode multistep) or kRK222 / kRK443 (IMEX RK)
     double b_bottom, b_top;  // Dirichlet values of b at z = 0 and z = Lz
 };
 
@@ -42,8 +42,8 @@ class Channel3Solver {
     void enable_graph(bool on) { graph_on_ = on; }
     double time() const { return t_; }
     long iteration() const { return iter_; }
-    // [z-backward, y-backward, x-pass, y-forward, z-forward, multistep rhs, solve]
-    static constexpr int kKinds = 7;
+    // [z-backward, y-backward, x-pass, y-forward, z-forward, multistep / stage rhs, solve, L x]
+    static constexpr int kKinds = 8;
     void profile(int nsteps, double* avg_ms, long* counts);
     void kernel_bytes(double* out) const;
 
@@ -51,9 +51,13 @@ class Channel3Solver {
     template <typename T> void rhs_t(void* F0, void* F1, void* F2, void* F3);
     template <typename T> void sbdf_t(bool first);
     template <typename T> void solve_t();
+    template <typename T> void lx_t(int st);
+    template <typename T> void stage_t(int st, bool final_update);
+    template <typename T> void zero_t(void* p, size_t n);
+    bool multistep() const { return d_.stepper == kSBDF2 || d_.stepper == kSBDF1; }
     template <typename T> void transpose_t(const void* in, void* out, int rows, int cols);
     double implicit_c(bool first) const;
-    bool first_step() const { return iter_ == 0 || d_.stepper == kSBDF1; }
+    bool first_step() const { return multistep() && (iter_ == 0 || d_.stepper == kSBDF1); }
     void factor(double c);
     void launch_step_body();
     void step_once(bool allow_graph);
@@ -71,6 +75,8 @@ class Channel3Solver {
     long iter_ = 0;
     DevBuf P_, B_, UX_, UY_, UZ_, TAU_;
     DevBuf F_[4], Fm1_[4], MXm1_[4], R_[5];
+    Tableau tab_;
+    DevBuf X0_[4], Frk_[4][4], LX_[4][5];  // IMEX RK stage storage
     DevBuf G_, Q_, H_;
     DevBuf BT_, BDT_, FT_;
     DevBuf kh_, order_, key_, rot_;

@@ -227,7 +227,7 @@ class _ChannelBackend:
         for bb in dom.bases:
             if bb.dealias != 1.5:
                 raise GPUTransformUnsupported("dealias must be 3/2 for the nonlinear GPU path")
-        allowed = (2, 4) if self.three else (0, 1, 2, 4)
+        allowed = (0, 1, 2, 4)
         if timestepper.code not in allowed:
             raise GPUTransformUnsupported(f"{timestepper.name} is not available for this channel family")
         self.taus = [problem.var(names[k]) for k in ("tau_p", "tau_b1", "tau_b2")]

@@ -280,8 +280,8 @@ class Chan3(Chan):
         return k.value
 
     def profile(self, n):
-        ms = (ctypes.c_double * 7)()
-        cnt = (ctypes.c_long * 7)()
+        ms = (ctypes.c_double * 8)()
+        cnt = (ctypes.c_long * 8)()
         _lib.call("tarang_chan3_profile", self.h, int(n), ms, cnt)
         return list(ms), list(cnt)
 

@@ -62,14 +62,15 @@ def test_channel3d_rhs_parity(gpu, oracle, dtype, nx, ny, N):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-@pytest.mark.parametrize("ts", ["SBDF2", "SBDF1"])
+@pytest.mark.parametrize("ts", ["SBDF2", "SBDF1", "RK222", "RK443"])
 def test_channel3d_steps_parity(gpu, oracle, ts, dtype):
     nx, ny, N, Lx, Ly, Lz = 16, 8, 12, 4.0, 3.0, 1.0
     print(f"channel3d steps {ts} {dtype.__name__}", flush=True)
     kappa, nu = kp()
     st, d = random_state3(oracle, nx, ny, N, Lx, Ly, Lz, seed=5)
     ref = oracle.RBC3D(nx, ny, N, Lx, Ly, Lz, kappa, nu, 5e-3, ts)
-    s = gpu.chan3(nx, ny, N, Lx, Ly, Lz, kappa, nu, 5e-3, stepper={"SBDF2": 2, "SBDF1": 4}[ts], dtype=dtype)
+    s = gpu.chan3(nx, ny, N, Lx, Ly, Lz, kappa, nu, 5e-3, stepper={"SBDF2": 2, "SBDF1": 4, "RK222": 0, "RK443": 1}[ts],
+                  dtype=dtype)
     for v, c in st.items():
         ref.set(v, c)
         s.set(v, c)
@@ -166,4 +167,4 @@ def test_channel3d_rejects_unsupported(gpu):
     with pytest.raises(_lib.TarangError):
         gpu.chan3(16, 16, 300, 4.0, 4.0, 1.0, kappa, nu, 1e-2)
     with pytest.raises(_lib.TarangError):
-        gpu.chan3(16, 16, 9, 4.0, 4.0, 1.0, kappa, nu, 1e-2, stepper=0)
+        gpu.chan3(16, 16, 9, 4.0, 4.0, 1.0, kappa, nu, 1e-2, stepper=3)
