@@ -314,6 +314,8 @@ class ScalarField:
 
     def __init__(self, domain, name: str = "", dtype=None, bases=None):
         if isinstance(domain, Distributor):  # ScalarField(dist, name, bases, dtype): constant/tau field
+            if isinstance(dtype, (tuple, list)):  # positional (bases, dtype) after the name
+                bases, dtype = dtype, bases
             if not isinstance(name, str):  # ScalarField(dist, bases, name) spelling
                 name, bases = bases, name
             self.domain = None

@@ -159,3 +159,17 @@ def test_nusselt3_reduces_to_2d():
     a = D.nusselt(b2, w2, nx, 1.0, 0.01, 1.0)
     c = D.nusselt3(b3, w3, nx, ny, 1.0, 0.01, 1.0)
     assert abs(a - c) < 1e-12 * abs(a)
+
+
+def test_tau_field_signatures():
+    """ScalarField(dist, name, bases, dtype) and VectorField(dist, coords, name, bases, dtype)
+    (test/test_subproblem_rk.jl:35-40) on a CPU distributor."""
+    coords = T.CartesianCoordinates("x", "z")
+    dist = T.Distributor(coords, dtype=np.float64)
+    xb = T.RealFourier(coords["x"], size=16, bounds=(0.0, 4.0))
+    tp = T.ScalarField(dist, "tau_p", (), np.float64)
+    tb = T.ScalarField(dist, "tau_b1", (xb,), np.float64)
+    tu = T.VectorField(dist, coords, "tau_u1", (xb,), np.float64)
+    assert tp.name == "tau_p" and tp.bases == () and tp.dtype == np.float64
+    assert tb.bases == (xb,) and tb.name == "tau_b1"
+    assert [c.name for c in tu.components] == ["tau_u1_x", "tau_u1_z"] and tu.components[0].bases == (xb,)

@@ -82,10 +82,9 @@ def test_channel3d_steps_parity(gpu, oracle, ts, dtype):
         for v in (oracle.RBC3_B, oracle.RBC3_UX, oracle.RBC3_UY, oracle.RBC3_UZ):
             assert relerr(s.get(v), ref.get(v)) < TOL[dtype], (nsteps, dt, v)
         assert relerr(s.get(oracle.RBC3_P), ref.get(oracle.RBC3_P)) < TOL[dtype] * 100
+        scale = max(np.abs(ref.tau(t)).max() for t in range(9))  # tau_p is roundoff around 0
         for t in range(9):
-            r = ref.tau(t)
-            if np.abs(r).max() > 0:
-                assert relerr(s.tau(t), r) < TOL[dtype] * 100, t
+            assert np.abs(s.tau(t) - ref.tau(t)).max() <= TOL[dtype] * 100 * scale, t
     s.close()
 
 
