@@ -315,143 +315,158 @@ static __global__ void k3_assemble(Ch3Sys s, double* Ab, double* Aup, double* Ap
 }
 
 struct Ch3SolveArgs {
-    const double* Ab;   // b factors, stride sb per (key, parity)
-    const int* permb;   // stride pb
-    const double* Ap;   // perp factors, stride sb
-    const int* permp;
-    const double* Aup;  // u-p factors, stride su
-    const int* permu;   // stride pu
-    const int* order;   // [2 nmodes] workgroup -> 2 mode + g, grouped by key
-    const int* key;     // [nmodes]
+    const double* Ib;   // b-system inverses, stride sb per (key, parity), column-major, ld = n
+    const double* Ip;   // perp-system inverses, stride sb
+    const double* Iu;   // u-p-system inverses, stride su
+    const int4* chunks; // [nchunks] (key, g, first, count): modes[first .. first + count), count <= 8
+    const int* modes;   // modes grouped by key
     const double* rot;  // [2 nmodes] (cos, sin) of the k_h direction per mode
-    int N, nmodes;
-    size_t sb, su, pb, pu;
-    int ld;
+    int N;
+    size_t sb, su;
     double c;
     double bc_b0, bc_b1;  // Nx Ny b(z=0), Nx Ny b(z=Lz) at kx = ky = 0
 };
 
 // 3-D tau slots: 0 tau_p, 1 tau_b1, 2 tau_b2, 3 tau_u1x, 4 tau_u1y, 5 tau_u1z, 6 tau_u2x, 7 tau_u2y, 8 tau_u2z
 constexpr int kCh3TauStride = 9;
+constexpr int kCh3ChunkModes = 8;  // modes per workgroup: 16 real right-hand sides
 
-__host__ __device__ inline size_t ch3_solve_lds_doubles(int ld, int N) {
-    return (size_t)2 * ld + 64 * 64 + 4 * (size_t)(N / 2 + 1) + 8;
+__host__ __device__ inline int round16(int n) { return (n + 15) & ~15; }
+
+__host__ __device__ inline size_t ch3_solve_lds_doubles(int N) {
+    return (size_t)16 * (2 * round16(chb_size(N, 0)) + round16(chu_size(N, 0, 0)));
 }
 
-// grid 2 nmodes: workgroup -> (mode, g): b system of parity h = g^1, perp system of
-// parity g, u-p system of parity g; un-rotates the parity-g horizontal velocity and taus.
+// Workgroup = (|k_h| key, parity g, up to 8 modes of that key).  Right-hand sides of
+// the modes are the columns of 16-column blocks in LDS (column 2 j = real, 2 j + 1 =
+// imaginary part of mode j); every system is applied as X = A^-1 Y on the FP64 matrix
+// cores, so each inverse is read once per chunk.  Order: b system of parity h = g^1
+// and the perp system of parity g together, then the u-p system of parity g (its
+// E3z rows take + c b), then the un-rotation of the parity-g horizontal velocity.
 template <typename T, int NT>
 __global__ __launch_bounds__(NT) void k3_solve(Ch3SolveArgs s, const cplx<T>* __restrict__ R1,
                                                const cplx<T>* __restrict__ R2, const cplx<T>* __restrict__ R3x,
                                                const cplx<T>* __restrict__ R3y, const cplx<T>* __restrict__ R3z,
                                                cplx<T>* P, cplx<T>* B, cplx<T>* UX, cplx<T>* UY, cplx<T>* UZ,
                                                cplx<T>* taus) {
+    static_assert(NT == 256, "four wavefronts per workgroup");
     extern __shared__ unsigned char smem_raw[];
-    const int N = s.N, ld = s.ld;
-    const int wm = s.order[blockIdx.x], mode = wm >> 1, g = wm & 1, h = g ^ 1;
-    const int key = s.key[mode];
-    const int PL = (N - 1) & 1;
-    const double cr = s.rot[2 * mode], sr = s.rot[2 * mode + 1];
-    double* y = reinterpret_cast<double*>(smem_raw);  // [2][ld]
-    double* blk = y + 2 * ld;                           // [64*64], also permutation scratch
-    double* bre = blk + 64 * 64;                        // [N/2+1] buoyancy of parity h
-    double* bim = bre + (N / 2 + 1);
-    double* pre = bim + (N / 2 + 1);                    // [N/2+1] u~perp of parity g
-    double* pim = pre + (N / 2 + 1);
-    double* ptau = pim + (N / 2 + 1);                   // [2] tau of the perp system
-    const size_t base = (size_t)mode * N;
-    const int kb = 2 * key;
-    // ---- b system of parity h ----
-    const int nh = npar(N, h), ng = npar(N, g);
-    for (int i = threadIdx.x; i < nh; i += NT) {
-        const cplx<T> r = R2[base + 2 * i + h];
-        y[i] = (double)r.x;
-        y[ld + i] = (double)r.y;
+    __shared__ int smode[kCh3ChunkModes];
+    __shared__ double srot[2 * kCh3ChunkModes];
+    const int4 ch = s.chunks[blockIdx.x];
+    const int key = ch.x, g = ch.y, first = ch.z, cnt = ch.w;
+    if (cnt <= 0) return;  // padding entry: the whole workgroup leaves before any barrier
+    const int N = s.N, h = g ^ 1, PL = (N - 1) & 1;
+    const int ng = npar(N, g), nh = npar(N, h);
+    const int nb = nh + 1, np = ng + 1, nu = chu_size(N, g, key);  // key 0 is kx = ky = 0 only
+    double* Yb = reinterpret_cast<double*>(smem_raw);  // [round16(nb)][16]
+    double* Yp = Yb + 16 * round16(nb);                 // [round16(np)][16]
+    double* Yu = Yp + 16 * round16(np);                 // [round16(nu)][16]
+    const int ytot = 16 * (round16(nb) + round16(np) + round16(nu));
+    if (threadIdx.x < kCh3ChunkModes) {
+        const int j = threadIdx.x;
+        const int q = j < cnt ? s.modes[first + j] : -1;
+        smode[j] = q;
+        srot[2 * j] = q >= 0 ? s.rot[2 * q] : 1.0;
+        srot[2 * j + 1] = q >= 0 ? s.rot[2 * q + 1] : 0.0;
     }
-    if (threadIdx.x == 0) {
-        const double gv = h == 0 ? 0.5 * (s.bc_b1 + s.bc_b0) : 0.5 * (s.bc_b1 - s.bc_b0);
-        y[nh] = mode == 0 ? s.c * gv : 0.0;
-        y[ld + nh] = 0.0;
-    }
+    for (int idx = threadIdx.x; idx < ytot; idx += NT) Yb[idx] = 0.0;
     __syncthreads();
-    lu_solve_blocked<NT, 2>(s.Ab + (size_t)(kb + h) * s.sb, s.permb + (size_t)(kb + h) * s.pb, nh + 1, y, ld, blk,
-                            blk);
-    for (int i = threadIdx.x; i < nh; i += NT) {
-        bre[i] = y[i];
-        bim[i] = y[ld + i];
-        B[base + 2 * i + h] = cplx<T>{T(y[i]), T(y[ld + i])};
+    // ---- right-hand sides ----
+    for (int idx = threadIdx.x; idx < cnt * nh; idx += NT) {
+        const int j = idx / nh, i = idx - j * nh;
+        const size_t base = (size_t)smode[j] * N;
+        const cplx<T> r = R2[base + 2 * i + h], rz = R3z[base + 2 * i + h];
+        Yb[i * 16 + 2 * j] = (double)r.x;
+        Yb[i * 16 + 2 * j + 1] = (double)r.y;
+        Yu[(2 * ng + i) * 16 + 2 * j] = (double)rz.x;
+        Yu[(2 * ng + i) * 16 + 2 * j + 1] = (double)rz.y;
     }
-    if (threadIdx.x == 0) taus[(size_t)mode * kCh3TauStride + (h == (PL ^ 1) ? 1 : 2)] = cplx<T>{T(y[nh]), T(y[ld + nh])};
-    __syncthreads();
-    // ---- perp system of parity g: rows -i (-sin E3x + cos E3y) ----
-    for (int i = threadIdx.x; i < ng; i += NT) {
-        const cplx<T> rx = R3x[base + 2 * i + g], ry = R3y[base + 2 * i + g];
-        const double ex_re = (double)rx.y, ex_im = -(double)rx.x;  // -i rx
-        const double ey_re = (double)ry.y, ey_im = -(double)ry.x;
-        y[i] = -sr * ex_re + cr * ey_re;
-        y[ld + i] = -sr * ex_im + cr * ey_im;
-    }
-    if (threadIdx.x == 0) {
-        y[ng] = 0.0;
-        y[ld + ng] = 0.0;
-    }
-    __syncthreads();
-    lu_solve_blocked<NT, 2>(s.Ap + (size_t)(kb + g) * s.sb, s.permp + (size_t)(kb + g) * s.pb, ng + 1, y, ld, blk,
-                            blk);
-    for (int i = threadIdx.x; i < ng; i += NT) {
-        pre[i] = y[i];
-        pim[i] = y[ld + i];
-    }
-    if (threadIdx.x == 0) {
-        ptau[0] = y[ng];
-        ptau[1] = y[ld + ng];
-    }
-    __syncthreads();
-    // ---- u-p system of parity g: rows -i (cos E3x + sin E3y), E1, E3z + c b ----
-    const int n = chu_size(N, g, key);
-    for (int i = threadIdx.x; i < ng; i += NT) {
+    for (int idx = threadIdx.x; idx < cnt * ng; idx += NT) {
+        const int j = idx / ng, i = idx - j * ng;
+        const size_t base = (size_t)smode[j] * N;
+        const double cr = srot[2 * j], sr = srot[2 * j + 1];
         const cplx<T> rx = R3x[base + 2 * i + g], ry = R3y[base + 2 * i + g], r1 = R1[base + 2 * i + g];
-        const double ex_re = (double)rx.y, ex_im = -(double)rx.x;
+        const double ex_re = (double)rx.y, ex_im = -(double)rx.x;  // rows -i E3x, -i E3y
         const double ey_re = (double)ry.y, ey_im = -(double)ry.x;
-        y[i] = cr * ex_re + sr * ey_re;
-        y[ld + i] = cr * ex_im + sr * ey_im;
-        y[ng + i] = (double)r1.x;
-        y[ld + ng + i] = (double)r1.y;
+        Yp[i * 16 + 2 * j] = -sr * ex_re + cr * ey_re;  // perp: -sin E3x + cos E3y
+        Yp[i * 16 + 2 * j + 1] = -sr * ex_im + cr * ey_im;
+        Yu[i * 16 + 2 * j] = cr * ex_re + sr * ey_re;  // par: cos E3x + sin E3y
+        Yu[i * 16 + 2 * j + 1] = cr * ex_im + sr * ey_im;
+        Yu[(ng + i) * 16 + 2 * j] = (double)r1.x;
+        Yu[(ng + i) * 16 + 2 * j + 1] = (double)r1.y;
     }
-    for (int i = threadIdx.x; i < nh; i += NT) {
-        const cplx<T> rz = R3z[base + 2 * i + h];
-        y[2 * ng + i] = (double)rz.x + s.c * bre[i];
-        y[ld + 2 * ng + i] = (double)rz.y + s.c * bim[i];
-    }
-    for (int r = 2 * ng + nh + threadIdx.x; r < n; r += NT) {
-        y[r] = 0.0;
-        y[ld + r] = 0.0;
+    if (threadIdx.x == 0 && key == 0) {  // wall values of b at kx = ky = 0 (column 0 = mode 0, real part)
+        const double gv = h == 0 ? 0.5 * (s.bc_b1 + s.bc_b0) : 0.5 * (s.bc_b1 - s.bc_b0);
+        Yb[nh * 16] = s.c * gv;
     }
     __syncthreads();
-    lu_solve_blocked<NT, 2>(s.Aup + (size_t)(kb + g) * s.su, s.permu + (size_t)(kb + g) * s.pu, n, y, ld, blk, blk);
-    // un-rotate: u~x = cos par - sin perp, u~y = sin par + cos perp; u = i u~
-    for (int i = threadIdx.x; i < ng; i += NT) {
-        const double par_re = y[i], par_im = y[ld + i];
-        const double tx_re = cr * par_re - sr * pre[i], tx_im = cr * par_im - sr * pim[i];
-        const double ty_re = sr * par_re + cr * pre[i], ty_im = sr * par_im + cr * pim[i];
-        UX[base + 2 * i + g] = cplx<T>{T(-tx_im), T(tx_re)};
-        UY[base + 2 * i + g] = cplx<T>{T(-ty_im), T(ty_re)};
-        P[base + 2 * i + g] = cplx<T>{T(y[ng + i]), T(y[ld + ng + i])};
+    // ---- b (parity h) and perp (parity g) systems ----
+    const int wave = threadIdx.x >> 6, nw = NT / 64, w0p = ((nb + 15) >> 4) % nw;
+    {
+        dbl4 ab[3], ap[3];
+        mfma_apply<3>(s.Ib + (size_t)(2 * key + h) * s.sb, nb, Yb, ab, wave, nw, 0);
+        mfma_apply<3>(s.Ip + (size_t)(2 * key + g) * s.sb, np, Yp, ap, wave, nw, w0p);
+        __syncthreads();
+        mfma_store<3>(ab, nb, Yb, wave, nw, 0);
+        mfma_store<3>(ap, np, Yp, wave, nw, w0p);
     }
-    for (int i = threadIdx.x; i < nh; i += NT) UZ[base + 2 * i + h] = cplx<T>{T(y[2 * ng + i]), T(y[ld + 2 * ng + i])};
-    if (threadIdx.x == 0) {
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < cnt * nh; idx += NT) {
+        const int j = idx / nh, i = idx - j * nh;
+        const double bre = Yb[i * 16 + 2 * j], bim = Yb[i * 16 + 2 * j + 1];
+        Yu[(2 * ng + i) * 16 + 2 * j] += s.c * bre;
+        Yu[(2 * ng + i) * 16 + 2 * j + 1] += s.c * bim;
+        B[(size_t)smode[j] * N + 2 * i + h] = cplx<T>{T(bre), T(bim)};
+    }
+    if (threadIdx.x < cnt) {
+        const int j = threadIdx.x;
+        taus[(size_t)smode[j] * kCh3TauStride + (h == (PL ^ 1) ? 1 : 2)] =
+            cplx<T>{T(Yb[nh * 16 + 2 * j]), T(Yb[nh * 16 + 2 * j + 1])};
+    }
+    __syncthreads();
+    // ---- u-p system of parity g ----
+    {
+        dbl4 au[7];
+        mfma_apply<7>(s.Iu + (size_t)(2 * key + g) * s.su, nu, Yu, au, wave, nw, 0);
+        __syncthreads();
+        mfma_store<7>(au, nu, Yu, wave, nw, 0);
+    }
+    __syncthreads();
+    // un-rotate: u~x = cos par - sin perp, u~y = sin par + cos perp; u = i u~
+    for (int idx = threadIdx.x; idx < cnt * ng; idx += NT) {
+        const int j = idx / ng, i = idx - j * ng;
+        const size_t o = (size_t)smode[j] * N + 2 * i + g;
+        const double cr = srot[2 * j], sr = srot[2 * j + 1];
+        const double par_re = Yu[i * 16 + 2 * j], par_im = Yu[i * 16 + 2 * j + 1];
+        const double pe_re = Yp[i * 16 + 2 * j], pe_im = Yp[i * 16 + 2 * j + 1];
+        const double tx_re = cr * par_re - sr * pe_re, tx_im = cr * par_im - sr * pe_im;
+        const double ty_re = sr * par_re + cr * pe_re, ty_im = sr * par_im + cr * pe_im;
+        UX[o] = cplx<T>{T(-tx_im), T(tx_re)};
+        UY[o] = cplx<T>{T(-ty_im), T(ty_re)};
+        P[o] = cplx<T>{T(Yu[(ng + i) * 16 + 2 * j]), T(Yu[(ng + i) * 16 + 2 * j + 1])};
+    }
+    for (int idx = threadIdx.x; idx < cnt * nh; idx += NT) {
+        const int j = idx / nh, i = idx - j * nh;
+        UZ[(size_t)smode[j] * N + 2 * i + h] =
+            cplx<T>{T(Yu[(2 * ng + i) * 16 + 2 * j]), T(Yu[(2 * ng + i) * 16 + 2 * j + 1])};
+    }
+    if (threadIdx.x < cnt) {
+        const int j = threadIdx.x, mode = smode[j];
+        const double cr = srot[2 * j], sr = srot[2 * j + 1];
+        const double pt_re = Yp[ng * 16 + 2 * j], pt_im = Yp[ng * 16 + 2 * j + 1];  // perp-system tau
         cplx<T>* tm = taus + (size_t)mode * kCh3TauStride;
         const int t0 = 2 * ng + nh;
-        for (int t = 0; t < n - t0; ++t) {
+        for (int t = 0; t < nu - t0; ++t) {
             const int slot = chu_tau_slot(N, g, t);  // 2-D slots: 0 p, 3 u1x, 4 u1z, 5 u2x, 6 u2z
-            const double re = y[t0 + t], im = y[ld + t0 + t];
+            const double re = Yu[(t0 + t) * 16 + 2 * j], im = Yu[(t0 + t) * 16 + 2 * j + 1];
             if (slot == 0) {
                 tm[0] = cplx<T>{T(re), T(im)};
             } else if (slot == 4 || slot == 6) {
                 tm[slot == 4 ? 5 : 8] = cplx<T>{T(re), T(im)};
             } else {  // horizontal lift tau: par from this system, perp from the perp system (same lift)
-                const double tx_re = cr * re - sr * ptau[0], tx_im = cr * im - sr * ptau[1];
-                const double ty_re = sr * re + cr * ptau[0], ty_im = sr * im + cr * ptau[1];
+                const double tx_re = cr * re - sr * pt_re, tx_im = cr * im - sr * pt_im;
+                const double ty_re = sr * re + cr * pt_re, ty_im = sr * im + cr * pt_im;
                 const int sx = slot == 3 ? 3 : 6;
                 tm[sx] = cplx<T>{T(-tx_im), T(tx_re)};
                 tm[sx + 1] = cplx<T>{T(-ty_im), T(ty_re)};

@@ -6,6 +6,7 @@
 //   x_{n+1} = (M + L/a0)^-1 rhs  per mode (three parity-split FP64 systems per parity)
 // SBDF1 on the first step after create / set_field, SBDF2 (w = dt/dt_prev) after.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -120,31 +121,49 @@ Channel3Solver::Channel3Solver(Context* ctx, const Channel3Desc& d) : ctx_(ctx),
     }
     // the only mode with key 0 is kx = ky = 0 (integer keys), so the tau_p system is key 0
     nkeys_ = (int)khv.size();
-    std::vector<int> order(2 * (size_t)nmodes_);
+    // chunks of up to 8 modes of one key, one workgroup per (chunk, parity); the chunks of a
+    // (key, parity) sit kXcds positions apart in the launch order, so that consecutive
+    // workgroups of one XCD share the key's inverses in that XCD's L2
+    std::vector<int> modes(nmodes_);
+    for (int q = 0; q < nmodes_; ++q) modes[q] = q;
+    std::stable_sort(modes.begin(), modes.end(), [&](int a, int b) { return key[a] < key[b]; });
+    key_count_.assign(nkeys_, 0);
+    for (int q = 0; q < nmodes_; ++q) key_count_[key[q]] += 1;
+    constexpr int kXcds = 8;
+    std::vector<std::vector<std::array<int, 4>>> bucket(kXcds);
     {
-        std::vector<int> modes(nmodes_);
-        for (int q = 0; q < nmodes_; ++q) modes[q] = q;
-        std::stable_sort(modes.begin(), modes.end(), [&](int a, int b) { return key[a] < key[b]; });
-        for (int i = 0; i < nmodes_; ++i) {
-            order[2 * i] = 2 * modes[i];
-            order[2 * i + 1] = 2 * modes[i] + 1;
-        }
+        std::vector<int> start(nkeys_ + 1, 0);
+        for (int k = 0; k < nkeys_; ++k) start[k + 1] = start[k] + key_count_[k];
+        for (int k = 0; k < nkeys_; ++k)
+            for (int g = 0; g < 2; ++g) {
+                size_t best = 0;
+                for (int x = 1; x < kXcds; ++x)
+                    if (bucket[x].size() < bucket[best].size()) best = x;
+                for (int f = start[k]; f < start[k + 1]; f += kCh3ChunkModes)
+                    bucket[best].push_back({k, g, f, std::min(kCh3ChunkModes, start[k + 1] - f)});
+            }
     }
+    size_t depth = 0;
+    for (auto& b : bucket) depth = std::max(depth, b.size());
+    std::vector<int> chunks(4 * kXcds * depth, 0);  // padding entries have count 0
+    for (size_t p = 0; p < kXcds * depth; ++p) {
+        const auto& b = bucket[p % kXcds];
+        if (p / kXcds < b.size())
+            for (int e = 0; e < 4; ++e) chunks[4 * p + e] = b[p / kXcds][e];
+    }
+    nchunks_ = (int)(kXcds * depth);
     auto up = [&](DevBuf& b, const void* h, size_t bytes) {
         b = DevBuf(ctx_, bytes);
         TG_HIP(hipMemcpyAsync(b.ptr, h, bytes, hipMemcpyHostToDevice, s));
         TG_HIP(hipStreamSynchronize(s));
     };
     up(kh_, khv.data(), khv.size() * sizeof(double));
-    up(order_, order.data(), order.size() * sizeof(int));
-    up(key_, key.data(), key.size() * sizeof(int));
+    up(modes_, modes.data(), modes.size() * sizeof(int));
+    up(chunks_, chunks.data(), chunks.size() * sizeof(int));
     up(rot_, rot.data(), rot.size() * sizeof(double));
-    Ab_ = DevBuf(ctx_, (size_t)2 * nkeys_ * nb_max_ * nb_max_ * sizeof(double));
-    Ap_ = DevBuf(ctx_, (size_t)2 * nkeys_ * nb_max_ * nb_max_ * sizeof(double));
-    Aup_ = DevBuf(ctx_, (size_t)2 * nkeys_ * nu_max_ * nu_max_ * sizeof(double));
-    permb_ = DevBuf(ctx_, (size_t)2 * nkeys_ * nb_max_ * sizeof(int));
-    permp_ = DevBuf(ctx_, (size_t)2 * nkeys_ * nb_max_ * sizeof(int));
-    permu_ = DevBuf(ctx_, (size_t)2 * nkeys_ * nu_max_ * sizeof(int));
+    Ib_ = DevBuf(ctx_, (size_t)2 * nkeys_ * nb_max_ * nb_max_ * sizeof(double));
+    Ip_ = DevBuf(ctx_, (size_t)2 * nkeys_ * nb_max_ * nb_max_ * sizeof(double));
+    Iu_ = DevBuf(ctx_, (size_t)2 * nkeys_ * nu_max_ * nu_max_ * sizeof(double));
     info_ = DevBuf(ctx_, (size_t)6 * nkeys_ * sizeof(int));
     ctx_->twiddle(mx_, (DType)d.dtype);
     ctx_->twiddle(my_, (DType)d.dtype);
@@ -219,19 +238,33 @@ void Channel3Solver::factor(double c) {
     hipStream_t s = ctx_->stream;
     const int N = d_.nz;
     const size_t sb = (size_t)nb_max_ * nb_max_, su = (size_t)nu_max_ * nu_max_;
-    Ch3Sys sys{N, nkeys_, sb, su, kh_.as<const double>(), 2.0 / d_.Lz, d_.kappa, d_.nu, c, d_.Lx * d_.Ly, d_.Lz};
     const unsigned nsys = (unsigned)(2 * nkeys_);
-    hipLaunchKernelGGL(k3_assemble, dim3(nsys, 3), dim3(256), 0, s, sys, Ab_.as<double>(), Aup_.as<double>(),
-                       Ap_.as<double>());
+    // assemble and LU-factor in scratch buffers, then keep only the explicit inverses
+    DevBuf Lb(ctx_, nsys * sb * sizeof(double)), Lp(ctx_, nsys * sb * sizeof(double)),
+        Lu(ctx_, nsys * su * sizeof(double));
+    DevBuf pb(ctx_, nsys * (size_t)nb_max_ * sizeof(int)), pp(ctx_, nsys * (size_t)nb_max_ * sizeof(int)),
+        pu(ctx_, nsys * (size_t)nu_max_ * sizeof(int));
+    Ch3Sys sys{N, nkeys_, sb, su, kh_.as<const double>(), 2.0 / d_.Lz, d_.kappa, d_.nu, c, d_.Lx * d_.Ly, d_.Lz};
+    hipLaunchKernelGGL(k3_assemble, dim3(nsys, 3), dim3(256), 0, s, sys, Lb.as<double>(), Lu.as<double>(),
+                       Lp.as<double>());
     TG_HIP(hipGetLastError());
     int* info = info_.as<int>();
     constexpr int NT = 256;
-    hipLaunchKernelGGL((k_lu_factor<NT>), dim3(nsys), dim3(NT), (size_t)nb_max_ * 24, s, Ab_.as<double>(),
-                       permb_.as<int>(), info, N, 0, sb, (size_t)nb_max_);
-    hipLaunchKernelGGL((k_lu_factor<NT>), dim3(nsys), dim3(NT), (size_t)nb_max_ * 24, s, Ap_.as<double>(),
-                       permp_.as<int>(), info + nsys, N, 0, sb, (size_t)nb_max_);
-    hipLaunchKernelGGL((k_lu_factor<NT>), dim3(nsys), dim3(NT), (size_t)nu_max_ * 24, s, Aup_.as<double>(),
-                       permu_.as<int>(), info + 2 * nsys, N, 1, su, (size_t)nu_max_);
+    hipLaunchKernelGGL((k_lu_factor<NT>), dim3(nsys), dim3(NT), (size_t)nb_max_ * 24, s, Lb.as<double>(),
+                       pb.as<int>(), info, N, 0, sb, (size_t)nb_max_);
+    hipLaunchKernelGGL((k_lu_factor<NT>), dim3(nsys), dim3(NT), (size_t)nb_max_ * 24, s, Lp.as<double>(),
+                       pp.as<int>(), info + nsys, N, 0, sb, (size_t)nb_max_);
+    hipLaunchKernelGGL((k_lu_factor<NT>), dim3(nsys), dim3(NT), (size_t)nu_max_ * 24, s, Lu.as<double>(),
+                       pu.as<int>(), info + 2 * nsys, N, 1, su, (size_t)nu_max_);
+    TG_HIP(hipGetLastError());
+    const size_t lb = lu_inverse_lds_doubles(nb_max_) * sizeof(double), lu = lu_inverse_lds_doubles(nu_max_) * sizeof(double);
+    const unsigned cb = (unsigned)((nb_max_ + 15) / 16), cu = (unsigned)((nu_max_ + 15) / 16);
+    hipLaunchKernelGGL((k_lu_inverse<NT>), dim3(nsys, cb), dim3(NT), lb, s, Lb.as<const double>(), pb.as<const int>(),
+                       Ib_.as<double>(), N, 0, sb, (size_t)nb_max_, sb, nb_max_);
+    hipLaunchKernelGGL((k_lu_inverse<NT>), dim3(nsys, cb), dim3(NT), lb, s, Lp.as<const double>(), pp.as<const int>(),
+                       Ip_.as<double>(), N, 0, sb, (size_t)nb_max_, sb, nb_max_);
+    hipLaunchKernelGGL((k_lu_inverse<NT>), dim3(nsys, cu), dim3(NT), lu, s, Lu.as<const double>(), pu.as<const int>(),
+                       Iu_.as<double>(), N, 1, su, (size_t)nu_max_, su, nu_max_);
     TG_HIP(hipGetLastError());
     std::vector<int> h((size_t)3 * nsys);
     TG_HIP(hipMemcpyAsync(h.data(), info, h.size() * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -358,14 +391,14 @@ void Channel3Solver::solve_t() {
     using V = cplx<T>;
     const int N = d_.nz;
     const double scale = (double)d_.nx * d_.ny;
-    Ch3SolveArgs sa{Ab_.as<const double>(), permb_.as<const int>(), Ap_.as<const double>(), permp_.as<const int>(),
-                    Aup_.as<const double>(), permu_.as<const int>(), order_.as<const int>(), key_.as<const int>(),
-                    rot_.as<const double>(), N, nmodes_, (size_t)nb_max_ * nb_max_, (size_t)nu_max_ * nu_max_,
-                    (size_t)nb_max_, (size_t)nu_max_, nu_max_, factored_c_, scale * d_.b_bottom, scale * d_.b_top};
+    Ch3SolveArgs sa{Ib_.as<const double>(), Ip_.as<const double>(), Iu_.as<const double>(),
+                    chunks_.as<const int4>(), modes_.as<const int>(), rot_.as<const double>(), N,
+                    (size_t)nb_max_ * nb_max_, (size_t)nu_max_ * nu_max_, factored_c_, scale * d_.b_bottom,
+                    scale * d_.b_top};
     constexpr int NT = 256;
-    const size_t lds = ch3_solve_lds_doubles(nu_max_, N) * sizeof(double);
+    const size_t lds = ch3_solve_lds_doubles(N) * sizeof(double);
     timed(6, [&] {
-        hipLaunchKernelGGL((k3_solve<T, NT>), dim3(2 * nmodes_), dim3(NT), lds, ctx_->stream, sa, R_[0].as<const V>(),
+        hipLaunchKernelGGL((k3_solve<T, NT>), dim3(nchunks_), dim3(NT), lds, ctx_->stream, sa, R_[0].as<const V>(),
                            R_[1].as<const V>(), R_[2].as<const V>(), R_[3].as<const V>(), R_[4].as<const V>(),
                            P_.as<V>(), B_.as<V>(), UX_.as<V>(), UY_.as<V>(), UZ_.as<V>(), TAU_.as<V>());
     });
@@ -573,12 +606,14 @@ void Channel3Solver::kernel_bytes(double* out) const {
     out[3] = (4 * pad + 4 * nc) * vb;                           // read 4 padded planes, write 4
     out[4] = (8 * nc) * vb + N * N * rb;                        // read 4 planes, write 4 rows
     out[5] = (4 * 4 + 4 + 5) * nc * vb;                         // state (x2 on the kx lines), F, histories, rhs
-    double fac = 0;                                             // every mode reads its key's factors
-    for (int q = 0; q < nmodes_; ++q)
+    double fac = 0;  // every chunk of up to 8 modes reads its key's three inverses once
+    for (int k = 0; k < nkeys_; ++k) {
+        const double chunks = (key_count_[k] + kCh3ChunkModes - 1) / kCh3ChunkModes;
         for (int g = 0; g < 2; ++g) {
-            const double nb = chb_size(d_.nz, g), nu = chu_size(d_.nz, g, q == 0 ? 0 : 1);
-            fac += (2 * nb * nb + nu * nu) * 8.0;
+            const double nb = chb_size(d_.nz, g ^ 1), np = chb_size(d_.nz, g), nu = chu_size(d_.nz, g, k);
+            fac += chunks * (nb * nb + np * np + nu * nu) * 8.0;
         }
+    }
     out[6] = fac + 10 * nc * vb;
     out[7] = 10 * nc * vb;  // L x: read p, b, u (and partners on two lines), write five rows
 }
@@ -590,8 +625,9 @@ size_t footprint(const Channel3Desc& d) {
     const size_t nb = chb_size(d.nz, 0), nu = chu_size(d.nz, 0, 0);
     // distinct |k_h|^2 keys: at most one per mode (about nmodes / 4 when Lx == Ly)
     const size_t keys = d.Lx == d.Ly ? nmodes / 3 + 64 : nmodes;
-    const size_t fac = 2 * keys * (2 * nb * nb + nu * nu) * 8 + 2 * keys * (2 * nb + nu) * 4;
-    return fields + fac + 3 * N * N * 8 + (1 << 20);
+    // inverses, plus the LU scratch while they are formed (same size, with pivots)
+    const size_t fac = 2 * keys * (2 * nb * nb + nu * nu) * 8;
+    return fields + 2 * fac + 2 * keys * (2 * nb + nu) * 4 + 3 * N * N * 8 + (1 << 20);
 }
 
 }  // namespace tarang

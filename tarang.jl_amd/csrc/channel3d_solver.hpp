@@ -1,9 +1,10 @@
 // channel3d_solver.hpp -- 3-D Rayleigh-Benard convection on RealFourier(x) x
 // RealFourier(y) x ChebyshevT(z) with the Chebyshev-tau method, per-mode
-// multistep (SBDF1/SBDF2) stepping on the GPU.  Per (kx, ky) mode the implicit
-// system is solved through three FP64 systems that depend on |k_h| only (buoyancy,
-// the velocity normal to k_h, and the u-p system along k_h), each split by parity;
-// modes with equal |k_h|^2 share factors.
+// multistep (SBDF1/SBDF2) or IMEX Runge-Kutta stepping on the GPU.  Per (kx, ky) mode
+// the implicit system is solved through three FP64 systems that depend on |k_h| only
+// (buoyancy, the velocity normal to k_h, and the u-p system along k_h), each split
+// by parity; modes with equal |k_h|^2 share one explicit inverse per system, applied
+// to up to 8 modes at once on the FP64 matrix cores.
 #pragma once
 #include <vector>
 
@@ -79,8 +80,10 @@ class Channel3Solver {
     DevBuf X0_[4], Frk_[4][4], LX_[4][5];  // IMEX RK stage storage
     DevBuf G_, Q_, H_;
     DevBuf BT_, BDT_, FT_;
-    DevBuf kh_, order_, key_, rot_;
-    DevBuf Ab_, Ap_, Aup_, permb_, permp_, permu_, info_;
+    DevBuf kh_, rot_, modes_, chunks_;
+    DevBuf Ib_, Ip_, Iu_, info_;  // explicit FP64 inverses per (key, parity) of the b, perp and u-p systems
+    int nchunks_ = 0;
+    std::vector<int> key_count_;  // modes per key (host copy, for the byte model)
     bool graph_on_ = true;
     hipGraphExec_t graph_ = nullptr;
     double graph_dt_ = 0, graph_dtp_ = 0;

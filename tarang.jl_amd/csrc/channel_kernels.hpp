@@ -750,6 +750,92 @@ __device__ void lu_solve_blocked(const double* __restrict__ a, const int* __rest
     }
 }
 
+// Explicit inverse from the LU factors: workgroup (system w, column block cb) solves
+// A x = e_c for the 16 columns c = 16 cb .. 16 cb + 15 and stores x as column c of
+// the inverse (column-major, leading dimension n, stride istride per system).
+// LDS: 16 ld + max(16 ld, 4096) doubles, ld >= n.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_lu_inverse(const double* __restrict__ LU, const int* __restrict__ perm,
+                                                   double* __restrict__ inv, int N, int kind, size_t stride,
+                                                   size_t pstride, size_t istride, int ld) {
+    extern __shared__ unsigned char smem_raw[];
+    const int w = blockIdx.x, m = w >> 1, g = w & 1;
+    const int n = kind == 0 ? chb_size(N, g) : chu_size(N, g, m);
+    const int c0 = 16 * (int)blockIdx.y;
+    if (c0 >= n) return;  // whole workgroup: no barrier reached
+    double* y = reinterpret_cast<double*>(smem_raw);  // [16][ld]
+    double* tmp = y + 16 * (size_t)ld;                 // [max(16 ld, 4096)], also the triangle block
+    for (int idx = threadIdx.x; idx < 16 * ld; idx += NT) {
+        const int r = idx / ld, i = idx - r * ld;
+        y[idx] = i == c0 + r ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    lu_solve_blocked<NT, 16>(LU + (size_t)w * stride, perm + (size_t)w * pstride, n, y, ld, tmp, tmp);
+    double* out = inv + (size_t)w * istride;
+    for (int r = 0; r < 16 && c0 + r < n; ++r)
+        for (int i = threadIdx.x; i < n; i += NT) out[(size_t)(c0 + r) * n + i] = y[r * ld + i];
+}
+
+__host__ __device__ inline size_t lu_inverse_lds_doubles(int ld) {
+    return 16 * (size_t)ld + (16 * (size_t)ld > 4096 ? 16 * (size_t)ld : 4096);
+}
+
+// ---------------------------------------------------------------------------
+// Dense products with FP64 matrix cores (v_mfma_f64_16x16x4f64, 64-lane wavefronts).
+// Operand layout of one product D(16x16) += A(16x4) B(4x16), lane l:
+//   A[l % 16][l / 16], B[l / 16][l % 16], D[4 v + l / 16][l % 16] in component v.
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+__device__ inline dbl4 mfma_f64_16x16x4(double a, double b, dbl4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// X = A Y for a column-major n x n matrix A (global, leading dimension n) and 16
+// right-hand sides Y[k*16 + c] in LDS (rows n .. 4 ceil(n/4) - 1 zero).  The 16-row
+// tiles t of X go to the wavefronts round robin from `wave0`: tile t belongs to wave
+// (t + wave0) % nw; acc[j] holds tile wave-local j.  Entries outside A count as zero
+// (their loads read element 0).
+template <int MT>
+__device__ inline void mfma_apply(const double* __restrict__ A, int n, const double* Y, dbl4 (&acc)[MT], int wave,
+                                  int nw, int wave0) {
+    const int lane = threadIdx.x & 63, r16 = lane & 15, q4 = lane >> 4;
+    const int ntiles = (n + 15) >> 4, ksteps = (n + 3) >> 2;
+    const int t0 = ((wave - wave0) % nw + nw) % nw;  // first tile of this wave
+#pragma unroll
+    for (int j = 0; j < MT; ++j) acc[j] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int ks = 0; ks < ksteps; ++ks) {
+        const int k = 4 * ks + q4;
+        const double b = Y[k * 16 + r16];
+        const bool kin = k < n;
+#pragma unroll
+        for (int j = 0; j < MT; ++j) {
+            const int t = t0 + j * nw;
+            if (t < ntiles) {  // wave-uniform
+                const int i = 16 * t + r16;
+                const bool in = kin && i < n;
+                const double a = A[in ? (size_t)k * n + i : 0];
+                acc[j] = mfma_f64_16x16x4(in ? a : 0.0, b, acc[j]);
+            }
+        }
+    }
+}
+
+// Store the tiles of mfma_apply into X[i*16 + c] (LDS, rows up to 16 ceil(n/16)).
+template <int MT>
+__device__ inline void mfma_store(const dbl4 (&acc)[MT], int n, double* X, int wave, int nw, int wave0) {
+    const int lane = threadIdx.x & 63, r16 = lane & 15, q4 = lane >> 4;
+    const int ntiles = (n + 15) >> 4;
+    const int t0 = ((wave - wave0) % nw + nw) % nw;
+#pragma unroll
+    for (int j = 0; j < MT; ++j) {
+        const int t = t0 + j * nw;
+        if (t < ntiles)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) X[(16 * t + 4 * v + q4) * 16 + r16] = acc[j][v];
+    }
+}
+
 struct ChSolveArgs {
     const double* Ab;   // b-system factors, stride sb
     const int* permb;   // stride pb

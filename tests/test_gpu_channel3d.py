@@ -109,6 +109,37 @@ def test_channel3d_graph_and_keys(gpu, oracle):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_channel3d_multichunk_keys(gpu, oracle, dtype):
+    """64 x 64 x 9 with Lx == Ly: |k_h|^2 keys of up to 12 modes, so one key's modes span two
+    8-mode workgroups of the solve; every field against the oracle after SBDF1 + SBDF2 steps."""
+    nx = ny = 64
+    N, L = 9, 2 * np.pi
+    print(f"channel3d multichunk {dtype.__name__}", flush=True)
+    counts = {}
+    for m in range(nx // 2 + 1):
+        for k in range(-ny // 2, ny // 2):
+            counts[m * m + k * k] = counts.get(m * m + k * k, 0) + 1
+    assert max(counts.values()) > 8
+    kappa, nu = kp()
+    st, _ = random_state3(oracle, nx, ny, N, L, L, 1.0, seed=13)
+    ref = oracle.RBC3D(nx, ny, N, L, L, 1.0, kappa, nu, 5e-3, "SBDF2")
+    s = gpu.chan3(nx, ny, N, L, L, 1.0, kappa, nu, 5e-3, stepper=2, dtype=dtype)
+    assert s.nkeys() == len(counts)
+    for v, c in st.items():
+        ref.set(v, c)
+        s.set(v, c)
+    ref.step(2)
+    s.step(2)
+    for v in (oracle.RBC3_B, oracle.RBC3_UX, oracle.RBC3_UY, oracle.RBC3_UZ):
+        assert relerr(s.get(v), ref.get(v)) < TOL[dtype], v
+    assert relerr(s.get(oracle.RBC3_P), ref.get(oracle.RBC3_P)) < TOL[dtype] * 100
+    scale = max(np.abs(ref.tau(t)).max() for t in range(9))
+    for t in range(9):
+        assert np.abs(s.tau(t) - ref.tau(t)).max() <= TOL[dtype] * 100 * scale, t
+    s.close()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_channel3d_full_size_modewise(gpu, oracle, dtype):
     """256 x 256 x 128 (the 3-D workload) vs a 16 x 16 x 128 oracle on modes kx, ky in {0, 1}."""
     nx = ny = 256
