@@ -315,9 +315,9 @@ static __global__ void k3_assemble(Ch3Sys s, double* Ab, double* Aup, double* Ap
 }
 
 struct Ch3SolveArgs {
-    const double* Ib;   // b-system inverses, stride sb per (key, parity), column-major, ld = n
-    const double* Ip;   // perp-system inverses, stride sb
-    const double* Iu;   // u-p-system inverses, stride su
+    const void* Ib;     // b-system inverses (state precision), stride sb per (key, parity), column-major, ld = n
+    const void* Ip;     // perp-system inverses, stride sb
+    const void* Iu;     // u-p-system inverses, stride su
     const int4* chunks; // [nchunks] (key, g, first, count): modes[first .. first + count), count <= 8
     const int* modes;   // modes grouped by key
     const double* rot;  // [2 nmodes] (cos, sin) of the k_h direction per mode
@@ -340,7 +340,9 @@ __host__ __device__ inline size_t ch3_solve_lds_doubles(int N) {
 // Workgroup = (|k_h| key, parity g, up to 8 modes of that key).  Right-hand sides of
 // the modes are the columns of 16-column blocks in LDS (column 2 j = real, 2 j + 1 =
 // imaginary part of mode j); every system is applied as X = A^-1 Y on the FP64 matrix
-// cores, so each inverse is read once per chunk.  Order: b system of parity h = g^1
+// cores, so each inverse is read once per chunk.  The inverses are stored in the state
+// precision T (formed in FP64; rounding them to FP32 costs the same relative error as
+// rounding the right-hand side to FP32), the products accumulate in FP64.  Order: b system of parity h = g^1
 // and the perp system of parity g together, then the u-p system of parity g (its
 // E3z rows take + c b), then the un-rotation of the parity-g horizontal velocity.
 template <typename T, int NT>
@@ -405,8 +407,8 @@ __global__ __launch_bounds__(NT) void k3_solve(Ch3SolveArgs s, const cplx<T>* __
     const int wave = threadIdx.x >> 6, nw = NT / 64, w0p = ((nb + 15) >> 4) % nw;
     {
         dbl4 ab[3], ap[3];
-        mfma_apply<3>(s.Ib + (size_t)(2 * key + h) * s.sb, nb, Yb, ab, wave, nw, 0);
-        mfma_apply<3>(s.Ip + (size_t)(2 * key + g) * s.sb, np, Yp, ap, wave, nw, w0p);
+        mfma_apply<3>(static_cast<const T*>(s.Ib) + (size_t)(2 * key + h) * s.sb, nb, Yb, ab, wave, nw, 0);
+        mfma_apply<3>(static_cast<const T*>(s.Ip) + (size_t)(2 * key + g) * s.sb, np, Yp, ap, wave, nw, w0p);
         __syncthreads();
         mfma_store<3>(ab, nb, Yb, wave, nw, 0);
         mfma_store<3>(ap, np, Yp, wave, nw, w0p);
@@ -428,7 +430,7 @@ __global__ __launch_bounds__(NT) void k3_solve(Ch3SolveArgs s, const cplx<T>* __
     // ---- u-p system of parity g ----
     {
         dbl4 au[7];
-        mfma_apply<7>(s.Iu + (size_t)(2 * key + g) * s.su, nu, Yu, au, wave, nw, 0);
+        mfma_apply<7>(static_cast<const T*>(s.Iu) + (size_t)(2 * key + g) * s.su, nu, Yu, au, wave, nw, 0);
         __syncthreads();
         mfma_store<7>(au, nu, Yu, wave, nw, 0);
     }

@@ -161,9 +161,10 @@ Channel3Solver::Channel3Solver(Context* ctx, const Channel3Desc& d) : ctx_(ctx),
     up(modes_, modes.data(), modes.size() * sizeof(int));
     up(chunks_, chunks.data(), chunks.size() * sizeof(int));
     up(rot_, rot.data(), rot.size() * sizeof(double));
-    Ib_ = DevBuf(ctx_, (size_t)2 * nkeys_ * nb_max_ * nb_max_ * sizeof(double));
-    Ip_ = DevBuf(ctx_, (size_t)2 * nkeys_ * nb_max_ * nb_max_ * sizeof(double));
-    Iu_ = DevBuf(ctx_, (size_t)2 * nkeys_ * nu_max_ * nu_max_ * sizeof(double));
+    const size_t ib = dtype_size((DType)d.dtype);  // inverses in the state precision
+    Ib_ = DevBuf(ctx_, (size_t)2 * nkeys_ * nb_max_ * nb_max_ * ib);
+    Ip_ = DevBuf(ctx_, (size_t)2 * nkeys_ * nb_max_ * nb_max_ * ib);
+    Iu_ = DevBuf(ctx_, (size_t)2 * nkeys_ * nu_max_ * nu_max_ * ib);
     info_ = DevBuf(ctx_, (size_t)6 * nkeys_ * sizeof(int));
     ctx_->twiddle(mx_, (DType)d.dtype);
     ctx_->twiddle(my_, (DType)d.dtype);
@@ -259,12 +260,17 @@ void Channel3Solver::factor(double c) {
     TG_HIP(hipGetLastError());
     const size_t lb = lu_inverse_lds_doubles(nb_max_) * sizeof(double), lu = lu_inverse_lds_doubles(nu_max_) * sizeof(double);
     const unsigned cb = (unsigned)((nb_max_ + 15) / 16), cu = (unsigned)((nu_max_ + 15) / 16);
-    hipLaunchKernelGGL((k_lu_inverse<NT>), dim3(nsys, cb), dim3(NT), lb, s, Lb.as<const double>(), pb.as<const int>(),
-                       Ib_.as<double>(), N, 0, sb, (size_t)nb_max_, sb, nb_max_);
-    hipLaunchKernelGGL((k_lu_inverse<NT>), dim3(nsys, cb), dim3(NT), lb, s, Lp.as<const double>(), pp.as<const int>(),
-                       Ip_.as<double>(), N, 0, sb, (size_t)nb_max_, sb, nb_max_);
-    hipLaunchKernelGGL((k_lu_inverse<NT>), dim3(nsys, cu), dim3(NT), lu, s, Lu.as<const double>(), pu.as<const int>(),
-                       Iu_.as<double>(), N, 1, su, (size_t)nu_max_, su, nu_max_);
+    auto invert = [&](auto ot) {
+        using OT = decltype(ot);
+        hipLaunchKernelGGL((k_lu_inverse<NT, OT>), dim3(nsys, cb), dim3(NT), lb, s, Lb.as<const double>(),
+                           pb.as<const int>(), Ib_.as<OT>(), N, 0, sb, (size_t)nb_max_, sb, nb_max_);
+        hipLaunchKernelGGL((k_lu_inverse<NT, OT>), dim3(nsys, cb), dim3(NT), lb, s, Lp.as<const double>(),
+                           pp.as<const int>(), Ip_.as<OT>(), N, 0, sb, (size_t)nb_max_, sb, nb_max_);
+        hipLaunchKernelGGL((k_lu_inverse<NT, OT>), dim3(nsys, cu), dim3(NT), lu, s, Lu.as<const double>(),
+                           pu.as<const int>(), Iu_.as<OT>(), N, 1, su, (size_t)nu_max_, su, nu_max_);
+    };
+    if (d_.dtype == kF32) invert(float{});
+    else invert(double{});
     TG_HIP(hipGetLastError());
     std::vector<int> h((size_t)3 * nsys);
     TG_HIP(hipMemcpyAsync(h.data(), info, h.size() * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -391,7 +397,7 @@ void Channel3Solver::solve_t() {
     using V = cplx<T>;
     const int N = d_.nz;
     const double scale = (double)d_.nx * d_.ny;
-    Ch3SolveArgs sa{Ib_.as<const double>(), Ip_.as<const double>(), Iu_.as<const double>(),
+    Ch3SolveArgs sa{Ib_.ptr, Ip_.ptr, Iu_.ptr,
                     chunks_.as<const int4>(), modes_.as<const int>(), rot_.as<const double>(), N,
                     (size_t)nb_max_ * nb_max_, (size_t)nu_max_ * nu_max_, factored_c_, scale * d_.b_bottom,
                     scale * d_.b_top};
@@ -611,7 +617,7 @@ void Channel3Solver::kernel_bytes(double* out) const {
         const double chunks = (key_count_[k] + kCh3ChunkModes - 1) / kCh3ChunkModes;
         for (int g = 0; g < 2; ++g) {
             const double nb = chb_size(d_.nz, g ^ 1), np = chb_size(d_.nz, g), nu = chu_size(d_.nz, g, k);
-            fac += chunks * (nb * nb + np * np + nu * nu) * 8.0;
+            fac += chunks * (nb * nb + np * np + nu * nu) * rb;
         }
     }
     out[6] = fac + 10 * nc * vb;
@@ -625,9 +631,9 @@ size_t footprint(const Channel3Desc& d) {
     const size_t nb = chb_size(d.nz, 0), nu = chu_size(d.nz, 0, 0);
     // distinct |k_h|^2 keys: at most one per mode (about nmodes / 4 when Lx == Ly)
     const size_t keys = d.Lx == d.Ly ? nmodes / 3 + 64 : nmodes;
-    // inverses, plus the LU scratch while they are formed (same size, with pivots)
-    const size_t fac = 2 * keys * (2 * nb * nb + nu * nu) * 8;
-    return fields + 2 * fac + 2 * keys * (2 * nb + nu) * 4 + 3 * N * N * 8 + (1 << 20);
+    // inverses in the state precision, plus the FP64 LU scratch while they are formed
+    const size_t ent = 2 * keys * (2 * nb * nb + nu * nu);
+    return fields + ent * (8 + dtype_size((DType)d.dtype)) + 2 * keys * (2 * nb + nu) * 4 + 3 * N * N * 8 + (1 << 20);
 }
 
 }  // namespace tarang

@@ -751,12 +751,12 @@ __device__ void lu_solve_blocked(const double* __restrict__ a, const int* __rest
 }
 
 // Explicit inverse from the LU factors: workgroup (system w, column block cb) solves
-// A x = e_c for the 16 columns c = 16 cb .. 16 cb + 15 and stores x as column c of
-// the inverse (column-major, leading dimension n, stride istride per system).
-// LDS: 16 ld + max(16 ld, 4096) doubles, ld >= n.
-template <int NT>
+// A x = e_c for the 16 columns c = 16 cb .. 16 cb + 15 in FP64 and stores x, rounded
+// to OT, as column c of the inverse (column-major, leading dimension n, stride
+// istride per system).  LDS: 16 ld + max(16 ld, 4096) doubles, ld >= n.
+template <int NT, typename OT>
 __global__ __launch_bounds__(NT) void k_lu_inverse(const double* __restrict__ LU, const int* __restrict__ perm,
-                                                   double* __restrict__ inv, int N, int kind, size_t stride,
+                                                   OT* __restrict__ inv, int N, int kind, size_t stride,
                                                    size_t pstride, size_t istride, int ld) {
     extern __shared__ unsigned char smem_raw[];
     const int w = blockIdx.x, m = w >> 1, g = w & 1;
@@ -771,9 +771,9 @@ __global__ __launch_bounds__(NT) void k_lu_inverse(const double* __restrict__ LU
     }
     __syncthreads();
     lu_solve_blocked<NT, 16>(LU + (size_t)w * stride, perm + (size_t)w * pstride, n, y, ld, tmp, tmp);
-    double* out = inv + (size_t)w * istride;
+    OT* out = inv + (size_t)w * istride;
     for (int r = 0; r < 16 && c0 + r < n; ++r)
-        for (int i = threadIdx.x; i < n; i += NT) out[(size_t)(c0 + r) * n + i] = y[r * ld + i];
+        for (int i = threadIdx.x; i < n; i += NT) out[(size_t)(c0 + r) * n + i] = OT(y[r * ld + i]);
 }
 
 __host__ __device__ inline size_t lu_inverse_lds_doubles(int ld) {
@@ -790,34 +790,58 @@ __device__ inline dbl4 mfma_f64_16x16x4(double a, double b, dbl4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-// X = A Y for a column-major n x n matrix A (global, leading dimension n) and 16
-// right-hand sides Y[k*16 + c] in LDS (rows n .. 4 ceil(n/4) - 1 zero).  The 16-row
+// X = A Y for a column-major n x n matrix A (global, leading dimension n, FP64 or
+// FP32 entries, FP64 products) and 16 right-hand sides Y[k*16 + c] in LDS (rows
+// n .. 16 ceil(n/16) - 1 zero).  The 16-row
 // tiles t of X go to the wavefronts round robin from `wave0`: tile t belongs to wave
 // (t + wave0) % nw; acc[j] holds tile wave-local j.  Entries outside A count as zero
-// (their loads read element 0).
-template <int MT>
-__device__ inline void mfma_apply(const double* __restrict__ A, int n, const double* Y, dbl4 (&acc)[MT], int wave,
-                                  int nw, int wave0) {
+// (their loads read element 0).  The k loop runs in batches of U steps whose loads
+// are all issued before the batch's products (NTW = this wave's tile count, so the
+// batch has no branches).
+template <int NTW, int U, typename AT>
+__device__ inline void mfma_apply_n(const AT* __restrict__ A, int n, const double* Y, dbl4* acc, int t0, int nw) {
     const int lane = threadIdx.x & 63, r16 = lane & 15, q4 = lane >> 4;
-    const int ntiles = (n + 15) >> 4, ksteps = (n + 3) >> 2;
-    const int t0 = ((wave - wave0) % nw + nw) % nw;  // first tile of this wave
+    const int ksteps = (n + 3) >> 2, krow_max = ((n + 15) & ~15) - 1;
+    int ii[NTW];
 #pragma unroll
-    for (int j = 0; j < MT; ++j) acc[j] = dbl4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll 4
-    for (int ks = 0; ks < ksteps; ++ks) {
-        const int k = 4 * ks + q4;
-        const double b = Y[k * 16 + r16];
-        const bool kin = k < n;
+    for (int j = 0; j < NTW; ++j) ii[j] = 16 * (t0 + j * nw) + r16;
+    for (int ks = 0; ks < ksteps; ks += U) {
+        double a[U][NTW], b[U];
 #pragma unroll
-        for (int j = 0; j < MT; ++j) {
-            const int t = t0 + j * nw;
-            if (t < ntiles) {  // wave-uniform
-                const int i = 16 * t + r16;
-                const bool in = kin && i < n;
-                const double a = A[in ? (size_t)k * n + i : 0];
-                acc[j] = mfma_f64_16x16x4(in ? a : 0.0, b, acc[j]);
+        for (int u = 0; u < U; ++u) {
+            const int k = 4 * (ks + u) + q4;
+            b[u] = Y[min(k, krow_max) * 16 + r16];
+#pragma unroll
+            for (int j = 0; j < NTW; ++j) {
+                const bool in = k < n && ii[j] < n;
+                a[u][j] = (double)A[in ? k * n + ii[j] : 0];  // n <= 512: 32-bit offsets from a uniform base
+                a[u][j] = in ? a[u][j] : 0.0;
             }
         }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int j = 0; j < NTW; ++j) acc[j] = mfma_f64_16x16x4(a[u][j], b[u], acc[j]);
+    }
+}
+
+template <int MT, typename AT>
+__device__ inline void mfma_apply(const AT* __restrict__ A, int n, const double* Y, dbl4 (&acc)[MT], int wave,
+                                  int nw, int wave0) {
+    const int ntiles = (n + 15) >> 4;
+    const int t0 = ((wave - wave0) % nw + nw) % nw;  // first tile of this wave
+    const int ntw = t0 < ntiles ? (ntiles - t0 + nw - 1) / nw : 0;
+#pragma unroll
+    for (int j = 0; j < MT; ++j) acc[j] = dbl4{0.0, 0.0, 0.0, 0.0};
+    switch (ntw < MT ? ntw : MT) {  // wave-uniform
+        case 1: mfma_apply_n<1, 8>(A, n, Y, acc, t0, nw); break;
+        case 2: if constexpr (MT >= 2) mfma_apply_n<2, 8>(A, n, Y, acc, t0, nw); break;
+        case 3: if constexpr (MT >= 3) mfma_apply_n<3, 8>(A, n, Y, acc, t0, nw); break;
+        case 4: if constexpr (MT >= 4) mfma_apply_n<4, 8>(A, n, Y, acc, t0, nw); break;
+        case 5: if constexpr (MT >= 5) mfma_apply_n<5, 4>(A, n, Y, acc, t0, nw); break;
+        case 6: if constexpr (MT >= 6) mfma_apply_n<6, 4>(A, n, Y, acc, t0, nw); break;
+        case 7: if constexpr (MT >= 7) mfma_apply_n<7, 4>(A, n, Y, acc, t0, nw); break;
+        default: break;
     }
 }
 
