@@ -35,68 +35,118 @@ __host__ __device__ inline double ch3_ky(const Ch3Geom& g, int n) {
 }
 
 // ---------------------------------------------------------------------------
-// k3_zbwd: z-backward transform of the operands.  Work item (tile of TK modes, job):
-// job = 2 f + d, f in (b, ux, uy, uz), d = 0 (B: value, dx, dy) / 1 (BD: dz).
-// On the kx = 0 and kx = Nx/2 lines the projected state's x derivative vanishes,
-// and so does its y derivative at the y Nyquist; the kx = Nx/2 line is halved.
+// k3_zbwd: z-backward transform of the operands on the matrix cores.  Workgroup =
+// (tile of QT modes, field f in (b, ux, uy, uz)); the field's coefficients of the
+// tile are the columns of an LDS block C[k][2 q + re/im] and
+//   value[z][col] = sum_k BT[k][z] C[k][col],   dz[z][col] = sum_k BDT[k][z] C[k][col]
+// as 16 x 16 tiles (wave w: z tiles 2w, 2w+1 (+8, ...), every column tile, both
+// products).  The accumulators are written straight to the z-major planes: a row of
+// 16 columns is 8 consecutive modes.  Derived planes: dx f = i kx value, dy f = i ky
+// value (the partner re/im lane by shuffle).  On the kx = 0 and kx = Nx/2 lines the
+// projected state's x derivative vanishes, and so does its y derivative at the y
+// Nyquist; the kx = Nx/2 line is halved.
 // ---------------------------------------------------------------------------
-template <typename T, int TK, int NT>
+template <typename T> struct ZbwdTile { static constexpr int QT = 32; };   // 64 columns
+template <> struct ZbwdTile<double> { static constexpr int QT = 16; };     // 32 columns
+
+__host__ __device__ inline int zbwd_lds_stride(int qt) { return 2 * qt + 1; }
+
+template <typename T>
+__device__ inline T lane_xor1(T v) {
+    if constexpr (sizeof(T) == 4) {
+        return __shfl_xor(v, 1);
+    } else {
+        const int lo = __shfl_xor(__double2loint(v), 1), hi = __shfl_xor(__double2hiint(v), 1);
+        return __hiloint2double(hi, lo);
+    }
+}
+
+template <typename T, int NT>
 __global__ __launch_bounds__(NT) void k3_zbwd(const cplx<T>* __restrict__ b, const cplx<T>* __restrict__ ux,
                                               const cplx<T>* __restrict__ uy, const cplx<T>* __restrict__ uz,
                                               Ch3Geom geo, const T* __restrict__ BT, const T* __restrict__ BDT,
-                                              cplx<T>* __restrict__ G) {
-    using V = cplx<T>;
+                                              T* __restrict__ G) {
+    static_assert(NT == 256, "four wavefronts per workgroup");
+    using Op = MfmaOp<T>;
+    using acc4 = typename Op::acc4;
+    constexpr int QT = ZbwdTile<T>::QT, CT = 2 * QT / 16, S = 2 * QT + 1;
     extern __shared__ unsigned char smem_raw[];
-    V* src = reinterpret_cast<V*>(smem_raw);  // [TK][N]
+    T* C = reinterpret_cast<T*>(smem_raw);  // [KP][S]
     const int N = geo.N, nmodes = geo.nmodes;
-    const int f = blockIdx.y >> 1, d = blockIdx.y & 1;
-    const int q0 = blockIdx.x * TK;
-    const V* in = f == 0 ? b : (f == 1 ? ux : (f == 2 ? uy : uz));
-    const T* mat = d == 0 ? BT : BDT;
-    for (int idx = threadIdx.x; idx < TK * N; idx += NT) {
-        const int c = idx / N, k = idx - c * N;
-        const int q = q0 + c;
-        src[c * N + k] = q < nmodes ? in[(size_t)q * N + k] : V{T(0), T(0)};
+    const int KP = (N + 15) & ~15, nzt = KP >> 4, ksteps = (N + 3) >> 2;
+    const int f = blockIdx.y, q0 = blockIdx.x * QT;
+    const cplx<T>* in = f == 0 ? b : (f == 1 ? ux : (f == 2 ? uy : uz));
+    for (int idx = threadIdx.x; idx < QT * KP; idx += NT) {
+        const int q = idx / KP, k = idx - q * KP;
+        cplx<T> v{T(0), T(0)};
+        if (q0 + q < nmodes && k < N) v = in[(size_t)(q0 + q) * N + k];
+        C[k * S + 2 * q] = v.x;
+        C[k * S + 2 * q + 1] = v.y;
     }
     __syncthreads();
+    const int lane = threadIdx.x & 63, r16 = lane & 15, q4 = lane >> 4, wave = threadIdx.x >> 6;
     const size_t plane = (size_t)N * nmodes;
     const bool xeven = (geo.nx % 2) == 0, yeven = (geo.ny % 2) == 0;
-    for (int z = threadIdx.x; z < N; z += NT) {
-        V acc[TK];
+    // per column tile: this lane's mode and its derivative factors
+    T kxv[CT], kyv[CT], wv[CT];
+    bool dxz[CT], dyz[CT], live[CT];
+    int qq[CT];
 #pragma unroll
-        for (int c = 0; c < TK; ++c) acc[c] = V{T(0), T(0)};
-        for (int k = 0; k < N; ++k) {
-            const T a = mat[(size_t)k * N + z];
+    for (int ct = 0; ct < CT; ++ct) {
+        const int q = q0 + (16 * ct + r16) / 2;
+        qq[ct] = q;
+        live[ct] = q < nmodes;
+        const int m = q % geo.nxh, n = q / geo.nxh;
+        const bool xny = xeven && m == geo.nx / 2, xline = m == 0 || xny;
+        kxv[ct] = T(geo.k0x * m);
+        kyv[ct] = T(ch3_ky(geo, n));
+        wv[ct] = xny ? T(0.5) : T(1);
+        dxz[ct] = xline;
+        dyz[ct] = xline && yeven && n == geo.ny / 2;
+    }
+    const bool im = (r16 & 1) != 0;  // this lane holds the imaginary part
+    for (int zt0 = 2 * wave; zt0 < nzt; zt0 += 8) {
+        acc4 acc[2][2][CT];
 #pragma unroll
-            for (int c = 0; c < TK; ++c) {
-                const V s = src[c * N + k];
-                acc[c].x += a * s.x;
-                acc[c].y += a * s.y;
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int p = 0; p < 2; ++p)
+#pragma unroll
+                for (int ct = 0; ct < CT; ++ct) acc[a][p][ct] = acc4{T(0), T(0), T(0), T(0)};
+        const int z0 = 16 * zt0 + r16, z1 = z0 + 16;
+        for (int ks = 0; ks < ksteps; ++ks) {
+            const int k = 4 * ks + q4;
+            const bool kin = k < N;
+            const T a00 = (kin && z0 < N) ? BT[k * N + z0] : T(0), a01 = (kin && z0 < N) ? BDT[k * N + z0] : T(0);
+            const T a10 = (kin && z1 < N) ? BT[k * N + z1] : T(0), a11 = (kin && z1 < N) ? BDT[k * N + z1] : T(0);
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct) {
+                const T bv = C[k * S + 16 * ct + r16];  // rows N .. KP - 1 are zero
+                acc[0][0][ct] = Op::mma(a00, bv, acc[0][0][ct]);
+                acc[0][1][ct] = Op::mma(a01, bv, acc[0][1][ct]);
+                acc[1][0][ct] = Op::mma(a10, bv, acc[1][0][ct]);
+                acc[1][1][ct] = Op::mma(a11, bv, acc[1][1][ct]);
             }
         }
 #pragma unroll
-        for (int c = 0; c < TK; ++c) {
-            const int q = q0 + c;
-            if (q >= nmodes) continue;
-            const int m = q % geo.nxh, n = q / geo.nxh;
-            const bool xline = m == 0 || (xeven && m == geo.nx / 2);
-            const T w = (xeven && m == geo.nx / 2) ? T(0.5) : T(1);
-            const V v{acc[c].x * w, acc[c].y * w};
-            const size_t o = (size_t)z * nmodes + q;
-            if (d == 1) {
-                G[(size_t)(5 + 3 * f) * plane + o] = v;  // dz f
-                continue;
-            }
-            const T kx = T(geo.k0x * m), ky = T(ch3_ky(geo, n));
-            V dx{-kx * v.y, kx * v.x}, dy{-ky * v.y, ky * v.x};
-            if (xline) {
-                dx = V{T(0), T(0)};
-                if (yeven && n == geo.ny / 2) dy = V{T(0), T(0)};
-            }
-            if (f > 0) G[(size_t)(f - 1) * plane + o] = v;  // ux, uy, uz
-            G[(size_t)(3 + 3 * f) * plane + o] = dx;
-            G[(size_t)(4 + 3 * f) * plane + o] = dy;
-        }
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int z = 16 * (zt0 + a) + Op::row(v, q4);
+                    const T val = acc[a][0][ct][v] * wv[ct], dzv = acc[a][1][ct][v] * wv[ct];
+                    const T other = lane_xor1(val);  // the partner part of the same mode
+                    if (z >= N || !live[ct]) continue;
+                    // i k v: real part -k Im v, imaginary part k Re v
+                    const T dx = dxz[ct] ? T(0) : (im ? kxv[ct] * other : -kxv[ct] * other);
+                    const T dy = dyz[ct] ? T(0) : (im ? kyv[ct] * other : -kyv[ct] * other);
+                    const size_t o = 2 * ((size_t)z * nmodes + qq[ct]) + (im ? 1 : 0);
+                    if (f > 0) G[2 * (size_t)(f - 1) * plane + o] = val;  // ux, uy, uz
+                    G[2 * (size_t)(3 + 3 * f) * plane + o] = dx;
+                    G[2 * (size_t)(4 + 3 * f) * plane + o] = dy;
+                    G[2 * (size_t)(5 + 3 * f) * plane + o] = dzv;
+                }
     }
 }
 

@@ -297,18 +297,18 @@ void Channel3Solver::rhs_t(void* F0, void* F1, void* F2, void* F3) {
     V* G = G_.as<V>();
     V* Q = Q_.as<V>();
     V* H = H_.as<V>();
-    constexpr int TKZ = 4;
+    constexpr int TKZ = 4;  // modes per workgroup of the z-forward transform
     const size_t zlds = (size_t)TKZ * N * sizeof(V);
     const unsigned ztiles = (unsigned)((nmodes_ + TKZ - 1) / TKZ);
-    auto zb = [&](auto NTC) {
-        constexpr int NT = decltype(NTC)::value;
+    {
+        constexpr int QT = ZbwdTile<T>::QT;
+        const size_t blds = (size_t)((N + 15) & ~15) * zbwd_lds_stride(QT) * sizeof(T);
         timed(0, [&] {
-            hipLaunchKernelGGL((k3_zbwd<T, TKZ, NT>), dim3(ztiles, 8), dim3(NT), zlds, s, B_.as<const V>(),
-                               UX_.as<const V>(), UY_.as<const V>(), UZ_.as<const V>(), geo, BT_.as<const T>(),
-                               BDT_.as<const T>(), G);
+            hipLaunchKernelGGL((k3_zbwd<T, 256>), dim3((unsigned)((nmodes_ + QT - 1) / QT), 4), dim3(256), blds, s,
+                               B_.as<const V>(), UX_.as<const V>(), UY_.as<const V>(), UZ_.as<const V>(), geo,
+                               BT_.as<const T>(), BDT_.as<const T>(), reinterpret_cast<T*>(G));
         });
-    };
-    if (N <= 64) zb(std::integral_constant<int, 64>{}); else zb(std::integral_constant<int, 256>{});
+    }
     TG_HIP(hipGetLastError());
     dispatch_size(d_.ny, [&](auto NC) {
         constexpr int NY = decltype(NC)::value;
@@ -606,7 +606,7 @@ void Channel3Solver::kernel_bytes(double* out) const {
     const double vb = 2.0 * dtype_size((DType)d_.dtype), rb = dtype_size((DType)d_.dtype);
     const double nc = (double)nmodes_ * d_.nz, N = d_.nz;
     const double pad = (double)d_.nz * my_ * nxh_;
-    out[0] = (8 * nc + kCh3Planes * nc) * vb + 2 * N * N * rb;  // read 4 fields x 2 jobs, write 15 planes
+    out[0] = (4 * nc + kCh3Planes * nc) * vb + 2 * N * N * rb;  // read 4 fields, write 15 planes
     out[1] = (kCh3Planes * nc + kCh3Planes * pad) * vb;         // read 15 planes, write 15 y-padded planes
     out[2] = (kCh3Planes * pad + 4 * pad) * vb;                 // read 15 padded planes, write 4
     out[3] = (4 * pad + 4 * nc) * vb;                           // read 4 padded planes, write 4
