@@ -37,76 +37,69 @@ __host__ __device__ inline double ch3_ky(const Ch3Geom& g, int n) {
 // ---------------------------------------------------------------------------
 // k3_zbwd: z-backward transform of the operands on the matrix cores.  Workgroup =
 // (tile of QT modes, field f in (b, ux, uy, uz)); the field's coefficients of the
-// tile are the columns of an LDS block C[k][2 q + re/im] and
+// tile are the columns of an LDS block C[k][col], in groups of 16 modes: column
+// 32 g + j is Re, 32 g + 16 + j is Im of mode q0 + 16 g + j, and
 //   value[z][col] = sum_k BT[k][z] C[k][col],   dz[z][col] = sum_k BDT[k][z] C[k][col]
 // as 16 x 16 tiles (wave w: z tiles 2w, 2w+1 (+8, ...), every column tile, both
-// products).  The accumulators are written straight to the z-major planes: a row of
-// 16 columns is 8 consecutive modes.  Derived planes: dx f = i kx value, dy f = i ky
-// value (the partner re/im lane by shuffle).  On the kx = 0 and kx = Nx/2 lines the
-// projected state's x derivative vanishes, and so does its y derivative at the y
-// Nyquist; the kx = Nx/2 line is halved.
+// products).  A lane then holds Re and Im of one mode in two accumulators and writes
+// whole complex values straight to the z-major planes (16 consecutive modes per row).
+// Derived planes: dx f = i kx value, dy f = i ky value.  On the kx = 0 and kx = Nx/2
+// lines the projected state's x derivative vanishes, and so does its y derivative at
+// the y Nyquist; the kx = Nx/2 line is halved.
 // ---------------------------------------------------------------------------
-template <typename T> struct ZbwdTile { static constexpr int QT = 32; };   // 64 columns
-template <> struct ZbwdTile<double> { static constexpr int QT = 16; };     // 32 columns
+// modes per workgroup (QT, a multiple of 16) and k steps per load batch (U)
+template <typename T> struct ZbwdTile { static constexpr int QT = 32, U = 2; };
+template <> struct ZbwdTile<double> { static constexpr int QT = 16, U = 4; };
 
 __host__ __device__ inline int zbwd_lds_stride(int qt) { return 2 * qt + 1; }
 
-template <typename T>
-__device__ inline T lane_xor1(T v) {
-    if constexpr (sizeof(T) == 4) {
-        return __shfl_xor(v, 1);
-    } else {
-        const int lo = __shfl_xor(__double2loint(v), 1), hi = __shfl_xor(__double2hiint(v), 1);
-        return __hiloint2double(hi, lo);
-    }
-}
-
 template <typename T, int NT>
-__global__ __launch_bounds__(NT) void k3_zbwd(const cplx<T>* __restrict__ b, const cplx<T>* __restrict__ ux,
-                                              const cplx<T>* __restrict__ uy, const cplx<T>* __restrict__ uz,
-                                              Ch3Geom geo, const T* __restrict__ BT, const T* __restrict__ BDT,
-                                              T* __restrict__ G) {
+__global__ __launch_bounds__(NT, 3) void k3_zbwd(const cplx<T>* __restrict__ b, const cplx<T>* __restrict__ ux,
+                                                 const cplx<T>* __restrict__ uy, const cplx<T>* __restrict__ uz,
+                                                 Ch3Geom geo, const T* __restrict__ BT, const T* __restrict__ BDT,
+                                                 cplx<T>* __restrict__ G) {
     static_assert(NT == 256, "four wavefronts per workgroup");
     using Op = MfmaOp<T>;
     using acc4 = typename Op::acc4;
-    constexpr int QT = ZbwdTile<T>::QT, CT = 2 * QT / 16, S = 2 * QT + 1;
+    using V = cplx<T>;
+    constexpr int QT = ZbwdTile<T>::QT, NG = QT / 16, CT = 2 * NG, S = 2 * QT + 1;
     extern __shared__ unsigned char smem_raw[];
     T* C = reinterpret_cast<T*>(smem_raw);  // [KP][S]
     const int N = geo.N, nmodes = geo.nmodes;
     const int KP = (N + 15) & ~15, nzt = KP >> 4, ksteps = (N + 3) >> 2;
     const int f = blockIdx.y, q0 = blockIdx.x * QT;
-    const cplx<T>* in = f == 0 ? b : (f == 1 ? ux : (f == 2 ? uy : uz));
+    const V* in = f == 0 ? b : (f == 1 ? ux : (f == 2 ? uy : uz));
     for (int idx = threadIdx.x; idx < QT * KP; idx += NT) {
         const int q = idx / KP, k = idx - q * KP;
-        cplx<T> v{T(0), T(0)};
+        V v{T(0), T(0)};
         if (q0 + q < nmodes && k < N) v = in[(size_t)(q0 + q) * N + k];
-        C[k * S + 2 * q] = v.x;
-        C[k * S + 2 * q + 1] = v.y;
+        const int col = 32 * (q >> 4) + (q & 15);
+        C[k * S + col] = v.x;
+        C[k * S + col + 16] = v.y;
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, r16 = lane & 15, q4 = lane >> 4, wave = threadIdx.x >> 6;
     const size_t plane = (size_t)N * nmodes;
     const bool xeven = (geo.nx % 2) == 0, yeven = (geo.ny % 2) == 0;
-    // per column tile: this lane's mode and its derivative factors
-    T kxv[CT], kyv[CT], wv[CT];
-    bool dxz[CT], dyz[CT], live[CT];
-    int qq[CT];
+    // per mode group: this lane's mode and its derivative factors
+    T kxv[NG], kyv[NG], wv[NG];
+    bool dxz[NG], dyz[NG], live[NG];
+    int qq[NG];
 #pragma unroll
-    for (int ct = 0; ct < CT; ++ct) {
-        const int q = q0 + (16 * ct + r16) / 2;
-        qq[ct] = q;
-        live[ct] = q < nmodes;
+    for (int g = 0; g < NG; ++g) {
+        const int q = q0 + 16 * g + r16;
+        qq[g] = q;
+        live[g] = q < nmodes;
         const int m = q % geo.nxh, n = q / geo.nxh;
         const bool xny = xeven && m == geo.nx / 2, xline = m == 0 || xny;
-        kxv[ct] = T(geo.k0x * m);
-        kyv[ct] = T(ch3_ky(geo, n));
-        wv[ct] = xny ? T(0.5) : T(1);
-        dxz[ct] = xline;
-        dyz[ct] = xline && yeven && n == geo.ny / 2;
+        kxv[g] = T(geo.k0x * m);
+        kyv[g] = T(ch3_ky(geo, n));
+        wv[g] = xny ? T(0.5) : T(1);
+        dxz[g] = xline;
+        dyz[g] = xline && yeven && n == geo.ny / 2;
     }
-    const bool im = (r16 & 1) != 0;  // this lane holds the imaginary part
     for (int zt0 = 2 * wave; zt0 < nzt; zt0 += 8) {
-        acc4 acc[2][2][CT];
+        acc4 acc[2][2][CT];  // [z tile][value, dz][column tile]
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -114,45 +107,61 @@ __global__ __launch_bounds__(NT) void k3_zbwd(const cplx<T>* __restrict__ b, con
 #pragma unroll
                 for (int ct = 0; ct < CT; ++ct) acc[a][p][ct] = acc4{T(0), T(0), T(0), T(0)};
         const int z0 = 16 * zt0 + r16, z1 = z0 + 16;
-        for (int ks = 0; ks < ksteps; ++ks) {
-            const int k = 4 * ks + q4;
-            const bool kin = k < N;
-            const T a00 = (kin && z0 < N) ? BT[k * N + z0] : T(0), a01 = (kin && z0 < N) ? BDT[k * N + z0] : T(0);
-            const T a10 = (kin && z1 < N) ? BT[k * N + z1] : T(0), a11 = (kin && z1 < N) ? BDT[k * N + z1] : T(0);
+        constexpr int U = ZbwdTile<T>::U;  // k steps per batch: its loads are issued before its products
+        for (int ks = 0; ks < ksteps; ks += U) {
+            T am[U][4], bv[U][CT];
 #pragma unroll
-            for (int ct = 0; ct < CT; ++ct) {
-                const T bv = C[k * S + 16 * ct + r16];  // rows N .. KP - 1 are zero
-                acc[0][0][ct] = Op::mma(a00, bv, acc[0][0][ct]);
-                acc[0][1][ct] = Op::mma(a01, bv, acc[0][1][ct]);
-                acc[1][0][ct] = Op::mma(a10, bv, acc[1][0][ct]);
-                acc[1][1][ct] = Op::mma(a11, bv, acc[1][1][ct]);
+            for (int u = 0; u < U; ++u) {
+                const int k = 4 * (ks + u) + q4;
+                const bool in0 = k < N && z0 < N, in1 = k < N && z1 < N;
+                const int o0 = in0 ? k * N + z0 : 0, o1 = in1 ? k * N + z1 : 0;
+                am[u][0] = BT[o0];
+                am[u][1] = BDT[o0];
+                am[u][2] = BT[o1];
+                am[u][3] = BDT[o1];
+                am[u][0] = in0 ? am[u][0] : T(0);
+                am[u][1] = in0 ? am[u][1] : T(0);
+                am[u][2] = in1 ? am[u][2] : T(0);
+                am[u][3] = in1 ? am[u][3] : T(0);
+                const int kr = k < KP ? k : KP - 1;  // rows N .. KP - 1 are zero; beyond, a is zero
+#pragma unroll
+                for (int ct = 0; ct < CT; ++ct) bv[u][ct] = C[kr * S + 16 * ct + r16];
             }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int ct = 0; ct < CT; ++ct) {
+                    acc[0][0][ct] = Op::mma(am[u][0], bv[u][ct], acc[0][0][ct]);
+                    acc[0][1][ct] = Op::mma(am[u][1], bv[u][ct], acc[0][1][ct]);
+                    acc[1][0][ct] = Op::mma(am[u][2], bv[u][ct], acc[1][0][ct]);
+                    acc[1][1][ct] = Op::mma(am[u][3], bv[u][ct], acc[1][1][ct]);
+                }
         }
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int ct = 0; ct < CT; ++ct)
+            for (int g = 0; g < NG; ++g)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const int z = 16 * (zt0 + a) + Op::row(v, q4);
-                    const T val = acc[a][0][ct][v] * wv[ct], dzv = acc[a][1][ct][v] * wv[ct];
-                    const T other = lane_xor1(val);  // the partner part of the same mode
-                    if (z >= N || !live[ct]) continue;
-                    // i k v: real part -k Im v, imaginary part k Re v
-                    const T dx = dxz[ct] ? T(0) : (im ? kxv[ct] * other : -kxv[ct] * other);
-                    const T dy = dyz[ct] ? T(0) : (im ? kyv[ct] * other : -kyv[ct] * other);
-                    const size_t o = 2 * ((size_t)z * nmodes + qq[ct]) + (im ? 1 : 0);
-                    if (f > 0) G[2 * (size_t)(f - 1) * plane + o] = val;  // ux, uy, uz
-                    G[2 * (size_t)(3 + 3 * f) * plane + o] = dx;
-                    G[2 * (size_t)(4 + 3 * f) * plane + o] = dy;
-                    G[2 * (size_t)(5 + 3 * f) * plane + o] = dzv;
+                    if (z >= N || !live[g]) continue;
+                    const V val{acc[a][0][2 * g][v] * wv[g], acc[a][0][2 * g + 1][v] * wv[g]};
+                    const V dzv{acc[a][1][2 * g][v] * wv[g], acc[a][1][2 * g + 1][v] * wv[g]};
+                    const V dx = dxz[g] ? V{T(0), T(0)} : V{-kxv[g] * val.y, kxv[g] * val.x};  // i kx v
+                    const V dy = dyz[g] ? V{T(0), T(0)} : V{-kyv[g] * val.y, kyv[g] * val.x};
+                    const size_t o = (size_t)z * nmodes + qq[g];
+                    if (f > 0) G[(size_t)(f - 1) * plane + o] = val;  // ux, uy, uz
+                    G[(size_t)(3 + 3 * f) * plane + o] = dx;
+                    G[(size_t)(4 + 3 * f) * plane + o] = dy;
+                    G[(size_t)(5 + 3 * f) * plane + o] = dzv;
                 }
     }
 }
 
 // ---------------------------------------------------------------------------
 // k3_ybwd: padded inverse y transform of TK kx-lines of one z level and plane.
-// grid (ceil(nxh/TK), N, planes).
+// grid (ceil(nxh/TK), N, planes), remapped so that the tiles of one row of kx (which
+// share cache lines: rows of nxh values are not line-aligned) run on one XCD.
 // ---------------------------------------------------------------------------
 template <typename T, int NY, int MY, int TK, int NT>
 __global__ __launch_bounds__(NT) void k3_ybwd(const cplx<T>* __restrict__ G, Ch3Geom geo, cplx<T>* __restrict__ Q,
@@ -164,8 +173,10 @@ __global__ __launch_bounds__(NT) void k3_ybwd(const cplx<T>* __restrict__ G, Ch3
     __shared__ V twl_lds[TW_LDS ? MY : 1];
     const V* twl = TW_LDS ? twl_lds : tw;
     if constexpr (TW_LDS) load_twiddles<T, MY, NT>(twl_lds, tw);
-    const int nxh = geo.nxh, z = blockIdx.y, pl = blockIdx.z;
-    const int m0 = blockIdx.x * TK;
+    const unsigned gxy = gridDim.x * gridDim.y;
+    const unsigned lin = xcd_remap(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), gxy * gridDim.z);
+    const int nxh = geo.nxh, z = (int)((lin / gridDim.x) % gridDim.y), pl = (int)(lin / gxy);
+    const int m0 = (int)(lin % gridDim.x) * TK;
     const size_t plane = (size_t)geo.N * geo.nmodes;
     const V* in = G + (size_t)pl * plane + (size_t)z * geo.nmodes;
     constexpr int GAP0 = NY / 2 + 1, GAP1 = MY - NY / 2;  // zero band [GAP0, GAP1)
@@ -274,7 +285,7 @@ __global__ __launch_bounds__(NT) void k3_xpass(const cplx<T>* __restrict__ Q, in
 // ---------------------------------------------------------------------------
 // k3_yfwd: forward y transform of TK padded kx-lines, truncation with the y
 // Nyquist fold, the x Nyquist fold on the kx = Nx/2 line.  Output z-grid planes.
-// grid (ceil(nxh/TK), N, 4).
+// grid (ceil(nxh/TK), N, 4), XCD-remapped like k3_ybwd.
 // ---------------------------------------------------------------------------
 template <typename T, int NY, int MY, int TK, int NT>
 __global__ __launch_bounds__(NT) void k3_yfwd(const cplx<T>* __restrict__ H, Ch3Geom geo, cplx<T>* __restrict__ F,
@@ -286,8 +297,10 @@ __global__ __launch_bounds__(NT) void k3_yfwd(const cplx<T>* __restrict__ H, Ch3
     __shared__ V twl_lds[TW_LDS ? MY : 1];
     const V* twl = TW_LDS ? twl_lds : tw;
     if constexpr (TW_LDS) load_twiddles<T, MY, NT>(twl_lds, tw);
-    const int nxh = geo.nxh, z = blockIdx.y, o = blockIdx.z;
-    const int m0 = blockIdx.x * TK;
+    const unsigned gxy = gridDim.x * gridDim.y;
+    const unsigned lin = xcd_remap(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), gxy * gridDim.z);
+    const int nxh = geo.nxh, z = (int)((lin / gridDim.x) % gridDim.y), o = (int)(lin / gxy);
+    const int m0 = (int)(lin % gridDim.x) * TK;
     const V* in = H + (size_t)o * geo.N * MY * nxh + (size_t)z * MY * nxh;
     for (int idx = threadIdx.x; idx < TK * MY; idx += NT) {
         const int c = idx % TK, j = idx / TK;

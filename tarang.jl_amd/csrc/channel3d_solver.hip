@@ -297,16 +297,13 @@ void Channel3Solver::rhs_t(void* F0, void* F1, void* F2, void* F3) {
     V* G = G_.as<V>();
     V* Q = Q_.as<V>();
     V* H = H_.as<V>();
-    constexpr int TKZ = 4;  // modes per workgroup of the z-forward transform
-    const size_t zlds = (size_t)TKZ * N * sizeof(V);
-    const unsigned ztiles = (unsigned)((nmodes_ + TKZ - 1) / TKZ);
     {
         constexpr int QT = ZbwdTile<T>::QT;
         const size_t blds = (size_t)((N + 15) & ~15) * zbwd_lds_stride(QT) * sizeof(T);
         timed(0, [&] {
             hipLaunchKernelGGL((k3_zbwd<T, 256>), dim3((unsigned)((nmodes_ + QT - 1) / QT), 4), dim3(256), blds, s,
                                B_.as<const V>(), UX_.as<const V>(), UY_.as<const V>(), UZ_.as<const V>(), geo,
-                               BT_.as<const T>(), BDT_.as<const T>(), reinterpret_cast<T*>(G));
+                               BT_.as<const T>(), BDT_.as<const T>(), G);
         });
     }
     TG_HIP(hipGetLastError());
@@ -348,15 +345,15 @@ void Channel3Solver::rhs_t(void* F0, void* F1, void* F2, void* F3) {
         });
     }, Sizes2D{});
     TG_HIP(hipGetLastError());
-    auto zf = [&](auto NTC) {
-        constexpr int NT = decltype(NTC)::value;
+    {
+        constexpr int QT = std::is_same<T, float>::value ? 32 : 16;  // 64 or 32 columns
+        const size_t flds = ch_zfwd_lds_elems(N, QT) * sizeof(T);
         timed(4, [&] {
-            hipLaunchKernelGGL((k_ch_zfwd<T, TKZ, NT>), dim3(ztiles, 4), dim3(NT), zlds, s, Fz, nmodes_, 1, N,
-                               FT_.as<const T>(), static_cast<V*>(F0), static_cast<V*>(F1), static_cast<V*>(F2),
-                               static_cast<V*>(F3));
+            hipLaunchKernelGGL((k_ch_zfwd<T, QT, 256>), dim3((unsigned)((nmodes_ + QT - 1) / QT), 4), dim3(256), flds, s,
+                               Fz, nmodes_, 1, N, FT_.as<const T>(), static_cast<V*>(F0), static_cast<V*>(F1),
+                               static_cast<V*>(F2), static_cast<V*>(F3));
         });
-    };
-    if (N <= 64) zf(std::integral_constant<int, 64>{}); else zf(std::integral_constant<int, 256>{});
+    }
     TG_HIP(hipGetLastError());
 }
 

@@ -64,6 +64,25 @@ inline void chebyshev_tables(int N, double Lz, std::vector<double>& BT, std::vec
         }
 }
 
+// Matrix-core product D(16x16) += A(16x4) B(4x16) in the arithmetic type T, and the
+// row of D held in accumulator component v by a lane in quarter q4 = lane / 16 (the
+// column is lane % 16 for both).
+template <typename T> struct MfmaOp;
+template <> struct MfmaOp<double> {
+    typedef double acc4 __attribute__((ext_vector_type(4)));
+    static __device__ inline acc4 mma(double a, double b, acc4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ inline int row(int v, int q4) { return 4 * v + q4; }
+};
+template <> struct MfmaOp<float> {
+    typedef float acc4 __attribute__((ext_vector_type(4)));
+    static __device__ inline acc4 mma(float a, float b, acc4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ inline int row(int v, int q4) { return 4 * q4 + v; }
+};
+
 // ---------------------------------------------------------------------------
 // k_ch_zbwd: z-backward transform of the product operands.  Work item (tile of
 // TK modes, job): job 0..5 = {B b, B ux, B uz, BD b, BD ux, BD uz} where B
@@ -217,50 +236,96 @@ __global__ __launch_bounds__(NT) void k_ch_xpass(const cplx<T>* __restrict__ G, 
 }
 
 // ---------------------------------------------------------------------------
-// k_ch_zfwd: z-forward Chebyshev transform of the products (grid (tiles, 3 or 4))
-// into mode-major F arrays; the kx = Nx/2 column takes the x Nyquist fold 2 Re(H)
-// (nx odd switches the fold off: the 3-D path folds in its y pass).
-// matT = transposed forward matrix (matT[z*N + k]).
+// k_ch_zfwd: z-forward Chebyshev transform of the product planes on the matrix cores
+// (grid (ceil(nmodes/QT), planes, k splits), XCD-remapped) into mode-major F arrays:
+//   F[m][k] = sum_z Fm[k][z] H[z][m],   FT[z*N + k] = Fm[k][z]
+// The plane tile H[z][q0 .. q0+QT) sits in LDS as columns in groups of 16 modes (Re
+// block, Im block); 16 x 16 tiles of (k, column).  Split s of the workgroup's k tiles
+// [s*kps, (s+1)*kps): wave w takes tiles 2w, 2w+1 of it (+8, ...).
+// A lane holds Re and Im of one mode for four k and stores them directly.  The
+// kx = Nx/2 column takes the x Nyquist fold 2 Re(H) (nx odd switches the fold off:
+// the 3-D path folds in its y pass).
 // ---------------------------------------------------------------------------
-template <typename T, int TK, int NT>
-__global__ __launch_bounds__(NT) void k_ch_zfwd(const cplx<T>* __restrict__ H, int nxh, int nx, int N,
-                                                const T* __restrict__ FT, cplx<T>* __restrict__ F0,
-                                                cplx<T>* __restrict__ F1, cplx<T>* __restrict__ F2,
-                                                cplx<T>* __restrict__ F3 = nullptr) {
+// z steps per load batch, per arithmetic type (modes per workgroup QT: 16 or 32)
+template <typename T> struct ZfwdTile { static constexpr int U = 2; };
+template <> struct ZfwdTile<double> { static constexpr int U = 4; };
+
+__host__ __device__ inline size_t ch_zfwd_lds_elems(int N, int qt) { return (size_t)((N + 15) & ~15) * (2 * qt + 1); }
+
+template <typename T, int QT, int NT>
+__global__ __launch_bounds__(NT, 3) void k_ch_zfwd(const cplx<T>* __restrict__ H, int nmodes, int nx, int N,
+                                                   const T* __restrict__ FT, cplx<T>* __restrict__ F0,
+                                                   cplx<T>* __restrict__ F1, cplx<T>* __restrict__ F2,
+                                                   cplx<T>* __restrict__ F3 = nullptr) {
+    static_assert(NT == 256 && QT % 16 == 0, "four wavefronts per workgroup, 16-mode groups");
     using V = cplx<T>;
+    using Op = MfmaOp<T>;
+    using acc4 = typename Op::acc4;
+    constexpr int NG = QT / 16, CT = 2 * NG, S = 2 * QT + 1;
     extern __shared__ unsigned char smem_raw[];
-    V* src = reinterpret_cast<V*>(smem_raw);  // [TK][N]
-    const int f = blockIdx.y;
-    const int m0 = blockIdx.x * TK;
-    const size_t plane = (size_t)N * nxh;
-    const V* in = H + (size_t)f * plane;
+    T* C = reinterpret_cast<T*>(smem_raw);  // [KP][S]
+    const int KP = (N + 15) & ~15, nkt = KP >> 4, zsteps = (N + 3) >> 2;
+    const unsigned gxy = gridDim.x * gridDim.y;
+    const unsigned lin = xcd_remap(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), gxy * gridDim.z);
+    const int f = (int)((lin / gridDim.x) % gridDim.y), q0 = (int)(lin % gridDim.x) * QT;
+    const int kps = (nkt + (int)gridDim.z - 1) / (int)gridDim.z, kb = (int)(lin / gxy) * kps;
+    const int ke = kb + kps < nkt ? kb + kps : nkt;
+    const V* in = H + (size_t)f * N * nmodes;
     const bool xeven = (nx % 2) == 0;
-    for (int idx = threadIdx.x; idx < TK * N; idx += NT) {
-        const int c = idx % TK, z = idx / TK;
-        const int m = m0 + c;
-        V v = m < nxh ? in[(size_t)z * nxh + m] : V{T(0), T(0)};
+    for (int idx = threadIdx.x; idx < QT * KP; idx += NT) {
+        const int z = idx / QT, q = idx - z * QT;
+        const int m = q0 + q;
+        V v{T(0), T(0)};
+        if (z < N && m < nmodes) v = in[(size_t)z * nmodes + m];
         if (xeven && m == nx / 2) v = V{T(2) * v.x, T(0)};
-        src[c * N + z] = v;
+        const int col = 32 * (q >> 4) + (q & 15);
+        C[z * S + col] = v.x;
+        C[z * S + col + 16] = v.y;
     }
     __syncthreads();
+    const int lane = threadIdx.x & 63, r16 = lane & 15, q4 = lane >> 4, wave = threadIdx.x >> 6;
     V* out = f == 0 ? F0 : (f == 1 ? F1 : (f == 2 ? F2 : F3));
-    for (int k = threadIdx.x; k < N; k += NT) {
-        V acc[TK];
+    for (int kt0 = kb + 2 * wave; kt0 < ke; kt0 += 8) {
+        acc4 acc[2][CT];  // [k tile][column tile]
 #pragma unroll
-        for (int c = 0; c < TK; ++c) acc[c] = V{T(0), T(0)};
-        for (int z = 0; z < N; ++z) {
-            const T a = FT[(size_t)z * N + k];
+        for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int c = 0; c < TK; ++c) {
-                const V s = src[c * N + z];
-                acc[c].x += a * s.x;
-                acc[c].y += a * s.y;
+            for (int ct = 0; ct < CT; ++ct) acc[a][ct] = acc4{T(0), T(0), T(0), T(0)};
+        const int k0 = 16 * kt0 + r16, k1 = k0 + 16;
+        constexpr int U = ZfwdTile<T>::U;
+        for (int zs = 0; zs < zsteps; zs += U) {
+            T am[U][2], bv[U][CT];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int z = 4 * (zs + u) + q4;
+                const bool in0 = z < N && k0 < N, in1 = z < N && k1 < N;
+                am[u][0] = FT[in0 ? z * N + k0 : 0];
+                am[u][1] = FT[in1 ? z * N + k1 : 0];
+                am[u][0] = in0 ? am[u][0] : T(0);
+                am[u][1] = in1 ? am[u][1] : T(0);
+                const int zr = z < KP ? z : KP - 1;  // rows N .. KP - 1 are zero; beyond, a is zero
+#pragma unroll
+                for (int ct = 0; ct < CT; ++ct) bv[u][ct] = C[zr * S + 16 * ct + r16];
             }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int ct = 0; ct < CT; ++ct) {
+                    acc[0][ct] = Op::mma(am[u][0], bv[u][ct], acc[0][ct]);
+                    acc[1][ct] = Op::mma(am[u][1], bv[u][ct], acc[1][ct]);
+                }
         }
 #pragma unroll
-        for (int c = 0; c < TK; ++c) {
-            const int m = m0 + c;
-            if (m < nxh) out[(size_t)m * N + k] = acc[c];
+        for (int g = 0; g < NG; ++g) {
+            const int m = q0 + 16 * g + r16;
+            if (m >= nmodes) continue;
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int k = 16 * (kt0 + a) + Op::row(v, q4);
+                    if (k < N && kt0 + a < ke) out[(size_t)m * N + k] = V{acc[a][2 * g][v], acc[a][2 * g + 1][v]};
+                }
         }
     }
 }
@@ -844,25 +909,6 @@ __device__ inline void mfma_apply(const AT* __restrict__ A, int n, const double*
         default: break;
     }
 }
-
-// Matrix-core product D(16x16) += A(16x4) B(4x16) in the arithmetic type T, and the
-// row of D held in accumulator component v by a lane in quarter q4 = lane / 16 (the
-// column is lane % 16 for both).
-template <typename T> struct MfmaOp;
-template <> struct MfmaOp<double> {
-    typedef double acc4 __attribute__((ext_vector_type(4)));
-    static __device__ inline acc4 mma(double a, double b, acc4 c) {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-    static __device__ inline int row(int v, int q4) { return 4 * v + q4; }
-};
-template <> struct MfmaOp<float> {
-    typedef float acc4 __attribute__((ext_vector_type(4)));
-    static __device__ inline acc4 mma(float a, float b, acc4 c) {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-    static __device__ inline int row(int v, int q4) { return 4 * q4 + v; }
-};
 
 // Store the tiles of mfma_apply into X[i*16 + c] (LDS, rows up to 16 ceil(n/16)).
 template <int MT>

@@ -240,14 +240,17 @@ void ChannelSolver::rhs_t(const void* b, const void* ux, const void* uz, void* F
         }
     }, Sizes2D{});
     TG_HIP(hipGetLastError());
-    auto zf = [&](auto NTC) {
-        constexpr int NT = decltype(NTC)::value;
+    {
+        // few modes: 16-mode tiles, and the k range split over workgroups (8 k tiles each)
+        constexpr int QT = 16;
+        const unsigned ksplit = (unsigned)std::max(1, ((N + 15) / 16) / 8);
+        const size_t flds = ch_zfwd_lds_elems(N, QT) * sizeof(T);
         timed(2, [&] {
-            hipLaunchKernelGGL((k_ch_zfwd<T, kTK, NT>), dim3(ntile, 3), dim3(NT), lds, s, H, nxh_, d_.nx, N,
-                               FT_.as<const T>(), static_cast<V*>(F0), static_cast<V*>(F1), static_cast<V*>(F2));
+            hipLaunchKernelGGL((k_ch_zfwd<T, QT, 256>), dim3((unsigned)((nxh_ + QT - 1) / QT), 3, ksplit), dim3(256),
+                               flds, s, H, nxh_, d_.nx, N, FT_.as<const T>(), static_cast<V*>(F0),
+                               static_cast<V*>(F1), static_cast<V*>(F2));
         });
-    };
-    if (N <= 64) zf(std::integral_constant<int, 64>{}); else zf(std::integral_constant<int, 256>{});
+    }
     TG_HIP(hipGetLastError());
 }
 

@@ -181,5 +181,8 @@ def test_rbc3d_through_api(oracle):
     assert relerr(b["c"].cpu().numpy(), ref.get(oracle.RBC3_B)) < 1e-12
     for i, v in enumerate((oracle.RBC3_UX, oracle.RBC3_UY, oracle.RBC3_UZ)):
         assert relerr(u[i]["c"].cpu().numpy(), ref.get(v)) < 1e-11
-    assert relerr(tau_b1._c.cpu().numpy().T, ref.tau(1)) < 1e-9  # lift taus: less well-conditioned
+    # lift taus are the ill-conditioned components of the per-mode tau systems (condition
+    # numbers up to ~3e7): the GPU applies explicit inverses, the oracle LU solves, and the
+    # two roundings differ there by ~cond x eps relative to the taus' own scale
+    assert relerr(tau_b1._c.cpu().numpy().T, ref.tau(1)) < 1e-8
     solver.close()
