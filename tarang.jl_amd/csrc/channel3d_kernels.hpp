@@ -397,7 +397,7 @@ constexpr int kCh3ChunkModes = 8;  // modes per workgroup: 16 real right-hand si
 __host__ __device__ inline int round16(int n) { return (n + 15) & ~15; }
 
 __host__ __device__ inline size_t ch3_solve_lds_doubles(int N) {
-    return (size_t)16 * (2 * round16(chb_size(N, 0)) + round16(chu_size(N, 0, 0)));
+    return (size_t)kYS * (2 * round16(chb_size(N, 0)) + round16(chu_size(N, 0, 0)));
 }
 
 // Workgroup = (|k_h| key, parity g, up to 8 modes of that key).  Right-hand sides of
@@ -424,10 +424,10 @@ __global__ __launch_bounds__(NT) void k3_solve(Ch3SolveArgs s, const cplx<T>* __
     const int N = s.N, h = g ^ 1, PL = (N - 1) & 1;
     const int ng = npar(N, g), nh = npar(N, h);
     const int nb = nh + 1, np = ng + 1, nu = chu_size(N, g, key);  // key 0 is kx = ky = 0 only
-    double* Yb = reinterpret_cast<double*>(smem_raw);  // [round16(nb)][16]
-    double* Yp = Yb + 16 * round16(nb);                 // [round16(np)][16]
-    double* Yu = Yp + 16 * round16(np);                 // [round16(nu)][16]
-    const int ytot = 16 * (round16(nb) + round16(np) + round16(nu));
+    double* Yb = reinterpret_cast<double*>(smem_raw);  // [round16(nb)][kYS]
+    double* Yp = Yb + kYS * round16(nb);                // [round16(np)][kYS]
+    double* Yu = Yp + kYS * round16(np);                // [round16(nu)][kYS]
+    const int ytot = kYS * (round16(nb) + round16(np) + round16(nu));
     if (threadIdx.x < kCh3ChunkModes) {
         const int j = threadIdx.x;
         const int q = j < cnt ? s.modes[first + j] : -1;
@@ -442,10 +442,10 @@ __global__ __launch_bounds__(NT) void k3_solve(Ch3SolveArgs s, const cplx<T>* __
         const int j = idx / nh, i = idx - j * nh;
         const size_t base = (size_t)smode[j] * N;
         const cplx<T> r = R2[base + 2 * i + h], rz = R3z[base + 2 * i + h];
-        Yb[i * 16 + 2 * j] = (double)r.x;
-        Yb[i * 16 + 2 * j + 1] = (double)r.y;
-        Yu[(2 * ng + i) * 16 + 2 * j] = (double)rz.x;
-        Yu[(2 * ng + i) * 16 + 2 * j + 1] = (double)rz.y;
+        Yb[i * kYS + 2 * j] = (double)r.x;
+        Yb[i * kYS + 2 * j + 1] = (double)r.y;
+        Yu[(2 * ng + i) * kYS + 2 * j] = (double)rz.x;
+        Yu[(2 * ng + i) * kYS + 2 * j + 1] = (double)rz.y;
     }
     for (int idx = threadIdx.x; idx < cnt * ng; idx += NT) {
         const int j = idx / ng, i = idx - j * ng;
@@ -454,16 +454,16 @@ __global__ __launch_bounds__(NT) void k3_solve(Ch3SolveArgs s, const cplx<T>* __
         const cplx<T> rx = R3x[base + 2 * i + g], ry = R3y[base + 2 * i + g], r1 = R1[base + 2 * i + g];
         const double ex_re = (double)rx.y, ex_im = -(double)rx.x;  // rows -i E3x, -i E3y
         const double ey_re = (double)ry.y, ey_im = -(double)ry.x;
-        Yp[i * 16 + 2 * j] = -sr * ex_re + cr * ey_re;  // perp: -sin E3x + cos E3y
-        Yp[i * 16 + 2 * j + 1] = -sr * ex_im + cr * ey_im;
-        Yu[i * 16 + 2 * j] = cr * ex_re + sr * ey_re;  // par: cos E3x + sin E3y
-        Yu[i * 16 + 2 * j + 1] = cr * ex_im + sr * ey_im;
-        Yu[(ng + i) * 16 + 2 * j] = (double)r1.x;
-        Yu[(ng + i) * 16 + 2 * j + 1] = (double)r1.y;
+        Yp[i * kYS + 2 * j] = -sr * ex_re + cr * ey_re;  // perp: -sin E3x + cos E3y
+        Yp[i * kYS + 2 * j + 1] = -sr * ex_im + cr * ey_im;
+        Yu[i * kYS + 2 * j] = cr * ex_re + sr * ey_re;  // par: cos E3x + sin E3y
+        Yu[i * kYS + 2 * j + 1] = cr * ex_im + sr * ey_im;
+        Yu[(ng + i) * kYS + 2 * j] = (double)r1.x;
+        Yu[(ng + i) * kYS + 2 * j + 1] = (double)r1.y;
     }
     if (threadIdx.x == 0 && key == 0) {  // wall values of b at kx = ky = 0 (column 0 = mode 0, real part)
         const double gv = h == 0 ? 0.5 * (s.bc_b1 + s.bc_b0) : 0.5 * (s.bc_b1 - s.bc_b0);
-        Yb[nh * 16] = s.c * gv;
+        Yb[nh * kYS] = s.c * gv;
     }
     __syncthreads();
     // ---- b (parity h) and perp (parity g) systems ----
@@ -479,15 +479,15 @@ __global__ __launch_bounds__(NT) void k3_solve(Ch3SolveArgs s, const cplx<T>* __
     __syncthreads();
     for (int idx = threadIdx.x; idx < cnt * nh; idx += NT) {
         const int j = idx / nh, i = idx - j * nh;
-        const double bre = Yb[i * 16 + 2 * j], bim = Yb[i * 16 + 2 * j + 1];
-        Yu[(2 * ng + i) * 16 + 2 * j] += s.c * bre;
-        Yu[(2 * ng + i) * 16 + 2 * j + 1] += s.c * bim;
+        const double bre = Yb[i * kYS + 2 * j], bim = Yb[i * kYS + 2 * j + 1];
+        Yu[(2 * ng + i) * kYS + 2 * j] += s.c * bre;
+        Yu[(2 * ng + i) * kYS + 2 * j + 1] += s.c * bim;
         B[(size_t)smode[j] * N + 2 * i + h] = cplx<T>{T(bre), T(bim)};
     }
     if (threadIdx.x < cnt) {
         const int j = threadIdx.x;
         taus[(size_t)smode[j] * kCh3TauStride + (h == (PL ^ 1) ? 1 : 2)] =
-            cplx<T>{T(Yb[nh * 16 + 2 * j]), T(Yb[nh * 16 + 2 * j + 1])};
+            cplx<T>{T(Yb[nh * kYS + 2 * j]), T(Yb[nh * kYS + 2 * j + 1])};
     }
     __syncthreads();
     // ---- u-p system of parity g ----
@@ -503,28 +503,28 @@ __global__ __launch_bounds__(NT) void k3_solve(Ch3SolveArgs s, const cplx<T>* __
         const int j = idx / ng, i = idx - j * ng;
         const size_t o = (size_t)smode[j] * N + 2 * i + g;
         const double cr = srot[2 * j], sr = srot[2 * j + 1];
-        const double par_re = Yu[i * 16 + 2 * j], par_im = Yu[i * 16 + 2 * j + 1];
-        const double pe_re = Yp[i * 16 + 2 * j], pe_im = Yp[i * 16 + 2 * j + 1];
+        const double par_re = Yu[i * kYS + 2 * j], par_im = Yu[i * kYS + 2 * j + 1];
+        const double pe_re = Yp[i * kYS + 2 * j], pe_im = Yp[i * kYS + 2 * j + 1];
         const double tx_re = cr * par_re - sr * pe_re, tx_im = cr * par_im - sr * pe_im;
         const double ty_re = sr * par_re + cr * pe_re, ty_im = sr * par_im + cr * pe_im;
         UX[o] = cplx<T>{T(-tx_im), T(tx_re)};
         UY[o] = cplx<T>{T(-ty_im), T(ty_re)};
-        P[o] = cplx<T>{T(Yu[(ng + i) * 16 + 2 * j]), T(Yu[(ng + i) * 16 + 2 * j + 1])};
+        P[o] = cplx<T>{T(Yu[(ng + i) * kYS + 2 * j]), T(Yu[(ng + i) * kYS + 2 * j + 1])};
     }
     for (int idx = threadIdx.x; idx < cnt * nh; idx += NT) {
         const int j = idx / nh, i = idx - j * nh;
         UZ[(size_t)smode[j] * N + 2 * i + h] =
-            cplx<T>{T(Yu[(2 * ng + i) * 16 + 2 * j]), T(Yu[(2 * ng + i) * 16 + 2 * j + 1])};
+            cplx<T>{T(Yu[(2 * ng + i) * kYS + 2 * j]), T(Yu[(2 * ng + i) * kYS + 2 * j + 1])};
     }
     if (threadIdx.x < cnt) {
         const int j = threadIdx.x, mode = smode[j];
         const double cr = srot[2 * j], sr = srot[2 * j + 1];
-        const double pt_re = Yp[ng * 16 + 2 * j], pt_im = Yp[ng * 16 + 2 * j + 1];  // perp-system tau
+        const double pt_re = Yp[ng * kYS + 2 * j], pt_im = Yp[ng * kYS + 2 * j + 1];  // perp-system tau
         cplx<T>* tm = taus + (size_t)mode * kCh3TauStride;
         const int t0 = 2 * ng + nh;
         for (int t = 0; t < nu - t0; ++t) {
             const int slot = chu_tau_slot(N, g, t);  // 2-D slots: 0 p, 3 u1x, 4 u1z, 5 u2x, 6 u2z
-            const double re = Yu[(t0 + t) * 16 + 2 * j], im = Yu[(t0 + t) * 16 + 2 * j + 1];
+            const double re = Yu[(t0 + t) * kYS + 2 * j], im = Yu[(t0 + t) * kYS + 2 * j + 1];
             if (slot == 0) {
                 tm[0] = cplx<T>{T(re), T(im)};
             } else if (slot == 4 || slot == 6) {

@@ -258,16 +258,11 @@ void Channel3Solver::factor(double c) {
     hipLaunchKernelGGL((k_lu_factor<NT>), dim3(nsys), dim3(NT), (size_t)nu_max_ * 24, s, Lu.as<double>(),
                        pu.as<int>(), info + 2 * nsys, N, 1, su, (size_t)nu_max_);
     TG_HIP(hipGetLastError());
-    const size_t lb = lu_inverse_lds_doubles(nb_max_) * sizeof(double), lu = lu_inverse_lds_doubles(nu_max_) * sizeof(double);
-    const unsigned cb = (unsigned)((nb_max_ + 15) / 16), cu = (unsigned)((nu_max_ + 15) / 16);
     auto invert = [&](auto ot) {
         using OT = decltype(ot);
-        hipLaunchKernelGGL((k_lu_inverse<NT, OT>), dim3(nsys, cb), dim3(NT), lb, s, Lb.as<const double>(),
-                           pb.as<const int>(), Ib_.as<OT>(), N, 0, sb, (size_t)nb_max_, sb, nb_max_);
-        hipLaunchKernelGGL((k_lu_inverse<NT, OT>), dim3(nsys, cb), dim3(NT), lb, s, Lp.as<const double>(),
-                           pp.as<const int>(), Ip_.as<OT>(), N, 0, sb, (size_t)nb_max_, sb, nb_max_);
-        hipLaunchKernelGGL((k_lu_inverse<NT, OT>), dim3(nsys, cu), dim3(NT), lu, s, Lu.as<const double>(),
-                           pu.as<const int>(), Iu_.as<OT>(), N, 1, su, (size_t)nu_max_, su, nu_max_);
+        lu_invert<NT, OT>(s, Lb.as<const double>(), pb.as<const int>(), Ib_.as<OT>(), N, 0, nsys, sb, nb_max_, nb_max_);
+        lu_invert<NT, OT>(s, Lp.as<const double>(), pp.as<const int>(), Ip_.as<OT>(), N, 0, nsys, sb, nb_max_, nb_max_);
+        lu_invert<NT, OT>(s, Lu.as<const double>(), pu.as<const int>(), Iu_.as<OT>(), N, 1, nsys, su, nu_max_, nu_max_);
     };
     if (d_.dtype == kF32) invert(float{});
     else invert(double{});

@@ -816,33 +816,49 @@ __device__ void lu_solve_blocked(const double* __restrict__ a, const int* __rest
 }
 
 // Explicit inverse from the LU factors: workgroup (system w, column block cb) solves
-// A x = e_c for the 16 columns c = 16 cb .. 16 cb + 15 in FP64 and stores x, rounded
-// to OT, as column c of the inverse (column-major, leading dimension n, stride
-// istride per system).  LDS: 16 ld + max(16 ld, 4096) doubles, ld >= n.
-template <int NT, typename OT>
+// A x = e_c for the NR columns c = NR cb .. NR cb + NR - 1 in FP64 and stores x,
+// rounded to OT, as column c of the inverse (column-major, leading dimension n,
+// stride istride per system).  LDS: NR ld + max(NR ld, 4096) doubles, ld >= n.
+template <int NT, typename OT, int NR>
 __global__ __launch_bounds__(NT) void k_lu_inverse(const double* __restrict__ LU, const int* __restrict__ perm,
                                                    OT* __restrict__ inv, int N, int kind, size_t stride,
                                                    size_t pstride, size_t istride, int ld) {
     extern __shared__ unsigned char smem_raw[];
     const int w = blockIdx.x, m = w >> 1, g = w & 1;
     const int n = kind == 0 ? chb_size(N, g) : chu_size(N, g, m);
-    const int c0 = 16 * (int)blockIdx.y;
+    const int c0 = NR * (int)blockIdx.y;
     if (c0 >= n) return;  // whole workgroup: no barrier reached
-    double* y = reinterpret_cast<double*>(smem_raw);  // [16][ld]
-    double* tmp = y + 16 * (size_t)ld;                 // [max(16 ld, 4096)], also the triangle block
-    for (int idx = threadIdx.x; idx < 16 * ld; idx += NT) {
+    double* y = reinterpret_cast<double*>(smem_raw);  // [NR][ld]
+    double* tmp = y + NR * (size_t)ld;                 // [max(NR ld, 4096)], also the triangle block
+    for (int idx = threadIdx.x; idx < NR * ld; idx += NT) {
         const int r = idx / ld, i = idx - r * ld;
         y[idx] = i == c0 + r ? 1.0 : 0.0;
     }
     __syncthreads();
-    lu_solve_blocked<NT, 16>(LU + (size_t)w * stride, perm + (size_t)w * pstride, n, y, ld, tmp, tmp);
+    lu_solve_blocked<NT, NR>(LU + (size_t)w * stride, perm + (size_t)w * pstride, n, y, ld, tmp, tmp);
     OT* out = inv + (size_t)w * istride;
-    for (int r = 0; r < 16 && c0 + r < n; ++r)
+    for (int r = 0; r < NR && c0 + r < n; ++r)
         for (int i = threadIdx.x; i < n; i += NT) out[(size_t)(c0 + r) * n + i] = OT(y[r * ld + i]);
 }
 
-__host__ __device__ inline size_t lu_inverse_lds_doubles(int ld) {
-    return 16 * (size_t)ld + (16 * (size_t)ld > 4096 ? 16 * (size_t)ld : 4096);
+__host__ __device__ inline size_t lu_inverse_lds_doubles(int ld, int nr) {
+    return (size_t)nr * ld + ((size_t)nr * ld > 4096 ? (size_t)nr * ld : 4096);
+}
+
+// Form the inverses of nsys systems of one kind (0: b-like, 1: u-p) from their LU
+// factors: 16 right-hand sides per workgroup when they fit in LDS, else 8.
+template <int NT, typename OT>
+inline void lu_invert(hipStream_t s, const double* LU, const int* perm, OT* inv, int N, int kind, unsigned nsys,
+                      size_t stride, size_t pstride, int nmax) {
+    if (lu_inverse_lds_doubles(nmax, 16) * sizeof(double) <= 160 * 1024) {
+        hipLaunchKernelGGL((k_lu_inverse<NT, OT, 16>), dim3(nsys, (unsigned)((nmax + 15) / 16)), dim3(NT),
+                           lu_inverse_lds_doubles(nmax, 16) * sizeof(double), s, LU, perm, inv, N, kind, stride,
+                           pstride, stride, nmax);
+    } else {
+        hipLaunchKernelGGL((k_lu_inverse<NT, OT, 8>), dim3(nsys, (unsigned)((nmax + 7) / 8)), dim3(NT),
+                           lu_inverse_lds_doubles(nmax, 8) * sizeof(double), s, LU, perm, inv, N, kind, stride,
+                           pstride, stride, nmax);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -856,13 +872,16 @@ __device__ inline dbl4 mfma_f64_16x16x4(double a, double b, dbl4 c) {
 }
 
 // X = A Y for a column-major n x n matrix A (global, leading dimension n, FP64 or
-// FP32 entries, FP64 products) and 16 right-hand sides Y[k*16 + c] in LDS (rows
-// n .. 16 ceil(n/16) - 1 zero).  The 16-row
+// FP32 entries, FP64 products) and 16 right-hand sides Y[k*kYS + c] in LDS (rows
+// n .. 16 ceil(n/16) - 1 zero; the row stride kYS = 17 keeps row-wise LDS walks off
+// a single bank).  The 16-row
 // tiles t of X go to the wavefronts round robin from `wave0`: tile t belongs to wave
 // (t + wave0) % nw; acc[j] holds tile wave-local j.  Entries outside A count as zero
 // (their loads read element 0).  The k loop runs in batches of U steps whose loads
 // are all issued before the batch's products (NTW = this wave's tile count, so the
 // batch has no branches).
+constexpr int kYS = 17;
+
 template <int NTW, int U, typename AT>
 __device__ inline void mfma_apply_n(const AT* __restrict__ A, int n, const double* Y, dbl4* acc, int t0, int nw) {
     const int lane = threadIdx.x & 63, r16 = lane & 15, q4 = lane >> 4;
@@ -871,22 +890,27 @@ __device__ inline void mfma_apply_n(const AT* __restrict__ A, int n, const doubl
 #pragma unroll
     for (int j = 0; j < NTW; ++j) ii[j] = 16 * (t0 + j * nw) + r16;
     for (int ks = 0; ks < ksteps; ks += U) {
-        double a[U][NTW], b[U];
+        AT raw[U][NTW];  // every load of the batch is issued before any is used
+        double b[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int k = 4 * (ks + u) + q4;
-            b[u] = Y[min(k, krow_max) * 16 + r16];
+            b[u] = Y[min(k, krow_max) * kYS + r16];
 #pragma unroll
             for (int j = 0; j < NTW; ++j) {
                 const bool in = k < n && ii[j] < n;
-                a[u][j] = (double)A[in ? k * n + ii[j] : 0];  // n <= 512: 32-bit offsets from a uniform base
-                a[u][j] = in ? a[u][j] : 0.0;
+                raw[u][j] = A[in ? k * n + ii[j] : 0];  // n <= 512: 32-bit offsets from a uniform base
             }
         }
 #pragma unroll
-        for (int u = 0; u < U; ++u)
+        for (int u = 0; u < U; ++u) {
+            const int k = 4 * (ks + u) + q4;
 #pragma unroll
-            for (int j = 0; j < NTW; ++j) acc[j] = mfma_f64_16x16x4(a[u][j], b[u], acc[j]);
+            for (int j = 0; j < NTW; ++j) {
+                const double a = (k < n && ii[j] < n) ? (double)raw[u][j] : 0.0;
+                acc[j] = mfma_f64_16x16x4(a, b[u], acc[j]);
+            }
+        }
     }
 }
 
@@ -910,7 +934,7 @@ __device__ inline void mfma_apply(const AT* __restrict__ A, int n, const double*
     }
 }
 
-// Store the tiles of mfma_apply into X[i*16 + c] (LDS, rows up to 16 ceil(n/16)).
+// Store the tiles of mfma_apply into X[i*kYS + c] (LDS, rows up to 16 ceil(n/16)).
 template <int MT>
 __device__ inline void mfma_store(const dbl4 (&acc)[MT], int n, double* X, int wave, int nw, int wave0) {
     const int lane = threadIdx.x & 63, r16 = lane & 15, q4 = lane >> 4;
@@ -921,31 +945,78 @@ __device__ inline void mfma_store(const dbl4 (&acc)[MT], int n, double* X, int w
         const int t = t0 + j * nw;
         if (t < ntiles)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) X[(16 * t + 4 * v + q4) * 16 + r16] = acc[j][v];
+            for (int v = 0; v < 4; ++v) X[(16 * t + 4 * v + q4) * kYS + r16] = acc[j][v];
     }
 }
 
 struct ChSolveArgs {
-    const double* Ab;   // b-system factors, stride sb
-    const int* permb;   // stride pb
-    const double* Aup;  // u-p factors, stride su
-    const int* permu;   // stride pu
+    const void* Ib;     // b-system inverses (state precision), stride sb per (mode, parity), column-major, ld = n
+    const void* Iu;     // u-p-system inverses, stride su
     int N, nxh;
-    size_t sb, su, pb, pu;
+    size_t sb, su;
     int ld;             // LDS row length of one right-hand side (>= largest system)
-    double c;           // dt * a_ii
+    double c;           // implicit coefficient of the step
     double bc_b0;       // b(z=0) and b(z=Lz) in coefficient space at kx = 0 (Nx * value)
     double bc_b1;
 };
 
-// LDS doubles of k_ch_solve: y (2 x ld), the 64 x 64 block (also the permutation
-// scratch, 2 ld <= 4096), buoyancy (2 x (N/2+1))
-__host__ __device__ inline size_t ch_solve_lds_doubles(int ld, int N) {
-    return (size_t)2 * ld + 64 * 64 + 2 * (size_t)(N / 2 + 1);
+// x[r*ld + i] = sum_k A[k*n + i] y[r*ld + k], r = 0, 1 (real and imaginary part), for
+// a column-major n x n matrix A (global) and y, x in LDS; thread t owns rows t + j NT
+// (j < RT), FP64 accumulation, loads of U columns issued together.
+template <int NT, int RT, int U, typename AT>
+__device__ inline void gemv2_rows(const AT* __restrict__ A, int n, const double* y, int ld, double* x) {
+    double acc[RT][2];
+    int row[RT];
+#pragma unroll
+    for (int r = 0; r < RT; ++r) {
+        acc[r][0] = acc[r][1] = 0.0;
+        row[r] = threadIdx.x + r * NT;
+    }
+    for (int k0 = 0; k0 < n; k0 += U) {
+        AT raw[U][RT];  // every load of the batch is issued before any is used
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int r = 0; r < RT; ++r) {
+                const bool in = k0 + u < n && row[r] < n;
+                raw[u][r] = A[in ? (k0 + u) * n + row[r] : 0];  // n <= 1024: 32-bit offsets
+            }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = k0 + u < n ? k0 + u : n - 1;  // past n the coefficient is zero
+            const double y0 = y[k], y1 = y[ld + k];
+#pragma unroll
+            for (int r = 0; r < RT; ++r) {
+                const double a = (k0 + u < n && row[r] < n) ? (double)raw[u][r] : 0.0;
+                acc[r][0] += a * y0;
+                acc[r][1] += a * y1;
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+        if (row[r] < n) {
+            x[row[r]] = acc[r][0];
+            x[ld + row[r]] = acc[r][1];
+        }
 }
 
-// grid 2 nxh: workgroup (m, g) solves the b system of parity h = g^1 (whose values
-// enter the E3z rows of u-p group g as buoyancy) and then u-p group g.
+template <int NT, typename AT>
+__device__ inline void gemv2(const AT* __restrict__ A, int n, const double* y, int ld, double* x) {
+    switch ((n + NT - 1) / NT) {  // workgroup-uniform
+        case 1: gemv2_rows<NT, 1, 8>(A, n, y, ld, x); break;
+        case 2: gemv2_rows<NT, 2, 8>(A, n, y, ld, x); break;
+        case 3: gemv2_rows<NT, 3, 4>(A, n, y, ld, x); break;
+        default: gemv2_rows<NT, 4, 4>(A, n, y, ld, x); break;  // n <= 4 NT
+    }
+}
+
+// LDS doubles of k_ch_solve: right-hand sides y and solutions x (2 x ld each)
+__host__ __device__ inline size_t ch_solve_lds_doubles(int ld) { return (size_t)4 * ld; }
+
+// grid 2 nxh: workgroup (m, g) applies the explicit inverse of the b system of parity
+// h = g^1 (whose values enter the E3z rows of u-p group g as buoyancy) and then of
+// u-p group g.  Inverses in the state precision T, products in FP64.
 // rhs rows E1, E2, E3x, E3z (mode-major) in; state p, b, ux, uz, taus out.
 template <typename T, int NT>
 __global__ __launch_bounds__(NT) void k_ch_solve(ChSolveArgs s, const cplx<T>* __restrict__ R1,
@@ -956,11 +1027,8 @@ __global__ __launch_bounds__(NT) void k_ch_solve(ChSolveArgs s, const cplx<T>* _
     const int N = s.N, ld = s.ld;
     const int wg = blockIdx.x, m = wg >> 1, g = wg & 1, h = g ^ 1;
     const int PL = (N - 1) & 1;
-    double* y = reinterpret_cast<double*>(smem_raw);  // [2][ld]
-    double* blk = y + 2 * ld;                           // [64*64], also the permutation scratch
-    double* tmp = blk;
-    double* bre = blk + 64 * 64;                        // [N/2+1] buoyancy of parity h (re)
-    double* bim = bre + (N / 2 + 1);                    // [N/2+1] (im)
+    double* y = reinterpret_cast<double*>(smem_raw);  // [2][ld] right-hand sides
+    double* x = y + 2 * ld;                             // [2][ld] solutions
     const size_t base = (size_t)m * N;
     // ---- b system of parity h ----
     const int nh = npar(N, h), nbh = nh + 1;
@@ -976,18 +1044,18 @@ __global__ __launch_bounds__(NT) void k_ch_solve(ChSolveArgs s, const cplx<T>* _
         y[ld + nh] = 0.0;
     }
     __syncthreads();
-    const int wb = 2 * m + h;
-    lu_solve_blocked<NT, 2>(s.Ab + (size_t)wb * s.sb, s.permb + (size_t)wb * s.pb, nbh, y, ld, tmp, blk);
-    for (int i = threadIdx.x; i < nh; i += NT) {
-        bre[i] = y[i];
-        bim[i] = y[ld + i];
-        B[base + 2 * i + h] = cplx<T>{T(y[i]), T(y[ld + i])};
-    }
-    if (threadIdx.x == 0) taus[(size_t)m * 8 + (h == (PL ^ 1) ? 1 : 2)] = cplx<T>{T(y[nh]), T(y[ld + nh])};
+    gemv2<NT>(static_cast<const T*>(s.Ib) + (size_t)(2 * m + h) * s.sb, nbh, y, ld, x);
     __syncthreads();
-    // ---- u-p system of parity g ----
+    // ---- u-p system of parity g (buoyancy moved to the right-hand side) ----
     const int ng = npar(N, g);
     const int n = chu_size(N, g, m);
+    for (int i = threadIdx.x; i < nh; i += NT) {
+        B[base + 2 * i + h] = cplx<T>{T(x[i]), T(x[ld + i])};
+        const cplx<T> rz = R3z[base + 2 * i + h];
+        y[2 * ng + i] = (double)rz.x + s.c * x[i];
+        y[ld + 2 * ng + i] = (double)rz.y + s.c * x[ld + i];
+    }
+    if (threadIdx.x == 0) taus[(size_t)m * 8 + (h == (PL ^ 1) ? 1 : 2)] = cplx<T>{T(x[nh]), T(x[ld + nh])};
     for (int i = threadIdx.x; i < ng; i += NT) {
         const cplx<T> rx = R3x[base + 2 * i + g], r1 = R1[base + 2 * i + g];
         y[i] = (double)rx.y;  // -i * rx
@@ -995,28 +1063,24 @@ __global__ __launch_bounds__(NT) void k_ch_solve(ChSolveArgs s, const cplx<T>* _
         y[ng + i] = (double)r1.x;
         y[ld + ng + i] = (double)r1.y;
     }
-    for (int i = threadIdx.x; i < nh; i += NT) {
-        const cplx<T> rz = R3z[base + 2 * i + h];
-        y[2 * ng + i] = (double)rz.x + s.c * bre[i];  // buoyancy moved to the right-hand side
-        y[ld + 2 * ng + i] = (double)rz.y + s.c * bim[i];
-    }
     for (int r = 2 * ng + nh + threadIdx.x; r < n; r += NT) {
         y[r] = 0.0;
         y[ld + r] = 0.0;
     }
     __syncthreads();
-    lu_solve_blocked<NT, 2>(s.Aup + (size_t)wg * s.su, s.permu + (size_t)wg * s.pu, n, y, ld, tmp, blk);
+    gemv2<NT>(static_cast<const T*>(s.Iu) + (size_t)wg * s.su, n, y, ld, x);
+    __syncthreads();
     for (int i = threadIdx.x; i < ng; i += NT) {
-        UX[base + 2 * i + g] = cplx<T>{T(-y[ld + i]), T(y[i])};  // ux = i ux~
-        P[base + 2 * i + g] = cplx<T>{T(y[ng + i]), T(y[ld + ng + i])};
+        UX[base + 2 * i + g] = cplx<T>{T(-x[ld + i]), T(x[i])};  // ux = i ux~
+        P[base + 2 * i + g] = cplx<T>{T(x[ng + i]), T(x[ld + ng + i])};
     }
-    for (int i = threadIdx.x; i < nh; i += NT) UZ[base + 2 * i + h] = cplx<T>{T(y[2 * ng + i]), T(y[ld + 2 * ng + i])};
+    for (int i = threadIdx.x; i < nh; i += NT) UZ[base + 2 * i + h] = cplx<T>{T(x[2 * ng + i]), T(x[ld + 2 * ng + i])};
     if (threadIdx.x == 0) {
         cplx<T>* tm = taus + (size_t)m * 8;
         const int t0 = 2 * ng + nh;
         for (int t = 0; t < n - t0; ++t) {
             const int slot = chu_tau_slot(N, g, t);
-            const double re = y[t0 + t], im = y[ld + t0 + t];
+            const double re = x[t0 + t], im = x[ld + t0 + t];
             tm[slot] = (slot == 3 || slot == 5) ? cplx<T>{T(-im), T(re)} : cplx<T>{T(re), T(im)};  // tau_u.x = i tau~
         }
         if (m != 0 && g == 0) tm[0] = cplx<T>{T(0), T(0)};

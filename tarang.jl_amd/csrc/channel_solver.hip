@@ -92,10 +92,9 @@ ChannelSolver::ChannelSolver(Context* ctx, const ChannelDesc& d) : ctx_(ctx), d_
     upload(BDT_, BDT);
     upload(FT_, FT);
     // two parity systems per mode and block
-    Ab_ = DevBuf(ctx_, (size_t)2 * nxh_ * nb_max_ * nb_max_ * sizeof(double));
-    Aup_ = DevBuf(ctx_, (size_t)2 * nxh_ * nu_max_ * nu_max_ * sizeof(double));
-    permb_ = DevBuf(ctx_, (size_t)2 * nxh_ * nb_max_ * sizeof(int));
-    permu_ = DevBuf(ctx_, (size_t)2 * nxh_ * nu_max_ * sizeof(int));
+    const size_t ib = dtype_size((DType)d.dtype);  // inverses in the state precision
+    Ib_ = DevBuf(ctx_, (size_t)2 * nxh_ * nb_max_ * nb_max_ * ib);
+    Iu_ = DevBuf(ctx_, (size_t)2 * nxh_ * nu_max_ * nu_max_ * ib);
     info_ = DevBuf(ctx_, (size_t)4 * nxh_ * sizeof(int));
     ctx_->twiddle(mx_, (DType)d.dtype);
     TG_HIP(hipStreamSynchronize(s));
@@ -177,17 +176,27 @@ void ChannelSolver::factor(double c) {
     const int N = d_.nz;
     hipStream_t s = ctx_->stream;
     const size_t sb = (size_t)nb_max_ * nb_max_, su = (size_t)nu_max_ * nu_max_;
-    ChSys sys{N, nxh_, sb, su, 2.0 * M_PI / d_.Lx, 2.0 / d_.Lz, d_.kappa, d_.nu, c, d_.Lx, d_.Lz};
     const unsigned nsys = (unsigned)(2 * nxh_);
-    hipLaunchKernelGGL(k_ch_assemble, dim3(nsys, 2), dim3(256), 0, s, sys, Ab_.as<double>(), Aup_.as<double>());
+    // assemble and LU-factor in FP64 scratch buffers, then keep only the explicit inverses
+    DevBuf Lb(ctx_, nsys * sb * sizeof(double)), Lu(ctx_, nsys * su * sizeof(double));
+    DevBuf pb(ctx_, nsys * (size_t)nb_max_ * sizeof(int)), pu(ctx_, nsys * (size_t)nu_max_ * sizeof(int));
+    ChSys sys{N, nxh_, sb, su, 2.0 * M_PI / d_.Lx, 2.0 / d_.Lz, d_.kappa, d_.nu, c, d_.Lx, d_.Lz};
+    hipLaunchKernelGGL(k_ch_assemble, dim3(nsys, 2), dim3(256), 0, s, sys, Lb.as<double>(), Lu.as<double>());
     TG_HIP(hipGetLastError());
     int* info = info_.as<int>();
     constexpr int NT = 256;
-    hipLaunchKernelGGL((k_lu_factor<NT>), dim3(nsys), dim3(NT), (size_t)nb_max_ * 24, s, Ab_.as<double>(),
-                       permb_.as<int>(), info, N, 0, sb, (size_t)nb_max_);
+    hipLaunchKernelGGL((k_lu_factor<NT>), dim3(nsys), dim3(NT), (size_t)nb_max_ * 24, s, Lb.as<double>(),
+                       pb.as<int>(), info, N, 0, sb, (size_t)nb_max_);
+    hipLaunchKernelGGL((k_lu_factor<NT>), dim3(nsys), dim3(NT), (size_t)nu_max_ * 24, s, Lu.as<double>(),
+                       pu.as<int>(), info + nsys, N, 1, su, (size_t)nu_max_);
     TG_HIP(hipGetLastError());
-    hipLaunchKernelGGL((k_lu_factor<NT>), dim3(nsys), dim3(NT), (size_t)nu_max_ * 24, s, Aup_.as<double>(),
-                       permu_.as<int>(), info + nsys, N, 1, su, (size_t)nu_max_);
+    auto invert = [&](auto ot) {
+        using OT = decltype(ot);
+        lu_invert<NT, OT>(s, Lb.as<const double>(), pb.as<const int>(), Ib_.as<OT>(), N, 0, nsys, sb, nb_max_, nb_max_);
+        lu_invert<NT, OT>(s, Lu.as<const double>(), pu.as<const int>(), Iu_.as<OT>(), N, 1, nsys, su, nu_max_, nu_max_);
+    };
+    if (d_.dtype == kF32) invert(float{});
+    else invert(double{});
     TG_HIP(hipGetLastError());
     std::vector<int> h((size_t)2 * nsys);
     TG_HIP(hipMemcpyAsync(h.data(), info, h.size() * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -301,11 +310,10 @@ template <typename T>
 void ChannelSolver::solve_t() {
     using V = cplx<T>;
     const int N = d_.nz;
-    ChSolveArgs sa{Ab_.as<const double>(), permb_.as<const int>(), Aup_.as<const double>(), permu_.as<const int>(),
-                   N, nxh_, (size_t)nb_max_ * nb_max_, (size_t)nu_max_ * nu_max_, (size_t)nb_max_, (size_t)nu_max_,
-                   nu_max_, factored_c_, (double)d_.nx * d_.b_bottom, (double)d_.nx * d_.b_top};
+    ChSolveArgs sa{Ib_.ptr, Iu_.ptr, N, nxh_, (size_t)nb_max_ * nb_max_, (size_t)nu_max_ * nu_max_, nu_max_,
+                   factored_c_, (double)d_.nx * d_.b_bottom, (double)d_.nx * d_.b_top};
     constexpr int NT = 256;
-    const size_t lds = ch_solve_lds_doubles(nu_max_, N) * sizeof(double);
+    const size_t lds = ch_solve_lds_doubles(nu_max_) * sizeof(double);
     timed(4, [&] {
         hipLaunchKernelGGL((k_ch_solve<T, NT>), dim3(2 * nxh_), dim3(NT), lds, ctx_->stream, sa, R_[0].as<const V>(),
                            R_[1].as<const V>(), R_[2].as<const V>(), R_[3].as<const V>(), P_.as<V>(), B_.as<V>(),
@@ -505,12 +513,12 @@ void ChannelSolver::kernel_bytes(double* out) const {
     out[1] = (8 * nc + 3 * nc) * vb;                    // read 8 planes, write 3
     out[2] = (3 * nc + 3 * nc) * vb + N * N * rb;       // read 3 planes, write 3 F
     out[3] = (4 * nc + 4 * nc) * vb;                    // read p, b, ux, uz; write 4 rows
-    // solve: read the four parity factor blocks per mode (FP64) and 4 rhs rows, write 4 fields
+    // solve: read the four parity inverses per mode (state precision) and 4 rhs rows, write 4 fields
     double fac = 0;
     for (int m = 0; m < nxh_; ++m)
         for (int g = 0; g < 2; ++g) {
             const double nb = chb_size(d_.nz, g), nu = chu_size(d_.nz, g, m);
-            fac += (nb * nb + nu * nu) * 8.0;
+            fac += (nb * nb + nu * nu) * rb;
         }
     out[4] = fac + 8 * nc * vb;
     out[5] = 0;
@@ -521,8 +529,9 @@ size_t footprint(const ChannelDesc& d) {
     const size_t S = (d.stepper == kSBDF2 || d.stepper == kSBDF1) ? 3 : (size_t)make_tableau(d.stepper).stages;
     const size_t fields = (4 + 3 + 7 * S + 4 + 8 + 3 + 6) * fb;
     const size_t nb = chb_size(d.nz, 0), nu = chu_size(d.nz, 0, 0);
-    const size_t fac = 2 * nxh * (nb * nb + nu * nu) * 8 + 2 * nxh * (nb + nu) * 4;
-    return fields + fac + 3 * N * N * 8 + (1 << 20);
+    // inverses in the state precision, plus the FP64 LU scratch while they are formed
+    const size_t ent = 2 * nxh * (nb * nb + nu * nu);
+    return fields + ent * (8 + dtype_size((DType)d.dtype)) + 2 * nxh * (nb + nu) * 4 + 3 * N * N * 8 + (1 << 20);
 }
 
 }  // namespace tarang

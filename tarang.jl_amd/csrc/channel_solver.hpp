@@ -83,7 +83,7 @@ class ChannelSolver {
     double solve_c_ = 0;       // c of the factored systems used by the solve kernel
     DevBuf G_, H_;
     DevBuf BT_, BDT_, FT_;
-    DevBuf Ab_, Aup_, permb_, permu_, info_;
+    DevBuf Ib_, Iu_, info_;  // explicit inverses per (mode, parity) of the b and u-p systems, state precision
     bool graph_on_ = true;
     hipGraphExec_t graph_ = nullptr;
     double graph_dt_ = 0, graph_dtp_ = 0;

@@ -15,7 +15,7 @@ from tests.conftest import relerr
 
 pytestmark = pytest.mark.gpu
 
-TS = {"RK222": 0, "RK443": 1}
+TS = {"RK222": 0, "RK443": 1, "SBDF2": 2, "SBDF1": 4}
 TOL = {np.float64: 1e-12, np.float32: 1e-5}
 
 
@@ -95,6 +95,31 @@ def test_channel_steps_parity(gpu, oracle, ts, dtype):
         assert np.abs(s.get(oracle.RBC_P)).max() == 0.0
         for t in range(7):
             assert np.abs(s.tau(t)).max() == 0.0
+    s.close()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("ts", ["RK222", "SBDF2"])
+def test_channel_large_nz_parity(gpu, oracle, ts, dtype):
+    """nz = 450: u-p systems of 678 unknowns, whose inverses are formed 8 columns per
+    workgroup (16 do not fit in LDS), and four solution rows per thread in the solve."""
+    nx, N, Lx, Lz, dt = 8, 450, 4.0, 1.0, 5e-3
+    print(f"channel large nz {ts} {dtype.__name__}", flush=True)
+    kappa, nu = params()
+    st = random_state(oracle, nx, N, Lx, Lz, seed=17)
+    ref = oracle.RBC2D(nx, N, Lx, Lz, kappa, nu, dt, ts)
+    s = gpu.chan(nx, N, Lx, Lz, kappa, nu, dt, stepper=TS[ts], dtype=dtype)
+    for var, c in st.items():
+        ref.set(var, c)
+        s.set(var, c)
+    ref.step(2)
+    s.step(2)
+    # at nz = 450 the u-p systems have condition numbers ~1e10: an FP64 LU solve and an
+    # FP64 explicit inverse each differ from the exact solution by up to ~3e-12 relative
+    # (checked against iteratively refined solves), so the two paths agree to ~1e-11
+    tol = {np.float64: 1e-10, np.float32: 1e-4}[dtype]
+    for var in (oracle.RBC_B, oracle.RBC_UX, oracle.RBC_UZ):
+        assert relerr(s.get(var), ref.get(var)) < tol, var
     s.close()
 
 
@@ -186,12 +211,14 @@ def test_channel_full_size_modewise(gpu, oracle, dtype):
     ref.step(3)
     s.step(3)
     tol = {np.float64: 1e-11, np.float32: 1e-4}[dtype]
+    b0 = np.abs(ref.get(oracle.RBC_B)[0]).max()  # the O(1) conduction profile carried by kx = 0
     for var in (oracle.RBC_B, oracle.RBC_UX, oracle.RBC_UZ):
         g = s.get(var) / scale
         r = ref.get(var)
         for m in (0, 1, 3):
-            # a mode that is roundoff (uz at kx = 0) is measured against the perturbation size
-            den = max(np.abs(r[m]).max(), np.abs(r[1]).max())
+            # modes 1, 3 against the perturbation size; at kx = 0 (where uz is roundoff) the
+            # solve's rounding scales with the conduction profile of that mode
+            den = max(np.abs(r[m]).max(), np.abs(r[1]).max(), b0 if m == 0 else 0.0)
             assert np.abs(g[m] - r[m]).max() / den < tol, (var, m)
         assert np.abs(g[small // 2 + 1:]).max() < 1e-9 * np.abs(r[1]).max()
     s.close()
