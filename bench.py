@@ -38,6 +38,30 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def measured_traffic(name, kernel_label):
+    """HBM bytes per launch of a kernel from a committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+    (profiles/<name>), or None when the kernel is not in it."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get(kernel_label.split()[0])
+
+
+def calibration_copy(mib):
+    """One plain device-to-device copy of `mib` MiB: the reference reading for HBM byte counters."""
+    import torch
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    src = torch.ones(mib * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"calibration copy: {mib} MiB read + {mib} MiB written")
+    del src, dst
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -55,6 +79,8 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (no collectives); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
+    if a.copy_mib > 0:
+        calibration_copy(a.copy_mib)
     if a.workload == "channel":
         return bench_channel(a)
     if a.workload == "serving":
@@ -69,15 +95,6 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     dev = torch.device("cuda", 0)
-    if a.copy_mib > 0:
-        n = a.copy_mib * (1 << 20) // 4
-        src = torch.ones(n, dtype=torch.float32, device=dev)
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"calibration copy: {a.copy_mib} MiB read + {a.copy_mib} MiB written")
-        del src, dst
     log(f"building initial condition {N}x{N}")
     q0 = configs.turbulence_2d(N, seed=42, kpeak=10.0, energy=0.5)
     C0 = np.fft.fft(np.fft.rfft(q0, axis=0), axis=1)  # reference layout (kx, ky), unnormalised
@@ -218,7 +235,7 @@ CH_KERNELS = ["k_ch_zbwd (Chebyshev evaluation of 8 operand planes)",
               "k_ch_xpass (inverse x FFTs M=1536, products, forward x FFTs)",
               "k_ch_zfwd (Chebyshev forward of 3 products)",
               "k_ch_lx (structured L x per mode)",
-              "k_ch_solve (per-mode FP64 LU solves, parity-split systems)",
+              "k_ch_solve (per-mode explicit inverses, FP64 GEMV, parity-split systems)",
               "stage / copy / zero"]
 
 
@@ -334,7 +351,8 @@ def bench_channel(a):
                                  "nusselt": abs(nus["f32"] - nus["f64"]) / abs(nus["f64"]), "after_steps": steps},
         "nusselt": nus,
         "roofline": {"bound": "hbm", "kernel": CH_KERNELS[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("traffic_channel2d_1024x256_f32.json", CH_KERNELS[dom])},
         "kernels_f32": kernels,
         "cpu_baseline": cpu,
     }
@@ -347,7 +365,7 @@ CH3_WORKLOAD = ("3-D Rayleigh-Benard, RealFourier^2 x ChebyshevT 256x256x128, Lx
 CH3_KERNELS = ["k3_zbwd (Chebyshev evaluation of 15 operand planes)", "k3_ybwd (padded inverse y FFT, MY=384)",
                "k3_xpass (inverse x FFTs MX=384, 12 products, forward x FFTs)",
                "k3_yfwd (forward y FFT, truncation, Nyquist folds)", "k_ch_zfwd (Chebyshev forward of 4 products)",
-               "k3_sbdf (multistep right-hand side)", "k3_solve (per-mode FP64 LU solves, rotated parity systems)",
+               "k3_sbdf (multistep right-hand side)", "k3_solve (explicit inverses per |k_h|^2 key, 8-mode chunks on FP64 matrix cores)",
                "k3_lx (structured L x, RK only)"]
 
 
@@ -465,7 +483,8 @@ def bench_channel3d(a):
                                  "after_steps": steps + max(a.warmup, 3)},
         "nusselt": nus,
         "roofline": {"bound": "hbm", "kernel": CH3_KERNELS[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("traffic_channel3d_256x256x128_f32.json", CH3_KERNELS[dom])},
         "kernels_f32": kernels,
         "cpu_baseline": cpu,
     }

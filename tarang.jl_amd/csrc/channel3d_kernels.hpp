@@ -349,7 +349,7 @@ __global__ __launch_bounds__(NT) void k3_yfwd(const cplx<T>* __restrict__ H, Ch3
 //   u~par = (kx ux~ + ky uy~)/|k_h|,  u~perp = (-ky ux~ + kx uy~)/|k_h|   (x, y at k_h = 0)
 // every mode splits into: the b system (kappa), the perp system (nu, walls 0) and
 // the 2-D u-p system with kx -> |k_h|, each in two parity groups.  The systems
-// depend on |k_h|^2 only; modes share the factors of their key.
+// depend on |k_h|^2 only; modes share the inverses of their key.
 // ---------------------------------------------------------------------------
 struct Ch3Sys {
     int N, nkeys;

@@ -3,7 +3,8 @@
 // Step (src/core/timesteppers/step_subproblem_multistep.jl:197-420, _sbdf2_coefs :544-552):
 //   F_n = F(P(x_n)) (z-backward, padded y and x passes, z-forward), M P(x_n)
 //   rhs = (c1 F_n + c2 F_{n-1} - a1 M P(x_n) - a2 M P(x_{n-1})) / a0, BC rows c ALG
-//   x_{n+1} = (M + L/a0)^-1 rhs  per mode (three parity-split FP64 systems per parity)
+//   x_{n+1} = (M + L/a0)^-1 rhs  per mode: three parity-split systems per parity, whose
+//   explicit inverses (formed in FP64 per |k_h|^2 key) are applied to 8-mode chunks
 // SBDF1 on the first step after create / set_field, SBDF2 (w = dt/dt_prev) after.
 #include <algorithm>
 #include <array>

@@ -1,7 +1,8 @@
 // channel_solver.hpp -- 2-D Rayleigh-Benard convection on RealFourier(x) x
-// ChebyshevT(z) with the Chebyshev-tau method, stepped on the GPU with IMEX RK.
-// Per Fourier mode the implicit system splits into a buoyancy block and a
-// velocity-pressure block; both are factored once per (dt * a_ii) in FP64.
+// ChebyshevT(z) with the Chebyshev-tau method, stepped on the GPU with IMEX RK or
+// SBDF.  Per Fourier mode the implicit system splits into a buoyancy block and a
+// velocity-pressure block (each by parity); both are factored in FP64 once per
+// implicit coefficient and kept as explicit inverses in the state precision.
 #pragma once
 #include <vector>
 
