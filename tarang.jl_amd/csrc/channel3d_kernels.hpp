@@ -409,7 +409,7 @@ __host__ __device__ inline size_t ch3_solve_lds_doubles(int N) {
 // and the perp system of parity g together, then the u-p system of parity g (its
 // E3z rows take + c b), then the un-rotation of the parity-g horizontal velocity.
 template <typename T, int NT>
-__global__ __launch_bounds__(NT) void k3_solve(Ch3SolveArgs s, const cplx<T>* __restrict__ R1,
+__global__ __launch_bounds__(NT, 3) void k3_solve(Ch3SolveArgs s, const cplx<T>* __restrict__ R1,
                                                const cplx<T>* __restrict__ R2, const cplx<T>* __restrict__ R3x,
                                                const cplx<T>* __restrict__ R3y, const cplx<T>* __restrict__ R3z,
                                                cplx<T>* P, cplx<T>* B, cplx<T>* UX, cplx<T>* UY, cplx<T>* UZ,

@@ -889,28 +889,39 @@ __device__ inline void mfma_apply_n(const AT* __restrict__ A, int n, const doubl
     int ii[NTW];
 #pragma unroll
     for (int j = 0; j < NTW; ++j) ii[j] = 16 * (t0 + j * nw) + r16;
-    for (int ks = 0; ks < ksteps; ks += U) {
-        AT raw[U][NTW];  // every load of the batch is issued before any is used
-        double b[U];
+    // two load buffers in turn: the loads of one batch are in flight while the other's
+    // products run (no copies between them, so no wait is forced early)
+    AT ra[U][NTW], rb[U][NTW];
+    auto load = [&](int ks, AT (&raw)[U][NTW]) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int k = 4 * (ks + u) + q4;
-            b[u] = Y[min(k, krow_max) * kYS + r16];
 #pragma unroll
             for (int j = 0; j < NTW; ++j) {
                 const bool in = k < n && ii[j] < n;
                 raw[u][j] = A[in ? k * n + ii[j] : 0];  // n <= 512: 32-bit offsets from a uniform base
             }
         }
+    };
+    auto multiply = [&](int ks, const AT (&raw)[U][NTW]) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int k = 4 * (ks + u) + q4;
+            const double b = Y[min(k, krow_max) * kYS + r16];
 #pragma unroll
             for (int j = 0; j < NTW; ++j) {
                 const double a = (k < n && ii[j] < n) ? (double)raw[u][j] : 0.0;
-                acc[j] = mfma_f64_16x16x4(a, b[u], acc[j]);
+                acc[j] = mfma_f64_16x16x4(a, b, acc[j]);
             }
         }
+    };
+    load(0, ra);
+    for (int ks = 0; ks < ksteps; ks += 2 * U) {
+        load(ks + U, rb);  // past the end these read element 0 and are never used
+        multiply(ks, ra);
+        if (ks + U >= ksteps) break;  // wave-uniform
+        load(ks + 2 * U, ra);
+        multiply(ks + U, rb);
     }
 }
 
@@ -922,14 +933,15 @@ __device__ inline void mfma_apply(const AT* __restrict__ A, int n, const double*
     const int ntw = t0 < ntiles ? (ntiles - t0 + nw - 1) / nw : 0;
 #pragma unroll
     for (int j = 0; j < MT; ++j) acc[j] = dbl4{0.0, 0.0, 0.0, 0.0};
-    switch (ntw < MT ? ntw : MT) {  // wave-uniform
-        case 1: mfma_apply_n<1, 8>(A, n, Y, acc, t0, nw); break;
-        case 2: if constexpr (MT >= 2) mfma_apply_n<2, 8>(A, n, Y, acc, t0, nw); break;
-        case 3: if constexpr (MT >= 3) mfma_apply_n<3, 8>(A, n, Y, acc, t0, nw); break;
-        case 4: if constexpr (MT >= 4) mfma_apply_n<4, 8>(A, n, Y, acc, t0, nw); break;
-        case 5: if constexpr (MT >= 5) mfma_apply_n<5, 4>(A, n, Y, acc, t0, nw); break;
-        case 6: if constexpr (MT >= 6) mfma_apply_n<6, 4>(A, n, Y, acc, t0, nw); break;
-        case 7: if constexpr (MT >= 7) mfma_apply_n<7, 4>(A, n, Y, acc, t0, nw); break;
+    constexpr bool F32 = sizeof(AT) == 4;  // batch depth: both load buffers stay within the registers
+    switch (ntw < MT ? ntw : MT) {         // wave-uniform
+        case 1: mfma_apply_n<1, 4>(A, n, Y, acc, t0, nw); break;
+        case 2: if constexpr (MT >= 2) mfma_apply_n<2, 4>(A, n, Y, acc, t0, nw); break;
+        case 3: if constexpr (MT >= 3) mfma_apply_n<3, (F32 ? 4 : 2)>(A, n, Y, acc, t0, nw); break;
+        case 4: if constexpr (MT >= 4) mfma_apply_n<4, (F32 ? 4 : 2)>(A, n, Y, acc, t0, nw); break;
+        case 5: if constexpr (MT >= 5) mfma_apply_n<5, 2>(A, n, Y, acc, t0, nw); break;
+        case 6: if constexpr (MT >= 6) mfma_apply_n<6, 2>(A, n, Y, acc, t0, nw); break;
+        case 7: if constexpr (MT >= 7) mfma_apply_n<7, 2>(A, n, Y, acc, t0, nw); break;
         default: break;
     }
 }

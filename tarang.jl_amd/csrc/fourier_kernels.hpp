@@ -98,8 +98,9 @@ struct BurgersSpec {
 // k_ybwd: one field and one tile of TK kx columns per workgroup; build the
 // padded column (multiplier, Hermitian projection at kx=0 / kx=Nx/2, Nyquist
 // split), inverse FFT of length MY, store to g[f] (nxh x MY, kx fastest).
-// Grid: (ceil(nxh/TK), NF); the launch id is remapped so that tiles sharing
-// cache lines run on the same XCD.
+// Grid: (ceil(nxh/TK), NF); the launch id is remapped so that the NF workgroups of
+// one tile (which read the same source columns) and neighbouring tiles (which share
+// cache lines) run side by side on the same XCD.
 // ---------------------------------------------------------------------------
 template <typename T, int NY, int MY, int TK, int NT, typename Spec>
 __global__ __launch_bounds__(NT) void k_ybwd(FourierParams<T> p, cplx<T>* __restrict__ g, size_t gstride,
@@ -110,10 +111,10 @@ __global__ __launch_bounds__(NT) void k_ybwd(FourierParams<T> p, cplx<T>* __rest
     constexpr int NN = NY - NY / 2 - 1;  // negative frequencies kept
     __shared__ V buf[TK * LS];
     __shared__ V twl[MY];
-    const unsigned ntiles = gridDim.x;
-    const unsigned lin = xcd_remap(blockIdx.x + blockIdx.y * ntiles, ntiles * gridDim.y);
-    const int f = (int)(lin / ntiles);
-    const int kx0 = (int)(lin % ntiles) * TK;
+    const unsigned ntiles = gridDim.x, nf = gridDim.y;
+    const unsigned lin = xcd_remap(blockIdx.x + blockIdx.y * ntiles, ntiles * nf);
+    const int f = (int)(lin % nf);  // field fastest: one tile's fields are neighbours
+    const int kx0 = (int)(lin / nf) * TK;
     const bool xeven = (p.nx % 2) == 0;
     load_twiddles<T, MY, NT>(twl, tw);
     // 1) issue every source load of this thread before touching LDS
