@@ -40,29 +40,31 @@ __host__ __device__ inline double ch3_ky(const Ch3Geom& g, int n) {
 // tile are the columns of an LDS block C[k][col], in groups of 16 modes: column
 // 32 g + j is Re, 32 g + 16 + j is Im of mode q0 + 16 g + j, and
 //   value[z][col] = sum_k BT[k][z] C[k][col],   dz[z][col] = sum_k BDT[k][z] C[k][col]
-// as 16 x 16 tiles (wave w: z tiles 2w, 2w+1 (+8, ...), every column tile, both
-// products).  A lane then holds Re and Im of one mode in two accumulators and writes
+// as 16 x 16 tiles (wave w: z tiles ZPW w .. ZPW w + ZPW - 1, then the next pass; every
+// column tile, both products).  A lane then holds Re and Im of one mode in two accumulators and writes
 // whole complex values straight to the z-major planes (16 consecutive modes per row).
 // Derived planes: dx f = i kx value, dy f = i ky value.  On the kx = 0 and kx = Nx/2
 // lines the projected state's x derivative vanishes, and so does its y derivative at
 // the y Nyquist; the kx = Nx/2 line is halved.
 // ---------------------------------------------------------------------------
-// modes per workgroup (QT, a multiple of 16) and k steps per load batch (U)
-template <typename T> struct ZbwdTile { static constexpr int QT = 32, U = 2; };
-template <> struct ZbwdTile<double> { static constexpr int QT = 16, U = 4; };
+// modes per workgroup (QT, a multiple of 16), k steps per load batch (U), threads per
+// workgroup (NT) and z tiles per wavefront and pass (ZPW)
+template <typename T> struct ZbwdTile { static constexpr int QT = 32, U = 2, NT = 512, ZPW = 1; };
+template <> struct ZbwdTile<double> { static constexpr int QT = 16, U = 4, NT = 256, ZPW = 2; };
 
 __host__ __device__ inline int zbwd_lds_stride(int qt) { return 2 * qt + 1; }
 
 template <typename T, int NT>
-__global__ __launch_bounds__(NT, 3) void k3_zbwd(const cplx<T>* __restrict__ b, const cplx<T>* __restrict__ ux,
+__global__ __launch_bounds__(NT, 768 / NT) void k3_zbwd(const cplx<T>* __restrict__ b, const cplx<T>* __restrict__ ux,
                                                  const cplx<T>* __restrict__ uy, const cplx<T>* __restrict__ uz,
                                                  Ch3Geom geo, const T* __restrict__ BT, const T* __restrict__ BDT,
                                                  cplx<T>* __restrict__ G) {
-    static_assert(NT == 256, "four wavefronts per workgroup");
+    static_assert(NT == ZbwdTile<T>::NT, "workgroup size of the tile plan");
     using Op = MfmaOp<T>;
     using acc4 = typename Op::acc4;
     using V = cplx<T>;
     constexpr int QT = ZbwdTile<T>::QT, NG = QT / 16, CT = 2 * NG, S = 2 * QT + 1;
+    constexpr int ZPW = ZbwdTile<T>::ZPW, NWV = NT / 64;
     extern __shared__ unsigned char smem_raw[];
     T* C = reinterpret_cast<T*>(smem_raw);  // [KP][S]
     const int N = geo.N, nmodes = geo.nmodes;
@@ -98,31 +100,30 @@ __global__ __launch_bounds__(NT, 3) void k3_zbwd(const cplx<T>* __restrict__ b, 
         dxz[g] = xline;
         dyz[g] = xline && yeven && n == geo.ny / 2;
     }
-    for (int zt0 = 2 * wave; zt0 < nzt; zt0 += 8) {
-        acc4 acc[2][2][CT];  // [z tile][value, dz][column tile]
+    for (int zt0 = ZPW * wave; zt0 < nzt; zt0 += ZPW * NWV) {
+        acc4 acc[ZPW][2][CT];  // [z tile][value, dz][column tile]
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
+        for (int a = 0; a < ZPW; ++a)
 #pragma unroll
             for (int p = 0; p < 2; ++p)
 #pragma unroll
                 for (int ct = 0; ct < CT; ++ct) acc[a][p][ct] = acc4{T(0), T(0), T(0), T(0)};
-        const int z0 = 16 * zt0 + r16, z1 = z0 + 16;
         constexpr int U = ZbwdTile<T>::U;  // k steps per batch: its loads are issued before its products
         for (int ks = 0; ks < ksteps; ks += U) {
-            T am[U][4], bv[U][CT];
+            T am[U][ZPW][2], bv[U][CT];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int k = 4 * (ks + u) + q4;
-                const bool in0 = k < N && z0 < N, in1 = k < N && z1 < N;
-                const int o0 = in0 ? k * N + z0 : 0, o1 = in1 ? k * N + z1 : 0;
-                am[u][0] = BT[o0];
-                am[u][1] = BDT[o0];
-                am[u][2] = BT[o1];
-                am[u][3] = BDT[o1];
-                am[u][0] = in0 ? am[u][0] : T(0);
-                am[u][1] = in0 ? am[u][1] : T(0);
-                am[u][2] = in1 ? am[u][2] : T(0);
-                am[u][3] = in1 ? am[u][3] : T(0);
+#pragma unroll
+                for (int a = 0; a < ZPW; ++a) {
+                    const int z = 16 * (zt0 + a) + r16;
+                    const bool in = k < N && z < N;
+                    const int o = in ? k * N + z : 0;
+                    am[u][a][0] = BT[o];
+                    am[u][a][1] = BDT[o];
+                    am[u][a][0] = in ? am[u][a][0] : T(0);
+                    am[u][a][1] = in ? am[u][a][1] : T(0);
+                }
                 const int kr = k < KP ? k : KP - 1;  // rows N .. KP - 1 are zero; beyond, a is zero
 #pragma unroll
                 for (int ct = 0; ct < CT; ++ct) bv[u][ct] = C[kr * S + 16 * ct + r16];
@@ -130,15 +131,15 @@ __global__ __launch_bounds__(NT, 3) void k3_zbwd(const cplx<T>* __restrict__ b, 
 #pragma unroll
             for (int u = 0; u < U; ++u)
 #pragma unroll
-                for (int ct = 0; ct < CT; ++ct) {
-                    acc[0][0][ct] = Op::mma(am[u][0], bv[u][ct], acc[0][0][ct]);
-                    acc[0][1][ct] = Op::mma(am[u][1], bv[u][ct], acc[0][1][ct]);
-                    acc[1][0][ct] = Op::mma(am[u][2], bv[u][ct], acc[1][0][ct]);
-                    acc[1][1][ct] = Op::mma(am[u][3], bv[u][ct], acc[1][1][ct]);
-                }
+                for (int a = 0; a < ZPW; ++a)
+#pragma unroll
+                    for (int ct = 0; ct < CT; ++ct) {
+                        acc[a][0][ct] = Op::mma(am[u][a][0], bv[u][ct], acc[a][0][ct]);
+                        acc[a][1][ct] = Op::mma(am[u][a][1], bv[u][ct], acc[a][1][ct]);
+                    }
         }
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
+        for (int a = 0; a < ZPW; ++a)
 #pragma unroll
             for (int g = 0; g < NG; ++g)
 #pragma unroll
@@ -409,12 +410,13 @@ __host__ __device__ inline size_t ch3_solve_lds_doubles(int N) {
 // and the perp system of parity g together, then the u-p system of parity g (its
 // E3z rows take + c b), then the un-rotation of the parity-g horizontal velocity.
 template <typename T, int NT>
-__global__ __launch_bounds__(NT, 3) void k3_solve(Ch3SolveArgs s, const cplx<T>* __restrict__ R1,
+__global__ __launch_bounds__(NT, 768 / NT) void k3_solve(Ch3SolveArgs s, const cplx<T>* __restrict__ R1,
                                                const cplx<T>* __restrict__ R2, const cplx<T>* __restrict__ R3x,
                                                const cplx<T>* __restrict__ R3y, const cplx<T>* __restrict__ R3z,
                                                cplx<T>* P, cplx<T>* B, cplx<T>* UX, cplx<T>* UY, cplx<T>* UZ,
                                                cplx<T>* taus) {
-    static_assert(NT == 256, "four wavefronts per workgroup");
+    static_assert(NT == 256 || NT == 512, "four or eight wavefronts per workgroup");
+    constexpr int MTB = NT == 256 ? 3 : 2, MTU = NT == 256 ? 7 : 4;  // tiles per wave for nz <= 256
     extern __shared__ unsigned char smem_raw[];
     __shared__ int smode[kCh3ChunkModes];
     __shared__ double srot[2 * kCh3ChunkModes];
@@ -469,12 +471,12 @@ __global__ __launch_bounds__(NT, 3) void k3_solve(Ch3SolveArgs s, const cplx<T>*
     // ---- b (parity h) and perp (parity g) systems ----
     const int wave = threadIdx.x >> 6, nw = NT / 64, w0p = ((nb + 15) >> 4) % nw;
     {
-        dbl4 ab[3], ap[3];
-        mfma_apply<3>(static_cast<const T*>(s.Ib) + (size_t)(2 * key + h) * s.sb, nb, Yb, ab, wave, nw, 0);
-        mfma_apply<3>(static_cast<const T*>(s.Ip) + (size_t)(2 * key + g) * s.sb, np, Yp, ap, wave, nw, w0p);
+        dbl4 ab[MTB], ap[MTB];
+        mfma_apply<MTB>(static_cast<const T*>(s.Ib) + (size_t)(2 * key + h) * s.sb, nb, Yb, ab, wave, nw, 0);
+        mfma_apply<MTB>(static_cast<const T*>(s.Ip) + (size_t)(2 * key + g) * s.sb, np, Yp, ap, wave, nw, w0p);
         __syncthreads();
-        mfma_store<3>(ab, nb, Yb, wave, nw, 0);
-        mfma_store<3>(ap, np, Yp, wave, nw, w0p);
+        mfma_store<MTB>(ab, nb, Yb, wave, nw, 0);
+        mfma_store<MTB>(ap, np, Yp, wave, nw, w0p);
     }
     __syncthreads();
     for (int idx = threadIdx.x; idx < cnt * nh; idx += NT) {
@@ -492,10 +494,10 @@ __global__ __launch_bounds__(NT, 3) void k3_solve(Ch3SolveArgs s, const cplx<T>*
     __syncthreads();
     // ---- u-p system of parity g ----
     {
-        dbl4 au[7];
-        mfma_apply<7>(static_cast<const T*>(s.Iu) + (size_t)(2 * key + g) * s.su, nu, Yu, au, wave, nw, 0);
+        dbl4 au[MTU];
+        mfma_apply<MTU>(static_cast<const T*>(s.Iu) + (size_t)(2 * key + g) * s.su, nu, Yu, au, wave, nw, 0);
         __syncthreads();
-        mfma_store<7>(au, nu, Yu, wave, nw, 0);
+        mfma_store<MTU>(au, nu, Yu, wave, nw, 0);
     }
     __syncthreads();
     // un-rotate: u~x = cos par - sin perp, u~y = sin par + cos perp; u = i u~

@@ -294,10 +294,10 @@ void Channel3Solver::rhs_t(void* F0, void* F1, void* F2, void* F3) {
     V* Q = Q_.as<V>();
     V* H = H_.as<V>();
     {
-        constexpr int QT = ZbwdTile<T>::QT;
+        constexpr int QT = ZbwdTile<T>::QT, NTZ = ZbwdTile<T>::NT;
         const size_t blds = (size_t)((N + 15) & ~15) * zbwd_lds_stride(QT) * sizeof(T);
         timed(0, [&] {
-            hipLaunchKernelGGL((k3_zbwd<T, 256>), dim3((unsigned)((nmodes_ + QT - 1) / QT), 4), dim3(256), blds, s,
+            hipLaunchKernelGGL((k3_zbwd<T, NTZ>), dim3((unsigned)((nmodes_ + QT - 1) / QT), 4), dim3(NTZ), blds, s,
                                B_.as<const V>(), UX_.as<const V>(), UY_.as<const V>(), UZ_.as<const V>(), geo,
                                BT_.as<const T>(), BDT_.as<const T>(), G);
         });
@@ -345,7 +345,8 @@ void Channel3Solver::rhs_t(void* F0, void* F1, void* F2, void* F3) {
         constexpr int QT = std::is_same<T, float>::value ? 32 : 16;  // 64 or 32 columns
         const size_t flds = ch_zfwd_lds_elems(N, QT) * sizeof(T);
         timed(4, [&] {
-            hipLaunchKernelGGL((k_ch_zfwd<T, QT, 256>), dim3((unsigned)((nmodes_ + QT - 1) / QT), 4), dim3(256), flds, s,
+            hipLaunchKernelGGL((k_ch_zfwd<T, QT, ZfwdTile<T>::NT>), dim3((unsigned)((nmodes_ + QT - 1) / QT), 4),
+                               dim3(ZfwdTile<T>::NT), flds, s,
                                Fz, nmodes_, 1, N, FT_.as<const T>(), static_cast<V*>(F0), static_cast<V*>(F1),
                                static_cast<V*>(F2), static_cast<V*>(F3));
         });
@@ -394,7 +395,7 @@ void Channel3Solver::solve_t() {
                     chunks_.as<const int4>(), modes_.as<const int>(), rot_.as<const double>(), N,
                     (size_t)nb_max_ * nb_max_, (size_t)nu_max_ * nu_max_, factored_c_, scale * d_.b_bottom,
                     scale * d_.b_top};
-    constexpr int NT = 256;
+    constexpr int NT = 256;  // four wavefronts: at most 3 b/perp and 7 u-p tiles each (nz <= 256)
     const size_t lds = ch3_solve_lds_doubles(N) * sizeof(double);
     timed(6, [&] {
         hipLaunchKernelGGL((k3_solve<T, NT>), dim3(nchunks_), dim3(NT), lds, ctx_->stream, sa, R_[0].as<const V>(),

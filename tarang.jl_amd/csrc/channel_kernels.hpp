@@ -241,27 +241,29 @@ __global__ __launch_bounds__(NT) void k_ch_xpass(const cplx<T>* __restrict__ G, 
 //   F[m][k] = sum_z Fm[k][z] H[z][m],   FT[z*N + k] = Fm[k][z]
 // The plane tile H[z][q0 .. q0+QT) sits in LDS as columns in groups of 16 modes (Re
 // block, Im block); 16 x 16 tiles of (k, column).  Split s of the workgroup's k tiles
-// [s*kps, (s+1)*kps): wave w takes tiles 2w, 2w+1 of it (+8, ...).
+// [s*kps, (s+1)*kps): wave w takes tiles KPW w .. KPW w + KPW - 1 of it, then the next pass.
 // A lane holds Re and Im of one mode for four k and stores them directly.  The
 // kx = Nx/2 column takes the x Nyquist fold 2 Re(H) (nx odd switches the fold off:
 // the 3-D path folds in its y pass).
 // ---------------------------------------------------------------------------
-// z steps per load batch, per arithmetic type (modes per workgroup QT: 16 or 32)
-template <typename T> struct ZfwdTile { static constexpr int U = 2; };
-template <> struct ZfwdTile<double> { static constexpr int U = 4; };
+// z steps per load batch (U), threads per workgroup (NT) and k tiles per wavefront and
+// pass (KPW), per arithmetic type (modes per workgroup QT: 16 or 32)
+template <typename T> struct ZfwdTile { static constexpr int U = 2, NT = 512, KPW = 1; };
+template <> struct ZfwdTile<double> { static constexpr int U = 4, NT = 256, KPW = 2; };
 
 __host__ __device__ inline size_t ch_zfwd_lds_elems(int N, int qt) { return (size_t)((N + 15) & ~15) * (2 * qt + 1); }
 
 template <typename T, int QT, int NT>
-__global__ __launch_bounds__(NT, 3) void k_ch_zfwd(const cplx<T>* __restrict__ H, int nmodes, int nx, int N,
+__global__ __launch_bounds__(NT, 768 / NT) void k_ch_zfwd(const cplx<T>* __restrict__ H, int nmodes, int nx, int N,
                                                    const T* __restrict__ FT, cplx<T>* __restrict__ F0,
                                                    cplx<T>* __restrict__ F1, cplx<T>* __restrict__ F2,
                                                    cplx<T>* __restrict__ F3 = nullptr) {
-    static_assert(NT == 256 && QT % 16 == 0, "four wavefronts per workgroup, 16-mode groups");
+    static_assert(NT == ZfwdTile<T>::NT && QT % 16 == 0, "workgroup size of the tile plan, 16-mode groups");
     using V = cplx<T>;
     using Op = MfmaOp<T>;
     using acc4 = typename Op::acc4;
     constexpr int NG = QT / 16, CT = 2 * NG, S = 2 * QT + 1;
+    constexpr int KPW = ZfwdTile<T>::KPW, NWV = NT / 64;
     extern __shared__ unsigned char smem_raw[];
     T* C = reinterpret_cast<T*>(smem_raw);  // [KP][S]
     const int KP = (N + 15) & ~15, nkt = KP >> 4, zsteps = (N + 3) >> 2;
@@ -285,24 +287,25 @@ __global__ __launch_bounds__(NT, 3) void k_ch_zfwd(const cplx<T>* __restrict__ H
     __syncthreads();
     const int lane = threadIdx.x & 63, r16 = lane & 15, q4 = lane >> 4, wave = threadIdx.x >> 6;
     V* out = f == 0 ? F0 : (f == 1 ? F1 : (f == 2 ? F2 : F3));
-    for (int kt0 = kb + 2 * wave; kt0 < ke; kt0 += 8) {
-        acc4 acc[2][CT];  // [k tile][column tile]
+    for (int kt0 = kb + KPW * wave; kt0 < ke; kt0 += KPW * NWV) {
+        acc4 acc[KPW][CT];  // [k tile][column tile]
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
+        for (int a = 0; a < KPW; ++a)
 #pragma unroll
             for (int ct = 0; ct < CT; ++ct) acc[a][ct] = acc4{T(0), T(0), T(0), T(0)};
-        const int k0 = 16 * kt0 + r16, k1 = k0 + 16;
         constexpr int U = ZfwdTile<T>::U;
         for (int zs = 0; zs < zsteps; zs += U) {
-            T am[U][2], bv[U][CT];
+            T am[U][KPW], bv[U][CT];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int z = 4 * (zs + u) + q4;
-                const bool in0 = z < N && k0 < N, in1 = z < N && k1 < N;
-                am[u][0] = FT[in0 ? z * N + k0 : 0];
-                am[u][1] = FT[in1 ? z * N + k1 : 0];
-                am[u][0] = in0 ? am[u][0] : T(0);
-                am[u][1] = in1 ? am[u][1] : T(0);
+#pragma unroll
+                for (int a = 0; a < KPW; ++a) {
+                    const int k = 16 * (kt0 + a) + r16;
+                    const bool in = z < N && k < N;
+                    am[u][a] = FT[in ? z * N + k : 0];
+                    am[u][a] = in ? am[u][a] : T(0);
+                }
                 const int zr = z < KP ? z : KP - 1;  // rows N .. KP - 1 are zero; beyond, a is zero
 #pragma unroll
                 for (int ct = 0; ct < CT; ++ct) bv[u][ct] = C[zr * S + 16 * ct + r16];
@@ -310,17 +313,16 @@ __global__ __launch_bounds__(NT, 3) void k_ch_zfwd(const cplx<T>* __restrict__ H
 #pragma unroll
             for (int u = 0; u < U; ++u)
 #pragma unroll
-                for (int ct = 0; ct < CT; ++ct) {
-                    acc[0][ct] = Op::mma(am[u][0], bv[u][ct], acc[0][ct]);
-                    acc[1][ct] = Op::mma(am[u][1], bv[u][ct], acc[1][ct]);
-                }
+                for (int a = 0; a < KPW; ++a)
+#pragma unroll
+                    for (int ct = 0; ct < CT; ++ct) acc[a][ct] = Op::mma(am[u][a], bv[u][ct], acc[a][ct]);
         }
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
             const int m = q0 + 16 * g + r16;
             if (m >= nmodes) continue;
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
+            for (int a = 0; a < KPW; ++a)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const int k = 16 * (kt0 + a) + Op::row(v, q4);

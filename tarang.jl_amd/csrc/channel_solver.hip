@@ -255,8 +255,8 @@ void ChannelSolver::rhs_t(const void* b, const void* ux, const void* uz, void* F
         const unsigned ksplit = (unsigned)std::max(1, ((N + 15) / 16) / 8);
         const size_t flds = ch_zfwd_lds_elems(N, QT) * sizeof(T);
         timed(2, [&] {
-            hipLaunchKernelGGL((k_ch_zfwd<T, QT, 256>), dim3((unsigned)((nxh_ + QT - 1) / QT), 3, ksplit), dim3(256),
-                               flds, s, H, nxh_, d_.nx, N, FT_.as<const T>(), static_cast<V*>(F0),
+            hipLaunchKernelGGL((k_ch_zfwd<T, QT, ZfwdTile<T>::NT>), dim3((unsigned)((nxh_ + QT - 1) / QT), 3, ksplit),
+                               dim3(ZfwdTile<T>::NT), flds, s, H, nxh_, d_.nx, N, FT_.as<const T>(), static_cast<V*>(F0),
                                static_cast<V*>(F1), static_cast<V*>(F2));
         });
     }
