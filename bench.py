@@ -365,7 +365,7 @@ CH3_WORKLOAD = ("3-D Rayleigh-Benard, RealFourier^2 x ChebyshevT 256x256x128, Lx
 CH3_KERNELS = ["k3_zbwd (Chebyshev evaluation of 15 operand planes)", "k3_ybwd (padded inverse y FFT, MY=384)",
                "k3_xpass (inverse x FFTs MX=384, 12 products, forward x FFTs)",
                "k3_yfwd (forward y FFT, truncation, Nyquist folds)", "k_ch_zfwd (Chebyshev forward of 4 products)",
-               "k3_sbdf (multistep right-hand side)", "k3_solve (explicit inverses per |k_h|^2 key, 8-mode chunks on FP64 matrix cores)",
+               "k3_sbdf (multistep right-hand side)", "k3_solve (explicit inverses per k_h^2 key, 8-mode chunks on FP64 matrix cores)",
                "k3_lx (structured L x, RK only)"]
 
 
