@@ -201,7 +201,7 @@ __global__ __launch_bounds__(NT) void k3_ybwd(const cplx<T>* __restrict__ G, Ch3
         }
     }
     __syncthreads();
-    fft_lds<T, MY, TK, NT, true>(buf, twl);
+    fft_lds<T, MY, TK, NT, true, sizeof(T) == 8>(buf, twl);  // FP32: radix 8/4/2/3 measured faster here
     const bool xeven = (geo.nx % 2) == 0;
     cplx<T>* out = Q + (size_t)pl * geo.N * MY * nxh + (size_t)z * MY * nxh;
     for (int idx = threadIdx.x; idx < TK * MY; idx += NT) {

@@ -1,6 +1,8 @@
 // fft_lds.hpp -- batched complex FFTs of compile-time length L staged in LDS.
 //
-// Stockham autosort passes (radix 8, 4, 2, 3, 5).  NLINES lines of length L sit
+// Stockham autosort passes (radix 16, 24, 12, 8, 6, 4, 2, 3, 5; the composite radices
+// run as two small DFTs in registers, which halves the LDS round trips of the usual
+// 3/2-padded lengths: 384 = 16 x 24, 1536 = 16 x 16 x 6).  NLINES lines of length L sit
 // in LDS; NT threads share the butterflies of every pass.  Each pass is done in
 // place: every thread first loads all of its butterflies' inputs into
 // registers, the workgroup synchronises, then the twiddled butterflies are
@@ -24,10 +26,16 @@ struct RadixPlan {
     int r[16];
 };
 
-constexpr RadixPlan radix_plan(int L) {
+// composite = false keeps radices 8, 4, 2, 3, 5 (more passes, more butterflies per pass)
+constexpr RadixPlan radix_plan(int L, bool composite = true) {
     RadixPlan p{0, {}};
     int m = L;
-    while (m % 8 == 0 && m != 16) { p.r[p.n++] = 8; m /= 8; }
+    while (composite && m % 16 == 0) { p.r[p.n++] = 16; m /= 16; }
+    if (composite && m % 3 == 0) {  // the remaining power of two (at most 8) joins one factor 3
+        const int r = m % 8 == 0 ? 24 : (m % 4 == 0 ? 12 : (m % 2 == 0 ? 6 : 1));
+        if (r > 1) { p.r[p.n++] = r; m /= r; }
+    }
+    while (m % 8 == 0 && (composite || m != 16)) { p.r[p.n++] = 8; m /= 8; }
     while (m % 4 == 0) { p.r[p.n++] = 4; m /= 4; }
     while (m % 2 == 0) { p.r[p.n++] = 2; m /= 2; }
     while (m % 3 == 0) { p.r[p.n++] = 3; m /= 3; }
@@ -50,6 +58,10 @@ __device__ inline void load_twiddles(cplx<T>* twl, const cplx<T>* __restrict__ t
 }
 
 // ---- small DFT kernels on registers; INV selects exp(+...) ----
+template <int R, bool INV, typename T>
+__device__ inline void dft_small(cplx<T>* v);
+template <int A, int B, bool INV, typename T>
+__device__ inline void dft_composite(cplx<T>* v);
 template <bool INV, typename V>
 __device__ inline V rot_mi(V a) {  // multiply by w4 = exp(-+ 2 pi i/4): -i forward, +i inverse
     return INV ? cmul_i(a) : cmul_mi(a);
@@ -122,6 +134,69 @@ __device__ inline void dft_small(cplx<T>* v) {
         v[4] = csub(m1, in1);
         v[2] = cadd(m2, in2);
         v[3] = csub(m2, in2);
+    } else if constexpr (R == 16) {
+        dft_composite<4, 4, INV, T>(v);
+    } else if constexpr (R == 24) {
+        dft_composite<8, 3, INV, T>(v);
+    } else if constexpr (R == 12) {
+        dft_composite<4, 3, INV, T>(v);
+    } else if constexpr (R == 6) {
+        dft_composite<2, 3, INV, T>(v);
+    } else {
+        static_assert(R == 2 || R == 3 || R == 4 || R == 5 || R == 8, "unsupported radix");
+    }
+}
+
+// cos(j * 7.5 degrees) for any integer j (the internal twiddles of the composite
+// radices 6, 12, 16 and 24 are all multiples of 7.5 degrees)
+__host__ __device__ constexpr double cos_step(int j) {
+    constexpr double c[13] = {1.0,
+                              0.99144486137381041114,  // 7.5
+                              0.96592582628906828675,  // 15
+                              0.92387953251128675613,  // 22.5
+                              0.86602540378443864676,  // 30
+                              0.79335334029123516458,  // 37.5
+                              0.70710678118654752440,  // 45
+                              0.60876142900872063942,  // 52.5
+                              0.5,                     // 60
+                              0.38268343236508977173,  // 67.5
+                              0.25881904510252076235,  // 75
+                              0.13052619222005159155,  // 82.5
+                              0.0};
+    const int r = ((j % 48) + 48) % 48;
+    return r <= 12 ? c[r] : (r <= 24 ? -c[24 - r] : (r <= 36 ? -c[r - 24] : c[48 - r]));
+}
+
+// DFT of length A*B in registers (decimation in time): DFT_A of each of the B
+// decimated subsequences, internal twiddles W_R^(n2 k1), DFT_B across them;
+// X[k1 + A k2] = sum_n2 W_B^(n2 k2) W_R^(n2 k1) sum_n1 x[B n1 + n2] W_A^(n1 k1).
+template <int A, int B, bool INV, typename T>
+__device__ inline void dft_composite(cplx<T>* v) {
+    using V = cplx<T>;
+    constexpr int R = A * B;
+    V y[R];
+#pragma unroll
+    for (int n2 = 0; n2 < B; ++n2) {
+        V t[A];
+#pragma unroll
+        for (int n1 = 0; n1 < A; ++n1) t[n1] = v[B * n1 + n2];
+        dft_small<A, INV, T>(t);
+#pragma unroll
+        for (int k1 = 0; k1 < A; ++k1) {
+            const int j = (48 / R) * ((n2 * k1) % R);  // angle in 7.5-degree steps
+            const T c = T(cos_step(j)), sn = T(cos_step(12 - j));
+            const V w = V{c, INV ? sn : -sn};  // exp(-+ 2 pi i n2 k1 / R)
+            y[n2 * A + k1] = (n2 * k1 == 0) ? t[k1] : cmul(t[k1], w);
+        }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) {
+        V u[B];
+#pragma unroll
+        for (int n2 = 0; n2 < B; ++n2) u[n2] = y[n2 * A + k1];
+        dft_small<B, INV, T>(u);
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2) v[k1 + A * k2] = u[k2];
     }
 }
 
@@ -169,22 +244,25 @@ __device__ inline void stockham_pass(cplx<T>* buf, const cplx<T>* twl) {
     __syncthreads();
 }
 
-template <typename T, int L, int NLINES, int NT, bool INV, int P = 0, int NS = 1>
+// COMPOSITE selects the plan: composite radices halve the LDS round trips but leave
+// fewer butterflies per pass (a caller whose lines are short and few may do better
+// without them).
+template <typename T, int L, int NLINES, int NT, bool INV, bool COMPOSITE, int P = 0, int NS = 1>
 __device__ inline void fft_passes(cplx<T>* buf, const cplx<T>* twl) {
-    constexpr RadixPlan plan = radix_plan(L);
+    constexpr RadixPlan plan = radix_plan(L, COMPOSITE);
     static_assert(plan.n >= 0, "FFT length must factor into 2, 3 and 5");
     if constexpr (P < plan.n) {
         constexpr int R = plan.r[P];
         stockham_pass<T, L, R, NS, NLINES, NT, INV>(buf, twl);
-        fft_passes<T, L, NLINES, NT, INV, P + 1, NS * R>(buf, twl);
+        fft_passes<T, L, NLINES, NT, INV, COMPOSITE, P + 1, NS * R>(buf, twl);
     }
 }
 
 // In-LDS batched FFT of NLINES padded lines; twl is the LDS twiddle table.
 // The caller must have synchronised after filling buf and twl.
-template <typename T, int L, int NLINES, int NT, bool INV>
+template <typename T, int L, int NLINES, int NT, bool INV, bool COMPOSITE = true>
 __device__ inline void fft_lds(cplx<T>* buf, const cplx<T>* twl) {
-    if constexpr (L > 1) fft_passes<T, L, NLINES, NT, INV>(buf, twl);
+    if constexpr (L > 1) fft_passes<T, L, NLINES, NT, INV, COMPOSITE>(buf, twl);
 }
 
 // LDS bytes of NLINES padded lines plus the twiddle table
