@@ -92,6 +92,43 @@ int tarang_backward_transform(tarang_plan_t plan, const void* coeff, void* grid)
 int tarang_padded_multiply(tarang_plan_t plan, const void* grid_a, const void* grid_b, void* grid_out);
 
 /* ------------------------------------------------------------------------ */
+/* Chebyshev grid-space derivative.  Replaces
+ *   Tarang._gpu_chebyshev_deriv!(result, operand, data_g, axis, order, scale)::Bool
+ * (ext/cuda/cheb_deriv.jl:425-465; its single-derivative chain :180-283 and the
+ * CPU restatement chebyshev_derivative_1d!, src/core/operators/derivatives/
+ * derivatives_polynomial.jl:326-421).
+ * data_g / result: device arrays of ndim (1..8) column-major sizes `shape`, real or
+ * (is_complex = 1) interleaved complex, of dtype TARANG_F32 / TARANG_F64.
+ * result = (scale d/dx)^order data_g along `axis` (0-based), the axis holding the
+ * Gauss-Lobatto values at x_j = -cos(pi j/(n-1)); scale = 2/(b-a) for bounds (a, b).
+ * order 0 copies; an axis of n <= 1 points gives zeros; order < 0 is
+ * TARANG_ERR_INVALID; n is limited to 4096.  result may alias data_g.  Runs on
+ * the context stream (asynchronous). */
+int tarang_cheb_deriv(tarang_ctx_t ctx, const void* data_g, void* result, int ndim, const int* shape, int axis,
+                      int order, double scale, int dtype, int is_complex);
+
+/* ------------------------------------------------------------------------ */
+/* Batched dense LU over Fourier modes.  Replaces
+ *   Tarang._batched_factor_impl!(s::BatchedDenseLU{<:DeviceArray})
+ *   Tarang._batched_solve_impl!(X, s::BatchedDenseLU{<:DeviceArray}, B)
+ * (src/tools/batched_matsolvers.jl:78-123; CUDA ext/cuda/batched_matsolvers.jl:80-126).
+ * A: device, nmodes column-major n x n ComplexF64 matrices one after another
+ * (Julia's (n, n, nmodes) array), factored IN PLACE into LAPACK getrf layout
+ * (unit-lower L below the diagonal, U on and above) -- the factorization destroys
+ * A, as the reference's GPU path does (assemble-then-factor, never factor twice).
+ * ipiv: device int32 (n, nmodes), LAPACK's 1-based row interchanges.
+ * info: host int32[nmodes] or NULL; info[m] = 0, or k (1-based) at the first
+ * exactly zero pivot.  Any singular mode makes the call return TARANG_ERR_SINGULAR
+ * with a message naming the modes (1-based) -- never a silent result.
+ * Synchronises the context stream (info is read back). */
+int tarang_batched_lu_factor(tarang_ctx_t ctx, void* A, int n, int nmodes, int* ipiv, int* info);
+/* X[:, m] = A_m^{-1} B[:, m] for every mode from the factors and pivots above;
+ * B and X are device (n, nmodes) ComplexF64, one right-hand side per mode; X may
+ * alias B.  n is limited to 6144.  Asynchronous on the context stream. */
+int tarang_batched_lu_solve(tarang_ctx_t ctx, const void* LU, const int* ipiv, int n, int nmodes, const void* B,
+                            void* X);
+
+/* ------------------------------------------------------------------------ */
 /* IVPs on RealFourier domains with a diagonal implicit operator nu*|k|^2.
  * Replaces InitialValueSolver + step!/run! for these equation families
  * (src/core/solvers/solver_types.jl:468-553, src/core/solvers/solver_stepping.jl:24-38,115-206)

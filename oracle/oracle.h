@@ -127,6 +127,19 @@ double orc_rbc3_time(const orc_rbc3 *s);
 void orc_rbc3_rhs_fields(orc_rbc3 *s, double *Fb, double *Fux, double *Fuy, double *Fuz);
 void orc_rbc3_matrices(const orc_rbc3 *s, int mode, double *M, double *L);
 
+/* ---- Chebyshev grid-space derivative (derivatives_polynomial.jl:326-421):
+ *      `batch` columns of n Gauss-Lobatto values, column-major (n, batch);
+ *      order 0 copies, n <= 1 gives zeros ---- */
+void orc_cheb_deriv(int n, long batch, const double *in, double *out, double scale, int order);
+
+/* ---- batched dense complex LU (batched_matsolvers.jl:78-123 via LAPACK zgetrf/zgetrs):
+ *      A (n, n, nmodes) column-major interleaved complex, factored in place;
+ *      ipiv (n, nmodes) 1-based interchanges; info[m] = 0 or the first zero pivot (1-based) ---- */
+int  orc_zgetrf(int n, double *A, int *ipiv);
+void orc_zgetrs(int n, const double *LU, const int *ipiv, double *b);
+void orc_batched_zgetrf(int n, int nmodes, double *A, int *ipiv, int *info);
+void orc_batched_zgetrs(int n, int nmodes, const double *LU, const int *ipiv, double *X);
+
 #ifdef __cplusplus
 }
 #endif

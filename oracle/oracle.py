@@ -50,6 +50,13 @@ def lib():
         L.orc_rfft.argtypes = [ctypes.c_int, dp, dp]
         L.orc_irfft.argtypes = [ctypes.c_int, dp, dp]
         L.orc_dct1.argtypes = [ctypes.c_int, dp, dp]
+        ip = P(ctypes.c_int)
+        L.orc_cheb_deriv.argtypes = [ctypes.c_int, ctypes.c_long, dp, dp, ctypes.c_double, ctypes.c_int]
+        L.orc_zgetrf.argtypes = [ctypes.c_int, dp, ip]
+        L.orc_zgetrf.restype = ctypes.c_int
+        L.orc_zgetrs.argtypes = [ctypes.c_int, dp, ip, dp]
+        L.orc_batched_zgetrf.argtypes = [ctypes.c_int, ctypes.c_int, dp, ip, ip]
+        L.orc_batched_zgetrs.argtypes = [ctypes.c_int, ctypes.c_int, dp, ip, dp]
         L.orc_coeff_shape.argtypes = [P(Domain), P(ctypes.c_int)]
         L.orc_forward_transform.argtypes = [P(Domain), dp, dp]
         L.orc_backward_transform.argtypes = [P(Domain), dp, dp]
@@ -158,6 +165,49 @@ def dct1(x):
     out = np.zeros_like(x)
     lib().orc_dct1(len(x), _dp(x), _dp(out))
     return out
+
+
+def cheb_deriv(data, axis, order, scale):
+    """Grid-space Chebyshev derivative of `order` along `axis` of a real or complex
+    array (any shape); complex data are differentiated as Re and Im separately
+    (derivatives_polynomial.jl:423-437)."""
+    a = np.asarray(data)
+    if np.iscomplexobj(a):
+        return (cheb_deriv(a.real, axis, order, scale)
+                + 1j * cheb_deriv(a.imag, axis, order, scale))
+    x = np.moveaxis(np.asarray(a, np.float64), axis, 0)
+    n = x.shape[0]
+    cols = np.ascontiguousarray(x.reshape(n, -1).T)          # (batch, n): each column contiguous
+    out = np.zeros_like(cols)
+    lib().orc_cheb_deriv(n, cols.shape[0], _dp(cols), _dp(out), float(scale), int(order))
+    return np.moveaxis(out.T.reshape(x.shape), 0, axis)
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def batched_lu_factor(A):
+    """LU-factor A[:, :, m] (n, n, nmodes complex) for every m.  Returns
+    (LU, ipiv, info): LU in LAPACK getrf layout, ipiv (n, nmodes) 1-based
+    interchanges, info[m] = 0 or the first zero pivot (1-based)."""
+    A = np.asarray(A, np.complex128)
+    n, _, nm = A.shape
+    LU = np.array(A, dtype=np.complex128, order="F")
+    ipiv = np.zeros((n, nm), np.int32, order="F")
+    info = np.zeros(nm, np.int32)
+    lib().orc_batched_zgetrf(n, nm, _dp(LU), _ip(ipiv), _ip(info))
+    return LU, ipiv, info
+
+
+def batched_lu_solve(LU, ipiv, B):
+    """X[:, m] = A_m^{-1} B[:, m] from batched_lu_factor's output."""
+    LU = np.asfortranarray(LU, np.complex128)
+    ipiv = np.asfortranarray(ipiv, np.int32)
+    n, _, nm = LU.shape
+    X = np.array(B, dtype=np.complex128, order="F").reshape(n, nm, order="F")
+    lib().orc_batched_zgetrs(n, nm, _dp(LU), _ip(ipiv), _dp(X))
+    return X
 
 
 def _fortran(a, dtype):

@@ -6,6 +6,7 @@
 #include "../../include/tarang_hip.h"
 #include "channel3d_solver.hpp"
 #include "channel_solver.hpp"
+#include "dense_ops.hpp"
 #include "fourier_solver.hpp"
 #include "runtime.hpp"
 #include "serving.hpp"
@@ -268,6 +269,38 @@ int tarang_padded_multiply(tarang_plan_t plan, const void* a, const void* b, voi
         NEED(b, "grid_b");
         NEED(out, "grid_out");
         plan->plan->padded_multiply(a, b, out);
+    });
+}
+
+int tarang_cheb_deriv(tarang_ctx_t ctx, const void* data_g, void* result, int ndim, const int* shape, int axis,
+                      int order, double scale, int dtype, int is_complex) {
+    return guard([&] {
+        NEED(ctx, "ctx");
+        NEED(shape, "shape");
+        TG_REQUIRE(ndim >= 1 && ndim <= 8, tarang::kErrInvalid, "cheb_deriv: ndim must be 1..8");
+        size_t total = 1;
+        for (int a = 0; a < ndim; ++a) total *= shape[a] > 0 ? (size_t)shape[a] : 0;
+        if (total > 0) {
+            NEED(data_g, "data_g");
+            NEED(result, "result");
+        }
+        tarang::cheb_deriv(&ctx->ctx, data_g, result, ndim, shape, axis, order, scale,
+                           static_cast<tarang::DType>(dtype), is_complex != 0);
+    });
+}
+
+int tarang_batched_lu_factor(tarang_ctx_t ctx, void* A, int n, int nmodes, int* ipiv, int* info) {
+    return guard([&] {
+        NEED(ctx, "ctx");
+        tarang::zgetrf_batched(&ctx->ctx, A, n, nmodes, ipiv, info);
+    });
+}
+
+int tarang_batched_lu_solve(tarang_ctx_t ctx, const void* LU, const int* ipiv, int n, int nmodes, const void* B,
+                            void* X) {
+    return guard([&] {
+        NEED(ctx, "ctx");
+        tarang::zgetrs_batched(&ctx->ctx, LU, ipiv, n, nmodes, B, X);
     });
 }
 

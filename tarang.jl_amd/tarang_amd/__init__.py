@@ -7,6 +7,8 @@ from ._lib import TarangError, LIB_PATH, load as load_library
 from .core import (CPU, GPU, CartesianCoordinates, ChannelDomain, ChannelDomain3D, ChebyshevT, Coordinate, Distributor,
                    Domain, GPUTransformUnsupported, PeriodicDomain, RealFourier, ScalarField, VectorField,
                    backward_transform_, ensure_layout_, forward_transform_, get_grid_coordinates, padded_multiply, set_)
+from .derivatives import evaluate_chebyshev_derivative_, gpu_chebyshev_deriv_
+from .matsolvers import BatchedDenseLU, batched_factor_, batched_solve_, device_array
 from .operators import Expr, derivative_basis, grad, lift, unit_vector_fields
 from .problems import IVP, add_bc_, add_equation_, add_parameters_
 from .solvers import (RK111, RK222, RK443, SBDF1, SBDF2, DiagonalIMEX_RK222, DiagonalIMEX_RK443, DiagonalIMEX_SBDF2,
@@ -19,4 +21,6 @@ __all__ = [
     "IVP", "add_bc_", "add_equation_", "add_parameters_", "RK111", "RK222", "RK443", "SBDF1", "SBDF2",
     "DiagonalIMEX_RK222", "DiagonalIMEX_RK443", "DiagonalIMEX_SBDF2", "InitialValueSolver", "run_", "step_",
     "TarangError", "LIB_PATH", "load_library", "Expr", "derivative_basis", "grad", "lift", "unit_vector_fields",
+    "evaluate_chebyshev_derivative_", "gpu_chebyshev_deriv_", "BatchedDenseLU", "batched_factor_", "batched_solve_",
+    "device_array",
 ]

@@ -20,6 +20,7 @@ class TarangError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"[tarang_hip status {code}] {msg}")
         self.code = code
+        self.msg = msg
 
 
 class DomainDesc(ctypes.Structure):
@@ -125,6 +126,9 @@ SIGNATURES = {
     "tarang_forward_transform": [_P, _P, _P],
     "tarang_backward_transform": [_P, _P, _P],
     "tarang_padded_multiply": [_P, _P, _P, _P],
+    "tarang_cheb_deriv": [_P, _P, _P, _I, _PI, _I, _I, _D, _I, _I],
+    "tarang_batched_lu_factor": [_P, _P, _I, _I, _P, _PI],
+    "tarang_batched_lu_solve": [_P, _P, _P, _I, _I, _P, _P],
     "tarang_fivp_create": [_P, ctypes.POINTER(FivpDesc), _PP],
     "tarang_fivp_destroy": [_P],
     "tarang_fivp_set_state": [_P, _P],
