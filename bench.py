@@ -10,7 +10,13 @@ events, and the FP64 oracle timed on the host cores on a bounded sample.
 (1024x256 RealFourier x ChebyshevT, Ra=2e6, Pr=1, RK222, dt=0.01): one step =
 3 stages, 3 dealiased RHS evaluations, 2 per-mode FP64 tau solves.
 
-Usage: python bench.py [--gpus 1] [--steps 1000] [--warmup 20] [--no-cpu] [--workload turbulence|channel]
+`--workload plugins` measures the stand-alone plug-ins instead, one JSON line
+each: the Chebyshev grid derivative on the 256x256x128 grid (along z and x, FP64
+and FP32 data) and the batched complex LU of the 2-D channel's stage matrices
+(513 modes, n = 1031: factor and solve).
+
+Usage: python bench.py [--gpus 1] [--steps 1000] [--warmup 20] [--no-cpu]
+                       [--workload turbulence|channel|channel3d|serving|plugins]
 """
 import argparse
 import ctypes
@@ -72,7 +78,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run one plain device-to-device copy of this many MiB (counter calibration)")
-    ap.add_argument("--workload", choices=["turbulence", "channel", "channel3d", "serving"], default="turbulence")
+    ap.add_argument("--workload", choices=["turbulence", "channel", "channel3d", "serving", "plugins"],
+                    default="turbulence")
     ap.add_argument("--requests", type=int, default=24, help="serving: number of requests")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -87,6 +94,8 @@ def main():
         return bench_serving(a)
     if a.workload == "channel3d":
         return bench_channel3d(a)
+    if a.workload == "plugins":
+        return bench_plugins(a)
 
     import numpy as np
     import torch
@@ -588,6 +597,140 @@ def bench_serving(a):
         "rejected": st.get("rejected", 0),
     }
     print(json.dumps(line), flush=True)
+
+
+FP64_MFMA_PEAK_TFLOPS = 78.6
+
+
+def bench_plugins(a):
+    """Chebyshev grid derivative and batched LU, timed with HIP events on the
+    library's stream (the context is bound to torch's current stream)."""
+    import numpy as np
+    import torch
+
+    from tarang_amd import _lib
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.Stream(dev)  # a real stream (the library reads NULL as "its own stream")
+    torch.cuda.set_stream(stream)
+    ctx = ctypes.c_void_p()
+    _lib.call("tarang_ctx_create", 0, ctypes.byref(ctx))
+    _lib.call("tarang_ctx_set_stream", ctx, ctypes.c_void_p(stream.cuda_stream))
+    reps = a.steps if a.steps != 1000 else 50
+
+    def timed(fn, n):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(n):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / n
+
+    # ---- Chebyshev grid derivative, 256 x 256 x 128 ----
+    nx, ny, nz = 256, 256, 128
+    shape = (ctypes.c_int * 3)(nx, ny, nz)
+    g = torch.Generator(device=dev).manual_seed(7)
+    for tdt, code, esz in ((torch.float64, _lib.F64, 8), (torch.float32, _lib.F32, 4)):
+        src = torch.rand((nz, ny, nx), dtype=torch.float64, device=dev, generator=g).to(tdt)  # column-major (nx, ny, nz)
+        dst = torch.empty_like(src)
+        for axis, n in ((2, nz), (0, nx)):
+            def call():
+                _lib.call("tarang_cheb_deriv", ctx, ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()),
+                          3, shape, axis, 1, 2.0, code, 0)
+            for _ in range(max(a.warmup, 3)):
+                call()
+            ms = timed(call, reps)
+            pts = nx * ny * nz
+            flops = float(n) * pts  # parity split: two (n/2)^2 products per line of n points
+            tflops = flops / (ms * 1e-3) / 1e12
+            gbs = 2.0 * esz * pts / (ms * 1e-3) / 1e9
+            mfma_bound = flops / (2.0 * esz * pts) > FP64_MFMA_PEAK_TFLOPS * 1e3 / HBM_PEAK_GBS
+            cpu = None
+            if not a.no_cpu:
+                from oracle import oracle as O
+                O.set_threads(min(os.cpu_count() or 1, 16))
+                sample = np.random.default_rng(0).uniform(-1, 1, (nx, 16, nz) if axis == 2 else (nx, 16, nz))
+                t0 = time.perf_counter()
+                O.cheb_deriv(sample, axis, 1, 2.0)
+                el = time.perf_counter() - t0
+                cpu = {"value": sample.size / el, "unit": "grid points/s", "cores": O.get_threads(), "kind": "port",
+                       "sample": f"first derivative of a {nx}x16x{nz} block along axis {axis} (DCT-I chain, FP64), "
+                                 f"{el:.3f} s"}
+            print(json.dumps({
+                "metric": "Chebyshev grid derivative throughput", "value": pts / (ms * 1e-3),
+                "unit": "grid points/s", "n_gpus": 1, "steps": reps, "warmup": max(a.warmup, 3), "ms_per_step": ms,
+                "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                "dtype": "f64" if code == _lib.F64 else "f32 data, f64 arithmetic", "data": "synthetic uniform(0,1)",
+                "config": {"workload": f"tarang_cheb_deriv, order 1, {nx}x{ny}x{nz} along axis {axis} (n={n})"},
+                "roofline": ({"bound": "mfma", "kernel": "k_axis_apply", "achieved": tflops,
+                              "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                              "frac": tflops / FP64_MFMA_PEAK_TFLOPS, "hbm_GBps": gbs, "traffic": None}
+                             if mfma_bound else
+                             {"bound": "hbm", "kernel": "k_axis_apply", "achieved": gbs, "peak": HBM_PEAK_GBS,
+                              "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "fp64_TFLOPs": tflops, "traffic": None}),
+                "cpu_baseline": cpu}), flush=True)
+        del src, dst
+    torch.cuda.empty_cache()
+
+    # ---- batched complex LU, 2-D channel stage matrices: 513 modes, n = 1031 ----
+    n, nm = 1031, 513
+    log(f"batched LU: {nm} modes of n={n} ({nm * n * n * 16 / 1e9:.2f} GB per copy)")
+    A0 = torch.randn((nm, n, n), dtype=torch.complex128, device=dev, generator=g)  # storage of (n, n, nm)
+    A0.diagonal(dim1=1, dim2=2).add_(2.0 * math.sqrt(n))
+    A = torch.empty_like(A0)
+    piv = torch.empty((nm, n), dtype=torch.int32, device=dev)
+    info = np.zeros(nm, np.int32)
+    B = torch.randn((nm, n), dtype=torch.complex128, device=dev, generator=g)
+    X = torch.empty_like(B)
+
+    def factor():
+        _lib.call("tarang_batched_lu_factor", ctx, ctypes.c_void_p(A.data_ptr()), n, nm,
+                  ctypes.c_void_p(piv.data_ptr()), info.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+
+    def solve():
+        _lib.call("tarang_batched_lu_solve", ctx, ctypes.c_void_p(A.data_ptr()), ctypes.c_void_p(piv.data_ptr()),
+                  n, nm, ctypes.c_void_p(B.data_ptr()), ctypes.c_void_p(X.data_ptr()))
+
+    fms = []
+    for _ in range(3):  # assemble (copy) then factor, never factor twice
+        A.copy_(A0)
+        fms.append(timed(factor, 1))
+    log(f"batched LU factor: {fms} ms")
+    for _ in range(3):
+        solve()
+    sms = timed(solve, max(reps // 5, 5))
+    # residual of a few modes: max |A x - b| / (|A| |x|)
+    idx = torch.arange(0, nm, max(nm // 8, 1), device=dev)
+    Ax = torch.matmul(A0[idx].transpose(1, 2).contiguous(), X[idx].unsqueeze(2)).squeeze(2)
+    res = float(((Ax - B[idx]).abs().max() / (A0[idx].abs().max() * X[idx].abs().max())).item())
+    flop_factor = nm * 8.0 / 3.0 * n ** 3
+    cpu = None
+    if not a.no_cpu:
+        from oracle import oracle as O
+        O.set_threads(min(os.cpu_count() or 1, 16))
+        ms_cpu = min(16, nm)
+        Ah = A0[:ms_cpu].cpu().numpy().transpose(2, 1, 0)  # logical (n, n, modes), column-major
+        t0 = time.perf_counter()
+        O.batched_lu_factor(np.asfortranarray(Ah))
+        el = time.perf_counter() - t0
+        cpu = {"value": ms_cpu / el, "unit": "modes factored/s", "cores": O.get_threads(), "kind": "port",
+               "sample": f"{ms_cpu} of the {nm} modes (n={n}), unblocked zgetf2 restatement, {el:.2f} s"}
+    fbest = min(fms)
+    print(json.dumps({
+        "metric": "batched complex LU factor + solve", "value": nm / (fbest * 1e-3), "unit": "modes factored/s",
+        "n_gpus": 1, "steps": 3, "warmup": 0, "ms_per_step": fbest, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c128", "data": "synthetic N(0,1) complex + 2 sqrt(n) I",
+        "config": {"workload": f"tarang_batched_lu_factor/solve, {nm} modes, n={n} (2-D channel 1024x256 stage "
+                               f"matrices' size)"},
+        "factor_ms": fms, "factor_TFLOPs": flop_factor / (fbest * 1e-3) / 1e12,
+        "solve_ms": sms, "solve_GBps": nm * n * n * 16 / (sms * 1e-3) / 1e9, "residual": res,
+        "roofline": {"bound": "hbm", "kernel": "k_zgetrs", "achieved": nm * n * n * 16 / (sms * 1e-3) / 1e9,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": nm * n * n * 16 / (sms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": None},
+        "cpu_baseline": cpu}), flush=True)
+    _lib.call("tarang_ctx_destroy", ctx)
 
 
 if __name__ == "__main__":

@@ -12,136 +12,161 @@
 namespace tarang {
 
 // ===========================================================================
-// Dense operator along one axis.
+// Dense operator along one axis, with the parity split of the Gauss-Lobatto grid.
 //
 // A real array viewed as (I, N, O) -- I the product of the sizes before the axis,
 // O of those after it; complex data are real data with I doubled -- is mapped to
 // out[i, j, o] = sum_k M[j, k] in[i, k, o].  A column is one (i, o) pair; column
 // col = i + I o starts at element (col mod I) + I N (col div I) and steps by I.
-// One workgroup (four wavefronts) computes a JB = 64 JW row by 64 column tile
-// of the output: the input tile is staged in LDS in chunks of 32 k-rows (loaded
-// one chunk ahead into registers), shared by the wavefronts; each wavefront owns
-// 16 JW output rows and reads its operator values from L2 (Mt[k N + j] = M[j][k],
-// transposed so that 16 lanes read 16 consecutive rows).  Loads and stores run
-// along whichever direction is contiguous in memory: k (I < 16, `kfast`) or the
-// column (I >= 16); a kfast tile is staged through LDS again on the way out.
+//
+// M = (scale D)^p is centrosymmetric up to the sign s = (-1)^p on the symmetric
+// grid: M[N-1-j][N-1-k] = s M[j][k].  With h = floor(N/2), H = N - h and, for odd
+// N, the middle point c = h:
+//   e_k = x_k + x_{N-1-k}, o_k = x_k - x_{N-1-k} (k < h),  e_c = x_c, o_c = 0,
+//   P[j][k] = (M[j][k] + M[j][N-1-k]) / 2, Q[j][k] = (M[j][k] - M[j][N-1-k]) / 2,
+//   P[j][c] = M[j][c], Q[j][c] = 0,
+//   u = P e, v = Q o:  y_j = u_j + v_j,  y_{N-1-j} = s (u_j - v_j)  (j < h; y_c = u_c + v_c),
+// two H x H products instead of one N x N: half the matrix-core work.
+//
+// One workgroup (four wavefronts) owns 64 rows j < H (16 per wavefront, so 128
+// output rows with their mirrors) of 64 columns.  The input is staged 16 k at a
+// time: each thread loads rows k and N-1-k (the chunk after next already in
+// flight), forms e and o in registers and stores them to LDS, where all four
+// wavefronts read them; P and Q come from L2 (transposed, Pt[k H + j], so 16
+// lanes read 16 consecutive rows).  Loads and stores run along whichever
+// direction is contiguous in memory: k (I < 16, `kfast`; the output tile is then
+// staged through LDS) or the column (I >= 16).
 // The arithmetic is FP64 for both storage types S: FP32 data are widened on
 // load and rounded once on store, so the result is the FP64 derivative of the
-// FP32 input.  (A spectral derivative amplifies rounding by ~n^2 per order; an
-// FP32 operator and FP32 sums would add that amplification to the rounding of
-// the operator itself.)
+// FP32 input (a spectral derivative amplifies rounding by ~n^2 per order; an
+// FP32 operator and FP32 sums would add that to the rounding of the operator).
 // ===========================================================================
-constexpr int kAxKC = 32;           // k rows per staged chunk
+constexpr int kAxKC = 16;           // k rows per staged chunk
 constexpr int kAxCT = 64;           // output columns per workgroup
+constexpr int kAxJB = 64;           // rows j < H per workgroup
 constexpr int kAxLd = kAxCT + 1;    // LDS row pitch (elements)
 
-template <typename S, int JW>
-__global__ __launch_bounds__(256, 2) void k_axis_apply(const S* __restrict__ in, S* __restrict__ out,
-                                                       const double* __restrict__ Mt, int N, long long I,
-                                                       long long ncols, int nrb, int kfast) {
+template <typename S>
+__global__ __launch_bounds__(256, 3) void k_axis_apply(const S* __restrict__ in, S* __restrict__ out,
+                                                       const double* __restrict__ Pt, const double* __restrict__ Qt,
+                                                       int N, long long I, long long ncols, int nrb, int kfast,
+                                                       double sgn) {
     using T = double;
     using Op = MfmaOp<T>;
     using acc4 = typename Op::acc4;
-    constexpr int JB = 64 * JW;
     constexpr int NL = kAxKC * kAxCT / 256;  // staged elements per thread per chunk
     constexpr int KS = kAxKC / 4;            // matrix-core steps per chunk
+    const int h = N / 2, H = N - h;
     extern __shared__ unsigned char smem_raw[];
-    T* xs = reinterpret_cast<T*>(smem_raw);  // [max(KC, JB)][kAxLd]
+    T* es = reinterpret_cast<T*>(smem_raw);  // [KC][kAxLd] even part (the epilogue: [2 JB][kAxLd])
+    T* os = es + kAxKC * kAxLd;              // [KC][kAxLd] odd part
     __shared__ long long colbase[kAxCT];
     const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
     const int rb = (int)(b % (unsigned)nrb);
     const long long col0 = (long long)(b / (unsigned)nrb) * kAxCT;
-    const int j0 = rb * JB;
+    const int j0 = rb * kAxJB;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6, q4 = lane >> 4, r16 = lane & 15;
     if (t < kAxCT) {
         const long long col = col0 + t;
         colbase[t] = col < ncols ? (col % I) + (col / I) * I * N : -1;
     }
     __syncthreads();
-    int lk[NL], lc[NL];
-#pragma unroll
-    for (int r = 0; r < NL; ++r) {
-        const int e = t + 256 * r;
-        lk[r] = kfast ? e % kAxKC : e / kAxCT;
-        lc[r] = kfast ? e / kAxKC : e % kAxCT;
-    }
-    T xr[NL];
-    T av[JW][KS], an[JW][KS];
+    // staged element r of this thread: row lk(r) of the chunk, column lc(r) of the tile
+    auto lk = [&](int r) { return kfast ? (t + 256 * r) % kAxKC : (t + 256 * r) / kAxCT; };
+    auto lc = [&](int r) { return kfast ? (t + 256 * r) / kAxKC : (t + 256 * r) % kAxCT; };
+    T xe[NL], xo[NL];
+    T ap[KS], aq[KS], np[KS], nq[KS];
     auto load_x = [&](int k0) {
 #pragma unroll
         for (int r = 0; r < NL; ++r) {
-            const int k = k0 + lk[r];
-            const long long base = colbase[lc[r]];
-            xr[r] = (base >= 0 && k < N) ? (T)in[base + (long long)k * I] : T(0);
-        }
-    };
-    auto load_m = [&](int k0, T (&dst)[JW][KS]) {
-#pragma unroll
-        for (int jt = 0; jt < JW; ++jt) {
-            const int j = j0 + 16 * (w * JW + jt) + r16;
-#pragma unroll
-            for (int s = 0; s < KS; ++s) {
-                const int k = k0 + 4 * s + q4;
-                dst[jt][s] = (j < N && k < N) ? Mt[(long long)k * N + j] : T(0);
+            const int k = k0 + lk(r);
+            const long long base = colbase[lc(r)];
+            T top = T(0), bot = T(0);
+            if (base >= 0 && k < H) {
+                top = (T)in[base + (long long)k * I];
+                if (k < h) bot = (T)in[base + (long long)(N - 1 - k) * I];
             }
+            xe[r] = top + bot;
+            xo[r] = k < h ? top - bot : T(0);
         }
     };
-    acc4 acc[JW][4];
+    const int jw = j0 + 16 * w + r16;  // this lane's operator row
+    auto load_m = [&](int k0) {
 #pragma unroll
-    for (int jt = 0; jt < JW; ++jt)
+        for (int s = 0; s < KS; ++s) {
+            const int k = k0 + 4 * s + q4;
+            const bool ok = jw < H && k < H;
+            np[s] = ok ? Pt[(long long)k * H + jw] : T(0);
+            nq[s] = ok ? Qt[(long long)k * H + jw] : T(0);
+        }
+    };
+    acc4 au[4], av[4];
 #pragma unroll
-        for (int ct = 0; ct < 4; ++ct) acc[jt][ct] = acc4{0, 0, 0, 0};
+    for (int ct = 0; ct < 4; ++ct) au[ct] = av[ct] = acc4{0, 0, 0, 0};
     load_x(0);
-    load_m(0, an);
-    for (int k0 = 0; k0 < N; k0 += kAxKC) {
+    load_m(0);
+    for (int k0 = 0; k0 < H; k0 += kAxKC) {
 #pragma unroll
-        for (int r = 0; r < NL; ++r) xs[lk[r] * kAxLd + lc[r]] = xr[r];
+        for (int r = 0; r < NL; ++r) {
+            es[lk(r) * kAxLd + lc(r)] = xe[r];
+            os[lk(r) * kAxLd + lc(r)] = xo[r];
+        }
 #pragma unroll
-        for (int jt = 0; jt < JW; ++jt)
-#pragma unroll
-            for (int s = 0; s < KS; ++s) av[jt][s] = an[jt][s];
+        for (int s = 0; s < KS; ++s) {
+            ap[s] = np[s];
+            aq[s] = nq[s];
+        }
         __syncthreads();
-        if (k0 + kAxKC < N) {  // next chunk in flight during the products
+        if (k0 + kAxKC < H) {  // next chunk in flight during the products
             load_x(k0 + kAxKC);
-            load_m(k0 + kAxKC, an);
+            load_m(k0 + kAxKC);
         }
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
 #pragma unroll
             for (int ct = 0; ct < 4; ++ct) {
-                const T bv = xs[(4 * s + q4) * kAxLd + 16 * ct + r16];
-#pragma unroll
-                for (int jt = 0; jt < JW; ++jt) acc[jt][ct] = Op::mma(av[jt][s], bv, acc[jt][ct]);
+                const T be = es[(4 * s + q4) * kAxLd + 16 * ct + r16];
+                const T bo = os[(4 * s + q4) * kAxLd + 16 * ct + r16];
+                au[ct] = Op::mma(ap[s], be, au[ct]);
+                av[ct] = Op::mma(aq[s], bo, av[ct]);
             }
         }
         __syncthreads();
     }
-    if (kfast) {  // through LDS, then contiguous along j
+    if (kfast) {  // through LDS, then contiguous along j (rows j, then the mirrored rows N-1-j)
 #pragma unroll
-        for (int jt = 0; jt < JW; ++jt)
+        for (int ct = 0; ct < 4; ++ct)
 #pragma unroll
-            for (int ct = 0; ct < 4; ++ct)
-#pragma unroll
-                for (int v = 0; v < 4; ++v)
-                    xs[(16 * (w * JW + jt) + Op::row(v, q4)) * kAxLd + 16 * ct + r16] = acc[jt][ct][v];
+            for (int v = 0; v < 4; ++v) {
+                const int jl = 16 * w + Op::row(v, q4), c = 16 * ct + r16;
+                es[jl * kAxLd + c] = au[ct][v] + av[ct][v];
+                es[(kAxJB + jl) * kAxLd + c] = sgn * (au[ct][v] - av[ct][v]);
+            }
         __syncthreads();
-        for (int e = t; e < JB * kAxCT; e += 256) {
-            const int jl = e % JB, c = e / JB, j = j0 + jl;
+        for (int e = t; e < 2 * kAxJB * kAxCT; e += 256) {
+            const int rr = e % (2 * kAxJB), c = e / (2 * kAxJB);
             const long long base = colbase[c];
-            if (base >= 0 && j < N) out[base + (long long)j * I] = (S)xs[jl * kAxLd + c];
+            if (base < 0) continue;
+            if (rr < kAxJB) {
+                const int j = j0 + rr;
+                if (j < H) out[base + (long long)j * I] = (S)es[rr * kAxLd + c];
+            } else {
+                const int j = j0 + rr - kAxJB;
+                if (j < h) out[base + (long long)(N - 1 - j) * I] = (S)es[rr * kAxLd + c];
+            }
         }
     } else {
 #pragma unroll
-        for (int jt = 0; jt < JW; ++jt)
+        for (int ct = 0; ct < 4; ++ct) {
+            const long long base = colbase[16 * ct + r16];
+            if (base < 0) continue;
 #pragma unroll
-            for (int ct = 0; ct < 4; ++ct) {
-                const long long base = colbase[16 * ct + r16];
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int j = j0 + 16 * (w * JW + jt) + Op::row(v, q4);
-                    if (base >= 0 && j < N) out[base + (long long)j * I] = (S)acc[jt][ct][v];
-                }
+            for (int v = 0; v < 4; ++v) {
+                const int j = j0 + 16 * w + Op::row(v, q4);
+                if (j < H) out[base + (long long)j * I] = (S)(au[ct][v] + av[ct][v]);
+                if (j < h) out[base + (long long)(N - 1 - j) * I] = (S)(sgn * (au[ct][v] - av[ct][v]));
             }
+        }
     }
 }
 
@@ -177,17 +202,18 @@ static std::vector<long double> collocation_d1(int n) {
     return D;
 }
 
-// (scale D)^order, transposed (Mt[k n + j]) and rounded to FP64, cached in the context.
-static const double* cheb_deriv_matrix(Context* c, int n, int order, double scale) {
-    using T = double;
+// The parity-split operator of (scale D)^order: Pt and Qt (each H x H, transposed,
+// Pt[k H + j] = P[j][k]) rounded to FP64 from long double, cached in the context
+// as one table [Pt | Qt].
+static const double* cheb_deriv_tables(Context* c, int n, int order, double scale) {
     uint64_t sbits;
     std::memcpy(&sbits, &scale, sizeof sbits);
-    const std::string key = "cheb_deriv_" + std::to_string(n) + "_" + std::to_string(order) + "_" +
+    const std::string key = "cheb_deriv_pq_" + std::to_string(n) + "_" + std::to_string(order) + "_" +
                             std::to_string(sbits);
     {
         std::lock_guard<std::mutex> lk(c->mu);
         auto it = c->tables.find(key);
-        if (it != c->tables.end()) return static_cast<const T*>(it->second);
+        if (it != c->tables.end()) return static_cast<const double*>(it->second);
     }
     std::vector<long double> S = collocation_d1(n);
     for (auto& v : S) v *= (long double)scale;
@@ -195,29 +221,36 @@ static const double* cheb_deriv_matrix(Context* c, int n, int order, double scal
     for (int p = 2; p <= order; ++p) {  // P <- S P
         for (int i = 0; i < n; ++i)
             for (int j = 0; j < n; ++j) {
-                long double s = 0.0L;
-                for (int k = 0; k < n; ++k) s += S[(size_t)i * n + k] * P[(size_t)k * n + j];
-                tmp[(size_t)i * n + j] = s;
+                long double acc = 0.0L;
+                for (int k = 0; k < n; ++k) acc += S[(size_t)i * n + k] * P[(size_t)k * n + j];
+                tmp[(size_t)i * n + j] = acc;
             }
         P.swap(tmp);
     }
-    std::vector<T> h((size_t)n * n);
-    for (int j = 0; j < n; ++j)
-        for (int k = 0; k < n; ++k) h[(size_t)k * n + j] = (T)P[(size_t)j * n + k];
-    return static_cast<const T*>(c->table(key, h.data(), h.size() * sizeof(T)));
+    const int h = n / 2, H = n - h;
+    std::vector<double> h_pq(2 * (size_t)H * H);
+    for (int j = 0; j < H; ++j)
+        for (int k = 0; k < H; ++k) {
+            const long double m = P[(size_t)j * n + k];
+            const long double mm = k < h ? P[(size_t)j * n + (n - 1 - k)] : 0.0L;
+            h_pq[(size_t)k * H + j] = (double)(k < h ? (m + mm) / 2 : m);
+            h_pq[(size_t)H * H + (size_t)k * H + j] = (double)(k < h ? (m - mm) / 2 : 0.0L);
+        }
+    return static_cast<const double*>(c->table(key, h_pq.data(), h_pq.size() * sizeof(double)));
 }
 
-template <typename S, int JW>
-static void launch_axis_apply(Context* c, const S* in, S* out, const double* Mt, int N, long long I, long long O) {
+template <typename S>
+static void launch_axis_apply(Context* c, const S* in, S* out, const double* PQ, int N, int order, long long I,
+                              long long O) {
     const long long ncols = I * O;
     const int kfast = I < 16 ? 1 : 0;
-    constexpr int JB = 64 * JW;
-    const int nrb = (N + JB - 1) / JB;
+    const int H = N - N / 2;
+    const int nrb = (H + kAxJB - 1) / kAxJB;
     const long long nct = (ncols + kAxCT - 1) / kAxCT;
     TG_REQUIRE(nct * nrb < (1LL << 31), kErrCapacity, "cheb_deriv: array too large for one launch");
-    const size_t lds = (size_t)(kfast ? (JB > kAxKC ? JB : kAxKC) : kAxKC) * kAxLd * sizeof(double);
-    hipLaunchKernelGGL((k_axis_apply<S, JW>), dim3((unsigned)(nct * nrb)), dim3(256), lds, c->stream, in, out, Mt,
-                       N, I, ncols, nrb, kfast);
+    const size_t lds = (size_t)(kfast ? 2 * kAxJB : 2 * kAxKC) * kAxLd * sizeof(double);
+    hipLaunchKernelGGL((k_axis_apply<S>), dim3((unsigned)(nct * nrb)), dim3(256), lds, c->stream, in, out, PQ,
+                       PQ + (size_t)H * H, N, I, ncols, nrb, kfast, (order & 1) ? -1.0 : 1.0);
     TG_HIP(hipGetLastError());
 }
 
@@ -236,22 +269,21 @@ static void cheb_deriv_t(Context* c, const T* in, T* out, int n, long long I, lo
         if (in != out) TG_HIP(hipMemcpyAsync(out, in, total * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
         return;
     }
-    const double* Mt = cheb_deriv_matrix(c, n, order, scale);
+    const double* PQ = cheb_deriv_tables(c, n, order, scale);
     // A workgroup reads every k of its columns before it writes; with more than one
     // row block per column tile other workgroups read what it writes, so an
-    // overlapping output is computed from a copy of the input.
-    const int JB = n <= 64 ? 64 : 128;
+    // overlapping output is then computed from a copy of the input.
+    const int nrb = (n - n / 2 + kAxJB - 1) / kAxJB;
     const uintptr_t a0 = (uintptr_t)in, a1 = a0 + total * sizeof(T);
     const uintptr_t b0 = (uintptr_t)out, b1 = b0 + total * sizeof(T);
     const bool overlap = a0 < b1 && b0 < a1;
     DevBuf copy;
-    if (overlap && (n > JB || in != out)) {
+    if (overlap && (nrb > 1 || in != out)) {
         copy = DevBuf(c, total * sizeof(T));
         TG_HIP(hipMemcpyAsync(copy.ptr, in, total * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
         in = copy.as<T>();
     }
-    if (n <= 64) launch_axis_apply<T, 1>(c, in, out, Mt, n, I, O);
-    else launch_axis_apply<T, 2>(c, in, out, Mt, n, I, O);
+    launch_axis_apply<T>(c, in, out, PQ, n, order, I, O);
     if (copy.ptr) TG_HIP(hipStreamSynchronize(c->stream));  // the copy is freed on return
 }
 
@@ -428,8 +460,30 @@ __device__ inline double2 readlane_c(double2 v, int lane) {
 // runs.  LDS: y[n] complex, ipiv[n], the 32 x 32 triangle.
 constexpr int kLsB = 32;
 
+// y[i] -= sum_{j < nb} a[(j0 + j) n + i] y[j0 + j] for rows i0 <= i < i1: each thread
+// issues its row's loads 16 at a time before the arithmetic (a column-major panel
+// read in contiguous runs across the workgroup, y broadcast from LDS).
 template <int NT>
-__global__ __launch_bounds__(NT) void k_zgetrs(const double2* __restrict__ LUall, const int* __restrict__ ipiv_all,
+__device__ inline void zpanel_update(const double2* __restrict__ a, int n, int j0, int nb, int i0, int i1,
+                                     double2* y) {
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += NT) {
+        const double2* colp = a + (size_t)j0 * n + i;
+        double2 acc{0.0, 0.0};
+#pragma unroll
+        for (int h = 0; h < kLsB; h += 16) {
+            double2 v[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) v[j] = h + j < nb ? colp[(size_t)(h + j) * n] : double2{0.0, 0.0};
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                if (h + j < nb) acc = cadd(acc, cmul(v[j], y[j0 + h + j]));
+        }
+        y[i] = csub(y[i], acc);
+    }
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT, 3) void k_zgetrs(const double2* __restrict__ LUall, const int* __restrict__ ipiv_all,
                                                int n, const double2* B, double2* X) {
     extern __shared__ unsigned char smem_raw[];
     double2* blk = reinterpret_cast<double2*>(smem_raw);  // [kLsB * kLsB]
@@ -469,13 +523,7 @@ __global__ __launch_bounds__(NT) void k_zgetrs(const double2* __restrict__ LUall
             if (i < nb) y[j0 + i] = v;
         }
         __syncthreads();
-        for (int i = j0 + nb + t; i < n; i += NT) {
-            double2 acc{0.0, 0.0};
-            const double2* colp = a + (size_t)j0 * n + i;
-#pragma unroll 8
-            for (int j = 0; j < nb; ++j) acc = cadd(acc, cmul(colp[(size_t)j * n], y[j0 + j]));
-            y[i] = csub(y[i], acc);
-        }
+        zpanel_update<NT>(a, n, j0, nb, j0 + nb, n, y);
         __syncthreads();
     }
     for (int j1 = n; j1 > 0; j1 -= kLsB) {  // U x = y
@@ -497,13 +545,7 @@ __global__ __launch_bounds__(NT) void k_zgetrs(const double2* __restrict__ LUall
             if (i < nb) y[j0 + i] = v;
         }
         __syncthreads();
-        for (int i = t; i < j0; i += NT) {
-            double2 acc{0.0, 0.0};
-            const double2* colp = a + (size_t)j0 * n + i;
-#pragma unroll 8
-            for (int j = 0; j < nb; ++j) acc = cadd(acc, cmul(colp[(size_t)j * n], y[j0 + j]));
-            y[i] = csub(y[i], acc);
-        }
+        zpanel_update<NT>(a, n, j0, nb, 0, j0, y);
         __syncthreads();
     }
     for (int i = t; i < n; i += NT) X[(size_t)m * n + i] = y[i];
