@@ -255,6 +255,24 @@ def channel_ic(np, nx, nz, Lz, seed=42):
     return (Lz - z)[None, :] + 1e-3 * rng.standard_normal((nx, nz)) * (z * (Lz - z))[None, :], z
 
 
+def dt_change_cost(_lib, ctx, h, pre, dt, steady_ms):
+    """Wall time of the two steps right after a dt change, against a steady step: the
+    per-mode systems are refactored (once for RK; for variable-step SBDF2 once for the
+    step with the new ratio and once when the ratio returns to 1) and the step graph is
+    captured again."""
+    out = {}
+    for name, new_dt in (("dt x 0.9", dt * 0.9), ("back to dt", dt)):
+        _lib.call(pre + "set_dt", h, new_dt)
+        _lib.call("tarang_ctx_sync", ctx)
+        ts = [time.perf_counter()]
+        for _ in range(3):
+            _lib.call(pre + "step", h, 1)
+            _lib.call("tarang_ctx_sync", ctx)
+            ts.append(time.perf_counter())
+        out[name] = [round((b - a) * 1e3, 3) for a, b in zip(ts, ts[1:])]
+    return {"ms_of_the_3_steps_after_a_change": out, "steady_ms_per_step": steady_ms}
+
+
 def bench_channel(a):
     import numpy as np
     import torch
@@ -325,6 +343,7 @@ def bench_channel(a):
                                "algorithmic_bytes": nb[i] or None,
                                "GBps": (nb[i] / (avg[i] * 1e-3) / 1e9) if nb[i] and avg[i] else None}
                for i in range(6)}
+    dtc = dt_change_cost(_lib, ctx, solvers["f32"], "tarang_chan_", dt, results["f32"][0])
     for h in solvers.values():
         _lib.call("tarang_chan_destroy", h)
     _lib.call("tarang_ctx_destroy", ctx)
@@ -354,7 +373,7 @@ def bench_channel(a):
         "config": {"workload": CH_WORKLOAD, "grid": [nx, nz], "timestepper": "RK222", "dt": dt, "Ra": Ra, "Pr": Pr,
                    "parallelism": "single GPU"},
         "fp64": {"value": v64, "ms_per_step": results["f64"][0], "setup_s": results["f64"][1]},
-        "setup_s_f32": results["f32"][1],
+        "setup_s_f32": results["f32"][1], "dt_change_f32": dtc,
         "speedup_fp32_vs_fp64": v32 / v64,
         "fp32_vs_fp64_rel_err": {"b_maxnorm": err_b, "uz_maxnorm": err_w,
                                  "nusselt": abs(nus["f32"] - nus["f64"]) / abs(nus["f64"]), "after_steps": steps},
@@ -456,6 +475,7 @@ def bench_channel3d(a):
     kernels = {CH3_KERNELS[i]: {"avg_ms": avg[i], "launches_per_step": cnt[i] / a.profile_steps,
                                 "algorithmic_bytes": nb[i], "GBps": nb[i] / (avg[i] * 1e-3) / 1e9 if avg[i] else None}
                for i in range(8)}
+    dtc = dt_change_cost(_lib, ctx, solvers["f32"], "tarang_chan3_", c["dt"], results["f32"][0])
     for h in solvers.values():
         _lib.call("tarang_chan3_destroy", h)
     _lib.call("tarang_ctx_destroy", ctx)
@@ -487,6 +507,7 @@ def bench_channel3d(a):
         "fp64": {"value": v64, "ms_per_step": results["f64"][0], "setup_s": results["f64"][1],
                  "warmup_s": setups["f64"]},
         "setup_s_f32": results["f32"][1], "warmup_s_f32": setups["f32"], "kh2_keys": keys["f32"],
+        "dt_change_f32": dtc,
         "speedup_fp32_vs_fp64": v32 / v64,
         "fp32_vs_fp64_rel_err": {"b_maxnorm": err_b, "nusselt": abs(nus["f32"] - nus["f64"]) / abs(nus["f64"]),
                                  "after_steps": steps + max(a.warmup, 3)},

@@ -108,6 +108,19 @@ int tarang_cheb_deriv(tarang_ctx_t ctx, const void* data_g, void* result, int nd
                       int order, double scale, int dtype, int is_complex);
 
 /* ------------------------------------------------------------------------ */
+/* CFL frequency.  Replaces the device-side reductions of the adaptive timestep
+ * controller compute_timestep(cfl::CFL) (src/extras/flow_tools/flow_tools_cfl.jl:268-290
+ * for velocities, :155-169 for diffusivities):
+ *   out = max over the npts grid points of  sum_i |comp_i| / dx_i   (ncomp = 1..3 device
+ *   arrays of npts values, dtype TARANG_F32 / TARANG_F64, evaluated in FP64 in that order),
+ * or, with dx = NULL and ncomp = 1, the plain (signed) maximum of comp_0.
+ * NaN propagates as in Julia's `maximum`; npts = 0 gives 0.  The host side of the
+ * controller (safety, max_dt, change limits, hysteresis) stays with the caller.
+ * Synchronises the context stream. */
+int tarang_cfl_frequency(tarang_ctx_t ctx, int ncomp, const void* const* comps, size_t npts, const double* dx,
+                         int dtype, double* out);
+
+/* ------------------------------------------------------------------------ */
 /* Batched dense LU over Fourier modes.  Replaces
  *   Tarang._batched_factor_impl!(s::BatchedDenseLU{<:DeviceArray})
  *   Tarang._batched_solve_impl!(X, s::BatchedDenseLU{<:DeviceArray}, B)

@@ -7,6 +7,7 @@
 #include "channel3d_solver.hpp"
 #include "channel_solver.hpp"
 #include "dense_ops.hpp"
+#include "flow_tools.hpp"
 #include "fourier_solver.hpp"
 #include "runtime.hpp"
 #include "serving.hpp"
@@ -286,6 +287,16 @@ int tarang_cheb_deriv(tarang_ctx_t ctx, const void* data_g, void* result, int nd
         }
         tarang::cheb_deriv(&ctx->ctx, data_g, result, ndim, shape, axis, order, scale,
                            static_cast<tarang::DType>(dtype), is_complex != 0);
+    });
+}
+
+int tarang_cfl_frequency(tarang_ctx_t ctx, int ncomp, const void* const* comps, size_t npts, const double* dx,
+                         int dtype, double* out) {
+    return guard([&] {
+        NEED(ctx, "ctx");
+        NEED(comps, "comps");
+        NEED(out, "out");
+        *out = tarang::cfl_frequency(&ctx->ctx, ncomp, comps, npts, dx, static_cast<tarang::DType>(dtype));
     });
 }
 

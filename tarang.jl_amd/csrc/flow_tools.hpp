@@ -1,0 +1,12 @@
+// flow_tools.hpp -- device reduction behind the adaptive CFL controller.
+#pragma once
+#include "runtime.hpp"
+
+namespace tarang {
+
+// max over npts points of sum_i |comp_i| / dx_i (ncomp 1..3), or, with dx == nullptr
+// and one component, the plain maximum of comp_0.  FP64 arithmetic; NaN propagates;
+// npts == 0 gives 0.  Synchronises the context stream.
+double cfl_frequency(Context* c, int ncomp, const void* const* comps, size_t npts, const double* dx, DType dt);
+
+}  // namespace tarang

@@ -8,6 +8,7 @@ from .core import (CPU, GPU, CartesianCoordinates, ChannelDomain, ChannelDomain3
                    Domain, GPUTransformUnsupported, PeriodicDomain, RealFourier, ScalarField, VectorField,
                    backward_transform_, ensure_layout_, forward_transform_, get_grid_coordinates, padded_multiply, set_)
 from .derivatives import evaluate_chebyshev_derivative_, gpu_chebyshev_deriv_
+from .flow_tools import CFL, add_diffusivity_, add_velocity_, compute_timestep, grid_spacing
 from .matsolvers import BatchedDenseLU, batched_factor_, batched_solve_, device_array
 from .operators import Expr, derivative_basis, grad, lift, unit_vector_fields
 from .problems import IVP, add_bc_, add_equation_, add_parameters_
@@ -22,5 +23,5 @@ __all__ = [
     "DiagonalIMEX_RK222", "DiagonalIMEX_RK443", "DiagonalIMEX_SBDF2", "InitialValueSolver", "run_", "step_",
     "TarangError", "LIB_PATH", "load_library", "Expr", "derivative_basis", "grad", "lift", "unit_vector_fields",
     "evaluate_chebyshev_derivative_", "gpu_chebyshev_deriv_", "BatchedDenseLU", "batched_factor_", "batched_solve_",
-    "device_array",
+    "device_array", "CFL", "add_diffusivity_", "add_velocity_", "compute_timestep", "grid_spacing",
 ]

@@ -128,6 +128,7 @@ SIGNATURES = {
     "tarang_padded_multiply": [_P, _P, _P, _P],
     "tarang_cheb_deriv": [_P, _P, _P, _I, _PI, _I, _I, _D, _I, _I],
     "tarang_batched_lu_factor": [_P, _P, _I, _I, _P, _PI],
+    "tarang_cfl_frequency": [_P, _I, _PP, ctypes.c_size_t, _PD, _I, _PD],
     "tarang_batched_lu_solve": [_P, _P, _P, _I, _I, _P, _P],
     "tarang_fivp_create": [_P, ctypes.POINTER(FivpDesc), _PP],
     "tarang_fivp_destroy": [_P],

@@ -305,10 +305,12 @@ def step_(solver: InitialValueSolver, dt: Optional[float] = None) -> InitialValu
 
 def run_(solver: InitialValueSolver, stop_time: Optional[float] = None, stop_iteration: Optional[int] = None,
          stop_wall_time: Optional[float] = None, callbacks: Sequence[Tuple[int, Callable]] = (),
-         log_interval: Optional[int] = None) -> InitialValueSolver:
-    """run!(solver; stop_time, stop_iteration, callbacks) (solver_stepping.jl:115-206).
+         log_interval: Optional[int] = None, cfl=None) -> InitialValueSolver:
+    """run!(solver; stop_time, stop_iteration, callbacks, cfl) (solver_stepping.jl:115-206).
 
-    Steps between callback/log points are issued as one library call each.
+    Steps between callback/log points are issued as one library call each.  With a
+    CFL controller every step first takes dt = compute_timestep(cfl)
+    (solver_stepping.jl:155-159); a changed dt refactors the implicit systems.
     """
     if stop_time is not None:
         solver.stop_sim_time = float(stop_time)
@@ -321,6 +323,17 @@ def run_(solver: InitialValueSolver, stop_time: Optional[float] = None, stop_ite
         intervals.append(int(log_interval))
     t0 = time.time()
     it0 = solver.iteration
+    while solver.proceed and cfl is not None:
+        from .flow_tools import compute_timestep
+        dt = compute_timestep(cfl)
+        if dt != solver.dt:
+            solver.set_dt(dt)
+        solver.step_many(1)
+        for k, fn in callbacks:
+            if solver.iteration % int(k) == 0:
+                fn(solver)
+        if log_interval and solver.iteration % int(log_interval) == 0:
+            print(f"Iteration {solver.iteration}, t = {solver.sim_time:.6g}, dt = {solver.dt:.6g}")
     while solver.proceed:
         n_left = None
         if solver.stop_iteration is not None:

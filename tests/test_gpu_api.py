@@ -93,13 +93,9 @@ def test_unsupported_refuses_loudly():
         T.InitialValueSolver(problem, T.RK222(), dt=0.01)
 
 
-@pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_rbc_example_through_api(oracle, dtype):
-    """test/test_subproblem_rk.jl:19-111 through the API mirror (Nx=16, Nz=8, Ra=1e4, RK222,
-    dt=0.01 to t=0.05); the reference draws its noise with Julia's RNG, here a seeded
-    numpy draw of the same form feeds both this solver and the oracle."""
-    Lx, Lz, Nx, Nz, Ra, Pr = 4.0, 1.0, 16, 8, 1e4, 1.0
-    kappa, nu = (Ra * Pr) ** -0.5, (Ra / Pr) ** -0.5
+def rbc_problem(dtype, Lx, Lz, Nx, Nz, kappa, nu, dt=0.01, stepper=None):
+    """The first-order tau RBC problem of test/test_subproblem_rk.jl:27-56 through the
+    API mirror; returns (solver, b, u, xbasis, zbasis)."""
     coords = T.CartesianCoordinates("x", "z")
     dist = T.Distributor(coords, dtype=dtype, device=T.GPU())
     xbasis = T.RealFourier(coords["x"], size=Nx, bounds=(0.0, Lx), dealias=3 / 2)
@@ -123,7 +119,18 @@ def test_rbc_example_through_api(oracle, dtype):
     T.add_equation_(problem, "∂t(u) - nu*div(grad_u) + ∇(p) - b*ez + τ_lift(tau_u2) = -u⋅∇(u)")
     for bc in ("b(z=0) = Lz", "u(z=0) = 0", "b(z=Lz) = 0", "u(z=Lz) = 0", "integ(p) = 0"):
         T.add_bc_(problem, bc)
-    solver = T.InitialValueSolver(problem, T.RK222(), dt=0.01)
+    solver = T.InitialValueSolver(problem, stepper or T.RK222(), dt=dt)
+    return solver, b, u, xbasis, zbasis
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_rbc_example_through_api(oracle, dtype):
+    """test/test_subproblem_rk.jl:19-111 through the API mirror (Nx=16, Nz=8, Ra=1e4, RK222,
+    dt=0.01 to t=0.05); the reference draws its noise with Julia's RNG, here a seeded
+    numpy draw of the same form feeds both this solver and the oracle."""
+    Lx, Lz, Nx, Nz, Ra, Pr = 4.0, 1.0, 16, 8, 1e4, 1.0
+    kappa, nu = (Ra * Pr) ** -0.5, (Ra / Pr) ** -0.5
+    solver, b, u, xbasis, zbasis = rbc_problem(dtype, Lx, Lz, Nx, Nz, kappa, nu, dt=0.01)
     x, z = xbasis.grid(), zbasis.grid()
     rng = np.random.default_rng(42)
     bg = 1e-3 * rng.standard_normal((Nx, Nz)) * (z * (Lz - z))[None, :] + (Lz - z)[None, :]

@@ -1,0 +1,150 @@
+"""Adaptive CFL timestep on the device.
+
+- tarang_cfl_frequency against numpy evaluating the same FP64 expression in the
+  same order (bit-exact), FP64 and FP32 data, 1-3 components, the plain maximum,
+  NaN propagation;
+- the reference's CFL tests (test/test_cfl.jl:4-58) on GPU domains, rtol 1e-6 as there;
+- a 2-D Rayleigh-Benard run under the controller (threshold 0, so dt changes and the
+  per-mode systems are refactored every step) against the oracle stepped with the same
+  dt sequence and the same grid round trip of u between steps, 1e-12 relative (FP64),
+  and that sequence against the controller applied to the oracle's own state (1e-10).
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from tests.conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def frequency(gpu, arrays, dx, code):
+    from tarang_amd import _lib
+    ptrs = (ctypes.c_void_p * len(arrays))(*[a.data_ptr() for a in arrays])
+    out = ctypes.c_double()
+    dxa = (ctypes.c_double * len(arrays))(*dx) if dx is not None else None
+    torch.cuda.synchronize()
+    _lib.call("tarang_cfl_frequency", gpu.ctx, len(arrays), ptrs, arrays[0].numel(), dxa, code, ctypes.byref(out))
+    return out.value
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("ncomp,n", [(1, 1), (1, 1000), (2, 65536 + 7), (3, 8 * 1024 * 1024 + 3)])
+def test_cfl_frequency_bit_exact(gpu, dtype, ncomp, n):
+    from tarang_amd import _lib
+    print(f"cfl frequency ncomp={ncomp} n={n} {dtype.__name__}", flush=True)
+    r = np.random.default_rng(n + ncomp)
+    us = [r.standard_normal(n).astype(dtype) for _ in range(ncomp)]
+    dx = [0.25, 0.0123, 1.7][:ncomp]
+    f = np.abs(us[0].astype(np.float64)) / dx[0]
+    for u, d in zip(us[1:], dx[1:]):
+        f = f + np.abs(u.astype(np.float64)) / d
+    code = _lib.F64 if dtype == np.float64 else _lib.F32
+    ts = [torch.as_tensor(u, device=gpu.torch_device) for u in us]
+    assert frequency(gpu, ts, dx, code) == f.max()
+    assert frequency(gpu, ts[:1], None, code) == float(us[0].max())  # plain (signed) maximum
+
+
+def test_cfl_frequency_nan_and_empty(gpu):
+    from tarang_amd import _lib
+    print("cfl frequency NaN / empty", flush=True)
+    u = torch.zeros(5000, dtype=torch.float64, device=gpu.torch_device)
+    u[4321] = float("nan")
+    assert math.isnan(frequency(gpu, [u], [1.0], _lib.F64))
+    e = torch.zeros(0, dtype=torch.float64, device=gpu.torch_device)
+    ptrs = (ctypes.c_void_p * 1)(e.data_ptr())
+    out = ctypes.c_double(-1.0)
+    dx = (ctypes.c_double * 1)(1.0)
+    _lib.call("tarang_cfl_frequency", gpu.ctx, 1, ptrs, 0, dx, _lib.F64, ctypes.byref(out))
+    assert out.value == 0.0
+
+
+def test_reference_cfl_cases(gpu):
+    """test/test_cfl.jl:4-58: constant velocity on 1-D and 2-D Fourier domains."""
+    import tarang_amd as T
+    print("reference CFL cases", flush=True)
+    for N in (16, 32):
+        for L in (1.0, 2.0):
+            for safety in (0.2, 0.4):
+                for vel in (0.5, 3.0):
+                    dom = T.PeriodicDomain(N, L=(L,), arch=T.GPU())
+                    u = T.VectorField(dom, "u")
+                    u[0]["g"] = np.full(N, vel)
+                    cfl = T.CFL(None, initial_dt=1.0, safety=safety, cadence=1)
+                    T.add_velocity_(cfl, u)
+                    dt = T.compute_timestep(cfl)
+                    assert dt == pytest.approx(safety * T.grid_spacing(dom)[0] / abs(vel), rel=1e-6)
+    dom = T.PeriodicDomain(16, 16, L=(1.0, 1.0), arch=T.GPU())
+    u = T.VectorField(dom, "u")
+    u[0]["g"] = np.ones((16, 16))
+    u[1]["g"] = np.ones((16, 16))
+    cfl = T.CFL(None, initial_dt=1.0, safety=0.5, cadence=1)
+    T.add_velocity_(cfl, u)
+    dx, dy = T.grid_spacing(dom)
+    assert T.compute_timestep(cfl) == pytest.approx(0.5 / (1 / dx + 1 / dy), rel=1e-6)
+
+
+def cellular_state(Nx, Nz, Lx, Lz, A):
+    """u = (-dpsi/dz, dpsi/dx) for psi = A sin(2 pi x/Lx) (z (Lz - z))^2: divergence-free,
+    zero on both walls."""
+    x = Lx * np.arange(Nx) / Nx
+    z = Lz * (1 - np.cos(np.pi * np.arange(Nz) / (Nz - 1))) / 2
+    k = 2 * np.pi / Lx
+    g = (z * (Lz - z)) ** 2
+    dg = 2 * z * (Lz - z) * (Lz - 2 * z)
+    ux = -A * np.sin(k * x)[:, None] * dg[None, :]
+    uz = A * k * np.cos(k * x)[:, None] * g[None, :]
+    return x, z, ux, uz
+
+
+def test_rbc_adaptive_dt_matches_oracle(gpu, oracle):
+    import tarang_amd as T
+    from tests.test_gpu_api import rbc_problem
+    print("RBC under the CFL controller", flush=True)
+    Lx, Lz, Nx, Nz = 4.0, 1.0, 16, 12
+    kappa, nu = (1e5) ** -0.5, (1e5) ** -0.5
+    solver, b, u, xb, zb = rbc_problem(np.float64, Lx, Lz, Nx, Nz, kappa, nu, dt=0.01)
+    x, z, ux, uz = cellular_state(Nx, Nz, Lx, Lz, 20.0)
+    rng = np.random.default_rng(7)
+    bg = (Lz - z)[None, :] + 1e-2 * rng.standard_normal((Nx, Nz)) * (z * (Lz - z))[None, :]
+    b["g"] = bg
+    u[0]["g"] = ux
+    u[1]["g"] = uz
+    cfl = T.CFL(solver, initial_dt=0.01, safety=0.5, threshold=0.0, max_change=1.5, min_change=0.5)
+    T.add_velocity_(cfl, u)
+    dts = []
+    T.run_(solver, stop_iteration=8, cfl=cfl, callbacks=[(1, lambda s: dts.append(s.dt))])
+    assert solver.iteration == 8 and len(dts) == 8
+    assert len(set(dts)) > 4  # dt moved, so the systems were refactored
+    ref = oracle.RBC2D(Nx, Nz, Lx, Lz, kappa, nu, 0.01, "RK222")
+    d = oracle.domain([oracle.REALFOURIER, oracle.CHEBYSHEVT], [Nx, Nz], bounds=[(0.0, Lx), (0.0, Lz)])
+    ref.set(oracle.RBC_B, oracle.forward_transform(d, bg))
+    ref.set(oracle.RBC_UX, oracle.forward_transform(d, ux))
+    ref.set(oracle.RBC_UZ, oracle.forward_transform(d, uz))
+    dx = T.grid_spacing(b.domain)
+    ref_dts, cur = [], 0.01
+    for it, dt in enumerate(dts):
+        # the controller on the oracle's state (flow_tools_cfl.jl:240-355 with threshold 0)
+        gx = oracle.backward_transform(d, ref.get(oracle.RBC_UX))
+        gz = oracle.backward_transform(d, ref.get(oracle.RBC_UZ))
+        prop = 0.5 / (np.abs(gx) / dx[0] + np.abs(gz) / dx[1]).max()
+        if it > 0 and cur != 0.01:
+            prop = min(max(prop, cur * 0.5), cur * 1.5)
+        cur = prop
+        ref_dts.append(prop)
+        if it > 0:
+            # the controller leaves u in grid layout, so the next step gathers it back
+            # through a forward transform (ensure_layout! :g then :c), which drops what the
+            # grid cannot hold (the imaginary part of the Nyquist column)
+            ref.set(oracle.RBC_UX, oracle.forward_transform(d, gx))
+            ref.set(oracle.RBC_UZ, oracle.forward_transform(d, gz))
+        ref.set_dt(dt)
+        ref.step(1)
+    assert relerr(np.array(dts), np.array(ref_dts)) < 1e-10
+    assert relerr(b["c"].cpu().numpy(), ref.get(oracle.RBC_B)) < 1e-12
+    assert relerr(u[0]["c"].cpu().numpy(), ref.get(oracle.RBC_UX)) < 1e-11
+    assert relerr(u[1]["c"].cpu().numpy(), ref.get(oracle.RBC_UZ)) < 1e-11
+    solver.close()
