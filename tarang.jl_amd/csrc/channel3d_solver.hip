@@ -252,11 +252,11 @@ void Channel3Solver::factor(double c) {
     TG_HIP(hipGetLastError());
     int* info = info_.as<int>();
     constexpr int NT = 256;
-    hipLaunchKernelGGL((k_lu_factor<NT>), dim3(nsys), dim3(NT), (size_t)nb_max_ * 24, s, Lb.as<double>(),
+    hipLaunchKernelGGL((k_lu_factor_blocked<NT>), dim3(nsys), dim3(NT), (size_t)nb_max_ * 8, s, Lb.as<double>(),
                        pb.as<int>(), info, N, 0, sb, (size_t)nb_max_);
-    hipLaunchKernelGGL((k_lu_factor<NT>), dim3(nsys), dim3(NT), (size_t)nb_max_ * 24, s, Lp.as<double>(),
+    hipLaunchKernelGGL((k_lu_factor_blocked<NT>), dim3(nsys), dim3(NT), (size_t)nb_max_ * 8, s, Lp.as<double>(),
                        pp.as<int>(), info + nsys, N, 0, sb, (size_t)nb_max_);
-    hipLaunchKernelGGL((k_lu_factor<NT>), dim3(nsys), dim3(NT), (size_t)nu_max_ * 24, s, Lu.as<double>(),
+    hipLaunchKernelGGL((k_lu_factor_blocked<NT>), dim3(nsys), dim3(NT), (size_t)nu_max_ * 8, s, Lu.as<double>(),
                        pu.as<int>(), info + 2 * nsys, N, 1, su, (size_t)nu_max_);
     TG_HIP(hipGetLastError());
     auto invert = [&](auto ot) {

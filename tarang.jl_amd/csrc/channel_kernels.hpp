@@ -707,6 +707,135 @@ __global__ __launch_bounds__(NT) void k_lu_factor(double* A, int* perm, int* inf
     for (int i = threadIdx.x; i < n; i += NT) perm[(size_t)w * pstride + i] = pm[i];
 }
 
+// Blocked form of k_lu_factor (same pivot sequence, same outputs: factors in place,
+// gather map perm, info): right-looking with 16-column panels.  A panel is factored
+// column by column (pivot = first row of largest magnitude, interchange inside the
+// panel, scale, rank-1 update of the panel); then the panel's interchanges go to the
+// other columns, U12 = L11^{-1} A12 is formed 128 columns at a time in LDS and the
+// trailing matrix takes one rank-16 update per panel -- the trailing matrix is read
+// and written once per 16 columns instead of once per column.
+// Dynamic LDS: 2 n ints (swap sequence, gather map).
+template <int NT>
+__global__ __launch_bounds__(NT) void k_lu_factor_blocked(double* A, int* perm, int* info, int N, int kind,
+                                                          size_t stride, size_t pstride) {
+    constexpr int NW = NT / 64, NB = 16, CC = 128;
+    extern __shared__ unsigned char smem_raw[];
+    __shared__ double red_v[NW];
+    __shared__ int red_i[NW];
+    __shared__ double l11[NB * NB];
+    __shared__ double u12[NB * CC];
+    const int w = blockIdx.x, m = w >> 1, g = w & 1;
+    const int n = kind == 0 ? chb_size(N, g) : chu_size(N, g, m);
+    int* sw = reinterpret_cast<int*>(smem_raw);  // [n] pivot row of each step
+    int* pm = sw + n;                            // [n] gather map
+    double* a = A + (size_t)w * stride;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    int bad = 0;
+    for (int k0 = 0; k0 < n; k0 += NB) {
+        const int kb = min(NB, n - k0);
+        for (int kk = 0; kk < kb; ++kk) {
+            const int k = k0 + kk;
+            double best = -1.0;
+            int bi = n;
+            for (int i = k + t; i < n; i += NT) {
+                const double v = fabs(a[(size_t)k * n + i]);
+                if (v > best) { best = v; bi = i; }
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ov = __shfl_xor(best, off);
+                const int oi = __shfl_xor(bi, off);
+                if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+            }
+            if (lane == 0) { red_v[wv] = best; red_i[wv] = bi; }
+            __syncthreads();
+            double pv = red_v[0];
+            int p = red_i[0];
+#pragma unroll
+            for (int q = 1; q < NW; ++q)
+                if (red_v[q] > pv || (red_v[q] == pv && red_i[q] < p)) { pv = red_v[q]; p = red_i[q]; }
+            if (p >= n) p = k;  // no comparable value (NaN column): keep the diagonal
+            const bool zero = pv == 0.0;
+            if (t == 0) sw[k] = zero ? k : p;
+            if (zero) { if (!bad) bad = k + 1; }
+            else if (p != k && t < kb) {
+                double* col = a + (size_t)(k0 + t) * n;
+                const double x = col[k];
+                col[k] = col[p];
+                col[p] = x;
+            }
+            __syncthreads();
+            if (!zero) {
+                const double r = 1.0 / a[(size_t)k * n + k];
+                for (int i = k + 1 + t; i < n; i += NT) {
+                    const double l = a[(size_t)k * n + i] * r;
+                    a[(size_t)k * n + i] = l;
+                    for (int c = kk + 1; c < kb; ++c) a[(size_t)(k0 + c) * n + i] -= l * a[(size_t)(k0 + c) * n + k];
+                }
+            }
+            __syncthreads();
+        }
+        for (int c = t; c < n; c += NT) {  // the panel's interchanges on every other column, in order
+            if (c >= k0 && c < k0 + kb) continue;
+            double* col = a + (size_t)c * n;
+            for (int q = 0; q < kb; ++q) {
+                const int p = sw[k0 + q];
+                if (p != k0 + q) {
+                    const double x = col[k0 + q];
+                    col[k0 + q] = col[p];
+                    col[p] = x;
+                }
+            }
+        }
+        for (int e = t; e < NB * NB; e += NT) {
+            const int c = e / NB, r = e % NB;
+            l11[e] = (c < kb && r < kb) ? a[(size_t)(k0 + c) * n + k0 + r] : 0.0;
+        }
+        __syncthreads();
+        const int c1 = k0 + kb;
+        for (int cc0 = c1; cc0 < n; cc0 += CC) {
+            const int cw = min(CC, n - cc0);
+            if (t < cw) {  // U12 column: forward substitution with the unit lower L11, in LDS
+                double* col = a + (size_t)(cc0 + t) * n + k0;
+                for (int q = 0; q < NB; ++q) u12[q * CC + t] = q < kb ? col[q] : 0.0;
+                for (int r = 1; r < kb; ++r) {
+                    double acc = u12[r * CC + t];
+                    for (int q = 0; q < r; ++q) acc -= l11[q * NB + r] * u12[q * CC + t];
+                    u12[r * CC + t] = acc;
+                    col[r] = acc;
+                }
+            }
+            __syncthreads();
+            for (int i = c1 + t; i < n; i += NT) {  // A22 -= L21 U12
+                double l[NB];
+#pragma unroll
+                for (int q = 0; q < NB; ++q) l[q] = q < kb ? a[(size_t)(k0 + q) * n + i] : 0.0;
+#pragma unroll 2
+                for (int c = 0; c < cw; ++c) {
+                    double* e = a + (size_t)(cc0 + c) * n + i;
+                    double acc = *e;
+#pragma unroll
+                    for (int q = 0; q < NB; ++q) acc -= l[q] * u12[q * CC + c];
+                    *e = acc;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (t == 0) {  // swap sequence -> gather map
+        for (int i = 0; i < n; ++i) pm[i] = i;
+        for (int k = 0; k < n; ++k) {
+            const int q = sw[k];
+            const int x = pm[k];
+            pm[k] = pm[q];
+            pm[q] = x;
+        }
+        info[w] = bad;
+    }
+    __syncthreads();
+    for (int i = t; i < n; i += NT) perm[(size_t)w * pstride + i] = pm[i];
+}
+
 // value of v in lane `lane` (wave-uniform index), through scalar registers
 __device__ inline double readlane_f64(double v, int lane) {
     const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
@@ -715,13 +844,16 @@ __device__ inline double readlane_f64(double v, int lane) {
 }
 
 // Solve with LU factors for NR real right-hand sides held in LDS (y[r*ld + i]).
-// Blocked by 64 columns: the diagonal triangle of each block runs in wavefront 0
-// (lane i owns row i, values broadcast with readlane), the rows outside the
-// block are updated by all threads as a coalesced column-panel product.
+// Blocked by 64 columns: the diagonal triangle of each block runs in every
+// wavefront for its share of the right-hand sides (lane i owns row i, values
+// broadcast with readlane), the rows outside the block are updated by all threads
+// as a coalesced column-panel product.
 // tmp: NR*ld doubles of LDS scratch (may alias blk); blk: 64*64 doubles of LDS.
 template <int NT, int NR>
 __device__ void lu_solve_blocked(const double* __restrict__ a, const int* __restrict__ perm, int n, double* y, int ld,
                                  double* tmp, double* blk) {
+    static_assert(NR % (NT / 64) == 0, "right-hand sides split evenly over the wavefronts");
+    constexpr int RW = NR / (NT / 64);
     for (int i = threadIdx.x; i < n; i += NT) {
         const int q = perm[i];
 #pragma unroll
@@ -740,19 +872,19 @@ __device__ void lu_solve_blocked(const double* __restrict__ a, const int* __rest
             if (rr < nb) blk[c * 64 + rr] = a[(size_t)(j0 + c) * n + j0 + rr];
         }
         __syncthreads();
-        if (threadIdx.x < 64) {
-            const int i = threadIdx.x;
-            double v[NR];
+        {  // every wavefront takes RW of the right-hand sides
+            const int i = threadIdx.x & 63, r0 = (threadIdx.x >> 6) * RW;
+            double v[RW];
 #pragma unroll
-            for (int r = 0; r < NR; ++r) v[r] = i < nb ? y[r * ld + j0 + i] : 0.0;
+            for (int r = 0; r < RW; ++r) v[r] = i < nb ? y[(r0 + r) * ld + j0 + i] : 0.0;
             for (int j = 0; j < nb - 1; ++j) {
                 const double l = (i > j && i < nb) ? blk[j * 64 + i] : 0.0;
 #pragma unroll
-                for (int r = 0; r < NR; ++r) v[r] -= l * readlane_f64(v[r], j);
+                for (int r = 0; r < RW; ++r) v[r] -= l * readlane_f64(v[r], j);
             }
             if (i < nb)
 #pragma unroll
-                for (int r = 0; r < NR; ++r) y[r * ld + j0 + i] = v[r];
+                for (int r = 0; r < RW; ++r) y[(r0 + r) * ld + j0 + i] = v[r];
         }
         __syncthreads();
         for (int i = j0 + nb + threadIdx.x; i < n; i += NT) {
@@ -760,7 +892,7 @@ __device__ void lu_solve_blocked(const double* __restrict__ a, const int* __rest
 #pragma unroll
             for (int r = 0; r < NR; ++r) acc[r] = 0.0;
             const double* colp = a + (size_t)j0 * n + i;
-#pragma unroll 16
+#pragma unroll 4
             for (int j = 0; j < nb; ++j) {
                 const double l = colp[(size_t)j * n];
 #pragma unroll
@@ -779,24 +911,24 @@ __device__ void lu_solve_blocked(const double* __restrict__ a, const int* __rest
             if (rr < nb) blk[c * 64 + rr] = a[(size_t)(j0 + c) * n + j0 + rr];
         }
         __syncthreads();
-        if (threadIdx.x < 64) {
-            const int i = threadIdx.x;
-            double v[NR];
+        {
+            const int i = threadIdx.x & 63, r0 = (threadIdx.x >> 6) * RW;
+            double v[RW];
 #pragma unroll
-            for (int r = 0; r < NR; ++r) v[r] = i < nb ? y[r * ld + j0 + i] : 0.0;
+            for (int r = 0; r < RW; ++r) v[r] = i < nb ? y[(r0 + r) * ld + j0 + i] : 0.0;
             const double rd = i < nb ? 1.0 / blk[i * 64 + i] : 0.0;  // reciprocals off the dependency chain
             for (int j = nb - 1; j >= 0; --j) {
                 const double u = i < j ? blk[j * 64 + i] : 0.0;
                 const double dj = readlane_f64(rd, j);
 #pragma unroll
-                for (int r = 0; r < NR; ++r) {
+                for (int r = 0; r < RW; ++r) {
                     const double xj = readlane_f64(v[r], j) * dj;
                     v[r] = i == j ? xj : v[r] - u * xj;
                 }
             }
             if (i < nb)
 #pragma unroll
-                for (int r = 0; r < NR; ++r) y[r * ld + j0 + i] = v[r];
+                for (int r = 0; r < RW; ++r) y[(r0 + r) * ld + j0 + i] = v[r];
         }
         __syncthreads();
         for (int i = threadIdx.x; i < j0; i += NT) {
@@ -804,7 +936,7 @@ __device__ void lu_solve_blocked(const double* __restrict__ a, const int* __rest
 #pragma unroll
             for (int r = 0; r < NR; ++r) acc[r] = 0.0;
             const double* colp = a + (size_t)j0 * n + i;
-#pragma unroll 16
+#pragma unroll 4
             for (int j = 0; j < nb; ++j) {
                 const double u = colp[(size_t)j * n];
 #pragma unroll
@@ -826,9 +958,9 @@ __global__ __launch_bounds__(NT) void k_lu_inverse(const double* __restrict__ LU
                                                    OT* __restrict__ inv, int N, int kind, size_t stride,
                                                    size_t pstride, size_t istride, int ld) {
     extern __shared__ unsigned char smem_raw[];
-    const int w = blockIdx.x, m = w >> 1, g = w & 1;
+    const int w = blockIdx.y, m = w >> 1, g = w & 1;  // the column blocks of one system run side by side
     const int n = kind == 0 ? chb_size(N, g) : chu_size(N, g, m);
-    const int c0 = NR * (int)blockIdx.y;
+    const int c0 = NR * (int)blockIdx.x;
     if (c0 >= n) return;  // whole workgroup: no barrier reached
     double* y = reinterpret_cast<double*>(smem_raw);  // [NR][ld]
     double* tmp = y + NR * (size_t)ld;                 // [max(NR ld, 4096)], also the triangle block
@@ -848,19 +980,14 @@ __host__ __device__ inline size_t lu_inverse_lds_doubles(int ld, int nr) {
 }
 
 // Form the inverses of nsys systems of one kind (0: b-like, 1: u-p) from their LU
-// factors: 16 right-hand sides per workgroup when they fit in LDS, else 8.
+// factors, 8 right-hand sides per workgroup.
 template <int NT, typename OT>
 inline void lu_invert(hipStream_t s, const double* LU, const int* perm, OT* inv, int N, int kind, unsigned nsys,
                       size_t stride, size_t pstride, int nmax) {
-    if (lu_inverse_lds_doubles(nmax, 16) * sizeof(double) <= 160 * 1024) {
-        hipLaunchKernelGGL((k_lu_inverse<NT, OT, 16>), dim3(nsys, (unsigned)((nmax + 15) / 16)), dim3(NT),
-                           lu_inverse_lds_doubles(nmax, 16) * sizeof(double), s, LU, perm, inv, N, kind, stride,
-                           pstride, stride, nmax);
-    } else {
-        hipLaunchKernelGGL((k_lu_inverse<NT, OT, 8>), dim3(nsys, (unsigned)((nmax + 7) / 8)), dim3(NT),
-                           lu_inverse_lds_doubles(nmax, 8) * sizeof(double), s, LU, perm, inv, N, kind, stride,
-                           pstride, stride, nmax);
-    }
+    // 8 right-hand sides per workgroup: 16 would need ~250 VGPRs (one wavefront per SIMD)
+    hipLaunchKernelGGL((k_lu_inverse<NT, OT, 8>), dim3((unsigned)((nmax + 7) / 8), nsys), dim3(NT),
+                       lu_inverse_lds_doubles(nmax, 8) * sizeof(double), s, LU, perm, inv, N, kind, stride, pstride,
+                       stride, nmax);
 }
 
 // ---------------------------------------------------------------------------
