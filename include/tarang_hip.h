@@ -166,6 +166,10 @@ int tarang_fivp_create(tarang_ctx_t ctx, const tarang_fivp_desc* desc, tarang_fi
 int tarang_fivp_destroy(tarang_fivp_t s);
 /* state = coefficient array of the evolved field (device pointer, reference layout) */
 int tarang_fivp_set_state(tarang_fivp_t s, const void* coeff);
+/* as tarang_fivp_set_state, but keeps the clock and the multistep history: a state the
+ * caller changed between steps, gathered again the way gather_inputs! does
+ * (src/core/subsystems/subproblem_io.jl:1143-1152) */
+int tarang_fivp_update_state(tarang_fivp_t s, const void* coeff);
 int tarang_fivp_get_state(tarang_fivp_t s, void* coeff);
 /* one right-hand-side evaluation F(coeff) (evaluate_rhs, src/core/timesteppers/state_utils.jl:261-274) */
 int tarang_fivp_rhs(tarang_fivp_t s, const void* coeff, void* F);
@@ -227,6 +231,9 @@ int tarang_chan_destroy(tarang_chan_t s);
 /* coefficient array of p, b, ux or uz: complex (nx/2+1, nz), reference layout, device pointer
  * (scatter_inputs / gather_inputs!, src/core/subsystems/subproblem_io.jl:1143-1152) */
 int tarang_chan_set_field(tarang_chan_t s, int var, const void* coeff);
+/* as tarang_chan_set_field, but keeps the clock and the multistep history (a state the
+ * caller changed between steps, gathered again as gather_inputs! does) */
+int tarang_chan_update_field(tarang_chan_t s, int var, const void* coeff);
 int tarang_chan_get_field(tarang_chan_t s, int var, void* coeff);
 /* tau variable t (0 tau_p, 1 tau_b1, 2 tau_b2, 3 tau_u1.x, 4 tau_u1.z, 5 tau_u2.x, 6 tau_u2.z):
  * nx/2+1 complex values (tau_p: entry 0 only) */
@@ -273,6 +280,8 @@ int tarang_chan3_create(tarang_ctx_t ctx, const tarang_chan3_desc* desc, tarang_
 int tarang_chan3_destroy(tarang_chan3_t s);
 /* complex (nx/2+1, ny, nz) coefficient arrays, reference layout, device pointers */
 int tarang_chan3_set_field(tarang_chan3_t s, int var, const void* coeff);
+/* as tarang_chan3_set_field, but keeps the clock and the multistep history */
+int tarang_chan3_update_field(tarang_chan3_t s, int var, const void* coeff);
 int tarang_chan3_get_field(tarang_chan3_t s, int var, void* coeff);
 /* tau t: 0 tau_p, 1 tau_b1, 2 tau_b2, 3..5 tau_u1.{x,y,z}, 6..8 tau_u2.{x,y,z}; (nx/2+1, ny) values */
 int tarang_chan3_get_tau(tarang_chan3_t s, int t, void* vals);

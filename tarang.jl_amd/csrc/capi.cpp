@@ -338,6 +338,14 @@ int tarang_fivp_set_state(tarang_fivp_t s, const void* coeff) {
     });
 }
 
+int tarang_fivp_update_state(tarang_fivp_t s, const void* coeff) {
+    return guard([&] {
+        NEED(s, "solver");
+        NEED(coeff, "coeff");
+        s->solver->set_state(coeff, false);
+    });
+}
+
 int tarang_fivp_get_state(tarang_fivp_t s, void* coeff) {
     return guard([&] {
         NEED(s, "solver");
@@ -461,6 +469,14 @@ int tarang_chan_set_field(tarang_chan_t s, int var, const void* coeff) {
     });
 }
 
+int tarang_chan_update_field(tarang_chan_t s, int var, const void* coeff) {
+    return guard([&] {
+        NEED(s, "solver");
+        NEED(coeff, "coeff");
+        s->solver->set_field(var, coeff, false);
+    });
+}
+
 int tarang_chan_get_field(tarang_chan_t s, int var, void* coeff) {
     return guard([&] {
         NEED(s, "solver");
@@ -552,6 +568,14 @@ int tarang_chan3_set_field(tarang_chan3_t s, int var, const void* coeff) {
         NEED(s, "solver");
         NEED(coeff, "coeff");
         s->solver->set_field(var, coeff);
+    });
+}
+
+int tarang_chan3_update_field(tarang_chan3_t s, int var, const void* coeff) {
+    return guard([&] {
+        NEED(s, "solver");
+        NEED(coeff, "coeff");
+        s->solver->set_field(var, coeff, false);
     });
 }
 

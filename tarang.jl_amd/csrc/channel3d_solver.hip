@@ -199,13 +199,15 @@ void Channel3Solver::transpose_t(const void* in, void* out, int rows, int cols) 
 }
 
 // reference layout (mode fastest, nz rows of nmodes) <-> mode-major (nmodes rows of nz)
-void Channel3Solver::set_field(int var, const void* dev) {
+void Channel3Solver::set_field(int var, const void* dev, bool restart) {
     ctx_->activate();
     void* dst = field_ptr(var);
     if (d_.dtype == kF32) transpose_t<float>(dev, dst, d_.nz, nmodes_);
     else transpose_t<double>(dev, dst, d_.nz, nmodes_);
-    t_ = 0;
-    iter_ = 0;  // a new initial condition restarts the multistep history
+    if (restart) {  // a new initial condition restarts the clock and the multistep history
+        t_ = 0;
+        iter_ = 0;
+    }
 }
 
 void Channel3Solver::get_field(int var, void* dev) {

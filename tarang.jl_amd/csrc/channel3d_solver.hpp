@@ -33,7 +33,7 @@ class Channel3Solver {
     size_t field_bytes() const { return field_elems() * 2 * dtype_size((DType)d_.dtype); }
     int nkeys() const { return nkeys_; }
     // coefficient arrays in the reference layout (nx/2+1, ny, nz), device pointers
-    void set_field(int var, const void* dev);
+    void set_field(int var, const void* dev, bool restart = true);  // restart: clock and multistep history
     void get_field(int var, void* dev);
     // tau t (0 tau_p, 1 tau_b1, 2 tau_b2, 3-5 tau_u1.{x,y,z}, 6-8 tau_u2.{x,y,z}): (nx/2+1, ny) values
     void get_tau(int t, void* dev);

@@ -143,12 +143,14 @@ void ChannelSolver::from_modes_t(const void* in, void* out) {
     TG_HIP(hipGetLastError());
 }
 
-void ChannelSolver::set_field(int var, const void* dev) {
+void ChannelSolver::set_field(int var, const void* dev, bool restart) {
     ctx_->activate();
     void* dst = field_ptr(var);
     if (d_.dtype == kF32) to_modes_t<float>(dev, dst); else to_modes_t<double>(dev, dst);
-    t_ = 0;
-    iter_ = 0;  // a new initial condition restarts the multistep history
+    if (restart) {  // a new initial condition restarts the clock and the multistep history
+        t_ = 0;
+        iter_ = 0;
+    }
 }
 
 void ChannelSolver::get_field(int var, void* dev) {

@@ -30,7 +30,7 @@ class ChannelSolver {
     size_t field_elems() const { return (size_t)nxh_ * d_.nz; }
     size_t field_bytes() const { return field_elems() * 2 * dtype_size((DType)d_.dtype); }
     // coefficient arrays in the reference layout (nx/2+1, nz), kx fastest, device pointers
-    void set_field(int var, const void* dev);
+    void set_field(int var, const void* dev, bool restart = true);  // restart: clock and multistep history
     void get_field(int var, void* dev);
     // tau variable t (0 tau_p, 1 tau_b1, 2 tau_b2, 3 tau_u1.x, 4 tau_u1.z, 5 tau_u2.x, 6 tau_u2.z), nx/2+1 values
     void get_tau(int t, void* dev);

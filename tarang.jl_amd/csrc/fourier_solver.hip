@@ -130,11 +130,13 @@ FourierSolver::~FourierSolver() {
     if (graph_) (void)hipGraphExecDestroy(graph_);
 }
 
-void FourierSolver::set_state(const void* dev) {
+void FourierSolver::set_state(const void* dev, bool restart) {
     ctx_->activate();
     TG_HIP(hipMemcpyAsync(X_.ptr, dev, coeff_bytes(), hipMemcpyDeviceToDevice, ctx_->stream));
-    t_ = 0;
-    iter_ = 0;
+    if (restart) {
+        t_ = 0;
+        iter_ = 0;
+    }
 }
 
 void FourierSolver::get_state(void* dev) const {

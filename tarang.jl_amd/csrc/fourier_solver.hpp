@@ -42,7 +42,7 @@ class FourierSolver {
     ~FourierSolver();
     size_t coeff_elems() const { return (size_t)nxh_ * ny_; }
     size_t coeff_bytes() const { return coeff_elems() * 2 * dtype_size((DType)d_.dtype); }
-    void set_state(const void* dev);           // device -> device
+    void set_state(const void* dev, bool restart = true);  // device -> device; restart: clock, history
     void get_state(void* dev) const;           // device -> device
     void rhs(const void* Y, void* F);          // one RHS evaluation F(Y)
     // algebraic fields of the vorticity problem (psi, u, v coefficients) from q

@@ -7,6 +7,7 @@ from ._lib import TarangError, LIB_PATH, load as load_library
 from .core import (CPU, GPU, CartesianCoordinates, ChannelDomain, ChannelDomain3D, ChebyshevT, Coordinate, Distributor,
                    Domain, GPUTransformUnsupported, PeriodicDomain, RealFourier, ScalarField, VectorField,
                    backward_transform_, ensure_layout_, forward_transform_, get_grid_coordinates, padded_multiply, set_)
+from .checkpoint import load_field_, load_state_, save_field, save_state
 from .derivatives import evaluate_chebyshev_derivative_, gpu_chebyshev_deriv_
 from .flow_tools import CFL, add_diffusivity_, add_velocity_, compute_timestep, grid_spacing
 from .matsolvers import BatchedDenseLU, batched_factor_, batched_solve_, device_array
@@ -24,4 +25,5 @@ __all__ = [
     "TarangError", "LIB_PATH", "load_library", "Expr", "derivative_basis", "grad", "lift", "unit_vector_fields",
     "evaluate_chebyshev_derivative_", "gpu_chebyshev_deriv_", "BatchedDenseLU", "batched_factor_", "batched_solve_",
     "device_array", "CFL", "add_diffusivity_", "add_velocity_", "compute_timestep", "grid_spacing",
+    "load_field_", "load_state_", "save_field", "save_state",
 ]

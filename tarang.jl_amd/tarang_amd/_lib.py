@@ -133,6 +133,7 @@ SIGNATURES = {
     "tarang_fivp_create": [_P, ctypes.POINTER(FivpDesc), _PP],
     "tarang_fivp_destroy": [_P],
     "tarang_fivp_set_state": [_P, _P],
+    "tarang_fivp_update_state": [_P, _P],
     "tarang_fivp_get_state": [_P, _P],
     "tarang_fivp_rhs": [_P, _P, _P],
     "tarang_fivp_aux": [_P, _P, _P, _P, _P],
@@ -147,6 +148,7 @@ SIGNATURES = {
     "tarang_chan_create": [_P, ctypes.POINTER(ChanDesc), _PP],
     "tarang_chan_destroy": [_P],
     "tarang_chan_set_field": [_P, _I, _P],
+    "tarang_chan_update_field": [_P, _I, _P],
     "tarang_chan_get_field": [_P, _I, _P],
     "tarang_chan_get_tau": [_P, _I, _P],
     "tarang_chan_rhs": [_P, _P, _P, _P],
@@ -159,6 +161,7 @@ SIGNATURES = {
     "tarang_chan3_create": [_P, ctypes.POINTER(Chan3Desc), _PP],
     "tarang_chan3_destroy": [_P],
     "tarang_chan3_set_field": [_P, _I, _P],
+    "tarang_chan3_update_field": [_P, _I, _P],
     "tarang_chan3_get_field": [_P, _I, _P],
     "tarang_chan3_get_tau": [_P, _I, _P],
     "tarang_chan3_rhs": [_P, _P, _P, _P, _P],

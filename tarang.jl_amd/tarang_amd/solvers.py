@@ -122,13 +122,17 @@ class InitialValueSolver:
         _lib.call("tarang_fivp_create", self.dev.handle, ctypes.byref(d), ctypes.byref(h))
         self.handle = h
         _lib.call("tarang_fivp_set_graph", h, 1 if use_graph else 0)
-        self._push_state()
+        self._push_state(restart=True)
 
     # -- state plumbing --
-    def _push_state(self):
+    def _push_state(self, restart: bool = False):
+        """Hand the evolved field to the library.  A new initial condition (restart)
+        resets the clock and the multistep history; a state changed between steps
+        keeps them, as the reference's gather does (subproblem_io.jl:1143-1152)."""
         ensure_layout_(self.field, "c")
         self.dev.before()
-        _lib.call("tarang_fivp_set_state", self.handle, _ptr(self.field._c))
+        _lib.call("tarang_fivp_set_state" if restart else "tarang_fivp_update_state", self.handle,
+                  _ptr(self.field._c))
         self.dev.after()
 
     def _pull_state(self):
@@ -183,6 +187,22 @@ class InitialValueSolver:
         if time.time() - self.start_wall_time >= self.stop_wall_time:
             return False
         return True
+
+    @property
+    def state(self):
+        """The state fields (problem variables with spatial data, by component)."""
+        from .checkpoint import state_fields
+        return state_fields(self)
+
+    def restart(self):
+        """Start the timestepper afresh from the current fields: the library takes them as
+        a new initial condition (clock and multistep history reset), as the reference does
+        after load_state! (timestepper_state = nothing)."""
+        if self._channel is not None:
+            self._channel.push(restart=True)
+        else:
+            self._push_state(restart=True)
+            self._dirty = False
 
     def set_dt(self, dt: float):
         _lib.call(self._channel.pre + "set_dt" if self._channel is not None else "tarang_fivp_set_dt", self.handle,
@@ -256,14 +276,16 @@ class _ChannelBackend:
         self.handle = h
         _lib.call(self.pre + "set_graph", h, 1 if use_graph else 0)
         self.fields = tuple(zip(vars_, [self.p, self.b] + list(u.components)))
-        self.push()
+        self.push(restart=True)
 
-    def push(self):
+    def push(self, restart: bool = False):
+        """Fields -> library.  restart (a new initial condition) resets the clock and the
+        multistep history; otherwise they are kept (gather_inputs!, subproblem_io.jl:1143)."""
         for _, f in self.fields:
             ensure_layout_(f, "c")
         self.dev.before()
         for var, f in self.fields:
-            _lib.call(self.pre + "set_field", self.handle, var, _ptr(f._c))
+            _lib.call(self.pre + ("set_field" if restart else "update_field"), self.handle, var, _ptr(f._c))
         self.dev.after()
 
     def pull(self):

@@ -193,3 +193,23 @@ def test_rbc3d_through_api(oracle):
     # two roundings differ there by ~cond x eps relative to the taus' own scale
     assert relerr(tau_b1._c.cpu().numpy().T, ref.tau(1)) < 1e-8
     solver.close()
+
+
+def test_rbc_sbdf2_in_batches_keeps_history(oracle):
+    """run_ issues one library call per callback interval; every call gathers the fields
+    again, which must not restart SBDF2's history (the reference's gather does not):
+    6 steps in batches of 2 equal the oracle's 6 continuous SBDF2 steps."""
+    Lx, Lz, Nx, Nz = 4.0, 1.0, 16, 10
+    kappa = nu = 1e-2
+    solver, b, u, xb, zb = rbc_problem(np.float64, Lx, Lz, Nx, Nz, kappa, nu, dt=0.01, stepper=T.SBDF2())
+    z = zb.grid()
+    bg = (Lz - z)[None, :] + 0.05 * np.cos(2 * np.pi * xb.grid() / Lx)[:, None] * (z * (Lz - z))[None, :]
+    b["g"] = bg
+    T.run_(solver, stop_iteration=6, callbacks=[(2, lambda s: None)])
+    ref = oracle.RBC2D(Nx, Nz, Lx, Lz, kappa, nu, 0.01, "SBDF2")
+    d = oracle.domain([oracle.REALFOURIER, oracle.CHEBYSHEVT], [Nx, Nz], bounds=[(0.0, Lx), (0.0, Lz)])
+    ref.set(oracle.RBC_B, oracle.forward_transform(d, bg))
+    ref.step(6)
+    assert relerr(b["c"].cpu().numpy(), ref.get(oracle.RBC_B)) < 1e-12
+    assert relerr(u[1]["c"].cpu().numpy(), ref.get(oracle.RBC_UZ)) < 1e-11
+    solver.close()
