@@ -234,6 +234,13 @@ int tarang_chan_set_field(tarang_chan_t s, int var, const void* coeff);
 /* as tarang_chan_set_field, but keeps the clock and the multistep history (a state the
  * caller changed between steps, gathered again as gather_inputs! does) */
 int tarang_chan_update_field(tarang_chan_t s, int var, const void* coeff);
+/* Space- (and, between steps, time-) dependent Dirichlet values of b: b_bottom / b_top are
+ * device arrays of the nx grid values of b on z = 0 and z = Lz (state dtype).  Their
+ * unnormalised rfft is the per-mode boundary value, the projection the reference gives an
+ * array right-hand side of a BC (ArrayOperator, src/core/subsystems/subproblem_rhs.jl:30-58,
+ * gathered per stage by gather_alg_F!, :523-607).  The values are read at every stage, so a
+ * call between steps takes effect from the next stage; they replace desc.b_bottom/b_top. */
+int tarang_chan_set_bc(tarang_chan_t s, const void* b_bottom, const void* b_top);
 int tarang_chan_get_field(tarang_chan_t s, int var, void* coeff);
 /* tau variable t (0 tau_p, 1 tau_b1, 2 tau_b2, 3 tau_u1.x, 4 tau_u1.z, 5 tau_u2.x, 6 tau_u2.z):
  * nx/2+1 complex values (tau_p: entry 0 only) */
@@ -282,6 +289,9 @@ int tarang_chan3_destroy(tarang_chan3_t s);
 int tarang_chan3_set_field(tarang_chan3_t s, int var, const void* coeff);
 /* as tarang_chan3_set_field, but keeps the clock and the multistep history */
 int tarang_chan3_update_field(tarang_chan3_t s, int var, const void* coeff);
+/* as tarang_chan_set_bc, with (nx, ny) wall arrays (x fastest) and their 2-D unnormalised
+ * rfft as the per-mode values */
+int tarang_chan3_set_bc(tarang_chan3_t s, const void* b_bottom, const void* b_top);
 int tarang_chan3_get_field(tarang_chan3_t s, int var, void* coeff);
 /* tau t: 0 tau_p, 1 tau_b1, 2 tau_b2, 3..5 tau_u1.{x,y,z}, 6..8 tau_u2.{x,y,z}; (nx/2+1, ny) values */
 int tarang_chan3_get_tau(tarang_chan3_t s, int t, void* vals);

@@ -235,6 +235,19 @@ void orc_rbc3_set_bc(orc_rbc3 *s, double b_bottom, double b_top) {
     }
 }
 
+/* per-mode boundary values of b (complex [nxh * ny], mode = kx + nxh ky): the unnormalised
+ * 2-D rfft of the wall arrays (subproblem_rhs.jl:30-58,523-607) */
+void orc_rbc3_set_bc_values(orc_rbc3 *s, const double *g0, const double *g1) {
+    const int nf = s->nfull, nmodes = s->nxh * s->ny;
+    const cplx *a = (const cplx *)g0, *b = (const cplx *)g1;
+    for (int mode = 0; mode < nmodes; ++mode)
+        for (int r = 0; r < s->nm[mode]; ++r) {
+            const int k = s->bcrow[(size_t)mode * nf + r];
+            if (k == 0) s->algf[(size_t)mode * nf + r] = a[mode];
+            if (k == 4) s->algf[(size_t)mode * nf + r] = b[mode];
+        }
+}
+
 /* ---- dense complex LU with partial pivoting (column-major) ---- */
 static int lu3_factor(cplx *A, int n, int *piv) {
     for (int k = 0; k < n; ++k) {

@@ -259,6 +259,19 @@ void orc_rbc_set_bc(orc_rbc *s, double b_bottom, double b_top) {
     (void)nf;
 }
 
+/* per-mode boundary values of b (complex [nxh]): the unnormalised rfft of the wall arrays,
+ * the projection gather_alg_F! gives an array BC (subproblem_rhs.jl:30-58,523-607) */
+void orc_rbc_set_bc_values(orc_rbc *s, const double *g0, const double *g1) {
+    const int nf = s->nfull;
+    const cplx *a = (const cplx *)g0, *b = (const cplx *)g1;
+    for (int m = 0; m < s->nxh; ++m)
+        for (int r = 0; r < s->nm[m]; ++r) {
+            const int k = s->bcrow[(size_t)m * nf + r];
+            if (k == 0) s->algf[(size_t)m * nf + r] = a[m];
+            if (k == 3) s->algf[(size_t)m * nf + r] = b[m];
+        }
+}
+
 /* ---- dense complex LU with partial pivoting (column-major) ---- */
 static int lu_factor(cplx *A, int n, int *piv) {
     for (int k = 0; k < n; ++k) {

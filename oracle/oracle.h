@@ -94,6 +94,8 @@ void orc_rbc_destroy(orc_rbc *s);
 int orc_rbc_size(const orc_rbc *s, int mode);
 /* Dirichlet values of b (default b(0) = Lz, b(Lz) = 0) */
 void orc_rbc_set_bc(orc_rbc *s, double b_bottom, double b_top);
+/* per-mode wall values of b, complex [nx/2+1] each (unnormalised rfft of the wall arrays) */
+void orc_rbc_set_bc_values(orc_rbc *s, const double *g0, const double *g1);
 /* coefficient arrays (nx/2+1, nz) complex, reference layout */
 void orc_rbc_set_field(orc_rbc *s, int var, const double *coeff);
 void orc_rbc_get_field(const orc_rbc *s, int var, double *coeff);
@@ -116,6 +118,8 @@ void orc_rbc3_destroy(orc_rbc3 *s);
 /* mode = m + (nx/2+1) n, m the kx index, n the ky index in FFT order */
 int orc_rbc3_size(const orc_rbc3 *s, int mode);
 void orc_rbc3_set_bc(orc_rbc3 *s, double b_bottom, double b_top);
+/* per-mode wall values of b, complex [(nx/2+1) ny] each, mode = kx + (nx/2+1) ky */
+void orc_rbc3_set_bc_values(orc_rbc3 *s, const double *g0, const double *g1);
 /* coefficient arrays (nx/2+1, ny, nz) complex, reference layout */
 void orc_rbc3_set_field(orc_rbc3 *s, int var, const double *coeff);
 void orc_rbc3_get_field(const orc_rbc3 *s, int var, double *coeff);

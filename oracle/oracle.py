@@ -82,6 +82,7 @@ def lib():
         L.orc_rbc_destroy.argtypes = [vp]
         L.orc_rbc_size.argtypes = [vp, ctypes.c_int]
         L.orc_rbc_set_bc.argtypes = [vp, ctypes.c_double, ctypes.c_double]
+        L.orc_rbc_set_bc_values.argtypes = [vp, dp, dp]
         L.orc_rbc_set_field.argtypes = [vp, ctypes.c_int, dp]
         L.orc_rbc_get_field.argtypes = [vp, ctypes.c_int, dp]
         L.orc_rbc_get_tau.argtypes = [vp, ctypes.c_int, dp]
@@ -97,6 +98,7 @@ def lib():
         L.orc_rbc3_destroy.argtypes = [vp]
         L.orc_rbc3_size.argtypes = [vp, ctypes.c_int]
         L.orc_rbc3_set_bc.argtypes = [vp, ctypes.c_double, ctypes.c_double]
+        L.orc_rbc3_set_bc_values.argtypes = [vp, dp, dp]
         L.orc_rbc3_set_field.argtypes = [vp, ctypes.c_int, dp]
         L.orc_rbc3_get_field.argtypes = [vp, ctypes.c_int, dp]
         L.orc_rbc3_get_tau.argtypes = [vp, ctypes.c_int, dp]
@@ -316,6 +318,12 @@ class RBC2D:
         """Dirichlet values of b at z = 0 and z = Lz."""
         lib().orc_rbc_set_bc(self.h, float(b_bottom), float(b_top))
 
+    def set_bc_values(self, g0, g1):
+        """Per-mode wall values of b: complex (nx/2+1) arrays (unnormalised rfft of the walls)."""
+        a = np.ascontiguousarray(g0, np.complex128)
+        b = np.ascontiguousarray(g1, np.complex128)
+        lib().orc_rbc_set_bc_values(self.h, _dp(a), _dp(b))
+
     def set(self, var, coeff):
         c = _fortran(coeff, np.complex128)
         lib().orc_rbc_set_field(self.h, var, _dp(c))
@@ -386,6 +394,12 @@ class RBC3D:
 
     def set_bc(self, b_bottom, b_top):
         lib().orc_rbc3_set_bc(self.h, float(b_bottom), float(b_top))
+
+    def set_bc_values(self, g0, g1):
+        """Per-mode wall values of b: complex (nx/2+1, ny) arrays, kx fastest."""
+        a = np.asfortranarray(g0, np.complex128)
+        b = np.asfortranarray(g1, np.complex128)
+        lib().orc_rbc3_set_bc_values(self.h, _dp(a), _dp(b))
 
     def set(self, var, coeff):
         c = _fortran(coeff, np.complex128)

@@ -477,6 +477,13 @@ int tarang_chan_update_field(tarang_chan_t s, int var, const void* coeff) {
     });
 }
 
+int tarang_chan_set_bc(tarang_chan_t s, const void* b_bottom, const void* b_top) {
+    return guard([&] {
+        NEED(s, "solver");
+        s->solver->set_bc(b_bottom, b_top);
+    });
+}
+
 int tarang_chan_get_field(tarang_chan_t s, int var, void* coeff) {
     return guard([&] {
         NEED(s, "solver");
@@ -576,6 +583,13 @@ int tarang_chan3_update_field(tarang_chan3_t s, int var, const void* coeff) {
         NEED(s, "solver");
         NEED(coeff, "coeff");
         s->solver->set_field(var, coeff, false);
+    });
+}
+
+int tarang_chan3_set_bc(tarang_chan3_t s, const void* b_bottom, const void* b_top) {
+    return guard([&] {
+        NEED(s, "solver");
+        s->solver->set_bc(b_bottom, b_top);
     });
 }
 

@@ -388,7 +388,7 @@ struct Ch3SolveArgs {
     int N;
     size_t sb, su;
     double c;
-    double bc_b0, bc_b1;  // Nx Ny b(z=0), Nx Ny b(z=Lz) at kx = ky = 0
+    const double2* bcv;  // [2 m + w]: b on wall w (0: z=0, 1: z=Lz) at mode m (unnormalised 2-D rfft)
 };
 
 // 3-D tau slots: 0 tau_p, 1 tau_b1, 2 tau_b2, 3 tau_u1x, 4 tau_u1y, 5 tau_u1z, 6 tau_u2x, 7 tau_u2y, 8 tau_u2z
@@ -463,9 +463,10 @@ __global__ __launch_bounds__(NT, 768 / NT) void k3_solve(Ch3SolveArgs s, const c
         Yu[(ng + i) * kYS + 2 * j] = (double)r1.x;
         Yu[(ng + i) * kYS + 2 * j + 1] = (double)r1.y;
     }
-    if (threadIdx.x == 0 && key == 0) {  // wall values of b at kx = ky = 0 (column 0 = mode 0, real part)
-        const double gv = h == 0 ? 0.5 * (s.bc_b1 + s.bc_b0) : 0.5 * (s.bc_b1 - s.bc_b0);
-        Yb[nh * kYS] = s.c * gv;
+    for (int j = threadIdx.x; j < cnt; j += NT) {  // wall rows of b: half-sum / half-difference per mode
+        const double2 g0 = s.bcv[2 * (size_t)smode[j]], g1 = s.bcv[2 * (size_t)smode[j] + 1];
+        Yb[nh * kYS + 2 * j] = s.c * (h == 0 ? 0.5 * (g1.x + g0.x) : 0.5 * (g1.x - g0.x));
+        Yb[nh * kYS + 2 * j + 1] = s.c * (h == 0 ? 0.5 * (g1.y + g0.y) : 0.5 * (g1.y - g0.y));
     }
     __syncthreads();
     // ---- b (parity h) and perp (parity g) systems ----

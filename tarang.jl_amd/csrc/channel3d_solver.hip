@@ -167,6 +167,12 @@ Channel3Solver::Channel3Solver(Context* ctx, const Channel3Desc& d) : ctx_(ctx),
     Ip_ = DevBuf(ctx_, (size_t)2 * nkeys_ * nb_max_ * nb_max_ * ib);
     Iu_ = DevBuf(ctx_, (size_t)2 * nkeys_ * nu_max_ * nu_max_ * ib);
     info_ = DevBuf(ctx_, (size_t)6 * nkeys_ * sizeof(int));
+    {
+        const auto bcv = constant_wall_values((size_t)nmodes_, (double)d.nx * d.ny, d.b_bottom, d.b_top);
+        bcv_ = DevBuf(ctx_, bcv.size() * sizeof(double2));
+        bch_ = DevBuf(ctx_, bcv.size() * sizeof(double2));
+        TG_HIP(hipMemcpyAsync(bcv_.ptr, bcv.data(), bcv.size() * sizeof(double2), hipMemcpyHostToDevice, s));
+    }
     ctx_->twiddle(mx_, (DType)d.dtype);
     ctx_->twiddle(my_, (DType)d.dtype);
     TG_HIP(hipStreamSynchronize(s));
@@ -208,6 +214,26 @@ void Channel3Solver::set_field(int var, const void* dev, bool restart) {
         t_ = 0;
         iter_ = 0;
     }
+}
+
+void Channel3Solver::set_bc(const void* b_bottom, const void* b_top) {
+    TG_REQUIRE(b_bottom && b_top, kErrInvalid, "channel solver: set_bc needs both wall arrays");
+    ctx_->activate();
+    const int nx = d_.nx, ny = d_.ny, nmodes = nxh_ * ny;
+    const unsigned nb = (unsigned)((2 * nmodes + 255) / 256);
+    const double2* twx = static_cast<const double2*>(ctx_->twiddle(nx, kF64));
+    const double2* twy = static_cast<const double2*>(ctx_->twiddle(ny, kF64));
+    if (d_.dtype == kF32)
+        hipLaunchKernelGGL((k_wall_dft_x<float>), dim3(nb), dim3(256), 0, ctx_->stream,
+                           static_cast<const float*>(b_bottom), static_cast<const float*>(b_top), nx, ny, nxh_, twx,
+                           bch_.as<double2>());
+    else
+        hipLaunchKernelGGL((k_wall_dft_x<double>), dim3(nb), dim3(256), 0, ctx_->stream,
+                           static_cast<const double*>(b_bottom), static_cast<const double*>(b_top), nx, ny, nxh_, twx,
+                           bch_.as<double2>());
+    hipLaunchKernelGGL(k_wall_dft_y, dim3(nb), dim3(256), 0, ctx_->stream, bch_.as<const double2>(), ny, nxh_, twy,
+                       bcv_.as<double2>());
+    TG_HIP(hipGetLastError());
 }
 
 void Channel3Solver::get_field(int var, void* dev) {
@@ -392,11 +418,9 @@ template <typename T>
 void Channel3Solver::solve_t() {
     using V = cplx<T>;
     const int N = d_.nz;
-    const double scale = (double)d_.nx * d_.ny;
     Ch3SolveArgs sa{Ib_.ptr, Ip_.ptr, Iu_.ptr,
                     chunks_.as<const int4>(), modes_.as<const int>(), rot_.as<const double>(), N,
-                    (size_t)nb_max_ * nb_max_, (size_t)nu_max_ * nu_max_, factored_c_, scale * d_.b_bottom,
-                    scale * d_.b_top};
+                    (size_t)nb_max_ * nb_max_, (size_t)nu_max_ * nu_max_, factored_c_, bcv_.as<const double2>()};
     constexpr int NT = 256;  // four wavefronts: at most 3 b/perp and 7 u-p tiles each (nz <= 256)
     const size_t lds = ch3_solve_lds_doubles(N) * sizeof(double);
     timed(6, [&] {

@@ -34,6 +34,9 @@ class Channel3Solver {
     int nkeys() const { return nkeys_; }
     // coefficient arrays in the reference layout (nx/2+1, ny, nz), device pointers
     void set_field(int var, const void* dev, bool restart = true);  // restart: clock and multistep history
+    // grid values of b on the walls z = 0 and z = Lz (device, state dtype, nx [x ny]): their
+    // per-mode projections become the Dirichlet values, read at every stage
+    void set_bc(const void* b_bottom, const void* b_top);
     void get_field(int var, void* dev);
     // tau t (0 tau_p, 1 tau_b1, 2 tau_b2, 3-5 tau_u1.{x,y,z}, 6-8 tau_u2.{x,y,z}): (nx/2+1, ny) values
     void get_tau(int t, void* dev);
@@ -81,6 +84,7 @@ class Channel3Solver {
     DevBuf G_, Q_, H_;
     DevBuf BT_, BDT_, FT_;
     DevBuf kh_, rot_, modes_, chunks_;
+    DevBuf bcv_, bch_;  // per-mode wall values of b [2 m + w], and the x-transform scratch
     DevBuf Ib_, Ip_, Iu_, info_;  // explicit FP64 inverses per (key, parity) of the b, perp and u-p systems
     int nchunks_ = 0;
     std::vector<int> key_count_;  // modes per key (host copy, for the byte model)

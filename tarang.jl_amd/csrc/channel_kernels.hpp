@@ -1097,9 +1097,52 @@ struct ChSolveArgs {
     size_t sb, su;
     int ld;             // LDS row length of one right-hand side (>= largest system)
     double c;           // implicit coefficient of the step
-    double bc_b0;       // b(z=0) and b(z=Lz) in coefficient space at kx = 0 (Nx * value)
-    double bc_b1;
+    const double2* bcv; // [2 m + w]: b on wall w (0: z=0, 1: z=Lz) at mode m, unnormalised rfft of the wall values
 };
+
+// Wall values of b -> per-mode boundary values: the unnormalised rfft along x (and FFT
+// along y) of each wall's grid array, in FP64, as the reference projects an array BC
+// onto a Fourier mode.  Stage 1: h[w][y][kx] = sum_x v_w[x + nx y] e^{-2 pi i kx x/nx};
+// stage 2: out[2 (kx + nxh ky) + w] = sum_y h[w][y][kx] e^{-2 pi i ky y/ny}.
+template <typename S>
+__global__ void k_wall_dft_x(const S* __restrict__ v0, const S* __restrict__ v1, int nx, int ny, int nxh,
+                             const double2* __restrict__ tw, double2* __restrict__ h) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= 2 * nxh * ny) return;
+    const int w = idx / (nxh * ny), r = idx - w * nxh * ny, kx = r % nxh, y = r / nxh;
+    const S* v = (w ? v1 : v0) + (size_t)y * nx;
+    double re = 0.0, im = 0.0;
+    for (int x = 0; x < nx; ++x) {
+        const double2 t = tw[(kx * x) % nx];
+        const double a = (double)v[x];
+        re += a * t.x;
+        im += a * t.y;
+    }
+    h[idx] = double2{re, im};
+}
+
+static __global__ void k_wall_dft_y(const double2* __restrict__ h, int ny, int nxh, const double2* __restrict__ tw,
+                             double2* __restrict__ out) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= 2 * nxh * ny) return;
+    const int w = idx / (nxh * ny), r = idx - w * nxh * ny, kx = r % nxh, ky = r / nxh;
+    const double2* hw = h + (size_t)w * nxh * ny;
+    double re = 0.0, im = 0.0;
+    for (int y = 0; y < ny; ++y) {
+        const double2 t = tw[(ky * y) % ny], a = hw[(size_t)y * nxh + kx];
+        re += a.x * t.x - a.y * t.y;
+        im += a.x * t.y + a.y * t.x;
+    }
+    out[2 * (size_t)(kx + nxh * ky) + w] = double2{re, im};
+}
+
+// Per-mode boundary values of constant walls: nx ny b_w at mode 0, zero elsewhere.
+inline std::vector<double2> constant_wall_values(size_t nmodes, double scale, double b0, double b1) {
+    std::vector<double2> v(2 * nmodes, double2{0.0, 0.0});
+    v[0] = double2{scale * b0, 0.0};
+    v[1] = double2{scale * b1, 0.0};
+    return v;
+}
 
 // x[r*ld + i] = sum_k A[k*n + i] y[r*ld + k], r = 0, 1 (real and imaginary part), for
 // a column-major n x n matrix A (global) and y, x in LDS; thread t owns rows t + j NT
@@ -1179,10 +1222,10 @@ __global__ __launch_bounds__(NT) void k_ch_solve(ChSolveArgs s, const cplx<T>* _
         y[ld + i] = (double)r.y;
     }
     if (threadIdx.x == 0) {
-        // wall rows: half-sum (even part) / half-difference (odd part) of the two Dirichlet values
-        const double gv = h == 0 ? 0.5 * (s.bc_b1 + s.bc_b0) : 0.5 * (s.bc_b1 - s.bc_b0);
-        y[nh] = m == 0 ? s.c * gv : 0.0;
-        y[ld + nh] = 0.0;
+        // wall row: half-sum (even part) / half-difference (odd part) of the two Dirichlet values
+        const double2 g0 = s.bcv[2 * m], g1 = s.bcv[2 * m + 1];
+        y[nh] = s.c * (h == 0 ? 0.5 * (g1.x + g0.x) : 0.5 * (g1.x - g0.x));
+        y[ld + nh] = s.c * (h == 0 ? 0.5 * (g1.y + g0.y) : 0.5 * (g1.y - g0.y));
     }
     __syncthreads();
     gemv2<NT>(static_cast<const T*>(s.Ib) + (size_t)(2 * m + h) * s.sb, nbh, y, ld, x);

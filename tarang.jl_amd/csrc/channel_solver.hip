@@ -96,6 +96,12 @@ ChannelSolver::ChannelSolver(Context* ctx, const ChannelDesc& d) : ctx_(ctx), d_
     Ib_ = DevBuf(ctx_, (size_t)2 * nxh_ * nb_max_ * nb_max_ * ib);
     Iu_ = DevBuf(ctx_, (size_t)2 * nxh_ * nu_max_ * nu_max_ * ib);
     info_ = DevBuf(ctx_, (size_t)4 * nxh_ * sizeof(int));
+    {
+        const auto bcv = constant_wall_values((size_t)nxh_, (double)d.nx, d.b_bottom, d.b_top);
+        bcv_ = DevBuf(ctx_, bcv.size() * sizeof(double2));
+        bch_ = DevBuf(ctx_, bcv.size() * sizeof(double2));
+        TG_HIP(hipMemcpyAsync(bcv_.ptr, bcv.data(), bcv.size() * sizeof(double2), hipMemcpyHostToDevice, s));
+    }
     ctx_->twiddle(mx_, (DType)d.dtype);
     TG_HIP(hipStreamSynchronize(s));
     factor(implicit_c(first_multistep()));
@@ -151,6 +157,26 @@ void ChannelSolver::set_field(int var, const void* dev, bool restart) {
         t_ = 0;
         iter_ = 0;
     }
+}
+
+void ChannelSolver::set_bc(const void* b_bottom, const void* b_top) {
+    TG_REQUIRE(b_bottom && b_top, kErrInvalid, "channel solver: set_bc needs both wall arrays");
+    ctx_->activate();
+    const int nx = d_.nx, ny = 1, nmodes = nxh_ * ny;
+    const unsigned nb = (unsigned)((2 * nmodes + 255) / 256);
+    const double2* twx = static_cast<const double2*>(ctx_->twiddle(nx, kF64));
+    const double2* twy = static_cast<const double2*>(ctx_->twiddle(ny, kF64));
+    if (d_.dtype == kF32)
+        hipLaunchKernelGGL((k_wall_dft_x<float>), dim3(nb), dim3(256), 0, ctx_->stream,
+                           static_cast<const float*>(b_bottom), static_cast<const float*>(b_top), nx, ny, nxh_, twx,
+                           bch_.as<double2>());
+    else
+        hipLaunchKernelGGL((k_wall_dft_x<double>), dim3(nb), dim3(256), 0, ctx_->stream,
+                           static_cast<const double*>(b_bottom), static_cast<const double*>(b_top), nx, ny, nxh_, twx,
+                           bch_.as<double2>());
+    hipLaunchKernelGGL(k_wall_dft_y, dim3(nb), dim3(256), 0, ctx_->stream, bch_.as<const double2>(), ny, nxh_, twy,
+                       bcv_.as<double2>());
+    TG_HIP(hipGetLastError());
 }
 
 void ChannelSolver::get_field(int var, void* dev) {
@@ -313,7 +339,7 @@ void ChannelSolver::solve_t() {
     using V = cplx<T>;
     const int N = d_.nz;
     ChSolveArgs sa{Ib_.ptr, Iu_.ptr, N, nxh_, (size_t)nb_max_ * nb_max_, (size_t)nu_max_ * nu_max_, nu_max_,
-                   factored_c_, (double)d_.nx * d_.b_bottom, (double)d_.nx * d_.b_top};
+                   factored_c_, bcv_.as<const double2>()};
     constexpr int NT = 256;
     const size_t lds = ch_solve_lds_doubles(nu_max_) * sizeof(double);
     timed(4, [&] {

@@ -31,6 +31,9 @@ class ChannelSolver {
     size_t field_bytes() const { return field_elems() * 2 * dtype_size((DType)d_.dtype); }
     // coefficient arrays in the reference layout (nx/2+1, nz), kx fastest, device pointers
     void set_field(int var, const void* dev, bool restart = true);  // restart: clock and multistep history
+    // grid values of b on the walls z = 0 and z = Lz (device, state dtype, nx [x ny]): their
+    // per-mode projections become the Dirichlet values, read at every stage
+    void set_bc(const void* b_bottom, const void* b_top);
     void get_field(int var, void* dev);
     // tau variable t (0 tau_p, 1 tau_b1, 2 tau_b2, 3 tau_u1.x, 4 tau_u1.z, 5 tau_u2.x, 6 tau_u2.z), nx/2+1 values
     void get_tau(int t, void* dev);
@@ -84,6 +87,7 @@ class ChannelSolver {
     double solve_c_ = 0;       // c of the factored systems used by the solve kernel
     DevBuf G_, H_;
     DevBuf BT_, BDT_, FT_;
+    DevBuf bcv_, bch_;  // per-mode wall values of b [2 m + w], and the x-transform scratch
     DevBuf Ib_, Iu_, info_;  // explicit inverses per (mode, parity) of the b and u-p systems, state precision
     bool graph_on_ = true;
     hipGraphExec_t graph_ = nullptr;

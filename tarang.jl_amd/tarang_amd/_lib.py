@@ -149,6 +149,7 @@ SIGNATURES = {
     "tarang_chan_destroy": [_P],
     "tarang_chan_set_field": [_P, _I, _P],
     "tarang_chan_update_field": [_P, _I, _P],
+    "tarang_chan_set_bc": [_P, _P, _P],
     "tarang_chan_get_field": [_P, _I, _P],
     "tarang_chan_get_tau": [_P, _I, _P],
     "tarang_chan_rhs": [_P, _P, _P, _P],
@@ -162,6 +163,7 @@ SIGNATURES = {
     "tarang_chan3_destroy": [_P],
     "tarang_chan3_set_field": [_P, _I, _P],
     "tarang_chan3_update_field": [_P, _I, _P],
+    "tarang_chan3_set_bc": [_P, _P, _P],
     "tarang_chan3_get_field": [_P, _I, _P],
     "tarang_chan3_get_tau": [_P, _I, _P],
     "tarang_chan3_rhs": [_P, _P, _P, _P, _P],

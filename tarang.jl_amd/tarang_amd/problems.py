@@ -121,7 +121,13 @@ def expand(problem: IVP, eq: str) -> str:
     return s
 
 
-def _bc_value(problem: IVP, token: str) -> float:
+def _bc_value(problem: IVP, token: str):
+    """A BC right-hand side: a number, or a parameter holding a number or a grid array
+    of wall values (a space-dependent BC, the reference's ArrayOperator,
+    src/core/subsystems/subproblem_rhs.jl:30-58)."""
+    v = problem.parameters.get(token)
+    if v is not None and not isinstance(v, float) and hasattr(v, "shape"):
+        return v
     return _param(problem, token)
 
 
@@ -195,7 +201,7 @@ def _recognize_rbc2d(problem: IVP) -> Optional[Recognized]:
     for key in need:
         if key not in bcs:
             raise NotImplementedError(f"missing boundary condition {key[0]}(z={key[1]:g})")
-    if bcs[(u, lo)] != 0.0 or bcs[(u, hi)] != 0.0:
+    if any(hasattr(bcs[k], "shape") for k in ((u, lo), (u, hi))) or bcs[(u, lo)] != 0.0 or bcs[(u, hi)] != 0.0:
         raise NotImplementedError("the channel GPU path supports no-slip walls u = 0 only")
     if "integ" not in bcs:
         raise NotImplementedError(f"missing gauge condition integ({p}) = 0")

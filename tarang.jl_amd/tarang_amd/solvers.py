@@ -271,7 +271,13 @@ class _ChannelBackend:
         d.dt = solver.dt
         d.dtype = _lib.F32 if dom.dtype == np.float32 else _lib.F64
         d.timestepper = timestepper.code
-        d.b_bottom, d.b_top = rec.extra["b_bottom"], rec.extra["b_top"]
+        walls = (rec.extra["b_bottom"], rec.extra["b_top"])
+        # array wall values (space-dependent BC) are handed over as grid arrays before every
+        # batch of steps, so a change between steps is seen (the reference re-gathers its
+        # ALG_F every stage, step_subproblem_rk_batched.jl:570-580)
+        self.walls = walls if any(hasattr(w, "shape") for w in walls) else None
+        self.wall_shape = tuple(bb.size for bb in dom.bases[:-1])
+        d.b_bottom, d.b_top = (0.0, 0.0) if self.walls else walls
         _lib.call(self.pre + "create", self.dev.handle, ctypes.byref(d), ctypes.byref(h))
         self.handle = h
         _lib.call(self.pre + "set_graph", h, 1 if use_graph else 0)
@@ -286,6 +292,18 @@ class _ChannelBackend:
         self.dev.before()
         for var, f in self.fields:
             _lib.call(self.pre + ("set_field" if restart else "update_field"), self.handle, var, _ptr(f._c))
+        if self.walls is not None:
+            import torch
+            dt = self.b._g.dtype
+            arr = []
+            for w in self.walls:
+                a = torch.as_tensor(w) if hasattr(w, "shape") else torch.full(self.wall_shape, float(w))
+                if tuple(a.shape) != self.wall_shape:
+                    raise ValueError(f"wall array of shape {tuple(a.shape)}: expected {self.wall_shape}")
+                # logical (x[, y]) -> column-major storage (x fastest)
+                arr.append(a.to(device=self.b._g.device, dtype=dt).permute(*reversed(range(a.dim()))).contiguous())
+            _lib.call(self.pre + "set_bc", self.handle, _ptr(arr[0]), _ptr(arr[1]))
+            self.dev.after()  # the arrays above are freed on return
         self.dev.after()
 
     def pull(self):

@@ -195,6 +195,14 @@ class Chan:
         self.dev.sync()
         return from_dev(c)
 
+    def set_bc(self, g0, g1):
+        """Wall values of b: grid arrays on z = 0 and z = Lz (logical order, x fastest in storage)."""
+        rt = torch.float32 if self.ct == torch.complex64 else torch.float64
+        a, b = to_dev(g0, rt, self.dev.torch_device), to_dev(g1, rt, self.dev.torch_device)
+        self.dev.sync()
+        _lib.call(("tarang_chan3_" if isinstance(self, Chan3) else "tarang_chan_") + "set_bc", self.h, ptr(a), ptr(b))
+        self.dev.sync()
+
     def tau(self, t):
         v = torch.zeros(self.cshape[0], dtype=self.ct, device=self.dev.torch_device)
         self.dev.sync()

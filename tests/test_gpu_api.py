@@ -93,9 +93,10 @@ def test_unsupported_refuses_loudly():
         T.InitialValueSolver(problem, T.RK222(), dt=0.01)
 
 
-def rbc_problem(dtype, Lx, Lz, Nx, Nz, kappa, nu, dt=0.01, stepper=None):
+def rbc_problem(dtype, Lx, Lz, Nx, Nz, kappa, nu, dt=0.01, stepper=None, bcs=("Lz", "0"), params=None):
     """The first-order tau RBC problem of test/test_subproblem_rk.jl:27-56 through the
-    API mirror; returns (solver, b, u, xbasis, zbasis)."""
+    API mirror (wall values of b: `bcs`, numbers or parameter names); returns
+    (solver, b, u, xbasis, zbasis)."""
     coords = T.CartesianCoordinates("x", "z")
     dist = T.Distributor(coords, dtype=dtype, device=T.GPU())
     xbasis = T.RealFourier(coords["x"], size=Nx, bounds=(0.0, Lx), dealias=3 / 2)
@@ -113,11 +114,12 @@ def rbc_problem(dtype, Lx, Lz, Nx, Nz, kappa, nu, dt=0.01, stepper=None):
     grad_u = T.grad(u) + ez * tau_lift(tau_u1)
     grad_b = T.grad(b) + ez * tau_lift(tau_b1)
     problem = T.IVP([p, b, u, tau_p, tau_b1, tau_b2, tau_u1, tau_u2])
-    T.add_parameters_(problem, kappa=kappa, nu=nu, Lz=Lz, ez=ez, grad_u=grad_u, grad_b=grad_b, τ_lift=tau_lift)
+    T.add_parameters_(problem, kappa=kappa, nu=nu, Lz=Lz, ez=ez, grad_u=grad_u, grad_b=grad_b, τ_lift=tau_lift,
+                      **(params or {}))
     T.add_equation_(problem, "trace(grad_u) + tau_p = 0")
     T.add_equation_(problem, "∂t(b) - kappa*div(grad_b) + τ_lift(tau_b2) = -u⋅∇(b)")
     T.add_equation_(problem, "∂t(u) - nu*div(grad_u) + ∇(p) - b*ez + τ_lift(tau_u2) = -u⋅∇(u)")
-    for bc in ("b(z=0) = Lz", "u(z=0) = 0", "b(z=Lz) = 0", "u(z=Lz) = 0", "integ(p) = 0"):
+    for bc in (f"b(z=0) = {bcs[0]}", "u(z=0) = 0", f"b(z=Lz) = {bcs[1]}", "u(z=Lz) = 0", "integ(p) = 0"):
         T.add_bc_(problem, bc)
     solver = T.InitialValueSolver(problem, stepper or T.RK222(), dt=dt)
     return solver, b, u, xbasis, zbasis

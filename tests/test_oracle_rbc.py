@@ -185,3 +185,42 @@ def test_rbc_sbdf2_variable_step_order(oracle):
         exact = 1 - z + eps * np.exp(-kappa * np.pi ** 2 * t) * np.sin(np.pi * z)
         errs.append(np.abs(b - exact[None, :]).max() / eps)
     assert errs[0] < 2e-5 and 3.3 < errs[0] / errs[1] < 4.7, errs
+
+
+def test_wall_array_bc_values(oracle):
+    """Per-mode wall values (orc_rbc_set_bc_values / orc_rbc3_set_bc_values): a constant wall
+    given as an array reproduces set_bc exactly, and an SBDF2 state -- one implicit solve per
+    step -- takes the wall arrays as its values on z = 0 and z = Lz (pins the mode mapping,
+    the unnormalised-rfft scaling and which row is which wall)."""
+    import numpy as np
+    nx, N, Lx, Lz, dt = 16, 14, 4.0, 1.0, 5e-3
+    kappa = nu = 1e-2
+    d = oracle.domain([oracle.REALFOURIER, oracle.CHEBYSHEVT], [nx, N], bounds=[(0.0, Lx), (0.0, Lz)])
+    z = Lz * (1 - np.cos(np.pi * np.arange(N) / (N - 1))) / 2
+    b0 = oracle.forward_transform(d, np.tile(Lz - z, (nx, 1)))
+    a = oracle.RBC2D(nx, N, Lx, Lz, kappa, nu, dt, "SBDF2")
+    c = oracle.RBC2D(nx, N, Lx, Lz, kappa, nu, dt, "SBDF2")
+    for s in (a, c):
+        s.set(oracle.RBC_B, b0)
+    a.set_bc(1.5, -0.25)
+    c.set_bc_values(np.fft.rfft(np.full(nx, 1.5)), np.fft.rfft(np.full(nx, -0.25)))
+    a.step(3)
+    c.step(3)
+    assert np.array_equal(a.get(oracle.RBC_B), c.get(oracle.RBC_B))
+    x = Lx * np.arange(nx) / nx
+    g0, g1 = 1 + 0.3 * np.cos(2 * np.pi * x / Lx), 0.2 * np.sin(4 * np.pi * x / Lx)
+    c.set_bc_values(np.fft.rfft(g0), np.fft.rfft(g1))
+    c.step(2)
+    grid = oracle.backward_transform(d, c.get(oracle.RBC_B))
+    assert np.abs(grid[:, 0] - g0).max() < 1e-12 and np.abs(grid[:, -1] - g1).max() < 1e-12
+    ny = 8
+    d3 = oracle.domain([oracle.REALFOURIER] * 2 + [oracle.CHEBYSHEVT], [nx, ny, N],
+                       bounds=[(0.0, Lx), (0.0, 2.0), (0.0, Lz)])
+    s3 = oracle.RBC3D(nx, ny, N, Lx, 2.0, Lz, kappa, nu, dt, "SBDF2")
+    s3.set(oracle.RBC3_B, oracle.forward_transform(d3, np.tile(Lz - z, (nx, ny, 1))))
+    X, Y = np.meshgrid(x, 2.0 * np.arange(ny) / ny, indexing="ij")
+    w0, w1 = 1 + 0.2 * np.cos(2 * np.pi * X / Lx) * np.sin(np.pi * Y), 0.1 * np.cos(np.pi * Y)
+    s3.set_bc_values(np.fft.fft(np.fft.rfft(w0, axis=0), axis=1), np.fft.fft(np.fft.rfft(w1, axis=0), axis=1))
+    s3.step(2)
+    g3 = oracle.backward_transform(d3, s3.get(oracle.RBC3_B))
+    assert np.abs(g3[:, :, 0] - w0).max() < 1e-12 and np.abs(g3[:, :, -1] - w1).max() < 1e-12
