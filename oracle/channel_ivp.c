@@ -594,6 +594,10 @@ void orc_rbc_set_field(orc_rbc *s, int var, const double *coeff) {
     s->iter = 0; /* a new initial condition restarts the multistep history */
     s->t = 0;
 }
+/* as orc_rbc_set_field, keeping the clock and the multistep history (a state changed between steps) */
+void orc_rbc_update_field(orc_rbc *s, int var, const double *coeff) {
+    memcpy(s->f[var], coeff, sizeof(cplx) * (size_t)s->nxh * s->nz);
+}
 void orc_rbc_get_field(const orc_rbc *s, int var, double *coeff) {
     memcpy(coeff, s->f[var], sizeof(cplx) * (size_t)s->nxh * s->nz);
 }

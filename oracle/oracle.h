@@ -98,6 +98,7 @@ void orc_rbc_set_bc(orc_rbc *s, double b_bottom, double b_top);
 void orc_rbc_set_bc_values(orc_rbc *s, const double *g0, const double *g1);
 /* coefficient arrays (nx/2+1, nz) complex, reference layout */
 void orc_rbc_set_field(orc_rbc *s, int var, const double *coeff);
+void orc_rbc_update_field(orc_rbc *s, int var, const double *coeff);  /* keeps clock and history */
 void orc_rbc_get_field(const orc_rbc *s, int var, double *coeff);
 /* tau values per mode: 0 tau_p, 1 tau_b1, 2 tau_b2, 3 tau_u1.x, 4 tau_u1.z, 5 tau_u2.x, 6 tau_u2.z */
 void orc_rbc_get_tau(const orc_rbc *s, int t, double *vals);

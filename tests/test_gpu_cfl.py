@@ -139,11 +139,50 @@ def test_rbc_adaptive_dt_matches_oracle(gpu, oracle):
             # the controller leaves u in grid layout, so the next step gathers it back
             # through a forward transform (ensure_layout! :g then :c), which drops what the
             # grid cannot hold (the imaginary part of the Nyquist column)
-            ref.set(oracle.RBC_UX, oracle.forward_transform(d, gx))
-            ref.set(oracle.RBC_UZ, oracle.forward_transform(d, gz))
+            ref.update(oracle.RBC_UX, oracle.forward_transform(d, gx))
+            ref.update(oracle.RBC_UZ, oracle.forward_transform(d, gz))
         ref.set_dt(dt)
         ref.step(1)
     assert relerr(np.array(dts), np.array(ref_dts)) < 1e-10
+    assert relerr(b["c"].cpu().numpy(), ref.get(oracle.RBC_B)) < 1e-12
+    assert relerr(u[0]["c"].cpu().numpy(), ref.get(oracle.RBC_UX)) < 1e-11
+    assert relerr(u[1]["c"].cpu().numpy(), ref.get(oracle.RBC_UZ)) < 1e-11
+    solver.close()
+
+
+def test_rbc_sbdf2_adaptive_dt_matches_oracle(gpu, oracle):
+    """SBDF2 under the controller: every committed dt makes the step variable (ratio
+    w = dt/dt_prev) and refactors twice; the history must survive the per-step field
+    gathers.  Same dt sequence and grid round trip on the oracle, 1e-12 relative."""
+    import tarang_amd as T
+    from tests.test_gpu_api import rbc_problem
+    print("RBC SBDF2 under the CFL controller", flush=True)
+    Lx, Lz, Nx, Nz = 4.0, 1.0, 16, 12
+    kappa = nu = (1e5) ** -0.5
+    solver, b, u, xb, zb = rbc_problem(np.float64, Lx, Lz, Nx, Nz, kappa, nu, dt=0.01, stepper=T.SBDF2())
+    x, z, ux, uz = cellular_state(Nx, Nz, Lx, Lz, 20.0)
+    bg = (Lz - z)[None, :] + 0.01 * np.cos(2 * np.pi * x / Lx)[:, None] * (z * (Lz - z))[None, :]
+    b["g"] = bg
+    u[0]["g"] = ux
+    u[1]["g"] = uz
+    cfl = T.CFL(solver, initial_dt=0.01, safety=0.5, threshold=0.0, max_change=1.5, min_change=0.5)
+    T.add_velocity_(cfl, u)
+    dts = []
+    T.run_(solver, stop_iteration=7, cfl=cfl, callbacks=[(1, lambda s: dts.append(s.dt))])
+    assert len(set(dts)) > 3
+    ref = oracle.RBC2D(Nx, Nz, Lx, Lz, kappa, nu, 0.01, "SBDF2")
+    d = oracle.domain([oracle.REALFOURIER, oracle.CHEBYSHEVT], [Nx, Nz], bounds=[(0.0, Lx), (0.0, Lz)])
+    ref.set(oracle.RBC_B, oracle.forward_transform(d, bg))
+    ref.set(oracle.RBC_UX, oracle.forward_transform(d, ux))
+    ref.set(oracle.RBC_UZ, oracle.forward_transform(d, uz))
+    for it, dt in enumerate(dts):
+        if it > 0:  # the grid round trip of u that the controller causes (see above)
+            gx = oracle.backward_transform(d, ref.get(oracle.RBC_UX))
+            gz = oracle.backward_transform(d, ref.get(oracle.RBC_UZ))
+            ref.update(oracle.RBC_UX, oracle.forward_transform(d, gx))
+            ref.update(oracle.RBC_UZ, oracle.forward_transform(d, gz))
+        ref.set_dt(dt)
+        ref.step(1)
     assert relerr(b["c"].cpu().numpy(), ref.get(oracle.RBC_B)) < 1e-12
     assert relerr(u[0]["c"].cpu().numpy(), ref.get(oracle.RBC_UX)) < 1e-11
     assert relerr(u[1]["c"].cpu().numpy(), ref.get(oracle.RBC_UZ)) < 1e-11
