@@ -555,6 +555,10 @@ void orc_rbc3_set_field(orc_rbc3 *s, int var, const double *coeff) {
     s->iter = 0;
     s->t = 0;
 }
+/* as orc_rbc3_set_field, keeping the clock and the multistep history */
+void orc_rbc3_update_field(orc_rbc3 *s, int var, const double *coeff) {
+    memcpy(s->f[var], coeff, sizeof(cplx) * (size_t)s->nmodes * s->nz);
+}
 void orc_rbc3_get_field(const orc_rbc3 *s, int var, double *coeff) {
     memcpy(coeff, s->f[var], sizeof(cplx) * (size_t)s->nmodes * s->nz);
 }

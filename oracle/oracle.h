@@ -123,6 +123,7 @@ void orc_rbc3_set_bc(orc_rbc3 *s, double b_bottom, double b_top);
 void orc_rbc3_set_bc_values(orc_rbc3 *s, const double *g0, const double *g1);
 /* coefficient arrays (nx/2+1, ny, nz) complex, reference layout */
 void orc_rbc3_set_field(orc_rbc3 *s, int var, const double *coeff);
+void orc_rbc3_update_field(orc_rbc3 *s, int var, const double *coeff);  /* keeps clock and history */
 void orc_rbc3_get_field(const orc_rbc3 *s, int var, double *coeff);
 /* taus: 0 tau_p, 1 tau_b1, 2 tau_b2, 3-5 tau_u1.{x,y,z}, 6-8 tau_u2.{x,y,z}; (nx/2+1, ny) values */
 void orc_rbc3_get_tau(const orc_rbc3 *s, int t, double *vals);

@@ -101,6 +101,7 @@ def lib():
         L.orc_rbc3_set_bc.argtypes = [vp, ctypes.c_double, ctypes.c_double]
         L.orc_rbc3_set_bc_values.argtypes = [vp, dp, dp]
         L.orc_rbc3_set_field.argtypes = [vp, ctypes.c_int, dp]
+        L.orc_rbc3_update_field.argtypes = [vp, ctypes.c_int, dp]
         L.orc_rbc3_get_field.argtypes = [vp, ctypes.c_int, dp]
         L.orc_rbc3_get_tau.argtypes = [vp, ctypes.c_int, dp]
         L.orc_rbc3_step.argtypes = [vp, ctypes.c_int]
@@ -410,6 +411,11 @@ class RBC3D:
     def set(self, var, coeff):
         c = _fortran(coeff, np.complex128)
         lib().orc_rbc3_set_field(self.h, var, _dp(c))
+
+    def update(self, var, coeff):
+        """Replace a field keeping the clock and the multistep history."""
+        c = _fortran(coeff, np.complex128)
+        lib().orc_rbc3_update_field(self.h, var, _dp(c))
 
     def get(self, var):
         out = np.zeros(self.shape, np.complex128, order="F")

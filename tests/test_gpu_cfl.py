@@ -187,3 +187,68 @@ def test_rbc_sbdf2_adaptive_dt_matches_oracle(gpu, oracle):
     assert relerr(u[0]["c"].cpu().numpy(), ref.get(oracle.RBC_UX)) < 1e-11
     assert relerr(u[1]["c"].cpu().numpy(), ref.get(oracle.RBC_UZ)) < 1e-11
     solver.close()
+
+
+def test_rbc3d_adaptive_dt_matches_oracle(gpu, oracle):
+    """The 3-D channel (SBDF2) under the controller with a three-component velocity:
+    per-step gathers keep the history (tarang_chan3_update_field), every dt change
+    refactors the per-key systems; oracle replay with the same dt sequence, 1e-12."""
+    import tarang_amd as T
+    print("RBC 3-D under the CFL controller", flush=True)
+    Lx, Ly, Lz, Nx, Ny, Nz = 4.0, 2.0, 1.0, 16, 8, 10
+    kappa = nu = 1e-2
+    coords = T.CartesianCoordinates("x", "y", "z")
+    dist = T.Distributor(coords, dtype=np.float64, device=T.GPU())
+    xb = T.RealFourier(coords["x"], size=Nx, bounds=(0.0, Lx), dealias=3 / 2)
+    yb = T.RealFourier(coords["y"], size=Ny, bounds=(0.0, Ly), dealias=3 / 2)
+    zb = T.ChebyshevT(coords["z"], size=Nz, bounds=(0.0, Lz), dealias=3 / 2)
+    domain = T.Domain(dist, (xb, yb, zb))
+    p, b, u = T.ScalarField(domain, "p"), T.ScalarField(domain, "b"), T.VectorField(domain, "u")
+    taus = [T.ScalarField(dist, "tau_p", (), np.float64), T.ScalarField(dist, "tau_b1", (xb, yb), np.float64),
+            T.ScalarField(dist, "tau_b2", (xb, yb), np.float64),
+            T.VectorField(dist, coords, "tau_u1", (xb, yb), np.float64),
+            T.VectorField(dist, coords, "tau_u2", (xb, yb), np.float64)]
+    ex, ey, ez = T.unit_vector_fields(coords, dist)
+    tau_lift = lambda A: T.lift(A, T.derivative_basis(zb, 1), -1)  # noqa: E731
+    problem = T.IVP([p, b, u] + taus)
+    T.add_parameters_(problem, kappa=kappa, nu=nu, Lz=Lz, ez=ez, τ_lift=tau_lift,
+                      grad_u=T.grad(u) + ez * tau_lift(taus[3]), grad_b=T.grad(b) + ez * tau_lift(taus[1]))
+    T.add_equation_(problem, "trace(grad_u) + tau_p = 0")
+    T.add_equation_(problem, "∂t(b) - kappa*div(grad_b) + τ_lift(tau_b2) = -u⋅∇(b)")
+    T.add_equation_(problem, "∂t(u) - nu*div(grad_u) + ∇(p) - b*ez + τ_lift(tau_u2) = -u⋅∇(u)")
+    for bc in ("b(z=0) = Lz", "u(z=0) = 0", "b(z=Lz) = 0", "u(z=Lz) = 0", "integ(p) = 0"):
+        T.add_bc_(problem, bc)
+    solver = T.InitialValueSolver(problem, T.SBDF2(), dt=0.01)
+    x = Lx * np.arange(Nx) / Nx
+    z = zb.grid()
+    g, dg = (z * (Lz - z)) ** 2, 2 * z * (Lz - z) * (Lz - 2 * z)
+    k = 2 * np.pi / Lx
+    A = 10.0
+    ux = -A * np.sin(k * x)[:, None, None] * np.ones(Ny)[None, :, None] * dg[None, None, :]
+    uz = A * k * np.cos(k * x)[:, None, None] * np.ones(Ny)[None, :, None] * g[None, None, :]
+    bg = np.broadcast_to((Lz - z)[None, None, :], (Nx, Ny, Nz)).copy()
+    b["g"] = bg
+    u[0]["g"] = ux
+    u[2]["g"] = uz
+    cfl = T.CFL(solver, initial_dt=0.01, safety=0.5, threshold=0.0, max_change=1.5, min_change=0.5)
+    T.add_velocity_(cfl, u)
+    dts = []
+    T.run_(solver, stop_iteration=5, cfl=cfl, callbacks=[(1, lambda s: dts.append(s.dt))])
+    assert len(set(dts)) > 2
+    ref = oracle.RBC3D(Nx, Ny, Nz, Lx, Ly, Lz, kappa, nu, 0.01, "SBDF2")
+    d = ref.d
+    ref.set(oracle.RBC3_B, oracle.forward_transform(d, bg))
+    ref.set(oracle.RBC3_UX, oracle.forward_transform(d, ux))
+    ref.set(oracle.RBC3_UZ, oracle.forward_transform(d, uz))
+    for it, dt in enumerate(dts):
+        if it > 0:  # the controller's grid round trip of u
+            for v in (oracle.RBC3_UX, oracle.RBC3_UY, oracle.RBC3_UZ):
+                ref.update(v, oracle.forward_transform(d, oracle.backward_transform(d, ref.get(v))))
+        ref.set_dt(dt)
+        ref.step(1)
+    assert relerr(b["c"].cpu().numpy(), ref.get(oracle.RBC3_B)) < 1e-12
+    refs = [ref.get(v) for v in (oracle.RBC3_UX, oracle.RBC3_UY, oracle.RBC3_UZ)]
+    scale = max(np.abs(r).max() for r in refs)  # uy stays ~0 for this y-independent flow
+    for i, r in enumerate(refs):
+        assert np.abs(u[i]["c"].cpu().numpy() - r).max() < 1e-11 * scale
+    solver.close()
