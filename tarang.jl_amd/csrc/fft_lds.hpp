@@ -265,6 +265,185 @@ __device__ inline void fft_lds(cplx<T>* buf, const cplx<T>* twl) {
     if constexpr (L > 1) fft_passes<T, L, NLINES, NT, INV, COMPOSITE>(buf, twl);
 }
 
+// ---- passes fed from and drained to registers ----
+// The first Stockham pass (NS = 1) needs no twiddles: butterfly j of a line takes
+// elements j + r*NB and writes j*R + r.  Taking its inputs straight from the caller's
+// loader, and handing the last pass's outputs straight to the caller's storer, saves
+// the LDS round trip (and two barriers) at each end.  In both, the lines vary fastest
+// over the threads, so a loader or storer whose lines are neighbouring columns of a
+// row-major array touches one contiguous piece of each row.
+
+// runs passes [P, PEND) of the plan in LDS
+template <typename T, int L, int NLINES, int NT, bool INV, int P, int NS, int PEND>
+__device__ inline void fft_mid_passes(cplx<T>* buf, const cplx<T>* twl) {
+    constexpr RadixPlan plan = radix_plan(L);
+    if constexpr (P < PEND) {
+        constexpr int R = plan.r[P];
+        stockham_pass<T, L, R, NS, NLINES, NT, INV>(buf, twl);
+        fft_mid_passes<T, L, NLINES, NT, INV, P + 1, NS * R, PEND>(buf, twl);
+    }
+}
+
+// first pass: loader -> DFT_R0 -> LDS, then a barrier (plans of two passes or more)
+template <typename T, int L, int NLINES, int NT, bool INV, typename Load>
+__device__ inline void fft_first_pass_regs(cplx<T>* buf, Load&& ld) {
+    using V = cplx<T>;
+    constexpr RadixPlan plan = radix_plan(L);
+    constexpr int LS = line_stride(L);
+    constexpr int R = plan.r[0];
+    constexpr int NB = L / R;
+    constexpr int TOT = NB * NLINES;
+    constexpr int PER = (TOT + NT - 1) / NT;
+    V v[PER][R];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int b = threadIdx.x + i * NT;
+        if (TOT % NT == 0 || b < TOT) {
+            const int line = b % NLINES, j = b / NLINES;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[i][r] = ld(line, j + r * NB);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int b = threadIdx.x + i * NT;
+        if (TOT % NT == 0 || b < TOT) {
+            const int line = b % NLINES, j = b / NLINES;
+            dft_small<R, INV, T>(v[i]);
+            V* dst = buf + line * LS;
+#pragma unroll
+            for (int r = 0; r < R; ++r) dst[pidx(j * R + r)] = v[i][r];
+        }
+    }
+    __syncthreads();
+}
+
+// FFT of NLINES lines whose first pass reads ld(line, i) (element i of the line) and
+// whose last pass ends in st(line, i, value).  twl must be filled before the call (the
+// first pass's barrier publishes it); buf is free on entry.  With FULL = false the last
+// pass also runs in LDS and the result is left in buf (after a barrier), st unused.
+template <typename T, int L, int NLINES, int NT, bool INV, bool FULL, typename Load, typename Store>
+__device__ inline void fft_regs_io(cplx<T>* buf, const cplx<T>* twl, Load&& ld, Store&& st) {
+    using V = cplx<T>;
+    constexpr RadixPlan plan = radix_plan(L);
+    static_assert(plan.n >= 1, "FFT length must factor into 2, 3 and 5");
+    constexpr int LS = line_stride(L);
+    if constexpr (plan.n == 1) {  // one butterfly per line: loader -> DFT_L -> storer
+        constexpr int PER = (NLINES + NT - 1) / NT;
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int line = threadIdx.x + i * NT;
+            if (NLINES % NT == 0 || line < NLINES) {
+                V v[L];
+#pragma unroll
+                for (int r = 0; r < L; ++r) v[r] = ld(line, r);
+                dft_small<L, INV, T>(v);
+#pragma unroll
+                for (int r = 0; r < L; ++r) {
+                    if constexpr (FULL) st(line, r, v[r]);
+                    else buf[line * LS + pidx(r)] = v[r];
+                }
+            }
+        }
+        if constexpr (!FULL) __syncthreads();
+    } else if constexpr (!FULL) {
+        fft_first_pass_regs<T, L, NLINES, NT, INV>(buf, ld);
+        fft_mid_passes<T, L, NLINES, NT, INV, 1, plan.r[0], plan.n>(buf, twl);
+    } else {
+        constexpr int RL = plan.r[plan.n - 1];
+        constexpr int NS = L / RL;  // points combined before the last pass
+        fft_first_pass_regs<T, L, NLINES, NT, INV>(buf, ld);
+        fft_mid_passes<T, L, NLINES, NT, INV, 1, plan.r[0], plan.n - 1>(buf, twl);
+        // last pass: LDS -> twiddles, DFT_RL -> storer; butterfly j < NS writes j + r*NS
+        constexpr int TOT = NS * NLINES;
+        constexpr int PER = (TOT + NT - 1) / NT;
+        V v[PER][RL];
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int b = threadIdx.x + i * NT;
+            if (TOT % NT == 0 || b < TOT) {
+                const int line = b % NLINES, j = b / NLINES;
+                const V* src = buf + line * LS;
+#pragma unroll
+                for (int r = 0; r < RL; ++r) v[i][r] = src[pidx(j + r * NS)];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int b = threadIdx.x + i * NT;
+            if (TOT % NT == 0 || b < TOT) {
+                const int line = b % NLINES, j = b / NLINES;
+#pragma unroll
+                for (int r = 1; r < RL; ++r) {
+                    const V w = twl[r * j];
+                    v[i][r] = INV ? cmulc(v[i][r], w) : cmul(v[i][r], w);
+                }
+                dft_small<RL, INV, T>(v[i]);
+#pragma unroll
+                for (int r = 0; r < RL; ++r) st(line, j + r * NS, v[i][r]);
+            }
+        }
+    }
+}
+
+// As fft_regs_io with FULL = true, except that each thread runs last-pass butterfly j of
+// every line and st(i, v) receives element i of all NLINES lines at once (v[line]).  A
+// barrier separates the last pass's LDS reads from the st calls, so st may write buf.
+template <typename T, int L, int NLINES, int NT, bool INV, typename Load, typename Store>
+__device__ inline void fft_regs_joint(cplx<T>* buf, const cplx<T>* twl, Load&& ld, Store&& st) {
+    using V = cplx<T>;
+    constexpr RadixPlan plan = radix_plan(L);
+    static_assert(plan.n >= 1, "FFT length must factor into 2, 3 and 5");
+    constexpr int LS = line_stride(L);
+    constexpr int RL = plan.r[plan.n - 1];
+    constexpr int NS = L / RL;  // points combined before the last pass (1: a single pass)
+    if constexpr (plan.n >= 2) {
+        fft_first_pass_regs<T, L, NLINES, NT, INV>(buf, ld);
+        fft_mid_passes<T, L, NLINES, NT, INV, 1, plan.r[0], plan.n - 1>(buf, twl);
+    }
+    constexpr int PER = (NS + NT - 1) / NT;
+    V v[PER][NLINES][RL];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int j = threadIdx.x + i * NT;
+        if (NS % NT == 0 || j < NS) {
+#pragma unroll
+            for (int line = 0; line < NLINES; ++line) {
+#pragma unroll
+                for (int r = 0; r < RL; ++r) {
+                    if constexpr (plan.n >= 2) v[i][line][r] = buf[line * LS + pidx(j + r * NS)];
+                    else v[i][line][r] = ld(line, r);
+                }
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int j = threadIdx.x + i * NT;
+        if (NS % NT == 0 || j < NS) {
+#pragma unroll
+            for (int line = 0; line < NLINES; ++line) {
+                if constexpr (plan.n >= 2) {
+#pragma unroll
+                    for (int r = 1; r < RL; ++r) {
+                        const V w = twl[r * j];
+                        v[i][line][r] = INV ? cmulc(v[i][line][r], w) : cmul(v[i][line][r], w);
+                    }
+                }
+                dft_small<RL, INV, T>(v[i][line]);
+            }
+#pragma unroll
+            for (int r = 0; r < RL; ++r) {
+                V o[NLINES];
+#pragma unroll
+                for (int line = 0; line < NLINES; ++line) o[line] = v[i][line][r];
+                st(j + r * NS, o);
+            }
+        }
+    }
+}
+
 // LDS bytes of NLINES padded lines plus the twiddle table
 template <typename T, int L, int NLINES>
 constexpr size_t fft_lds_bytes() {

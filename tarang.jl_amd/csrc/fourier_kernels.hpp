@@ -117,63 +117,34 @@ __global__ __launch_bounds__(NT) void k_ybwd(FourierParams<T> p, cplx<T>* __rest
     const int kx0 = (int)(lin / nf) * TK;
     const bool xeven = (p.nx % 2) == 0;
     load_twiddles<T, MY, NT>(twl, tw);
-    // 1) issue every source load of this thread before touching LDS
-    constexpr int NSRC = TK * NY;
-    constexpr int PS = (NSRC + NT - 1) / NT;
-    V va[PS], vb[PS];
-#pragma unroll
-    for (int e = 0; e < PS; ++e) {
-        const int idx = threadIdx.x + e * NT;
-        va[e] = vb[e] = V{T(0), T(0)};
-        if (NSRC % NT == 0 || idx < NSRC) {
-            const int c = idx % TK, s = idx / TK;
-            const int kx = kx0 + c;
-            if (kx < p.nxh) {
-                va[e] = Spec::value(f, p, kx, s);
-                if (kx == 0 || (xeven && kx == p.nx / 2)) vb[e] = Spec::value(f, p, kx, (NY - s) % NY);
-            }
-        }
-    }
-    // 2) zero the padding gap between +NY/2 and the negative frequencies
-    constexpr int GAP0 = NY / 2 + 1, GAP1 = EVEN ? MY - NY / 2 : MY - NN;  // [GAP0, GAP1)
-    for (int idx = threadIdx.x; idx < TK * (GAP1 - GAP0); idx += NT) {
-        const int c = idx % TK, d = GAP0 + idx / TK;
-        buf[c * LS + pidx(d)] = V{T(0), T(0)};
-    }
-    // 3) projection (kx = 0, Nx/2), Nyquist split, placement
-#pragma unroll
-    for (int e = 0; e < PS; ++e) {
-        const int idx = threadIdx.x + e * NT;
-        if (NSRC % NT == 0 || idx < NSRC) {
-            const int c = idx % TK, s = idx / TK;
-            const int kx = kx0 + c;
-            V a = va[e];
-            T w = T(1);
-            const bool xnyq = xeven && kx == p.nx / 2;
-            if (kx == 0 || xnyq) {
-                a = V{(a.x + vb[e].x) * T(0.5), (a.y - vb[e].y) * T(0.5)};
-                if (xnyq) w = T(0.5);
-            }
-            if (kx >= p.nxh) a = V{T(0), T(0)};
-            if (EVEN && s == NY / 2) {
-                w *= T(0.5);
-                const V h{a.x * w, a.y * w};
-                buf[c * LS + pidx(NY / 2)] = h;
-                buf[c * LS + pidx(MY - NY / 2)] = h;
-            } else {
-                const int d = s <= NY / 2 ? s : MY - (NY - s);
-                buf[c * LS + pidx(d)] = V{a.x * w, a.y * w};
-            }
-        }
-    }
-    __syncthreads();
-    fft_lds<T, MY, TK, NT, true>(buf, twl);
-    V* out = g + (size_t)f * gstride;
-    for (int idx = threadIdx.x; idx < TK * MY; idx += NT) {
-        const int c = idx % TK, d = idx / TK;
+    // padded position d of column c: the source row it takes (zero in the gap between
+    // +NY/2 and the negative frequencies), Hermitian projection at kx = 0 and kx = Nx/2,
+    // halving of the Nyquist row (placed at d = NY/2 and d = MY - NY/2) and column
+    constexpr int GAP1 = EVEN ? MY - NY / 2 : MY - NN;  // first negative-frequency slot
+    auto load = [&](int c, int d) -> V {
         const int kx = kx0 + c;
-        if (kx < p.nxh) out[(size_t)d * p.nxh + kx] = buf[c * LS + pidx(d)];
-    }
+        int s;
+        if (d <= NY / 2) s = d;
+        else if (d >= GAP1) s = NY - (MY - d);
+        else return V{T(0), T(0)};
+        if (kx >= p.nxh) return V{T(0), T(0)};
+        V a = Spec::value(f, p, kx, s);
+        T w = T(1);
+        const bool xnyq = xeven && kx == p.nx / 2;
+        if (kx == 0 || xnyq) {
+            const V b = Spec::value(f, p, kx, (NY - s) % NY);
+            a = V{(a.x + b.x) * T(0.5), (a.y - b.y) * T(0.5)};
+            if (xnyq) w = T(0.5);
+        }
+        if (EVEN && s == NY / 2) w *= T(0.5);
+        return V{a.x * w, a.y * w};
+    };
+    V* out = g + (size_t)f * gstride;
+    auto store = [&](int c, int d, V v) {
+        const int kx = kx0 + c;
+        if (kx < p.nxh) out[(size_t)d * p.nxh + kx] = v;
+    };
+    fft_regs_io<T, MY, TK, NT, true, true>(buf, twl, load, store);
 }
 
 // ---------------------------------------------------------------------------
@@ -193,59 +164,50 @@ __global__ __launch_bounds__(NT) void k_xpass(const cplx<T>* __restrict__ g, siz
     __shared__ V twl[MX];
     const int y0 = 2 * (int)xcd_remap(blockIdx.x, gridDim.x);
     load_twiddles<T, MX, NT>(twl, tw);
-    // the stored half spectrum (k < nxh) of every packed line, loaded before any LDS write
+    // element i of packed line (row, pair pp): a + i b at k = i < NXH (only the real
+    // parts at k = 0, which is all C2R keeps of the DC term), conj(a) + i conj(b) at the
+    // mirror i = MX - k, zero in between
     constexpr int NXH = NX / 2 + 1;
-    constexpr int NSRC = NL * NXH;
-    constexpr int PS = (NSRC + NT - 1) / NT;
-    V va[PS], vb[PS];
-#pragma unroll
-    for (int e = 0; e < PS; ++e) {
-        const int idx = threadIdx.x + e * NT;
-        if (NSRC % NT == 0 || idx < NSRC) {
-            const int line = idx / NXH, k = idx - line * NXH;
-            const int row = line / NP, pp = line - row * NP;
-            const size_t off = (size_t)(y0 + row) * nxh + k;
-            va[e] = g[(size_t)(2 * pp) * gstride + off];
-            vb[e] = g[(size_t)(2 * pp + 1) * gstride + off];
-        }
-    }
-    // zero the band between the stored modes and their mirrors: [NXH, MX - NXH]
-    constexpr int GAPN = MX - 2 * NXH + 1;
-    for (int idx = threadIdx.x; idx < NL * GAPN; idx += NT) {
-        const int line = idx / GAPN, k = NXH + (idx - line * GAPN);
-        buf[line * LS + pidx(k)] = V{T(0), T(0)};
-    }
-#pragma unroll
-    for (int e = 0; e < PS; ++e) {
-        const int idx = threadIdx.x + e * NT;
-        if (NSRC % NT == 0 || idx < NSRC) {
-            const int line = idx / NXH, k = idx - line * NXH;
-            const V a = va[e], b = vb[e];
-            V* L = buf + line * LS;
-            if (k == 0) {
-                L[pidx(0)] = V{a.x, b.x};  // C2R keeps only the real part of the DC term
-            } else {
-                L[pidx(k)] = V{a.x - b.y, a.y + b.x};           // a + i b
-                L[pidx(MX - k)] = V{a.x + b.y, b.x - a.y};      // conj(a) + i conj(b)
-            }
-        }
-    }
-    __syncthreads();
-    fft_lds<T, MX, NL, NT, true>(buf, twl);
-    for (int x = threadIdx.x; x < MX; x += NT) {
+    auto load = [&](int line, int i) -> V {
+        const int row = line / NP, pp = line - row * NP;
+        const bool mirror = i > MX - NXH;
+        if (i >= NXH && !mirror) return V{T(0), T(0)};
+        const int k = mirror ? MX - i : i;
+        const size_t off = (size_t)(y0 + row) * nxh + k;
+        const V a = g[(size_t)(2 * pp) * gstride + off], b = g[(size_t)(2 * pp + 1) * gstride + off];
+        if (k == 0) return V{a.x, b.x};
+        return mirror ? V{a.x + b.y, b.x - a.y} : V{a.x - b.y, a.y + b.x};
+    };
+    // the inverse transform's last pass hands over every line's value at grid point x:
+    // the product of row 0 and row 1, packed p0 + i p1 into line 0
+    auto product = [&](int x, const V* z) {
         T pr[2];
 #pragma unroll
         for (int row = 0; row < 2; ++row) {
             T v[Spec::NF];
 #pragma unroll
             for (int pp = 0; pp < NP; ++pp) {
-                const V z = buf[(row * NP + pp) * LS + pidx(x)];
-                v[2 * pp] = z.x;
-                v[2 * pp + 1] = z.y;
+                v[2 * pp] = z[row * NP + pp].x;
+                v[2 * pp + 1] = z[row * NP + pp].y;
             }
             pr[row] = Spec::combine(v) * scale;
         }
         buf[pidx(x)] = V{pr[0], pr[1]};
+    };
+    // the joint last pass holds NL x (last radix) values per thread: kept where that
+    // stays within 384 bytes, else the last pass runs in LDS and the product reads it there
+    constexpr RadixPlan plan = radix_plan(MX);
+    constexpr bool JOINT = NL * plan.r[plan.n - 1] * sizeof(V) <= 384;
+    if constexpr (JOINT) {
+        fft_regs_joint<T, MX, NL, NT, true>(buf, twl, load, product);
+    } else {
+        fft_regs_io<T, MX, NL, NT, true, false>(buf, twl, load, [](int, int, V) {});
+        for (int x = threadIdx.x; x < MX; x += NT) {
+            V z[NL];
+#pragma unroll
+            for (int line = 0; line < NL; ++line) z[line] = buf[line * LS + pidx(x)];
+            product(x, z);  // writes element x of line 0, read above by this thread only
+        }
     }
     __syncthreads();
     fft_lds<T, MX, 1, NT, false>(buf, twl);
@@ -280,23 +242,11 @@ __global__ __launch_bounds__(NT) void k_yfwd(const cplx<T>* __restrict__ h, int 
     const int tile = (int)xcd_remap(blockIdx.x, gridDim.x);
     const int kx0 = tile * TK;
     load_twiddles<T, MY, NT>(twl, tw);
-    constexpr int NSRC = TK * MY;
-    constexpr int PS = (NSRC + NT - 1) / NT;
-    V va[PS];
-#pragma unroll
-    for (int e = 0; e < PS; ++e) {
-        const int idx = threadIdx.x + e * NT;
-        const int c = idx % TK, d = idx / TK;
+    auto load = [&](int c, int d) -> V {
         const int kx = kx0 + c;
-        va[e] = ((NSRC % NT == 0 || idx < NSRC) && kx < nxh) ? h[(size_t)d * nxh + kx] : V{T(0), T(0)};
-    }
-#pragma unroll
-    for (int e = 0; e < PS; ++e) {
-        const int idx = threadIdx.x + e * NT;
-        if (NSRC % NT == 0 || idx < NSRC) buf[(idx % TK) * LS + pidx(idx / TK)] = va[e];
-    }
-    __syncthreads();
-    fft_lds<T, MY, TK, NT, false>(buf, twl);
+        return kx < nxh ? h[(size_t)d * nxh + kx] : V{T(0), T(0)};
+    };
+    fft_regs_io<T, MY, TK, NT, false, false>(buf, twl, load, [](int, int, V) {});
     const bool xeven = (nx % 2) == 0;
     for (int idx = threadIdx.x; idx < TK * NY; idx += NT) {
         const int c = idx % TK, ky = idx / TK;
