@@ -180,38 +180,29 @@ __global__ __launch_bounds__(NT) void k3_ybwd(const cplx<T>* __restrict__ G, Ch3
     const int m0 = (int)(lin % gridDim.x) * TK;
     const size_t plane = (size_t)geo.N * geo.nmodes;
     const V* in = G + (size_t)pl * plane + (size_t)z * geo.nmodes;
-    constexpr int GAP0 = NY / 2 + 1, GAP1 = MY - NY / 2;  // zero band [GAP0, GAP1)
-    for (int idx = threadIdx.x; idx < TK * (GAP1 - GAP0); idx += NT) {
-        const int c = idx % TK, j = GAP0 + idx / TK;
-        buf[c * LS + pidx(j)] = V{T(0), T(0)};
-    }
-    for (int idx = threadIdx.x; idx < TK * NY; idx += NT) {
-        const int c = idx % TK, n = idx / TK;
-        const int m = m0 + c;
-        V v = m < nxh ? in[(size_t)n * nxh + m] : V{T(0), T(0)};
-        V* L = buf + c * LS;
-        if (n < NY / 2) {
-            L[pidx(n)] = v;
-        } else if (n == NY / 2) {  // Nyquist split
-            v = V{v.x * T(0.5), v.y * T(0.5)};
-            L[pidx(NY / 2)] = v;
-            L[pidx(MY - NY / 2)] = v;
-        } else {
-            L[pidx(MY - NY + n)] = v;
-        }
-    }
-    __syncthreads();
-    fft_lds<T, MY, TK, NT, true, sizeof(T) == 8>(buf, twl);  // FP32: radix 8/4/2/3 measured faster here
     const bool xeven = (geo.nx % 2) == 0;
     cplx<T>* out = Q + (size_t)pl * geo.N * MY * nxh + (size_t)z * MY * nxh;
-    for (int idx = threadIdx.x; idx < TK * MY; idx += NT) {
-        const int c = idx % TK, j = idx / TK;
+    // padded position j: ky = j below NY/2, the Nyquist row halved at j = NY/2 and
+    // j = MY - NY/2, the negative ky above, zero in between
+    auto load = [&](int c, int j) -> V {
         const int m = m0 + c;
-        if (m >= nxh) continue;
-        V v = buf[c * LS + pidx(j)];
+        int n;
+        bool half = false;
+        if (j < NY / 2) n = j;
+        else if (j == NY / 2 || j == MY - NY / 2) { n = NY / 2; half = true; }
+        else if (j > MY - NY / 2) n = NY - MY + j;
+        else return V{T(0), T(0)};
+        if (m >= nxh) return V{T(0), T(0)};
+        const V v = in[(size_t)n * nxh + m];
+        return half ? V{v.x * T(0.5), v.y * T(0.5)} : v;
+    };
+    auto store = [&](int c, int j, V v) {
+        const int m = m0 + c;
+        if (m >= nxh) return;
         if (m == 0 || (xeven && m == geo.nx / 2)) v.y = T(0);  // real lines of the grid round trip
         out[(size_t)j * nxh + m] = v;
-    }
+    };
+    fft_regs_io<T, MY, TK, NT, true, true, false>(buf, twl, load, store);  // radix 8/4/2/3: fewer registers
 }
 
 // ---------------------------------------------------------------------------
@@ -234,26 +225,19 @@ __global__ __launch_bounds__(NT) void k3_xpass(const cplx<T>* __restrict__ Q, in
     if constexpr (TW_LDS) load_twiddles<T, MX, NT>(twl_lds, tw);
     const size_t row = (size_t)blockIdx.y * MY + blockIdx.x;  // (z, j)
     const size_t plane = (size_t)N * MY * NXH;
-    constexpr int GAPN = MX - 2 * NXH + 1;
-    for (int idx = threadIdx.x; idx < NL * GAPN; idx += NT) {
-        const int line = idx / GAPN, k = NXH + (idx - line * GAPN);
-        buf[line * LS + pidx(k)] = V{T(0), T(0)};
-    }
-    for (int idx = threadIdx.x; idx < NL * NXH; idx += NT) {
-        const int line = idx / NXH, k = idx - line * NXH;
+    // element i of packed line l (planes 2l, 2l+1): a + i b at k = i < NXH (real parts
+    // only at k = 0), conj(a) + i conj(b) at the mirror i = MX - k, zero in between
+    auto load = [&](int line, int i) -> V {
+        const bool mirror = i > MX - NXH;
+        if (i >= NXH && !mirror) return V{T(0), T(0)};
+        const int k = mirror ? MX - i : i;
         const size_t off = row * NXH + k;
         const V a = Q[(size_t)(2 * line) * plane + off];
         const V bb = (2 * line + 1 < kCh3Planes) ? Q[(size_t)(2 * line + 1) * plane + off] : V{T(0), T(0)};
-        V* L = buf + line * LS;
-        if (k == 0) {
-            L[pidx(0)] = V{a.x, bb.x};
-        } else {
-            L[pidx(k)] = V{a.x - bb.y, a.y + bb.x};
-            L[pidx(MX - k)] = V{a.x + bb.y, bb.x - a.y};
-        }
-    }
-    __syncthreads();
-    fft_lds<T, MX, NL, NT, true>(buf, twl);
+        if (k == 0) return V{a.x, bb.x};
+        return mirror ? V{a.x + bb.y, bb.x - a.y} : V{a.x - bb.y, a.y + bb.x};
+    };
+    fft_regs_io<T, MX, NL, NT, true, false>(buf, twl, load, [](int, int, V) {});
     // line l holds planes (2l, 2l+1): l0 = (ux, uy), l1 = (uz, dx b), l2 = (dy b, dz b),
     // l3 = (dx ux, dy ux), l4 = (dz ux, dx uy), l5 = (dy uy, dz uy), l6 = (dx uz, dy uz), l7 = (dz uz, 0)
     for (int x = threadIdx.x; x < MX; x += NT) {
@@ -303,16 +287,18 @@ __global__ __launch_bounds__(NT) void k3_yfwd(const cplx<T>* __restrict__ H, Ch3
     const int nxh = geo.nxh, z = (int)((lin / gridDim.x) % gridDim.y), o = (int)(lin / gxy);
     const int m0 = (int)(lin % gridDim.x) * TK;
     const V* in = H + (size_t)o * geo.N * MY * nxh + (size_t)z * MY * nxh;
-    for (int idx = threadIdx.x; idx < TK * MY; idx += NT) {
-        const int c = idx % TK, j = idx / TK;
+    auto load = [&](int c, int j) -> V {
         const int m = m0 + c;
-        buf[c * LS + pidx(j)] = m < nxh ? in[(size_t)j * nxh + m] : V{T(0), T(0)};
-    }
-    __syncthreads();
-    fft_lds<T, MY, TK, NT, false>(buf, twl);
+        return m < nxh ? in[(size_t)j * nxh + m] : V{T(0), T(0)};
+    };
+    fft_regs_io<T, MY, TK, NT, false, false>(buf, twl, load, [](int, int, V) {});
     // truncation (in place: entries n < NY/2 stay, n > NY/2 come from MY - NY + n, Nyquist folded)
-    V tr[(TK * NY + NT - 1) / NT];
-    for (int e = 0, idx = threadIdx.x; idx < TK * NY; ++e, idx += NT) {
+    constexpr int NTR = TK * NY, PT = (NTR + NT - 1) / NT;
+    V tr[PT];
+#pragma unroll
+    for (int e = 0; e < PT; ++e) {
+        const int idx = threadIdx.x + e * NT;
+        if (NTR % NT != 0 && idx >= NTR) break;
         const int c = idx % TK, n = idx / TK;
         const V* L = buf + c * LS;
         V v;
@@ -324,7 +310,10 @@ __global__ __launch_bounds__(NT) void k3_yfwd(const cplx<T>* __restrict__ H, Ch3
         tr[e] = v;
     }
     __syncthreads();
-    for (int e = 0, idx = threadIdx.x; idx < TK * NY; ++e, idx += NT) {
+#pragma unroll
+    for (int e = 0; e < PT; ++e) {
+        const int idx = threadIdx.x + e * NT;
+        if (NTR % NT != 0 && idx >= NTR) break;
         const int c = idx % TK, n = idx / TK;
         buf[c * LS + pidx(n)] = tr[e];
     }

@@ -274,21 +274,21 @@ __device__ inline void fft_lds(cplx<T>* buf, const cplx<T>* twl) {
 // row-major array touches one contiguous piece of each row.
 
 // runs passes [P, PEND) of the plan in LDS
-template <typename T, int L, int NLINES, int NT, bool INV, int P, int NS, int PEND>
+template <typename T, int L, int NLINES, int NT, bool INV, int P, int NS, int PEND, bool COMPOSITE = true>
 __device__ inline void fft_mid_passes(cplx<T>* buf, const cplx<T>* twl) {
-    constexpr RadixPlan plan = radix_plan(L);
+    constexpr RadixPlan plan = radix_plan(L, COMPOSITE);
     if constexpr (P < PEND) {
         constexpr int R = plan.r[P];
         stockham_pass<T, L, R, NS, NLINES, NT, INV>(buf, twl);
-        fft_mid_passes<T, L, NLINES, NT, INV, P + 1, NS * R, PEND>(buf, twl);
+        fft_mid_passes<T, L, NLINES, NT, INV, P + 1, NS * R, PEND, COMPOSITE>(buf, twl);
     }
 }
 
 // first pass: loader -> DFT_R0 -> LDS, then a barrier (plans of two passes or more)
-template <typename T, int L, int NLINES, int NT, bool INV, typename Load>
+template <typename T, int L, int NLINES, int NT, bool INV, bool COMPOSITE = true, typename Load>
 __device__ inline void fft_first_pass_regs(cplx<T>* buf, Load&& ld) {
     using V = cplx<T>;
-    constexpr RadixPlan plan = radix_plan(L);
+    constexpr RadixPlan plan = radix_plan(L, COMPOSITE);
     constexpr int LS = line_stride(L);
     constexpr int R = plan.r[0];
     constexpr int NB = L / R;
@@ -322,10 +322,12 @@ __device__ inline void fft_first_pass_regs(cplx<T>* buf, Load&& ld) {
 // whose last pass ends in st(line, i, value).  twl must be filled before the call (the
 // first pass's barrier publishes it); buf is free on entry.  With FULL = false the last
 // pass also runs in LDS and the result is left in buf (after a barrier), st unused.
-template <typename T, int L, int NLINES, int NT, bool INV, bool FULL, typename Load, typename Store>
+// COMPOSITE selects the radix plan as for fft_lds.
+template <typename T, int L, int NLINES, int NT, bool INV, bool FULL, bool COMPOSITE = true, typename Load,
+          typename Store>
 __device__ inline void fft_regs_io(cplx<T>* buf, const cplx<T>* twl, Load&& ld, Store&& st) {
     using V = cplx<T>;
-    constexpr RadixPlan plan = radix_plan(L);
+    constexpr RadixPlan plan = radix_plan(L, COMPOSITE);
     static_assert(plan.n >= 1, "FFT length must factor into 2, 3 and 5");
     constexpr int LS = line_stride(L);
     if constexpr (plan.n == 1) {  // one butterfly per line: loader -> DFT_L -> storer
@@ -347,13 +349,13 @@ __device__ inline void fft_regs_io(cplx<T>* buf, const cplx<T>* twl, Load&& ld, 
         }
         if constexpr (!FULL) __syncthreads();
     } else if constexpr (!FULL) {
-        fft_first_pass_regs<T, L, NLINES, NT, INV>(buf, ld);
-        fft_mid_passes<T, L, NLINES, NT, INV, 1, plan.r[0], plan.n>(buf, twl);
+        fft_first_pass_regs<T, L, NLINES, NT, INV, COMPOSITE>(buf, ld);
+        fft_mid_passes<T, L, NLINES, NT, INV, 1, plan.r[0], plan.n, COMPOSITE>(buf, twl);
     } else {
         constexpr int RL = plan.r[plan.n - 1];
         constexpr int NS = L / RL;  // points combined before the last pass
-        fft_first_pass_regs<T, L, NLINES, NT, INV>(buf, ld);
-        fft_mid_passes<T, L, NLINES, NT, INV, 1, plan.r[0], plan.n - 1>(buf, twl);
+        fft_first_pass_regs<T, L, NLINES, NT, INV, COMPOSITE>(buf, ld);
+        fft_mid_passes<T, L, NLINES, NT, INV, 1, plan.r[0], plan.n - 1, COMPOSITE>(buf, twl);
         // last pass: LDS -> twiddles, DFT_RL -> storer; butterfly j < NS writes j + r*NS
         constexpr int TOT = NS * NLINES;
         constexpr int PER = (TOT + NT - 1) / NT;
