@@ -57,6 +57,16 @@ __device__ inline void load_twiddles(cplx<T>* twl, const cplx<T>* __restrict__ t
     for (int m = threadIdx.x; m < L; m += NT) twl[m] = tw[m];
 }
 
+// Keep a twiddle table in LDS only when it does not lower the number of workgroups
+// whose NLINES lines fit in one CU's 160 KB; otherwise the passes read the global
+// table (L1/L2-resident) and the extra workgroup per CU is worth more.
+template <typename T, int L, int NLINES>
+constexpr bool twiddles_in_lds() {
+    constexpr size_t cap = 160 * 1024;
+    constexpr size_t lines = sizeof(cplx<T>) * (size_t)NLINES * line_stride(L), tw = sizeof(cplx<T>) * (size_t)L;
+    return cap / (lines + tw) >= cap / lines;
+}
+
 // ---- small DFT kernels on registers; INV selects exp(+...) ----
 template <int R, bool INV, typename T>
 __device__ inline void dft_small(cplx<T>* v);

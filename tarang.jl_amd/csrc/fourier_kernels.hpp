@@ -109,14 +109,16 @@ __global__ __launch_bounds__(NT) void k_ybwd(FourierParams<T> p, cplx<T>* __rest
     constexpr int LS = line_stride(MY);
     constexpr bool EVEN = (NY % 2) == 0;
     constexpr int NN = NY - NY / 2 - 1;  // negative frequencies kept
+    constexpr bool TW_LDS = twiddles_in_lds<T, MY, TK>();
     __shared__ V buf[TK * LS];
-    __shared__ V twl[MY];
+    __shared__ V twl_lds[TW_LDS ? MY : 1];
+    const V* twl = TW_LDS ? twl_lds : tw;
     const unsigned ntiles = gridDim.x, nf = gridDim.y;
     const unsigned lin = xcd_remap(blockIdx.x + blockIdx.y * ntiles, ntiles * nf);
     const int f = (int)(lin % nf);  // field fastest: one tile's fields are neighbours
     const int kx0 = (int)(lin / nf) * TK;
     const bool xeven = (p.nx % 2) == 0;
-    load_twiddles<T, MY, NT>(twl, tw);
+    if constexpr (TW_LDS) load_twiddles<T, MY, NT>(twl_lds, tw);
     // padded position d of column c: the source row it takes (zero in the gap between
     // +NY/2 and the negative frequencies), Hermitian projection at kx = 0 and kx = Nx/2,
     // halving of the Nyquist row (placed at d = NY/2 and d = MY - NY/2) and column
@@ -160,10 +162,12 @@ __global__ __launch_bounds__(NT) void k_xpass(const cplx<T>* __restrict__ g, siz
     constexpr int NP = Spec::NF / 2;
     constexpr int NL = 2 * NP;
     constexpr int LS = line_stride(MX);
+    constexpr bool TW_LDS = twiddles_in_lds<T, MX, NL>();
     __shared__ V buf[NL * LS];
-    __shared__ V twl[MX];
+    __shared__ V twl_lds[TW_LDS ? MX : 1];
+    const V* twl = TW_LDS ? twl_lds : tw;
     const int y0 = 2 * (int)xcd_remap(blockIdx.x, gridDim.x);
-    load_twiddles<T, MX, NT>(twl, tw);
+    if constexpr (TW_LDS) load_twiddles<T, MX, NT>(twl_lds, tw);
     // element i of packed line (row, pair pp): a + i b at k = i < NXH (only the real
     // parts at k = 0, which is all C2R keeps of the DC term), conj(a) + i conj(b) at the
     // mirror i = MX - k, zero in between
@@ -237,11 +241,13 @@ __global__ __launch_bounds__(NT) void k_yfwd(const cplx<T>* __restrict__ h, int 
                                              cplx<T>* __restrict__ F, const cplx<T>* __restrict__ tw) {
     using V = cplx<T>;
     constexpr int LS = line_stride(MY);
+    constexpr bool TW_LDS = twiddles_in_lds<T, MY, TK>();
     __shared__ V buf[TK * LS];
-    __shared__ V twl[MY];
+    __shared__ V twl_lds[TW_LDS ? MY : 1];
+    const V* twl = TW_LDS ? twl_lds : tw;
     const int tile = (int)xcd_remap(blockIdx.x, gridDim.x);
     const int kx0 = tile * TK;
-    load_twiddles<T, MY, NT>(twl, tw);
+    if constexpr (TW_LDS) load_twiddles<T, MY, NT>(twl_lds, tw);
     auto load = [&](int c, int d) -> V {
         const int kx = kx0 + c;
         return kx < nxh ? h[(size_t)d * nxh + kx] : V{T(0), T(0)};
