@@ -350,20 +350,29 @@ __device__ inline T lhat_at(const StageArgs<T>& a, size_t i) {
 
 template <typename T>
 __global__ void k_rk_stage(StageArgs<T> a, size_t n) {
+    using V = cplx<T>;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const T L = lhat_at(a, i);
-        cplx<T> y = a.X[i];
-        for (int j = 0; j < a.nterms; ++j) {
+        // every operand load is issued before the first use (at most 4 terms; the
+        // coefficients and pointers are uniform, so nothing diverges)
+        V f[4], yv[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            f[j] = (j < a.nterms && a.F[j]) ? a.F[j][i] : V{T(0), T(0)};
+            yv[j] = (j < a.nterms && a.Y[j]) ? a.Y[j][i] : V{T(0), T(0)};
+        }
+        V y = a.X[i];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j >= a.nterms) break;
             if (a.F[j]) {
-                const cplx<T> f = a.F[j][i];
-                y.x += a.dt * a.ae[j] * f.x;
-                y.y += a.dt * a.ae[j] * f.y;
+                y.x += a.dt * a.ae[j] * f[j].x;
+                y.y += a.dt * a.ae[j] * f[j].y;
             }
             if (a.Y[j]) {
-                const cplx<T> yj = a.Y[j][i];
                 const T c = -a.dt * a.ai[j] * L;
-                y.x += c * yj.x;
-                y.y += c * yj.y;
+                y.x += c * yv[j].x;
+                y.y += c * yv[j].y;
             }
         }
         if (a.gamma != T(0)) {
