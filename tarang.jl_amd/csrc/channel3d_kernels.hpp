@@ -237,7 +237,7 @@ __global__ __launch_bounds__(NT) void k3_xpass(const cplx<T>* __restrict__ Q, in
         if (k == 0) return V{a.x, bb.x};
         return mirror ? V{a.x + bb.y, bb.x - a.y} : V{a.x - bb.y, a.y + bb.x};
     };
-    fft_regs_io<T, MX, NL, NT, true, false>(buf, twl, load, [](int, int, V) {});
+    fft_regs_io<T, MX, NL, NT, true, false, true, false>(buf, twl, load, [](int, int, V) {});  // lines are rows
     // line l holds planes (2l, 2l+1): l0 = (ux, uy), l1 = (uz, dx b), l2 = (dy b, dz b),
     // l3 = (dx ux, dy ux), l4 = (dz ux, dx uy), l5 = (dy uy, dz uy), l6 = (dx uz, dy uz), l7 = (dz uz, 0)
     for (int x = threadIdx.x; x < MX; x += NT) {

@@ -279,9 +279,10 @@ __device__ inline void fft_lds(cplx<T>* buf, const cplx<T>* twl) {
 // The first Stockham pass (NS = 1) needs no twiddles: butterfly j of a line takes
 // elements j + r*NB and writes j*R + r.  Taking its inputs straight from the caller's
 // loader, and handing the last pass's outputs straight to the caller's storer, saves
-// the LDS round trip (and two barriers) at each end.  In both, the lines vary fastest
-// over the threads, so a loader or storer whose lines are neighbouring columns of a
-// row-major array touches one contiguous piece of each row.
+// the LDS round trip (and two barriers) at each end.  LINES_FAST = true gives
+// neighbouring threads neighbouring lines (for lines that are adjacent columns of a
+// row-major array, so a wave touches one contiguous piece of each row); false gives them
+// neighbouring elements of one line (for lines that are rows).
 
 // runs passes [P, PEND) of the plan in LDS
 template <typename T, int L, int NLINES, int NT, bool INV, int P, int NS, int PEND, bool COMPOSITE = true>
@@ -295,7 +296,8 @@ __device__ inline void fft_mid_passes(cplx<T>* buf, const cplx<T>* twl) {
 }
 
 // first pass: loader -> DFT_R0 -> LDS, then a barrier (plans of two passes or more)
-template <typename T, int L, int NLINES, int NT, bool INV, bool COMPOSITE = true, typename Load>
+template <typename T, int L, int NLINES, int NT, bool INV, bool COMPOSITE = true, bool LINES_FAST = true,
+          typename Load>
 __device__ inline void fft_first_pass_regs(cplx<T>* buf, Load&& ld) {
     using V = cplx<T>;
     constexpr RadixPlan plan = radix_plan(L, COMPOSITE);
@@ -309,7 +311,7 @@ __device__ inline void fft_first_pass_regs(cplx<T>* buf, Load&& ld) {
     for (int i = 0; i < PER; ++i) {
         const int b = threadIdx.x + i * NT;
         if (TOT % NT == 0 || b < TOT) {
-            const int line = b % NLINES, j = b / NLINES;
+            const int line = LINES_FAST ? b % NLINES : b / NB, j = LINES_FAST ? b / NLINES : b % NB;
 #pragma unroll
             for (int r = 0; r < R; ++r) v[i][r] = ld(line, j + r * NB);
         }
@@ -318,7 +320,7 @@ __device__ inline void fft_first_pass_regs(cplx<T>* buf, Load&& ld) {
     for (int i = 0; i < PER; ++i) {
         const int b = threadIdx.x + i * NT;
         if (TOT % NT == 0 || b < TOT) {
-            const int line = b % NLINES, j = b / NLINES;
+            const int line = LINES_FAST ? b % NLINES : b / NB, j = LINES_FAST ? b / NLINES : b % NB;
             dft_small<R, INV, T>(v[i]);
             V* dst = buf + line * LS;
 #pragma unroll
@@ -333,8 +335,8 @@ __device__ inline void fft_first_pass_regs(cplx<T>* buf, Load&& ld) {
 // first pass's barrier publishes it); buf is free on entry.  With FULL = false the last
 // pass also runs in LDS and the result is left in buf (after a barrier), st unused.
 // COMPOSITE selects the radix plan as for fft_lds.
-template <typename T, int L, int NLINES, int NT, bool INV, bool FULL, bool COMPOSITE = true, typename Load,
-          typename Store>
+template <typename T, int L, int NLINES, int NT, bool INV, bool FULL, bool COMPOSITE = true, bool LINES_FAST = true,
+          typename Load, typename Store>
 __device__ inline void fft_regs_io(cplx<T>* buf, const cplx<T>* twl, Load&& ld, Store&& st) {
     using V = cplx<T>;
     constexpr RadixPlan plan = radix_plan(L, COMPOSITE);
@@ -359,12 +361,12 @@ __device__ inline void fft_regs_io(cplx<T>* buf, const cplx<T>* twl, Load&& ld, 
         }
         if constexpr (!FULL) __syncthreads();
     } else if constexpr (!FULL) {
-        fft_first_pass_regs<T, L, NLINES, NT, INV, COMPOSITE>(buf, ld);
+        fft_first_pass_regs<T, L, NLINES, NT, INV, COMPOSITE, LINES_FAST>(buf, ld);
         fft_mid_passes<T, L, NLINES, NT, INV, 1, plan.r[0], plan.n, COMPOSITE>(buf, twl);
     } else {
         constexpr int RL = plan.r[plan.n - 1];
         constexpr int NS = L / RL;  // points combined before the last pass
-        fft_first_pass_regs<T, L, NLINES, NT, INV, COMPOSITE>(buf, ld);
+        fft_first_pass_regs<T, L, NLINES, NT, INV, COMPOSITE, LINES_FAST>(buf, ld);
         fft_mid_passes<T, L, NLINES, NT, INV, 1, plan.r[0], plan.n - 1, COMPOSITE>(buf, twl);
         // last pass: LDS -> twiddles, DFT_RL -> storer; butterfly j < NS writes j + r*NS
         constexpr int TOT = NS * NLINES;
@@ -374,7 +376,7 @@ __device__ inline void fft_regs_io(cplx<T>* buf, const cplx<T>* twl, Load&& ld, 
         for (int i = 0; i < PER; ++i) {
             const int b = threadIdx.x + i * NT;
             if (TOT % NT == 0 || b < TOT) {
-                const int line = b % NLINES, j = b / NLINES;
+                const int line = LINES_FAST ? b % NLINES : b / NS, j = LINES_FAST ? b / NLINES : b % NS;
                 const V* src = buf + line * LS;
 #pragma unroll
                 for (int r = 0; r < RL; ++r) v[i][r] = src[pidx(j + r * NS)];
@@ -384,7 +386,7 @@ __device__ inline void fft_regs_io(cplx<T>* buf, const cplx<T>* twl, Load&& ld, 
         for (int i = 0; i < PER; ++i) {
             const int b = threadIdx.x + i * NT;
             if (TOT % NT == 0 || b < TOT) {
-                const int line = b % NLINES, j = b / NLINES;
+                const int line = LINES_FAST ? b % NLINES : b / NS, j = LINES_FAST ? b / NLINES : b % NS;
 #pragma unroll
                 for (int r = 1; r < RL; ++r) {
                     const V w = twl[r * j];
@@ -401,7 +403,7 @@ __device__ inline void fft_regs_io(cplx<T>* buf, const cplx<T>* twl, Load&& ld, 
 // As fft_regs_io with FULL = true, except that each thread runs last-pass butterfly j of
 // every line and st(i, v) receives element i of all NLINES lines at once (v[line]).  A
 // barrier separates the last pass's LDS reads from the st calls, so st may write buf.
-template <typename T, int L, int NLINES, int NT, bool INV, typename Load, typename Store>
+template <typename T, int L, int NLINES, int NT, bool INV, bool LINES_FAST = true, typename Load, typename Store>
 __device__ inline void fft_regs_joint(cplx<T>* buf, const cplx<T>* twl, Load&& ld, Store&& st) {
     using V = cplx<T>;
     constexpr RadixPlan plan = radix_plan(L);
@@ -410,7 +412,7 @@ __device__ inline void fft_regs_joint(cplx<T>* buf, const cplx<T>* twl, Load&& l
     constexpr int RL = plan.r[plan.n - 1];
     constexpr int NS = L / RL;  // points combined before the last pass (1: a single pass)
     if constexpr (plan.n >= 2) {
-        fft_first_pass_regs<T, L, NLINES, NT, INV>(buf, ld);
+        fft_first_pass_regs<T, L, NLINES, NT, INV, true, LINES_FAST>(buf, ld);
         fft_mid_passes<T, L, NLINES, NT, INV, 1, plan.r[0], plan.n - 1>(buf, twl);
     }
     constexpr int PER = (NS + NT - 1) / NT;

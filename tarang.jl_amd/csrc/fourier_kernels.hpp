@@ -203,9 +203,9 @@ __global__ __launch_bounds__(NT) void k_xpass(const cplx<T>* __restrict__ g, siz
     constexpr RadixPlan plan = radix_plan(MX);
     constexpr bool JOINT = NL * plan.r[plan.n - 1] * sizeof(V) <= 384;
     if constexpr (JOINT) {
-        fft_regs_joint<T, MX, NL, NT, true>(buf, twl, load, product);
+        fft_regs_joint<T, MX, NL, NT, true, false>(buf, twl, load, product);  // lines are rows
     } else {
-        fft_regs_io<T, MX, NL, NT, true, false>(buf, twl, load, [](int, int, V) {});
+        fft_regs_io<T, MX, NL, NT, true, false, true, false>(buf, twl, load, [](int, int, V) {});
         for (int x = threadIdx.x; x < MX; x += NT) {
             V z[NL];
 #pragma unroll
