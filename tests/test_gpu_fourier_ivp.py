@@ -18,7 +18,7 @@ def turb_ic(N, seed=7, kpeak=4.0):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-@pytest.mark.parametrize("N", [16, 64, 256])
+@pytest.mark.parametrize("N", [16, 64, 256, 512, 1024])  # padded lengths 24, 96, 384, 768, 1536
 def test_turbulence_rhs_parity(gpu, oracle, dtype, N):
     print(f"turbulence rhs N={N} {dtype.__name__}", flush=True)
     d = oracle.domain([0, 0], [N, N])
