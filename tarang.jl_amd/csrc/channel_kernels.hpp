@@ -445,15 +445,17 @@ __global__ __launch_bounds__(NT) void k_ch_lx(const cplx<T>* __restrict__ p, con
         v[idx] = double2{(double)s.x, proj ? 0.0 : (double)s.y};
     }
     __syncthreads();
+    // Dz of the 4 fields: output k sums (N - k)/2 terms, so odd fields run k backwards
+    // and a thread's items pair short sums with long ones
     for (int idx = threadIdx.x; idx < 4 * N; idx += NT) {
-        const int f = idx / N, k = idx - f * N;
+        const int f = idx / N, r = idx - f * N, k = (f & 1) ? N - 1 - r : r;
         double2 acc{0, 0};
         for (int j = k + 1; j < N; j += 2) {
             const double c = cheb_dz(k, j, sz);
             acc.x += c * v[f * N + j].x;
             acc.y += c * v[f * N + j].y;
         }
-        d1[idx] = acc;
+        d1[f * N + k] = acc;
     }
     __syncthreads();
     double2 tau[7];
