@@ -48,8 +48,14 @@ constexpr bool fft_length_supported(int L) { return L >= 1 && radix_plan(L).n >=
 
 // padded LDS index of element i within a line
 __host__ __device__ constexpr int pidx(int i) { return i + (i >> 4); }
-// distance between consecutive lines of length L
-__host__ __device__ constexpr int line_stride(int L) { return pidx(L - 1) + 1 + ((pidx(L - 1) + 1) & 1); }
+// distance between consecutive lines of length L: at least the padded line, rounded up
+// to 16 mod 32 complex values, so that in FP32 two neighbouring lines start half the 64
+// LDS banks apart (passes whose neighbouring lanes take neighbouring lines then fill
+// complementary banks instead of all hitting the same ones)
+__host__ __device__ constexpr int line_stride(int L) {
+    const int b = pidx(L - 1) + 1;
+    return b + (48 - b % 32) % 32;
+}
 
 // copy the global twiddle table into LDS (caller synchronises before use)
 template <typename T, int L, int NT>
