@@ -10,8 +10,9 @@
 //
 // LDS layout: element i of a line lives at pidx(i) = i + i/16 (one pad slot
 // per 16 complex values), so the stride-R writes of the early passes do not
-// collide on LDS banks.  Lines are LINE_STRIDE apart.  Twiddles are read from
-// an LDS copy of tw[m] = exp(-2 pi i m/L) (built in FP64 on the host).
+// collide on LDS banks.  Lines are line_stride(L) apart.  Twiddles tw[m] =
+// exp(-2 pi i m/L) (built in FP64 on the host) are read from an LDS copy, or from the
+// global table where the copy would cost a workgroup slot per CU (twiddles_in_lds).
 //
 // Conventions: forward = exp(-2 pi i jk/L), inverse = exp(+2 pi i jk/L), both
 // unnormalised (the FFTW convention the reference relies on,
