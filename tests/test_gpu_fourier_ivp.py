@@ -72,6 +72,25 @@ def test_turbulence_steps_parity(gpu, oracle, ts, dtype):
     s.close()
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_turbulence_benchmark_configuration_steps(gpu, oracle, dtype):
+    """The benchmarked configuration itself: 1024^2, nu = 1e-4, dt = 1e-3, RK443, the
+    seeded E(k) ~ k^4 exp(-2 (k/10)^2) initial vorticity with E = 0.5; two steps through
+    the captured step graph against the oracle."""
+    N, nu, dt = 1024, 1e-4, 1e-3
+    print(f"turbulence benchmark configuration {dtype.__name__}", flush=True)
+    d = oracle.domain([0, 0], [N, N])
+    C = oracle.forward_transform(d, configs.turbulence_2d(N, seed=42, kpeak=10.0, energy=0.5))
+    ref = oracle.FourierIVP(d, oracle.PROB_TURB2D, "RK443", nu, dt)
+    ref.set_coeff(C)
+    s = gpu.fivp([N, N], [2 * np.pi, 2 * np.pi], TURB, TS["RK443"], nu, dt, dtype=dtype)
+    s.set_state(C)
+    ref.step(2)
+    s.step(2)
+    assert relerr(s.get_state(), ref.get_coeff()) < TOL[dtype]
+    s.close()
+
+
 def test_graph_replay_matches_eager(gpu, oracle):
     N = 64
     d = oracle.domain([0, 0], [N, N])
