@@ -123,6 +123,28 @@ def test_channel_large_nz_parity(gpu, oracle, ts, dtype):
     s.close()
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_channel_benchmark_x_size_steps(gpu, oracle, dtype):
+    """The benchmarked 2-D configuration's x size and parameters (Nx = 1024, so the
+    x pass runs at M = 1536; Lx = 4, Ra = 2e6, Pr = 1, RK222, dt = 0.01) at Nz = 64,
+    where the oracle's per-mode factorisations take about a second: two steps."""
+    nx, N, Lx, Lz, dt = 1024, 64, 4.0, 1.0, 1e-2
+    print(f"channel benchmark x size {dtype.__name__}", flush=True)
+    kappa, nu = params(Ra=2e6, Pr=1.0)
+    st = random_state(oracle, nx, N, Lx, Lz, seed=17)
+    ref = oracle.RBC2D(nx, N, Lx, Lz, kappa, nu, dt, "RK222")
+    s = gpu.chan(nx, N, Lx, Lz, kappa, nu, dt, stepper=TS["RK222"], dtype=dtype)
+    for var, c in st.items():
+        ref.set(var, c)
+        s.set(var, c)
+    ref.step(2)
+    s.step(2)
+    tol = TOL[dtype] * (10 if dtype == np.float32 else 1)
+    for var in (oracle.RBC_B, oracle.RBC_UX, oracle.RBC_UZ):
+        assert relerr(s.get(var), ref.get(var)) < tol, var
+    s.close()
+
+
 def test_channel_graph_matches_eager(gpu, oracle):
     nx, N, Lx, Lz, dt = 32, 16, 4.0, 1.0, 5e-3
     kappa, nu = params()
