@@ -312,13 +312,19 @@ int tarang_chan3_kernel_bytes(tarang_chan3_t s, double* bytes8);
 /* Serving loop: continuous batching of IVP solve requests on one GPU.  No reference
  * counterpart (the reference runs one IVP per process).  Requests arrive at given
  * times (seconds after tarang_serve_run starts); an arrived request is admitted when
- * its device footprint fits mem_budget beside the resident ones, is created on the
- * least-loaded of `lanes` HIP streams, keeps its state and factored systems in HBM
- * and advances chunk_steps steps at a time, overlapping with the other active
+ * one of `lanes` slots is free and its device footprint fits mem_budget beside the
+ * resident ones (a later request may go ahead of one that does not fit yet, at most 8
+ * times per waiting request).  Each admitted request gets its own HIP stream; its
+ * solver is built there off the loop thread, it keeps its state and factored systems
+ * in HBM and advances chunk_steps steps at a time, overlapping with the other active
  * requests.  A finished request's final state is copied to `out` and its memory
  * released, which admits waiting requests.  A request whose footprint exceeds the
- * budget is rejected (status -1).  ic / out are device pointers in the reference
- * layout: periodic: [0] state; 2-D channel: p, b, ux, uz; 3-D channel: p, b, ux, uy, uz
+ * budget is rejected (status -1).  A request that fails (a singular per-mode system,
+ * an error in set-up or in a chunk, a NaN or Inf in its state after a chunk) gets
+ * status -2 and a message (tarang_serve_record_error); its memory is freed and the
+ * other requests continue.  An unsupported descriptor is refused by
+ * tarang_serve_submit.  ic / out are device pointers in the reference layout:
+ * periodic: [0] state; 2-D channel: p, b, ux, uz; 3-D channel: p, b, ux, uy, uz
  * (a NULL ic entry keeps that field zero). */
 #define TARANG_SERVE_PERIODIC 0
 #define TARANG_SERVE_CHANNEL 1
@@ -339,9 +345,14 @@ int tarang_serve_destroy(tarang_server_t s);
 int tarang_serve_submit(tarang_server_t s, const tarang_serve_request* r, int* id);
 /* process every submitted request; returns when all are done or rejected */
 int tarang_serve_run(tarang_server_t s);
-/* times3 = (arrival, admitted, done) in seconds (-1 if never admitted); status 2 done, -1 rejected */
+/* times3 = (arrival, admitted, done) in seconds (-1 if never admitted); lane = the slot;
+ * status 2 done, -1 rejected, -2 failed */
 int tarang_serve_record(tarang_server_t s, int id, double* times3, double* grid_point_steps, size_t* footprint,
                         int* lane, int* status);
+/* steps completed and the failure message of a request ("" unless it failed or was rejected) */
+int tarang_serve_record_error(tarang_server_t s, int id, int* steps_done, char* msg, size_t cap);
+/* device bytes a request's solver holds (the admission footprint) */
+int tarang_serve_footprint(const tarang_serve_request* r, size_t* bytes);
 
 #ifdef __cplusplus
 }

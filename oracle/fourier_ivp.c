@@ -265,6 +265,10 @@ static void step_sbdf(orc_fivp *p) {
     free(Fn);
 }
 
+/* a new step size for the following steps; the next SBDF2 step uses the ratio
+ * w = dt / dt_prev with dt_prev the last step's size (step_diagonal_imex.jl:331-437) */
+void orc_fivp_set_dt(orc_fivp *p, double dt) { p->dt = dt; }
+
 void orc_fivp_step(orc_fivp *p, int nsteps) {
     for (int n = 0; n < nsteps; ++n) {
         if (p->ts == ORC_TS_SBDF2 || p->ts == ORC_TS_SBDF1) step_sbdf(p);

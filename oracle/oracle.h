@@ -77,6 +77,7 @@ void orc_fivp_destroy(orc_fivp *p);
 void orc_fivp_set_coeff(orc_fivp *p, const double *coeff);
 void orc_fivp_get_coeff(const orc_fivp *p, double *coeff);
 void orc_fivp_step(orc_fivp *p, int nsteps);
+void orc_fivp_set_dt(orc_fivp *p, double dt);
 /* RHS F(X) in coefficient space for the current state (one evaluation) */
 void orc_fivp_rhs(orc_fivp *p, const double *coeff, double *F);
 double orc_fivp_time(const orc_fivp *p);

@@ -71,6 +71,7 @@ def lib():
         L.orc_fivp_set_coeff.argtypes = [ctypes.c_void_p, dp]
         L.orc_fivp_get_coeff.argtypes = [ctypes.c_void_p, dp]
         L.orc_fivp_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_fivp_set_dt.argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.orc_fivp_rhs.argtypes = [ctypes.c_void_p, dp, dp]
         L.orc_fivp_time.argtypes = [ctypes.c_void_p]
         L.orc_fivp_time.restype = ctypes.c_double
@@ -279,6 +280,9 @@ class FourierIVP:
 
     def step(self, n=1):
         lib().orc_fivp_step(self.h, int(n))
+
+    def set_dt(self, dt):
+        lib().orc_fivp_set_dt(self.h, float(dt))
 
     @property
     def time(self):

@@ -1,4 +1,6 @@
 // capi.cpp -- extern "C" entry points declared in include/tarang_hip.h.
+#include <algorithm>
+#include <cstring>
 #include <exception>
 #include <memory>
 #include <new>
@@ -708,6 +710,17 @@ int tarang_serve_submit(tarang_server_t s, const tarang_serve_request* r, int* i
     });
 }
 
+int tarang_serve_footprint(const tarang_serve_request* r, size_t* bytes) {
+    return guard([&] {
+        NEED(r, "request");
+        NEED(bytes, "bytes");
+        if (r->kind == TARANG_SERVE_PERIODIC) *bytes = tarang::footprint(conv_desc(&r->fivp));
+        else if (r->kind == TARANG_SERVE_CHANNEL) *bytes = tarang::footprint(conv_desc(&r->chan));
+        else if (r->kind == TARANG_SERVE_CHANNEL3) *bytes = tarang::footprint(conv_desc(&r->chan3));
+        else TG_REQUIRE(false, tarang::kErrInvalid, "serving: unknown request kind");
+    });
+}
+
 int tarang_serve_run(tarang_server_t s) {
     return guard([&] {
         NEED(s, "server");
@@ -730,6 +743,20 @@ int tarang_serve_record(tarang_server_t s, int id, double* times3, double* grid_
         if (footprint) *footprint = r.footprint;
         if (lane) *lane = r.lane;
         if (status) *status = r.status;
+    });
+}
+
+int tarang_serve_record_error(tarang_server_t s, int id, int* steps_done, char* msg, size_t cap) {
+    return guard([&] {
+        NEED(s, "server");
+        TG_REQUIRE(id >= 0 && id < s->server->size(), tarang::kErrInvalid, "request id out of range");
+        const tarang::ServeRecord& r = s->server->record(id);
+        if (steps_done) *steps_done = r.steps_done;
+        if (msg && cap > 0) {
+            const size_t n = std::min(cap - 1, r.error.size());
+            std::memcpy(msg, r.error.data(), n);
+            msg[n] = '\0';
+        }
     });
 }
 

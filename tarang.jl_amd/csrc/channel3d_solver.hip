@@ -16,6 +16,7 @@
 
 #include "channel3d_kernels.hpp"
 #include "channel3d_solver.hpp"
+#include "flow_tools.hpp"
 #include "dispatch.hpp"
 
 namespace tarang {
@@ -28,7 +29,7 @@ bool size_ok(int n) {
 }
 }  // namespace
 
-Channel3Solver::Channel3Solver(Context* ctx, const Channel3Desc& d) : ctx_(ctx), d_(d) {
+void Channel3Solver::validate(const Channel3Desc& d) {
     TG_REQUIRE(d.dtype == kF32 || d.dtype == kF64, kErrInvalid, "3-D channel solver: dtype must be f32 or f64");
     TG_REQUIRE(d.stepper == kSBDF2 || d.stepper == kSBDF1 || d.stepper == kRK222 || d.stepper == kRK443, kErrInvalid,
                "3-D channel solver: timestepper must be SBDF2, SBDF1, RK222 or RK443");
@@ -40,6 +41,10 @@ Channel3Solver::Channel3Solver(Context* ctx, const Channel3Desc& d) : ctx_(ctx),
                "3-D channel solver: ChebyshevT size " + std::to_string(d.nz) + " unsupported (4..256)");
     TG_REQUIRE(d.Lx > 0 && d.Ly > 0 && d.Lz > 0 && d.dt > 0, kErrInvalid,
                "3-D channel solver: lengths and dt must be positive");
+}
+
+Channel3Solver::Channel3Solver(Context* ctx, const Channel3Desc& d) : ctx_(ctx), d_(d) {
+    validate(d);
     nxh_ = d.nx / 2 + 1;
     nmodes_ = nxh_ * d.ny;
     mx_ = padded_len(d.nx);
@@ -654,6 +659,11 @@ size_t footprint(const Channel3Desc& d) {
     // inverses in the state precision, plus the FP64 LU scratch while they are formed
     const size_t ent = 2 * keys * (2 * nb * nb + nu * nu);
     return fields + ent * (8 + dtype_size((DType)d.dtype)) + 2 * keys * (2 * nb + nu) * 4 + 3 * N * N * 8 + (1 << 20);
+}
+
+void Channel3Solver::flag_nonfinite(int* dflag) const {
+    for (const DevBuf* b : {&P_, &B_, &UX_, &UY_, &UZ_})
+        tarang::flag_nonfinite(ctx_, b->ptr, 2 * field_elems(), (DType)d_.dtype, dflag);
 }
 
 }  // namespace tarang

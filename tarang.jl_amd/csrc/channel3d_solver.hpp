@@ -27,6 +27,8 @@ enum Channel3Var : int { kC3P = 0, kC3B = 1, kC3UX = 2, kC3UY = 3, kC3UZ = 4 };
 
 class Channel3Solver {
   public:
+    // the descriptor checks of the constructor, without allocating (throws tarang::Error)
+    static void validate(const Channel3Desc& d);
     Channel3Solver(Context* ctx, const Channel3Desc& d);
     ~Channel3Solver();
     size_t field_elems() const { return (size_t)nmodes_ * d_.nz; }
@@ -42,6 +44,8 @@ class Channel3Solver {
     void get_tau(int t, void* dev);
     void rhs(void* Fb, void* Fux, void* Fuy, void* Fuz);
     void step(int nsteps);
+    // *dflag = 1 if the state holds a NaN or an infinity (device int, asynchronous)
+    void flag_nonfinite(int* dflag) const;
     void set_dt(double dt);
     void enable_graph(bool on) { graph_on_ = on; }
     double time() const { return t_; }

@@ -15,6 +15,7 @@
 
 #include "channel_kernels.hpp"
 #include "channel_solver.hpp"
+#include "flow_tools.hpp"
 #include "dispatch.hpp"
 
 namespace tarang {
@@ -29,7 +30,7 @@ bool channel_nx_ok(int nx, bool f64) {
 }
 }  // namespace
 
-ChannelSolver::ChannelSolver(Context* ctx, const ChannelDesc& d) : ctx_(ctx), d_(d) {
+void ChannelSolver::validate(const ChannelDesc& d) {
     TG_REQUIRE(d.dtype == kF32 || d.dtype == kF64, kErrInvalid, "channel solver: dtype must be f32 or f64");
     TG_REQUIRE(d.stepper == kRK222 || d.stepper == kRK443 || d.stepper == kSBDF2 || d.stepper == kSBDF1,
                kErrInvalid, "channel solver: timestepper must be RK222, RK443, SBDF2 or SBDF1");
@@ -39,6 +40,10 @@ ChannelSolver::ChannelSolver(Context* ctx, const ChannelDesc& d) : ctx_(ctx), d_
     TG_REQUIRE(d.nz >= 4 && d.nz <= 512, kErrInvalid,
                "channel solver: ChebyshevT size " + std::to_string(d.nz) + " unsupported (4..512)");
     TG_REQUIRE(d.Lx > 0 && d.Lz > 0 && d.dt > 0, kErrInvalid, "channel solver: Lx, Lz and dt must be positive");
+}
+
+ChannelSolver::ChannelSolver(Context* ctx, const ChannelDesc& d) : ctx_(ctx), d_(d) {
+    validate(d);
     nxh_ = d.nx / 2 + 1;
     mx_ = padded_len(d.nx);
     nb_max_ = chb_size(d.nz, 0);
@@ -560,6 +565,11 @@ size_t footprint(const ChannelDesc& d) {
     // inverses in the state precision, plus the FP64 LU scratch while they are formed
     const size_t ent = 2 * nxh * (nb * nb + nu * nu);
     return fields + ent * (8 + dtype_size((DType)d.dtype)) + 2 * nxh * (nb + nu) * 4 + 3 * N * N * 8 + (1 << 20);
+}
+
+void ChannelSolver::flag_nonfinite(int* dflag) const {
+    for (const DevBuf* b : {&P_, &B_, &UX_, &UZ_})
+        tarang::flag_nonfinite(ctx_, b->ptr, 2 * field_elems(), (DType)d_.dtype, dflag);
 }
 
 }  // namespace tarang

@@ -25,6 +25,8 @@ enum ChannelVar : int { kChP = 0, kChB = 1, kChUX = 2, kChUZ = 3 };
 
 class ChannelSolver {
   public:
+    // the descriptor checks of the constructor, without allocating (throws tarang::Error)
+    static void validate(const ChannelDesc& d);
     ChannelSolver(Context* ctx, const ChannelDesc& d);
     ~ChannelSolver();
     size_t field_elems() const { return (size_t)nxh_ * d_.nz; }
@@ -40,6 +42,8 @@ class ChannelSolver {
     // F_b, F_ux, F_uz of the current state (as the timestepper evaluates it), reference layout
     void rhs(void* Fb, void* Fux, void* Fuz);
     void step(int nsteps);
+    // *dflag = 1 if the state holds a NaN or an infinity (device int, asynchronous)
+    void flag_nonfinite(int* dflag) const;
     void set_dt(double dt);
     void enable_graph(bool on) { graph_on_ = on; }
     double time() const { return t_; }

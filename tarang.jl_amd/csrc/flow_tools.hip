@@ -88,4 +88,24 @@ double cfl_frequency(Context* c, int ncomp, const void* const* comps, size_t npt
     return out;
 }
 
+template <typename S>
+__global__ __launch_bounds__(256) void k_flag_nonfinite(const S* __restrict__ p, size_t n, int* flag) {
+    bool bad = false;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        bad |= !isfinite(p[i]);
+    if (__any(bad) && (threadIdx.x & 63) == 0) *flag = 1;  // every writer stores the same value
+}
+
+void flag_nonfinite(Context* c, const void* p, size_t nreal, DType dt, int* dflag) {
+    if (nreal == 0) return;
+    const unsigned nblk = (unsigned)std::min<size_t>(1024, (nreal + 255) / 256);
+    if (dt == kF32)
+        hipLaunchKernelGGL((k_flag_nonfinite<float>), dim3(nblk), dim3(256), 0, c->stream,
+                           static_cast<const float*>(p), nreal, dflag);
+    else
+        hipLaunchKernelGGL((k_flag_nonfinite<double>), dim3(nblk), dim3(256), 0, c->stream,
+                           static_cast<const double*>(p), nreal, dflag);
+    TG_HIP(hipGetLastError());
+}
+
 }  // namespace tarang

@@ -9,4 +9,8 @@ namespace tarang {
 // npts == 0 gives 0.  Synchronises the context stream.
 double cfl_frequency(Context* c, int ncomp, const void* const* comps, size_t npts, const double* dx, DType dt);
 
+// *dflag = 1 (on the context stream, no synchronisation) if any of the nreal real values
+// at p is NaN or infinite; *dflag is left untouched otherwise.  dflag is a device int.
+void flag_nonfinite(Context* c, const void* p, size_t nreal, DType dt, int* dflag);
+
 }  // namespace tarang

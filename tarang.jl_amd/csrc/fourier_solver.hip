@@ -13,6 +13,7 @@
 #include "dispatch.hpp"
 #include "fourier_kernels.hpp"
 #include "fourier_solver.hpp"
+#include "flow_tools.hpp"
 
 namespace tarang {
 
@@ -71,7 +72,7 @@ static bool fourier_size_ok_2d(int n) {
     return ok;
 }
 
-FourierSolver::FourierSolver(Context* ctx, const FourierDesc& d) : ctx_(ctx), d_(d) {
+void FourierSolver::validate(const FourierDesc& d) {
     TG_REQUIRE(d.ndim == 1 || d.ndim == 2, kErrInvalid, "FourierSolver: ndim must be 1 or 2");
     TG_REQUIRE(d.dtype == kF32 || d.dtype == kF64, kErrInvalid, "FourierSolver: dtype must be f32 or f64");
     TG_REQUIRE(d.stepper >= 0 && d.stepper <= 4, kErrInvalid, "FourierSolver: unknown timestepper");
@@ -88,6 +89,11 @@ FourierSolver::FourierSolver(Context* ctx, const FourierDesc& d) : ctx_(ctx), d_
                        " have no fused 2-D kernels (supported: 8..2048, powers of 2)");
     TG_REQUIRE(!(d.ndim == 2 && d.dtype == kF64 && (nx > 1024 || ny > 1024)), kErrInvalid,
                "FP64 2-D fused kernels support sizes up to 1024");
+}
+
+FourierSolver::FourierSolver(Context* ctx, const FourierDesc& d) : ctx_(ctx), d_(d) {
+    validate(d);
+    const int nx = d.n[0], ny = d.ndim > 1 ? d.n[1] : 1;
     nxh_ = nx / 2 + 1;
     ny_ = ny;
     mx_ = padded_len(nx);
@@ -523,8 +529,9 @@ void FourierSolver::step_once(bool allow_graph) {
     if (!use_graph) {
         launch_step_body(first);
     } else {
+        // a multistep graph bakes in the step ratio dt / dt_prev: recapture when either changes
         if (graph_ && (graph_first_ != first || graph_dt_ != d_.dt ||
-                       (is_multistep(d_.stepper) && dt_prev_ != d_.dt))) {
+                       (is_multistep(d_.stepper) && graph_dtp_ != dt_prev_))) {
             (void)hipGraphExecDestroy(graph_);
             graph_ = nullptr;
         }
@@ -543,6 +550,7 @@ void FourierSolver::step_once(bool allow_graph) {
             TG_HIP(hipGraphDestroy(g));
             graph_first_ = first;
             graph_dt_ = d_.dt;
+            graph_dtp_ = dt_prev_;
         }
         TG_HIP(hipGraphLaunch(graph_, ctx_->stream));
     }
@@ -607,6 +615,10 @@ size_t footprint(const FourierDesc& d) {
     const size_t state = is_multistep(d.stepper) ? 4 * cb : (1 + 2 * (size_t)t.stages) * cb;
     const size_t work = (d.problem != kDiffusion && d.ndim == 2) ? 5 * nxh * my * vb : 0;
     return state + work + (1 << 20);
+}
+
+void FourierSolver::flag_nonfinite(int* dflag) const {
+    tarang::flag_nonfinite(ctx_, X_.ptr, 2 * coeff_elems(), (DType)d_.dtype, dflag);
 }
 
 }  // namespace tarang

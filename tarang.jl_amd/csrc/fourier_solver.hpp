@@ -38,6 +38,8 @@ struct FourierDesc {
 
 class FourierSolver {
   public:
+    // the descriptor checks of the constructor, without allocating (throws tarang::Error)
+    static void validate(const FourierDesc& d);
     FourierSolver(Context* ctx, const FourierDesc& d);
     ~FourierSolver();
     size_t coeff_elems() const { return (size_t)nxh_ * ny_; }
@@ -48,6 +50,8 @@ class FourierSolver {
     // algebraic fields of the vorticity problem (psi, u, v coefficients) from q
     void aux(const void* q, void* psi, void* u, void* v);
     void step(int nsteps);
+    // *dflag = 1 if the state holds a NaN or an infinity (device int, asynchronous)
+    void flag_nonfinite(int* dflag) const;
     void set_dt(double dt);
     void enable_graph(bool on);
     double time() const { return t_; }
@@ -95,6 +99,7 @@ class FourierSolver {
     hipGraphExec_t graph_ = nullptr;
     bool graph_first_ = false;  // graph was captured for the multistep start step
     double graph_dt_ = 0;
+    double graph_dtp_ = 0;      // previous step's dt the graph was captured with (SBDF2 ratio)
 };
 
 }  // namespace tarang

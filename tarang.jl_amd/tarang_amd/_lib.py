@@ -179,6 +179,8 @@ SIGNATURES = {
     "tarang_serve_submit": [_P, ctypes.POINTER(ServeRequest), _PI],
     "tarang_serve_run": [_P],
     "tarang_serve_record": [_P, _I, _PD, _PD, ctypes.POINTER(ctypes.c_size_t), _PI, _PI],
+    "tarang_serve_record_error": [_P, _I, _PI, ctypes.c_char_p, ctypes.c_size_t],
+    "tarang_serve_footprint": [ctypes.POINTER(ServeRequest), ctypes.POINTER(ctypes.c_size_t)],
 }
 
 _lib = None

@@ -23,8 +23,10 @@ class Record:
     done: float
     grid_point_steps: float
     footprint: int
-    lane: int
-    status: int          # 2 done, -1 rejected (footprint above the budget)
+    lane: int            # the slot the request occupied
+    status: int          # 2 done, -1 rejected (footprint above the budget), -2 failed
+    steps_done: int = 0
+    error: str = ""      # why the request failed or was rejected
 
     @property
     def latency(self) -> float:
@@ -32,8 +34,31 @@ class Record:
         return self.done - self.arrival
 
 
+def _request(kind: int, desc, nsteps: int = 0, arrival: float = 0.0):
+    r = _lib.ServeRequest()
+    r.kind = kind
+    if kind == _lib.SERVE_PERIODIC:
+        r.fivp = desc
+    elif kind == _lib.SERVE_CHANNEL:
+        r.chan = desc
+    else:
+        r.chan3 = desc
+    r.nsteps = int(nsteps)
+    r.arrival = float(arrival)
+    return r
+
+
+def footprint(kind: int, desc) -> int:
+    """Device bytes a request of this kind and descriptor holds while resident."""
+    b = ctypes.c_size_t()
+    _lib.call("tarang_serve_footprint", ctypes.byref(_request(kind, desc)), ctypes.byref(b))
+    return b.value
+
+
 class Server:
-    """Continuous-batching request loop (tarang_serve_create/submit/run/record)."""
+    """Continuous-batching request loop (tarang_serve_create/submit/run/record).
+
+    `lanes` is the number of slots: requests resident at once, each on its own stream."""
 
     def __init__(self, device: int = 0, lanes: int = 4, mem_budget: int = 64 << 30, chunk_steps: int = 10):
         h = ctypes.c_void_p()
@@ -44,16 +69,7 @@ class Server:
 
     def submit(self, kind: int, desc, nsteps: int, arrival: float, ic: Sequence, out: Sequence) -> int:
         """ic / out: torch device tensors (or None) in the reference layout; kept alive until run() ends."""
-        r = _lib.ServeRequest()
-        r.kind = kind
-        if kind == _lib.SERVE_PERIODIC:
-            r.fivp = desc
-        elif kind == _lib.SERVE_CHANNEL:
-            r.chan = desc
-        else:
-            r.chan3 = desc
-        r.nsteps = int(nsteps)
-        r.arrival = float(arrival)
+        r = _request(kind, desc, nsteps, arrival)
         for i, t in enumerate(ic):
             r.ic[i] = t.data_ptr() if t is not None else None
         for i, t in enumerate(out):
@@ -74,7 +90,10 @@ class Server:
         st = ctypes.c_int()
         _lib.call("tarang_serve_record", self.h, int(rid), t, ctypes.byref(gps), ctypes.byref(fp),
                   ctypes.byref(lane), ctypes.byref(st))
-        return Record(t[0], t[1], t[2], gps.value, fp.value, lane.value, st.value)
+        nd = ctypes.c_int()
+        msg = ctypes.create_string_buffer(1024)
+        _lib.call("tarang_serve_record_error", self.h, int(rid), ctypes.byref(nd), msg, 1024)
+        return Record(t[0], t[1], t[2], gps.value, fp.value, lane.value, st.value, nd.value, msg.value.decode())
 
     def close(self):
         if getattr(self, "h", None):
@@ -99,4 +118,5 @@ def latency_stats(records: Sequence[Record]) -> dict:
     return {"n": len(done), "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
             "mean_s": float(lat.mean()), "makespan_s": span,
             "grid_point_steps_per_s": float(sum(r.grid_point_steps for r in done) / span) if span > 0 else None,
-            "rejected": sum(1 for r in records if r.status == -1)}
+            "rejected": sum(1 for r in records if r.status == -1),
+            "failed": sum(1 for r in records if r.status == -2)}

@@ -156,6 +156,7 @@ class InitialValueSolver:
         if self._channel is not None:
             return self._channel.rhs()
         out = out or ScalarField(self.field.domain, "F")
+        ensure_layout_(self.field, "c")  # a field last written on the grid is transformed first
         self.dev.before()
         _lib.call("tarang_fivp_rhs", self.handle, _ptr(self.field._c), _ptr(out._c))
         self.dev.after()
@@ -171,8 +172,10 @@ class InitialValueSolver:
             self.iteration += n
             self.sim_time += n * self.dt
             return
-        if self.field.current_layout != "c" or getattr(self, "_dirty", False):
-            self._push_state()
+        # the field is handed over before every batch: an edit made in either layout since the
+        # last batch (q['g'] = ..., q['c'] = ..., or in place) is seen; the clock and the
+        # multistep history are kept (update_state), as the channel backend does
+        self._push_state()
         _lib.call("tarang_fivp_step", self.handle, int(n))
         self.iteration += n
         self.sim_time += n * self.dt
@@ -202,7 +205,6 @@ class InitialValueSolver:
             self._channel.push(restart=True)
         else:
             self._push_state(restart=True)
-            self._dirty = False
 
     def set_dt(self, dt: float):
         _lib.call(self._channel.pre + "set_dt" if self._channel is not None else "tarang_fivp_set_dt", self.handle,

@@ -167,6 +167,9 @@ class Fivp:
     def set_graph(self, on):
         _lib.call("tarang_fivp_set_graph", self.h, 1 if on else 0)
 
+    def set_dt(self, dt):
+        _lib.call("tarang_fivp_set_dt", self.h, ctypes.c_double(dt))
+
     def close(self):
         _lib.call("tarang_fivp_destroy", self.h)
 

@@ -69,6 +69,40 @@ def test_turbulence_example_through_api(oracle, stepper):
     assert abs(E - E_ref) / E_ref < 1e-11
 
 
+def test_coefficient_edits_between_steps_are_seen(oracle):
+    """Writes in coefficient layout after the solver is built (an assignment, then an
+    in-place edit of the device array) are handed to the library before the next steps."""
+    N, nu, dt = 32, 1e-3, 2e-3
+    dom = T.PeriodicDomain(N, N, arch=T.GPU())
+    q, psi, u = T.ScalarField(dom, "q"), T.ScalarField(dom, "psi"), T.VectorField(dom, "u")
+    tau = T.ScalarField(dom.dist, "tau_psi", dtype=np.float64)
+    problem = T.IVP([q, psi, u, tau])
+    T.add_parameters_(problem, nu=nu)
+    T.add_equation_(problem, "∂t(q) - nu*Δ(q) = -u⋅∇(q)")
+    T.add_equation_(problem, "Δ(psi) + tau_psi - q = 0")
+    T.add_equation_(problem, "u - skew(grad(psi)) = 0")
+    T.add_bc_(problem, "integ(psi) = 0")
+    q0 = turb_ic(N)
+    q["g"] = q0
+    solver = T.InitialValueSolver(problem, T.RK443(), dt=dt)
+    d = oracle.domain([0, 0], [N, N])
+    ref = oracle.FourierIVP(d, oracle.PROB_TURB2D, "RK443", nu, dt)
+    ref.set_coeff(oracle.forward_transform(d, q0))
+    T.run_(solver, stop_iteration=2)
+    ref.step(2)
+    c = q["c"].cpu().numpy() * 0.5
+    q["c"] = c  # assignment in coefficient layout
+    ref.set_coeff(ref.get_coeff() * 0.5)
+    T.run_(solver, stop_iteration=4)
+    ref.step(2)
+    assert relerr(q["c"].cpu().numpy(), ref.get_coeff()) < 1e-12
+    q["c"].mul_(2.0)  # in-place edit of the coefficient array
+    ref.set_coeff(ref.get_coeff() * 2.0)
+    T.run_(solver, stop_iteration=6)
+    ref.step(2)
+    assert relerr(q["c"].cpu().numpy(), ref.get_coeff()) < 1e-12
+
+
 def test_diffusion_config1_through_api(golden):
     g = golden("ivp.npz")
     dom = T.PeriodicDomain(64, arch=T.GPU())

@@ -106,6 +106,32 @@ def test_graph_replay_matches_eager(gpu, oracle):
     assert np.array_equal(out[0], out[1])
 
 
+def test_sbdf2_dt_change_graph_matches_eager_and_oracle(gpu, oracle):
+    """SBDF2 with one dt change, then several steps at the new dt: the step after the change
+    runs at the ratio w = dt2/dt1, the following ones at w = 1 again.  The step graph must be
+    recaptured for both (a graph baked at w != 1 replayed at w = 1 gives the wrong derivative)."""
+    N, nu, dt1, dt2 = 64, 1e-3, 2e-3, 4e-3
+    d = oracle.domain([0, 0], [N, N])
+    C = oracle.forward_transform(d, turb_ic(N))
+    ref = oracle.FourierIVP(d, oracle.PROB_TURB2D, "SBDF2", nu, dt1)
+    ref.set_coeff(C)
+    ref.step(3)
+    ref.set_dt(dt2)
+    ref.step(4)
+    out = []
+    for graph in (False, True):
+        s = gpu.fivp([N, N], [2 * np.pi, 2 * np.pi], TURB, TS["SBDF2"], nu, dt1)
+        s.set_graph(graph)
+        s.set_state(C)
+        s.step(3)
+        s.set_dt(dt2)
+        s.step(4)
+        out.append(s.get_state())
+        s.close()
+    assert np.array_equal(out[0], out[1]), "graph replay differs from eager stepping"
+    assert relerr(out[1], ref.get_coeff()) < 1e-12
+
+
 @pytest.mark.parametrize("ts", ["RK222", "RK443", "SBDF2"])
 def test_diffusion_exact_decay(gpu, oracle, golden, ts):
     g = golden("ivp.npz")

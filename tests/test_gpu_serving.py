@@ -135,3 +135,78 @@ def test_serving_budget_serialises_and_rejects(gpu):
     # a request larger than the whole budget is rejected, the rest still complete
     res, recs = run_server(dev, reqs, lanes=2, budget=max(fps) - 1)
     assert any(r.status == -1 for r in recs) and all(r.status in (2, -1) for r in recs)
+
+
+def test_serving_failures_stay_with_their_request(gpu):
+    """A request whose per-mode system is singular (kappa = 0: the b system's lift column
+    vanishes, caught by the factorisation's info check) and one whose state turns
+    non-finite (a NaN in its initial condition, caught by the check after its first
+    chunk) fail with a message and free their memory; every other request finishes with
+    the bits it has alone."""
+    dev = gpu.torch_device
+    reqs = requests(dev)
+    bad_chan = chan_desc(32, 16, 1e5, 0, 5e-3)
+    bad_chan.kappa = 0.0
+    nz = 16
+    Bc = np.zeros((17, nz), complex)
+    Bc[0, 0] = 1.0
+    u = np.fft.rfft(np.sin(2 * np.pi * np.arange(128) / 128))
+    u[3] = np.nan
+    bad = [(_lib.SERVE_CHANNEL, bad_chan, 20, 0.001, [None, H.to_dev(Bc, torch.complex64, dev), None, None],
+            [(17, nz)] * 4),
+           (_lib.SERVE_PERIODIC, periodic_desc([128], 2, 0, 1e-2, 1e-3), 30, 0.003,
+            [H.to_dev(u, torch.complex64, dev)], [(65,)])]
+    mix, recs = run_server(dev, reqs[:2] + bad + reqs[2:], lanes=3, budget=8 << 30)
+    good = [r for i, r in enumerate(recs) if i not in (2, 3)]
+    assert all(r.status == 2 for r in good), [(r.status, r.error) for r in recs]
+    assert recs[2].status == -2 and "singular" in recs[2].error, recs[2]
+    assert recs[3].status == -2 and "non-finite" in recs[3].error and recs[3].steps_done == 7, recs[3]
+    st = latency_stats(recs)
+    assert st["failed"] == 2 and st["n"] == len(reqs)
+    keep = mix[:2] + mix[4:]
+    for i, rq in enumerate(reqs):
+        alone, rec = run_server(dev, [rq[:3] + (0.0,) + rq[4:]], lanes=1, budget=8 << 30)
+        assert rec[0].status == 2
+        for a, b in zip(keep[i], alone[0]):
+            assert np.array_equal(a, b), i
+
+
+def test_serving_submit_refuses_unsupported_descriptors(gpu):
+    dev = gpu.torch_device
+    srv = Server(0, lanes=2, mem_budget=1 << 30)
+    o = [torch.zeros((12, 7), dtype=torch.complex64, device=dev)]
+    ic = [torch.zeros((12, 7), dtype=torch.complex64, device=dev)]
+    with pytest.raises(_lib.TarangError, match="no fused 2-D kernels"):
+        srv.submit(_lib.SERVE_PERIODIC, periodic_desc([12, 12], 1, 1, 1e-3, 1e-3), 5, 0.0, ic, o)
+    bad = chan_desc(32, 600, 1e5, 0, 1e-3)
+    with pytest.raises(_lib.TarangError, match="ChebyshevT size"):
+        srv.submit(_lib.SERVE_CHANNEL, bad, 5, 0.0, [None, ic[0], None, None], o * 4)
+    srv.close()
+
+
+@pytest.mark.parametrize("which", ["periodic", "channel", "channel3"])
+def test_serving_footprint_bounds_the_allocation(gpu, which):
+    """The admission footprint is an upper bound of the device memory a solver holds once
+    built (it also counts the FP64 factor scratch of set-up), and not a gross overestimate."""
+    import gc
+    from tarang_amd.serving import footprint
+    if which == "periodic":
+        kind, desc = _lib.SERVE_PERIODIC, periodic_desc([512, 512], 1, 1, 1e-4, 1e-3)
+        make = lambda: gpu.fivp([512, 512], [2 * np.pi] * 2, 1, 1, 1e-4, 1e-3, dtype=np.float32)  # noqa: E731
+    elif which == "channel":
+        kind, desc = _lib.SERVE_CHANNEL, chan_desc(512, 128, 1e6, 0, 1e-3)
+        make = lambda: gpu.chan(512, 128, 4.0, 1.0, 1e-3, 1e-3, 1e-3, stepper=0, dtype=np.float32)  # noqa: E731
+    else:
+        kind, desc = _lib.SERVE_CHANNEL3, chan3_desc(64, 64, 33, 1e5, 1e-3)
+        make = lambda: gpu.chan3(64, 64, 33, 4.0, 4.0, 1.0, 1e-3, 1e-3, 1e-3, dtype=np.float32)  # noqa: E731
+    fp = footprint(kind, desc)
+    gc.collect()
+    torch.cuda.synchronize()
+    free0, _ = torch.cuda.mem_get_info()
+    s = make()
+    gpu.sync()
+    free1, _ = torch.cuda.mem_get_info()
+    held = free0 - free1
+    s.close()
+    print(f"{which}: footprint {fp} B, held after set-up {held} B", flush=True)
+    assert 0 < held <= fp and fp <= 4 * held + (32 << 20), (held, fp)

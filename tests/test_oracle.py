@@ -160,6 +160,32 @@ def test_convergence_order(oracle, ts, order):
     assert abs(observed - order) < 0.35, (ts, errs, observed)
 
 
+def _burgers_alternating(oracle, h, pairs, N=32):
+    d = oracle.domain([0], [N])
+    x = 2 * np.pi * np.arange(N) / N
+    p = oracle.FourierIVP(d, oracle.PROB_BURGERS1D, "SBDF2", 0.05, h)
+    p.set_coeff(oracle.forward_transform(d, 0.5 * np.sin(x) + 0.2 * np.cos(2 * x)))
+    for _ in range(pairs):
+        p.set_dt(h)
+        p.step(1)
+        p.set_dt(h / 2)
+        p.step(1)
+    return p.get_coeff(), p.time
+
+
+def test_variable_step_sbdf2_second_order(oracle):
+    # steps alternating h, h/2 (ratio w = 1/2 and 2): the variable-step weights keep order 2;
+    # the constant-step weights (w = 1 at every step) would not converge at that order
+    ref = _burgers_run(oracle, "RK443", 1e-4, 0.36)
+    errs = []
+    for h, pairs in ((0.02, 12), (0.01, 24)):
+        c, t = _burgers_alternating(oracle, h, pairs)
+        assert abs(t - 0.36) < 1e-12
+        errs.append(np.abs(c - ref).max())
+    observed = math.log2(errs[0] / errs[1])
+    assert abs(observed - 2) < 0.35, (errs, observed)
+
+
 def test_inviscid_turbulence_conserves_energy_enstrophy(oracle):
     # Galerkin-truncated 2-D Euler conserves E and Z up to time-discretisation error
     N = 32
