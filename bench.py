@@ -54,6 +54,25 @@ def measured_traffic(name, kernel_label):
         return json.load(f).get(kernel_label.split()[0])
 
 
+def host_threads():
+    """Threads for the CPU baseline: the cores this process may run on, at most 16 (a GPU box
+    gives each GPU a 16-core share of its host)."""
+    try:
+        visible = len(os.sched_getaffinity(0))
+    except AttributeError:
+        visible = os.cpu_count() or 1
+    return max(1, min(visible, 16))
+
+
+def host_info():
+    try:
+        visible = len(os.sched_getaffinity(0))
+    except AttributeError:
+        visible = None
+    return {"nproc": os.cpu_count(), "affinity_cores": visible, "threads_used": host_threads(),
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+
+
 def calibration_copy(mib):
     """One plain device-to-device copy of `mib` MiB: the reference reading for HBM byte counters."""
     import torch
@@ -78,8 +97,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run one plain device-to-device copy of this many MiB (counter calibration)")
-    ap.add_argument("--workload", choices=["turbulence", "channel", "channel3d", "serving", "plugins"],
-                    default="turbulence")
+    ap.add_argument("--workload", choices=["all", "turbulence", "channel", "channel3d", "serving", "plugins"],
+                    default="all")
     ap.add_argument("--requests", type=int, default=24, help="serving: number of requests")
     ap.add_argument("--horizon", type=int, default=None,
                     help="channel workloads: steps before the FP32-vs-FP64 comparison (default: the timed steps "
@@ -91,14 +110,36 @@ def main():
         sys.exit(2)
     if a.copy_mib > 0:
         calibration_copy(a.copy_mib)
-    if a.workload == "channel":
-        return bench_channel(a)
-    if a.workload == "serving":
-        return bench_serving(a)
-    if a.workload == "channel3d":
-        return bench_channel3d(a)
     if a.workload == "plugins":
         return bench_plugins(a)
+    run = {"turbulence": bench_turbulence, "channel": bench_channel, "serving": bench_serving,
+           "channel3d": bench_channel3d}
+    if a.workload != "all":
+        print(json.dumps(run[a.workload](a)), flush=True)
+        return
+    # every configuration in one run; the line is the 3-D Rayleigh-Benard configuration (the
+    # metric's FP32+refinement vs FP64 comparison), the others are summarised inside it
+    others = {}
+    for name, fn, steps in (("turbulence", bench_turbulence, 200), ("channel", bench_channel, 100),
+                            ("serving", bench_serving, None)):
+        sub = argparse.Namespace(**vars(a))
+        sub.no_cpu = True
+        if steps is not None and a.steps == 1000:
+            sub.steps = steps
+        if name == "serving":
+            sub.requests = min(a.requests, 16)
+        log(f"all: {name}")
+        r = fn(sub)
+        others[r["config"]["workload"]] = {k: r.get(k) for k in (
+            "value", "unit", "ms_per_step", "fp64", "speedup_fp32_vs_fp64", "fp32_vs_fp64_rel_err", "roofline",
+            "latency_s", "makespan_s", "rejected") if r.get(k) is not None}
+    log("all: channel3d (headline)")
+    line = bench_channel3d(a)
+    line["other_configs"] = others
+    print(json.dumps(line), flush=True)
+
+
+def bench_turbulence(a):
 
     import numpy as np
     import torch
@@ -196,7 +237,7 @@ def main():
     cpu = None
     if not a.no_cpu:
         from oracle import oracle as O
-        threads = min(os.cpu_count() or 1, 16)
+        threads = host_threads()
         O.set_threads(threads)
         d = O.domain([0, 0], [N, N])
         p = O.FourierIVP(d, O.PROB_TURB2D, "RK443", NU, DT)
@@ -205,7 +246,7 @@ def main():
         t0 = time.perf_counter()
         p.step(a.cpu_steps)
         el = time.perf_counter() - t0
-        cpu = {"value": N * N * a.cpu_steps / el, "unit": "grid-point steps/s", "cores": threads, "kind": "port",
+        cpu = {"value": N * N * a.cpu_steps / el, "unit": "grid-point steps/s", "cores": threads, "host": host_info(), "kind": "port",
                "sample": f"{a.cpu_steps} RK443 steps of the same 1024^2 workload from the same initial condition, "
                          f"FP64 oracle (C, OpenMP over transform lines), {el:.2f} s"}
 
@@ -237,7 +278,7 @@ def main():
         "autotuned_variants": tuning,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
+    return line
 
 
 CH = dict(nx=1024, nz=256, Lx=4.0, Lz=1.0, Ra=2e6, Pr=1.0, dt=0.01)
@@ -298,10 +339,13 @@ def bench_channel(a):
     ctx = ctypes.c_void_p()
     _lib.call("tarang_ctx_create", 0, ctypes.byref(ctx))
     results, finals, solvers = {}, {}, {}
-    for name, code, ct in (("f64", _lib.F64, torch.complex128), ("f32", _lib.F32, torch.complex64)):
+    forms = (("f64", _lib.F64, torch.complex128, 0), ("f32", _lib.F32, torch.complex64, HEADLINE_SWEEPS),
+             ("f32_sweeps0", _lib.F32, torch.complex64, 0), ("f32_sweeps2", _lib.F32, torch.complex64, 2))
+    for name, code, ct, sweeps in forms:
         d = _lib.ChanDesc()
         d.nx, d.nz, d.Lx, d.Lz, d.kappa, d.nu, d.dt = nx, nz, Lx, Lz, kappa, nu, dt
         d.dtype, d.timestepper, d.b_bottom, d.b_top = code, 0, Lz, 0.0
+        d.refine_sweeps = sweeps
         h = ctypes.c_void_p()
         t0 = time.perf_counter()
         _lib.call("tarang_chan_create", ctx, ctypes.byref(d), ctypes.byref(h))
@@ -338,6 +382,12 @@ def bench_channel(a):
         log(f"channel {name}: {results[name][0]:.4f} ms/step")
     nus = {k: diagnostics.nusselt(v[0], v[1], nx, Lz, kappa, Lz) for k, v in finals.items()}
     err_b = float(np.abs(finals["f32"][0] - finals["f64"][0]).max() / np.abs(finals["f64"][0]).max())
+    by_sweeps = {}
+    for name, sw in (("f32_sweeps0", 0), ("f32", HEADLINE_SWEEPS), ("f32_sweeps2", 2)):
+        by_sweeps[str(sw)] = {
+            "ms_per_step": results[name][0], "speedup_vs_fp64": results["f64"][0] / results[name][0],
+            "nusselt_rel_err_vs_fp64": abs(nus[name] - nus["f64"]) / abs(nus["f64"]),
+            "b_rel_err_vs_fp64": float(np.abs(finals[name][0] - finals["f64"][0]).max() / np.abs(finals["f64"][0]).max())}
     err_w = float(np.abs(finals["f32"][1] - finals["f64"][1]).max() / max(np.abs(finals["f64"][1]).max(), 1e-300))
     # live per-kernel durations, FP32 path
     avg = (ctypes.c_double * 6)()
@@ -346,7 +396,10 @@ def bench_channel(a):
     nb = (ctypes.c_double * 6)()
     _lib.call("tarang_chan_kernel_bytes", solvers["f32"], nb)
     per_step = [avg[i] * cnt[i] / a.profile_steps for i in range(6)]
-    dom = max(range(5), key=lambda i: per_step[i])
+    dom = max((0, 1, 2, 3, 5), key=lambda i: per_step[i])  # HBM class; the solve (4) is reported on its own
+    solve_class = {"kernel": CH_KERNELS[4], "ms_per_step": per_step[4], "bytes_per_launch": nb[4],
+                   "GBps": nb[4] / (avg[4] * 1e-3) / 1e9 if avg[4] else None,
+                   "share_of_step": per_step[4] / sum(per_step) if sum(per_step) else None}
     achieved = nb[dom] / (avg[dom] * 1e-3) / 1e9
     kernels = {CH_KERNELS[i]: {"avg_ms": avg[i], "launches_per_step": cnt[i] / a.profile_steps,
                                "algorithmic_bytes": nb[i] or None,
@@ -359,7 +412,7 @@ def bench_channel(a):
     cpu = None
     if not a.no_cpu:
         from oracle import oracle as O
-        threads = min(os.cpu_count() or 1, 16)
+        threads = host_threads()
         O.set_threads(threads)
         sx, cs = 32, 2
         ref = O.RBC2D(sx, nz, Lx, Lz, kappa, nu, dt, "RK222")
@@ -369,7 +422,7 @@ def bench_channel(a):
         t0 = time.perf_counter()
         ref.step(cs)
         el = time.perf_counter() - t0
-        cpu = {"value": sx * nz * cs / el, "unit": "grid-point steps/s", "cores": threads, "kind": "port",
+        cpu = {"value": sx * nz * cs / el, "unit": "grid-point steps/s", "cores": threads, "host": host_info(), "kind": "port",
                "sample": f"{cs} RK222 steps at Nx={sx}, Nz={nz} (17 of the 513 per-mode systems, same size each), "
                          f"FP64 oracle with dense complex per-mode LU, factorisation excluded, {el:.2f} s"}
     v32 = nx * nz / (results["f32"][0] * 1e-3)
@@ -384,26 +437,31 @@ def bench_channel(a):
         "fp64": {"value": v64, "ms_per_step": results["f64"][0], "setup_s": results["f64"][1]},
         "setup_s_f32": results["f32"][1], "dt_change_f32": dtc,
         "speedup_fp32_vs_fp64": v32 / v64,
+        "refine_sweeps": HEADLINE_SWEEPS,
         "fp32_vs_fp64_rel_err": {"b_maxnorm": err_b, "uz_maxnorm": err_w,
                                  "nusselt": abs(nus["f32"] - nus["f64"]) / abs(nus["f64"]),
                                  "after_steps": max(horizon, steps)},
+        "fp32_by_refine_sweeps": by_sweeps,
         "nusselt": nus,
         "roofline": {"bound": "hbm", "kernel": CH_KERNELS[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic("traffic_channel2d_1024x256_f32.json", CH_KERNELS[dom])},
+        "solve_class_f32": solve_class,
         "kernels_f32": kernels,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
+    return line
 
 
 CH3 = dict(nx=256, ny=256, nz=128, Lx=4.0, Ly=4.0, Lz=1.0, Ra=1e7, Pr=0.7, dt=2e-3)
+HEADLINE_SWEEPS = 1  # FP64-residual refinement sweeps of the FP32 form in the headline line
 CH3_WORKLOAD = ("3-D Rayleigh-Benard, RealFourier^2 x ChebyshevT 256x256x128, Lx=Ly=4, Lz=1, Ra=1e7, Pr=0.7, "
                 "no-slip walls, b fixed at both walls, 3/2 dealiasing, SBDF2, dt=2e-3")
 CH3_KERNELS = ["k3_zbwd (Chebyshev evaluation of 15 operand planes)", "k3_ybwd (padded inverse y FFT, MY=384)",
                "k3_xpass (inverse x FFTs MX=384, 12 products, forward x FFTs)",
                "k3_yfwd (forward y FFT, truncation, Nyquist folds)", "k_ch_zfwd (Chebyshev forward of 4 products)",
-               "k3_sbdf (multistep right-hand side)", "k3_solve (explicit inverses per k_h^2 key, 8-mode chunks on FP64 matrix cores)",
+               "k3_sbdf (multistep right-hand side)",
+               "k3_solve (explicit inverses per k_h^2 key, 8-mode chunks on FP64 matrix cores)",
                "k3_lx (structured L x, RK only)"]
 
 
@@ -429,10 +487,14 @@ def bench_channel3d(a):
     ctx = ctypes.c_void_p()
     _lib.call("tarang_ctx_create", 0, ctypes.byref(ctx))
     results, finals, solvers, keys, setups = {}, {}, {}, {}, {}
-    for name, code, ct in (("f64", _lib.F64, torch.complex128), ("f32", _lib.F32, torch.complex64)):
+    # FP64; FP32 with 1 FP64-residual refinement sweep per solve (the headline form); FP32 with 0 and 2
+    forms = (("f64", _lib.F64, torch.complex128, 0), ("f32", _lib.F32, torch.complex64, HEADLINE_SWEEPS),
+             ("f32_sweeps0", _lib.F32, torch.complex64, 0), ("f32_sweeps2", _lib.F32, torch.complex64, 2))
+    for name, code, ct, sweeps in forms:
         d = _lib.Chan3Desc()
         d.nx, d.ny, d.nz, d.Lx, d.Ly, d.Lz = nx, ny, nz, c["Lx"], c["Ly"], Lz
         d.kappa, d.nu, d.dt, d.dtype, d.timestepper, d.b_bottom, d.b_top = kappa, nu, c["dt"], code, 2, Lz, 0.0
+        d.refine_sweeps = sweeps
         h = ctypes.c_void_p()
         t0 = time.perf_counter()
         _lib.call("tarang_chan3_create", ctx, ctypes.byref(d), ctypes.byref(h))
@@ -477,14 +539,25 @@ def bench_channel3d(a):
         log(f"channel3d {name}: {results[name][0]:.4f} ms/step")
     nus = {kx: diagnostics.nusselt3(v[0], v[1], nx, ny, Lz, kappa, Lz) for kx, v in finals.items()}
     err_b = float(np.abs(finals["f32"][0] - finals["f64"][0]).max() / np.abs(finals["f64"][0]).max())
+    by_sweeps = {}
+    for name, sw in (("f32_sweeps0", 0), ("f32", HEADLINE_SWEEPS), ("f32_sweeps2", 2)):
+        by_sweeps[str(sw)] = {
+            "ms_per_step": results[name][0], "speedup_vs_fp64": results["f64"][0] / results[name][0],
+            "nusselt_rel_err_vs_fp64": abs(nus[name] - nus["f64"]) / abs(nus["f64"]),
+            "b_rel_err_vs_fp64": float(np.abs(finals[name][0] - finals["f64"][0]).max() / np.abs(finals["f64"][0]).max())}
     avg = (ctypes.c_double * 8)()
     cnt = (ctypes.c_long * 8)()
     _lib.call("tarang_chan3_profile", solvers["f32"], a.profile_steps, avg, cnt)
     nb = (ctypes.c_double * 8)()
     _lib.call("tarang_chan3_kernel_bytes", solvers["f32"], nb)
     per_step = [avg[i] * cnt[i] / a.profile_steps for i in range(8)]
-    dom = max(range(8), key=lambda i: per_step[i])
+    # the roofline names the dominant HBM-class kernel (transforms, products, stage sweeps);
+    # the solve class (band factors and sweeps) is reported on its own
+    dom = max(range(6), key=lambda i: per_step[i])
     achieved = nb[dom] / (avg[dom] * 1e-3) / 1e9
+    solve_class = {"kernel": CH3_KERNELS[6], "ms_per_step": per_step[6], "bytes_per_launch": nb[6],
+                   "GBps": nb[6] / (avg[6] * 1e-3) / 1e9 if avg[6] else None,
+                   "share_of_step": per_step[6] / sum(per_step) if sum(per_step) else None}
     kernels = {CH3_KERNELS[i]: {"avg_ms": avg[i], "launches_per_step": cnt[i] / a.profile_steps,
                                 "algorithmic_bytes": nb[i], "GBps": nb[i] / (avg[i] * 1e-3) / 1e9 if avg[i] else None}
                for i in range(8)}
@@ -495,7 +568,7 @@ def bench_channel3d(a):
     cpu = None
     if not a.no_cpu:
         from oracle import oracle as O
-        threads = min(os.cpu_count() or 1, 16)
+        threads = host_threads()
         O.set_threads(threads)
         sx, cs = 16, 2
         ref = O.RBC3D(sx, sx, nz, c["Lx"], c["Ly"], Lz, kappa, nu, c["dt"], "SBDF2")
@@ -505,7 +578,7 @@ def bench_channel3d(a):
         t0 = time.perf_counter()
         ref.step(cs)
         el = time.perf_counter() - t0
-        cpu = {"value": sx * sx * nz * cs / el, "unit": "grid-point steps/s", "cores": threads, "kind": "port",
+        cpu = {"value": sx * sx * nz * cs / el, "unit": "grid-point steps/s", "cores": threads, "host": host_info(), "kind": "port",
                "sample": f"{cs} SBDF2 steps at 16x16x128 (144 of the 33,024 per-mode systems, same size), FP64 "
                          f"oracle with dense complex per-mode LU, factorisation excluded, {el:.2f} s"}
     pts = nx * ny * nz
@@ -522,22 +595,26 @@ def bench_channel3d(a):
         "setup_s_f32": results["f32"][1], "warmup_s_f32": setups["f32"], "kh2_keys": keys["f32"],
         "dt_change_f32": dtc,
         "speedup_fp32_vs_fp64": v32 / v64,
+        "refine_sweeps": HEADLINE_SWEEPS,
         "fp32_vs_fp64_rel_err": {"b_maxnorm": err_b, "nusselt": abs(nus["f32"] - nus["f64"]) / abs(nus["f64"]),
                                  "after_steps": max(horizon, steps + max(a.warmup, 3))},
+        "fp32_by_refine_sweeps": by_sweeps,
         "nusselt": nus,
         "roofline": {"bound": "hbm", "kernel": CH3_KERNELS[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic("traffic_channel3d_256x256x128_f32.json", CH3_KERNELS[dom])},
+        "solve_class_f32": solve_class,
         "kernels_f32": kernels,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
+    return line
 
 
 SERVE_WORKLOAD = ("serving stream: Poisson arrivals at 80% of the measured saturation capacity; 40% 2-D turbulence "
                   "N in {128,256,512,1024}^2 (RK443, dt=1e-3), 40% 2-D Rayleigh-Benard Nx x Nz in {128x32, 256x64, "
                   "512x128, 1024x256}, Ra log-uniform in [1e5, 1e8], Pr=1 (RK222, dt=1e-3), 20% 1-D Burgers N in "
-                  "{64..4096}, nu=1e-2 (RK222, dt=1e-3); FP32 state with FP64 tau solves; steps log-uniform in [200, 2000]")
+                  "{64..4096}, nu=1e-2 (RK222, dt=1e-3); FP32 state, FP64 tau factorisation with one FP64-residual refinement "
+                  "sweep per solve; steps log-uniform in [200, 2000]")
 
 
 def serving_stream(np, torch, _lib, configs, n, seed, dev):
@@ -565,6 +642,7 @@ def serving_stream(np, torch, _lib, configs, n, seed, dev):
             d = _lib.ChanDesc()
             d.nx, d.nz, d.Lx, d.Lz, d.kappa, d.nu, d.dt = nx, nz, 4.0, 1.0, Ra ** -0.5, Ra ** -0.5, 1e-3
             d.dtype, d.timestepper, d.b_bottom, d.b_top = _lib.F32, 0, 1.0, 0.0
+            d.refine_sweeps = HEADLINE_SWEEPS  # FP32 with FP64-residual refinement of the tau solves
             bg, _ = channel_ic(np, nx, nz, 1.0, seed=int(rng.integers(1 << 30)))
             k = np.arange(nz)
             T = np.cos(np.outer(np.pi * np.arange(nz) / (nz - 1), k)) * ((-1.0) ** k)[None, :]
@@ -630,7 +708,7 @@ def bench_serving(a):
         "offered_rate_req_per_s": rate, "makespan_s": el, "total_grid_point_steps": total,
         "rejected": st.get("rejected", 0),
     }
-    print(json.dumps(line), flush=True)
+    return line
 
 
 FP64_MFMA_PEAK_TFLOPS = 78.6

@@ -176,6 +176,12 @@ int tarang_fivp_rhs(tarang_fivp_t s, const void* coeff, void* F);
 /* algebraic refresh of the vorticity problem (src/core/timesteppers/state_utils.jl:843-877,
  * 1059-1108): psi = -q/|k|^2, u = -dy psi, v = dx psi, as coefficients; any output may be NULL */
 int tarang_fivp_aux(tarang_fivp_t s, const void* q, void* psi, void* u, void* v);
+/* energy and enstrophy of the current state, reduced on the device in FP64 (the reference
+ * reduces E = 1/2 mean(u^2 + v^2), Z = 1/2 mean(q^2) on the device,
+ * examples/2d_decaying_turbulence_gpu.jl:84-92): for the vorticity problem exactly those by
+ * Parseval over the coefficients; for the others E = 1/2 <f^2>, Z = 1/2 <|grad f|^2>.
+ * Synchronises the context stream. */
+int tarang_fivp_diagnostics(tarang_fivp_t s, double* E, double* Z);
 /* advance nsteps (step!); use_graph != 0 replays each step as one captured hipGraph */
 int tarang_fivp_step(tarang_fivp_t s, int nsteps);
 int tarang_fivp_set_graph(tarang_fivp_t s, int use_graph);
@@ -218,6 +224,8 @@ typedef struct {
     int dtype;
     int timestepper;
     double b_bottom, b_top;  /* Dirichlet values b(z=0), b(z=Lz) (the test problem: Lz, 0) */
+    int refine_sweeps;       /* 0, 1 or 2 sweeps of x += A^-1 (y - A x) per implicit solve, with the
+                                residual y - A x formed in FP64 from the structured tau systems */
 } tarang_chan_desc;
 
 #define TARANG_CHAN_P 0
@@ -245,6 +253,11 @@ int tarang_chan_get_field(tarang_chan_t s, int var, void* coeff);
 /* tau variable t (0 tau_p, 1 tau_b1, 2 tau_b2, 3 tau_u1.x, 4 tau_u1.z, 5 tau_u2.x, 6 tau_u2.z):
  * nx/2+1 complex values (tau_p: entry 0 only) */
 int tarang_chan_get_tau(tarang_chan_t s, int t, void* vals);
+/* Nusselt number of the current state, Nu = <uz b - kappa dz b>_V / (kappa (b_bottom - b_top) / Lz),
+ * reduced on the device in FP64 (horizontal mean by Parseval, z mean by Clenshaw-Curtis
+ * quadrature on the Gauss-Lobatto points).  The reference has no Nusselt implementation
+ * (examples/ivp/rayleigh_benard_2d.jl:24 names it only).  Synchronises the context stream. */
+int tarang_chan_nusselt(tarang_chan_t s, double* nu);
 /* F_b, F_ux, F_uz of the current state as the stage loop evaluates them (grid round trip of
  * the state, 3/2-padded products; src/core/timesteppers/state_utils.jl:261-274); NULL skips one */
 int tarang_chan_rhs(tarang_chan_t s, void* Fb, void* Fux, void* Fuz);
@@ -275,6 +288,7 @@ typedef struct {
     int dtype;
     int timestepper;
     double b_bottom, b_top;
+    int refine_sweeps;       /* as in tarang_chan_desc (nz <= 208 when refine_sweeps > 0) */
 } tarang_chan3_desc;
 
 #define TARANG_CHAN3_P 0
@@ -295,6 +309,8 @@ int tarang_chan3_set_bc(tarang_chan3_t s, const void* b_bottom, const void* b_to
 int tarang_chan3_get_field(tarang_chan3_t s, int var, void* coeff);
 /* tau t: 0 tau_p, 1 tau_b1, 2 tau_b2, 3..5 tau_u1.{x,y,z}, 6..8 tau_u2.{x,y,z}; (nx/2+1, ny) values */
 int tarang_chan3_get_tau(tarang_chan3_t s, int t, void* vals);
+/* as tarang_chan_nusselt, horizontal mean over x and y */
+int tarang_chan3_nusselt(tarang_chan3_t s, double* nu);
 /* F_b, F_ux, F_uy, F_uz of the current state (grid round trip, 3/2-padded x-y products) */
 int tarang_chan3_rhs(tarang_chan3_t s, void* Fb, void* Fux, void* Fuy, void* Fuz);
 int tarang_chan3_step(tarang_chan3_t s, int nsteps);

@@ -216,6 +216,30 @@ orc_rbc3 *orc_rbc3_create(int nx, int ny, int nz, double Lx, double Ly, double L
     return s;
 }
 
+/* The right-hand side alone (orc_rbc3_rhs_fields): fields, transforms and Dz, without the
+ * per-mode matrices, so that the 256 x 256 x 128 workload fits in host memory.  Only
+ * set_field / get_field / rhs_fields / destroy may be called on it. */
+orc_rbc3 *orc_rbc3_create_rhs_only(int nx, int ny, int nz, double Lx, double Ly, double Lz) {
+    orc_rbc3 *s = (orc_rbc3 *)calloc(1, sizeof(orc_rbc3));
+    s->nx = nx; s->ny = ny; s->nz = nz; s->nxh = nx / 2 + 1; s->nmodes = s->nxh * ny;
+    s->Lx = Lx; s->Ly = Ly; s->Lz = Lz;
+    s->nfull = NVAR3 * nz + NTAU3;
+    s->dom.ndim = 3;
+    s->dom.kind[0] = ORC_REALFOURIER; s->dom.kind[1] = ORC_REALFOURIER; s->dom.kind[2] = ORC_CHEBYSHEVT;
+    s->dom.n[0] = nx; s->dom.n[1] = ny; s->dom.n[2] = nz;
+    s->dom.lo[0] = 0; s->dom.hi[0] = Lx; s->dom.lo[1] = 0; s->dom.hi[1] = Ly; s->dom.lo[2] = 0; s->dom.hi[2] = Lz;
+    for (int a = 0; a < 3; ++a) s->dom.dealias[a] = 1.5;
+    s->dom2 = s->dom;
+    s->dom2.ndim = 2;
+    const int N = nz;
+    s->D = (double *)calloc((size_t)N * N, sizeof(double));
+    for (int k = 0; k < N; ++k)
+        for (int j = k + 1; j < N; j += 2) s->D[k * N + j] = (k == 0 ? 1.0 : 2.0) * j * (2.0 / Lz);
+    for (int v = 0; v < NVAR3; ++v) s->f[v] = (cplx *)calloc((size_t)s->nmodes * N, sizeof(cplx));
+    for (int t = 0; t < NTAU3; ++t) s->tau[t] = (cplx *)calloc((size_t)s->nmodes, sizeof(cplx));
+    return s;
+}
+
 void orc_rbc3_destroy(orc_rbc3 *s) {
     if (!s) return;
     for (int v = 0; v < NVAR3; ++v) free(s->f[v]);

@@ -236,6 +236,26 @@ orc_rbc *orc_rbc_create(int nx, int nz, double Lx, double Lz, double kappa, doub
     return s;
 }
 
+/* The right-hand side alone (orc_rbc_rhs_fields, lazy_rhs.jl:795-829): fields, transforms and
+ * Dz, without the per-mode matrices, so that workload-size grids fit in host memory.  Only
+ * set_field / get_field / rhs_fields / destroy may be called on it. */
+orc_rbc *orc_rbc_create_rhs_only(int nx, int nz, double Lx, double Lz) {
+    orc_rbc *s = (orc_rbc *)calloc(1, sizeof(orc_rbc));
+    s->nx = nx; s->nz = nz; s->nxh = nx / 2 + 1;
+    s->Lx = Lx; s->Lz = Lz;
+    s->nfull = 4 * nz + NTAU;
+    s->dom.ndim = 2;
+    s->dom.kind[0] = ORC_REALFOURIER; s->dom.kind[1] = ORC_CHEBYSHEVT;
+    s->dom.n[0] = nx; s->dom.n[1] = nz;
+    s->dom.lo[0] = 0; s->dom.hi[0] = Lx; s->dom.lo[1] = 0; s->dom.hi[1] = Lz;
+    s->dom.dealias[0] = 1.5; s->dom.dealias[1] = 1.5;
+    s->D = (double *)malloc(sizeof(double) * nz * nz);
+    build_D(s->D, nz, Lz);
+    for (int v = 0; v < 4; ++v) s->f[v] = (cplx *)calloc((size_t)s->nxh * nz, sizeof(cplx));
+    for (int t = 0; t < NTAU; ++t) s->tau[t] = (cplx *)calloc((size_t)s->nxh, sizeof(cplx));
+    return s;
+}
+
 void orc_rbc_destroy(orc_rbc *s) {
     if (!s) return;
     for (int v = 0; v < 4; ++v) free(s->f[v]);

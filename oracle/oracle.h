@@ -91,6 +91,7 @@ void orc_fivp_diagnostics(orc_fivp *p, double *E, double *Z);
 enum { ORC_RBC_P = 0, ORC_RBC_B = 1, ORC_RBC_UX = 2, ORC_RBC_UZ = 3 };
 typedef struct orc_rbc orc_rbc;
 orc_rbc *orc_rbc_create(int nx, int nz, double Lx, double Lz, double kappa, double nu, double dt, int timestepper);
+orc_rbc *orc_rbc_create_rhs_only(int nx, int nz, double Lx, double Lz);
 void orc_rbc_destroy(orc_rbc *s);
 int orc_rbc_size(const orc_rbc *s, int mode);
 /* Dirichlet values of b (default b(0) = Lz, b(Lz) = 0) */
@@ -116,6 +117,7 @@ enum { ORC_RBC3_P = 0, ORC_RBC3_B = 1, ORC_RBC3_UX = 2, ORC_RBC3_UY = 3, ORC_RBC
 typedef struct orc_rbc3 orc_rbc3;
 orc_rbc3 *orc_rbc3_create(int nx, int ny, int nz, double Lx, double Ly, double Lz, double kappa, double nu, double dt,
                           int timestepper);
+orc_rbc3 *orc_rbc3_create_rhs_only(int nx, int ny, int nz, double Lx, double Ly, double Lz);
 void orc_rbc3_destroy(orc_rbc3 *s);
 /* mode = m + (nx/2+1) n, m the kx index, n the ky index in FFT order */
 int orc_rbc3_size(const orc_rbc3 *s, int mode);

@@ -80,6 +80,8 @@ def lib():
         L.orc_rbc_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                      ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int]
         L.orc_rbc_create.restype = vp
+        L.orc_rbc_create_rhs_only.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.orc_rbc_create_rhs_only.restype = vp
         L.orc_rbc_destroy.argtypes = [vp]
         L.orc_rbc_size.argtypes = [vp, ctypes.c_int]
         L.orc_rbc_set_bc.argtypes = [vp, ctypes.c_double, ctypes.c_double]
@@ -97,6 +99,9 @@ def lib():
         L.orc_rbc3_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int]
         L.orc_rbc3_create.restype = vp
+        L.orc_rbc3_create_rhs_only.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                               ctypes.c_double, ctypes.c_double]
+        L.orc_rbc3_create_rhs_only.restype = vp
         L.orc_rbc3_destroy.argtypes = [vp]
         L.orc_rbc3_size.argtypes = [vp, ctypes.c_int]
         L.orc_rbc3_set_bc.argtypes = [vp, ctypes.c_double, ctypes.c_double]
@@ -301,10 +306,12 @@ RBC_P, RBC_B, RBC_UX, RBC_UZ = 0, 1, 2, 3
 class RBC2D:
     """2-D Rayleigh-Benard channel, Chebyshev-tau per-mode solves (oracle)."""
 
-    def __init__(self, nx, nz, Lx, Lz, kappa, nu, dt, timestepper="RK222"):
+    def __init__(self, nx, nz, Lx, Lz, kappa, nu, dt, timestepper="RK222", rhs_only=False):
+        """rhs_only: no per-mode matrices (set / get / rhs only), for workload-size grids."""
         self.nx, self.nz = nx, nz
         self.shape = (nx // 2 + 1, nz)
-        self.h = lib().orc_rbc_create(nx, nz, Lx, Lz, kappa, nu, dt, TS[timestepper])
+        self.h = (lib().orc_rbc_create_rhs_only(nx, nz, Lx, Lz) if rhs_only
+                  else lib().orc_rbc_create(nx, nz, Lx, Lz, kappa, nu, dt, TS[timestepper]))
         if not self.h:
             raise ValueError("per-mode system is not square after filtering")
         self.d = domain([REALFOURIER, CHEBYSHEVT], [nx, nz], bounds=[(0.0, Lx), (0.0, Lz)])
@@ -380,10 +387,12 @@ RBC3_P, RBC3_B, RBC3_UX, RBC3_UY, RBC3_UZ = range(5)
 class RBC3D:
     """3-D Rayleigh-Benard channel, Chebyshev-tau per-(kx, ky) solves (oracle)."""
 
-    def __init__(self, nx, ny, nz, Lx, Ly, Lz, kappa, nu, dt, timestepper="SBDF2"):
+    def __init__(self, nx, ny, nz, Lx, Ly, Lz, kappa, nu, dt, timestepper="SBDF2", rhs_only=False):
+        """rhs_only: no per-mode matrices (set / get / rhs only), for workload-size grids."""
         self.nx, self.ny, self.nz = nx, ny, nz
         self.shape = (nx // 2 + 1, ny, nz)
-        self.h = lib().orc_rbc3_create(nx, ny, nz, Lx, Ly, Lz, kappa, nu, dt, TS[timestepper])
+        self.h = (lib().orc_rbc3_create_rhs_only(nx, ny, nz, Lx, Ly, Lz) if rhs_only
+                  else lib().orc_rbc3_create(nx, ny, nz, Lx, Ly, Lz, kappa, nu, dt, TS[timestepper]))
         if not self.h:
             raise ValueError("per-mode system is not square after filtering")
         self.d = domain([REALFOURIER, REALFOURIER, CHEBYSHEVT], [nx, ny, nz],

@@ -85,6 +85,7 @@ tarang::ChannelDesc conv_desc(const tarang_chan_desc* desc) {
     d.dtype = desc->dtype;
     d.b_bottom = desc->b_bottom;
     d.b_top = desc->b_top;
+    d.refine_sweeps = desc->refine_sweeps;
     switch (desc->timestepper) {
         case 0: d.stepper = tarang::kRK222; break;
         case 1: d.stepper = tarang::kRK443; break;
@@ -109,6 +110,7 @@ tarang::Channel3Desc conv_desc(const tarang_chan3_desc* desc) {
     d.dtype = desc->dtype;
     d.b_bottom = desc->b_bottom;
     d.b_top = desc->b_top;
+    d.refine_sweeps = desc->refine_sweeps;
     switch (desc->timestepper) {
         case 0: d.stepper = tarang::kRK222; break;
         case 1: d.stepper = tarang::kRK443; break;
@@ -356,6 +358,15 @@ int tarang_fivp_get_state(tarang_fivp_t s, void* coeff) {
     });
 }
 
+int tarang_fivp_diagnostics(tarang_fivp_t s, double* E, double* Z) {
+    return guard([&] {
+        NEED(s, "solver");
+        NEED(E, "E");
+        NEED(Z, "Z");
+        s->solver->diagnostics(E, Z);
+    });
+}
+
 int tarang_fivp_rhs(tarang_fivp_t s, const void* coeff, void* F) {
     return guard([&] {
         NEED(s, "solver");
@@ -494,6 +505,14 @@ int tarang_chan_get_field(tarang_chan_t s, int var, void* coeff) {
     });
 }
 
+int tarang_chan_nusselt(tarang_chan_t s, double* nu) {
+    return guard([&] {
+        NEED(s, "solver");
+        NEED(nu, "nu");
+        *nu = s->solver->nusselt();
+    });
+}
+
 int tarang_chan_get_tau(tarang_chan_t s, int t, void* vals) {
     return guard([&] {
         NEED(s, "solver");
@@ -600,6 +619,14 @@ int tarang_chan3_get_field(tarang_chan3_t s, int var, void* coeff) {
         NEED(s, "solver");
         NEED(coeff, "coeff");
         s->solver->get_field(var, coeff);
+    });
+}
+
+int tarang_chan3_nusselt(tarang_chan3_t s, double* nu) {
+    return guard([&] {
+        NEED(s, "solver");
+        NEED(nu, "nu");
+        *nu = s->solver->nusselt();
     });
 }
 

@@ -378,6 +378,11 @@ struct Ch3SolveArgs {
     size_t sb, su;
     double c;
     const double2* bcv;  // [2 m + w]: b on wall w (0: z=0, 1: z=Lz) at mode m (unnormalised 2-D rfft)
+    // iterative refinement: sweeps of x += A^-1 (y - A x) with the residual formed in FP64 from
+    // the structured systems (chb_residual / chu_residual); 0 = none
+    int sweeps;
+    const double* kh;  // [nkeys] |k_h| per key
+    double kappa, nu, sz, area, Lz;
 };
 
 // 3-D tau slots: 0 tau_p, 1 tau_b1, 2 tau_b2, 3 tau_u1x, 4 tau_u1y, 5 tau_u1z, 6 tau_u2x, 7 tau_u2y, 8 tau_u2z
@@ -386,8 +391,9 @@ constexpr int kCh3ChunkModes = 8;  // modes per workgroup: 16 real right-hand si
 
 __host__ __device__ inline int round16(int n) { return (n + 15) & ~15; }
 
-__host__ __device__ inline size_t ch3_solve_lds_doubles(int N) {
-    return (size_t)kYS * (2 * round16(chb_size(N, 0)) + round16(chu_size(N, 0, 0)));
+// right-hand sides, and with refinement a second block set for the solutions
+__host__ __device__ inline size_t ch3_solve_lds_doubles(int N, bool refine = false) {
+    return (size_t)kYS * (2 * round16(chb_size(N, 0)) + round16(chu_size(N, 0, 0))) * (refine ? 2 : 1);
 }
 
 // Workgroup = (|k_h| key, parity g, up to 8 modes of that key).  Right-hand sides of
@@ -398,13 +404,14 @@ __host__ __device__ inline size_t ch3_solve_lds_doubles(int N) {
 // rounding the right-hand side to FP32), the products accumulate in FP64.  Order: b system of parity h = g^1
 // and the perp system of parity g together, then the u-p system of parity g (its
 // E3z rows take + c b), then the un-rotation of the parity-g horizontal velocity.
-template <typename T, int NT>
-__global__ __launch_bounds__(NT, 768 / NT) void k3_solve(Ch3SolveArgs s, const cplx<T>* __restrict__ R1,
+template <typename T, int NT, bool REFINE = false>
+__global__ __launch_bounds__(NT, REFINE ? 1 : 768 / NT) void k3_solve(Ch3SolveArgs s, const cplx<T>* __restrict__ R1,
                                                const cplx<T>* __restrict__ R2, const cplx<T>* __restrict__ R3x,
                                                const cplx<T>* __restrict__ R3y, const cplx<T>* __restrict__ R3z,
                                                cplx<T>* P, cplx<T>* B, cplx<T>* UX, cplx<T>* UY, cplx<T>* UZ,
                                                cplx<T>* taus) {
     static_assert(NT == 256 || NT == 512, "four or eight wavefronts per workgroup");
+    using TI = T;  // inverses in the state precision
     constexpr int MTB = NT == 256 ? 3 : 2, MTU = NT == 256 ? 7 : 4;  // tiles per wave for nz <= 256
     extern __shared__ unsigned char smem_raw[];
     __shared__ int smode[kCh3ChunkModes];
@@ -419,6 +426,11 @@ __global__ __launch_bounds__(NT, 768 / NT) void k3_solve(Ch3SolveArgs s, const c
     double* Yp = Yb + kYS * round16(nb);                // [round16(np)][kYS]
     double* Yu = Yp + kYS * round16(np);                // [round16(nu)][kYS]
     const int ytot = kYS * (round16(nb) + round16(np) + round16(nu));
+    // solutions: in place over the right-hand sides, or (refinement) in blocks of their own
+    const int sweeps = REFINE ? s.sweeps : 0;  // the LDS holds the solution blocks only when refining
+    double* Xb = sweeps ? Yb + ytot : Yb;
+    double* Xp = sweeps ? Yp + ytot : Yp;
+    double* Xu = sweeps ? Yu + ytot : Yu;
     if (threadIdx.x < kCh3ChunkModes) {
         const int j = threadIdx.x;
         const int q = j < cnt ? s.modes[first + j] : -1;
@@ -426,7 +438,7 @@ __global__ __launch_bounds__(NT, 768 / NT) void k3_solve(Ch3SolveArgs s, const c
         srot[2 * j] = q >= 0 ? s.rot[2 * q] : 1.0;
         srot[2 * j + 1] = q >= 0 ? s.rot[2 * q + 1] : 0.0;
     }
-    for (int idx = threadIdx.x; idx < ytot; idx += NT) Yb[idx] = 0.0;
+    for (int idx = threadIdx.x; idx < (sweeps ? 2 : 1) * ytot; idx += NT) Yb[idx] = 0.0;
     __syncthreads();
     // ---- right-hand sides ----
     for (int idx = threadIdx.x; idx < cnt * nh; idx += NT) {
@@ -460,18 +472,32 @@ __global__ __launch_bounds__(NT, 768 / NT) void k3_solve(Ch3SolveArgs s, const c
     __syncthreads();
     // ---- b (parity h) and perp (parity g) systems ----
     const int wave = threadIdx.x >> 6, nw = NT / 64, w0p = ((nb + 15) >> 4) % nw;
+    const double kh = s.kh[key];
     {
+        const TI* Ab = static_cast<const TI*>(s.Ib) + (size_t)(2 * key + h) * s.sb;
+        const TI* Ap = static_cast<const TI*>(s.Ip) + (size_t)(2 * key + g) * s.sb;
         dbl4 ab[MTB], ap[MTB];
-        mfma_apply<MTB>(static_cast<const T*>(s.Ib) + (size_t)(2 * key + h) * s.sb, nb, Yb, ab, wave, nw, 0);
-        mfma_apply<MTB>(static_cast<const T*>(s.Ip) + (size_t)(2 * key + g) * s.sb, np, Yp, ap, wave, nw, w0p);
+        mfma_apply<MTB>(Ab, nb, Yb, ab, wave, nw, 0);
+        mfma_apply<MTB>(Ap, np, Yp, ap, wave, nw, w0p);
         __syncthreads();
-        mfma_store<MTB>(ab, nb, Yb, wave, nw, 0);
-        mfma_store<MTB>(ap, np, Yp, wave, nw, w0p);
+        mfma_store<MTB>(ab, nb, Xb, wave, nw, 0);
+        mfma_store<MTB>(ap, np, Xp, wave, nw, w0p);
+        for (int sw = 0; REFINE && sw < sweeps; ++sw) {
+            __syncthreads();
+            if (threadIdx.x < 16) chb_residual(N, h, kh, s.kappa, s.c, s.sz, Yb, Xb, kYS, 1, threadIdx.x);
+            else if (threadIdx.x < 32) chb_residual(N, g, kh, s.nu, s.c, s.sz, Yp, Xp, kYS, 1, threadIdx.x - 16);
+            __syncthreads();
+            mfma_apply<MTB>(Ab, nb, Xb, ab, wave, nw, 0, true);  // x += A^-1 r
+            mfma_apply<MTB>(Ap, np, Xp, ap, wave, nw, w0p, true);
+            __syncthreads();
+            mfma_store<MTB>(ab, nb, Xb, wave, nw, 0);
+            mfma_store<MTB>(ap, np, Xp, wave, nw, w0p);
+        }
     }
     __syncthreads();
     for (int idx = threadIdx.x; idx < cnt * nh; idx += NT) {
         const int j = idx / nh, i = idx - j * nh;
-        const double bre = Yb[i * kYS + 2 * j], bim = Yb[i * kYS + 2 * j + 1];
+        const double bre = Xb[i * kYS + 2 * j], bim = Xb[i * kYS + 2 * j + 1];
         Yu[(2 * ng + i) * kYS + 2 * j] += s.c * bre;
         Yu[(2 * ng + i) * kYS + 2 * j + 1] += s.c * bim;
         B[(size_t)smode[j] * N + 2 * i + h] = cplx<T>{T(bre), T(bim)};
@@ -479,15 +505,25 @@ __global__ __launch_bounds__(NT, 768 / NT) void k3_solve(Ch3SolveArgs s, const c
     if (threadIdx.x < cnt) {
         const int j = threadIdx.x;
         taus[(size_t)smode[j] * kCh3TauStride + (h == (PL ^ 1) ? 1 : 2)] =
-            cplx<T>{T(Yb[nh * kYS + 2 * j]), T(Yb[nh * kYS + 2 * j + 1])};
+            cplx<T>{T(Xb[nh * kYS + 2 * j]), T(Xb[nh * kYS + 2 * j + 1])};
     }
     __syncthreads();
     // ---- u-p system of parity g ----
     {
+        const TI* Au = static_cast<const TI*>(s.Iu) + (size_t)(2 * key + g) * s.su;
         dbl4 au[MTU];
-        mfma_apply<MTU>(static_cast<const T*>(s.Iu) + (size_t)(2 * key + g) * s.su, nu, Yu, au, wave, nw, 0);
+        mfma_apply<MTU>(Au, nu, Yu, au, wave, nw, 0);
         __syncthreads();
-        mfma_store<MTU>(au, nu, Yu, wave, nw, 0);
+        mfma_store<MTU>(au, nu, Xu, wave, nw, 0);
+        for (int sw = 0; REFINE && sw < sweeps; ++sw) {
+            __syncthreads();
+            if (threadIdx.x < 16)
+                chu_residual(N, g, key == 0, kh, s.nu, s.c, s.sz, s.area, s.Lz, Yu, Xu, kYS, 1, threadIdx.x);
+            __syncthreads();
+            mfma_apply<MTU>(Au, nu, Xu, au, wave, nw, 0, true);  // x += A^-1 r
+            __syncthreads();
+            mfma_store<MTU>(au, nu, Xu, wave, nw, 0);
+        }
     }
     __syncthreads();
     // un-rotate: u~x = cos par - sin perp, u~y = sin par + cos perp; u = i u~
@@ -495,28 +531,28 @@ __global__ __launch_bounds__(NT, 768 / NT) void k3_solve(Ch3SolveArgs s, const c
         const int j = idx / ng, i = idx - j * ng;
         const size_t o = (size_t)smode[j] * N + 2 * i + g;
         const double cr = srot[2 * j], sr = srot[2 * j + 1];
-        const double par_re = Yu[i * kYS + 2 * j], par_im = Yu[i * kYS + 2 * j + 1];
-        const double pe_re = Yp[i * kYS + 2 * j], pe_im = Yp[i * kYS + 2 * j + 1];
+        const double par_re = Xu[i * kYS + 2 * j], par_im = Xu[i * kYS + 2 * j + 1];
+        const double pe_re = Xp[i * kYS + 2 * j], pe_im = Xp[i * kYS + 2 * j + 1];
         const double tx_re = cr * par_re - sr * pe_re, tx_im = cr * par_im - sr * pe_im;
         const double ty_re = sr * par_re + cr * pe_re, ty_im = sr * par_im + cr * pe_im;
         UX[o] = cplx<T>{T(-tx_im), T(tx_re)};
         UY[o] = cplx<T>{T(-ty_im), T(ty_re)};
-        P[o] = cplx<T>{T(Yu[(ng + i) * kYS + 2 * j]), T(Yu[(ng + i) * kYS + 2 * j + 1])};
+        P[o] = cplx<T>{T(Xu[(ng + i) * kYS + 2 * j]), T(Xu[(ng + i) * kYS + 2 * j + 1])};
     }
     for (int idx = threadIdx.x; idx < cnt * nh; idx += NT) {
         const int j = idx / nh, i = idx - j * nh;
         UZ[(size_t)smode[j] * N + 2 * i + h] =
-            cplx<T>{T(Yu[(2 * ng + i) * kYS + 2 * j]), T(Yu[(2 * ng + i) * kYS + 2 * j + 1])};
+            cplx<T>{T(Xu[(2 * ng + i) * kYS + 2 * j]), T(Xu[(2 * ng + i) * kYS + 2 * j + 1])};
     }
     if (threadIdx.x < cnt) {
         const int j = threadIdx.x, mode = smode[j];
         const double cr = srot[2 * j], sr = srot[2 * j + 1];
-        const double pt_re = Yp[ng * kYS + 2 * j], pt_im = Yp[ng * kYS + 2 * j + 1];  // perp-system tau
+        const double pt_re = Xp[ng * kYS + 2 * j], pt_im = Xp[ng * kYS + 2 * j + 1];  // perp-system tau
         cplx<T>* tm = taus + (size_t)mode * kCh3TauStride;
         const int t0 = 2 * ng + nh;
         for (int t = 0; t < nu - t0; ++t) {
             const int slot = chu_tau_slot(N, g, t);  // 2-D slots: 0 p, 3 u1x, 4 u1z, 5 u2x, 6 u2z
-            const double re = Yu[(t0 + t) * kYS + 2 * j], im = Yu[(t0 + t) * kYS + 2 * j + 1];
+            const double re = Xu[(t0 + t) * kYS + 2 * j], im = Xu[(t0 + t) * kYS + 2 * j + 1];
             if (slot == 0) {
                 tm[0] = cplx<T>{T(re), T(im)};
             } else if (slot == 4 || slot == 6) {

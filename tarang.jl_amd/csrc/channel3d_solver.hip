@@ -16,6 +16,7 @@
 
 #include "channel3d_kernels.hpp"
 #include "channel3d_solver.hpp"
+#include "diagnostics.hpp"
 #include "flow_tools.hpp"
 #include "dispatch.hpp"
 
@@ -39,6 +40,10 @@ void Channel3Solver::validate(const Channel3Desc& d) {
                    " unsupported (powers of 2; x 8.." + std::to_string(xmax) + ", y 8..1024)");
     TG_REQUIRE(d.nz >= 4 && d.nz <= 256, kErrInvalid,
                "3-D channel solver: ChebyshevT size " + std::to_string(d.nz) + " unsupported (4..256)");
+    TG_REQUIRE(d.refine_sweeps >= 0 && d.refine_sweeps <= 2, kErrInvalid,
+               "3-D channel solver: refine_sweeps must be 0, 1 or 2");
+    TG_REQUIRE(d.refine_sweeps == 0 || ch3_solve_lds_doubles(d.nz, true) * sizeof(double) <= 160 * 1024, kErrInvalid,
+               "3-D channel solver: refinement sweeps need nz <= 208 (LDS of the solve)");
     TG_REQUIRE(d.Lx > 0 && d.Ly > 0 && d.Lz > 0 && d.dt > 0, kErrInvalid,
                "3-D channel solver: lengths and dt must be positive");
 }
@@ -425,13 +430,18 @@ void Channel3Solver::solve_t() {
     const int N = d_.nz;
     Ch3SolveArgs sa{Ib_.ptr, Ip_.ptr, Iu_.ptr,
                     chunks_.as<const int4>(), modes_.as<const int>(), rot_.as<const double>(), N,
-                    (size_t)nb_max_ * nb_max_, (size_t)nu_max_ * nu_max_, factored_c_, bcv_.as<const double2>()};
+                    (size_t)nb_max_ * nb_max_, (size_t)nu_max_ * nu_max_, factored_c_, bcv_.as<const double2>(),
+                    d_.refine_sweeps, kh_.as<const double>(), d_.kappa, d_.nu, 2.0 / d_.Lz, d_.Lx * d_.Ly, d_.Lz};
     constexpr int NT = 256;  // four wavefronts: at most 3 b/perp and 7 u-p tiles each (nz <= 256)
-    const size_t lds = ch3_solve_lds_doubles(N) * sizeof(double);
+    const size_t lds = ch3_solve_lds_doubles(N, d_.refine_sweeps > 0) * sizeof(double);
     timed(6, [&] {
-        hipLaunchKernelGGL((k3_solve<T, NT>), dim3(nchunks_), dim3(NT), lds, ctx_->stream, sa, R_[0].as<const V>(),
-                           R_[1].as<const V>(), R_[2].as<const V>(), R_[3].as<const V>(), R_[4].as<const V>(),
-                           P_.as<V>(), B_.as<V>(), UX_.as<V>(), UY_.as<V>(), UZ_.as<V>(), TAU_.as<V>());
+        auto launch = [&](auto kern) {
+            hipLaunchKernelGGL(kern, dim3(nchunks_), dim3(NT), lds, ctx_->stream, sa, R_[0].as<const V>(),
+                               R_[1].as<const V>(), R_[2].as<const V>(), R_[3].as<const V>(), R_[4].as<const V>(),
+                               P_.as<V>(), B_.as<V>(), UX_.as<V>(), UY_.as<V>(), UZ_.as<V>(), TAU_.as<V>());
+        };
+        if (d_.refine_sweeps > 0) launch(k3_solve<T, NT, true>);
+        else launch(k3_solve<T, NT, false>);
     });
     TG_HIP(hipGetLastError());
 }
@@ -649,16 +659,30 @@ void Channel3Solver::kernel_bytes(double* out) const {
     out[7] = 10 * nc * vb;  // L x: read p, b, u (and partners on two lines), write five rows
 }
 
+double Channel3Solver::nusselt() {
+    ctx_->activate();
+    return channel_nusselt(ctx_, B_.ptr, UZ_.ptr, (DType)d_.dtype, d_.nx, d_.ny, d_.nz, d_.Lz, d_.kappa,
+                           d_.b_bottom - d_.b_top);
+}
+
 size_t footprint(const Channel3Desc& d) {
+    // every buffer the constructor allocates, in 2 MiB pages, plus the set-up scratch
     const size_t nxh = d.nx / 2 + 1, nmodes = nxh * d.ny, N = d.nz, vb = 2 * dtype_size((DType)d.dtype);
-    const size_t fb = nmodes * N * vb, prow = N * (size_t)padded_len(d.ny) * nxh * vb;
-    const size_t fields = (5 + 4 + 4 + 4 + 5 + kCh3Planes + 1) * fb + (kCh3Planes + 4) * prow;
+    const size_t fb = nmodes * N * vb, prow = N * (size_t)padded_len(d.ny) * nxh * vb, ib = dtype_size((DType)d.dtype);
+    const bool ms = d.stepper == kSBDF2 || d.stepper == kSBDF1;
+    const size_t S = ms ? 0 : (size_t)make_tableau(d.stepper).stages;
+    const size_t nfields = 5 + 12 + 5 + (ms ? 0 : 4 + 9 * S);  // state, history, rhs rows, RK stage rows
     const size_t nb = chb_size(d.nz, 0), nu = chu_size(d.nz, 0, 0);
-    // distinct |k_h|^2 keys: at most one per mode (about nmodes / 4 when Lx == Ly)
-    const size_t keys = d.Lx == d.Ly ? nmodes / 3 + 64 : nmodes;
-    // inverses in the state precision, plus the FP64 LU scratch while they are formed
-    const size_t ent = 2 * keys * (2 * nb * nb + nu * nu);
-    return fields + ent * (8 + dtype_size((DType)d.dtype)) + 2 * keys * (2 * nb + nu) * 4 + 3 * N * N * 8 + (1 << 20);
+    // distinct |k_h|^2 keys: at most one per mode (about nmodes / 5.6 at 256 x 256 when Lx == Ly)
+    const size_t keys = d.Lx == d.Ly ? nmodes / 3 + 64 : nmodes, nsys = 2 * keys;
+    size_t b = nfields * alloc_granule(fb) + alloc_granule(kCh3Planes * fb) + alloc_granule(kCh3Planes * prow) +
+               alloc_granule(4 * prow);                                                           // G, Q, H
+    b += alloc_granule(nmodes * kCh3TauStride * vb) + 3 * alloc_granule(N * N * ib);           // taus, tables
+    b += 2 * alloc_granule(nsys * nb * nb * ib) + alloc_granule(nsys * nu * nu * ib);           // inverses
+    b += 6 * alloc_granule(nmodes * 16);                                                      // keys, rotations, walls
+    b += 2 * alloc_granule(nsys * nb * nb * 8) + alloc_granule(nsys * nu * nu * 8) +
+         alloc_granule(nsys * (2 * nb + nu) * 4);                                             // FP64 LU scratch
+    return b;
 }
 
 void Channel3Solver::flag_nonfinite(int* dflag) const {

@@ -21,6 +21,7 @@ struct Channel3Desc {
     int dtype;               // kF32 / kF64 (state, RHS, stage arithmetic; solves are FP64)
     int stepper;             // kSBDF2 / kSBDF1 (per-mode multistep) or kRK222 / kRK443 (IMEX RK)
     double b_bottom, b_top;  // Dirichlet values of b at z = 0 and z = Lz
+    int refine_sweeps = 0;   // FP64-residual refinement sweeps per implicit solve (0..2)
 };
 
 enum Channel3Var : int { kC3P = 0, kC3B = 1, kC3UX = 2, kC3UY = 3, kC3UZ = 4 };
@@ -44,6 +45,9 @@ class Channel3Solver {
     void get_tau(int t, void* dev);
     void rhs(void* Fb, void* Fux, void* Fuy, void* Fuz);
     void step(int nsteps);
+    // Nusselt number of the current state, reduced on the device (diagnostics.hpp), with the
+    // descriptor's wall values b_bottom - b_top as the imposed difference
+    double nusselt();
     // *dflag = 1 if the state holds a NaN or an infinity (device int, asynchronous)
     void flag_nonfinite(int* dflag) const;
     void set_dt(double dt);

@@ -609,6 +609,92 @@ __host__ __device__ inline double chu_entry(const ChSys& s, int g, int m, int ro
     return chu_entry_kh(s.N, g, m == 0, s.k0x * m, s.nu, s.c, s.sz, s.Lx, s.Lz, row, col);
 }
 
+// ---------------------------------------------------------------------------
+// FP64 residuals of the parity systems for iterative refinement, one right-hand side
+// column at a time: on entry X holds the unknowns (chb / chu column order), on exit the
+// residual R = Y - A X of the same system (chb / chu row order), in place.  A is applied
+// in structured form: Dz and Dzz through the descending coefficient recurrences
+//   (Dz f)_m = (Dz f)_{m+2} + 2 (m+1) sz f_{m+1},  halved at m = 0,
+// O(N) per column.  Element (row, col) of Y and X is at [row * rs + col * cs].
+// ---------------------------------------------------------------------------
+__host__ __device__ inline void chb_residual(int N, int q, double kh, double diff, double c, double sz,
+                                             const double* Y, double* X, int rs, int cs, int col) {
+    const int PL = (N - 1) & 1, nq = npar(N, q);
+    const double alpha = 1.0 + c * diff * kh * kh, beta = c * diff;
+    const double tau = X[nq * rs + col * cs];
+    double sum = 0.0, w = 0.0, z = 0.0;  // w = Dz x at degree k+1, z = Dzz x at degree k+2
+    for (int i = nq - 1; i >= 0; --i) {
+        const int k = q + 2 * i;
+        double zk = z + 2.0 * (k + 1) * sz * w;
+        if (k == 0) zk *= 0.5;
+        const double x = X[i * rs + col * cs];
+        sum += x;
+        const double tc = q == (PL ^ 1) ? -beta * cheb_dz(k, N - 1, sz) : (k == N - 1 ? c : 0.0);
+        X[i * rs + col * cs] = Y[i * rs + col * cs] - (alpha * x - beta * zk + tc * tau);
+        if (k >= 1) {
+            w += 2.0 * k * sz * x;
+            if (k == 1) w *= 0.5;
+        }
+        z = zk;
+    }
+    X[nq * rs + col * cs] = Y[nq * rs + col * cs] - c * sum;
+}
+
+__host__ __device__ inline void chu_residual(int N, int g, bool mode0, double kh, double nu, double c, double sz,
+                                             double area, double Lz, const double* Y, double* X, int rs, int cs, int col) {
+    const int PL = (N - 1) & 1, h = g ^ 1, ng = npar(N, g), nh = npar(N, h), t0 = 2 * ng + nh;
+    const bool tp = mode0 && g == 0;
+    const double alpha = 1.0 + c * nu * kh * kh, beta = c * nu;
+    const double tau0 = X[t0 * rs + col * cs], tau1 = X[(t0 + 1) * rs + col * cs], taup = tp ? X[(t0 + 2) * rs + col * cs] : 0.0;
+    double su = 0.0, suz = 0.0, sint = 0.0;
+    double wu = 0.0, zu = 0.0;  // Dz u (parity h), Dzz u (parity g)
+    double wz = 0.0, zz = 0.0;  // Dz uz (parity g), Dzz uz (parity h)
+    double wp = 0.0;            // Dz p (parity h)
+    for (int k = N - 1; k >= 0; --k) {
+        if ((k & 1) == g) {
+            const int i = (k - g) >> 1;
+            double zuk = zu + 2.0 * (k + 1) * sz * wu;
+            if (k == 0) zuk *= 0.5;
+            const double u = X[i * rs + col * cs], p = X[(ng + i) * rs + col * cs];
+            su += u;
+            if (tp) sint += p * 2.0 / (1.0 - (double)k * k);
+            double e3x = alpha * u - beta * zuk + c * kh * p;
+            e3x += g == PL ? (k == N - 1 ? c * tau0 : 0.0) : -beta * cheb_dz(k, N - 1, sz) * tau0;
+            double e1 = -c * kh * u + c * wz;  // wz = Dz uz at degree k
+            if (g == PL && k == N - 1) e1 += c * tau1;
+            if (tp && k == 0) e1 += c * taup;
+            X[i * rs + col * cs] = Y[i * rs + col * cs] - e3x;
+            X[(ng + i) * rs + col * cs] = Y[(ng + i) * rs + col * cs] - e1;
+            zu = zuk;
+            if (k >= 1) {
+                wu += 2.0 * k * sz * u;
+                wp += 2.0 * k * sz * p;
+                if (k == 1) {
+                    wu *= 0.5;
+                    wp *= 0.5;
+                }
+            }
+        } else {
+            const int i = (k - h) >> 1;
+            double zzk = zz + 2.0 * (k + 1) * sz * wz;  // wz = Dz uz at degree k+1
+            if (k == 0) zzk *= 0.5;
+            const double uz = X[(2 * ng + i) * rs + col * cs];
+            suz += uz;
+            double e3z = alpha * uz - beta * zzk + c * wp;  // wp = Dz p at degree k
+            e3z += g == PL ? -beta * cheb_dz(k, N - 1, sz) * tau1 : (k == N - 1 ? c * tau1 : 0.0);
+            X[(2 * ng + i) * rs + col * cs] = Y[(2 * ng + i) * rs + col * cs] - e3z;
+            zz = zzk;
+            if (k >= 1) {
+                wz += 2.0 * k * sz * uz;
+                if (k == 1) wz *= 0.5;
+            }
+        }
+    }
+    X[t0 * rs + col * cs] = Y[t0 * rs + col * cs] - c * su;
+    X[(t0 + 1) * rs + col * cs] = Y[(t0 + 1) * rs + col * cs] - c * suz;
+    if (tp) X[(t0 + 2) * rs + col * cs] = Y[(t0 + 2) * rs + col * cs] - c * area * (Lz / 2.0) * sint;
+}
+
 // grid (2 nxh, 2): blockIdx.x = 2 m + g, blockIdx.y = 0 b system / 1 u-p system
 static __global__ void k_ch_assemble(ChSys s, double* Ab, double* Aup) {
     const int w = blockIdx.x, m = w >> 1, g = w & 1;
@@ -1056,14 +1142,16 @@ __device__ inline void mfma_apply_n(const AT* __restrict__ A, int n, const doubl
     }
 }
 
+// accumulate = true adds A Y to acc instead of overwriting it (a refinement correction)
 template <int MT, typename AT>
 __device__ inline void mfma_apply(const AT* __restrict__ A, int n, const double* Y, dbl4 (&acc)[MT], int wave,
-                                  int nw, int wave0) {
+                                  int nw, int wave0, bool accumulate = false) {
     const int ntiles = (n + 15) >> 4;
     const int t0 = ((wave - wave0) % nw + nw) % nw;  // first tile of this wave
     const int ntw = t0 < ntiles ? (ntiles - t0 + nw - 1) / nw : 0;
+    if (!accumulate)
 #pragma unroll
-    for (int j = 0; j < MT; ++j) acc[j] = dbl4{0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < MT; ++j) acc[j] = dbl4{0.0, 0.0, 0.0, 0.0};
     constexpr bool F32 = sizeof(AT) == 4;  // batch depth: both load buffers stay within the registers
     switch (ntw < MT ? ntw : MT) {         // wave-uniform
         case 1: mfma_apply_n<1, 4>(A, n, Y, acc, t0, nw); break;
@@ -1100,6 +1188,10 @@ struct ChSolveArgs {
     int ld;             // LDS row length of one right-hand side (>= largest system)
     double c;           // implicit coefficient of the step
     const double2* bcv; // [2 m + w]: b on wall w (0: z=0, 1: z=Lz) at mode m, unnormalised rfft of the wall values
+    // iterative refinement: sweeps of x += A^-1 (y - A x), the residual formed in FP64 from the
+    // structured systems (chb_residual / chu_residual); 0 = none
+    int sweeps;
+    double k0x, kappa, nu, sz, Lx, Lz;
 };
 
 // Wall values of b -> per-mode boundary values: the unnormalised rfft along x (and FFT
@@ -1198,13 +1290,16 @@ __device__ inline void gemv2(const AT* __restrict__ A, int n, const double* y, i
 }
 
 // LDS doubles of k_ch_solve: right-hand sides y and solutions x (2 x ld each)
-__host__ __device__ inline size_t ch_solve_lds_doubles(int ld) { return (size_t)4 * ld; }
+// (refinement adds the residual and the correction, two more)
+__host__ __device__ inline size_t ch_solve_lds_doubles(int ld, bool refine = false) {
+    return (size_t)(refine ? 8 : 4) * ld;
+}
 
 // grid 2 nxh: workgroup (m, g) applies the explicit inverse of the b system of parity
 // h = g^1 (whose values enter the E3z rows of u-p group g as buoyancy) and then of
 // u-p group g.  Inverses in the state precision T, products in FP64.
 // rhs rows E1, E2, E3x, E3z (mode-major) in; state p, b, ux, uz, taus out.
-template <typename T, int NT>
+template <typename T, int NT, bool REFINE = false>
 __global__ __launch_bounds__(NT) void k_ch_solve(ChSolveArgs s, const cplx<T>* __restrict__ R1,
                                                  const cplx<T>* __restrict__ R2, const cplx<T>* __restrict__ R3x,
                                                  const cplx<T>* __restrict__ R3z, cplx<T>* P, cplx<T>* B, cplx<T>* UX,
@@ -1215,6 +1310,22 @@ __global__ __launch_bounds__(NT) void k_ch_solve(ChSolveArgs s, const cplx<T>* _
     const int PL = (N - 1) & 1;
     double* y = reinterpret_cast<double*>(smem_raw);  // [2][ld] right-hand sides
     double* x = y + 2 * ld;                             // [2][ld] solutions
+    double* r = x + 2 * ld;                             // [2][ld] residuals (refinement)
+    double* dx = r + 2 * ld;                            // [2][ld] corrections (refinement)
+    const double kh = s.k0x * m;
+    // x += A^-1 (y - A x), sweeps times; `resid` forms y - A x of one column into r
+    auto refine = [&](const T* Ainv, int n, auto resid) {
+        for (int sw = 0; REFINE && sw < s.sweeps; ++sw) {
+            for (int i = threadIdx.x; i < 2 * ld; i += NT) r[i] = x[i];
+            __syncthreads();
+            if (threadIdx.x < 2) resid(threadIdx.x);
+            __syncthreads();
+            gemv2<NT>(Ainv, n, r, ld, dx);
+            __syncthreads();
+            for (int i = threadIdx.x; i < 2 * ld; i += NT) x[i] += dx[i];
+            __syncthreads();
+        }
+    };
     const size_t base = (size_t)m * N;
     // ---- b system of parity h ----
     const int nh = npar(N, h), nbh = nh + 1;
@@ -1232,6 +1343,8 @@ __global__ __launch_bounds__(NT) void k_ch_solve(ChSolveArgs s, const cplx<T>* _
     __syncthreads();
     gemv2<NT>(static_cast<const T*>(s.Ib) + (size_t)(2 * m + h) * s.sb, nbh, y, ld, x);
     __syncthreads();
+    refine(static_cast<const T*>(s.Ib) + (size_t)(2 * m + h) * s.sb, nbh,
+           [&](int col) { chb_residual(N, h, kh, s.kappa, s.c, s.sz, y, r, 1, ld, col); });
     // ---- u-p system of parity g (buoyancy moved to the right-hand side) ----
     const int ng = npar(N, g);
     const int n = chu_size(N, g, m);
@@ -1256,6 +1369,8 @@ __global__ __launch_bounds__(NT) void k_ch_solve(ChSolveArgs s, const cplx<T>* _
     __syncthreads();
     gemv2<NT>(static_cast<const T*>(s.Iu) + (size_t)wg * s.su, n, y, ld, x);
     __syncthreads();
+    refine(static_cast<const T*>(s.Iu) + (size_t)wg * s.su, n,
+           [&](int col) { chu_residual(N, g, m == 0, kh, s.nu, s.c, s.sz, s.Lx, s.Lz, y, r, 1, ld, col); });
     for (int i = threadIdx.x; i < ng; i += NT) {
         UX[base + 2 * i + g] = cplx<T>{T(-x[ld + i]), T(x[i])};  // ux = i ux~
         P[base + 2 * i + g] = cplx<T>{T(x[ng + i]), T(x[ld + ng + i])};

@@ -15,6 +15,7 @@
 
 #include "channel_kernels.hpp"
 #include "channel_solver.hpp"
+#include "diagnostics.hpp"
 #include "flow_tools.hpp"
 #include "dispatch.hpp"
 
@@ -39,6 +40,8 @@ void ChannelSolver::validate(const ChannelDesc& d) {
                    " unsupported (powers of 2, 8..2048; FP64 up to 1024)");
     TG_REQUIRE(d.nz >= 4 && d.nz <= 512, kErrInvalid,
                "channel solver: ChebyshevT size " + std::to_string(d.nz) + " unsupported (4..512)");
+    TG_REQUIRE(d.refine_sweeps >= 0 && d.refine_sweeps <= 2, kErrInvalid,
+               "channel solver: refine_sweeps must be 0, 1 or 2");
     TG_REQUIRE(d.Lx > 0 && d.Lz > 0 && d.dt > 0, kErrInvalid, "channel solver: Lx, Lz and dt must be positive");
 }
 
@@ -344,13 +347,18 @@ void ChannelSolver::solve_t() {
     using V = cplx<T>;
     const int N = d_.nz;
     ChSolveArgs sa{Ib_.ptr, Iu_.ptr, N, nxh_, (size_t)nb_max_ * nb_max_, (size_t)nu_max_ * nu_max_, nu_max_,
-                   factored_c_, bcv_.as<const double2>()};
+                   factored_c_, bcv_.as<const double2>(), d_.refine_sweeps, 2.0 * M_PI / d_.Lx, d_.kappa, d_.nu,
+                   2.0 / d_.Lz, d_.Lx, d_.Lz};
     constexpr int NT = 256;
-    const size_t lds = ch_solve_lds_doubles(nu_max_) * sizeof(double);
+    const size_t lds = ch_solve_lds_doubles(nu_max_, d_.refine_sweeps > 0) * sizeof(double);
     timed(4, [&] {
-        hipLaunchKernelGGL((k_ch_solve<T, NT>), dim3(2 * nxh_), dim3(NT), lds, ctx_->stream, sa, R_[0].as<const V>(),
-                           R_[1].as<const V>(), R_[2].as<const V>(), R_[3].as<const V>(), P_.as<V>(), B_.as<V>(),
-                           UX_.as<V>(), UZ_.as<V>(), TAU_.as<V>());
+        auto launch = [&](auto kern) {
+            hipLaunchKernelGGL(kern, dim3(2 * nxh_), dim3(NT), lds, ctx_->stream, sa, R_[0].as<const V>(),
+                               R_[1].as<const V>(), R_[2].as<const V>(), R_[3].as<const V>(), P_.as<V>(), B_.as<V>(),
+                               UX_.as<V>(), UZ_.as<V>(), TAU_.as<V>());
+        };
+        if (d_.refine_sweeps > 0) launch(k_ch_solve<T, NT, true>);
+        else launch(k_ch_solve<T, NT, false>);
     });
     TG_HIP(hipGetLastError());
 }
@@ -557,14 +565,27 @@ void ChannelSolver::kernel_bytes(double* out) const {
     out[5] = 0;
 }
 
+double ChannelSolver::nusselt() {
+    ctx_->activate();
+    return channel_nusselt(ctx_, B_.ptr, UZ_.ptr, (DType)d_.dtype, d_.nx, 1, d_.nz, d_.Lz, d_.kappa,
+                           d_.b_bottom - d_.b_top);
+}
+
 size_t footprint(const ChannelDesc& d) {
-    const size_t nxh = d.nx / 2 + 1, N = d.nz, fb = nxh * N * 2 * dtype_size((DType)d.dtype);
-    const size_t S = (d.stepper == kSBDF2 || d.stepper == kSBDF1) ? 3 : (size_t)make_tableau(d.stepper).stages;
-    const size_t fields = (4 + 3 + 7 * S + 4 + 8 + 3 + 6) * fb;
-    const size_t nb = chb_size(d.nz, 0), nu = chu_size(d.nz, 0, 0);
-    // inverses in the state precision, plus the FP64 LU scratch while they are formed
-    const size_t ent = 2 * nxh * (nb * nb + nu * nu);
-    return fields + ent * (8 + dtype_size((DType)d.dtype)) + 2 * nxh * (nb + nu) * 4 + 3 * N * N * 8 + (1 << 20);
+    // every buffer the constructor allocates, in 2 MiB pages, plus the set-up scratch
+    const size_t nxh = d.nx / 2 + 1, N = d.nz, vb = 2 * dtype_size((DType)d.dtype), fb = nxh * N * vb;
+    const bool ms = d.stepper == kSBDF2 || d.stepper == kSBDF1;
+    const size_t S = ms ? 0 : (size_t)make_tableau(d.stepper).stages;
+    const size_t nfields = 4 + 3 + 4 + (ms ? 9 : 7 * S);  // state, X0, rhs rows, history or stage rows
+    const size_t nb = chb_size(d.nz, 0), nu = chu_size(d.nz, 0, 0), ib = dtype_size((DType)d.dtype);
+    const size_t nsys = 2 * nxh;
+    size_t b = nfields * alloc_granule(fb) + alloc_granule(8 * fb) + alloc_granule(3 * fb);  // G, H
+    b += alloc_granule(nxh * 8 * vb) + 3 * alloc_granule(N * N * ib);                       // taus, tables
+    b += alloc_granule(nsys * nb * nb * ib) + alloc_granule(nsys * nu * nu * ib);            // inverses
+    b += 3 * alloc_granule(1);                                                               // info, walls
+    // FP64 LU scratch and pivots while the inverses are formed
+    b += alloc_granule(nsys * nb * nb * 8) + alloc_granule(nsys * nu * nu * 8) + alloc_granule(nsys * (nb + nu) * 4);
+    return b;
 }
 
 void ChannelSolver::flag_nonfinite(int* dflag) const {

@@ -19,6 +19,7 @@ struct ChannelDesc {
     int dtype;           // kF32 / kF64 (state and RHS; the implicit solves are FP64)
     int stepper;         // kRK222 / kRK443 / kSBDF2 / kSBDF1
     double b_bottom, b_top;  // Dirichlet values of b at z = 0 and z = Lz
+    int refine_sweeps = 0;   // FP64-residual refinement sweeps per implicit solve (0..2)
 };
 
 enum ChannelVar : int { kChP = 0, kChB = 1, kChUX = 2, kChUZ = 3 };
@@ -42,6 +43,9 @@ class ChannelSolver {
     // F_b, F_ux, F_uz of the current state (as the timestepper evaluates it), reference layout
     void rhs(void* Fb, void* Fux, void* Fuz);
     void step(int nsteps);
+    // Nusselt number of the current state, reduced on the device (diagnostics.hpp), with the
+    // descriptor's wall values b_bottom - b_top as the imposed difference
+    double nusselt();
     // *dflag = 1 if the state holds a NaN or an infinity (device int, asynchronous)
     void flag_nonfinite(int* dflag) const;
     void set_dt(double dt);

@@ -13,6 +13,7 @@
 #include "dispatch.hpp"
 #include "fourier_kernels.hpp"
 #include "fourier_solver.hpp"
+#include "diagnostics.hpp"
 #include "flow_tools.hpp"
 
 namespace tarang {
@@ -608,13 +609,22 @@ void FourierSolver::step(int nsteps) {
     for (int i = 0; i < nsteps; ++i) step_once(true);
 }
 
+void FourierSolver::diagnostics(double* E, double* Z) {
+    ctx_->activate();
+    const double k0x = 2.0 * M_PI / d_.L[0], k0y = d_.ndim > 1 ? 2.0 * M_PI / d_.L[1] : 0.0;
+    spectral_energy(ctx_, X_.ptr, (DType)d_.dtype, d_.n[0], d_.ndim > 1 ? d_.n[1] : 1, k0x, k0y,
+                    d_.problem == kTurb2D, E, Z);
+}
+
 size_t footprint(const FourierDesc& d) {
     const size_t nxh = d.n[0] / 2 + 1, ny = d.ndim > 1 ? d.n[1] : 1, my = d.ndim > 1 ? padded_len(d.n[1]) : 1;
     const size_t vb = 2 * dtype_size((DType)d.dtype), cb = nxh * ny * vb;
     const Tableau t = make_tableau(d.stepper);
-    const size_t state = is_multistep(d.stepper) ? 4 * cb : (1 + 2 * (size_t)t.stages) * cb;
-    const size_t work = (d.problem != kDiffusion && d.ndim == 2) ? 5 * nxh * my * vb : 0;
-    return state + work + (1 << 20);
+    // X plus (Xm1, Fm1, F0) or (F_s, Y_s for s > 0); G and H; twiddle tables of new lengths
+    const size_t nstate = is_multistep(d.stepper) ? 4 : 2 * (size_t)t.stages;
+    const size_t work = (d.problem != kDiffusion && d.ndim == 2)
+                            ? alloc_granule(4 * nxh * my * vb) + alloc_granule(nxh * my * vb) : 0;
+    return nstate * alloc_granule(cb) + work + 4 * alloc_granule(1);
 }
 
 void FourierSolver::flag_nonfinite(int* dflag) const {

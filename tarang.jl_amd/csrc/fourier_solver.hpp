@@ -43,6 +43,9 @@ class FourierSolver {
     FourierSolver(Context* ctx, const FourierDesc& d);
     ~FourierSolver();
     size_t coeff_elems() const { return (size_t)nxh_ * ny_; }
+    // energy and enstrophy of the current state reduced on the device (diagnostics.hpp):
+    // the vorticity problem E = 1/2 <u^2+v^2>, Z = 1/2 <q^2>; otherwise 1/2 <f^2>, 1/2 <|grad f|^2>
+    void diagnostics(double* E, double* Z);
     size_t coeff_bytes() const { return coeff_elems() * 2 * dtype_size((DType)d_.dtype); }
     void set_state(const void* dev, bool restart = true);  // device -> device; restart: clock, history
     void get_state(void* dev) const;           // device -> device

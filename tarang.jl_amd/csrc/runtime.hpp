@@ -52,5 +52,10 @@ struct DevBuf {
 };
 
 inline size_t dtype_size(DType dt) { return dt == kF32 ? 4 : 8; }
+// device bytes a hipMalloc of b bytes holds: large allocations come in 2 MiB pages
+inline size_t alloc_granule(size_t b) {
+    const size_t g = size_t(2) << 20;
+    return (b + g - 1) / g * g;
+}
 
 }  // namespace tarang

@@ -61,6 +61,7 @@ class ChanDesc(ctypes.Structure):
         ("timestepper", ctypes.c_int),
         ("b_bottom", ctypes.c_double),
         ("b_top", ctypes.c_double),
+        ("refine_sweeps", ctypes.c_int),
     ]
 
 
@@ -82,6 +83,7 @@ class Chan3Desc(ctypes.Structure):
         ("timestepper", ctypes.c_int),
         ("b_bottom", ctypes.c_double),
         ("b_top", ctypes.c_double),
+        ("refine_sweeps", ctypes.c_int),
     ]
 
 
@@ -129,6 +131,9 @@ SIGNATURES = {
     "tarang_cheb_deriv": [_P, _P, _P, _I, _PI, _I, _I, _D, _I, _I],
     "tarang_batched_lu_factor": [_P, _P, _I, _I, _P, _PI],
     "tarang_cfl_frequency": [_P, _I, _PP, ctypes.c_size_t, _PD, _I, _PD],
+    "tarang_fivp_diagnostics": [_P, _PD, _PD],
+    "tarang_chan_nusselt": [_P, _PD],
+    "tarang_chan3_nusselt": [_P, _PD],
     "tarang_batched_lu_solve": [_P, _P, _P, _I, _I, _P, _P],
     "tarang_fivp_create": [_P, ctypes.POINTER(FivpDesc), _PP],
     "tarang_fivp_destroy": [_P],

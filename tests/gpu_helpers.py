@@ -69,8 +69,9 @@ class Device:
         return Fivp(self, h, sizes, dtype)
 
 
-    def chan(self, nx, nz, Lx, Lz, kappa, nu, dt, stepper=0, dtype=np.float64, b_bottom=None, b_top=0.0):
+    def chan(self, nx, nz, Lx, Lz, kappa, nu, dt, stepper=0, dtype=np.float64, b_bottom=None, b_top=0.0, refine=0):
         d = _lib.ChanDesc()
+        d.refine_sweeps = refine
         d.nx, d.nz, d.Lx, d.Lz = nx, nz, Lx, Lz
         d.kappa, d.nu, d.dt = kappa, nu, dt
         d.dtype = _lib.F32 if dtype == np.float32 else _lib.F64
@@ -82,8 +83,10 @@ class Device:
         return Chan(self, h, nx, nz, dtype)
 
 
-    def chan3(self, nx, ny, nz, Lx, Ly, Lz, kappa, nu, dt, stepper=2, dtype=np.float64, b_bottom=None, b_top=0.0):
+    def chan3(self, nx, ny, nz, Lx, Ly, Lz, kappa, nu, dt, stepper=2, dtype=np.float64, b_bottom=None, b_top=0.0,
+              refine=0):
         d = _lib.Chan3Desc()
+        d.refine_sweeps = refine
         d.nx, d.ny, d.nz, d.Lx, d.Ly, d.Lz = nx, ny, nz, Lx, Ly, Lz
         d.kappa, d.nu, d.dt = kappa, nu, dt
         d.dtype = _lib.F32 if dtype == np.float32 else _lib.F64
@@ -167,6 +170,12 @@ class Fivp:
     def set_graph(self, on):
         _lib.call("tarang_fivp_set_graph", self.h, 1 if on else 0)
 
+    def diagnostics(self):
+        E, Z = ctypes.c_double(), ctypes.c_double()
+        self.dev.sync()
+        _lib.call("tarang_fivp_diagnostics", self.h, ctypes.byref(E), ctypes.byref(Z))
+        return E.value, Z.value
+
     def set_dt(self, dt):
         _lib.call("tarang_fivp_set_dt", self.h, ctypes.c_double(dt))
 
@@ -197,6 +206,12 @@ class Chan:
         _lib.call("tarang_chan_get_field", self.h, int(var), ptr(c))
         self.dev.sync()
         return from_dev(c)
+
+    def nusselt(self):
+        nu = ctypes.c_double()
+        self.dev.sync()
+        _lib.call("tarang_chan_nusselt", self.h, ctypes.byref(nu))
+        return nu.value
 
     def set_bc(self, g0, g1):
         """Wall values of b: grid arrays on z = 0 and z = Lz (logical order, x fastest in storage)."""
@@ -260,6 +275,12 @@ class Chan3(Chan):
         _lib.call("tarang_chan3_get_field", self.h, int(var), ptr(c))
         self.dev.sync()
         return from_dev(c)
+
+    def nusselt(self):
+        nu = ctypes.c_double()
+        self.dev.sync()
+        _lib.call("tarang_chan3_nusselt", self.h, ctypes.byref(nu))
+        return nu.value
 
     def tau(self, t):
         v = torch.zeros(tuple(reversed(self.cshape[:2])), dtype=self.ct, device=self.dev.torch_device)

@@ -210,3 +210,53 @@ def test_serving_footprint_bounds_the_allocation(gpu, which):
     s.close()
     print(f"{which}: footprint {fp} B, held after set-up {held} B", flush=True)
     assert 0 < held <= fp and fp <= 4 * held + (32 << 20), (held, fp)
+
+
+def config_size_requests(dev):
+    """Requests at the serving stream's own sizes: 2-D turbulence 128^2 and 1024^2, 2-D
+    Rayleigh-Benard 128 x 32 (Ra 1e5), 512 x 128 (Ra 3e6) and 1024 x 256 (Ra 1e8), 1-D
+    Burgers N = 64 and 4096; FP32, the channel solves with one FP64-residual refinement sweep.
+    Fewer steps than the stream (20-60) keep the test short."""
+    cdt = torch.complex64
+    rng = np.random.default_rng(31)
+    out = []
+    for N, steps, arr in ((128, 60, 0.0), (1024, 20, 0.001)):
+        q = configs.turbulence_2d(N, seed=int(rng.integers(1 << 30)), kpeak=10.0, energy=0.5)
+        C = np.fft.fft(np.fft.rfft(q, axis=0), axis=1)
+        out.append((_lib.SERVE_PERIODIC, periodic_desc([N, N], 1, 1, 1e-4, 1e-3), steps, arr,
+                    [H.to_dev(C, cdt, dev)], [(N // 2 + 1, N)]))
+    for (nx, nz, Ra, steps, arr) in ((128, 32, 1e5, 60, 0.002), (512, 128, 3e6, 30, 0.003), (1024, 256, 1e8, 20, 0.004)):
+        d = chan_desc(nx, nz, Ra, 0, 1e-3)
+        d.refine_sweeps = 1
+        z = (1 - np.cos(np.pi * np.arange(nz) / (nz - 1))) / 2
+        bg = (1 - z)[None, :] + 1e-3 * rng.standard_normal((nx, nz)) * (z * (1 - z))[None, :]
+        k = np.arange(nz)
+        T = np.cos(np.outer(np.pi * np.arange(nz) / (nz - 1), k)) * ((-1.0) ** k)[None, :]
+        Bc = np.linalg.solve(T, np.fft.rfft(bg, axis=0).T).T
+        out.append((_lib.SERVE_CHANNEL, d, steps, arr, [None, H.to_dev(Bc, cdt, dev), None, None],
+                    [(nx // 2 + 1, nz)] * 4))
+    for N, arr in ((64, 0.005), (4096, 0.006)):
+        x = 2 * np.pi * np.arange(N) / N
+        u0 = np.sin(x) + 0.1 * sum(rng.standard_normal() * np.sin(j * x + rng.uniform(0, 6.3)) for j in (2, 3, 4))
+        out.append((_lib.SERVE_PERIODIC, periodic_desc([N], 2, 0, 1e-2, 1e-3), 60, arr,
+                    [H.to_dev(np.fft.rfft(u0), cdt, dev)], [(N // 2 + 1,)]))
+    return out
+
+
+def test_serving_config_sizes_mix_equals_alone(gpu):
+    dev = gpu.torch_device
+    reqs = config_size_requests(dev)
+    print(f"serving {len(reqs)} requests at the stream's sizes", flush=True)
+    mix, recs = run_server(dev, reqs, lanes=4, budget=64 << 30, chunk=10)
+    assert all(r.status == 2 for r in recs), [r.error for r in recs]
+    spans = sorted((r.admitted, r.done) for r in recs)
+    assert any(spans[i + 1][0] < spans[i][1] for i in range(len(spans) - 1))
+    for i, rq in enumerate(reqs):
+        alone, rec = run_server(dev, [rq[:3] + (0.0,) + rq[4:]], lanes=1, budget=64 << 30, chunk=10)
+        assert rec[0].status == 2
+        for a, b in zip(mix[i], alone[0]):
+            assert np.array_equal(a, b), i
+            assert np.isfinite(a).all(), i
+    st = latency_stats(recs)
+    print(f"  p50 {st['p50_s']:.3f} s, p99 {st['p99_s']:.3f} s, {st['grid_point_steps_per_s']:.3e} gp-steps/s",
+          flush=True)

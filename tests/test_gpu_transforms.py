@@ -63,7 +63,9 @@ def test_padded_product_parity(gpu, oracle, golden, dtype, sizes):
     ref = oracle.padded_multiply(d, a, b)
     p = gpu.plan([RF] * len(sizes), sizes, dtype=dtype)
     out = p.multiply(a, b)
-    assert relerr(out, ref) < TOL[dtype] * (10 if dtype == np.float32 else 1)
+    e = relerr(out, ref)
+    print(f"  rel err {e:.3e}", flush=True)
+    assert e < TOL[dtype] * (10 if dtype == np.float32 else 1)
     p.close()
 
 
