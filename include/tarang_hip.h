@@ -119,6 +119,16 @@ int tarang_cheb_deriv(tarang_ctx_t ctx, const void* data_g, void* result, int nd
  * Synchronises the context stream. */
 int tarang_cfl_frequency(tarang_ctx_t ctx, int ncomp, const void* const* comps, size_t npts, const double* dx,
                          int dtype, double* out);
+/* No counterpart in the reference: the variant of tarang_cfl_frequency for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops).
+ * Same arguments and result, but the result goes to out_dev, one device double, written
+ * on the context stream without synchronisation or host copy.  No hipMalloc/hipFree: the
+ * partial maxima live in a fixed 8 KiB device array per GPU (1024 values, for any npts),
+ * shared by every call on that GPU and left holding the last call's values; calls that
+ * overlap on one GPU (two streams or two contexts at once) race and must be ordered by
+ * the caller. */
+int tarang_cfl_frequency_noalloc(tarang_ctx_t ctx, int ncomp, const void* const* comps, size_t npts,
+                                 const double* dx, int dtype, double* out_dev);
 
 /* ------------------------------------------------------------------------ */
 /* Batched dense LU over Fourier modes.  Replaces

@@ -304,6 +304,15 @@ int tarang_cfl_frequency(tarang_ctx_t ctx, int ncomp, const void* const* comps, 
     });
 }
 
+int tarang_cfl_frequency_noalloc(tarang_ctx_t ctx, int ncomp, const void* const* comps, size_t npts,
+                                 const double* dx, int dtype, double* out_dev) {
+    return guard([&] {
+        NEED(ctx, "ctx");
+        NEED(comps, "comps");
+        tarang::cfl_frequency_noalloc(&ctx->ctx, ncomp, comps, npts, dx, static_cast<tarang::DType>(dtype), out_dev);
+    });
+}
+
 int tarang_batched_lu_factor(tarang_ctx_t ctx, void* A, int n, int nmodes, int* ipiv, int* info) {
     return guard([&] {
         NEED(ctx, "ctx");

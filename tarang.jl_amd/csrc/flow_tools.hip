@@ -1,5 +1,7 @@
 // flow_tools.hip -- device reduction behind the adaptive CFL controller.
+#include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "flow_tools.hpp"
@@ -86,6 +88,64 @@ double cfl_frequency(Context* c, int ncomp, const void* const* comps, size_t npt
     TG_HIP(hipMemcpyAsync(&out, partial + kCflBlocks, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     TG_HIP(hipStreamSynchronize(c->stream));
     return out;
+}
+
+// ---- the same reduction without allocation ----
+// The partial maxima live in a file-scope device array, one copy per GPU (held until the
+// process ends), instead of a context table allocated on first use.  Its size does not
+// depend on npts: the partial kernel's grid is capped at kCflBlocks workgroups.
+constexpr int kCflScratchDoubles = kCflBlocks;  // 8 KiB per GPU
+static __device__ __align__(256) double scratch[kCflScratchDoubles];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kScratchDevices] = {};
+}  // namespace
+
+// device address of this GPU's copy of `scratch` (looked up once per device, with that
+// device current: the array's host-side name is not a device address)
+static double* cfl_scratch(Context* c) {
+    TG_REQUIRE(c->device >= 0 && c->device < kScratchDevices, kErrInvalid, "cfl_frequency_noalloc: device id out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    double*& p = g_scratch_ptr[c->device];
+    if (!p) {
+        void* addr = nullptr;
+        TG_HIP(hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch)));
+        TG_REQUIRE(addr != nullptr, kErrHip, "cfl_frequency_noalloc: no device address for the scratch array");
+        p = static_cast<double*>(addr);
+    }
+    return p;
+}
+
+__global__ void k_cfl_zero(double* out) { *out = 0.0; }
+
+void cfl_frequency_noalloc(Context* c, int ncomp, const void* const* comps, size_t npts, const double* dx, DType dt,
+                           double* out_dev) {
+    const bool plain = dx == nullptr;
+    TG_REQUIRE(out_dev != nullptr, kErrInvalid, "cfl_frequency_noalloc: NULL output");
+    TG_REQUIRE(ncomp >= 1 && ncomp <= 3, kErrInvalid, "cfl_frequency: 1 to 3 components");
+    TG_REQUIRE(!plain || ncomp == 1, kErrInvalid, "cfl_frequency: a plain maximum takes one array (dx = NULL)");
+    TG_REQUIRE(dt == kF32 || dt == kF64, kErrInvalid, "cfl_frequency: dtype must be F32 or F64");
+    if (npts > 0) {
+        for (int i = 0; i < ncomp; ++i) TG_REQUIRE(comps[i] != nullptr, kErrInvalid, "cfl_frequency: NULL component");
+        if (!plain)
+            for (int i = 0; i < ncomp; ++i)
+                TG_REQUIRE(dx[i] > 0, kErrInvalid, "cfl_frequency: grid spacing must be > 0");
+    }
+    c->activate();
+    double* partial = cfl_scratch(c);
+    if (npts == 0) {  // an empty slab contributes 0 (flow_tools_cfl.jl:286-289)
+        hipLaunchKernelGGL(k_cfl_zero, dim3(1), dim3(1), 0, c->stream, out_dev);
+        TG_HIP(hipGetLastError());
+        return;
+    }
+    const unsigned nblk = (unsigned)std::min<size_t>(kCflBlocks, (npts + 1023) / 1024);
+    TG_REQUIRE(nblk <= (unsigned)kCflScratchDoubles, kErrCapacity, "cfl_frequency_noalloc: scratch too small");
+    if (dt == kF32) cfl_launch<float>(c, comps, ncomp, npts, dx, plain, partial, nblk);
+    else cfl_launch<double>(c, comps, ncomp, npts, dx, plain, partial, nblk);
+    hipLaunchKernelGGL(k_cfl_final, dim3(1), dim3(256), 0, c->stream, partial, (int)nblk, out_dev);
+    TG_HIP(hipGetLastError());
 }
 
 template <typename S>

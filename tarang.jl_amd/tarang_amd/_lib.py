@@ -131,6 +131,7 @@ SIGNATURES = {
     "tarang_cheb_deriv": [_P, _P, _P, _I, _PI, _I, _I, _D, _I, _I],
     "tarang_batched_lu_factor": [_P, _P, _I, _I, _P, _PI],
     "tarang_cfl_frequency": [_P, _I, _PP, ctypes.c_size_t, _PD, _I, _PD],
+    "tarang_cfl_frequency_noalloc": [_P, _I, _PP, ctypes.c_size_t, _PD, _I, _P],
     "tarang_fivp_diagnostics": [_P, _PD, _PD],
     "tarang_chan_nusselt": [_P, _PD],
     "tarang_chan3_nusselt": [_P, _PD],
