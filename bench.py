@@ -44,14 +44,25 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def measured_traffic(name, kernel_label):
-    """HBM bytes per launch of a kernel from a committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
-    (profiles/<name>), or None when the kernel is not in it."""
+def measured_traffic(name, kernel_label, dtype="float", variant=None):
+    """(HBM bytes per launch, full template name) of a kernel from a committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (profiles/<name>), keyed by the kernels' full template
+    names: the one entry for this kernel and dtype (and `variant`, a piece of its template
+    argument list), or (None, None) when there is not exactly one."""
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
-        return None
+        return None, None
     with open(path) as f:
-        return json.load(f).get(kernel_label.split()[0])
+        d = json.load(f)
+    short = kernel_label.split()[0]
+    if "kernels" not in d:  # summaries keyed by the short kernel name
+        return d.get(short), d.get(short + "_counters", {}).get("kernel")
+    cands = [k for k in d["kernels"] if k.startswith(short + "<" + dtype)]
+    if variant is not None:
+        cands = [k for k in cands if variant in k]
+    if len(cands) != 1:
+        return None, None
+    return d["kernels"][cands[0]]["bytes"], cands[0]
 
 
 def host_threads():
@@ -131,8 +142,8 @@ def main():
         log(f"all: {name}")
         r = fn(sub)
         others[r["config"]["workload"]] = {k: r.get(k) for k in (
-            "value", "unit", "ms_per_step", "fp64", "speedup_fp32_vs_fp64", "fp32_vs_fp64_rel_err", "roofline",
-            "latency_s", "makespan_s", "rejected") if r.get(k) is not None}
+            "value", "unit", "ms_per_step", "fp64", "speedup_fp32_vs_fp64", "fp32_form", "fp32_vs_fp64_rel_err",
+            "fp32_forms", "roofline", "latency_s", "makespan_s", "rejected") if r.get(k) is not None}
     log("all: channel3d (headline)")
     line = bench_channel3d(a)
     line["other_configs"] = others
@@ -317,6 +328,26 @@ def dt_change_cost(_lib, ctx, h, pre, dt, steady_ms):
     return {"ms_of_the_3_steps_after_a_change": out, "steady_ms_per_step": steady_ms}
 
 
+# FP32 forms of the channel solvers: (name, refine_sweeps, split_inverse); the first is the
+# headline form.  split_inverse stores every tau-system inverse entry as an FP32 value plus a
+# bfloat16 remainder (2^-33 relative); refine_sweeps adds FP64-residual refinement sweeps.
+FP32_FORMS = (("f32", 0, 1), ("f32_plain", 0, 0), ("f32_sweeps1", 1, 0), ("f32_sweeps2", 2, 0))
+FP32_HEADLINE = {"refine_sweeps": FP32_FORMS[0][1], "split_inverse": FP32_FORMS[0][2]}
+
+
+def fp32_form_table(np, results, finals, nus):
+    """Per FP32 form: time, speed against FP64 and its FP32-vs-FP64 errors."""
+    table = {}
+    for name, sw, sp in FP32_FORMS:
+        table[name] = {
+            "refine_sweeps": sw, "split_inverse": sp,
+            "ms_per_step": results[name][0], "speedup_vs_fp64": results["f64"][0] / results[name][0],
+            "nusselt_rel_err_vs_fp64": abs(nus[name] - nus["f64"]) / abs(nus["f64"]),
+            "b_rel_err_vs_fp64": float(np.abs(finals[name][0] - finals["f64"][0]).max() /
+                                       np.abs(finals["f64"][0]).max())}
+    return table
+
+
 def bench_channel(a):
     import numpy as np
     import torch
@@ -339,13 +370,13 @@ def bench_channel(a):
     ctx = ctypes.c_void_p()
     _lib.call("tarang_ctx_create", 0, ctypes.byref(ctx))
     results, finals, solvers = {}, {}, {}
-    forms = (("f64", _lib.F64, torch.complex128, 0), ("f32", _lib.F32, torch.complex64, HEADLINE_SWEEPS),
-             ("f32_sweeps0", _lib.F32, torch.complex64, 0), ("f32_sweeps2", _lib.F32, torch.complex64, 2))
-    for name, code, ct, sweeps in forms:
+    forms = (("f64", _lib.F64, torch.complex128, 0, 0),) + tuple(
+        (name, _lib.F32, torch.complex64, sw, sp) for name, sw, sp in FP32_FORMS)
+    for name, code, ct, sweeps, split in forms:
         d = _lib.ChanDesc()
         d.nx, d.nz, d.Lx, d.Lz, d.kappa, d.nu, d.dt = nx, nz, Lx, Lz, kappa, nu, dt
         d.dtype, d.timestepper, d.b_bottom, d.b_top = code, 0, Lz, 0.0
-        d.refine_sweeps = sweeps
+        d.refine_sweeps, d.split_inverse = sweeps, split
         h = ctypes.c_void_p()
         t0 = time.perf_counter()
         _lib.call("tarang_chan_create", ctx, ctypes.byref(d), ctypes.byref(h))
@@ -382,12 +413,7 @@ def bench_channel(a):
         log(f"channel {name}: {results[name][0]:.4f} ms/step")
     nus = {k: diagnostics.nusselt(v[0], v[1], nx, Lz, kappa, Lz) for k, v in finals.items()}
     err_b = float(np.abs(finals["f32"][0] - finals["f64"][0]).max() / np.abs(finals["f64"][0]).max())
-    by_sweeps = {}
-    for name, sw in (("f32_sweeps0", 0), ("f32", HEADLINE_SWEEPS), ("f32_sweeps2", 2)):
-        by_sweeps[str(sw)] = {
-            "ms_per_step": results[name][0], "speedup_vs_fp64": results["f64"][0] / results[name][0],
-            "nusselt_rel_err_vs_fp64": abs(nus[name] - nus["f64"]) / abs(nus["f64"]),
-            "b_rel_err_vs_fp64": float(np.abs(finals[name][0] - finals["f64"][0]).max() / np.abs(finals["f64"][0]).max())}
+    by_form = fp32_form_table(np, results, finals, nus)
     err_w = float(np.abs(finals["f32"][1] - finals["f64"][1]).max() / max(np.abs(finals["f64"][1]).max(), 1e-300))
     # live per-kernel durations, FP32 path
     avg = (ctypes.c_double * 6)()
@@ -397,6 +423,7 @@ def bench_channel(a):
     _lib.call("tarang_chan_kernel_bytes", solvers["f32"], nb)
     per_step = [avg[i] * cnt[i] / a.profile_steps for i in range(6)]
     dom = max((0, 1, 2, 3, 5), key=lambda i: per_step[i])  # HBM class; the solve (4) is reported on its own
+    tr_bytes, tr_name = measured_traffic("traffic_channel2d_1024x256_f32.json", CH_KERNELS[dom])
     solve_class = {"kernel": CH_KERNELS[4], "ms_per_step": per_step[4], "bytes_per_launch": nb[4],
                    "GBps": nb[4] / (avg[4] * 1e-3) / 1e9 if avg[4] else None,
                    "share_of_step": per_step[4] / sum(per_step) if sum(per_step) else None}
@@ -437,15 +464,15 @@ def bench_channel(a):
         "fp64": {"value": v64, "ms_per_step": results["f64"][0], "setup_s": results["f64"][1]},
         "setup_s_f32": results["f32"][1], "dt_change_f32": dtc,
         "speedup_fp32_vs_fp64": v32 / v64,
-        "refine_sweeps": HEADLINE_SWEEPS,
+        "fp32_form": FP32_HEADLINE,
         "fp32_vs_fp64_rel_err": {"b_maxnorm": err_b, "uz_maxnorm": err_w,
                                  "nusselt": abs(nus["f32"] - nus["f64"]) / abs(nus["f64"]),
                                  "after_steps": max(horizon, steps)},
-        "fp32_by_refine_sweeps": by_sweeps,
+        "fp32_forms": by_form,
         "nusselt": nus,
         "roofline": {"bound": "hbm", "kernel": CH_KERNELS[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic("traffic_channel2d_1024x256_f32.json", CH_KERNELS[dom])},
+                     "traffic": tr_bytes, "traffic_kernel": tr_name},
         "solve_class_f32": solve_class,
         "kernels_f32": kernels,
         "cpu_baseline": cpu,
@@ -454,7 +481,6 @@ def bench_channel(a):
 
 
 CH3 = dict(nx=256, ny=256, nz=128, Lx=4.0, Ly=4.0, Lz=1.0, Ra=1e7, Pr=0.7, dt=2e-3)
-HEADLINE_SWEEPS = 1  # FP64-residual refinement sweeps of the FP32 form in the headline line
 CH3_WORKLOAD = ("3-D Rayleigh-Benard, RealFourier^2 x ChebyshevT 256x256x128, Lx=Ly=4, Lz=1, Ra=1e7, Pr=0.7, "
                 "no-slip walls, b fixed at both walls, 3/2 dealiasing, SBDF2, dt=2e-3")
 CH3_KERNELS = ["k3_zbwd (Chebyshev evaluation of 15 operand planes)", "k3_ybwd (padded inverse y FFT, MY=384)",
@@ -487,14 +513,14 @@ def bench_channel3d(a):
     ctx = ctypes.c_void_p()
     _lib.call("tarang_ctx_create", 0, ctypes.byref(ctx))
     results, finals, solvers, keys, setups = {}, {}, {}, {}, {}
-    # FP64; FP32 with 1 FP64-residual refinement sweep per solve (the headline form); FP32 with 0 and 2
-    forms = (("f64", _lib.F64, torch.complex128, 0), ("f32", _lib.F32, torch.complex64, HEADLINE_SWEEPS),
-             ("f32_sweeps0", _lib.F32, torch.complex64, 0), ("f32_sweeps2", _lib.F32, torch.complex64, 2))
-    for name, code, ct, sweeps in forms:
+    # FP64, then the FP32 forms (the headline form first)
+    forms = (("f64", _lib.F64, torch.complex128, 0, 0),) + tuple(
+        (name, _lib.F32, torch.complex64, sw, sp) for name, sw, sp in FP32_FORMS)
+    for name, code, ct, sweeps, split in forms:
         d = _lib.Chan3Desc()
         d.nx, d.ny, d.nz, d.Lx, d.Ly, d.Lz = nx, ny, nz, c["Lx"], c["Ly"], Lz
         d.kappa, d.nu, d.dt, d.dtype, d.timestepper, d.b_bottom, d.b_top = kappa, nu, c["dt"], code, 2, Lz, 0.0
-        d.refine_sweeps = sweeps
+        d.refine_sweeps, d.split_inverse = sweeps, split
         h = ctypes.c_void_p()
         t0 = time.perf_counter()
         _lib.call("tarang_chan3_create", ctx, ctypes.byref(d), ctypes.byref(h))
@@ -539,12 +565,7 @@ def bench_channel3d(a):
         log(f"channel3d {name}: {results[name][0]:.4f} ms/step")
     nus = {kx: diagnostics.nusselt3(v[0], v[1], nx, ny, Lz, kappa, Lz) for kx, v in finals.items()}
     err_b = float(np.abs(finals["f32"][0] - finals["f64"][0]).max() / np.abs(finals["f64"][0]).max())
-    by_sweeps = {}
-    for name, sw in (("f32_sweeps0", 0), ("f32", HEADLINE_SWEEPS), ("f32_sweeps2", 2)):
-        by_sweeps[str(sw)] = {
-            "ms_per_step": results[name][0], "speedup_vs_fp64": results["f64"][0] / results[name][0],
-            "nusselt_rel_err_vs_fp64": abs(nus[name] - nus["f64"]) / abs(nus["f64"]),
-            "b_rel_err_vs_fp64": float(np.abs(finals[name][0] - finals["f64"][0]).max() / np.abs(finals["f64"][0]).max())}
+    by_form = fp32_form_table(np, results, finals, nus)
     avg = (ctypes.c_double * 8)()
     cnt = (ctypes.c_long * 8)()
     _lib.call("tarang_chan3_profile", solvers["f32"], a.profile_steps, avg, cnt)
@@ -555,6 +576,7 @@ def bench_channel3d(a):
     # the solve class (band factors and sweeps) is reported on its own
     dom = max(range(6), key=lambda i: per_step[i])
     achieved = nb[dom] / (avg[dom] * 1e-3) / 1e9
+    tr_bytes, tr_name = measured_traffic("traffic_channel3d_256x256x128_f32.json", CH3_KERNELS[dom])
     solve_class = {"kernel": CH3_KERNELS[6], "ms_per_step": per_step[6], "bytes_per_launch": nb[6],
                    "GBps": nb[6] / (avg[6] * 1e-3) / 1e9 if avg[6] else None,
                    "share_of_step": per_step[6] / sum(per_step) if sum(per_step) else None}
@@ -595,14 +617,14 @@ def bench_channel3d(a):
         "setup_s_f32": results["f32"][1], "warmup_s_f32": setups["f32"], "kh2_keys": keys["f32"],
         "dt_change_f32": dtc,
         "speedup_fp32_vs_fp64": v32 / v64,
-        "refine_sweeps": HEADLINE_SWEEPS,
+        "fp32_form": FP32_HEADLINE,
         "fp32_vs_fp64_rel_err": {"b_maxnorm": err_b, "nusselt": abs(nus["f32"] - nus["f64"]) / abs(nus["f64"]),
                                  "after_steps": max(horizon, steps + max(a.warmup, 3))},
-        "fp32_by_refine_sweeps": by_sweeps,
+        "fp32_forms": by_form,
         "nusselt": nus,
         "roofline": {"bound": "hbm", "kernel": CH3_KERNELS[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic("traffic_channel3d_256x256x128_f32.json", CH3_KERNELS[dom])},
+                     "traffic": tr_bytes, "traffic_kernel": tr_name},
         "solve_class_f32": solve_class,
         "kernels_f32": kernels,
         "cpu_baseline": cpu,
@@ -613,8 +635,8 @@ def bench_channel3d(a):
 SERVE_WORKLOAD = ("serving stream: Poisson arrivals at 80% of the measured saturation capacity; 40% 2-D turbulence "
                   "N in {128,256,512,1024}^2 (RK443, dt=1e-3), 40% 2-D Rayleigh-Benard Nx x Nz in {128x32, 256x64, "
                   "512x128, 1024x256}, Ra log-uniform in [1e5, 1e8], Pr=1 (RK222, dt=1e-3), 20% 1-D Burgers N in "
-                  "{64..4096}, nu=1e-2 (RK222, dt=1e-3); FP32 state, FP64 tau factorisation with one FP64-residual refinement "
-                  "sweep per solve; steps log-uniform in [200, 2000]")
+                  "{64..4096}, nu=1e-2 (RK222, dt=1e-3); FP32 state, FP64 tau factorisation, inverses stored as FP32 "
+                  "values plus bfloat16 remainders; steps log-uniform in [200, 2000]")
 
 
 def serving_stream(np, torch, _lib, configs, n, seed, dev):
@@ -642,7 +664,8 @@ def serving_stream(np, torch, _lib, configs, n, seed, dev):
             d = _lib.ChanDesc()
             d.nx, d.nz, d.Lx, d.Lz, d.kappa, d.nu, d.dt = nx, nz, 4.0, 1.0, Ra ** -0.5, Ra ** -0.5, 1e-3
             d.dtype, d.timestepper, d.b_bottom, d.b_top = _lib.F32, 0, 1.0, 0.0
-            d.refine_sweeps = HEADLINE_SWEEPS  # FP32 with FP64-residual refinement of the tau solves
+            d.refine_sweeps = FP32_HEADLINE["refine_sweeps"]  # FP32 state, refined tau inverses
+            d.split_inverse = FP32_HEADLINE["split_inverse"]
             bg, _ = channel_ic(np, nx, nz, 1.0, seed=int(rng.integers(1 << 30)))
             k = np.arange(nz)
             T = np.cos(np.outer(np.pi * np.arange(nz) / (nz - 1), k)) * ((-1.0) ** k)[None, :]

@@ -236,6 +236,9 @@ typedef struct {
     double b_bottom, b_top;  /* Dirichlet values b(z=0), b(z=Lz) (the test problem: Lz, 0) */
     int refine_sweeps;       /* 0, 1 or 2 sweeps of x += A^-1 (y - A x) per implicit solve, with the
                                 residual y - A x formed in FP64 from the structured tau systems */
+    int split_inverse;       /* FP32 state only: 1 stores every inverse entry as an FP32 value plus
+                                a bfloat16 remainder (2^-33 relative instead of 2^-24, 6 bytes per
+                                entry instead of 4); 0 plain FP32 inverses */
 } tarang_chan_desc;
 
 #define TARANG_CHAN_P 0
@@ -299,6 +302,7 @@ typedef struct {
     int timestepper;
     double b_bottom, b_top;
     int refine_sweeps;       /* as in tarang_chan_desc (nz <= 208 when refine_sweeps > 0) */
+    int split_inverse;       /* as in tarang_chan_desc */
 } tarang_chan3_desc;
 
 #define TARANG_CHAN3_P 0

@@ -86,6 +86,7 @@ tarang::ChannelDesc conv_desc(const tarang_chan_desc* desc) {
     d.b_bottom = desc->b_bottom;
     d.b_top = desc->b_top;
     d.refine_sweeps = desc->refine_sweeps;
+    d.split_inverse = desc->split_inverse;
     switch (desc->timestepper) {
         case 0: d.stepper = tarang::kRK222; break;
         case 1: d.stepper = tarang::kRK443; break;
@@ -111,6 +112,7 @@ tarang::Channel3Desc conv_desc(const tarang_chan3_desc* desc) {
     d.b_bottom = desc->b_bottom;
     d.b_top = desc->b_top;
     d.refine_sweeps = desc->refine_sweeps;
+    d.split_inverse = desc->split_inverse;
     switch (desc->timestepper) {
         case 0: d.stepper = tarang::kRK222; break;
         case 1: d.stepper = tarang::kRK443; break;

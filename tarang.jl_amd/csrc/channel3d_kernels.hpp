@@ -368,9 +368,13 @@ static __global__ void k3_assemble(Ch3Sys s, double* Ab, double* Aup, double* Ap
 }
 
 struct Ch3SolveArgs {
-    const void* Ib;     // b-system inverses (state precision), stride sb per (key, parity), column-major, ld = n
+    // b-system inverses, stride sb per (key, parity): FP64 column-major (ld = n), or FP32
+    // packed by pk_index
+    const void* Ib;
     const void* Ip;     // perp-system inverses, stride sb
     const void* Iu;     // u-p-system inverses, stride su
+    // bfloat16 remainders of the FP32 inverses (split inverses), same offsets; null otherwise
+    const unsigned short *Iblo, *Iplo, *Iulo;
     const int4* chunks; // [nchunks] (key, g, first, count): modes[first .. first + count), count <= 8
     const int* modes;   // modes grouped by key
     const double* rot;  // [2 nmodes] (cos, sin) of the k_h direction per mode
@@ -391,9 +395,26 @@ constexpr int kCh3ChunkModes = 8;  // modes per workgroup: 16 real right-hand si
 
 __host__ __device__ inline int round16(int n) { return (n + 15) & ~15; }
 
+// k3_solve's small size class for 256-thread workgroups (4 waves)
+inline bool ch3_solve_small(int nb_max, int nu_max) {
+    return (nu_max + 15) / 16 <= 16 && (nb_max + 15) / 16 <= 8;
+}
+
 // right-hand sides, and with refinement a second block set for the solutions
 __host__ __device__ inline size_t ch3_solve_lds_doubles(int N, bool refine = false) {
     return (size_t)kYS * (2 * round16(chb_size(N, 0)) + round16(chu_size(N, 0, 0))) * (refine ? 2 : 1);
+}
+
+// A^-1 Y for one system: FP64 inverses column-major; FP32 inverses packed, with their
+// bfloat16 remainders when SPLIT
+template <int MT, typename T, bool SPLIT>
+__device__ inline void ch3_apply(const void* I, const unsigned short* Ilo, size_t off, int n, const double* Y,
+                                 dbl4 (&acc)[MT], int wave, int nw, int wave0, bool accumulate = false) {
+    if constexpr (sizeof(T) == 4)
+        mfma_apply_pk<MT, SPLIT>(static_cast<const float*>(I) + off, SPLIT ? Ilo + off : nullptr, n, Y, acc, wave, nw,
+                                 wave0, accumulate);
+    else
+        mfma_apply<MT>(static_cast<const double*>(I) + off, n, Y, acc, wave, nw, wave0, accumulate);
 }
 
 // Workgroup = (|k_h| key, parity g, up to 8 modes of that key).  Right-hand sides of
@@ -404,15 +425,18 @@ __host__ __device__ inline size_t ch3_solve_lds_doubles(int N, bool refine = fal
 // rounding the right-hand side to FP32), the products accumulate in FP64.  Order: b system of parity h = g^1
 // and the perp system of parity g together, then the u-p system of parity g (its
 // E3z rows take + c b), then the un-rotation of the parity-g horizontal velocity.
-template <typename T, int NT, bool REFINE = false>
-__global__ __launch_bounds__(NT, REFINE ? 1 : 768 / NT) void k3_solve(Ch3SolveArgs s, const cplx<T>* __restrict__ R1,
+// SMALL: at most 4 u-p and 2 b/perp tiles per wave (nu <= 256, nb <= 128 with NT = 256),
+// which keeps the accumulators and load buffers of a 3-workgroup-per-CU launch in
+// registers; the larger FP32 class (wider packed-load buffers) runs 2 workgroups per CU
+template <typename T, int NT, bool REFINE = false, bool SPLIT = false, bool SMALL = false>
+__global__ __launch_bounds__(NT, REFINE ? 1 : (SMALL || sizeof(T) == 8 ? 768 : 512) / NT) void k3_solve(Ch3SolveArgs s, const cplx<T>* __restrict__ R1,
                                                const cplx<T>* __restrict__ R2, const cplx<T>* __restrict__ R3x,
                                                const cplx<T>* __restrict__ R3y, const cplx<T>* __restrict__ R3z,
                                                cplx<T>* P, cplx<T>* B, cplx<T>* UX, cplx<T>* UY, cplx<T>* UZ,
                                                cplx<T>* taus) {
     static_assert(NT == 256 || NT == 512, "four or eight wavefronts per workgroup");
-    using TI = T;  // inverses in the state precision
-    constexpr int MTB = NT == 256 ? 3 : 2, MTU = NT == 256 ? 7 : 4;  // tiles per wave for nz <= 256
+    static_assert(!SPLIT || sizeof(T) == 4, "split inverses belong to the FP32 state");
+    constexpr int MTB = SMALL || NT == 512 ? 2 : 3, MTU = SMALL || NT == 512 ? 4 : 7;  // tiles per wave (nz <= 256)
     extern __shared__ unsigned char smem_raw[];
     __shared__ int smode[kCh3ChunkModes];
     __shared__ double srot[2 * kCh3ChunkModes];
@@ -471,14 +495,13 @@ __global__ __launch_bounds__(NT, REFINE ? 1 : 768 / NT) void k3_solve(Ch3SolveAr
     }
     __syncthreads();
     // ---- b (parity h) and perp (parity g) systems ----
-    const int wave = threadIdx.x >> 6, nw = NT / 64, w0p = ((nb + 15) >> 4) % nw;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nw = NT / 64, w0p = ((nb + 15) >> 4) % nw;
     const double kh = s.kh[key];
     {
-        const TI* Ab = static_cast<const TI*>(s.Ib) + (size_t)(2 * key + h) * s.sb;
-        const TI* Ap = static_cast<const TI*>(s.Ip) + (size_t)(2 * key + g) * s.sb;
+        const size_t ob = (size_t)(2 * key + h) * s.sb, op = (size_t)(2 * key + g) * s.sb;
         dbl4 ab[MTB], ap[MTB];
-        mfma_apply<MTB>(Ab, nb, Yb, ab, wave, nw, 0);
-        mfma_apply<MTB>(Ap, np, Yp, ap, wave, nw, w0p);
+        ch3_apply<MTB, T, SPLIT>(s.Ib, s.Iblo, ob, nb, Yb, ab, wave, nw, 0);
+        ch3_apply<MTB, T, SPLIT>(s.Ip, s.Iplo, op, np, Yp, ap, wave, nw, w0p);
         __syncthreads();
         mfma_store<MTB>(ab, nb, Xb, wave, nw, 0);
         mfma_store<MTB>(ap, np, Xp, wave, nw, w0p);
@@ -487,8 +510,8 @@ __global__ __launch_bounds__(NT, REFINE ? 1 : 768 / NT) void k3_solve(Ch3SolveAr
             if (threadIdx.x < 16) chb_residual(N, h, kh, s.kappa, s.c, s.sz, Yb, Xb, kYS, 1, threadIdx.x);
             else if (threadIdx.x < 32) chb_residual(N, g, kh, s.nu, s.c, s.sz, Yp, Xp, kYS, 1, threadIdx.x - 16);
             __syncthreads();
-            mfma_apply<MTB>(Ab, nb, Xb, ab, wave, nw, 0, true);  // x += A^-1 r
-            mfma_apply<MTB>(Ap, np, Xp, ap, wave, nw, w0p, true);
+            ch3_apply<MTB, T, SPLIT>(s.Ib, s.Iblo, ob, nb, Xb, ab, wave, nw, 0, true);  // x += A^-1 r
+            ch3_apply<MTB, T, SPLIT>(s.Ip, s.Iplo, op, np, Xp, ap, wave, nw, w0p, true);
             __syncthreads();
             mfma_store<MTB>(ab, nb, Xb, wave, nw, 0);
             mfma_store<MTB>(ap, np, Xp, wave, nw, w0p);
@@ -510,9 +533,9 @@ __global__ __launch_bounds__(NT, REFINE ? 1 : 768 / NT) void k3_solve(Ch3SolveAr
     __syncthreads();
     // ---- u-p system of parity g ----
     {
-        const TI* Au = static_cast<const TI*>(s.Iu) + (size_t)(2 * key + g) * s.su;
+        const size_t ou = (size_t)(2 * key + g) * s.su;
         dbl4 au[MTU];
-        mfma_apply<MTU>(Au, nu, Yu, au, wave, nw, 0);
+        ch3_apply<MTU, T, SPLIT>(s.Iu, s.Iulo, ou, nu, Yu, au, wave, nw, 0);
         __syncthreads();
         mfma_store<MTU>(au, nu, Xu, wave, nw, 0);
         for (int sw = 0; REFINE && sw < sweeps; ++sw) {
@@ -520,7 +543,7 @@ __global__ __launch_bounds__(NT, REFINE ? 1 : 768 / NT) void k3_solve(Ch3SolveAr
             if (threadIdx.x < 16)
                 chu_residual(N, g, key == 0, kh, s.nu, s.c, s.sz, s.area, s.Lz, Yu, Xu, kYS, 1, threadIdx.x);
             __syncthreads();
-            mfma_apply<MTU>(Au, nu, Xu, au, wave, nw, 0, true);  // x += A^-1 r
+            ch3_apply<MTU, T, SPLIT>(s.Iu, s.Iulo, ou, nu, Xu, au, wave, nw, 0, true);  // x += A^-1 r
             __syncthreads();
             mfma_store<MTU>(au, nu, Xu, wave, nw, 0);
         }

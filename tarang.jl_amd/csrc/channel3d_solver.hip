@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <type_traits>
 #include <cstring>
 #include <map>
 #include <string>
@@ -44,6 +45,8 @@ void Channel3Solver::validate(const Channel3Desc& d) {
                "3-D channel solver: refine_sweeps must be 0, 1 or 2");
     TG_REQUIRE(d.refine_sweeps == 0 || ch3_solve_lds_doubles(d.nz, true) * sizeof(double) <= 160 * 1024, kErrInvalid,
                "3-D channel solver: refinement sweeps need nz <= 208 (LDS of the solve)");
+    TG_REQUIRE(d.split_inverse == 0 || (d.split_inverse == 1 && d.dtype == kF32), kErrInvalid,
+               "3-D channel solver: split_inverse is 0, or 1 with an FP32 state");
     TG_REQUIRE(d.Lx > 0 && d.Ly > 0 && d.Lz > 0 && d.dt > 0, kErrInvalid,
                "3-D channel solver: lengths and dt must be positive");
 }
@@ -172,10 +175,27 @@ Channel3Solver::Channel3Solver(Context* ctx, const Channel3Desc& d) : ctx_(ctx),
     up(modes_, modes.data(), modes.size() * sizeof(int));
     up(chunks_, chunks.data(), chunks.size() * sizeof(int));
     up(rot_, rot.data(), rot.size() * sizeof(double));
-    const size_t ib = dtype_size((DType)d.dtype);  // inverses in the state precision
-    Ib_ = DevBuf(ctx_, (size_t)2 * nkeys_ * nb_max_ * nb_max_ * ib);
-    Ip_ = DevBuf(ctx_, (size_t)2 * nkeys_ * nb_max_ * nb_max_ * ib);
-    Iu_ = DevBuf(ctx_, (size_t)2 * nkeys_ * nu_max_ * nu_max_ * ib);
+    {
+        // inverses in the state precision; FP32 ones packed for 16-byte loads, their padding
+        // columns zero (the factorisations rewrite only the entries of each system)
+        const bool pack = d.dtype == kF32;
+        const size_t ib = dtype_size((DType)d.dtype);
+        sb_ = inverse_stride(nb_max_, pack);
+        su_ = inverse_stride(nu_max_, pack);
+        auto zeroed = [&](size_t bytes) {
+            DevBuf b(ctx_, bytes);
+            TG_HIP(hipMemsetAsync(b.ptr, 0, bytes, s));
+            return b;
+        };
+        Ib_ = zeroed((size_t)2 * nkeys_ * sb_ * ib);
+        Ip_ = zeroed((size_t)2 * nkeys_ * sb_ * ib);
+        Iu_ = zeroed((size_t)2 * nkeys_ * su_ * ib);
+        if (d.split_inverse) {
+            Iblo_ = zeroed((size_t)2 * nkeys_ * sb_ * 2);
+            Iplo_ = zeroed((size_t)2 * nkeys_ * sb_ * 2);
+            Iulo_ = zeroed((size_t)2 * nkeys_ * su_ * 2);
+        }
+    }
     info_ = DevBuf(ctx_, (size_t)6 * nkeys_ * sizeof(int));
     {
         const auto bcv = constant_wall_values((size_t)nmodes_, (double)d.nx * d.ny, d.b_bottom, d.b_top);
@@ -299,9 +319,13 @@ void Channel3Solver::factor(double c) {
     TG_HIP(hipGetLastError());
     auto invert = [&](auto ot) {
         using OT = decltype(ot);
-        lu_invert<NT, OT>(s, Lb.as<const double>(), pb.as<const int>(), Ib_.as<OT>(), N, 0, nsys, sb, nb_max_, nb_max_);
-        lu_invert<NT, OT>(s, Lp.as<const double>(), pp.as<const int>(), Ip_.as<OT>(), N, 0, nsys, sb, nb_max_, nb_max_);
-        lu_invert<NT, OT>(s, Lu.as<const double>(), pu.as<const int>(), Iu_.as<OT>(), N, 1, nsys, su, nu_max_, nu_max_);
+        const bool pack = d_.dtype == kF32;
+        lu_invert<NT, OT>(s, Lb.as<const double>(), pb.as<const int>(), Ib_.as<OT>(), N, 0, nsys, sb, nb_max_, nb_max_,
+                          pack, Iblo_.as<unsigned short>());
+        lu_invert<NT, OT>(s, Lp.as<const double>(), pp.as<const int>(), Ip_.as<OT>(), N, 0, nsys, sb, nb_max_, nb_max_,
+                          pack, Iplo_.as<unsigned short>());
+        lu_invert<NT, OT>(s, Lu.as<const double>(), pu.as<const int>(), Iu_.as<OT>(), N, 1, nsys, su, nu_max_, nu_max_,
+                          pack, Iulo_.as<unsigned short>());
     };
     if (d_.dtype == kF32) invert(float{});
     else invert(double{});
@@ -428,9 +452,10 @@ template <typename T>
 void Channel3Solver::solve_t() {
     using V = cplx<T>;
     const int N = d_.nz;
-    Ch3SolveArgs sa{Ib_.ptr, Ip_.ptr, Iu_.ptr,
+    Ch3SolveArgs sa{Ib_.ptr, Ip_.ptr, Iu_.ptr, Iblo_.as<const unsigned short>(), Iplo_.as<const unsigned short>(),
+                    Iulo_.as<const unsigned short>(),
                     chunks_.as<const int4>(), modes_.as<const int>(), rot_.as<const double>(), N,
-                    (size_t)nb_max_ * nb_max_, (size_t)nu_max_ * nu_max_, factored_c_, bcv_.as<const double2>(),
+                    sb_, su_, factored_c_, bcv_.as<const double2>(),
                     d_.refine_sweeps, kh_.as<const double>(), d_.kappa, d_.nu, 2.0 / d_.Lz, d_.Lx * d_.Ly, d_.Lz};
     constexpr int NT = 256;  // four wavefronts: at most 3 b/perp and 7 u-p tiles each (nz <= 256)
     const size_t lds = ch3_solve_lds_doubles(N, d_.refine_sweeps > 0) * sizeof(double);
@@ -440,8 +465,22 @@ void Channel3Solver::solve_t() {
                                R_[1].as<const V>(), R_[2].as<const V>(), R_[3].as<const V>(), R_[4].as<const V>(),
                                P_.as<V>(), B_.as<V>(), UX_.as<V>(), UY_.as<V>(), UZ_.as<V>(), TAU_.as<V>());
         };
-        if (d_.refine_sweeps > 0) launch(k3_solve<T, NT, true>);
-        else launch(k3_solve<T, NT, false>);
+        auto by_size = [&](auto refine, auto split) {
+            constexpr bool R = decltype(refine)::value, S = decltype(split)::value;
+            if (ch3_solve_small(nb_max_, nu_max_)) launch(k3_solve<T, NT, R, S, true>);
+            else launch(k3_solve<T, NT, R, S, false>);
+        };
+        using yes = std::true_type;
+        using no = std::false_type;
+        if constexpr (sizeof(T) == 4) {
+            if (d_.split_inverse) {
+                if (d_.refine_sweeps > 0) by_size(yes{}, yes{});
+                else by_size(no{}, yes{});
+                return;
+            }
+        }
+        if (d_.refine_sweeps > 0) by_size(yes{}, no{});
+        else by_size(no{}, no{});
     });
     TG_HIP(hipGetLastError());
 }
@@ -648,11 +687,12 @@ void Channel3Solver::kernel_bytes(double* out) const {
     out[4] = (8 * nc) * vb + N * N * rb;                        // read 4 planes, write 4 rows
     out[5] = (4 * 4 + 4 + 5) * nc * vb;                         // state (x2 on the kx lines), F, histories, rhs
     double fac = 0;  // every chunk of up to 8 modes reads its key's three inverses once
+    const double eb = rb + (d_.split_inverse ? 2.0 : 0.0);  // bytes per inverse entry
     for (int k = 0; k < nkeys_; ++k) {
         const double chunks = (key_count_[k] + kCh3ChunkModes - 1) / kCh3ChunkModes;
         for (int g = 0; g < 2; ++g) {
             const double nb = chb_size(d_.nz, g ^ 1), np = chb_size(d_.nz, g), nu = chu_size(d_.nz, g, k);
-            fac += chunks * (nb * nb + np * np + nu * nu) * rb;
+            fac += chunks * (nb * nb + np * np + nu * nu) * eb;
         }
     }
     out[6] = fac + 10 * nc * vb;
@@ -678,7 +718,10 @@ size_t footprint(const Channel3Desc& d) {
     size_t b = nfields * alloc_granule(fb) + alloc_granule(kCh3Planes * fb) + alloc_granule(kCh3Planes * prow) +
                alloc_granule(4 * prow);                                                           // G, Q, H
     b += alloc_granule(nmodes * kCh3TauStride * vb) + 3 * alloc_granule(N * N * ib);           // taus, tables
-    b += 2 * alloc_granule(nsys * nb * nb * ib) + alloc_granule(nsys * nu * nu * ib);           // inverses
+    const bool pack = d.dtype == kF32;
+    const size_t sb = inverse_stride((int)nb, pack), su = inverse_stride((int)nu, pack);
+    b += 2 * alloc_granule(nsys * sb * ib) + alloc_granule(nsys * su * ib);                     // inverses
+    if (d.split_inverse) b += 2 * alloc_granule(nsys * sb * 2) + alloc_granule(nsys * su * 2);  // remainders
     b += 6 * alloc_granule(nmodes * 16);                                                      // keys, rotations, walls
     b += 2 * alloc_granule(nsys * nb * nb * 8) + alloc_granule(nsys * nu * nu * 8) +
          alloc_granule(nsys * (2 * nb + nu) * 4);                                             // FP64 LU scratch

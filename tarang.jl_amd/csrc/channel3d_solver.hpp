@@ -22,6 +22,9 @@ struct Channel3Desc {
     int stepper;             // kSBDF2 / kSBDF1 (per-mode multistep) or kRK222 / kRK443 (IMEX RK)
     double b_bottom, b_top;  // Dirichlet values of b at z = 0 and z = Lz
     int refine_sweeps = 0;   // FP64-residual refinement sweeps per implicit solve (0..2)
+    // FP32 state only: each inverse entry stored as an FP32 value plus a bfloat16
+    // remainder (2^-33 relative instead of 2^-24; 6 bytes per entry instead of 4)
+    int split_inverse = 0;
 };
 
 enum Channel3Var : int { kC3P = 0, kC3B = 1, kC3UX = 2, kC3UY = 3, kC3UZ = 4 };
@@ -93,7 +96,10 @@ class Channel3Solver {
     DevBuf BT_, BDT_, FT_;
     DevBuf kh_, rot_, modes_, chunks_;
     DevBuf bcv_, bch_;  // per-mode wall values of b [2 m + w], and the x-transform scratch
-    DevBuf Ib_, Ip_, Iu_, info_;  // explicit FP64 inverses per (key, parity) of the b, perp and u-p systems
+    // explicit inverses per (key, parity) of the b, perp and u-p systems: FP64 column-major,
+    // or FP32 packed (pk_index) with the bfloat16 remainders in the *lo_ buffers when split
+    DevBuf Ib_, Ip_, Iu_, Iblo_, Iplo_, Iulo_, info_;
+    size_t sb_ = 0, su_ = 0;  // per-system strides of the inverses
     int nchunks_ = 0;
     std::vector<int> key_count_;  // modes per key (host copy, for the byte model)
     bool graph_on_ = true;

@@ -1037,14 +1037,48 @@ __device__ void lu_solve_blocked(const double* __restrict__ a, const int* __rest
     }
 }
 
+// Packed storage of an n x n matrix for 16-byte loads by the matrix-core products
+// (mfma_apply_pk): 16-row tiles t, the last one with R = n - 16 t rows; inside a tile,
+// groups of four k-steps (16 columns) in which lane (q4, r16) of the product holds the
+// four values (row 16 t + r16, column 16 gi + 4 u + q4), u = 0..3, side by side; the
+// ceil(n/4) mod 4 trailing k-steps one value per lane.  Columns n .. 4 ceil(n/4) - 1 are
+// zero, so a system takes n * pk_cols(n) values and every offset of a 4-value group is a
+// multiple of 4.
+__host__ __device__ inline int pk_cols(int n) { return (n + 3) & ~3; }
+__host__ __device__ inline size_t pk_index(int n, int i, int k) {
+    const int K = pk_cols(n) >> 2, KG = K >> 2;
+    const int t = i >> 4, r16 = i & 15, R = n - 16 * t < 16 ? n - 16 * t : 16;
+    const int ks = k >> 2, q4 = k & 3;
+    const size_t base = (size_t)64 * t * K;  // the earlier tiles are full: 16 rows x 4K columns
+    if (ks < 4 * KG) return base + (size_t)(ks >> 2) * 16 * R + (size_t)(q4 * R + r16) * 4 + (ks & 3);
+    return base + (size_t)16 * R * KG + (size_t)(ks - 4 * KG) * 4 * R + (size_t)(q4 * R + r16);
+}
+
+// bfloat16 remainder of an FP32-rounded FP64 value: v ~ float(v) + bf16(v - float(v)) to
+// 2^-33 relative (the remainder is below 2^-24 |v| and keeps 8 significant bits)
+__device__ inline unsigned short bf16_rn(float f) {
+    unsigned u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (finite values)
+    return (unsigned short)(u >> 16);
+}
+__device__ inline float bf16_to_f32(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
 // Explicit inverse from the LU factors: workgroup (system w, column block cb) solves
 // A x = e_c for the NR columns c = NR cb .. NR cb + NR - 1 in FP64 and stores x,
-// rounded to OT, as column c of the inverse (column-major, leading dimension n,
-// stride istride per system).  LDS: NR ld + max(NR ld, 4096) doubles, ld >= n.
-template <int NT, typename OT, int NR>
+// rounded to OT, as column c of the inverse (column-major with leading dimension n, or
+// packed by pk_index when PACK; stride istride per system).  lo, when not null, receives
+// the bfloat16 remainder of every FP32-rounded entry: packed, at the entry's own offset;
+// column-major, in column pairs (lo_pair_index, stride lo_pair_stride per system).
+// LDS: NR ld + max(NR ld, 4096) doubles, ld >= n.
+__host__ __device__ inline size_t lo_pair_stride(int nmax) { return (size_t)nmax * ((nmax + 1) & ~1); }
+// remainders of columns 2j and 2j+1 of row i side by side: one 32-bit load per lane and pair
+__host__ __device__ inline size_t lo_pair_index(int n, int i, int c) {
+    return ((size_t)(c >> 1) * n + i) * 2 + (c & 1);
+}
+template <int NT, typename OT, int NR, bool PACK = false>
 __global__ __launch_bounds__(NT) void k_lu_inverse(const double* __restrict__ LU, const int* __restrict__ perm,
-                                                   OT* __restrict__ inv, int N, int kind, size_t stride,
-                                                   size_t pstride, size_t istride, int ld) {
+                                                   OT* __restrict__ inv, unsigned short* __restrict__ lo, int N,
+                                                   int kind, size_t stride, size_t pstride, size_t istride, int ld) {
     extern __shared__ unsigned char smem_raw[];
     const int w = blockIdx.y, m = w >> 1, g = w & 1;  // the column blocks of one system run side by side
     const int n = kind == 0 ? chb_size(N, g) : chu_size(N, g, m);
@@ -1059,23 +1093,39 @@ __global__ __launch_bounds__(NT) void k_lu_inverse(const double* __restrict__ LU
     __syncthreads();
     lu_solve_blocked<NT, NR>(LU + (size_t)w * stride, perm + (size_t)w * pstride, n, y, ld, tmp, tmp);
     OT* out = inv + (size_t)w * istride;
+    unsigned short* olo = lo ? lo + (size_t)w * (PACK ? istride : lo_pair_stride(ld)) : nullptr;
     for (int r = 0; r < NR && c0 + r < n; ++r)
-        for (int i = threadIdx.x; i < n; i += NT) out[(size_t)(c0 + r) * n + i] = OT(y[r * ld + i]);
+        for (int i = threadIdx.x; i < n; i += NT) {
+            const double v = y[r * ld + i];
+            const size_t o = PACK ? pk_index(n, i, c0 + r) : (size_t)(c0 + r) * n + i;
+            out[o] = OT(v);
+            if (olo) olo[PACK ? o : lo_pair_index(n, i, c0 + r)] = bf16_rn((float)(v - (double)OT(v)));
+        }
 }
 
 __host__ __device__ inline size_t lu_inverse_lds_doubles(int ld, int nr) {
     return (size_t)nr * ld + ((size_t)nr * ld > 4096 ? (size_t)nr * ld : 4096);
 }
 
+// per-system stride of the inverses: column-major n^2, or packed n * pk_cols(n)
+__host__ __device__ inline size_t inverse_stride(int nmax, bool pack) {
+    return pack ? (size_t)nmax * pk_cols(nmax) : (size_t)nmax * nmax;
+}
+
 // Form the inverses of nsys systems of one kind (0: b-like, 1: u-p) from their LU
-// factors, 8 right-hand sides per workgroup.
+// factors, 8 right-hand sides per workgroup, column-major or packed (stride
+// inverse_stride(nmax, pack)), with the bfloat16 remainders in lo when it is not null.
 template <int NT, typename OT>
 inline void lu_invert(hipStream_t s, const double* LU, const int* perm, OT* inv, int N, int kind, unsigned nsys,
-                      size_t stride, size_t pstride, int nmax) {
+                      size_t stride, size_t pstride, int nmax, bool pack = false, unsigned short* lo = nullptr) {
     // 8 right-hand sides per workgroup: 16 would need ~250 VGPRs (one wavefront per SIMD)
-    hipLaunchKernelGGL((k_lu_inverse<NT, OT, 8>), dim3((unsigned)((nmax + 7) / 8), nsys), dim3(NT),
-                       lu_inverse_lds_doubles(nmax, 8) * sizeof(double), s, LU, perm, inv, N, kind, stride, pstride,
-                       stride, nmax);
+    auto launch = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3((unsigned)((nmax + 7) / 8), nsys), dim3(NT),
+                           lu_inverse_lds_doubles(nmax, 8) * sizeof(double), s, LU, perm, inv, lo, N, kind, stride,
+                           pstride, inverse_stride(nmax, pack), nmax);
+    };
+    if (pack) launch(k_lu_inverse<NT, OT, 8, true>);
+    else launch(k_lu_inverse<NT, OT, 8, false>);
 }
 
 // ---------------------------------------------------------------------------
@@ -1165,6 +1215,108 @@ __device__ inline void mfma_apply(const AT* __restrict__ A, int n, const double*
     }
 }
 
+// X = A Y as mfma_apply, for A packed by pk_index with FP32 entries and, when LO, a
+// bfloat16 remainder per entry at the same offset in L (entry = A + L, formed in FP64).
+// Each lane loads W = 4 (16 bytes of A, 8 of L) or W = 2 consecutive values of its
+// group of four k-steps at a time; W = 2 halves the load buffers for waves with many
+// tiles.
+typedef float flt4 __attribute__((ext_vector_type(4)));
+typedef unsigned short us4 __attribute__((ext_vector_type(4)));
+typedef float flt2 __attribute__((ext_vector_type(2)));
+typedef unsigned short us2 __attribute__((ext_vector_type(2)));
+template <int W> struct PkVec;
+template <> struct PkVec<4> { using F = flt4; using U = us4; };
+template <> struct PkVec<2> { using F = flt2; using U = us2; };
+
+template <int NTW, bool LO, int W>
+__device__ inline void mfma_apply_pk_n(const float* __restrict__ A, const unsigned short* __restrict__ L, int n,
+                                       const double* Y, dbl4* acc, int t0, int nw) {
+    using F = typename PkVec<W>::F;
+    using UV = typename PkVec<W>::U;
+    constexpr int PARTS = 4 / W;  // loads per group of four k-steps
+    const int lane = threadIdx.x & 63, r16 = lane & 15, q4 = lane >> 4;
+    const int K = pk_cols(n) >> 2, KG = K >> 2, NP = KG * PARTS;
+    int R[NTW];
+    unsigned tb[NTW], lo16[NTW];  // tile base; this lane's offset inside a group
+#pragma unroll
+    for (int j = 0; j < NTW; ++j) {
+        const int t = t0 + j * nw;
+        R[j] = min(16, n - 16 * t);
+        tb[j] = 64u * (unsigned)t * (unsigned)K;
+        lo16[j] = r16 < R[j] ? (unsigned)(q4 * R[j] + r16) * 4u : 0u;  // lanes past a short tile read its first group
+    }
+    // two load buffers in turn, as in mfma_apply_n; part p = group p / PARTS, values
+    // W (p % PARTS) .. W (p % PARTS) + W - 1 of the lane's four
+    F ha[NTW], hb[NTW];
+    UV la[NTW], lb[NTW];
+    auto load = [&](int p, F (&h)[NTW], UV (&l)[NTW]) {
+        const unsigned gi = (unsigned)(p / PARTS), sub = (unsigned)(W * (p % PARTS));
+#pragma unroll
+        for (int j = 0; j < NTW; ++j) {
+            const unsigned off = tb[j] + gi * 16u * (unsigned)R[j] + lo16[j] + sub;
+            h[j] = *reinterpret_cast<const F*>(A + off);
+            if constexpr (LO) l[j] = *reinterpret_cast<const UV*>(L + off);
+        }
+    };
+    auto multiply = [&](int p, const F (&h)[NTW], const UV (&l)[NTW]) {
+        const int k0 = 16 * (p / PARTS) + 4 * W * (p % PARTS) + q4;
+#pragma unroll
+        for (int u = 0; u < W; ++u) {
+            const double b = Y[(k0 + 4 * u) * kYS + r16];
+#pragma unroll
+            for (int j = 0; j < NTW; ++j) {
+                double a = (double)h[j][u];
+                if constexpr (LO) a += (double)bf16_to_f32(l[j][u]);
+                acc[j] = mfma_f64_16x16x4(r16 < R[j] ? a : 0.0, b, acc[j]);
+            }
+        }
+    };
+    if (NP > 0) {  // wave-uniform
+        load(0, ha, la);
+        for (int p = 0; p < NP; p += 2) {
+            if (p + 1 < NP) load(p + 1, hb, lb);
+            multiply(p, ha, la);
+            if (p + 1 >= NP) break;
+            if (p + 2 < NP) load(p + 2, ha, la);
+            multiply(p + 1, hb, lb);
+        }
+    }
+    for (int v = 0; v < K - 4 * KG; ++v) {  // trailing k-steps, one value per lane
+        const double b = Y[(4 * (4 * KG + v) + q4) * kYS + r16];
+#pragma unroll
+        for (int j = 0; j < NTW; ++j) {
+            const bool in = r16 < R[j];
+            const unsigned off = tb[j] + 16u * (unsigned)(R[j] * KG) + (unsigned)(v * 4 * R[j]) +
+                                 (in ? (unsigned)(q4 * R[j] + r16) : 0u);
+            double a = (double)A[off];
+            if constexpr (LO) a += (double)bf16_to_f32(L[off]);
+            acc[j] = mfma_f64_16x16x4(in ? a : 0.0, b, acc[j]);
+        }
+    }
+}
+
+template <int MT, bool LO>
+__device__ inline void mfma_apply_pk(const float* __restrict__ A, const unsigned short* __restrict__ L, int n,
+                                     const double* Y, dbl4 (&acc)[MT], int wave, int nw, int wave0,
+                                     bool accumulate = false) {
+    const int ntiles = (n + 15) >> 4;
+    const int t0 = ((wave - wave0) % nw + nw) % nw;
+    const int ntw = t0 < ntiles ? (ntiles - t0 + nw - 1) / nw : 0;
+    if (!accumulate)
+#pragma unroll
+        for (int j = 0; j < MT; ++j) acc[j] = dbl4{0.0, 0.0, 0.0, 0.0};
+    switch (ntw < MT ? ntw : MT) {  // wave-uniform
+        case 1: mfma_apply_pk_n<1, LO, 4>(A, L, n, Y, acc, t0, nw); break;
+        case 2: if constexpr (MT >= 2) mfma_apply_pk_n<2, LO, 4>(A, L, n, Y, acc, t0, nw); break;
+        case 3: if constexpr (MT >= 3) mfma_apply_pk_n<3, LO, 4>(A, L, n, Y, acc, t0, nw); break;
+        case 4: if constexpr (MT >= 4) mfma_apply_pk_n<4, LO, 4>(A, L, n, Y, acc, t0, nw); break;
+        case 5: if constexpr (MT >= 5) mfma_apply_pk_n<5, LO, 2>(A, L, n, Y, acc, t0, nw); break;
+        case 6: if constexpr (MT >= 6) mfma_apply_pk_n<6, LO, 2>(A, L, n, Y, acc, t0, nw); break;
+        case 7: if constexpr (MT >= 7) mfma_apply_pk_n<7, LO, 2>(A, L, n, Y, acc, t0, nw); break;
+        default: break;
+    }
+}
+
 // Store the tiles of mfma_apply into X[i*kYS + c] (LDS, rows up to 16 ceil(n/16)).
 template <int MT>
 __device__ inline void mfma_store(const dbl4 (&acc)[MT], int n, double* X, int wave, int nw, int wave0) {
@@ -1183,6 +1335,10 @@ __device__ inline void mfma_store(const dbl4 (&acc)[MT], int n, double* X, int w
 struct ChSolveArgs {
     const void* Ib;     // b-system inverses (state precision), stride sb per (mode, parity), column-major, ld = n
     const void* Iu;     // u-p-system inverses, stride su
+    // bfloat16 remainders of FP32 inverses (split inverses) in column pairs, per-system
+    // strides sblo / sulo; null otherwise
+    const unsigned short *Iblo, *Iulo;
+    size_t sblo, sulo;
     int N, nxh;
     size_t sb, su;
     int ld;             // LDS row length of one right-hand side (>= largest system)
@@ -1240,9 +1396,12 @@ inline std::vector<double2> constant_wall_values(size_t nmodes, double scale, do
 
 // x[r*ld + i] = sum_k A[k*n + i] y[r*ld + k], r = 0, 1 (real and imaginary part), for
 // a column-major n x n matrix A (global) and y, x in LDS; thread t owns rows t + j NT
-// (j < RT), FP64 accumulation, loads of U columns issued together.
-template <int NT, int RT, int U, typename AT>
-__device__ inline void gemv2_rows(const AT* __restrict__ A, int n, const double* y, int ld, double* x) {
+// (j < RT), FP64 accumulation, loads of U columns issued together.  LO: the entries are
+// A + L with L the bfloat16 remainders of FP32 entries in column pairs (lo_pair_index).
+template <int NT, int RT, int U, typename AT, bool LO = false>
+__device__ inline void gemv2_rows(const AT* __restrict__ A, const unsigned short* __restrict__ L, int n,
+                                  const double* y, int ld, double* x) {
+    static_assert(U % 2 == 0, "column pairs of remainders");
     double acc[RT][2];
     int row[RT];
 #pragma unroll
@@ -1252,12 +1411,16 @@ __device__ inline void gemv2_rows(const AT* __restrict__ A, int n, const double*
     }
     for (int k0 = 0; k0 < n; k0 += U) {
         AT raw[U][RT];  // every load of the batch is issued before any is used
+        unsigned rlo[U / 2][RT];
 #pragma unroll
         for (int u = 0; u < U; ++u)
 #pragma unroll
             for (int r = 0; r < RT; ++r) {
                 const bool in = k0 + u < n && row[r] < n;
                 raw[u][r] = A[in ? (k0 + u) * n + row[r] : 0];  // n <= 1024: 32-bit offsets
+                if constexpr (LO)
+                    if (u % 2 == 0)
+                        rlo[u / 2][r] = reinterpret_cast<const unsigned*>(L)[in ? ((k0 + u) >> 1) * n + row[r] : 0];
             }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -1265,7 +1428,9 @@ __device__ inline void gemv2_rows(const AT* __restrict__ A, int n, const double*
             const double y0 = y[k], y1 = y[ld + k];
 #pragma unroll
             for (int r = 0; r < RT; ++r) {
-                const double a = (k0 + u < n && row[r] < n) ? (double)raw[u][r] : 0.0;
+                double a = (double)raw[u][r];
+                if constexpr (LO) a += (double)__uint_as_float(u % 2 ? rlo[u / 2][r] & 0xFFFF0000u : rlo[u / 2][r] << 16);
+                a = (k0 + u < n && row[r] < n) ? a : 0.0;
                 acc[r][0] += a * y0;
                 acc[r][1] += a * y1;
             }
@@ -1279,13 +1444,16 @@ __device__ inline void gemv2_rows(const AT* __restrict__ A, int n, const double*
         }
 }
 
-template <int NT, typename AT>
-__device__ inline void gemv2(const AT* __restrict__ A, int n, const double* y, int ld, double* x) {
+// MAXRT: the largest rows-per-thread case compiled in (n <= MAXRT NT); a smaller bound
+// keeps the register count low enough for more workgroups per CU
+template <int NT, bool LO = false, int MAXRT = 4, typename AT>
+__device__ inline void gemv2(const AT* __restrict__ A, const unsigned short* __restrict__ L, int n, const double* y,
+                             int ld, double* x) {
     switch ((n + NT - 1) / NT) {  // workgroup-uniform
-        case 1: gemv2_rows<NT, 1, 8>(A, n, y, ld, x); break;
-        case 2: gemv2_rows<NT, 2, 8>(A, n, y, ld, x); break;
-        case 3: gemv2_rows<NT, 3, 4>(A, n, y, ld, x); break;
-        default: gemv2_rows<NT, 4, 4>(A, n, y, ld, x); break;  // n <= 4 NT
+        case 1: gemv2_rows<NT, 1, 8, AT, LO>(A, L, n, y, ld, x); break;
+        case 2: if constexpr (MAXRT >= 2) gemv2_rows<NT, 2, 8, AT, LO>(A, L, n, y, ld, x); break;
+        case 3: if constexpr (MAXRT >= 3) gemv2_rows<NT, 3, 4, AT, LO>(A, L, n, y, ld, x); break;
+        default: if constexpr (MAXRT >= 4) gemv2_rows<NT, 4, 4, AT, LO>(A, L, n, y, ld, x); break;  // n <= 4 NT
     }
 }
 
@@ -1299,7 +1467,8 @@ __host__ __device__ inline size_t ch_solve_lds_doubles(int ld, bool refine = fal
 // h = g^1 (whose values enter the E3z rows of u-p group g as buoyancy) and then of
 // u-p group g.  Inverses in the state precision T, products in FP64.
 // rhs rows E1, E2, E3x, E3z (mode-major) in; state p, b, ux, uz, taus out.
-template <typename T, int NT, bool REFINE = false>
+// MAXRT: rows per thread of the largest system (2 when every system has at most 2 NT rows)
+template <typename T, int NT, bool REFINE = false, bool SPLIT = false, int MAXRT = 4>
 __global__ __launch_bounds__(NT) void k_ch_solve(ChSolveArgs s, const cplx<T>* __restrict__ R1,
                                                  const cplx<T>* __restrict__ R2, const cplx<T>* __restrict__ R3x,
                                                  const cplx<T>* __restrict__ R3z, cplx<T>* P, cplx<T>* B, cplx<T>* UX,
@@ -1314,13 +1483,14 @@ __global__ __launch_bounds__(NT) void k_ch_solve(ChSolveArgs s, const cplx<T>* _
     double* dx = r + 2 * ld;                            // [2][ld] corrections (refinement)
     const double kh = s.k0x * m;
     // x += A^-1 (y - A x), sweeps times; `resid` forms y - A x of one column into r
-    auto refine = [&](const T* Ainv, int n, auto resid) {
+    static_assert(!SPLIT || sizeof(T) == 4, "split inverses belong to the FP32 state");
+    auto refine = [&](const T* Ainv, const unsigned short* Alo, int n, auto resid) {
         for (int sw = 0; REFINE && sw < s.sweeps; ++sw) {
             for (int i = threadIdx.x; i < 2 * ld; i += NT) r[i] = x[i];
             __syncthreads();
             if (threadIdx.x < 2) resid(threadIdx.x);
             __syncthreads();
-            gemv2<NT>(Ainv, n, r, ld, dx);
+            gemv2<NT, SPLIT, MAXRT>(Ainv, Alo, n, r, ld, dx);
             __syncthreads();
             for (int i = threadIdx.x; i < 2 * ld; i += NT) x[i] += dx[i];
             __syncthreads();
@@ -1341,9 +1511,11 @@ __global__ __launch_bounds__(NT) void k_ch_solve(ChSolveArgs s, const cplx<T>* _
         y[ld + nh] = s.c * (h == 0 ? 0.5 * (g1.y + g0.y) : 0.5 * (g1.y - g0.y));
     }
     __syncthreads();
-    gemv2<NT>(static_cast<const T*>(s.Ib) + (size_t)(2 * m + h) * s.sb, nbh, y, ld, x);
+    const size_t ob = (size_t)(2 * m + h) * s.sb;
+    const unsigned short* blo = SPLIT ? s.Iblo + (size_t)(2 * m + h) * s.sblo : nullptr;
+    gemv2<NT, SPLIT, MAXRT>(static_cast<const T*>(s.Ib) + ob, blo, nbh, y, ld, x);
     __syncthreads();
-    refine(static_cast<const T*>(s.Ib) + (size_t)(2 * m + h) * s.sb, nbh,
+    refine(static_cast<const T*>(s.Ib) + ob, blo, nbh,
            [&](int col) { chb_residual(N, h, kh, s.kappa, s.c, s.sz, y, r, 1, ld, col); });
     // ---- u-p system of parity g (buoyancy moved to the right-hand side) ----
     const int ng = npar(N, g);
@@ -1367,9 +1539,11 @@ __global__ __launch_bounds__(NT) void k_ch_solve(ChSolveArgs s, const cplx<T>* _
         y[ld + r] = 0.0;
     }
     __syncthreads();
-    gemv2<NT>(static_cast<const T*>(s.Iu) + (size_t)wg * s.su, n, y, ld, x);
+    const size_t ou = (size_t)wg * s.su;
+    const unsigned short* ulo = SPLIT ? s.Iulo + (size_t)wg * s.sulo : nullptr;
+    gemv2<NT, SPLIT, MAXRT>(static_cast<const T*>(s.Iu) + ou, ulo, n, y, ld, x);
     __syncthreads();
-    refine(static_cast<const T*>(s.Iu) + (size_t)wg * s.su, n,
+    refine(static_cast<const T*>(s.Iu) + ou, ulo, n,
            [&](int col) { chu_residual(N, g, m == 0, kh, s.nu, s.c, s.sz, s.Lx, s.Lz, y, r, 1, ld, col); });
     for (int i = threadIdx.x; i < ng; i += NT) {
         UX[base + 2 * i + g] = cplx<T>{T(-x[ld + i]), T(x[i])};  // ux = i ux~

@@ -9,6 +9,7 @@
 //   final:   (b, ux, uz) = X0 + dt sum_j (bE_j F_j - bI_j L P(x_j));  p = taus = 0
 // P is the grid round trip of the state (real part at kx = 0 and kx = Nx/2).
 #include <cmath>
+#include <type_traits>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -42,6 +43,8 @@ void ChannelSolver::validate(const ChannelDesc& d) {
                "channel solver: ChebyshevT size " + std::to_string(d.nz) + " unsupported (4..512)");
     TG_REQUIRE(d.refine_sweeps >= 0 && d.refine_sweeps <= 2, kErrInvalid,
                "channel solver: refine_sweeps must be 0, 1 or 2");
+    TG_REQUIRE(d.split_inverse == 0 || (d.split_inverse == 1 && d.dtype == kF32), kErrInvalid,
+               "channel solver: split_inverse is 0, or 1 with an FP32 state");
     TG_REQUIRE(d.Lx > 0 && d.Lz > 0 && d.dt > 0, kErrInvalid, "channel solver: Lx, Lz and dt must be positive");
 }
 
@@ -103,6 +106,12 @@ ChannelSolver::ChannelSolver(Context* ctx, const ChannelDesc& d) : ctx_(ctx), d_
     const size_t ib = dtype_size((DType)d.dtype);  // inverses in the state precision
     Ib_ = DevBuf(ctx_, (size_t)2 * nxh_ * nb_max_ * nb_max_ * ib);
     Iu_ = DevBuf(ctx_, (size_t)2 * nxh_ * nu_max_ * nu_max_ * ib);
+    if (d.split_inverse) {  // remainders in column pairs; the odd column's partner stays zero
+        Iblo_ = DevBuf(ctx_, (size_t)2 * nxh_ * lo_pair_stride(nb_max_) * 2);
+        Iulo_ = DevBuf(ctx_, (size_t)2 * nxh_ * lo_pair_stride(nu_max_) * 2);
+        TG_HIP(hipMemsetAsync(Iblo_.ptr, 0, Iblo_.bytes, ctx_->stream));
+        TG_HIP(hipMemsetAsync(Iulo_.ptr, 0, Iulo_.bytes, ctx_->stream));
+    }
     info_ = DevBuf(ctx_, (size_t)4 * nxh_ * sizeof(int));
     {
         const auto bcv = constant_wall_values((size_t)nxh_, (double)d.nx, d.b_bottom, d.b_top);
@@ -228,8 +237,10 @@ void ChannelSolver::factor(double c) {
     TG_HIP(hipGetLastError());
     auto invert = [&](auto ot) {
         using OT = decltype(ot);
-        lu_invert<NT, OT>(s, Lb.as<const double>(), pb.as<const int>(), Ib_.as<OT>(), N, 0, nsys, sb, nb_max_, nb_max_);
-        lu_invert<NT, OT>(s, Lu.as<const double>(), pu.as<const int>(), Iu_.as<OT>(), N, 1, nsys, su, nu_max_, nu_max_);
+        lu_invert<NT, OT>(s, Lb.as<const double>(), pb.as<const int>(), Ib_.as<OT>(), N, 0, nsys, sb, nb_max_, nb_max_,
+                          false, Iblo_.as<unsigned short>());
+        lu_invert<NT, OT>(s, Lu.as<const double>(), pu.as<const int>(), Iu_.as<OT>(), N, 1, nsys, su, nu_max_, nu_max_,
+                          false, Iulo_.as<unsigned short>());
     };
     if (d_.dtype == kF32) invert(float{});
     else invert(double{});
@@ -346,7 +357,8 @@ template <typename T>
 void ChannelSolver::solve_t() {
     using V = cplx<T>;
     const int N = d_.nz;
-    ChSolveArgs sa{Ib_.ptr, Iu_.ptr, N, nxh_, (size_t)nb_max_ * nb_max_, (size_t)nu_max_ * nu_max_, nu_max_,
+    ChSolveArgs sa{Ib_.ptr, Iu_.ptr, Iblo_.as<const unsigned short>(), Iulo_.as<const unsigned short>(),
+                   lo_pair_stride(nb_max_), lo_pair_stride(nu_max_), N, nxh_, (size_t)nb_max_ * nb_max_, (size_t)nu_max_ * nu_max_, nu_max_,
                    factored_c_, bcv_.as<const double2>(), d_.refine_sweeps, 2.0 * M_PI / d_.Lx, d_.kappa, d_.nu,
                    2.0 / d_.Lz, d_.Lx, d_.Lz};
     constexpr int NT = 256;
@@ -357,8 +369,23 @@ void ChannelSolver::solve_t() {
                                R_[1].as<const V>(), R_[2].as<const V>(), R_[3].as<const V>(), P_.as<V>(), B_.as<V>(),
                                UX_.as<V>(), UZ_.as<V>(), TAU_.as<V>());
         };
-        if (d_.refine_sweeps > 0) launch(k_ch_solve<T, NT, true>);
-        else launch(k_ch_solve<T, NT, false>);
+        // rows per thread: 2 covers every system up to 512 unknowns (nz <= 340)
+        auto by_size = [&](auto refine, auto split) {
+            constexpr bool R = decltype(refine)::value, S = decltype(split)::value;
+            if (nu_max_ <= 2 * NT) launch(k_ch_solve<T, NT, R, S, 2>);
+            else launch(k_ch_solve<T, NT, R, S, 4>);
+        };
+        using yes = std::true_type;
+        using no = std::false_type;
+        if constexpr (sizeof(T) == 4) {
+            if (d_.split_inverse) {
+                if (d_.refine_sweeps > 0) by_size(yes{}, yes{});
+                else by_size(no{}, yes{});
+                return;
+            }
+        }
+        if (d_.refine_sweeps > 0) by_size(yes{}, no{});
+        else by_size(no{}, no{});
     });
     TG_HIP(hipGetLastError());
 }
@@ -559,7 +586,7 @@ void ChannelSolver::kernel_bytes(double* out) const {
     for (int m = 0; m < nxh_; ++m)
         for (int g = 0; g < 2; ++g) {
             const double nb = chb_size(d_.nz, g), nu = chu_size(d_.nz, g, m);
-            fac += (nb * nb + nu * nu) * rb;
+            fac += (nb * nb + nu * nu) * (rb + (d_.split_inverse ? 2.0 : 0.0));
         }
     out[4] = fac + 8 * nc * vb;
     out[5] = 0;
@@ -582,6 +609,8 @@ size_t footprint(const ChannelDesc& d) {
     size_t b = nfields * alloc_granule(fb) + alloc_granule(8 * fb) + alloc_granule(3 * fb);  // G, H
     b += alloc_granule(nxh * 8 * vb) + 3 * alloc_granule(N * N * ib);                       // taus, tables
     b += alloc_granule(nsys * nb * nb * ib) + alloc_granule(nsys * nu * nu * ib);            // inverses
+    if (d.split_inverse)  // remainders
+        b += alloc_granule(nsys * lo_pair_stride((int)nb) * 2) + alloc_granule(nsys * lo_pair_stride((int)nu) * 2);
     b += 3 * alloc_granule(1);                                                               // info, walls
     // FP64 LU scratch and pivots while the inverses are formed
     b += alloc_granule(nsys * nb * nb * 8) + alloc_granule(nsys * nu * nu * 8) + alloc_granule(nsys * (nb + nu) * 4);

@@ -62,6 +62,7 @@ class ChanDesc(ctypes.Structure):
         ("b_bottom", ctypes.c_double),
         ("b_top", ctypes.c_double),
         ("refine_sweeps", ctypes.c_int),
+        ("split_inverse", ctypes.c_int),
     ]
 
 
@@ -84,6 +85,7 @@ class Chan3Desc(ctypes.Structure):
         ("b_bottom", ctypes.c_double),
         ("b_top", ctypes.c_double),
         ("refine_sweeps", ctypes.c_int),
+        ("split_inverse", ctypes.c_int),
     ]
 
 

@@ -69,9 +69,10 @@ class Device:
         return Fivp(self, h, sizes, dtype)
 
 
-    def chan(self, nx, nz, Lx, Lz, kappa, nu, dt, stepper=0, dtype=np.float64, b_bottom=None, b_top=0.0, refine=0):
+    def chan(self, nx, nz, Lx, Lz, kappa, nu, dt, stepper=0, dtype=np.float64, b_bottom=None, b_top=0.0, refine=0,
+             split=0):
         d = _lib.ChanDesc()
-        d.refine_sweeps = refine
+        d.refine_sweeps, d.split_inverse = refine, split
         d.nx, d.nz, d.Lx, d.Lz = nx, nz, Lx, Lz
         d.kappa, d.nu, d.dt = kappa, nu, dt
         d.dtype = _lib.F32 if dtype == np.float32 else _lib.F64
@@ -84,9 +85,9 @@ class Device:
 
 
     def chan3(self, nx, ny, nz, Lx, Ly, Lz, kappa, nu, dt, stepper=2, dtype=np.float64, b_bottom=None, b_top=0.0,
-              refine=0):
+              refine=0, split=0):
         d = _lib.Chan3Desc()
-        d.refine_sweeps = refine
+        d.refine_sweeps, d.split_inverse = refine, split
         d.nx, d.ny, d.nz, d.Lx, d.Ly, d.Lz = nx, ny, nz, Lx, Ly, Lz
         d.kappa, d.nu, d.dt = kappa, nu, dt
         d.dtype = _lib.F32 if dtype == np.float32 else _lib.F64

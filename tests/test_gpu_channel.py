@@ -266,6 +266,9 @@ def test_channel_rejects_unsupported(gpu):
     assert e.value.code == _lib.ERR_INVALID
     with pytest.raises(_lib.TarangError):
         gpu.chan(2048, 16, 4.0, 1.0, kappa, nu, 1e-2, dtype=np.float64)
+    with pytest.raises(_lib.TarangError) as e:  # split inverses belong to the FP32 state
+        gpu.chan(32, 16, 4.0, 1.0, kappa, nu, 1e-2, dtype=np.float64, split=1)
+    assert e.value.code == _lib.ERR_INVALID and "split_inverse" in str(e.value)
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
@@ -300,10 +303,11 @@ def test_channel_sbdf_parity(gpu, oracle, ts, dtype):
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("ts", ["RK222", "SBDF2"])
 def test_channel_refinement_sweeps(gpu, oracle, ts, dtype):
-    """FP64-residual refinement of the per-mode solves (refine_sweeps 1 and 2) against the
-    oracle at the 2-D workload's parameters (Ra = 2e6, Pr = 1, Nz = 129): FP64 stays at
-    1e-12; in FP32 the error of the FP32-rounded inverses is removed, so the state error is
-    at most that without refinement and within 1e-5."""
+    """FP64-residual refinement of the per-mode solves (refine_sweeps 1 and 2) and, in FP32,
+    split inverses (FP32 + bfloat16 remainder, alone and with a sweep) against the oracle
+    at the 2-D workload's parameters (Ra = 2e6, Pr = 1, Nz = 129): FP64 stays at 1e-12; in
+    FP32 the error of the FP32-rounded inverses is removed, so the state error is at most
+    that of the plain FP32 form and within 1e-5."""
     nx, N, Lx, Lz, dt = 32, 129, 4.0, 1.0, 1e-2
     kappa, nu = params(Ra=2e6)
     st = random_state(oracle, nx, N, Lx, Lz, seed=13)
@@ -313,20 +317,52 @@ def test_channel_refinement_sweeps(gpu, oracle, ts, dtype):
     ref.step(3)
     want = {v: ref.get(v) for v in (oracle.RBC_B, oracle.RBC_UX, oracle.RBC_UZ)}
     errs = {}
-    for sweeps in (0, 1, 2):
-        print(f"channel refinement {ts} sweeps={sweeps} {dtype.__name__}", flush=True)
-        s = gpu.chan(nx, N, Lx, Lz, kappa, nu, dt, stepper=TS[ts], dtype=dtype, refine=sweeps)
+    forms = [(0, 0), (1, 0), (2, 0)] + ([(0, 1), (1, 1)] if dtype == np.float32 else [])
+    for sweeps, split in forms:
+        print(f"channel refinement {ts} sweeps={sweeps} split={split} {dtype.__name__}", flush=True)
+        s = gpu.chan(nx, N, Lx, Lz, kappa, nu, dt, stepper=TS[ts], dtype=dtype, refine=sweeps, split=split)
         for var, c in st.items():
             s.set(var, c)
         s.step(3)
-        errs[sweeps] = max(relerr(s.get(v), w) for v, w in want.items())
+        errs[(sweeps, split)] = max(relerr(s.get(v), w) for v, w in want.items())
         s.close()
-    print(f"  max rel err by sweeps: {errs}", flush=True)
+    print(f"  max rel err by (sweeps, split): {errs}", flush=True)
     if dtype == np.float64:
         assert max(errs.values()) < 1e-12, errs
     else:
-        assert errs[1] <= errs[0] * 1.05 and errs[2] <= errs[0] * 1.05, errs
-        assert errs[1] < 1e-5 and errs[2] < 1e-5, errs
+        for f in forms[1:]:
+            assert errs[f] <= errs[(0, 0)] * 1.05 and errs[f] < 1e-5, (f, errs)
+
+
+def test_channel_fp32_forms_long_run(gpu):
+    """The FP32 forms against the FP64 form (itself pinned to the oracle at 1e-12) over 100
+    RK222 steps of the 2-D workload's physics (Ra = 2e6, Pr = 1, dt = 1e-2, Nz = 256,
+    conduction plus noise) on 32 x 256: split inverses (FP32 + bfloat16 remainder) and
+    refinement sweeps remove the error of the FP32-rounded inverses; b stays within 1e-6
+    of FP64."""
+    nx, N, Lx, Lz, dt, steps = 32, 256, 4.0, 1.0, 1e-2, 100
+    kappa, nu = params(Ra=2e6)
+    z = Lz * (1 - np.cos(np.pi * np.arange(N) / (N - 1))) / 2
+    rng = np.random.default_rng(42)
+    bg = (Lz - z)[None, :] + 1e-3 * rng.standard_normal((nx, N)) * (z * (Lz - z))[None, :]
+    k = np.arange(N)
+    T = np.cos(np.outer(np.pi * np.arange(N) / (N - 1), k)) * ((-1.0) ** k)[None, :]
+    Bc = np.linalg.solve(T, np.fft.rfft(bg, axis=0).T).T
+    out = {}
+    for name, dtype, sweeps, split in (("f64", np.float64, 0, 0), ("plain", np.float32, 0, 0),
+                                       ("split", np.float32, 0, 1), ("sweeps1", np.float32, 1, 0)):
+        print(f"channel long run {name}", flush=True)
+        s = gpu.chan(nx, N, Lx, Lz, kappa, nu, dt, stepper=0, dtype=dtype, refine=sweeps, split=split)
+        s.set(_lib.CHAN_B, Bc)
+        for v in (_lib.CHAN_P, _lib.CHAN_UX, _lib.CHAN_UZ):
+            s.set(v, np.zeros_like(Bc))
+        s.step(steps)
+        out[name] = s.get(_lib.CHAN_B)
+        s.close()
+    errs = {n: relerr(out[n], out["f64"]) for n in ("plain", "split", "sweeps1")}
+    print(f"  b rel err vs FP64 after {steps} steps: {errs}", flush=True)
+    assert errs["split"] < 1e-6 and errs["sweeps1"] < 1e-6, errs
+    assert errs["split"] <= errs["plain"] and errs["sweeps1"] <= errs["plain"], errs
 
 
 def test_channel_rhs_full_size(gpu, oracle):

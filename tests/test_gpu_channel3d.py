@@ -200,15 +200,19 @@ def test_channel3d_rejects_unsupported(gpu):
         gpu.chan3(16, 16, 300, 4.0, 4.0, 1.0, kappa, nu, 1e-2)
     with pytest.raises(_lib.TarangError):
         gpu.chan3(16, 16, 9, 4.0, 4.0, 1.0, kappa, nu, 1e-2, stepper=3)
+    with pytest.raises(_lib.TarangError) as e:  # split inverses belong to the FP32 state
+        gpu.chan3(16, 16, 9, 4.0, 4.0, 1.0, kappa, nu, 1e-2, dtype=np.float64, split=1)
+    assert e.value.code == _lib.ERR_INVALID and "split_inverse" in str(e.value)
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_channel3d_refinement_sweeps(gpu, oracle, dtype):
-    """FP64-residual refinement of the per-mode solves (refine_sweeps 1 and 2) against the
-    oracle: FP64 stays at 1e-12; in FP32 the solve error of the FP32-rounded inverses is
-    removed, so the state error after SBDF2 steps is at most that without refinement and
-    within 1e-5.  Nz = 65 keeps the tau systems as ill-conditioned as the 3-D workload's
-    (condition numbers ~1e7)."""
+    """FP64-residual refinement of the per-mode solves (refine_sweeps 1 and 2) and, in FP32,
+    split inverses (FP32 + bfloat16 remainder, alone and with a sweep) against the oracle:
+    FP64 stays at 1e-12; in FP32 the solve error of the FP32-rounded inverses is removed,
+    so the state error after SBDF2 steps is at most that of the plain FP32 form and within
+    1e-5.  Nz = 65 keeps the tau systems as ill-conditioned as the 3-D workload's (condition
+    numbers ~1e7)."""
     nx, ny, N, Lx, Ly, Lz = 16, 16, 65, 4.0, 4.0, 1.0
     kappa, nu = (1e7 * 0.7) ** -0.5, (1e7 / 0.7) ** -0.5
     st, d = random_state3(oracle, nx, ny, N, Lx, Ly, Lz, seed=7)
@@ -218,20 +222,53 @@ def test_channel3d_refinement_sweeps(gpu, oracle, dtype):
     ref.step(3)
     want = {v: ref.get(v) for v in (oracle.RBC3_B, oracle.RBC3_UX, oracle.RBC3_UY, oracle.RBC3_UZ)}
     errs = {}
-    for sweeps in (0, 1, 2):
-        print(f"channel3d refinement sweeps={sweeps} {dtype.__name__}", flush=True)
-        s = gpu.chan3(nx, ny, N, Lx, Ly, Lz, kappa, nu, 2e-3, dtype=dtype, refine=sweeps)
+    forms = [(0, 0), (1, 0), (2, 0)] + ([(0, 1), (1, 1)] if dtype == np.float32 else [])
+    for sweeps, split in forms:
+        print(f"channel3d refinement sweeps={sweeps} split={split} {dtype.__name__}", flush=True)
+        s = gpu.chan3(nx, ny, N, Lx, Ly, Lz, kappa, nu, 2e-3, dtype=dtype, refine=sweeps, split=split)
         for v, c in st.items():
             s.set(v, c)
         s.step(3)
-        errs[sweeps] = max(relerr(s.get(v), w) for v, w in want.items())
+        errs[(sweeps, split)] = max(relerr(s.get(v), w) for v, w in want.items())
         s.close()
-    print(f"  max rel err by sweeps: {errs}", flush=True)
+    print(f"  max rel err by (sweeps, split): {errs}", flush=True)
     if dtype == np.float64:
         assert max(errs.values()) < 1e-12, errs
     else:
-        assert errs[1] <= errs[0] * 1.05 and errs[2] <= errs[0] * 1.05, errs
-        assert errs[1] < 1e-5 and errs[2] < 1e-5, errs
+        for f in forms[1:]:
+            assert errs[f] <= errs[(0, 0)] * 1.05 and errs[f] < 1e-5, (f, errs)
+
+
+def test_channel3d_fp32_forms_long_run(gpu):
+    """The FP32 forms against the FP64 form (itself pinned to the oracle at 1e-12) over 300
+    SBDF2 steps of the 3-D workload's physics (Ra = 1e7, Pr = 0.7, dt = 2e-3, Nz = 128,
+    conduction plus noise) on a 16 x 16 grid: the FP32-rounded inverses act as a slightly
+    different operator at every step, an error that split inverses (FP32 + bfloat16
+    remainder) and refinement sweeps remove; both stay within 1e-6 of FP64 in b."""
+    nx = ny = 16
+    N, Lx, Ly, Lz, dt, steps = 128, 4.0, 4.0, 1.0, 2e-3, 300
+    kappa, nu = (1e7 * 0.7) ** -0.5, (1e7 / 0.7) ** -0.5
+    z = zgrid(N, Lz)
+    rng = np.random.default_rng(42)
+    bg = (Lz - z)[None, None, :] + 1e-3 * rng.standard_normal((nx, ny, N)) * (z * (Lz - z))[None, None, :]
+    k = np.arange(N)
+    T = np.cos(np.outer(np.pi * np.arange(N) / (N - 1), k)) * ((-1.0) ** k)[None, :]
+    Bc = np.einsum("kz,mnz->mnk", np.linalg.inv(T), np.fft.fft(np.fft.rfft(bg, axis=0), axis=1))
+    out = {}
+    for name, dtype, sweeps, split in (("f64", np.float64, 0, 0), ("plain", np.float32, 0, 0),
+                                       ("split", np.float32, 0, 1), ("sweeps1", np.float32, 1, 0)):
+        print(f"channel3d long run {name}", flush=True)
+        s = gpu.chan3(nx, ny, N, Lx, Ly, Lz, kappa, nu, dt, stepper=2, dtype=dtype, refine=sweeps, split=split)
+        s.set(_lib.CHAN3_B, Bc)
+        for v in (_lib.CHAN3_P, _lib.CHAN3_UX, _lib.CHAN3_UY, _lib.CHAN3_UZ):
+            s.set(v, np.zeros_like(Bc))
+        s.step(steps)
+        out[name] = s.get(_lib.CHAN3_B)
+        s.close()
+    errs = {n: relerr(out[n], out["f64"]) for n in ("plain", "split", "sweeps1")}
+    print(f"  b rel err vs FP64 after {steps} steps: {errs}", flush=True)
+    assert errs["split"] < 1e-6 and errs["sweeps1"] < 1e-6, errs
+    assert errs["split"] <= errs["plain"] and errs["sweeps1"] <= errs["plain"], errs
 
 
 def test_channel3d_rhs_full_size(gpu, oracle):

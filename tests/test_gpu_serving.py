@@ -227,7 +227,8 @@ def config_size_requests(dev):
                     [H.to_dev(C, cdt, dev)], [(N // 2 + 1, N)]))
     for (nx, nz, Ra, steps, arr) in ((128, 32, 1e5, 60, 0.002), (512, 128, 3e6, 30, 0.003), (1024, 256, 1e8, 20, 0.004)):
         d = chan_desc(nx, nz, Ra, 0, 1e-3)
-        d.refine_sweeps = 1
+        d.split_inverse = 1  # the stream's FP32 form; one request also refines
+        d.refine_sweeps = 1 if nx == 512 else 0
         z = (1 - np.cos(np.pi * np.arange(nz) / (nz - 1))) / 2
         bg = (1 - z)[None, :] + 1e-3 * rng.standard_normal((nx, nz)) * (z * (1 - z))[None, :]
         k = np.arange(nz)
