@@ -24,139 +24,12 @@ namespace tarang {
 //   0 ux, 1 uy, 2 uz, then for f in (b, ux, uy, uz): 3+3f dx f, 4+3f dy f, 5+3f dz f
 constexpr int kCh3Planes = 15;
 
-struct Ch3Geom {
-    int nx, ny, nxh, N, nmodes;
-    double k0x, k0y;
-};
-
-__host__ __device__ inline double ch3_ky(const Ch3Geom& g, int n) {
-    const int kk = n < (g.ny + 1) / 2 ? n : n - g.ny;  // FFT order
-    return g.k0y * kk;
-}
-
-// ---------------------------------------------------------------------------
-// k3_zbwd: z-backward transform of the operands on the matrix cores.  Workgroup =
-// (tile of QT modes, field f in (b, ux, uy, uz)); the field's coefficients of the
-// tile are the columns of an LDS block C[k][col], in groups of 16 modes: column
-// 32 g + j is Re, 32 g + 16 + j is Im of mode q0 + 16 g + j, and
-//   value[z][col] = sum_k BT[k][z] C[k][col],   dz[z][col] = sum_k BDT[k][z] C[k][col]
-// as 16 x 16 tiles (wave w: z tiles ZPW w .. ZPW w + ZPW - 1, then the next pass; every
-// column tile, both products).  A lane then holds Re and Im of one mode in two accumulators and writes
-// whole complex values straight to the z-major planes (16 consecutive modes per row).
-// Derived planes: dx f = i kx value, dy f = i ky value.  On the kx = 0 and kx = Nx/2
-// lines the projected state's x derivative vanishes, and so does its y derivative at
-// the y Nyquist; the kx = Nx/2 line is halved.
-// ---------------------------------------------------------------------------
-// modes per workgroup (QT, a multiple of 16), k steps per load batch (U), threads per
-// workgroup (NT) and z tiles per wavefront and pass (ZPW)
-template <typename T> struct ZbwdTile { static constexpr int QT = 32, U = 2, NT = 512, ZPW = 1; };
-template <> struct ZbwdTile<double> { static constexpr int QT = 16, U = 4, NT = 256, ZPW = 2; };
-
-__host__ __device__ inline int zbwd_lds_stride(int qt) { return 2 * qt + 1; }
-
 template <typename T, int NT>
 __global__ __launch_bounds__(NT, 768 / NT) void k3_zbwd(const cplx<T>* __restrict__ b, const cplx<T>* __restrict__ ux,
                                                  const cplx<T>* __restrict__ uy, const cplx<T>* __restrict__ uz,
                                                  Ch3Geom geo, const T* __restrict__ BT, const T* __restrict__ BDT,
                                                  cplx<T>* __restrict__ G) {
-    static_assert(NT == ZbwdTile<T>::NT, "workgroup size of the tile plan");
-    using Op = MfmaOp<T>;
-    using acc4 = typename Op::acc4;
-    using V = cplx<T>;
-    constexpr int QT = ZbwdTile<T>::QT, NG = QT / 16, CT = 2 * NG, S = 2 * QT + 1;
-    constexpr int ZPW = ZbwdTile<T>::ZPW, NWV = NT / 64;
-    extern __shared__ unsigned char smem_raw[];
-    T* C = reinterpret_cast<T*>(smem_raw);  // [KP][S]
-    const int N = geo.N, nmodes = geo.nmodes;
-    const int KP = (N + 15) & ~15, nzt = KP >> 4, ksteps = (N + 3) >> 2;
-    const int f = blockIdx.y, q0 = blockIdx.x * QT;
-    const V* in = f == 0 ? b : (f == 1 ? ux : (f == 2 ? uy : uz));
-    for (int idx = threadIdx.x; idx < QT * KP; idx += NT) {
-        const int q = idx / KP, k = idx - q * KP;
-        V v{T(0), T(0)};
-        if (q0 + q < nmodes && k < N) v = in[(size_t)(q0 + q) * N + k];
-        const int col = 32 * (q >> 4) + (q & 15);
-        C[k * S + col] = v.x;
-        C[k * S + col + 16] = v.y;
-    }
-    __syncthreads();
-    const int lane = threadIdx.x & 63, r16 = lane & 15, q4 = lane >> 4, wave = threadIdx.x >> 6;
-    const size_t plane = (size_t)N * nmodes;
-    const bool xeven = (geo.nx % 2) == 0, yeven = (geo.ny % 2) == 0;
-    // per mode group: this lane's mode and its derivative factors
-    T kxv[NG], kyv[NG], wv[NG];
-    bool dxz[NG], dyz[NG], live[NG];
-    int qq[NG];
-#pragma unroll
-    for (int g = 0; g < NG; ++g) {
-        const int q = q0 + 16 * g + r16;
-        qq[g] = q;
-        live[g] = q < nmodes;
-        const int m = q % geo.nxh, n = q / geo.nxh;
-        const bool xny = xeven && m == geo.nx / 2, xline = m == 0 || xny;
-        kxv[g] = T(geo.k0x * m);
-        kyv[g] = T(ch3_ky(geo, n));
-        wv[g] = xny ? T(0.5) : T(1);
-        dxz[g] = xline;
-        dyz[g] = xline && yeven && n == geo.ny / 2;
-    }
-    for (int zt0 = ZPW * wave; zt0 < nzt; zt0 += ZPW * NWV) {
-        acc4 acc[ZPW][2][CT];  // [z tile][value, dz][column tile]
-#pragma unroll
-        for (int a = 0; a < ZPW; ++a)
-#pragma unroll
-            for (int p = 0; p < 2; ++p)
-#pragma unroll
-                for (int ct = 0; ct < CT; ++ct) acc[a][p][ct] = acc4{T(0), T(0), T(0), T(0)};
-        constexpr int U = ZbwdTile<T>::U;  // k steps per batch: its loads are issued before its products
-        for (int ks = 0; ks < ksteps; ks += U) {
-            T am[U][ZPW][2], bv[U][CT];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int k = 4 * (ks + u) + q4;
-#pragma unroll
-                for (int a = 0; a < ZPW; ++a) {
-                    const int z = 16 * (zt0 + a) + r16;
-                    const bool in = k < N && z < N;
-                    const int o = in ? k * N + z : 0;
-                    am[u][a][0] = BT[o];
-                    am[u][a][1] = BDT[o];
-                    am[u][a][0] = in ? am[u][a][0] : T(0);
-                    am[u][a][1] = in ? am[u][a][1] : T(0);
-                }
-                const int kr = k < KP ? k : KP - 1;  // rows N .. KP - 1 are zero; beyond, a is zero
-#pragma unroll
-                for (int ct = 0; ct < CT; ++ct) bv[u][ct] = C[kr * S + 16 * ct + r16];
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-#pragma unroll
-                for (int a = 0; a < ZPW; ++a)
-#pragma unroll
-                    for (int ct = 0; ct < CT; ++ct) {
-                        acc[a][0][ct] = Op::mma(am[u][a][0], bv[u][ct], acc[a][0][ct]);
-                        acc[a][1][ct] = Op::mma(am[u][a][1], bv[u][ct], acc[a][1][ct]);
-                    }
-        }
-#pragma unroll
-        for (int a = 0; a < ZPW; ++a)
-#pragma unroll
-            for (int g = 0; g < NG; ++g)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int z = 16 * (zt0 + a) + Op::row(v, q4);
-                    if (z >= N || !live[g]) continue;
-                    const V val{acc[a][0][2 * g][v] * wv[g], acc[a][0][2 * g + 1][v] * wv[g]};
-                    const V dzv{acc[a][1][2 * g][v] * wv[g], acc[a][1][2 * g + 1][v] * wv[g]};
-                    const V dx = dxz[g] ? V{T(0), T(0)} : V{-kxv[g] * val.y, kxv[g] * val.x};  // i kx v
-                    const V dy = dyz[g] ? V{T(0), T(0)} : V{-kyv[g] * val.y, kyv[g] * val.x};
-                    const size_t o = (size_t)z * nmodes + qq[g];
-                    if (f > 0) G[(size_t)(f - 1) * plane + o] = val;  // ux, uy, uz
-                    G[(size_t)(3 + 3 * f) * plane + o] = dx;
-                    G[(size_t)(4 + 3 * f) * plane + o] = dy;
-                    G[(size_t)(5 + 3 * f) * plane + o] = dzv;
-                }
-    }
+    zbwd_mfma<T, NT, false>(b, ux, uy, uz, geo, BT, BDT, G);
 }
 
 // ---------------------------------------------------------------------------

@@ -84,76 +84,193 @@ template <> struct MfmaOp<float> {
 };
 
 // ---------------------------------------------------------------------------
-// k_ch_zbwd: z-backward transform of the product operands.  Work item (tile of
-// TK modes, job): job 0..5 = {B b, B ux, B uz, BD b, BD ux, BD uz} where B
-// evaluates a Chebyshev series on the Gauss-Lobatto points and BD = B*Dz.
-// Outputs (kx fastest, one N x nxh plane each): 0 ux, 1 uz, 2 dx b, 3 dz b,
-// 4 dx ux, 5 dz ux, 6 dx uz, 7 dz uz.  The kx = 0 and kx = Nx/2 columns take the
-// real part (the grid round trip of the reference), the kx = Nx/2 column is
-// halved (Nyquist split on the padded x grid).
-// Matrices are stored transposed (matT[k*N + z]) so that threads z read
-// consecutive words.
+struct Ch3Geom {
+    int nx, ny, nxh, N, nmodes;
+    double k0x, k0y;
+};
+
+__host__ __device__ inline double ch3_ky(const Ch3Geom& g, int n) {
+    const int kk = n < (g.ny + 1) / 2 ? n : n - g.ny;  // FFT order
+    return g.k0y * kk;
+}
+
 // ---------------------------------------------------------------------------
-template <typename T, int TK, int NT>
-__global__ __launch_bounds__(NT) void k_ch_zbwd(const cplx<T>* __restrict__ b, const cplx<T>* __restrict__ ux,
-                                                const cplx<T>* __restrict__ uz, int nxh, int nx, int N,
-                                                const T* __restrict__ BT, const T* __restrict__ BDT, T k0x,
-                                                cplx<T>* __restrict__ G) {
+// zbwd_mfma: z-backward transform of the operands on the matrix cores (k3_zbwd in 3-D,
+// k_ch_zbwd in 2-D).  Workgroup = (tile of QT modes, field f in (b, ux, uy, uz), or
+// (b, ux, uz) in 2-D; with gridDim.z > 1 a share of the z tiles); the field's
+// coefficients of the tile are the columns of an LDS block C[k][col], in groups of 16
+// modes: column 32 g + j is Re, 32 g + 16 + j is Im of mode q0 + 16 g + j.
+// Parity split: the Gauss-Lobatto grid is symmetric (point N-1-z mirrors z) and
+// T_k(-x) = (-1)^k T_k(x), T_k'(-x) = (-1)^(k+1) T_k'(x), so with the even- and odd-k sums
+//   E = sum_{k even} BT[k][z] C[k],  O = sum_{k odd} BT[k][z] C[k]  (and E', O' with BDT)
+// over the first half of the points (z < ceil(N/2)):
+//   value[z] = E + O,  value[N-1-z] = E - O,  dz[z] = E' + O',  dz[N-1-z] = O' - E'
+// half the products of evaluating every point.  Wave w takes first-half z tile w (then
+// the next pass) with every column tile, both products and both parities, its loads one
+// batch ahead of its products.  A lane then holds Re and Im of one mode and writes whole
+// complex values straight to the z-major planes (16 consecutive modes per row).
+// Derived planes: dx f = i kx value, dy f = i ky value.  On the kx = 0 and kx = Nx/2
+// lines the projected state's x derivative vanishes, and so does its y derivative at
+// the y Nyquist; the kx = Nx/2 line is halved.  2-D: eight planes (no y), and the kx = 0
+// and kx = Nx/2 lines keep only their real part (the grid round trip of the state).
+// ---------------------------------------------------------------------------
+// modes per workgroup (QT, a multiple of 16), k steps per load batch (U) and threads per
+// workgroup (NT)
+template <typename T> struct ZbwdTile { static constexpr int QT = 32, U = 4, NT = 256; };
+template <> struct ZbwdTile<double> { static constexpr int QT = 16, U = 2, NT = 256; };
+
+__host__ __device__ inline int zbwd_lds_stride(int qt) { return 2 * qt + 1; }
+// first-half z tiles of the parity-split evaluation
+__host__ __device__ inline int zbwd_half_tiles(int N) { return ((N + 1) / 2 + 15) / 16; }
+
+template <typename T, int NT, bool TWO_D>
+__device__ inline void zbwd_mfma(const cplx<T>* __restrict__ b, const cplx<T>* __restrict__ ux,
+                                 const cplx<T>* __restrict__ uy, const cplx<T>* __restrict__ uz, const Ch3Geom& geo,
+                                 const T* __restrict__ BT, const T* __restrict__ BDT, cplx<T>* __restrict__ G) {
+    static_assert(NT == ZbwdTile<T>::NT, "workgroup size of the tile plan");
+    using Op = MfmaOp<T>;
+    using acc4 = typename Op::acc4;
     using V = cplx<T>;
+    constexpr int QT = ZbwdTile<T>::QT, NG = QT / 16, CT = 2 * NG, S = 2 * QT + 1;
+    constexpr int U = ZbwdTile<T>::U, NWV = NT / 64;
     extern __shared__ unsigned char smem_raw[];
-    V* src = reinterpret_cast<V*>(smem_raw);  // [TK][N]
-    const int job = blockIdx.y;
-    const int m0 = blockIdx.x * TK;
-    const V* in = (job % 3 == 0) ? b : (job % 3 == 1 ? ux : uz);
-    const T* mat = job < 3 ? BT : BDT;
-    for (int idx = threadIdx.x; idx < TK * N; idx += NT) {
-        const int c = idx / N, k = idx - c * N;
-        const int m = m0 + c;
-        src[c * N + k] = m < nxh ? in[(size_t)m * N + k] : V{T(0), T(0)};
+    T* C = reinterpret_cast<T*>(smem_raw);  // [KP][S]
+    const int N = geo.N, nmodes = geo.nmodes;
+    const int KP = (N + 15) & ~15, NH = (N + 1) / 2, nzh = zbwd_half_tiles(N);
+    const int f = blockIdx.y, q0 = blockIdx.x * QT;
+    const V* in = f == 0 ? b : (f == 1 ? ux : (f == 2 && !TWO_D ? uy : uz));
+    for (int idx = threadIdx.x; idx < QT * KP; idx += NT) {
+        const int q = idx / KP, k = idx - q * KP;
+        V v{T(0), T(0)};
+        if (q0 + q < nmodes && k < N) v = in[(size_t)(q0 + q) * N + k];
+        const int col = 32 * (q >> 4) + (q & 15);
+        C[k * S + col] = v.x;
+        C[k * S + col + 16] = v.y;
     }
     __syncthreads();
-    const size_t plane = (size_t)N * nxh;
-    const bool xeven = (nx % 2) == 0;
-    for (int z = threadIdx.x; z < N; z += NT) {
-        V acc[TK];
+    const int lane = threadIdx.x & 63, r16 = lane & 15, q4 = lane >> 4;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const size_t plane = (size_t)N * nmodes;
+    const bool xeven = (geo.nx % 2) == 0, yeven = (geo.ny % 2) == 0;
+    // per mode group: this lane's mode and its derivative factors
+    T kxv[NG], kyv[NG], wv[NG];
+    bool dxz[NG], dyz[NG], live[NG];
+    int qq[NG];
 #pragma unroll
-        for (int c = 0; c < TK; ++c) acc[c] = V{T(0), T(0)};
-        for (int k = 0; k < N; ++k) {
-            const T a = mat[(size_t)k * N + z];
-#pragma unroll
-            for (int c = 0; c < TK; ++c) {
-                const V s = src[c * N + k];
-                acc[c].x += a * s.x;
-                acc[c].y += a * s.y;
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < TK; ++c) {
-            const int m = m0 + c;
-            if (m >= nxh) continue;
-            // The state reaches the product through the grid: at kx = 0 and kx = Nx/2 only
-            // the real part survives, and the x derivative of that real value is
-            // imaginary there, so it drops out too.
-            V v = acc[c];
-            const bool nyq = xeven && m == nx / 2;
-            const T kx = k0x * T(m);
-            V dx{-kx * v.y, kx * v.x};  // (i kx) v
-            if (m == 0 || nyq) {
-                v.y = T(0);
-                dx = V{T(0), T(0)};
-            }
-            if (nyq) v = V{v.x * T(0.5), T(0)};
-            const size_t o = (size_t)z * nxh + m;
-            switch (job) {
-                case 0: G[2 * plane + o] = dx; break;                       // dx b
-                case 1: G[0 * plane + o] = v; G[4 * plane + o] = dx; break;  // ux, dx ux
-                case 2: G[1 * plane + o] = v; G[6 * plane + o] = dx; break;  // uz, dx uz
-                case 3: G[3 * plane + o] = v; break;                        // dz b
-                case 4: G[5 * plane + o] = v; break;                        // dz ux
-                default: G[7 * plane + o] = v; break;                       // dz uz
-            }
-        }
+    for (int g = 0; g < NG; ++g) {
+        const int q = q0 + 16 * g + r16;
+        qq[g] = q;
+        live[g] = q < nmodes;
+        const int m = q % geo.nxh, n = q / geo.nxh;
+        const bool xny = xeven && m == geo.nx / 2, xline = m == 0 || xny;
+        kxv[g] = T(geo.k0x * m);
+        kyv[g] = T(ch3_ky(geo, n));
+        wv[g] = xny ? T(0.5) : T(1);
+        dxz[g] = xline;
+        dyz[g] = xline && yeven && n == geo.ny / 2;
     }
+    // first-half z tiles of this workgroup: all, or a share when the grid splits z (blockIdx.z)
+    const int zper = (nzh + (int)gridDim.z - 1) / (int)gridDim.z;
+    const int zbeg = zper * (int)blockIdx.z, zend = min(nzh, zbeg + zper);
+    for (int zt = zbeg + wave; zt < zend; zt += NWV) {
+        const int z = 16 * zt + r16;  // this lane's row of the A operand
+        acc4 acc[2][2][CT];           // [parity][value, dz][column tile]
+#pragma unroll
+        for (int par = 0; par < 2; ++par)
+#pragma unroll
+            for (int p = 0; p < 2; ++p)
+#pragma unroll
+                for (int ct = 0; ct < CT; ++ct) acc[par][p][ct] = acc4{T(0), T(0), T(0), T(0)};
+        // k steps of both parities in one sequence: step s < s0 is even-k step s, then odd;
+        // k = 2 (4 s' + q4) + parity
+        const int s0 = (NH + 3) >> 2, s1 = (N / 2 + 3) >> 2, nsteps = s0 + s1;
+        // two batches in turn: one batch's loads are in flight while the other's products run
+        T amA[U][2], bvA[U][CT], amB[U][2], bvB[U][CT];
+        auto load = [&](int sb, T (&am)[U][2], T (&bv)[U][CT]) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int st = sb + u, par = st < s0 ? 0 : 1, sp = par ? st - s0 : st;
+                const int k = 2 * (4 * sp + q4) + par;
+                const bool ok = st < nsteps && k < N && z < N;
+                const int o = ok ? k * N + z : 0;
+                am[u][0] = BT[o];
+                am[u][1] = BDT[o];
+                am[u][0] = ok ? am[u][0] : T(0);
+                am[u][1] = ok ? am[u][1] : T(0);
+                const int kr = k < KP ? k : KP - 1;  // rows N .. KP - 1 are zero; beyond, a is zero
+#pragma unroll
+                for (int ct = 0; ct < CT; ++ct) bv[u][ct] = C[kr * S + 16 * ct + r16];
+            }
+        };
+        auto multiply = [&](int sb, const T (&am)[U][2], const T (&bv)[U][CT]) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (sb + u >= nsteps) break;  // wave-uniform
+#pragma unroll
+                for (int ct = 0; ct < CT; ++ct) {
+                    if (sb + u < s0) {  // even k
+                        acc[0][0][ct] = Op::mma(am[u][0], bv[u][ct], acc[0][0][ct]);
+                        acc[0][1][ct] = Op::mma(am[u][1], bv[u][ct], acc[0][1][ct]);
+                    } else {
+                        acc[1][0][ct] = Op::mma(am[u][0], bv[u][ct], acc[1][0][ct]);
+                        acc[1][1][ct] = Op::mma(am[u][1], bv[u][ct], acc[1][1][ct]);
+                    }
+                }
+            }
+        };
+        load(0, amA, bvA);
+        for (int sb = 0; sb < nsteps; sb += 2 * U) {
+            if (sb + U < nsteps) load(sb + U, amB, bvB);
+            multiply(sb, amA, bvA);
+            if (sb + U >= nsteps) break;
+            if (sb + 2 * U < nsteps) load(sb + 2 * U, amA, bvA);
+            multiply(sb + U, amB, bvB);
+        }
+        auto put = [&](int zz, int g, V val, V dzv) {
+            const V dx = dxz[g] ? V{T(0), T(0)} : V{-kxv[g] * val.y, kxv[g] * val.x};  // i kx v
+            const size_t o = (size_t)zz * nmodes + qq[g];
+            if constexpr (TWO_D) {
+                // no y transform follows: the kx = 0 and kx = Nx/2 lines keep their real part here
+                if (dxz[g]) val.y = dzv.y = T(0);
+                // planes 0 ux, 1 uz, 2 dx b, 3 dz b, 4 dx ux, 5 dz ux, 6 dx uz, 7 dz uz
+                if (f > 0) G[(size_t)(f - 1) * plane + o] = val;
+                G[(size_t)(2 + 2 * f) * plane + o] = dx;
+                G[(size_t)(3 + 2 * f) * plane + o] = dzv;
+            } else {
+                const V dy = dyz[g] ? V{T(0), T(0)} : V{-kyv[g] * val.y, kyv[g] * val.x};
+                if (f > 0) G[(size_t)(f - 1) * plane + o] = val;  // ux, uy, uz
+                G[(size_t)(3 + 3 * f) * plane + o] = dx;
+                G[(size_t)(4 + 3 * f) * plane + o] = dy;
+                G[(size_t)(5 + 3 * f) * plane + o] = dzv;
+            }
+        };
+#pragma unroll
+        for (int g = 0; g < NG; ++g)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int zr = 16 * zt + Op::row(v, q4);
+                if (zr >= NH || !live[g]) continue;
+                const T w = wv[g];
+                const T er = acc[0][0][2 * g][v], ei = acc[0][0][2 * g + 1][v];
+                const T orr = acc[1][0][2 * g][v], oi = acc[1][0][2 * g + 1][v];
+                const T dr = acc[0][1][2 * g][v], di = acc[0][1][2 * g + 1][v];
+                const T pr = acc[1][1][2 * g][v], pi = acc[1][1][2 * g + 1][v];
+                put(zr, g, V{(er + orr) * w, (ei + oi) * w}, V{(dr + pr) * w, (di + pi) * w});
+                const int zm = N - 1 - zr;
+                if (zm != zr) put(zm, g, V{(er - orr) * w, (ei - oi) * w}, V{(pr - dr) * w, (pi - di) * w});
+            }
+    }
+}
+
+// k_ch_zbwd: the 2-D z-backward transform, zbwd_mfma with planes 0 ux, 1 uz, 2 dx b,
+// 3 dz b, 4 dx ux, 5 dz ux, 6 dx uz, 7 dz uz (kx fastest, one N x nxh plane each).
+// grid (ceil(nxh / QT), 3 fields, z shares).
+template <typename T, int NT>
+__global__ __launch_bounds__(NT, 768 / NT) void k_ch_zbwd(const cplx<T>* __restrict__ b, const cplx<T>* __restrict__ ux,
+                                                   const cplx<T>* __restrict__ uz, Ch3Geom geo,
+                                                   const T* __restrict__ BT, const T* __restrict__ BDT,
+                                                   cplx<T>* __restrict__ G) {
+    zbwd_mfma<T, NT, true>(b, ux, nullptr, uz, geo, BT, BDT, G);
 }
 
 // ---------------------------------------------------------------------------
@@ -427,36 +544,48 @@ __global__ void k_ch_zero(cplx<T>* a, size_t n) {
 
 // L * P(x) per mode (one workgroup per mode), P = real part at kx = 0, Nx/2.
 // taus[m*8 + t]: 0 tau_p, 1 tau_b1, 2 tau_b2, 3 tau_u1x, 4 tau_u1z, 5 tau_u2x, 6 tau_u2z
+// The z derivatives come from the descending recurrence u_k = u_{k+2} + 2 (k+1) sz f_{k+1},
+// (Dz f)_k = u_k (halved at k = 0): one lane per (field, real / imaginary part, parity of
+// k), O(N) work per lane; Dzz = Dz(Dz .).
 template <typename T, int NT>
 __global__ __launch_bounds__(NT) void k_ch_lx(const cplx<T>* __restrict__ p, const cplx<T>* __restrict__ b,
                                               const cplx<T>* __restrict__ ux, const cplx<T>* __restrict__ uz,
                                               const cplx<T>* __restrict__ taus, int nxh, int nx, int N, double k0x,
                                               double sz, double kappa, double nu, cplx<T>* E1, cplx<T>* E2,
                                               cplx<T>* E3x, cplx<T>* E3z) {
+    static_assert(NT >= 16, "one lane per recurrence");
     extern __shared__ unsigned char smem_raw[];
-    double2* v = reinterpret_cast<double2*>(smem_raw);  // [4][N]: p, b, ux, uz
-    double2* d1 = v + 4 * N;                             // Dz of each
+    double* v = reinterpret_cast<double*>(smem_raw);  // [4][N][2]: p, b, ux, uz
+    double* d1 = v + 8 * N;                             // Dz of each
+    double* d2 = d1 + 8 * N;                            // Dzz of b, ux, uz
     const int m = blockIdx.x;
     const bool proj = (m == 0) || ((nx % 2) == 0 && m == nx / 2);
     const cplx<T>* src[4] = {p, b, ux, uz};
     for (int idx = threadIdx.x; idx < 4 * N; idx += NT) {
         const int f = idx / N, k = idx - f * N;
         const cplx<T> s = src[f][(size_t)m * N + k];
-        v[idx] = double2{(double)s.x, proj ? 0.0 : (double)s.y};
+        v[2 * idx] = (double)s.x;
+        v[2 * idx + 1] = proj ? 0.0 : (double)s.y;
     }
     __syncthreads();
-    // Dz of the 4 fields: output k sums (N - k)/2 terms, so odd fields run k backwards
-    // and a thread's items pair short sums with long ones
-    for (int idx = threadIdx.x; idx < 4 * N; idx += NT) {
-        const int f = idx / N, r = idx - f * N, k = (f & 1) ? N - 1 - r : r;
-        double2 acc{0, 0};
-        for (int j = k + 1; j < N; j += 2) {
-            const double c = cheb_dz(k, j, sz);
-            acc.x += c * v[f * N + j].x;
-            acc.y += c * v[f * N + j].y;
+    // out[(f - f0) N + k] = (Dz in[f])_k for nf fields starting at f0 (real and imaginary parts
+    // interleaved)
+    auto dz = [&](const double* in, double* out, int f0, int nf) {
+        const int t = threadIdx.x;
+        if (t >= 4 * nf) return;
+        const int f = f0 + (t >> 2), c = (t >> 1) & 1, par = t & 1;
+        const double* fi = in + 2 * (size_t)f * N + c;
+        double* fo = out + 2 * (size_t)(f - f0) * N + c;
+        double u = 0.0;
+#pragma unroll 4
+        for (int k = ((N - 1) & 1) == par ? N - 1 : N - 2; k >= 0; k -= 2) {
+            if (k + 1 < N) u += 2.0 * (k + 1) * sz * fi[2 * (k + 1)];
+            fo[2 * k] = k == 0 ? 0.5 * u : u;
         }
-        d1[f * N + k] = acc;
-    }
+    };
+    dz(v, d1, 0, 4);
+    __syncthreads();
+    dz(d1, d2, 1, 3);
     __syncthreads();
     double2 tau[7];
     for (int t = 0; t < 7; ++t) {  // tau_p has no x basis: it does not go through the x transform
@@ -464,19 +593,12 @@ __global__ __launch_bounds__(NT) void k_ch_lx(const cplx<T>* __restrict__ p, con
         tau[t] = double2{(double)s.x, (proj && t > 0) ? 0.0 : (double)s.y};
     }
     const double kx = k0x * m;
+    auto at = [&](const double* a, int f, int k) { return double2{a[2 * (f * N + k)], a[2 * (f * N + k) + 1]}; };
     for (int k = threadIdx.x; k < N; k += NT) {
-        double2 dzz[3] = {{0, 0}, {0, 0}, {0, 0}};  // Dzz of b, ux, uz = Dz(Dz .)
-        for (int j = k + 1; j < N; j += 2) {
-            const double c = cheb_dz(k, j, sz);
-#pragma unroll
-            for (int f = 0; f < 3; ++f) {
-                dzz[f].x += c * d1[(f + 1) * N + j].x;
-                dzz[f].y += c * d1[(f + 1) * N + j].y;
-            }
-        }
+        const double2 dzz[3] = {at(d2, 0, k), at(d2, 1, k), at(d2, 2, k)};  // Dzz of b, ux, uz
         const double dN = cheb_dz(k, N - 1, sz);  // Dz[:, N-1], the derivative of the lift
-        const double2 P = v[k], Bv = v[N + k], Ux = v[2 * N + k], Uz = v[3 * N + k];
-        const double2 DP = d1[k], DUz = d1[3 * N + k];
+        const double2 P = at(v, 0, k), Bv = at(v, 1, k), Ux = at(v, 2, k), Uz = at(v, 3, k);
+        const double2 DP = at(d1, 0, k), DUz = at(d1, 3, k);
         double2 e1{-kx * Ux.y + DUz.x, kx * Ux.x + DUz.y};
         if (k == N - 1) { e1.x += tau[4].x; e1.y += tau[4].y; }
         if (k == 0 && m == 0) { e1.x += tau[0].x; e1.y += tau[0].y; }

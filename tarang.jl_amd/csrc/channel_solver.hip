@@ -8,6 +8,7 @@
 //            x_i = (M + dt a_ii L)^-1 rhs;  F_i = F(P(x_i))
 //   final:   (b, ux, uz) = X0 + dt sum_j (bE_j F_j - bI_j L P(x_j));  p = taus = 0
 // P is the grid round trip of the state (real part at kx = 0 and kx = Nx/2).
+#include <algorithm>
 #include <cmath>
 #include <type_traits>
 #include <cstring>
@@ -23,7 +24,6 @@
 namespace tarang {
 
 namespace {
-constexpr int kTK = 4;
 
 bool channel_nx_ok(int nx, bool f64) {
     bool ok = false;
@@ -269,17 +269,20 @@ void ChannelSolver::rhs_t(const void* b, const void* ux, const void* uz, void* F
     const DType dt = std::is_same<T, float>::value ? kF32 : kF64;
     V* G = G_.as<V>();
     V* H = H_.as<V>();
-    const size_t lds = (size_t)kTK * N * sizeof(V);
-    const unsigned ntile = (unsigned)((nxh_ + kTK - 1) / kTK);
-    auto zb = [&](auto NTC) {
-        constexpr int NT = decltype(NTC)::value;
+    {
+        // Chebyshev evaluation on the matrix cores: tiles of QT modes x 3 fields, and the z
+        // tiles shared out over workgroups so that the few modes of a 2-D grid fill the GPU
+        constexpr int QT = ZbwdTile<T>::QT, NTZ = ZbwdTile<T>::NT;
+        constexpr int ZPASS = NTZ / 64;  // first-half z tiles of one pass of the waves
+        const size_t blds = (size_t)((N + 15) & ~15) * zbwd_lds_stride(QT) * sizeof(T);
+        const unsigned zsplit = (unsigned)std::max(1, (zbwd_half_tiles(N) + ZPASS - 1) / ZPASS);
+        const Ch3Geom geo{d_.nx, 1, nxh_, N, nxh_, 2.0 * M_PI / d_.Lx, 0.0};
         timed(0, [&] {
-            hipLaunchKernelGGL((k_ch_zbwd<T, kTK, NT>), dim3(ntile, 6), dim3(NT), lds, s, static_cast<const V*>(b),
-                               static_cast<const V*>(ux), static_cast<const V*>(uz), nxh_, d_.nx, N,
-                               BT_.as<const T>(), BDT_.as<const T>(), T(2.0 * M_PI / d_.Lx), G);
+            hipLaunchKernelGGL((k_ch_zbwd<T, NTZ>), dim3((unsigned)((nxh_ + QT - 1) / QT), 3, zsplit), dim3(NTZ), blds,
+                               s, static_cast<const V*>(b), static_cast<const V*>(ux), static_cast<const V*>(uz), geo,
+                               BT_.as<const T>(), BDT_.as<const T>(), G);
         });
-    };
-    if (N <= 64) zb(std::integral_constant<int, 64>{}); else zb(std::integral_constant<int, 256>{});
+    }
     TG_HIP(hipGetLastError());
     const T scale = T(1.0 / ((double)d_.nx * (double)mx_));
     dispatch_size(d_.nx, [&](auto NC) {
@@ -316,7 +319,7 @@ void ChannelSolver::lx_t(int st) {
     const int N = d_.nz;
     constexpr int NT = 256;
     timed(3, [&] {
-        hipLaunchKernelGGL((k_ch_lx<T, NT>), dim3(nxh_), dim3(NT), (size_t)8 * N * sizeof(double2), ctx_->stream,
+        hipLaunchKernelGGL((k_ch_lx<T, NT>), dim3(nxh_), dim3(NT), (size_t)22 * N * sizeof(double), ctx_->stream,
                            P_.as<const V>(), B_.as<const V>(), UX_.as<const V>(), UZ_.as<const V>(),
                            TAU_.as<const V>(), nxh_, d_.nx, N, 2.0 * M_PI / d_.Lx, 2.0 / d_.Lz, d_.kappa, d_.nu,
                            LX_[st][0].as<V>(), LX_[st][1].as<V>(), LX_[st][2].as<V>(), LX_[st][3].as<V>());
@@ -577,7 +580,7 @@ void ChannelSolver::profile(int nsteps, double* avg_ms, long* counts) {
 void ChannelSolver::kernel_bytes(double* out) const {
     const double vb = 2.0 * dtype_size((DType)d_.dtype), rb = dtype_size((DType)d_.dtype);
     const double nc = (double)nxh_ * d_.nz, N = d_.nz;
-    out[0] = (6 * nc + 8 * nc) * vb + 2 * N * N * rb;  // read b, ux, uz per job, write 8 planes, matrices
+    out[0] = (3 * nc + 8 * nc) * vb + 2 * N * N * rb;  // read b, ux, uz, write 8 planes, matrices
     out[1] = (8 * nc + 3 * nc) * vb;                    // read 8 planes, write 3
     out[2] = (3 * nc + 3 * nc) * vb + N * N * rb;       // read 3 planes, write 3 F
     out[3] = (4 * nc + 4 * nc) * vb;                    // read p, b, ux, uz; write 4 rows

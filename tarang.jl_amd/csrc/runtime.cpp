@@ -12,6 +12,14 @@ static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
 const char* last_error() { return g_last_error.c_str(); }
 
+// Host-to-device copy ordered on the context's own stream, then waited for.  A plain
+// hipMemcpy runs on the legacy stream, which HIP refuses while another thread captures a
+// step graph (a set-up thread building tables while the serving loop captures).
+void Context::upload(void* d, const void* host, size_t bytes) {
+    TG_HIP(hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, stream));
+    TG_HIP(hipStreamSynchronize(stream));
+}
+
 const void* Context::twiddle(int L, DType dt) {
     std::lock_guard<std::mutex> lk(mu);
     auto key = std::make_pair(L, static_cast<int>(dt));
@@ -29,12 +37,12 @@ const void* Context::twiddle(int L, DType dt) {
         std::vector<float2> h(L);
         for (int m = 0; m < L; ++m) h[m] = float2{(float)re[m], (float)im[m]};
         TG_HIP(hipMalloc(&d, sizeof(float2) * L));
-        TG_HIP(hipMemcpy(d, h.data(), sizeof(float2) * L, hipMemcpyHostToDevice));
+        upload(d, h.data(), sizeof(float2) * L);
     } else {
         std::vector<double2> h(L);
         for (int m = 0; m < L; ++m) h[m] = double2{re[m], im[m]};
         TG_HIP(hipMalloc(&d, sizeof(double2) * L));
-        TG_HIP(hipMemcpy(d, h.data(), sizeof(double2) * L, hipMemcpyHostToDevice));
+        upload(d, h.data(), sizeof(double2) * L);
     }
     twiddles[key] = d;
     return d;
@@ -46,7 +54,7 @@ void* Context::table(const std::string& key, const void* host, size_t bytes) {
     if (it != tables.end()) return it->second;
     void* d = nullptr;
     TG_HIP(hipMalloc(&d, bytes));
-    TG_HIP(hipMemcpy(d, host, bytes, hipMemcpyHostToDevice));
+    upload(d, host, bytes);
     tables[key] = d;
     return d;
 }

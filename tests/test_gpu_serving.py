@@ -98,7 +98,7 @@ def test_serving_mix_equals_alone(gpu):
     dev = gpu.torch_device
     reqs = requests(dev)
     mix, recs = run_server(dev, reqs, lanes=3, budget=8 << 30)
-    assert all(r.status == 2 for r in recs)
+    assert all(r.status == 2 for r in recs), [(r.status, r.error) for r in recs]
     # more than one request was resident at once (the work overlapped)
     spans = sorted((r.admitted, r.done) for r in recs)
     assert any(spans[i + 1][0] < spans[i][1] for i in range(len(spans) - 1))
@@ -122,7 +122,7 @@ def test_serving_budget_serialises_and_rejects(gpu):
     srv.close()
     # the largest request cannot share the GPU with any other one
     mix, recs = run_server(dev, reqs, lanes=2, budget=max(fps) + min(fps) - 1)
-    assert all(r.status == 2 for r in recs)
+    assert all(r.status == 2 for r in recs), [(r.status, r.error) for r in recs]
     order = sorted(recs, key=lambda r: r.admitted)
     used = []
     for r in order:  # at every admission the resident footprints fit the budget
