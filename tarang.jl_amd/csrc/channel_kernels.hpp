@@ -356,19 +356,28 @@ __global__ __launch_bounds__(NT) void k_ch_xpass(const cplx<T>* __restrict__ G, 
 // k_ch_zfwd: z-forward Chebyshev transform of the product planes on the matrix cores
 // (grid (ceil(nmodes/QT), planes, k splits), XCD-remapped) into mode-major F arrays:
 //   F[m][k] = sum_z Fm[k][z] H[z][m],   FT[z*N + k] = Fm[k][z]
-// The plane tile H[z][q0 .. q0+QT) sits in LDS as columns in groups of 16 modes (Re
-// block, Im block); 16 x 16 tiles of (k, column).  Split s of the workgroup's k tiles
-// [s*kps, (s+1)*kps): wave w takes tiles KPW w .. KPW w + KPW - 1 of it, then the next pass.
-// A lane holds Re and Im of one mode for four k and stores them directly.  The
-// kx = Nx/2 column takes the x Nyquist fold 2 Re(H) (nx odd switches the fold off:
-// the 3-D path folds in its y pass).
+// Parity split: Fm[k][N-1-z] = (-1)^k Fm[k][z] on the symmetric Gauss-Lobatto grid, so
+//   F[m][k even] = sum_{z < N/2} Fm[k][z] (H[z] + H[N-1-z]) (+ Fm[k][mid] H[mid], N odd)
+//   F[m][k odd]  = sum_{z < N/2} Fm[k][z] (H[z] - H[N-1-z])
+// half the products.  The sums HS and the differences HA of the plane tile sit in LDS as
+// columns in groups of 16 modes (Re block, Im block), first-half rows only.  Output tiles
+// hold 16 k of one parity (k = 2 (16 t + r) + parity; even tiles first, then odd); split s
+// of the workgroup takes tiles [s*kps, (s+1)*kps), wave w tiles KPW w .. KPW w + KPW - 1 of
+// it, then the next pass.  A lane holds Re and Im of one mode for four k and stores them
+// directly.  The kx = Nx/2 column takes the x Nyquist fold 2 Re(H) (nx odd switches the
+// fold off: the 3-D path folds in its y pass).
 // ---------------------------------------------------------------------------
 // z steps per load batch (U), threads per workgroup (NT) and k tiles per wavefront and
 // pass (KPW), per arithmetic type (modes per workgroup QT: 16 or 32)
 template <typename T> struct ZfwdTile { static constexpr int U = 2, NT = 512, KPW = 1; };
 template <> struct ZfwdTile<double> { static constexpr int U = 4, NT = 256, KPW = 2; };
 
-__host__ __device__ inline size_t ch_zfwd_lds_elems(int N, int qt) { return (size_t)((N + 15) & ~15) * (2 * qt + 1); }
+// LDS elements of k_ch_zfwd: the sum and difference blocks of the first-half rows
+__host__ __device__ inline size_t ch_zfwd_lds_elems(int N, int qt) {
+    return (size_t)2 * (((N + 1) / 2 + 15) & ~15) * (2 * qt + 1);
+}
+// output k tiles of k_ch_zfwd: even-k tiles, then odd-k tiles
+__host__ __device__ inline int zfwd_k_tiles(int N) { return ((N + 1) / 2 + 15) / 16 + (N / 2 + 15) / 16; }
 
 template <typename T, int QT, int NT>
 __global__ __launch_bounds__(NT, 768 / NT) void k_ch_zfwd(const cplx<T>* __restrict__ H, int nmodes, int nx, int N,
@@ -382,8 +391,10 @@ __global__ __launch_bounds__(NT, 768 / NT) void k_ch_zfwd(const cplx<T>* __restr
     constexpr int NG = QT / 16, CT = 2 * NG, S = 2 * QT + 1;
     constexpr int KPW = ZfwdTile<T>::KPW, NWV = NT / 64;
     extern __shared__ unsigned char smem_raw[];
-    T* C = reinterpret_cast<T*>(smem_raw);  // [KP][S]
-    const int KP = (N + 15) & ~15, nkt = KP >> 4, zsteps = (N + 3) >> 2;
+    const int NH = (N + 1) / 2, NHP = (NH + 15) & ~15, zsteps = (NH + 3) >> 2;
+    T* CS = reinterpret_cast<T*>(smem_raw);  // [NHP][S] sums (even k)
+    T* CA = CS + (size_t)NHP * S;            // [NHP][S] differences (odd k)
+    const int nke = (NH + 15) / 16, nkt = zfwd_k_tiles(N);  // even tiles, all tiles
     const unsigned gxy = gridDim.x * gridDim.y;
     const unsigned lin = xcd_remap(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), gxy * gridDim.z);
     const int f = (int)((lin / gridDim.x) % gridDim.y), q0 = (int)(lin % gridDim.x) * QT;
@@ -391,20 +402,37 @@ __global__ __launch_bounds__(NT, 768 / NT) void k_ch_zfwd(const cplx<T>* __restr
     const int ke = kb + kps < nkt ? kb + kps : nkt;
     const V* in = H + (size_t)f * N * nmodes;
     const bool xeven = (nx % 2) == 0;
-    for (int idx = threadIdx.x; idx < QT * KP; idx += NT) {
-        const int z = idx / QT, q = idx - z * QT;
-        const int m = q0 + q;
-        V v{T(0), T(0)};
-        if (z < N && m < nmodes) v = in[(size_t)z * nmodes + m];
-        if (xeven && m == nx / 2) v = V{T(2) * v.x, T(0)};
+    for (int idx = threadIdx.x; idx < QT * NHP; idx += NT) {
+        const int zz = idx / QT, q = idx - zz * QT;
+        const int m = q0 + q, zm = N - 1 - zz;
+        V hz{T(0), T(0)}, hm{T(0), T(0)};
+        if (zz < NH && m < nmodes) {
+            hz = in[(size_t)zz * nmodes + m];
+            if (zm != zz) hm = in[(size_t)zm * nmodes + m];
+        }
+        if (xeven && m == nx / 2) {
+            hz = V{T(2) * hz.x, T(0)};
+            hm = V{T(2) * hm.x, T(0)};
+        }
         const int col = 32 * (q >> 4) + (q & 15);
-        C[z * S + col] = v.x;
-        C[z * S + col + 16] = v.y;
+        const bool mid = zm == zz;  // the middle point (N odd) counts once, in the even sums
+        CS[zz * S + col] = hz.x + hm.x;
+        CS[zz * S + col + 16] = hz.y + hm.y;
+        CA[zz * S + col] = mid ? T(0) : hz.x - hm.x;
+        CA[zz * S + col + 16] = mid ? T(0) : hz.y - hm.y;
     }
     __syncthreads();
-    const int lane = threadIdx.x & 63, r16 = lane & 15, q4 = lane >> 4, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, r16 = lane & 15, q4 = lane >> 4;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     V* out = f == 0 ? F0 : (f == 1 ? F1 : (f == 2 ? F2 : F3));
     for (int kt0 = kb + KPW * wave; kt0 < ke; kt0 += KPW * NWV) {
+        int par[KPW], kk[KPW];  // parity of each tile and this lane's k as the A operand's row
+#pragma unroll
+        for (int a = 0; a < KPW; ++a) {
+            const int t = kt0 + a;
+            par[a] = t < nke ? 0 : 1;
+            kk[a] = 2 * (16 * (par[a] ? t - nke : t) + r16) + par[a];
+        }
         acc4 acc[KPW][CT];  // [k tile][column tile]
 #pragma unroll
         for (int a = 0; a < KPW; ++a)
@@ -412,39 +440,45 @@ __global__ __launch_bounds__(NT, 768 / NT) void k_ch_zfwd(const cplx<T>* __restr
             for (int ct = 0; ct < CT; ++ct) acc[a][ct] = acc4{T(0), T(0), T(0), T(0)};
         constexpr int U = ZfwdTile<T>::U;
         for (int zs = 0; zs < zsteps; zs += U) {
-            T am[U][KPW], bv[U][CT];
+            T am[U][KPW], bs[U][CT], ba[U][CT];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const int z = 4 * (zs + u) + q4;
+                const int zz = 4 * (zs + u) + q4;
 #pragma unroll
                 for (int a = 0; a < KPW; ++a) {
-                    const int k = 16 * (kt0 + a) + r16;
-                    const bool in = z < N && k < N;
-                    am[u][a] = FT[in ? z * N + k : 0];
-                    am[u][a] = in ? am[u][a] : T(0);
+                    const bool ok = zz < NH && kk[a] < N && kt0 + a < ke;
+                    am[u][a] = FT[ok ? zz * N + kk[a] : 0];
+                    am[u][a] = ok ? am[u][a] : T(0);
                 }
-                const int zr = z < KP ? z : KP - 1;  // rows N .. KP - 1 are zero; beyond, a is zero
+                const int zr = zz < NHP ? zz : NHP - 1;  // rows NH .. NHP - 1 are zero; beyond, a is zero
 #pragma unroll
-                for (int ct = 0; ct < CT; ++ct) bv[u][ct] = C[zr * S + 16 * ct + r16];
+                for (int ct = 0; ct < CT; ++ct) {
+                    bs[u][ct] = CS[zr * S + 16 * ct + r16];
+                    ba[u][ct] = CA[zr * S + 16 * ct + r16];
+                }
             }
 #pragma unroll
             for (int u = 0; u < U; ++u)
 #pragma unroll
                 for (int a = 0; a < KPW; ++a)
 #pragma unroll
-                    for (int ct = 0; ct < CT; ++ct) acc[a][ct] = Op::mma(am[u][a], bv[u][ct], acc[a][ct]);
+                    for (int ct = 0; ct < CT; ++ct)
+                        acc[a][ct] = Op::mma(am[u][a], par[a] ? ba[u][ct] : bs[u][ct], acc[a][ct]);
         }
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
             const int m = q0 + 16 * g + r16;
             if (m >= nmodes) continue;
 #pragma unroll
-            for (int a = 0; a < KPW; ++a)
+            for (int a = 0; a < KPW; ++a) {
+                if (kt0 + a >= ke) continue;
+                const int t = kt0 + a, base = 16 * (par[a] ? t - nke : t);
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    const int k = 16 * (kt0 + a) + Op::row(v, q4);
-                    if (k < N && kt0 + a < ke) out[(size_t)m * N + k] = V{acc[a][2 * g][v], acc[a][2 * g + 1][v]};
+                    const int k = 2 * (base + Op::row(v, q4)) + par[a];
+                    if (k < N) out[(size_t)m * N + k] = V{acc[a][2 * g][v], acc[a][2 * g + 1][v]};
                 }
+            }
         }
     }
 }

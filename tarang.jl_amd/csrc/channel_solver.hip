@@ -300,9 +300,9 @@ void ChannelSolver::rhs_t(const void* b, const void* ux, const void* uz, void* F
     }, Sizes2D{});
     TG_HIP(hipGetLastError());
     {
-        // few modes: 16-mode tiles, and the k range split over workgroups (8 k tiles each)
+        // few modes: 16-mode tiles, and the k tiles split over workgroups (8 each)
         constexpr int QT = 16;
-        const unsigned ksplit = (unsigned)std::max(1, ((N + 15) / 16) / 8);
+        const unsigned ksplit = (unsigned)std::max(1, zfwd_k_tiles(N) / 8);
         const size_t flds = ch_zfwd_lds_elems(N, QT) * sizeof(T);
         timed(2, [&] {
             hipLaunchKernelGGL((k_ch_zfwd<T, QT, ZfwdTile<T>::NT>), dim3((unsigned)((nxh_ + QT - 1) / QT), 3, ksplit),

@@ -67,9 +67,13 @@ class InitialValueSolver:
 
     Keyword arguments mirror the reference (`dt`, `matsolver`, `batched_modes`,
     ...); `use_graph` (default True) replays each step as one captured hipGraph.
+    Channel families only: `split_inverse` (default: on for an FP32 domain) stores the
+    per-mode inverses as FP32 values plus bfloat16 remainders, and `refine_sweeps` (0-2)
+    adds FP64-residual refinement sweeps to every implicit solve.
     """
 
-    def __init__(self, problem: IVP, timestepper, dt: float, matsolver=None, use_graph: bool = True, **_):
+    def __init__(self, problem: IVP, timestepper, dt: float, matsolver=None, use_graph: bool = True,
+                 split_inverse=None, refine_sweeps: int = 0, **_):
         if isinstance(timestepper, type):
             timestepper = timestepper()
         self.problem = problem
@@ -84,7 +88,7 @@ class InitialValueSolver:
         rec = recognize(problem)
         self.recognized = rec
         if rec.family in ("rbc2d", "rbc3d"):
-            self._channel = _ChannelBackend(self, problem, rec, timestepper, use_graph)
+            self._channel = _ChannelBackend(self, problem, rec, timestepper, use_graph, split_inverse, refine_sweeps)
             self.dev = self._channel.dev
             self.handle = self._channel.handle
             self.field = self._channel.b
@@ -230,7 +234,7 @@ class _ChannelBackend:
     factored per-mode systems.  Fields are pushed before and pulled after every batch
     of steps (scatter_inputs / gather_inputs!, subproblem_io.jl:1143-1152)."""
 
-    def __init__(self, solver, problem, rec, timestepper, use_graph):
+    def __init__(self, solver, problem, rec, timestepper, use_graph, split_inverse=None, refine_sweeps=0):
         names = rec.names
         self.problem = problem
         self.b = problem.var(names["b"])
@@ -273,6 +277,11 @@ class _ChannelBackend:
         d.dt = solver.dt
         d.dtype = _lib.F32 if dom.dtype == np.float32 else _lib.F64
         d.timestepper = timestepper.code
+        # the FP32 state's default solve form: inverses with bfloat16 remainders (the library
+        # refuses split inverses for an FP64 state, so an explicit request there fails loudly)
+        split = (dom.dtype == np.float32) if split_inverse is None else bool(split_inverse)
+        d.split_inverse = 1 if split else 0
+        d.refine_sweeps = int(refine_sweeps)
         walls = (rec.extra["b_bottom"], rec.extra["b_top"])
         # array wall values (space-dependent BC) are handed over as grid arrays before every
         # batch of steps, so a change between steps is seen (the reference re-gathers its

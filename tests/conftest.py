@@ -52,3 +52,12 @@ def relerr(a, b):
     b = np.asarray(b)
     den = max(np.abs(b).max(), 1e-300)
     return float(np.abs(a - b).max() / den)
+
+
+def staterr(pairs):
+    """Largest error over several fields of one state, relative to the state's largest
+    entry: the measure for fields far smaller than the others (a velocity 1e-5 of the
+    buoyancy), whose own-scale error is set by the rounding of the large fields."""
+    num = max(float(np.abs(np.asarray(a) - np.asarray(b)).max()) for a, b in pairs)
+    den = max(float(np.abs(np.asarray(b)).max()) for _, b in pairs)
+    return num / max(den, 1e-300)

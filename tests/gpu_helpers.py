@@ -70,9 +70,11 @@ class Device:
 
 
     def chan(self, nx, nz, Lx, Lz, kappa, nu, dt, stepper=0, dtype=np.float64, b_bottom=None, b_top=0.0, refine=0,
-             split=0):
+             split=None):
+        """split=None: the library's recommended form (split inverses for an FP32 state)."""
         d = _lib.ChanDesc()
-        d.refine_sweeps, d.split_inverse = refine, split
+        d.refine_sweeps = refine
+        d.split_inverse = (1 if dtype == np.float32 else 0) if split is None else split
         d.nx, d.nz, d.Lx, d.Lz = nx, nz, Lx, Lz
         d.kappa, d.nu, d.dt = kappa, nu, dt
         d.dtype = _lib.F32 if dtype == np.float32 else _lib.F64
@@ -85,9 +87,10 @@ class Device:
 
 
     def chan3(self, nx, ny, nz, Lx, Ly, Lz, kappa, nu, dt, stepper=2, dtype=np.float64, b_bottom=None, b_top=0.0,
-              refine=0, split=0):
+              refine=0, split=None):
         d = _lib.Chan3Desc()
-        d.refine_sweeps, d.split_inverse = refine, split
+        d.refine_sweeps = refine
+        d.split_inverse = (1 if dtype == np.float32 else 0) if split is None else split
         d.nx, d.ny, d.nz, d.Lx, d.Ly, d.Lz = nx, ny, nz, Lx, Ly, Lz
         d.kappa, d.nu, d.dt = kappa, nu, dt
         d.dtype = _lib.F32 if dtype == np.float32 else _lib.F64

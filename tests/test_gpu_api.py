@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import tarang_amd as T
-from tests.conftest import relerr
+from tests.conftest import relerr, staterr
 from tests.test_gpu_fourier_ivp import turb_ic
 
 pytestmark = pytest.mark.gpu
@@ -60,13 +60,13 @@ def test_turbulence_example_through_api(oracle, stepper):
     ref = oracle.FourierIVP(d, oracle.PROB_TURB2D, ts, nu, dt)
     ref.set_coeff(oracle.forward_transform(d, q0))
     ref.step(10)
-    assert relerr(q["c"].cpu().numpy(), ref.get_coeff()) < 1e-11
+    assert relerr(q["c"].cpu().numpy(), ref.get_coeff()) < 1e-12
     # velocity refreshed from q: energy from u matches the oracle's diagnostic
     ux = u.components[0]["g"].cpu().numpy()
     uy = u.components[1]["g"].cpu().numpy()
     E = 0.5 * np.mean(ux ** 2 + uy ** 2)
     E_ref, _ = ref.diagnostics()
-    assert abs(E - E_ref) / E_ref < 1e-11
+    assert abs(E - E_ref) / E_ref < 1e-12
 
 
 def test_coefficient_edits_between_steps_are_seen(oracle):
@@ -177,10 +177,14 @@ def test_rbc_example_through_api(oracle, dtype):
     d = oracle.domain([oracle.REALFOURIER, oracle.CHEBYSHEVT], [Nx, Nz], bounds=[(0.0, Lx), (0.0, Lz)])
     ref.set(oracle.RBC_B, oracle.forward_transform(d, bg.astype(dtype).astype(np.float64)))
     ref.step(5)
-    tol = 1e-12 if dtype == np.float64 else 1e-4
-    assert relerr(b["c"].cpu().numpy(), ref.get(oracle.RBC_B)) < tol
-    assert relerr(u[0]["c"].cpu().numpy(), ref.get(oracle.RBC_UX)) < tol * 10
-    assert relerr(u[1]["c"].cpu().numpy(), ref.get(oracle.RBC_UZ)) < tol * 10
+    tol = 1e-12 if dtype == np.float64 else 1e-5
+    pairs = [(b["c"].cpu().numpy(), ref.get(oracle.RBC_B)), (u[0]["c"].cpu().numpy(), ref.get(oracle.RBC_UX)),
+             (u[1]["c"].cpu().numpy(), ref.get(oracle.RBC_UZ))]
+    errs = [relerr(a, r) for a, r in pairs]
+    print(f"  rbc through api {dtype.__name__}: per-field {errs}, state {staterr(pairs):.3e}", flush=True)
+    assert errs[0] < tol
+    # the velocity is ~1e-5 of b after 5 steps: judged on the state's scale
+    assert staterr(pairs) < tol
     for f in (b, u[0], u[1]):
         assert np.all(np.isfinite(f["g"].cpu().numpy()))
     solver.close()
@@ -221,9 +225,14 @@ def test_rbc3d_through_api(oracle):
     ref = oracle.RBC3D(Nx, Ny, Nz, Lx, Ly, Lz, kappa, nu, 2e-3, "SBDF2")
     ref.set(oracle.RBC3_B, oracle.forward_transform(ref.d, bg))
     ref.step(4)
-    assert relerr(b["c"].cpu().numpy(), ref.get(oracle.RBC3_B)) < 1e-12
-    for i, v in enumerate((oracle.RBC3_UX, oracle.RBC3_UY, oracle.RBC3_UZ)):
-        assert relerr(u[i]["c"].cpu().numpy(), ref.get(v)) < 1e-11
+    pairs = [(b["c"].cpu().numpy(), ref.get(oracle.RBC3_B))] + [
+        (u[i]["c"].cpu().numpy(), ref.get(v)) for i, v in enumerate((oracle.RBC3_UX, oracle.RBC3_UY, oracle.RBC3_UZ))]
+    errs = [relerr(a, r) for a, r in pairs]
+    print(f"  rbc3d through api: per-field {errs}, state {staterr(pairs):.3e}", flush=True)
+    assert errs[0] < 1e-12
+    # the velocity is ~1e-6 of b after 4 steps: judged on the state's scale (per field it
+    # is within 1e-11)
+    assert staterr(pairs) < 1e-12 and max(errs[1:]) < 1e-11
     # lift taus are the ill-conditioned components of the per-mode tau systems (condition
     # numbers up to ~3e7): the GPU applies explicit inverses, the oracle LU solves, and the
     # two roundings differ there by ~cond x eps relative to the taus' own scale
@@ -247,5 +256,5 @@ def test_rbc_sbdf2_in_batches_keeps_history(oracle):
     ref.set(oracle.RBC_B, oracle.forward_transform(d, bg))
     ref.step(6)
     assert relerr(b["c"].cpu().numpy(), ref.get(oracle.RBC_B)) < 1e-12
-    assert relerr(u[1]["c"].cpu().numpy(), ref.get(oracle.RBC_UZ)) < 1e-11
+    assert relerr(u[1]["c"].cpu().numpy(), ref.get(oracle.RBC_UZ)) < 1e-12
     solver.close()

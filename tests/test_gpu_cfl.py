@@ -145,8 +145,8 @@ def test_rbc_adaptive_dt_matches_oracle(gpu, oracle):
         ref.step(1)
     assert relerr(np.array(dts), np.array(ref_dts)) < 1e-10
     assert relerr(b["c"].cpu().numpy(), ref.get(oracle.RBC_B)) < 1e-12
-    assert relerr(u[0]["c"].cpu().numpy(), ref.get(oracle.RBC_UX)) < 1e-11
-    assert relerr(u[1]["c"].cpu().numpy(), ref.get(oracle.RBC_UZ)) < 1e-11
+    assert relerr(u[0]["c"].cpu().numpy(), ref.get(oracle.RBC_UX)) < 1e-12
+    assert relerr(u[1]["c"].cpu().numpy(), ref.get(oracle.RBC_UZ)) < 1e-12
     solver.close()
 
 
@@ -184,8 +184,8 @@ def test_rbc_sbdf2_adaptive_dt_matches_oracle(gpu, oracle):
         ref.set_dt(dt)
         ref.step(1)
     assert relerr(b["c"].cpu().numpy(), ref.get(oracle.RBC_B)) < 1e-12
-    assert relerr(u[0]["c"].cpu().numpy(), ref.get(oracle.RBC_UX)) < 1e-11
-    assert relerr(u[1]["c"].cpu().numpy(), ref.get(oracle.RBC_UZ)) < 1e-11
+    assert relerr(u[0]["c"].cpu().numpy(), ref.get(oracle.RBC_UX)) < 1e-12
+    assert relerr(u[1]["c"].cpu().numpy(), ref.get(oracle.RBC_UZ)) < 1e-12
     solver.close()
 
 
@@ -250,5 +250,5 @@ def test_rbc3d_adaptive_dt_matches_oracle(gpu, oracle):
     refs = [ref.get(v) for v in (oracle.RBC3_UX, oracle.RBC3_UY, oracle.RBC3_UZ)]
     scale = max(np.abs(r).max() for r in refs)  # uy stays ~0 for this y-independent flow
     for i, r in enumerate(refs):
-        assert np.abs(u[i]["c"].cpu().numpy() - r).max() < 1e-11 * scale
+        assert np.abs(u[i]["c"].cpu().numpy() - r).max() < 1e-12 * scale
     solver.close()

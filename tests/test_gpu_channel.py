@@ -68,7 +68,7 @@ def test_channel_rhs_parity(gpu, oracle, dtype, nx, N):
     for var, c in st.items():
         s.set(var, c)
     Fg = s.rhs()
-    tol = TOL[dtype] * (10 if dtype == np.float32 else 1)
+    tol = TOL[dtype]
     errs = [relerr(a, b) for a, b in zip(Fg, Fr)]
     print(f"  rel errs {errs}", flush=True)
     assert max(errs) < tol
@@ -87,7 +87,7 @@ def test_channel_steps_parity(gpu, oracle, ts, dtype):
     for var, c in st.items():
         ref.set(var, c)
         s.set(var, c)
-    tol = TOL[dtype] * (10 if dtype == np.float32 else 1)
+    tol = TOL[dtype]
     for nsteps in (1, 5):
         ref.step(nsteps)
         s.step(nsteps)
@@ -118,7 +118,10 @@ def test_channel_large_nz_parity(gpu, oracle, ts, dtype):
     s.step(2)
     # at nz = 450 the u-p systems have condition numbers ~1e10: an FP64 LU solve and an
     # FP64 explicit inverse each differ from the exact solution by up to ~3e-12 relative
-    # (checked against iteratively refined solves), so the two paths agree to ~1e-11
+    # (checked against iteratively refined solves), so the two paths agree to ~1e-11.
+    # FP32 at this size: 1e-4 (measured 6e-5 with split inverses): the FP32 Chebyshev sums
+    # over 450 terms and the growth of the z derivatives with Nz put the FP32-state floor
+    # above 1e-5 here; the workload sizes (Nz <= 256) meet 1e-5
     tol = {np.float64: 1e-10, np.float32: 1e-4}[dtype]
     for var in (oracle.RBC_B, oracle.RBC_UX, oracle.RBC_UZ):
         assert relerr(s.get(var), ref.get(var)) < tol, var
@@ -141,7 +144,7 @@ def test_channel_benchmark_x_size_steps(gpu, oracle, dtype):
         s.set(var, c)
     ref.step(2)
     s.step(2)
-    tol = TOL[dtype] * (10 if dtype == np.float32 else 1)
+    tol = TOL[dtype]
     for var in (oracle.RBC_B, oracle.RBC_UX, oracle.RBC_UZ):
         assert relerr(s.get(var), ref.get(var)) < tol, var
     s.close()
@@ -235,7 +238,7 @@ def test_channel_full_size_modewise(gpu, oracle, dtype):
         s.set(var, big)
     ref.step(3)
     s.step(3)
-    tol = {np.float64: 1e-11, np.float32: 1e-4}[dtype]
+    tol = {np.float64: 1e-12, np.float32: 1e-5}[dtype]
     b0 = np.abs(ref.get(oracle.RBC_B)[0]).max()  # the O(1) conduction profile carried by kx = 0
     worst = 0.0
     for var in (oracle.RBC_B, oracle.RBC_UX, oracle.RBC_UZ):
@@ -285,7 +288,7 @@ def test_channel_sbdf_parity(gpu, oracle, ts, dtype):
     for var, c in st.items():
         ref.set(var, c)
         s.set(var, c)
-    tol = TOL[dtype] * (10 if dtype == np.float32 else 1)
+    tol = TOL[dtype]
     for nsteps, dt in ((1, 5e-3), (3, 5e-3), (2, 2.5e-3), (2, 4e-3)):
         ref.set_dt(dt)
         s.set_dt(dt)

@@ -1,7 +1,7 @@
 """3-D Rayleigh-Benard channel on the GPU (tarang_chan3_*) vs the FP64 oracle.
 
 Tolerances: FP64 1e-12 relative (max-norm against the largest oracle entry) for the
-right-hand side and steps, FP32 1e-4 (1e-5 x10 for Chebyshev sums in FP32).  At the
+right-hand side and steps, FP32 1e-5.  At the
 256 x 256 x 128 workload the check is mode-wise against a small oracle grid on a
 small-amplitude state confined to modes kx, ky in {0, 1}, whose products are alias-free
 on both grids.
@@ -14,7 +14,7 @@ from tests.conftest import relerr
 
 pytestmark = pytest.mark.gpu
 
-TOL = {np.float64: 1e-12, np.float32: 1e-4}
+TOL = {np.float64: 1e-12, np.float32: 1e-5}
 
 
 def zgrid(N, Lz):
@@ -143,7 +143,10 @@ def test_channel3d_multichunk_keys(gpu, oracle, dtype):
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_channel3d_full_size_modewise(gpu, oracle, dtype):
-    """256 x 256 x 128 (the 3-D workload) vs a 16 x 16 x 128 oracle on modes kx, ky in {0, 1}."""
+    """256 x 256 x 128 (the 3-D workload) vs a 16 x 16 x 128 oracle on modes kx, ky in {0, 1}.
+    Errors are relative to each mode's own size or the perturbation amplitude (~3e-5 of the
+    O(1) conduction profile), whichever is larger: FP64 1e-11 there, since the profile's own
+    rounding (1e-16 absolute) is already 3e-12 of the perturbation; FP32 1e-5."""
     nx = ny = 256
     N, Lx, Ly, Lz, dt = 128, 4.0, 4.0, 1.0, 2e-3
     print(f"channel3d full size {dtype.__name__}", flush=True)
@@ -180,14 +183,18 @@ def test_channel3d_full_size_modewise(gpu, oracle, dtype):
         s.set(v, big)
     ref.step(3)
     s.step(3)
-    tol = {np.float64: 1e-10, np.float32: 1e-4}[dtype]
+    tol = {np.float64: 1e-11, np.float32: 1e-5}[dtype]
+    worst = 0.0
     for v in (oracle.RBC3_B, oracle.RBC3_UX, oracle.RBC3_UY, oracle.RBC3_UZ):
         g = s.get(v) / scale
         r = ref.get(v)
         pert = max(np.abs(r[1, 1]).max(), np.abs(r[1, 0]).max())
         for (m, n, ns) in ((0, 0, 0), (1, 0, 0), (1, 1, 1), (1, ny - 1, small - 1)):
             den = max(np.abs(r[m, ns]).max(), pert)
-            assert np.abs(g[m, n] - r[m, ns]).max() / den < tol, (v, m, n)
+            err = np.abs(g[m, n] - r[m, ns]).max() / den
+            worst = max(worst, err)
+            assert err < tol, (v, m, n, err)
+    print(f"  worst mode-wise rel err {worst:.3e}", flush=True)
     s.close()
 
 

@@ -62,7 +62,7 @@ def test_channel_restart_equivalence(tmp_path):
     assert solver2.iteration == 4 and solver2.sim_time == pytest.approx(0.04) and solver2.dt == 0.01
     T.run_(solver2, stop_iteration=7)
     assert relerr(b2["c"].cpu().numpy(), A[0]) < 1e-12
-    assert relerr(u2[1]["c"].cpu().numpy(), A[1]) < 1e-11
+    assert relerr(u2[1]["c"].cpu().numpy(), A[1]) < 1e-12
     solver2.close()
 
 

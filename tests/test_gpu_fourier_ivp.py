@@ -29,7 +29,7 @@ def test_turbulence_rhs_parity(gpu, oracle, dtype, N):
     F = s.rhs(C)
     e = relerr(F, ref)
     print(f"  rel err {e:.3e}", flush=True)
-    assert e < TOL[dtype] * (10 if dtype == np.float32 else 1)
+    assert e < TOL[dtype]
     s.close()
 
 
@@ -52,7 +52,7 @@ def test_burgers_rhs_parity(gpu, oracle, dtype, N):
     s = gpu.fivp([N], [2 * np.pi], BURG, TS["RK222"], 1e-2, 1e-3, dtype=dtype)
     e = relerr(s.rhs(C), ref)
     print(f"burgers rhs N={N} {dtype.__name__} rel err {e:.3e}", flush=True)
-    assert e < TOL[dtype] * (10 if dtype == np.float32 else 1)
+    assert e < TOL[dtype]
     s.close()
 
 
@@ -165,7 +165,7 @@ def test_burgers_trajectory(gpu, oracle, dtype):
     s = gpu.fivp([N], [2 * np.pi], BURG, TS["RK222"], nu, dt, dtype=dtype)
     s.set_state(C)
     s.step(200)
-    assert relerr(s.get_state(), ref.get_coeff()) < (1e-11 if dtype == np.float64 else 1e-4)
+    assert relerr(s.get_state(), ref.get_coeff()) < (1e-12 if dtype == np.float64 else 1e-5)
     s.close()
 
 

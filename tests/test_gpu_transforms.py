@@ -65,7 +65,7 @@ def test_padded_product_parity(gpu, oracle, golden, dtype, sizes):
     out = p.multiply(a, b)
     e = relerr(out, ref)
     print(f"  rel err {e:.3e}", flush=True)
-    assert e < TOL[dtype] * (10 if dtype == np.float32 else 1)
+    assert e < TOL[dtype]
     p.close()
 
 
