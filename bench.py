@@ -809,6 +809,49 @@ def bench_plugins(a):
         del src, dst
     torch.cuda.empty_cache()
 
+    # ---- transform plug-in: RealFourier^2 x ChebyshevT 256 x 256 x 128, forward and backward ----
+    for tdt, ct, code, esz in ((torch.float64, torch.complex128, _lib.F64, 8),
+                               (torch.float32, torch.complex64, _lib.F32, 4)):
+        d = _lib.DomainDesc()
+        d.ndim = 3
+        for i, (kind, n_, hi) in enumerate(((_lib.REALFOURIER, nx, 4.0), (_lib.REALFOURIER, ny, 4.0),
+                                            (_lib.CHEBYSHEVT, nz, 1.0))):
+            d.kind[i], d.n[i], d.lo[i], d.hi[i], d.dealias[i] = kind, n_, 0.0, hi, 1.5
+        d.dtype = code
+        plan = ctypes.c_void_p()
+        _lib.call("tarang_plan_create", ctx, ctypes.byref(d), ctypes.byref(plan))
+        grid = torch.rand((nz, ny, nx), dtype=torch.float64, device=dev, generator=g).to(tdt)
+        coeff = torch.empty((nz, ny, nx // 2 + 1), dtype=ct, device=dev)
+        back = torch.empty_like(grid)
+        fwd = lambda: _lib.call("tarang_forward_transform", plan, ctypes.c_void_p(grid.data_ptr()),  # noqa: E731
+                                ctypes.c_void_p(coeff.data_ptr()))
+        bwd = lambda: _lib.call("tarang_backward_transform", plan, ctypes.c_void_p(coeff.data_ptr()),  # noqa: E731
+                                ctypes.c_void_p(back.data_ptr()))
+        for _ in range(max(a.warmup, 3)):
+            fwd()
+            bwd()
+        torch.cuda.synchronize()
+        rt = float(((back - grid).abs().max() / grid.abs().max()).item())
+        for name, fn in (("forward", fwd), ("backward", bwd)):
+            ms = timed(fn, reps)
+            # algorithmic bytes: the grid array once and the coefficient array once
+            nbytes = nx * ny * nz * esz + (nx // 2 + 1) * ny * nz * 2 * esz
+            gbs = nbytes / (ms * 1e-3) / 1e9
+            print(json.dumps({
+                "metric": "spectral transform throughput", "value": nx * ny * nz / (ms * 1e-3),
+                "unit": "grid points/s", "n_gpus": 1, "steps": reps, "warmup": max(a.warmup, 3), "ms_per_step": ms,
+                "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                "dtype": "f64" if code == _lib.F64 else "f32", "data": "synthetic uniform(0,1)",
+                "config": {"workload": f"tarang_{name}_transform, RealFourier^2 x ChebyshevT {nx}x{ny}x{nz} "
+                                       f"(rfft x, FFT y, Chebyshev z on the matrix cores)"},
+                "round_trip_rel_err": rt,
+                "roofline": {"bound": "hbm", "kernel": f"{name} chain (rows, y FFT, k_cheb_axis)", "achieved": gbs,
+                             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": None},
+                "cpu_baseline": None}), flush=True)
+        _lib.call("tarang_plan_destroy", plan)
+        del grid, coeff, back
+    torch.cuda.empty_cache()
+
     # ---- batched complex LU, 2-D channel stage matrices: 513 modes, n = 1031 ----
     n, nm = 1031, 513
     log(f"batched LU: {nm} modes of n={n} ({nm * n * n * 16 / 1e9:.2f} GB per copy)")

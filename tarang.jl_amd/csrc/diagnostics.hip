@@ -108,6 +108,32 @@ double* diag_scratch(Context* c) {
     static const std::vector<double> zeros(2 * kDiagBlocks + 2, 0.0);
     return static_cast<double*>(c->table("diag_partials", zeros.data(), zeros.size() * sizeof(double)));
 }
+
+// T_k at the Gauss-Lobatto points z_j = -cos(pi j / (N-1)) (T[k N + j]) and the
+// Clenshaw-Curtis weights, built once per context
+void nusselt_tables(Context* c, int N, const double** Td, const double** wd) {
+    const std::string key = "diag_cheb_" + std::to_string(N);
+    *Td = static_cast<const double*>(c->find_table(key));
+    *wd = static_cast<const double*>(c->find_table(key + "_w"));
+    if (*Td && *wd) return;
+    std::vector<double> T((size_t)N * N), wq(N);
+    for (int k = 0; k < N; ++k)
+        for (int j = 0; j < N; ++j)
+            T[(size_t)k * N + j] = ((k & 1) ? -1.0 : 1.0) * (double)cosl(3.14159265358979323846264338327950288L *
+                                                                            (long double)k * j / (N - 1));
+    const int M = N - 1;
+    for (int j = 0; j < N; ++j) {
+        long double s = 0.0L;
+        for (int k = 1; k <= M / 2; ++k) {
+            const long double bk = (2 * k == M) ? 1.0L : 2.0L;
+            s += bk * cosl(2.0L * k * 3.14159265358979323846264338327950288L * j / M) / (4.0L * k * k - 1.0L);
+        }
+        const long double cj = (j == 0 || j == M) ? 1.0L : 2.0L;
+        wq[j] = (double)(cj / M * (1.0L - s));
+    }
+    *Td = static_cast<const double*>(c->table(key, T.data(), T.size() * sizeof(double)));
+    *wd = static_cast<const double*>(c->table(key + "_w", wq.data(), wq.size() * sizeof(double)));
+}
 }  // namespace
 
 void spectral_energy(Context* c, const void* coeff, DType dt, int nx, int ny, double k0x, double k0y, bool vorticity,
@@ -140,27 +166,8 @@ double channel_nusselt(Context* c, const void* b, const void* uz, DType dt, int 
     TG_REQUIRE(b && uz && nx > 0 && ny > 0 && N > 1, kErrInvalid, "channel_nusselt: bad arguments");
     TG_REQUIRE(kappa > 0 && dB != 0, kErrInvalid, "channel_nusselt: kappa and the wall difference must be nonzero");
     c->activate();
-    // T_k at the Gauss-Lobatto points z_j = -cos(pi j / (N-1)) and the Clenshaw-Curtis weights
-    std::vector<double> T((size_t)N * N), wq(N);
-    for (int k = 0; k < N; ++k)
-        for (int j = 0; j < N; ++j)
-            T[(size_t)k * N + j] = ((k & 1) ? -1.0 : 1.0) * (double)cosl(3.14159265358979323846264338327950288L *
-                                                                            (long double)k * j / (N - 1));
-    {
-        const int M = N - 1;
-        for (int j = 0; j < N; ++j) {
-            long double s = 0.0L;
-            for (int k = 1; k <= M / 2; ++k) {
-                const long double bk = (2 * k == M) ? 1.0L : 2.0L;
-                s += bk * cosl(2.0L * k * 3.14159265358979323846264338327950288L * j / M) / (4.0L * k * k - 1.0L);
-            }
-            const long double cj = (j == 0 || j == M) ? 1.0L : 2.0L;
-            wq[j] = (double)(cj / M * (1.0L - s));
-        }
-    }
-    const std::string key = "diag_cheb_" + std::to_string(N);
-    const double* Td = static_cast<const double*>(c->table(key, T.data(), T.size() * sizeof(double)));
-    const double* wd = static_cast<const double*>(c->table(key + "_w", wq.data(), wq.size() * sizeof(double)));
+    const double *Td = nullptr, *wd = nullptr;
+    nusselt_tables(c, N, &Td, &wd);
     double* partial = diag_scratch(c);
     const int nxh = nx / 2 + 1;
     const size_t nmodes = (size_t)nxh * ny, n = nmodes * N;

@@ -59,6 +59,12 @@ void* Context::table(const std::string& key, const void* host, size_t bytes) {
     return d;
 }
 
+void* Context::find_table(const std::string& key) {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = tables.find(key);
+    return it == tables.end() ? nullptr : it->second;
+}
+
 void* Context::alloc(size_t bytes) {
     void* p = nullptr;
     if (bytes == 0) bytes = 256;

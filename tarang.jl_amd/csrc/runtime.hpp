@@ -22,6 +22,7 @@ struct Context {
     void activate() const { TG_HIP(hipSetDevice(device)); }
     const void* twiddle(int L, DType dt);
     void* table(const std::string& key, const void* host, size_t bytes);
+    void* find_table(const std::string& key);  // nullptr when the table was never built
     void upload(void* d, const void* host, size_t bytes);  // ordered on `stream`, then waited for
     void* alloc(size_t bytes);
     void free(void* p);

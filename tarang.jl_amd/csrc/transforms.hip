@@ -13,6 +13,7 @@
 
 #include "dispatch.hpp"
 #include "fourier_kernels.hpp"
+#include "mfma.hpp"
 #include "transforms.hpp"
 
 namespace tarang {
@@ -98,34 +99,62 @@ __global__ __launch_bounds__(NT) void k_c2c_axis(cplx<T>* __restrict__ a, size_t
     }
 }
 
-// out[c][k] = sum_j mat[k][j] in[c][j] along a strided axis of complex data (inner, n, outer)
-template <typename T, int TK, int NT>
-__global__ __launch_bounds__(NT) void k_cheb_axis(cplx<T>* __restrict__ a, size_t inner, int n,
-                                                  const T* __restrict__ mat) {
-    using V = cplx<T>;
+// Matrix along one axis on the matrix cores: out[k][c] = sum_j M[k][j] in[j][c] for each
+// outer index, the n x ncols real block of complex data (inner, n, outer) with its real
+// and imaginary parts as interleaved columns (ncols = 2 inner).  Workgroup = (tile of
+// 16 CT columns, outer index), every k of the tile: the tile's input is read whole into
+// LDS before any output is written, so out may equal in.  MT[j n + k] = M[k][j] (lanes
+// of one k-tile read consecutive entries); wave w takes k tiles w, w + NT/64, ...; the
+// loads of U reduction steps are issued before their products.
+template <typename T, int CT, int NT>
+__global__ __launch_bounds__(NT) void k_cheb_axis(const T* __restrict__ in, T* __restrict__ out, size_t ncols, int n,
+                                                  const T* __restrict__ MT) {
+    using Op = MfmaOp<T>;
+    using acc4 = typename Op::acc4;
+    constexpr int W = 16 * CT, S = W + 1, NWV = NT / 64, U = 4;
     extern __shared__ unsigned char smem_raw[];
-    V* buf = reinterpret_cast<V*>(smem_raw);
-    const size_t i0 = (size_t)blockIdx.x * TK;
-    const size_t base = (size_t)blockIdx.y * n * inner;
-    for (int idx = threadIdx.x; idx < TK * n; idx += NT) {
-        const int c = idx % TK, d = idx / TK;
-        const size_t i = i0 + c;
-        buf[c * n + d] = i < inner ? a[base + (size_t)d * inner + i] : V{T(0), T(0)};
+    T* C = reinterpret_cast<T*>(smem_raw);  // [KP][S]
+    const int KP = (n + 15) & ~15, nkt = KP >> 4, ksteps = (n + 3) >> 2;
+    const size_t c0 = (size_t)blockIdx.x * W;
+    const size_t base = (size_t)blockIdx.y * n * ncols;
+    for (int idx = threadIdx.x; idx < KP * W; idx += NT) {
+        const int j = idx / W, c = idx - j * W;
+        T v = T(0);
+        if (j < n && c0 + c < ncols) v = in[base + (size_t)j * ncols + c0 + c];
+        C[j * S + c] = v;
     }
     __syncthreads();
-    for (int idx = threadIdx.x; idx < TK * n; idx += NT) {
-        const int c = idx % TK, k = idx / TK;
-        const size_t i = i0 + c;
-        if (i >= inner) continue;
-        const T* row = mat + (size_t)k * n;
-        const V* col = buf + c * n;
-        T sr = T(0), si = T(0);
-        for (int j = 0; j < n; ++j) {
-            const T m = row[j];
-            sr += m * col[j].x;
-            si += m * col[j].y;
+    const int lane = threadIdx.x & 63, r16 = lane & 15, q4 = lane >> 4;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int kt = wave; kt < nkt; kt += NWV) {
+        const int k = 16 * kt + r16;
+        acc4 acc[CT];
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) acc[ct] = acc4{T(0), T(0), T(0), T(0)};
+        for (int s0 = 0; s0 < ksteps; s0 += U) {
+            T am[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int j = 4 * (s0 + u) + q4;
+                const bool ok = j < n && k < n;
+                am[u] = MT[ok ? (size_t)j * n + k : 0];
+                am[u] = ok ? am[u] : T(0);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int j = 4 * (s0 + u) + q4, jr = j < KP ? j : KP - 1;  // past n, a is zero
+#pragma unroll
+                for (int ct = 0; ct < CT; ++ct) acc[ct] = Op::mma(am[u], C[jr * S + 16 * ct + r16], acc[ct]);
+            }
         }
-        a[base + (size_t)k * inner + i] = V{sr, si};
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int kk = 16 * kt + Op::row(v, q4);
+                const size_t c = c0 + 16 * ct + r16;
+                if (kk < n && c < ncols) out[base + (size_t)kk * ncols + c] = acc[ct][v];
+            }
     }
 }
 
@@ -175,7 +204,8 @@ size_t TransformPlan::coeff_elems() const {
 
 const void* TransformPlan::cheb_matrix(int n, bool forward) {
     const bool f32 = d_.dtype == kF32;
-    const std::string key = std::string("cheb_") + (forward ? "fwd_" : "bwd_") + std::to_string(n) + (f32 ? "_f32" : "_f64");
+    const std::string key = std::string("chebT_") + (forward ? "fwd_" : "bwd_") + std::to_string(n) + (f32 ? "_f32" : "_f64");
+    if (void* t = ctx_->find_table(key)) return t;  // built once per context: the host work is O(n^2)
     std::vector<double> m((size_t)n * n);
     const long period = 2L * (n - 1);
     auto cospi = [&](long num) {  // cos(pi num / (n-1)) with exact range reduction
@@ -198,11 +228,15 @@ const void* TransformPlan::cheb_matrix(int n, bool forward) {
             }
         }
     }
+    // stored transposed, MT[j n + k] = M[k][j], for the matrix-core kernel's loads
+    std::vector<double> mt((size_t)n * n);
+    for (int k = 0; k < n; ++k)
+        for (int j = 0; j < n; ++j) mt[(size_t)j * n + k] = m[(size_t)k * n + j];
     if (f32) {
-        std::vector<float> h(m.begin(), m.end());
+        std::vector<float> h(mt.begin(), mt.end());
         return ctx_->table(key, h.data(), h.size() * sizeof(float));
     }
-    return ctx_->table(key, m.data(), m.size() * sizeof(double));
+    return ctx_->table(key, mt.data(), mt.size() * sizeof(double));
 }
 
 template <typename T>
@@ -229,18 +263,25 @@ void TransformPlan::axis_fft(void* data, int axis, bool inverse) {
 }
 
 template <typename T>
-void TransformPlan::axis_cheb(void* data, int axis, bool forward) {
-    using V = cplx<T>;
+void TransformPlan::axis_cheb(const void* in, void* out, int axis, bool forward) {
     size_t inner = 1, outer = 1;
     for (int a = 0; a < axis; ++a) inner *= (size_t)cshape_[a];
     for (int a = axis + 1; a < d_.ndim; ++a) outer *= (size_t)cshape_[a];
     const int n = d_.n[axis];
-    const T* mat = static_cast<const T*>(cheb_matrix(n, forward));
-    constexpr int NT = 256, TK = 4;
-    const size_t lds = (size_t)TK * n * sizeof(V);
-    dim3 grid((unsigned)((inner + TK - 1) / TK), (unsigned)outer);
-    hipLaunchKernelGGL((k_cheb_axis<T, TK, NT>), grid, dim3(NT), lds, ctx_->stream, static_cast<V*>(data), inner, n,
-                       mat);
+    const T* mt = static_cast<const T*>(cheb_matrix(n, forward));
+    const size_t ncols = 2 * inner, KP = (size_t)((n + 15) & ~15);
+    constexpr int NT = 256;
+    // the widest column tile whose LDS block leaves room for several workgroups per CU
+    auto launch = [&](auto CTC) {
+        constexpr int CT = decltype(CTC)::value;
+        const size_t lds = KP * (16 * CT + 1) * sizeof(T);
+        dim3 grid((unsigned)((ncols + 16 * CT - 1) / (16 * CT)), (unsigned)outer);
+        hipLaunchKernelGGL((k_cheb_axis<T, CT, NT>), grid, dim3(NT), lds, ctx_->stream, static_cast<const T*>(in),
+                           static_cast<T*>(out), ncols, n, mt);
+    };
+    if (KP * 65 * sizeof(T) <= 64 * 1024) launch(std::integral_constant<int, 4>{});
+    else if (KP * 33 * sizeof(T) <= 80 * 1024) launch(std::integral_constant<int, 2>{});
+    else launch(std::integral_constant<int, 1>{});
     TG_HIP(hipGetLastError());
 }
 
@@ -260,7 +301,7 @@ void TransformPlan::forward_t(const void* grid, void* coeff) {
     TG_HIP(hipGetLastError());
     for (int a = 1; a < d_.ndim; ++a) {
         if (d_.kind[a] == kRealFourier) axis_fft<T>(coeff, a, false);
-        else axis_cheb<T>(coeff, a, true);
+        else axis_cheb<T>(coeff, coeff, a, true);
     }
 }
 
@@ -272,10 +313,16 @@ void TransformPlan::backward_t(const void* coeff, void* grid) {
     const size_t cb = coeff_elems() * sizeof(V);
     if (!scratch_[0].ptr || scratch_[0].bytes < cb) scratch_[0] = DevBuf(ctx_, cb);
     void* work = scratch_[0].ptr;
-    TG_HIP(hipMemcpyAsync(work, coeff, cb, hipMemcpyDeviceToDevice, ctx_->stream));
+    // the input stays untouched: a Chebyshev last axis reads it straight into the work
+    // array; an in-place FFT axis needs a copy first
+    const void* src = coeff;
+    if (d_.ndim == 1 || d_.kind[d_.ndim - 1] == kRealFourier) {
+        TG_HIP(hipMemcpyAsync(work, coeff, cb, hipMemcpyDeviceToDevice, ctx_->stream));
+        src = work;
+    }
     for (int a = d_.ndim - 1; a >= 1; --a) {
         if (d_.kind[a] == kRealFourier) axis_fft<T>(work, a, true);
-        else axis_cheb<T>(work, a, false);
+        else axis_cheb<T>(a == d_.ndim - 1 ? src : work, work, a, false);
     }
     size_t nlines = 1;
     for (int a = 1; a < d_.ndim; ++a) nlines *= (size_t)d_.n[a];

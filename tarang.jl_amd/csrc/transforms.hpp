@@ -34,7 +34,7 @@ class TransformPlan {
     template <typename T> void backward_t(const void* coeff, void* grid);
     template <typename T> void product_t(const void* ga, const void* gb, void* out);
     template <typename T> void axis_fft(void* data, int axis, bool inverse);
-    template <typename T> void axis_cheb(void* data, int axis, bool forward);
+    template <typename T> void axis_cheb(const void* in, void* out, int axis, bool forward);  // out may be in
     const void* cheb_matrix(int n, bool forward);
 
     Context* ctx_;
