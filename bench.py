@@ -111,6 +111,9 @@ def main():
     ap.add_argument("--workload", choices=["all", "turbulence", "channel", "channel3d", "serving", "plugins"],
                     default="all")
     ap.add_argument("--requests", type=int, default=24, help="serving: number of requests")
+    ap.add_argument("--cfl-horizon", action="store_true",
+                    help="channel: also run the configuration to its t = 20 horizon with CFL-controlled dt "
+                         "(FP64 and FP32 split-inverse forms through the API)")
     ap.add_argument("--horizon", type=int, default=None,
                     help="channel workloads: steps before the FP32-vs-FP64 comparison (default: the timed steps "
                          "in 2-D, the configuration's 500 in 3-D)")
@@ -477,7 +480,70 @@ def bench_channel(a):
         "kernels_f32": kernels,
         "cpu_baseline": cpu,
     }
+    if a.cfl_horizon:
+        line["cfl_horizon"] = channel_cfl_horizon(a)
     return line
+
+
+def channel_cfl_horizon(a):
+    """The 2-D configuration run to its horizon t = 20 (2000 steps of the fixed dt = 0.01 at
+    most) with the CFL controller choosing dt (max_dt = 0.01, safety 0.5, cadence 10), FP64
+    and the FP32 split-inverse form, through the API mirror.  Convection at Ra = 2e6 is
+    chaotic, so the two trajectories decorrelate: they are compared by the time-averaged
+    Nusselt number over t in [10, 20], not pointwise."""
+    import numpy as np
+
+    import tarang_amd as T
+    from tarang_amd import _lib, flow_tools
+
+    nx, nz, Lx, Lz, Ra, Pr = (CH[k] for k in ("nx", "nz", "Lx", "Lz", "Ra", "Pr"))
+    kappa, nu = (Ra * Pr) ** -0.5, (Ra / Pr) ** -0.5
+    out = {}
+    for name, dtype in (("f64", np.float64), ("f32", np.float32)):
+        coords = T.CartesianCoordinates("x", "z")
+        dist = T.Distributor(coords, dtype=dtype, device=T.GPU())
+        xb = T.RealFourier(coords["x"], size=nx, bounds=(0.0, Lx), dealias=3 / 2)
+        zb = T.ChebyshevT(coords["z"], size=nz, bounds=(0.0, Lz), dealias=3 / 2)
+        dom = T.Domain(dist, (xb, zb))
+        p, b, u = T.ScalarField(dom, "p"), T.ScalarField(dom, "b"), T.VectorField(dom, "u")
+        taus = [T.ScalarField(dist, "tau_p", (), dtype), T.ScalarField(dist, "tau_b1", (xb,), dtype),
+                T.ScalarField(dist, "tau_b2", (xb,), dtype), T.VectorField(dist, coords, "tau_u1", (xb,), dtype),
+                T.VectorField(dist, coords, "tau_u2", (xb,), dtype)]
+        ex, ez = T.unit_vector_fields(coords, dist)
+        lift = lambda A: T.lift(A, T.derivative_basis(zb, 1), -1)  # noqa: E731
+        pb = T.IVP([p, b, u] + taus)
+        T.add_parameters_(pb, kappa=kappa, nu=nu, Lz=Lz, ez=ez, τ_lift=lift,
+                          grad_u=T.grad(u) + ez * lift(taus[3]), grad_b=T.grad(b) + ez * lift(taus[1]))
+        T.add_equation_(pb, "trace(grad_u) + tau_p = 0")
+        T.add_equation_(pb, "∂t(b) - kappa*div(grad_b) + τ_lift(tau_b2) = -u⋅∇(b)")
+        T.add_equation_(pb, "∂t(u) - nu*div(grad_u) + ∇(p) - b*ez + τ_lift(tau_u2) = -u⋅∇(u)")
+        for bc in ("b(z=0) = Lz", "u(z=0) = 0", "b(z=Lz) = 0", "u(z=Lz) = 0", "integ(p) = 0"):
+            T.add_bc_(pb, bc)
+        solver = T.InitialValueSolver(pb, T.RK222(), dt=CH["dt"])
+        bg, _ = channel_ic(np, nx, nz, Lz)
+        b["g"] = bg
+        cfl = flow_tools.CFL(solver, initial_dt=CH["dt"], cadence=10, safety=0.5, max_dt=CH["dt"])
+        flow_tools.add_velocity_(cfl, u)
+        nus, dts = [], set()
+
+        def sample(sv):
+            v = ctypes.c_double()
+            _lib.call("tarang_chan_nusselt", sv.handle, ctypes.byref(v))
+            nus.append((sv.sim_time, v.value))
+            dts.add(sv.dt)
+        t0 = time.perf_counter()
+        T.run_(solver, stop_time=20.0, cfl=cfl, callbacks=[(50, sample)])
+        el = time.perf_counter() - t0
+        late = [v for t, v in nus if t >= 10.0]
+        out[name] = {"steps": solver.iteration, "sim_time": solver.sim_time, "wall_s": el,
+                     "distinct_dt": len(dts), "finite": bool(np.all(np.isfinite(np.asarray([v for _, v in nus])))),
+                     "nusselt_final": nus[-1][1] if nus else None,
+                     "nusselt_mean_t10_20": float(np.mean(late)) if late else None,
+                     "nusselt_std_t10_20": float(np.std(late)) if late else None}
+        log(f"channel cfl horizon {name}: {solver.iteration} steps to t = {solver.sim_time:.3f} in {el:.1f} s, "
+            f"<Nu>(10..20) = {out[name]['nusselt_mean_t10_20']}")
+        solver.close()
+    return out
 
 
 CH3 = dict(nx=256, ny=256, nz=128, Lx=4.0, Ly=4.0, Lz=1.0, Ra=1e7, Pr=0.7, dt=2e-3)
@@ -512,7 +578,7 @@ def bench_channel3d(a):
     Bc = np.einsum("kz,mnz->mnk", np.linalg.inv(T), F)
     ctx = ctypes.c_void_p()
     _lib.call("tarang_ctx_create", 0, ctypes.byref(ctx))
-    results, finals, solvers, keys, setups = {}, {}, {}, {}, {}
+    results, finals, solvers, keys, setups, tunings = {}, {}, {}, {}, {}, {}
     # FP64, then the FP32 forms (the headline form first)
     forms = (("f64", _lib.F64, torch.complex128, 0, 0),) + tuple(
         (name, _lib.F32, torch.complex64, sw, sp) for name, sw, sp in FP32_FORMS)
@@ -529,6 +595,9 @@ def bench_channel3d(a):
         kk = ctypes.c_int()
         _lib.call("tarang_chan3_num_keys", h, ctypes.byref(kk))
         keys[name] = kk.value
+        tv = (ctypes.c_int * 2)()
+        _lib.call("tarang_chan3_get_tuning", h, tv)
+        tunings[name] = {"solve_8_waves": tv[0], "zbwd_16_mode_tile": tv[1]}
         Bd = torch.as_tensor(np.ascontiguousarray(Bc.T), device=dev).to(ct).contiguous()
         Zd = torch.zeros_like(Bd)
         torch.cuda.synchronize()
@@ -615,6 +684,7 @@ def bench_channel3d(a):
         "fp64": {"value": v64, "ms_per_step": results["f64"][0], "setup_s": results["f64"][1],
                  "warmup_s": setups["f64"]},
         "setup_s_f32": results["f32"][1], "warmup_s_f32": setups["f32"], "kh2_keys": keys["f32"],
+        "autotuned_variants": tunings,
         "dt_change_f32": dtc,
         "speedup_fp32_vs_fp64": v32 / v64,
         "fp32_form": FP32_HEADLINE,

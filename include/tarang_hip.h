@@ -337,6 +337,11 @@ int tarang_chan3_num_keys(tarang_chan3_t s, int* nkeys);
  * multistep / stage rhs, per-mode solve, L x] average ms and counts.  No reference counterpart. */
 int tarang_chan3_profile(tarang_chan3_t s, int nsteps, double* avg_ms8, long* counts8);
 int tarang_chan3_kernel_bytes(tarang_chan3_t s, double* bytes8);
+/* kernel variants picked by the autotuner at creation: [per-mode solve (0: 4 wavefronts per
+ * workgroup, 1: 8), z-backward (0: the default mode tile, 1: 16 modes, FP32 only)].  Every
+ * variant gives the same bits; TARANG_AUTOTUNE=0 keeps [0, 0].  No reference counterpart. */
+int tarang_chan3_get_tuning(tarang_chan3_t s, int* variants2);
+int tarang_chan3_set_tuning(tarang_chan3_t s, const int* variants2);
 
 /* ------------------------------------------------------------------------ */
 /* Serving loop: continuous batching of IVP solve requests on one GPU.  No reference

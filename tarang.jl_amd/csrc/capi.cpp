@@ -710,6 +710,27 @@ int tarang_chan3_kernel_bytes(tarang_chan3_t s, double* bytes8) {
     });
 }
 
+int tarang_chan3_get_tuning(tarang_chan3_t s, int* variants2) {
+    return guard([&] {
+        NEED(s, "solver");
+        NEED(variants2, "variants2");
+        const tarang::Chan3Tuning t = s->solver->tuning();
+        variants2[0] = t.solve;
+        variants2[1] = t.zbwd;
+    });
+}
+
+int tarang_chan3_set_tuning(tarang_chan3_t s, const int* variants2) {
+    return guard([&] {
+        NEED(s, "solver");
+        NEED(variants2, "variants2");
+        tarang::Chan3Tuning t;
+        t.solve = variants2[0];
+        t.zbwd = variants2[1];
+        s->solver->set_tuning(t);
+    });
+}
+
 // ---- serving loop ----
 int tarang_serve_create(int device, int lanes, size_t mem_budget, int chunk_steps, tarang_server_t* out) {
     return guard([&] {

@@ -24,12 +24,12 @@ namespace tarang {
 //   0 ux, 1 uy, 2 uz, then for f in (b, ux, uy, uz): 3+3f dx f, 4+3f dy f, 5+3f dz f
 constexpr int kCh3Planes = 15;
 
-template <typename T, int NT>
+template <typename T, int NT, int QT = ZbwdTile<T>::QT>
 __global__ __launch_bounds__(NT, 768 / NT) void k3_zbwd(const cplx<T>* __restrict__ b, const cplx<T>* __restrict__ ux,
                                                  const cplx<T>* __restrict__ uy, const cplx<T>* __restrict__ uz,
                                                  Ch3Geom geo, const T* __restrict__ BT, const T* __restrict__ BDT,
                                                  cplx<T>* __restrict__ G) {
-    zbwd_mfma<T, NT, false>(b, ux, uy, uz, geo, BT, BDT, G);
+    zbwd_mfma<T, NT, false, QT>(b, ux, uy, uz, geo, BT, BDT, G);
 }
 
 // ---------------------------------------------------------------------------

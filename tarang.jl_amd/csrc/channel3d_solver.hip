@@ -9,6 +9,8 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <mutex>
+#include <cstdlib>
 #include <type_traits>
 #include <cstring>
 #include <map>
@@ -207,6 +209,83 @@ Channel3Solver::Channel3Solver(Context* ctx, const Channel3Desc& d) : ctx_(ctx),
     ctx_->twiddle(my_, (DType)d.dtype);
     TG_HIP(hipStreamSynchronize(s));
     factor(implicit_c(first_step()));
+    autotune();
+}
+
+void Channel3Solver::set_tuning(const Chan3Tuning& t) {
+    TG_REQUIRE(t.solve == 0 || t.solve == 1, kErrInvalid, "3-D channel solver: solve variant is 0 or 1");
+    TG_REQUIRE(t.zbwd == 0 || (t.zbwd == 1 && d_.dtype == kF32), kErrInvalid,
+               "3-D channel solver: z-backward variant is 0, or 1 with an FP32 state");
+    tune_ = t;
+    invalidate_graph();
+}
+
+// Times the variants of k3_solve and k3_zbwd on this solver's own buffers (HIP events,
+// median of 5 launches) at creation.  A variant replaces the default only when it is at
+// least 3 % faster, so timing noise does not flip the choice from run to run.  The
+// choices are cached per (nx, ny, nz, dtype, solve form, |k_h|^2 keys) for the life of the process;
+// TARANG_AUTOTUNE=0 keeps the defaults.
+void Channel3Solver::autotune() {
+    const char* env = std::getenv("TARANG_AUTOTUNE");
+    if (env && std::string(env) == "0") return;
+    const std::array<int, 7> key{d_.nx, d_.ny, d_.nz, d_.dtype, d_.refine_sweeps, d_.split_inverse, nkeys_};
+    static std::mutex mu;
+    static std::map<std::array<int, 7>, Chan3Tuning> cache;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) {
+            tune_ = it->second;
+            return;
+        }
+    }
+    ctx_->activate();
+    hipStream_t s = ctx_->stream;
+    hipEvent_t a, b;
+    TG_HIP(hipEventCreate(&a));
+    TG_HIP(hipEventCreate(&b));
+    auto measure = [&](auto&& launch) {
+        std::vector<float> v;
+        for (int r = 0; r < 6; ++r) {
+            TG_HIP(hipEventRecord(a, s));
+            launch();
+            TG_HIP(hipEventRecord(b, s));
+            TG_HIP(hipEventSynchronize(b));
+            float ms = 0;
+            TG_HIP(hipEventElapsedTime(&ms, a, b));
+            if (r > 0) v.push_back(ms);
+        }
+        std::sort(v.begin(), v.end());
+        return v[v.size() / 2];
+    };
+    Chan3Tuning chosen;
+    auto go = [&](auto tag) {
+        using T = decltype(tag);
+        // k3_solve: variant 1 replaces the default when 3 % faster
+        tune_.solve = 0;
+        const float t0 = measure([&] { solve_t<T>(); });
+        tune_.solve = 1;
+        const float t1 = measure([&] { solve_t<T>(); });
+        chosen.solve = t1 < 0.97f * t0 ? 1 : 0;
+        if (sizeof(T) == 4) {
+            tune_.zbwd = 0;
+            const float z0 = measure([&] { rhs_t<T>(F_[0].ptr, F_[1].ptr, F_[2].ptr, F_[3].ptr); });
+            tune_.zbwd = 1;
+            const float z1 = measure([&] { rhs_t<T>(F_[0].ptr, F_[1].ptr, F_[2].ptr, F_[3].ptr); });
+            chosen.zbwd = z1 < 0.97f * z0 ? 1 : 0;
+        }
+    };
+    if (d_.dtype == kF32) go(float{});
+    else go(double{});
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    tune_ = chosen;
+    // the timed launches wrote the state and the products: back to the zero state of a new solver
+    for (DevBuf* x : {&P_, &B_, &UX_, &UY_, &UZ_, &TAU_, &F_[0], &F_[1], &F_[2], &F_[3]})
+        TG_HIP(hipMemsetAsync(x->ptr, 0, x->bytes, s));
+    TG_HIP(hipStreamSynchronize(s));
+    std::lock_guard<std::mutex> lk(mu);
+    cache[key] = tune_;
 }
 
 Channel3Solver::~Channel3Solver() {
@@ -356,13 +435,18 @@ void Channel3Solver::rhs_t(void* F0, void* F1, void* F2, void* F3) {
     V* Q = Q_.as<V>();
     V* H = H_.as<V>();
     {
-        constexpr int QT = ZbwdTile<T>::QT, NTZ = ZbwdTile<T>::NT;
-        const size_t blds = (size_t)((N + 15) & ~15) * zbwd_lds_stride(QT) * sizeof(T);
-        timed(0, [&] {
-            hipLaunchKernelGGL((k3_zbwd<T, NTZ>), dim3((unsigned)((nmodes_ + QT - 1) / QT), 4), dim3(NTZ), blds, s,
-                               B_.as<const V>(), UX_.as<const V>(), UY_.as<const V>(), UZ_.as<const V>(), geo,
-                               BT_.as<const T>(), BDT_.as<const T>(), G);
-        });
+        constexpr int NTZ = ZbwdTile<T>::NT;
+        auto zb = [&](auto QTC) {
+            constexpr int QT = decltype(QTC)::value;
+            const size_t blds = (size_t)((N + 15) & ~15) * zbwd_lds_stride(QT) * sizeof(T);
+            timed(0, [&] {
+                hipLaunchKernelGGL((k3_zbwd<T, NTZ, QT>), dim3((unsigned)((nmodes_ + QT - 1) / QT), 4), dim3(NTZ),
+                                   blds, s, B_.as<const V>(), UX_.as<const V>(), UY_.as<const V>(), UZ_.as<const V>(),
+                                   geo, BT_.as<const T>(), BDT_.as<const T>(), G);
+            });
+        };
+        if (sizeof(T) == 4 && tune_.zbwd == 1) zb(std::integral_constant<int, 16>{});
+        else zb(std::integral_constant<int, ZbwdTile<T>::QT>{});
     }
     TG_HIP(hipGetLastError());
     dispatch_size(d_.ny, [&](auto NC) {
@@ -457,18 +541,19 @@ void Channel3Solver::solve_t() {
                     chunks_.as<const int4>(), modes_.as<const int>(), rot_.as<const double>(), N,
                     sb_, su_, factored_c_, bcv_.as<const double2>(),
                     d_.refine_sweeps, kh_.as<const double>(), d_.kappa, d_.nu, 2.0 / d_.Lz, d_.Lx * d_.Ly, d_.Lz};
-    constexpr int NT = 256;  // four wavefronts: at most 3 b/perp and 7 u-p tiles each (nz <= 256)
+    // four wavefronts (at most 3 b/perp and 7 u-p tiles each for nz <= 256), or eight
     const size_t lds = ch3_solve_lds_doubles(N, d_.refine_sweeps > 0) * sizeof(double);
     timed(6, [&] {
-        auto launch = [&](auto kern) {
-            hipLaunchKernelGGL(kern, dim3(nchunks_), dim3(NT), lds, ctx_->stream, sa, R_[0].as<const V>(),
+        auto launch = [&](auto kern, int nt) {
+            hipLaunchKernelGGL(kern, dim3(nchunks_), dim3(nt), lds, ctx_->stream, sa, R_[0].as<const V>(),
                                R_[1].as<const V>(), R_[2].as<const V>(), R_[3].as<const V>(), R_[4].as<const V>(),
                                P_.as<V>(), B_.as<V>(), UX_.as<V>(), UY_.as<V>(), UZ_.as<V>(), TAU_.as<V>());
         };
         auto by_size = [&](auto refine, auto split) {
             constexpr bool R = decltype(refine)::value, S = decltype(split)::value;
-            if (ch3_solve_small(nb_max_, nu_max_)) launch(k3_solve<T, NT, R, S, true>);
-            else launch(k3_solve<T, NT, R, S, false>);
+            if (tune_.solve == 1) launch(k3_solve<T, 512, R, S, true>, 512);  // 8 waves: small tiles per wave
+            else if (ch3_solve_small(nb_max_, nu_max_)) launch(k3_solve<T, 256, R, S, true>, 256);
+            else launch(k3_solve<T, 256, R, S, false>, 256);
         };
         using yes = std::true_type;
         using no = std::false_type;

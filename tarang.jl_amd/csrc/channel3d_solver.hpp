@@ -29,6 +29,12 @@ struct Channel3Desc {
 
 enum Channel3Var : int { kC3P = 0, kC3B = 1, kC3UX = 2, kC3UY = 3, kC3UZ = 4 };
 
+// kernel variants picked by the autotuner at creation; every variant gives the same bits
+struct Chan3Tuning {
+    int solve = 0;  // k3_solve workgroup: 0 four wavefronts, 1 eight
+    int zbwd = 0;   // k3_zbwd mode tile: 0 the type's default (32 FP32 / 16 FP64), 1 16 modes (FP32 only)
+};
+
 class Channel3Solver {
   public:
     // the descriptor checks of the constructor, without allocating (throws tarang::Error)
@@ -61,6 +67,8 @@ class Channel3Solver {
     static constexpr int kKinds = 8;
     void profile(int nsteps, double* avg_ms, long* counts);
     void kernel_bytes(double* out) const;
+    Chan3Tuning tuning() const { return tune_; }
+    void set_tuning(const Chan3Tuning& t);
 
   private:
     template <typename T> void rhs_t(void* F0, void* F1, void* F2, void* F3);
@@ -74,6 +82,7 @@ class Channel3Solver {
     double implicit_c(bool first) const;
     bool first_step() const { return multistep() && (iter_ == 0 || d_.stepper == kSBDF1); }
     void factor(double c);
+    void autotune();
     void launch_step_body();
     void step_once(bool allow_graph);
     template <typename F> void timed(int kind, F&& launch);
@@ -100,6 +109,7 @@ class Channel3Solver {
     // or FP32 packed (pk_index) with the bfloat16 remainders in the *lo_ buffers when split
     DevBuf Ib_, Ip_, Iu_, Iblo_, Iplo_, Iulo_, info_;
     size_t sb_ = 0, su_ = 0;  // per-system strides of the inverses
+    Chan3Tuning tune_;
     int nchunks_ = 0;
     std::vector<int> key_count_;  // modes per key (host copy, for the byte model)
     bool graph_on_ = true;

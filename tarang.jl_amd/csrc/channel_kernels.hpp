@@ -105,15 +105,15 @@ __host__ __device__ inline int zbwd_lds_stride(int qt) { return 2 * qt + 1; }
 // first-half z tiles of the parity-split evaluation
 __host__ __device__ inline int zbwd_half_tiles(int N) { return ((N + 1) / 2 + 15) / 16; }
 
-template <typename T, int NT, bool TWO_D>
+template <typename T, int NT, bool TWO_D, int QT = ZbwdTile<T>::QT>
 __device__ inline void zbwd_mfma(const cplx<T>* __restrict__ b, const cplx<T>* __restrict__ ux,
                                  const cplx<T>* __restrict__ uy, const cplx<T>* __restrict__ uz, const Ch3Geom& geo,
                                  const T* __restrict__ BT, const T* __restrict__ BDT, cplx<T>* __restrict__ G) {
-    static_assert(NT == ZbwdTile<T>::NT, "workgroup size of the tile plan");
+    static_assert(NT == ZbwdTile<T>::NT && QT % 16 == 0, "workgroup size of the tile plan, 16-mode groups");
     using Op = MfmaOp<T>;
     using acc4 = typename Op::acc4;
     using V = cplx<T>;
-    constexpr int QT = ZbwdTile<T>::QT, NG = QT / 16, CT = 2 * NG, S = 2 * QT + 1;
+    constexpr int NG = QT / 16, CT = 2 * NG, S = 2 * QT + 1;
     constexpr int U = ZbwdTile<T>::U, NWV = NT / 64;
     extern __shared__ unsigned char smem_raw[];
     T* C = reinterpret_cast<T*>(smem_raw);  // [KP][S]

@@ -313,9 +313,9 @@ void FourierSolver::launch_yfwd(int variant, const cplx<T>* H, cplx<T>* F) {
     TG_HIP(hipGetLastError());
 }
 
-// Times every variant of the three RHS kernels on this solver's own buffers and
-// keeps the fastest (HIP events, median of 5 launches); results are cached per
-// (problem, Nx, Ny, dtype) for the life of the process.
+// Times every variant of the three RHS kernels on this solver's own buffers (HIP
+// events, median of 5 launches) and keeps the fastest by a 3 % margin over the earlier
+// variants; results are cached per (problem, Nx, Ny, dtype) for the life of the process.
 static std::map<std::array<int, 4>, FourierTuning>& tuning_cache() {
     static std::map<std::array<int, 4>, FourierTuning> m;
     return m;
@@ -368,21 +368,19 @@ void FourierSolver::autotune() {
         V* G = G_.as<V>();
         V* H = H_.as<V>();
         V* F = F_[0].as<V>();
-        float best = 1e30f;
-        for (int v = 0; v < 4; ++v) {
-            const float t = measure([&] { launch_ybwd<T, TurbSpec>(v, p, G, gstride); });
-            if (t < best) { best = t; tune_.ybwd = v; }
-        }
-        best = 1e30f;
-        for (int v = 0; v < 2; ++v) {
-            const float t = measure([&] { launch_xpass<T, TurbSpec>(v, G, gstride, T(1), H); });
-            if (t < best) { best = t; tune_.xpass = v; }
-        }
-        best = 1e30f;
-        for (int v = 0; v < 4; ++v) {
-            const float t = measure([&] { launch_yfwd<T>(v, H, F); });
-            if (t < best) { best = t; tune_.yfwd = v; }
-        }
+        // a variant replaces the current choice only when at least 3 % faster, so timing
+        // noise does not flip the choice from run to run
+        auto pick = [&](int nvar, int& slot, auto&& run) {
+            slot = 0;
+            float best = measure([&] { run(0); });
+            for (int v = 1; v < nvar; ++v) {
+                const float t = measure([&] { run(v); });
+                if (t < 0.97f * best) { best = t; slot = v; }
+            }
+        };
+        pick(4, tune_.ybwd, [&](int v) { launch_ybwd<T, TurbSpec>(v, p, G, gstride); });
+        pick(2, tune_.xpass, [&](int v) { launch_xpass<T, TurbSpec>(v, G, gstride, T(1), H); });
+        pick(4, tune_.yfwd, [&](int v) { launch_yfwd<T>(v, H, F); });
     };
     if (d_.dtype == kF32) run(float{});
     else run(double{});

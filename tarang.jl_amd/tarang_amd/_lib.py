@@ -182,6 +182,8 @@ SIGNATURES = {
     "tarang_chan3_num_keys": [_P, _PI],
     "tarang_chan3_profile": [_P, _I, _PD, ctypes.POINTER(ctypes.c_long)],
     "tarang_chan3_kernel_bytes": [_P, _PD],
+    "tarang_chan3_get_tuning": [_P, _PI],
+    "tarang_chan3_set_tuning": [_P, _PI],
     "tarang_serve_create": [_I, _I, ctypes.c_size_t, _I, _PP],
     "tarang_serve_destroy": [_P],
     "tarang_serve_submit": [_P, ctypes.POINTER(ServeRequest), _PI],

@@ -6,6 +6,8 @@ right-hand side and steps, FP32 1e-5.  At the
 small-amplitude state confined to modes kx, ky in {0, 1}, whose products are alias-free
 on both grids.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -108,6 +110,38 @@ def test_channel3d_graph_and_keys(gpu, oracle):
         s.close()
     for a, b in zip(*out):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_channel3d_tuning_variants_same_bits(gpu, oracle, dtype):
+    """Every kernel variant the autotuner can pick (k3_solve with 4 or 8 wavefronts, k3_zbwd
+    with 32- or 16-mode tiles) gives the same bits, on a 32 x 32 x 24 state over 3 SBDF2 steps
+    (multi-chunk keys, the split-inverse FP32 form)."""
+    nx, ny, N, L = 32, 32, 24, 4.0
+    kappa, nu = kp()
+    st, _ = random_state3(oracle, nx, ny, N, L, L, 1.0, seed=11)
+    variants = [(0, 0), (1, 0)] + ([(0, 1), (1, 1)] if dtype == np.float32 else [])
+    out = []
+    for v in variants:
+        print(f"channel3d tuning variant {v} {dtype.__name__}", flush=True)
+        s = gpu.chan3(nx, ny, N, L, L, 1.0, kappa, nu, 5e-3, dtype=dtype)
+        t = (ctypes.c_int * 2)(*v)
+        _lib.call("tarang_chan3_set_tuning", s.h, t)
+        got = (ctypes.c_int * 2)()
+        _lib.call("tarang_chan3_get_tuning", s.h, got)
+        assert tuple(got) == v
+        for var, c in st.items():
+            s.set(var, c)
+        s.step(3)
+        out.append([s.get(var) for var in (oracle.RBC3_B, oracle.RBC3_UX, oracle.RBC3_UY, oracle.RBC3_UZ)])
+        s.close()
+    for o in out[1:]:
+        for a, b in zip(out[0], o):
+            assert np.array_equal(a, b)
+    s = gpu.chan3(nx, ny, N, L, L, 1.0, kappa, nu, 5e-3, dtype=np.float64)
+    with pytest.raises(_lib.TarangError):  # the 16-mode z tile is an FP32 variant
+        _lib.call("tarang_chan3_set_tuning", s.h, (ctypes.c_int * 2)(0, 1))
+    s.close()
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
