@@ -114,6 +114,8 @@ def main():
     ap.add_argument("--cfl-horizon", action="store_true",
                     help="channel: also run the configuration to its t = 20 horizon with CFL-controlled dt "
                          "(FP64 and FP32 split-inverse forms through the API)")
+    ap.add_argument("--cfl-wall", type=float, default=480.0,
+                    help="channel --cfl-horizon: wall-clock limit per form in seconds")
     ap.add_argument("--horizon", type=int, default=None,
                     help="channel workloads: steps before the FP32-vs-FP64 comparison (default: the timed steps "
                          "in 2-D, the configuration's 500 in 3-D)")
@@ -526,16 +528,26 @@ def channel_cfl_horizon(a):
         flow_tools.add_velocity_(cfl, u)
         nus, dts = [], set()
 
+        t0 = time.perf_counter()
+        last = [t0]
+
         def sample(sv):
             v = ctypes.c_double()
             _lib.call("tarang_chan_nusselt", sv.handle, ctypes.byref(v))
             nus.append((sv.sim_time, v.value))
             dts.add(sv.dt)
-        t0 = time.perf_counter()
-        T.run_(solver, stop_time=20.0, cfl=cfl, callbacks=[(50, sample)])
+            now = time.perf_counter()
+            if now - last[0] >= 15.0:
+                last[0] = now
+                log(f"channel cfl horizon {name}: step {sv.iteration}, t = {sv.sim_time:.4f}, dt = {sv.dt:.4g}, "
+                    f"Nu = {v.value:.6g}, {now - t0:.0f} s")
+        # the wall-clock guard ends a run that the CFL dt would make longer than the bench allows
+        T.run_(solver, stop_time=20.0, stop_wall_time=a.cfl_wall, cfl=cfl, callbacks=[(50, sample)])
         el = time.perf_counter() - t0
         late = [v for t, v in nus if t >= 10.0]
         out[name] = {"steps": solver.iteration, "sim_time": solver.sim_time, "wall_s": el,
+                     "reached_horizon": bool(solver.sim_time >= 20.0 - 1e-9),
+                     "dt_min": min(dts) if dts else None, "dt_max": max(dts) if dts else None,
                      "distinct_dt": len(dts), "finite": bool(np.all(np.isfinite(np.asarray([v for _, v in nus])))),
                      "nusselt_final": nus[-1][1] if nus else None,
                      "nusselt_mean_t10_20": float(np.mean(late)) if late else None,

@@ -379,7 +379,19 @@ def run_(solver: InitialValueSolver, stop_time: Optional[float] = None, stop_ite
         dt = compute_timestep(cfl)
         if dt != solver.dt:
             solver.set_dt(dt)
-        solver.step_many(1)
+        # compute_timestep only evaluates the velocities on every cadence-th call and returns
+        # the current dt unchanged in between: those steps go to the device as one call
+        # (the dt sequence is the one a step-by-step loop produces)
+        n = max(1, cfl.cadence - cfl.iteration_count % cfl.cadence) if cfl.cadence > 1 else 1
+        if solver.stop_iteration is not None:
+            n = min(n, solver.stop_iteration - solver.iteration)
+        if math.isfinite(solver.stop_sim_time):
+            n = min(n, int(math.ceil((solver.stop_sim_time - solver.sim_time) / solver.dt - 1e-9)))
+        for k in intervals:
+            n = min(n, k - solver.iteration % k)
+        n = max(1, n)
+        solver.step_many(n)
+        cfl.iteration_count += n - 1
         for k, fn in callbacks:
             if solver.iteration % int(k) == 0:
                 fn(solver)

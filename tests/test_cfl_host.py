@@ -71,3 +71,68 @@ def test_constant_diffusivity_limit():
     T.add_diffusivity_(bad, 0.01)
     with pytest.raises(ValueError, match="cannot determine the grid"):
         T.compute_timestep(bad)
+
+
+class _FakeSolver:
+    """The parts of InitialValueSolver that run_ uses, recording the dt of every step."""
+
+    def __init__(self, dt):
+        self.dt, self.iteration, self.sim_time = dt, 0, 0.0
+        self.stop_sim_time, self.stop_iteration, self.stop_wall_time = math.inf, None, math.inf
+        self.calls, self.step_dts = 0, []
+
+    @property
+    def proceed(self):
+        if self.sim_time >= self.stop_sim_time - 1e-12 * max(1.0, abs(self.stop_sim_time)):
+            return False
+        return self.stop_iteration is None or self.iteration < self.stop_iteration
+
+    def set_dt(self, dt):
+        self.dt = float(dt)
+
+    def step_many(self, n):
+        self.calls += 1
+        self.step_dts += [self.dt] * n
+        self.iteration += n
+        self.sim_time += n * self.dt
+
+
+class _StateCFL(T.CFL):
+    """Velocity frequency read from the solver's iteration, as a device reduction of the
+    current state would be."""
+
+    def __init__(self, solver, **kw):
+        super().__init__(solver, **kw)
+        self.velocities = [object()]
+
+    def _velocity_frequency(self, u):
+        return 10.0 + 7.0 * math.sin(0.37 * self.solver.iteration) + 0.05 * self.solver.iteration
+
+
+@pytest.mark.parametrize("cadence", [1, 3, 10])
+def test_run_with_cfl_matches_step_by_step(cadence):
+    """run_(cfl=...) hands the steps between two CFL evaluations to the device as one call;
+    the dt sequence, the stop and the callback iterations are those of the per-step loop of
+    run! (solver_stepping.jl:148-180): compute_timestep, then one step."""
+    kw = dict(initial_dt=0.01, cadence=cadence, safety=0.5, threshold=0.1, max_dt=0.05)
+    ref = _FakeSolver(0.01)
+    ref.stop_sim_time = 3.0
+    rc = _StateCFL(ref, **kw)
+    ref_hits = []
+    while ref.proceed:
+        dt = T.compute_timestep(rc)
+        if dt != ref.dt:
+            ref.set_dt(dt)
+        ref.step_many(1)
+        if ref.iteration % 7 == 0:
+            ref_hits.append((ref.iteration, ref.dt))
+    run = _FakeSolver(0.01)
+    c = _StateCFL(run, **kw)
+    hits = []
+    T.run_(run, stop_time=3.0, cfl=c, callbacks=[(7, lambda s: hits.append((s.iteration, s.dt)))])
+    assert run.step_dts == ref.step_dts
+    assert hits == ref_hits and run.iteration == ref.iteration
+    assert c.iteration_count == rc.iteration_count and c.current_dt == rc.current_dt
+    assert len(set(ref.step_dts)) > 3          # the controller did move dt
+    if cadence > 1:
+        assert run.calls < ref.calls
