@@ -335,7 +335,7 @@ def dt_change_cost(_lib, ctx, h, pre, dt, steady_ms):
 
 # FP32 forms of the channel solvers: (name, refine_sweeps, split_inverse); the first is the
 # headline form.  split_inverse stores every tau-system inverse entry as an FP32 value plus a
-# bfloat16 remainder (2^-33 relative); refine_sweeps adds FP64-residual refinement sweeps.
+# 8-bit remainder (2^-32 relative); refine_sweeps adds FP64-residual refinement sweeps.
 FP32_FORMS = (("f32", 0, 1), ("f32_plain", 0, 0), ("f32_sweeps1", 1, 0), ("f32_sweeps2", 2, 0))
 FP32_HEADLINE = {"refine_sweeps": FP32_FORMS[0][1], "split_inverse": FP32_FORMS[0][2]}
 
@@ -609,7 +609,7 @@ def bench_channel3d(a):
         keys[name] = kk.value
         tv = (ctypes.c_int * 2)()
         _lib.call("tarang_chan3_get_tuning", h, tv)
-        tunings[name] = {"solve_8_waves": tv[0], "zbwd_16_mode_tile": tv[1]}
+        tunings[name] = {"solve_variant": tv[0], "zbwd_16_mode_tile": tv[1]}
         Bd = torch.as_tensor(np.ascontiguousarray(Bc.T), device=dev).to(ct).contiguous()
         Zd = torch.zeros_like(Bd)
         torch.cuda.synchronize()
@@ -718,7 +718,7 @@ SERVE_WORKLOAD = ("serving stream: Poisson arrivals at 80% of the measured satur
                   "N in {128,256,512,1024}^2 (RK443, dt=1e-3), 40% 2-D Rayleigh-Benard Nx x Nz in {128x32, 256x64, "
                   "512x128, 1024x256}, Ra log-uniform in [1e5, 1e8], Pr=1 (RK222, dt=1e-3), 20% 1-D Burgers N in "
                   "{64..4096}, nu=1e-2 (RK222, dt=1e-3); FP32 state, FP64 tau factorisation, inverses stored as FP32 "
-                  "values plus bfloat16 remainders; steps log-uniform in [200, 2000]")
+                  "values plus 8-bit remainders; steps log-uniform in [200, 2000]")
 
 
 def serving_stream(np, torch, _lib, configs, n, seed, dev):

@@ -237,7 +237,7 @@ typedef struct {
     int refine_sweeps;       /* 0, 1 or 2 sweeps of x += A^-1 (y - A x) per implicit solve, with the
                                 residual y - A x formed in FP64 from the structured tau systems */
     int split_inverse;       /* FP32 state only: 1 stores every inverse entry as an FP32 value plus
-                                a bfloat16 remainder (2^-33 relative instead of 2^-24, 6 bytes per
+                                an 8-bit remainder (2^-32 relative instead of 2^-24, 5 bytes per
                                 entry instead of 4); 0 plain FP32 inverses */
 } tarang_chan_desc;
 
@@ -338,8 +338,9 @@ int tarang_chan3_num_keys(tarang_chan3_t s, int* nkeys);
 int tarang_chan3_profile(tarang_chan3_t s, int nsteps, double* avg_ms8, long* counts8);
 int tarang_chan3_kernel_bytes(tarang_chan3_t s, double* bytes8);
 /* kernel variants picked by the autotuner at creation: [per-mode solve (0: 4 wavefronts per
- * workgroup, 1: 8), z-backward (0: the default mode tile, 1: 16 modes, FP32 only)].  Every
- * variant gives the same bits; TARANG_AUTOTUNE=0 keeps [0, 0].  No reference counterpart. */
+ * workgroup, 1: 8, 2: 4 with a 3-deep ring of inverse loads), z-backward (0: the default mode
+ * tile, 1: 16 modes, FP32 only)].  Every variant gives the same bits; TARANG_AUTOTUNE=0 keeps
+ * [0, 0].  No reference counterpart. */
 int tarang_chan3_get_tuning(tarang_chan3_t s, int* variants2);
 int tarang_chan3_set_tuning(tarang_chan3_t s, const int* variants2);
 

@@ -246,8 +246,8 @@ struct Ch3SolveArgs {
     const void* Ib;
     const void* Ip;     // perp-system inverses, stride sb
     const void* Iu;     // u-p-system inverses, stride su
-    // bfloat16 remainders of the FP32 inverses (split inverses), same offsets; null otherwise
-    const unsigned short *Iblo, *Iplo, *Iulo;
+    // 8-bit remainders of the FP32 inverses (split inverses), same offsets; null otherwise
+    const signed char *Iblo, *Iplo, *Iulo;
     const int4* chunks; // [nchunks] (key, g, first, count): modes[first .. first + count), count <= 8
     const int* modes;   // modes grouped by key
     const double* rot;  // [2 nmodes] (cos, sin) of the k_h direction per mode
@@ -279,15 +279,15 @@ __host__ __device__ inline size_t ch3_solve_lds_doubles(int N, bool refine = fal
 }
 
 // A^-1 Y for one system: FP64 inverses column-major; FP32 inverses packed, with their
-// bfloat16 remainders when SPLIT
-template <int MT, typename T, bool SPLIT>
-__device__ inline void ch3_apply(const void* I, const unsigned short* Ilo, size_t off, int n, const double* Y,
+// 8-bit remainders when SPLIT
+template <int MT, typename T, bool SPLIT, int D = 2>
+__device__ inline void ch3_apply(const void* I, const signed char* Ilo, size_t off, int n, const double* Y,
                                  dbl4 (&acc)[MT], int wave, int nw, int wave0, bool accumulate = false) {
     if constexpr (sizeof(T) == 4)
-        mfma_apply_pk<MT, SPLIT>(static_cast<const float*>(I) + off, SPLIT ? Ilo + off : nullptr, n, Y, acc, wave, nw,
-                                 wave0, accumulate);
+        mfma_apply_pk<MT, SPLIT, D>(static_cast<const float*>(I) + off, SPLIT ? Ilo + off : nullptr, n, Y, acc, wave,
+                                    nw, wave0, accumulate);
     else
-        mfma_apply<MT>(static_cast<const double*>(I) + off, n, Y, acc, wave, nw, wave0, accumulate);
+        mfma_apply<MT, double, D>(static_cast<const double*>(I) + off, n, Y, acc, wave, nw, wave0, accumulate);
 }
 
 // Workgroup = (|k_h| key, parity g, up to 8 modes of that key).  Right-hand sides of
@@ -300,8 +300,9 @@ __device__ inline void ch3_apply(const void* I, const unsigned short* Ilo, size_
 // E3z rows take + c b), then the un-rotation of the parity-g horizontal velocity.
 // SMALL: at most 4 u-p and 2 b/perp tiles per wave (nu <= 256, nb <= 128 with NT = 256),
 // which keeps the accumulators and load buffers of a 3-workgroup-per-CU launch in
-// registers; the larger FP32 class (wider packed-load buffers) runs 2 workgroups per CU
-template <typename T, int NT, bool REFINE = false, bool SPLIT = false, bool SMALL = false>
+// registers; the larger FP32 class (wider packed-load buffers) runs 2 workgroups per CU.
+// D: depth of the inverse-load ring (D - 1 batches of entries in flight per wave)
+template <typename T, int NT, bool REFINE = false, bool SPLIT = false, bool SMALL = false, int D = 2>
 __global__ __launch_bounds__(NT, REFINE ? 1 : (SMALL || sizeof(T) == 8 ? 768 : 512) / NT) void k3_solve(Ch3SolveArgs s, const cplx<T>* __restrict__ R1,
                                                const cplx<T>* __restrict__ R2, const cplx<T>* __restrict__ R3x,
                                                const cplx<T>* __restrict__ R3y, const cplx<T>* __restrict__ R3z,
@@ -373,8 +374,8 @@ __global__ __launch_bounds__(NT, REFINE ? 1 : (SMALL || sizeof(T) == 8 ? 768 : 5
     {
         const size_t ob = (size_t)(2 * key + h) * s.sb, op = (size_t)(2 * key + g) * s.sb;
         dbl4 ab[MTB], ap[MTB];
-        ch3_apply<MTB, T, SPLIT>(s.Ib, s.Iblo, ob, nb, Yb, ab, wave, nw, 0);
-        ch3_apply<MTB, T, SPLIT>(s.Ip, s.Iplo, op, np, Yp, ap, wave, nw, w0p);
+        ch3_apply<MTB, T, SPLIT, D>(s.Ib, s.Iblo, ob, nb, Yb, ab, wave, nw, 0);
+        ch3_apply<MTB, T, SPLIT, D>(s.Ip, s.Iplo, op, np, Yp, ap, wave, nw, w0p);
         __syncthreads();
         mfma_store<MTB>(ab, nb, Xb, wave, nw, 0);
         mfma_store<MTB>(ap, np, Xp, wave, nw, w0p);
@@ -383,8 +384,8 @@ __global__ __launch_bounds__(NT, REFINE ? 1 : (SMALL || sizeof(T) == 8 ? 768 : 5
             if (threadIdx.x < 16) chb_residual(N, h, kh, s.kappa, s.c, s.sz, Yb, Xb, kYS, 1, threadIdx.x);
             else if (threadIdx.x < 32) chb_residual(N, g, kh, s.nu, s.c, s.sz, Yp, Xp, kYS, 1, threadIdx.x - 16);
             __syncthreads();
-            ch3_apply<MTB, T, SPLIT>(s.Ib, s.Iblo, ob, nb, Xb, ab, wave, nw, 0, true);  // x += A^-1 r
-            ch3_apply<MTB, T, SPLIT>(s.Ip, s.Iplo, op, np, Xp, ap, wave, nw, w0p, true);
+            ch3_apply<MTB, T, SPLIT, D>(s.Ib, s.Iblo, ob, nb, Xb, ab, wave, nw, 0, true);  // x += A^-1 r
+            ch3_apply<MTB, T, SPLIT, D>(s.Ip, s.Iplo, op, np, Xp, ap, wave, nw, w0p, true);
             __syncthreads();
             mfma_store<MTB>(ab, nb, Xb, wave, nw, 0);
             mfma_store<MTB>(ap, np, Xp, wave, nw, w0p);
@@ -408,7 +409,7 @@ __global__ __launch_bounds__(NT, REFINE ? 1 : (SMALL || sizeof(T) == 8 ? 768 : 5
     {
         const size_t ou = (size_t)(2 * key + g) * s.su;
         dbl4 au[MTU];
-        ch3_apply<MTU, T, SPLIT>(s.Iu, s.Iulo, ou, nu, Yu, au, wave, nw, 0);
+        ch3_apply<MTU, T, SPLIT, D>(s.Iu, s.Iulo, ou, nu, Yu, au, wave, nw, 0);
         __syncthreads();
         mfma_store<MTU>(au, nu, Xu, wave, nw, 0);
         for (int sw = 0; REFINE && sw < sweeps; ++sw) {
@@ -416,7 +417,7 @@ __global__ __launch_bounds__(NT, REFINE ? 1 : (SMALL || sizeof(T) == 8 ? 768 : 5
             if (threadIdx.x < 16)
                 chu_residual(N, g, key == 0, kh, s.nu, s.c, s.sz, s.area, s.Lz, Yu, Xu, kYS, 1, threadIdx.x);
             __syncthreads();
-            ch3_apply<MTU, T, SPLIT>(s.Iu, s.Iulo, ou, nu, Xu, au, wave, nw, 0, true);  // x += A^-1 r
+            ch3_apply<MTU, T, SPLIT, D>(s.Iu, s.Iulo, ou, nu, Xu, au, wave, nw, 0, true);  // x += A^-1 r
             __syncthreads();
             mfma_store<MTU>(au, nu, Xu, wave, nw, 0);
         }

@@ -193,9 +193,9 @@ Channel3Solver::Channel3Solver(Context* ctx, const Channel3Desc& d) : ctx_(ctx),
         Ip_ = zeroed((size_t)2 * nkeys_ * sb_ * ib);
         Iu_ = zeroed((size_t)2 * nkeys_ * su_ * ib);
         if (d.split_inverse) {
-            Iblo_ = zeroed((size_t)2 * nkeys_ * sb_ * 2);
-            Iplo_ = zeroed((size_t)2 * nkeys_ * sb_ * 2);
-            Iulo_ = zeroed((size_t)2 * nkeys_ * su_ * 2);
+            Iblo_ = zeroed((size_t)2 * nkeys_ * sb_);
+            Iplo_ = zeroed((size_t)2 * nkeys_ * sb_);
+            Iulo_ = zeroed((size_t)2 * nkeys_ * su_);
         }
     }
     info_ = DevBuf(ctx_, (size_t)6 * nkeys_ * sizeof(int));
@@ -213,7 +213,7 @@ Channel3Solver::Channel3Solver(Context* ctx, const Channel3Desc& d) : ctx_(ctx),
 }
 
 void Channel3Solver::set_tuning(const Chan3Tuning& t) {
-    TG_REQUIRE(t.solve == 0 || t.solve == 1, kErrInvalid, "3-D channel solver: solve variant is 0 or 1");
+    TG_REQUIRE(t.solve >= 0 && t.solve <= 2, kErrInvalid, "3-D channel solver: solve variant is 0, 1 or 2");
     TG_REQUIRE(t.zbwd == 0 || (t.zbwd == 1 && d_.dtype == kF32), kErrInvalid,
                "3-D channel solver: z-backward variant is 0, or 1 with an FP32 state");
     tune_ = t;
@@ -261,12 +261,20 @@ void Channel3Solver::autotune() {
     Chan3Tuning chosen;
     auto go = [&](auto tag) {
         using T = decltype(tag);
-        // k3_solve: variant 1 replaces the default when 3 % faster
+        // k3_solve: the fastest variant replaces the default when 3 % faster (variant 2, the
+        // deeper load ring, applies to the small size class without refinement sweeps)
         tune_.solve = 0;
         const float t0 = measure([&] { solve_t<T>(); });
-        tune_.solve = 1;
-        const float t1 = measure([&] { solve_t<T>(); });
-        chosen.solve = t1 < 0.97f * t0 ? 1 : 0;
+        const int nvar = ch3_solve_small(nb_max_, nu_max_) && d_.refine_sweeps == 0 ? 3 : 2;
+        float best = t0;
+        for (int v = 1; v < nvar; ++v) {
+            tune_.solve = v;
+            const float tv = measure([&] { solve_t<T>(); });
+            if (tv < 0.97f * t0 && tv < best) {
+                best = tv;
+                chosen.solve = v;
+            }
+        }
         if (sizeof(T) == 4) {
             tune_.zbwd = 0;
             const float z0 = measure([&] { rhs_t<T>(F_[0].ptr, F_[1].ptr, F_[2].ptr, F_[3].ptr); });
@@ -400,11 +408,11 @@ void Channel3Solver::factor(double c) {
         using OT = decltype(ot);
         const bool pack = d_.dtype == kF32;
         lu_invert<NT, OT>(s, Lb.as<const double>(), pb.as<const int>(), Ib_.as<OT>(), N, 0, nsys, sb, nb_max_, nb_max_,
-                          pack, Iblo_.as<unsigned short>());
+                          pack, Iblo_.as<signed char>());
         lu_invert<NT, OT>(s, Lp.as<const double>(), pp.as<const int>(), Ip_.as<OT>(), N, 0, nsys, sb, nb_max_, nb_max_,
-                          pack, Iplo_.as<unsigned short>());
+                          pack, Iplo_.as<signed char>());
         lu_invert<NT, OT>(s, Lu.as<const double>(), pu.as<const int>(), Iu_.as<OT>(), N, 1, nsys, su, nu_max_, nu_max_,
-                          pack, Iulo_.as<unsigned short>());
+                          pack, Iulo_.as<signed char>());
     };
     if (d_.dtype == kF32) invert(float{});
     else invert(double{});
@@ -536,8 +544,8 @@ template <typename T>
 void Channel3Solver::solve_t() {
     using V = cplx<T>;
     const int N = d_.nz;
-    Ch3SolveArgs sa{Ib_.ptr, Ip_.ptr, Iu_.ptr, Iblo_.as<const unsigned short>(), Iplo_.as<const unsigned short>(),
-                    Iulo_.as<const unsigned short>(),
+    Ch3SolveArgs sa{Ib_.ptr, Ip_.ptr, Iu_.ptr, Iblo_.as<const signed char>(), Iplo_.as<const signed char>(),
+                    Iulo_.as<const signed char>(),
                     chunks_.as<const int4>(), modes_.as<const int>(), rot_.as<const double>(), N,
                     sb_, su_, factored_c_, bcv_.as<const double2>(),
                     d_.refine_sweeps, kh_.as<const double>(), d_.kappa, d_.nu, 2.0 / d_.Lz, d_.Lx * d_.Ly, d_.Lz};
@@ -551,8 +559,10 @@ void Channel3Solver::solve_t() {
         };
         auto by_size = [&](auto refine, auto split) {
             constexpr bool R = decltype(refine)::value, S = decltype(split)::value;
+            const bool small = ch3_solve_small(nb_max_, nu_max_);
             if (tune_.solve == 1) launch(k3_solve<T, 512, R, S, true>, 512);  // 8 waves: small tiles per wave
-            else if (ch3_solve_small(nb_max_, nu_max_)) launch(k3_solve<T, 256, R, S, true>, 256);
+            else if (small && !R && tune_.solve == 2) launch(k3_solve<T, 256, R, S, true, 3>, 256);
+            else if (small) launch(k3_solve<T, 256, R, S, true>, 256);
             else launch(k3_solve<T, 256, R, S, false>, 256);
         };
         using yes = std::true_type;
@@ -772,7 +782,7 @@ void Channel3Solver::kernel_bytes(double* out) const {
     out[4] = (8 * nc) * vb + N * N * rb;                        // read 4 planes, write 4 rows
     out[5] = (4 * 4 + 4 + 5) * nc * vb;                         // state (x2 on the kx lines), F, histories, rhs
     double fac = 0;  // every chunk of up to 8 modes reads its key's three inverses once
-    const double eb = rb + (d_.split_inverse ? 2.0 : 0.0);  // bytes per inverse entry
+    const double eb = rb + (d_.split_inverse ? 1.0 : 0.0);  // bytes per inverse entry
     for (int k = 0; k < nkeys_; ++k) {
         const double chunks = (key_count_[k] + kCh3ChunkModes - 1) / kCh3ChunkModes;
         for (int g = 0; g < 2; ++g) {
@@ -806,7 +816,7 @@ size_t footprint(const Channel3Desc& d) {
     const bool pack = d.dtype == kF32;
     const size_t sb = inverse_stride((int)nb, pack), su = inverse_stride((int)nu, pack);
     b += 2 * alloc_granule(nsys * sb * ib) + alloc_granule(nsys * su * ib);                     // inverses
-    if (d.split_inverse) b += 2 * alloc_granule(nsys * sb * 2) + alloc_granule(nsys * su * 2);  // remainders
+    if (d.split_inverse) b += 2 * alloc_granule(nsys * sb) + alloc_granule(nsys * su);  // remainders
     b += 6 * alloc_granule(nmodes * 16);                                                      // keys, rotations, walls
     b += 2 * alloc_granule(nsys * nb * nb * 8) + alloc_granule(nsys * nu * nu * 8) +
          alloc_granule(nsys * (2 * nb + nu) * 4);                                             // FP64 LU scratch

@@ -22,8 +22,8 @@ struct Channel3Desc {
     int stepper;             // kSBDF2 / kSBDF1 (per-mode multistep) or kRK222 / kRK443 (IMEX RK)
     double b_bottom, b_top;  // Dirichlet values of b at z = 0 and z = Lz
     int refine_sweeps = 0;   // FP64-residual refinement sweeps per implicit solve (0..2)
-    // FP32 state only: each inverse entry stored as an FP32 value plus a bfloat16
-    // remainder (2^-33 relative instead of 2^-24; 6 bytes per entry instead of 4)
+    // FP32 state only: each inverse entry stored as an FP32 value plus an 8-bit
+    // remainder (2^-32 relative instead of 2^-24; 5 bytes per entry instead of 4)
     int split_inverse = 0;
 };
 
@@ -31,7 +31,9 @@ enum Channel3Var : int { kC3P = 0, kC3B = 1, kC3UX = 2, kC3UY = 3, kC3UZ = 4 };
 
 // kernel variants picked by the autotuner at creation; every variant gives the same bits
 struct Chan3Tuning {
-    int solve = 0;  // k3_solve workgroup: 0 four wavefronts, 1 eight
+    // k3_solve: 0 four wavefronts, 1 eight, 2 four with a 3-deep inverse-load ring (small
+    // size class without refinement sweeps; otherwise as 0)
+    int solve = 0;
     int zbwd = 0;   // k3_zbwd mode tile: 0 the type's default (32 FP32 / 16 FP64), 1 16 modes (FP32 only)
 };
 
@@ -106,7 +108,7 @@ class Channel3Solver {
     DevBuf kh_, rot_, modes_, chunks_;
     DevBuf bcv_, bch_;  // per-mode wall values of b [2 m + w], and the x-transform scratch
     // explicit inverses per (key, parity) of the b, perp and u-p systems: FP64 column-major,
-    // or FP32 packed (pk_index) with the bfloat16 remainders in the *lo_ buffers when split
+    // or FP32 packed (pk_index) with the 8-bit remainders in the *lo_ buffers when split
     DevBuf Ib_, Ip_, Iu_, Iblo_, Iplo_, Iulo_, info_;
     size_t sb_ = 0, su_ = 0;  // per-system strides of the inverses
     Chan3Tuning tune_;

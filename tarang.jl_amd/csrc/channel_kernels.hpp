@@ -1192,30 +1192,36 @@ __host__ __device__ inline size_t pk_index(int n, int i, int k) {
     return base + (size_t)16 * R * KG + (size_t)(ks - 4 * KG) * 4 * R + (size_t)(q4 * R + r16);
 }
 
-// bfloat16 remainder of an FP32-rounded FP64 value: v ~ float(v) + bf16(v - float(v)) to
-// 2^-33 relative (the remainder is below 2^-24 |v| and keeps 8 significant bits)
-__device__ inline unsigned short bf16_rn(float f) {
-    unsigned u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (finite values)
-    return (unsigned short)(u >> 16);
+// Remainder of an FP32-rounded FP64 value v, f = float(v), as a signed 8-bit multiple q of
+// 2^-31 f: v ~ f (1 + q 2^-31).  |v - f| <= 2^-24 |f| puts q in [-128, 128] (clamped to
+// 127), and the pair (f, q) holds v to 2^-32 relative in 5 bytes.
+__device__ inline signed char lo8_encode(double v, float f) {
+    if (f == 0.0f) return 0;
+    const double q = rint((v - (double)f) / (double)f * 0x1p31);
+    return (signed char)fmin(fmax(q, -127.0), 127.0);
 }
-__device__ inline float bf16_to_f32(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+// f (1 + q 2^-31) in FP64: the remainder f q 2^-31 is formed in FP32 (its rounding is
+// 2^-48 |f|), the sum in FP64
+__device__ inline double lo8_value(float f, signed char q) {
+    return (double)f + (double)(f * ((float)q * 0x1p-31f));
+}
 
 // Explicit inverse from the LU factors: workgroup (system w, column block cb) solves
 // A x = e_c for the NR columns c = NR cb .. NR cb + NR - 1 in FP64 and stores x,
 // rounded to OT, as column c of the inverse (column-major with leading dimension n, or
 // packed by pk_index when PACK; stride istride per system).  lo, when not null, receives
-// the bfloat16 remainder of every FP32-rounded entry: packed, at the entry's own offset;
-// column-major, in column pairs (lo_pair_index, stride lo_pair_stride per system).
+// the 8-bit remainder (lo8_encode) of every FP32-rounded entry: packed, at the entry's own
+// offset; column-major, in groups of four columns (lo_quad_index, stride lo_quad_stride
+// per system).
 // LDS: NR ld + max(NR ld, 4096) doubles, ld >= n.
-__host__ __device__ inline size_t lo_pair_stride(int nmax) { return (size_t)nmax * ((nmax + 1) & ~1); }
-// remainders of columns 2j and 2j+1 of row i side by side: one 32-bit load per lane and pair
-__host__ __device__ inline size_t lo_pair_index(int n, int i, int c) {
-    return ((size_t)(c >> 1) * n + i) * 2 + (c & 1);
+__host__ __device__ inline size_t lo_quad_stride(int nmax) { return (size_t)nmax * ((nmax + 3) & ~3); }
+// remainders of columns 4j .. 4j+3 of row i side by side: one 32-bit load per lane and quad
+__host__ __device__ inline size_t lo_quad_index(int n, int i, int c) {
+    return ((size_t)(c >> 2) * n + i) * 4 + (c & 3);
 }
 template <int NT, typename OT, int NR, bool PACK = false>
 __global__ __launch_bounds__(NT) void k_lu_inverse(const double* __restrict__ LU, const int* __restrict__ perm,
-                                                   OT* __restrict__ inv, unsigned short* __restrict__ lo, int N,
+                                                   OT* __restrict__ inv, signed char* __restrict__ lo, int N,
                                                    int kind, size_t stride, size_t pstride, size_t istride, int ld) {
     extern __shared__ unsigned char smem_raw[];
     const int w = blockIdx.y, m = w >> 1, g = w & 1;  // the column blocks of one system run side by side
@@ -1231,13 +1237,13 @@ __global__ __launch_bounds__(NT) void k_lu_inverse(const double* __restrict__ LU
     __syncthreads();
     lu_solve_blocked<NT, NR>(LU + (size_t)w * stride, perm + (size_t)w * pstride, n, y, ld, tmp, tmp);
     OT* out = inv + (size_t)w * istride;
-    unsigned short* olo = lo ? lo + (size_t)w * (PACK ? istride : lo_pair_stride(ld)) : nullptr;
+    signed char* olo = lo ? lo + (size_t)w * (PACK ? istride : lo_quad_stride(ld)) : nullptr;
     for (int r = 0; r < NR && c0 + r < n; ++r)
         for (int i = threadIdx.x; i < n; i += NT) {
             const double v = y[r * ld + i];
             const size_t o = PACK ? pk_index(n, i, c0 + r) : (size_t)(c0 + r) * n + i;
             out[o] = OT(v);
-            if (olo) olo[PACK ? o : lo_pair_index(n, i, c0 + r)] = bf16_rn((float)(v - (double)OT(v)));
+            if (olo) olo[PACK ? o : lo_quad_index(n, i, c0 + r)] = lo8_encode(v, (float)OT(v));
         }
 }
 
@@ -1252,10 +1258,10 @@ __host__ __device__ inline size_t inverse_stride(int nmax, bool pack) {
 
 // Form the inverses of nsys systems of one kind (0: b-like, 1: u-p) from their LU
 // factors, 8 right-hand sides per workgroup, column-major or packed (stride
-// inverse_stride(nmax, pack)), with the bfloat16 remainders in lo when it is not null.
+// inverse_stride(nmax, pack)), with the 8-bit remainders in lo when it is not null.
 template <int NT, typename OT>
 inline void lu_invert(hipStream_t s, const double* LU, const int* perm, OT* inv, int N, int kind, unsigned nsys,
-                      size_t stride, size_t pstride, int nmax, bool pack = false, unsigned short* lo = nullptr) {
+                      size_t stride, size_t pstride, int nmax, bool pack = false, signed char* lo = nullptr) {
     // 8 right-hand sides per workgroup: 16 would need ~250 VGPRs (one wavefront per SIMD)
     auto launch = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3((unsigned)((nmax + 7) / 8), nsys), dim3(NT),
@@ -1284,19 +1290,20 @@ __device__ inline dbl4 mfma_f64_16x16x4(double a, double b, dbl4 c) {
 // (t + wave0) % nw; acc[j] holds tile wave-local j.  Entries outside A count as zero
 // (their loads read element 0).  The k loop runs in batches of U steps whose loads
 // are all issued before the batch's products (NTW = this wave's tile count, so the
-// batch has no branches).
+// batch has no branches), from a ring of D load buffers: D - 1 batches are in flight
+// while one batch's products run.
 constexpr int kYS = 17;
 
-template <int NTW, int U, typename AT>
+template <int NTW, int U, typename AT, int D = 2>
 __device__ inline void mfma_apply_n(const AT* __restrict__ A, int n, const double* Y, dbl4* acc, int t0, int nw) {
     const int lane = threadIdx.x & 63, r16 = lane & 15, q4 = lane >> 4;
     const int ksteps = (n + 3) >> 2, krow_max = ((n + 15) & ~15) - 1;
     int ii[NTW];
 #pragma unroll
     for (int j = 0; j < NTW; ++j) ii[j] = 16 * (t0 + j * nw) + r16;
-    // two load buffers in turn: the loads of one batch are in flight while the other's
-    // products run (no copies between them, so no wait is forced early)
-    AT ra[U][NTW], rb[U][NTW];
+    // the ring's buffers are indexed by unrolled constants (no copies between them, so no
+    // wait is forced early)
+    AT ring[D][U][NTW];
     auto load = [&](int ks, AT (&raw)[U][NTW]) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -1320,18 +1327,24 @@ __device__ inline void mfma_apply_n(const AT* __restrict__ A, int n, const doubl
             }
         }
     };
-    load(0, ra);
-    for (int ks = 0; ks < ksteps; ks += 2 * U) {
-        load(ks + U, rb);  // past the end these read element 0 and are never used
-        multiply(ks, ra);
-        if (ks + U >= ksteps) break;  // wave-uniform
-        load(ks + 2 * U, ra);
-        multiply(ks + U, rb);
+    const int nbat = (ksteps + U - 1) / U;  // wave-uniform
+#pragma unroll
+    for (int d = 0; d < D - 1; ++d)
+        if (d < nbat) load(d * U, ring[d]);
+    for (int b0 = 0; b0 < nbat; b0 += D) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int b = b0 + d;
+            if (b >= nbat) break;
+            if (b + D - 1 < nbat) load((b + D - 1) * U, ring[(d + D - 1) % D]);
+            multiply(b * U, ring[d]);
+        }
     }
 }
 
-// accumulate = true adds A Y to acc instead of overwriting it (a refinement correction)
-template <int MT, typename AT>
+// accumulate = true adds A Y to acc instead of overwriting it (a refinement correction);
+// D: load-ring depth of the waves with up to four tiles (more tiles keep two buffers)
+template <int MT, typename AT, int D = 2>
 __device__ inline void mfma_apply(const AT* __restrict__ A, int n, const double* Y, dbl4 (&acc)[MT], int wave,
                                   int nw, int wave0, bool accumulate = false) {
     const int ntiles = (n + 15) >> 4;
@@ -1342,10 +1355,10 @@ __device__ inline void mfma_apply(const AT* __restrict__ A, int n, const double*
         for (int j = 0; j < MT; ++j) acc[j] = dbl4{0.0, 0.0, 0.0, 0.0};
     constexpr bool F32 = sizeof(AT) == 4;  // batch depth: both load buffers stay within the registers
     switch (ntw < MT ? ntw : MT) {         // wave-uniform
-        case 1: mfma_apply_n<1, 4>(A, n, Y, acc, t0, nw); break;
-        case 2: if constexpr (MT >= 2) mfma_apply_n<2, 4>(A, n, Y, acc, t0, nw); break;
-        case 3: if constexpr (MT >= 3) mfma_apply_n<3, (F32 ? 4 : 2)>(A, n, Y, acc, t0, nw); break;
-        case 4: if constexpr (MT >= 4) mfma_apply_n<4, (F32 ? 4 : 2)>(A, n, Y, acc, t0, nw); break;
+        case 1: mfma_apply_n<1, 4, AT, D>(A, n, Y, acc, t0, nw); break;
+        case 2: if constexpr (MT >= 2) mfma_apply_n<2, 4, AT, D>(A, n, Y, acc, t0, nw); break;
+        case 3: if constexpr (MT >= 3) mfma_apply_n<3, (F32 ? 4 : 2), AT, D>(A, n, Y, acc, t0, nw); break;
+        case 4: if constexpr (MT >= 4) mfma_apply_n<4, (F32 ? 4 : 2), AT, D>(A, n, Y, acc, t0, nw); break;
         case 5: if constexpr (MT >= 5) mfma_apply_n<5, 2>(A, n, Y, acc, t0, nw); break;
         case 6: if constexpr (MT >= 6) mfma_apply_n<6, 2>(A, n, Y, acc, t0, nw); break;
         case 7: if constexpr (MT >= 7) mfma_apply_n<7, 2>(A, n, Y, acc, t0, nw); break;
@@ -1353,21 +1366,21 @@ __device__ inline void mfma_apply(const AT* __restrict__ A, int n, const double*
     }
 }
 
-// X = A Y as mfma_apply, for A packed by pk_index with FP32 entries and, when LO, a
-// bfloat16 remainder per entry at the same offset in L (entry = A + L, formed in FP64).
-// Each lane loads W = 4 (16 bytes of A, 8 of L) or W = 2 consecutive values of its
+// X = A Y as mfma_apply, for A packed by pk_index with FP32 entries and, when LO, an
+// 8-bit remainder per entry at the same offset in L (entry = lo8_value(A, L), in FP64).
+// Each lane loads W = 4 (16 bytes of A, 4 of L) or W = 2 consecutive values of its
 // group of four k-steps at a time; W = 2 halves the load buffers for waves with many
-// tiles.
+// tiles.  D load buffers in a ring, as in mfma_apply_n.
 typedef float flt4 __attribute__((ext_vector_type(4)));
-typedef unsigned short us4 __attribute__((ext_vector_type(4)));
+typedef signed char sc4 __attribute__((ext_vector_type(4)));
 typedef float flt2 __attribute__((ext_vector_type(2)));
-typedef unsigned short us2 __attribute__((ext_vector_type(2)));
+typedef signed char sc2 __attribute__((ext_vector_type(2)));
 template <int W> struct PkVec;
-template <> struct PkVec<4> { using F = flt4; using U = us4; };
-template <> struct PkVec<2> { using F = flt2; using U = us2; };
+template <> struct PkVec<4> { using F = flt4; using U = sc4; };
+template <> struct PkVec<2> { using F = flt2; using U = sc2; };
 
-template <int NTW, bool LO, int W>
-__device__ inline void mfma_apply_pk_n(const float* __restrict__ A, const unsigned short* __restrict__ L, int n,
+template <int NTW, bool LO, int W, int D = 2>
+__device__ inline void mfma_apply_pk_n(const float* __restrict__ A, const signed char* __restrict__ L, int n,
                                        const double* Y, dbl4* acc, int t0, int nw) {
     using F = typename PkVec<W>::F;
     using UV = typename PkVec<W>::U;
@@ -1383,10 +1396,10 @@ __device__ inline void mfma_apply_pk_n(const float* __restrict__ A, const unsign
         tb[j] = 64u * (unsigned)t * (unsigned)K;
         lo16[j] = r16 < R[j] ? (unsigned)(q4 * R[j] + r16) * 4u : 0u;  // lanes past a short tile read its first group
     }
-    // two load buffers in turn, as in mfma_apply_n; part p = group p / PARTS, values
+    // a ring of D load buffers, as in mfma_apply_n; part p = group p / PARTS, values
     // W (p % PARTS) .. W (p % PARTS) + W - 1 of the lane's four
-    F ha[NTW], hb[NTW];
-    UV la[NTW], lb[NTW];
+    F hr[D][NTW];
+    UV lr[D][NTW];
     auto load = [&](int p, F (&h)[NTW], UV (&l)[NTW]) {
         const unsigned gi = (unsigned)(p / PARTS), sub = (unsigned)(W * (p % PARTS));
 #pragma unroll
@@ -1403,20 +1416,23 @@ __device__ inline void mfma_apply_pk_n(const float* __restrict__ A, const unsign
             const double b = Y[(k0 + 4 * u) * kYS + r16];
 #pragma unroll
             for (int j = 0; j < NTW; ++j) {
-                double a = (double)h[j][u];
-                if constexpr (LO) a += (double)bf16_to_f32(l[j][u]);
+                double a;
+                if constexpr (LO) a = lo8_value(h[j][u], l[j][u]);
+                else a = (double)h[j][u];
                 acc[j] = mfma_f64_16x16x4(r16 < R[j] ? a : 0.0, b, acc[j]);
             }
         }
     };
-    if (NP > 0) {  // wave-uniform
-        load(0, ha, la);
-        for (int p = 0; p < NP; p += 2) {
-            if (p + 1 < NP) load(p + 1, hb, lb);
-            multiply(p, ha, la);
-            if (p + 1 >= NP) break;
-            if (p + 2 < NP) load(p + 2, ha, la);
-            multiply(p + 1, hb, lb);
+#pragma unroll
+    for (int d = 0; d < D - 1; ++d)  // wave-uniform conditions throughout
+        if (d < NP) load(d, hr[d], lr[d]);
+    for (int p0 = 0; p0 < NP; p0 += D) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int p = p0 + d;
+            if (p >= NP) break;
+            if (p + D - 1 < NP) load(p + D - 1, hr[(d + D - 1) % D], lr[(d + D - 1) % D]);
+            multiply(p, hr[d], lr[d]);
         }
     }
     for (int v = 0; v < K - 4 * KG; ++v) {  // trailing k-steps, one value per lane
@@ -1426,15 +1442,16 @@ __device__ inline void mfma_apply_pk_n(const float* __restrict__ A, const unsign
             const bool in = r16 < R[j];
             const unsigned off = tb[j] + 16u * (unsigned)(R[j] * KG) + (unsigned)(v * 4 * R[j]) +
                                  (in ? (unsigned)(q4 * R[j] + r16) : 0u);
-            double a = (double)A[off];
-            if constexpr (LO) a += (double)bf16_to_f32(L[off]);
+            double a;
+            if constexpr (LO) a = lo8_value(A[off], L[off]);
+            else a = (double)A[off];
             acc[j] = mfma_f64_16x16x4(in ? a : 0.0, b, acc[j]);
         }
     }
 }
 
-template <int MT, bool LO>
-__device__ inline void mfma_apply_pk(const float* __restrict__ A, const unsigned short* __restrict__ L, int n,
+template <int MT, bool LO, int D = 2>
+__device__ inline void mfma_apply_pk(const float* __restrict__ A, const signed char* __restrict__ L, int n,
                                      const double* Y, dbl4 (&acc)[MT], int wave, int nw, int wave0,
                                      bool accumulate = false) {
     const int ntiles = (n + 15) >> 4;
@@ -1444,10 +1461,10 @@ __device__ inline void mfma_apply_pk(const float* __restrict__ A, const unsigned
 #pragma unroll
         for (int j = 0; j < MT; ++j) acc[j] = dbl4{0.0, 0.0, 0.0, 0.0};
     switch (ntw < MT ? ntw : MT) {  // wave-uniform
-        case 1: mfma_apply_pk_n<1, LO, 4>(A, L, n, Y, acc, t0, nw); break;
-        case 2: if constexpr (MT >= 2) mfma_apply_pk_n<2, LO, 4>(A, L, n, Y, acc, t0, nw); break;
-        case 3: if constexpr (MT >= 3) mfma_apply_pk_n<3, LO, 4>(A, L, n, Y, acc, t0, nw); break;
-        case 4: if constexpr (MT >= 4) mfma_apply_pk_n<4, LO, 4>(A, L, n, Y, acc, t0, nw); break;
+        case 1: mfma_apply_pk_n<1, LO, 4, D>(A, L, n, Y, acc, t0, nw); break;
+        case 2: if constexpr (MT >= 2) mfma_apply_pk_n<2, LO, 4, D>(A, L, n, Y, acc, t0, nw); break;
+        case 3: if constexpr (MT >= 3) mfma_apply_pk_n<3, LO, 4, D>(A, L, n, Y, acc, t0, nw); break;
+        case 4: if constexpr (MT >= 4) mfma_apply_pk_n<4, LO, 4, D>(A, L, n, Y, acc, t0, nw); break;
         case 5: if constexpr (MT >= 5) mfma_apply_pk_n<5, LO, 2>(A, L, n, Y, acc, t0, nw); break;
         case 6: if constexpr (MT >= 6) mfma_apply_pk_n<6, LO, 2>(A, L, n, Y, acc, t0, nw); break;
         case 7: if constexpr (MT >= 7) mfma_apply_pk_n<7, LO, 2>(A, L, n, Y, acc, t0, nw); break;
@@ -1473,9 +1490,9 @@ __device__ inline void mfma_store(const dbl4 (&acc)[MT], int n, double* X, int w
 struct ChSolveArgs {
     const void* Ib;     // b-system inverses (state precision), stride sb per (mode, parity), column-major, ld = n
     const void* Iu;     // u-p-system inverses, stride su
-    // bfloat16 remainders of FP32 inverses (split inverses) in column pairs, per-system
-    // strides sblo / sulo; null otherwise
-    const unsigned short *Iblo, *Iulo;
+    // 8-bit remainders of FP32 inverses (split inverses) in column quads (lo_quad_index),
+    // per-system strides sblo / sulo; null otherwise
+    const signed char *Iblo, *Iulo;
     size_t sblo, sulo;
     int N, nxh;
     size_t sb, su;
@@ -1535,11 +1552,11 @@ inline std::vector<double2> constant_wall_values(size_t nmodes, double scale, do
 // x[r*ld + i] = sum_k A[k*n + i] y[r*ld + k], r = 0, 1 (real and imaginary part), for
 // a column-major n x n matrix A (global) and y, x in LDS; thread t owns rows t + j NT
 // (j < RT), FP64 accumulation, loads of U columns issued together.  LO: the entries are
-// A + L with L the bfloat16 remainders of FP32 entries in column pairs (lo_pair_index).
+// lo8_value(A, L) with L the 8-bit remainders of FP32 entries in column quads (lo_quad_index).
 template <int NT, int RT, int U, typename AT, bool LO = false>
-__device__ inline void gemv2_rows(const AT* __restrict__ A, const unsigned short* __restrict__ L, int n,
+__device__ inline void gemv2_rows(const AT* __restrict__ A, const signed char* __restrict__ L, int n,
                                   const double* y, int ld, double* x) {
-    static_assert(U % 2 == 0, "column pairs of remainders");
+    static_assert(U % 4 == 0, "column quads of remainders");
     double acc[RT][2];
     int row[RT];
 #pragma unroll
@@ -1549,7 +1566,7 @@ __device__ inline void gemv2_rows(const AT* __restrict__ A, const unsigned short
     }
     for (int k0 = 0; k0 < n; k0 += U) {
         AT raw[U][RT];  // every load of the batch is issued before any is used
-        unsigned rlo[U / 2][RT];
+        unsigned rlo[U / 4][RT];
 #pragma unroll
         for (int u = 0; u < U; ++u)
 #pragma unroll
@@ -1557,8 +1574,8 @@ __device__ inline void gemv2_rows(const AT* __restrict__ A, const unsigned short
                 const bool in = k0 + u < n && row[r] < n;
                 raw[u][r] = A[in ? (k0 + u) * n + row[r] : 0];  // n <= 1024: 32-bit offsets
                 if constexpr (LO)
-                    if (u % 2 == 0)
-                        rlo[u / 2][r] = reinterpret_cast<const unsigned*>(L)[in ? ((k0 + u) >> 1) * n + row[r] : 0];
+                    if (u % 4 == 0)
+                        rlo[u / 4][r] = reinterpret_cast<const unsigned*>(L)[in ? ((k0 + u) >> 2) * n + row[r] : 0];
             }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -1566,8 +1583,9 @@ __device__ inline void gemv2_rows(const AT* __restrict__ A, const unsigned short
             const double y0 = y[k], y1 = y[ld + k];
 #pragma unroll
             for (int r = 0; r < RT; ++r) {
-                double a = (double)raw[u][r];
-                if constexpr (LO) a += (double)__uint_as_float(u % 2 ? rlo[u / 2][r] & 0xFFFF0000u : rlo[u / 2][r] << 16);
+                double a;
+                if constexpr (LO) a = lo8_value(raw[u][r], (signed char)(rlo[u / 4][r] >> (8 * (u % 4))));
+                else a = (double)raw[u][r];
                 a = (k0 + u < n && row[r] < n) ? a : 0.0;
                 acc[r][0] += a * y0;
                 acc[r][1] += a * y1;
@@ -1585,7 +1603,7 @@ __device__ inline void gemv2_rows(const AT* __restrict__ A, const unsigned short
 // MAXRT: the largest rows-per-thread case compiled in (n <= MAXRT NT); a smaller bound
 // keeps the register count low enough for more workgroups per CU
 template <int NT, bool LO = false, int MAXRT = 4, typename AT>
-__device__ inline void gemv2(const AT* __restrict__ A, const unsigned short* __restrict__ L, int n, const double* y,
+__device__ inline void gemv2(const AT* __restrict__ A, const signed char* __restrict__ L, int n, const double* y,
                              int ld, double* x) {
     switch ((n + NT - 1) / NT) {  // workgroup-uniform
         case 1: gemv2_rows<NT, 1, 8, AT, LO>(A, L, n, y, ld, x); break;
@@ -1622,7 +1640,7 @@ __global__ __launch_bounds__(NT) void k_ch_solve(ChSolveArgs s, const cplx<T>* _
     const double kh = s.k0x * m;
     // x += A^-1 (y - A x), sweeps times; `resid` forms y - A x of one column into r
     static_assert(!SPLIT || sizeof(T) == 4, "split inverses belong to the FP32 state");
-    auto refine = [&](const T* Ainv, const unsigned short* Alo, int n, auto resid) {
+    auto refine = [&](const T* Ainv, const signed char* Alo, int n, auto resid) {
         for (int sw = 0; REFINE && sw < s.sweeps; ++sw) {
             for (int i = threadIdx.x; i < 2 * ld; i += NT) r[i] = x[i];
             __syncthreads();
@@ -1650,7 +1668,7 @@ __global__ __launch_bounds__(NT) void k_ch_solve(ChSolveArgs s, const cplx<T>* _
     }
     __syncthreads();
     const size_t ob = (size_t)(2 * m + h) * s.sb;
-    const unsigned short* blo = SPLIT ? s.Iblo + (size_t)(2 * m + h) * s.sblo : nullptr;
+    const signed char* blo = SPLIT ? s.Iblo + (size_t)(2 * m + h) * s.sblo : nullptr;
     gemv2<NT, SPLIT, MAXRT>(static_cast<const T*>(s.Ib) + ob, blo, nbh, y, ld, x);
     __syncthreads();
     refine(static_cast<const T*>(s.Ib) + ob, blo, nbh,
@@ -1678,7 +1696,7 @@ __global__ __launch_bounds__(NT) void k_ch_solve(ChSolveArgs s, const cplx<T>* _
     }
     __syncthreads();
     const size_t ou = (size_t)wg * s.su;
-    const unsigned short* ulo = SPLIT ? s.Iulo + (size_t)wg * s.sulo : nullptr;
+    const signed char* ulo = SPLIT ? s.Iulo + (size_t)wg * s.sulo : nullptr;
     gemv2<NT, SPLIT, MAXRT>(static_cast<const T*>(s.Iu) + ou, ulo, n, y, ld, x);
     __syncthreads();
     refine(static_cast<const T*>(s.Iu) + ou, ulo, n,

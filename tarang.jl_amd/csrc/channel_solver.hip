@@ -106,9 +106,9 @@ ChannelSolver::ChannelSolver(Context* ctx, const ChannelDesc& d) : ctx_(ctx), d_
     const size_t ib = dtype_size((DType)d.dtype);  // inverses in the state precision
     Ib_ = DevBuf(ctx_, (size_t)2 * nxh_ * nb_max_ * nb_max_ * ib);
     Iu_ = DevBuf(ctx_, (size_t)2 * nxh_ * nu_max_ * nu_max_ * ib);
-    if (d.split_inverse) {  // remainders in column pairs; the odd column's partner stays zero
-        Iblo_ = DevBuf(ctx_, (size_t)2 * nxh_ * lo_pair_stride(nb_max_) * 2);
-        Iulo_ = DevBuf(ctx_, (size_t)2 * nxh_ * lo_pair_stride(nu_max_) * 2);
+    if (d.split_inverse) {  // remainders in column quads; the slots past the last column stay zero
+        Iblo_ = DevBuf(ctx_, (size_t)2 * nxh_ * lo_quad_stride(nb_max_));
+        Iulo_ = DevBuf(ctx_, (size_t)2 * nxh_ * lo_quad_stride(nu_max_));
         TG_HIP(hipMemsetAsync(Iblo_.ptr, 0, Iblo_.bytes, ctx_->stream));
         TG_HIP(hipMemsetAsync(Iulo_.ptr, 0, Iulo_.bytes, ctx_->stream));
     }
@@ -238,9 +238,9 @@ void ChannelSolver::factor(double c) {
     auto invert = [&](auto ot) {
         using OT = decltype(ot);
         lu_invert<NT, OT>(s, Lb.as<const double>(), pb.as<const int>(), Ib_.as<OT>(), N, 0, nsys, sb, nb_max_, nb_max_,
-                          false, Iblo_.as<unsigned short>());
+                          false, Iblo_.as<signed char>());
         lu_invert<NT, OT>(s, Lu.as<const double>(), pu.as<const int>(), Iu_.as<OT>(), N, 1, nsys, su, nu_max_, nu_max_,
-                          false, Iulo_.as<unsigned short>());
+                          false, Iulo_.as<signed char>());
     };
     if (d_.dtype == kF32) invert(float{});
     else invert(double{});
@@ -360,8 +360,8 @@ template <typename T>
 void ChannelSolver::solve_t() {
     using V = cplx<T>;
     const int N = d_.nz;
-    ChSolveArgs sa{Ib_.ptr, Iu_.ptr, Iblo_.as<const unsigned short>(), Iulo_.as<const unsigned short>(),
-                   lo_pair_stride(nb_max_), lo_pair_stride(nu_max_), N, nxh_, (size_t)nb_max_ * nb_max_, (size_t)nu_max_ * nu_max_, nu_max_,
+    ChSolveArgs sa{Ib_.ptr, Iu_.ptr, Iblo_.as<const signed char>(), Iulo_.as<const signed char>(),
+                   lo_quad_stride(nb_max_), lo_quad_stride(nu_max_), N, nxh_, (size_t)nb_max_ * nb_max_, (size_t)nu_max_ * nu_max_, nu_max_,
                    factored_c_, bcv_.as<const double2>(), d_.refine_sweeps, 2.0 * M_PI / d_.Lx, d_.kappa, d_.nu,
                    2.0 / d_.Lz, d_.Lx, d_.Lz};
     constexpr int NT = 256;
@@ -589,7 +589,7 @@ void ChannelSolver::kernel_bytes(double* out) const {
     for (int m = 0; m < nxh_; ++m)
         for (int g = 0; g < 2; ++g) {
             const double nb = chb_size(d_.nz, g), nu = chu_size(d_.nz, g, m);
-            fac += (nb * nb + nu * nu) * (rb + (d_.split_inverse ? 2.0 : 0.0));
+            fac += (nb * nb + nu * nu) * (rb + (d_.split_inverse ? 1.0 : 0.0));
         }
     out[4] = fac + 8 * nc * vb;
     out[5] = 0;
@@ -613,7 +613,7 @@ size_t footprint(const ChannelDesc& d) {
     b += alloc_granule(nxh * 8 * vb) + 3 * alloc_granule(N * N * ib);                       // taus, tables
     b += alloc_granule(nsys * nb * nb * ib) + alloc_granule(nsys * nu * nu * ib);            // inverses
     if (d.split_inverse)  // remainders
-        b += alloc_granule(nsys * lo_pair_stride((int)nb) * 2) + alloc_granule(nsys * lo_pair_stride((int)nu) * 2);
+        b += alloc_granule(nsys * lo_quad_stride((int)nb)) + alloc_granule(nsys * lo_quad_stride((int)nu));
     b += 3 * alloc_granule(1);                                                               // info, walls
     // FP64 LU scratch and pivots while the inverses are formed
     b += alloc_granule(nsys * nb * nb * 8) + alloc_granule(nsys * nu * nu * 8) + alloc_granule(nsys * (nb + nu) * 4);

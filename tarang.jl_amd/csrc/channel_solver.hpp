@@ -20,8 +20,8 @@ struct ChannelDesc {
     int stepper;         // kRK222 / kRK443 / kSBDF2 / kSBDF1
     double b_bottom, b_top;  // Dirichlet values of b at z = 0 and z = Lz
     int refine_sweeps = 0;   // FP64-residual refinement sweeps per implicit solve (0..2)
-    // FP32 state only: each inverse entry stored as an FP32 value plus a bfloat16
-    // remainder (2^-33 relative instead of 2^-24; 6 bytes per entry instead of 4)
+    // FP32 state only: each inverse entry stored as an FP32 value plus an 8-bit
+    // remainder (2^-32 relative instead of 2^-24; 5 bytes per entry instead of 4)
     int split_inverse = 0;
 };
 
@@ -100,7 +100,7 @@ class ChannelSolver {
     DevBuf BT_, BDT_, FT_;
     DevBuf bcv_, bch_;  // per-mode wall values of b [2 m + w], and the x-transform scratch
     // explicit inverses per (mode, parity) of the b and u-p systems, state precision, with
-    // the bfloat16 remainders of FP32 entries in the *lo_ buffers when split
+    // the 8-bit remainders of FP32 entries in the *lo_ buffers when split
     DevBuf Ib_, Iu_, Iblo_, Iulo_, info_;
     bool graph_on_ = true;
     hipGraphExec_t graph_ = nullptr;

@@ -68,7 +68,7 @@ class InitialValueSolver:
     Keyword arguments mirror the reference (`dt`, `matsolver`, `batched_modes`,
     ...); `use_graph` (default True) replays each step as one captured hipGraph.
     Channel families only: `split_inverse` (default: on for an FP32 domain) stores the
-    per-mode inverses as FP32 values plus bfloat16 remainders, and `refine_sweeps` (0-2)
+    per-mode inverses as FP32 values plus 8-bit remainders, and `refine_sweeps` (0-2)
     adds FP64-residual refinement sweeps to every implicit solve.
     """
 
@@ -277,7 +277,7 @@ class _ChannelBackend:
         d.dt = solver.dt
         d.dtype = _lib.F32 if dom.dtype == np.float32 else _lib.F64
         d.timestepper = timestepper.code
-        # the FP32 state's default solve form: inverses with bfloat16 remainders (the library
+        # the FP32 state's default solve form: inverses with 8-bit remainders (the library
         # refuses split inverses for an FP64 state, so an explicit request there fails loudly)
         split = (dom.dtype == np.float32) if split_inverse is None else bool(split_inverse)
         d.split_inverse = 1 if split else 0

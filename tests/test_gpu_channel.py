@@ -307,7 +307,7 @@ def test_channel_sbdf_parity(gpu, oracle, ts, dtype):
 @pytest.mark.parametrize("ts", ["RK222", "SBDF2"])
 def test_channel_refinement_sweeps(gpu, oracle, ts, dtype):
     """FP64-residual refinement of the per-mode solves (refine_sweeps 1 and 2) and, in FP32,
-    split inverses (FP32 + bfloat16 remainder, alone and with a sweep) against the oracle
+    split inverses (FP32 + 8-bit remainder, alone and with a sweep) against the oracle
     at the 2-D workload's parameters (Ra = 2e6, Pr = 1, Nz = 129): FP64 stays at 1e-12; in
     FP32 the error of the FP32-rounded inverses is removed, so the state error is at most
     that of the plain FP32 form and within 1e-5."""
@@ -340,7 +340,7 @@ def test_channel_refinement_sweeps(gpu, oracle, ts, dtype):
 def test_channel_fp32_forms_long_run(gpu):
     """The FP32 forms against the FP64 form (itself pinned to the oracle at 1e-12) over 100
     RK222 steps of the 2-D workload's physics (Ra = 2e6, Pr = 1, dt = 1e-2, Nz = 256,
-    conduction plus noise) on 32 x 256: split inverses (FP32 + bfloat16 remainder) and
+    conduction plus noise) on 32 x 256: split inverses (FP32 + 8-bit remainder) and
     refinement sweeps remove the error of the FP32-rounded inverses; b stays within 1e-6
     of FP64."""
     nx, N, Lx, Lz, dt, steps = 32, 256, 4.0, 1.0, 1e-2, 100

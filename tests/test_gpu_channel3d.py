@@ -114,13 +114,13 @@ def test_channel3d_graph_and_keys(gpu, oracle):
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_channel3d_tuning_variants_same_bits(gpu, oracle, dtype):
-    """Every kernel variant the autotuner can pick (k3_solve with 4 or 8 wavefronts, k3_zbwd
-    with 32- or 16-mode tiles) gives the same bits, on a 32 x 32 x 24 state over 3 SBDF2 steps
-    (multi-chunk keys, the split-inverse FP32 form)."""
+    """Every kernel variant the autotuner can pick (k3_solve with 4 or 8 wavefronts or a
+    3-deep inverse-load ring, k3_zbwd with 32- or 16-mode tiles) gives the same bits, on a
+    32 x 32 x 24 state over 3 SBDF2 steps (multi-chunk keys, the split-inverse FP32 form)."""
     nx, ny, N, L = 32, 32, 24, 4.0
     kappa, nu = kp()
     st, _ = random_state3(oracle, nx, ny, N, L, L, 1.0, seed=11)
-    variants = [(0, 0), (1, 0)] + ([(0, 1), (1, 1)] if dtype == np.float32 else [])
+    variants = [(0, 0), (1, 0), (2, 0)] + ([(0, 1), (2, 1)] if dtype == np.float32 else [])
     out = []
     for v in variants:
         print(f"channel3d tuning variant {v} {dtype.__name__}", flush=True)
@@ -141,6 +141,8 @@ def test_channel3d_tuning_variants_same_bits(gpu, oracle, dtype):
     s = gpu.chan3(nx, ny, N, L, L, 1.0, kappa, nu, 5e-3, dtype=np.float64)
     with pytest.raises(_lib.TarangError):  # the 16-mode z tile is an FP32 variant
         _lib.call("tarang_chan3_set_tuning", s.h, (ctypes.c_int * 2)(0, 1))
+    with pytest.raises(_lib.TarangError):
+        _lib.call("tarang_chan3_set_tuning", s.h, (ctypes.c_int * 2)(3, 0))
     s.close()
 
 
@@ -249,7 +251,7 @@ def test_channel3d_rejects_unsupported(gpu):
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_channel3d_refinement_sweeps(gpu, oracle, dtype):
     """FP64-residual refinement of the per-mode solves (refine_sweeps 1 and 2) and, in FP32,
-    split inverses (FP32 + bfloat16 remainder, alone and with a sweep) against the oracle:
+    split inverses (FP32 + 8-bit remainder, alone and with a sweep) against the oracle:
     FP64 stays at 1e-12; in FP32 the solve error of the FP32-rounded inverses is removed,
     so the state error after SBDF2 steps is at most that of the plain FP32 form and within
     1e-5.  Nz = 65 keeps the tau systems as ill-conditioned as the 3-D workload's (condition
@@ -284,7 +286,7 @@ def test_channel3d_fp32_forms_long_run(gpu):
     """The FP32 forms against the FP64 form (itself pinned to the oracle at 1e-12) over 300
     SBDF2 steps of the 3-D workload's physics (Ra = 1e7, Pr = 0.7, dt = 2e-3, Nz = 128,
     conduction plus noise) on a 16 x 16 grid: the FP32-rounded inverses act as a slightly
-    different operator at every step, an error that split inverses (FP32 + bfloat16
+    different operator at every step, an error that split inverses (FP32 + 8-bit
     remainder) and refinement sweeps remove; both stay within 1e-6 of FP64 in b."""
     nx = ny = 16
     N, Lx, Ly, Lz, dt, steps = 128, 4.0, 4.0, 1.0, 2e-3, 300
