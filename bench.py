@@ -482,9 +482,92 @@ def bench_channel(a):
         "kernels_f32": kernels,
         "cpu_baseline": cpu,
     }
+    line["group"] = channel_group_bench(a)
     if a.cfl_horizon:
         line["cfl_horizon"] = channel_cfl_horizon(a)
     return line
+
+
+# a group of uneven 2-D channel problems (nx, nz, Ra): FP32 split inverses, RK222, dt = 1e-3
+GROUP_MEMBERS = [(256, 64, 1e6)] * 4 + [(512, 128, 2e6)] * 4 + [(1024, 64, 5e5)] * 2 + [(128, 256, 1e7)] * 2
+
+
+def channel_group_bench(a):
+    """The GROUP_MEMBERS problems stepped together (tarang_chan_group_step: one per-mode solve
+    launch per implicit stage for all of them, one graph per group step) against the same
+    problems stepped one after another (one graph, with its own solve launches, per problem
+    and step), on one context stream."""
+    import numpy as np
+    import torch
+
+    from tarang_amd import _lib
+
+    dev = torch.device("cuda", 0)
+    ctx = ctypes.c_void_p()
+    _lib.call("tarang_ctx_create", 0, ctypes.byref(ctx))
+    steps = 50
+    out = {}
+    sets = {}
+    for label in ("alone", "group"):
+        hs, keep = [], []
+        for i, (nx, nz, Ra) in enumerate(GROUP_MEMBERS):
+            d = _lib.ChanDesc()
+            d.nx, d.nz, d.Lx, d.Lz = nx, nz, 4.0, 1.0
+            d.kappa, d.nu, d.dt = Ra ** -0.5, Ra ** -0.5, 1e-3
+            d.dtype, d.timestepper, d.b_bottom, d.b_top = _lib.F32, 0, 1.0, 0.0
+            d.refine_sweeps, d.split_inverse = 0, 1
+            h = ctypes.c_void_p()
+            _lib.call("tarang_chan_create", ctx, ctypes.byref(d), ctypes.byref(h))
+            bg, _ = channel_ic(np, nx, nz, 1.0, seed=100 + i)
+            k = np.arange(nz)
+            T = np.cos(np.outer(np.pi * np.arange(nz) / (nz - 1), k)) * ((-1.0) ** k)[None, :]
+            Bc = np.linalg.solve(T, np.fft.rfft(bg, axis=0).T).T
+            Bd = torch.as_tensor(np.ascontiguousarray(Bc.T), device=dev).to(torch.complex64).contiguous()
+            torch.cuda.synchronize()
+            _lib.call("tarang_chan_set_field", h, _lib.CHAN_B, ctypes.c_void_p(Bd.data_ptr()))
+            keep.append(Bd)
+            hs.append(h)
+        sets[label] = (hs, keep)
+    g = ctypes.c_void_p()
+    hs = sets["group"][0]
+    _lib.call("tarang_chan_group_create", (ctypes.c_void_p * len(hs))(*[h.value for h in hs]), len(hs),
+              ctypes.byref(g))
+    points = sum(nx * nz for nx, nz, _ in GROUP_MEMBERS)
+    for label in ("alone", "group"):
+        hs = sets[label][0]
+
+        def run(n):
+            if label == "group":
+                _lib.call("tarang_chan_group_step", g, n)
+            else:
+                for _ in range(n):
+                    for h in hs:
+                        _lib.call("tarang_chan_step", h, 1)
+        run(5)
+        _lib.call("tarang_ctx_sync", ctx)
+        t0 = time.perf_counter()
+        run(steps)
+        _lib.call("tarang_ctx_sync", ctx)
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        out[label] = {"ms_per_step": ms, "grid_point_steps_per_s": points / (ms * 1e-3)}
+        log(f"channel group of {len(GROUP_MEMBERS)} problems, {label}: {ms:.3f} ms per step of the set")
+    # the two sets took the same steps from the same states: every member's state agrees
+    same = True
+    for i, (nx, nz, _) in enumerate(GROUP_MEMBERS):
+        o = [torch.zeros((nz, nx // 2 + 1), dtype=torch.complex64, device=dev) for _ in range(2)]
+        for j, label in enumerate(("alone", "group")):
+            _lib.call("tarang_chan_get_field", sets[label][0][i], _lib.CHAN_B, ctypes.c_void_p(o[j].data_ptr()))
+        _lib.call("tarang_ctx_sync", ctx)
+        same = same and bool(torch.equal(o[0], o[1]))
+    _lib.call("tarang_chan_group_destroy", g)
+    for label in ("alone", "group"):
+        for h in sets[label][0]:
+            _lib.call("tarang_chan_destroy", h)
+    _lib.call("tarang_ctx_destroy", ctx)
+    return {"members": [{"nx": nx, "nz": nz, "Ra": Ra} for nx, nz, Ra in GROUP_MEMBERS],
+            "form": "FP32 split inverses, RK222, dt = 1e-3", "steps": steps, "alone": out["alone"],
+            "group": out["group"], "speedup": out["alone"]["ms_per_step"] / out["group"]["ms_per_step"],
+            "same_bits": same}
 
 
 def channel_cfl_horizon(a):

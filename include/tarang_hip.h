@@ -43,6 +43,7 @@ typedef struct tarang_plan_s* tarang_plan_t;
 typedef struct tarang_fivp_s* tarang_fivp_t;
 typedef struct tarang_chan_s* tarang_chan_t;
 typedef struct tarang_chan3_s* tarang_chan3_t;
+typedef struct tarang_chan_group_s* tarang_chan_group_t;
 typedef struct tarang_server_s* tarang_server_t;
 
 /* ------------------------------------------------------------------------ */
@@ -284,6 +285,18 @@ int tarang_chan_time(tarang_chan_t s, double* t, long* iteration);
 int tarang_chan_profile(tarang_chan_t s, int nsteps, double* avg_ms6, long* counts6);
 /* algorithmic bytes per launch of the same six kernel kinds */
 int tarang_chan_kernel_bytes(tarang_chan_t s, double* bytes6);
+/* Several 2-D channel problems stepped together.  The members share one context, dtype,
+ * timestepper, refine_sweeps and split_inverse, and may differ in nx, nz, Lx, Lz, kappa,
+ * nu, dt and walls.  Every implicit stage solves all members' per-mode tau systems in one
+ * launch (workgroups over the modes of every member), and a group step is one hipGraph
+ * replay.  Each member keeps its own clock and history, gives the same bits as when stepped
+ * alone, and may still be stepped alone; members must outlive the group.  The reference
+ * batches the modes of one problem (src/core/subsystems/mode_batch.jl:522-552); it has no
+ * batching across problems. */
+int tarang_chan_group_create(const tarang_chan_t* members, int n, tarang_chan_group_t* out);
+int tarang_chan_group_step(tarang_chan_group_t g, int nsteps);
+int tarang_chan_group_set_graph(tarang_chan_group_t g, int use_graph);
+int tarang_chan_group_destroy(tarang_chan_group_t g);
 
 /* ------------------------------------------------------------------------ */
 /* 3-D Rayleigh-Benard, RealFourier(x, [0, Lx)) x RealFourier(y, [0, Ly)) x

@@ -30,6 +30,9 @@ struct tarang_chan_s {
 struct tarang_chan3_s {
     std::unique_ptr<tarang::Channel3Solver> solver;
 };
+struct tarang_chan_group_s {
+    std::unique_ptr<tarang::ChannelGroup> group;
+};
 struct tarang_server_s {
     std::unique_ptr<tarang::Server> server;
 };
@@ -551,6 +554,40 @@ int tarang_chan_set_graph(tarang_chan_t s, int use_graph) {
         NEED(s, "solver");
         s->solver->enable_graph(use_graph != 0);
     });
+}
+
+int tarang_chan_group_create(const tarang_chan_t* members, int n, tarang_chan_group_t* out) {
+    return guard([&] {
+        NEED(out, "out");
+        NEED(members, "members");
+        TG_REQUIRE(n >= 1, tarang::kErrInvalid, "channel group: n must be >= 1");
+        std::vector<tarang::ChannelSolver*> v;
+        for (int i = 0; i < n; ++i) {
+            NEED(members[i], "member");
+            v.push_back(members[i]->solver.get());
+        }
+        auto g = std::make_unique<tarang_chan_group_s>();
+        g->group = std::make_unique<tarang::ChannelGroup>(v);
+        *out = g.release();
+    });
+}
+
+int tarang_chan_group_step(tarang_chan_group_t g, int nsteps) {
+    return guard([&] {
+        NEED(g, "group");
+        g->group->step(nsteps);
+    });
+}
+
+int tarang_chan_group_set_graph(tarang_chan_group_t g, int use_graph) {
+    return guard([&] {
+        NEED(g, "group");
+        g->group->enable_graph(use_graph != 0);
+    });
+}
+
+int tarang_chan_group_destroy(tarang_chan_group_t g) {
+    return guard([&] { delete g; });
 }
 
 int tarang_chan_set_dt(tarang_chan_t s, double dt) {

@@ -1624,14 +1624,14 @@ __host__ __device__ inline size_t ch_solve_lds_doubles(int ld, bool refine = fal
 // u-p group g.  Inverses in the state precision T, products in FP64.
 // rhs rows E1, E2, E3x, E3z (mode-major) in; state p, b, ux, uz, taus out.
 // MAXRT: rows per thread of the largest system (2 when every system has at most 2 NT rows)
-template <typename T, int NT, bool REFINE = false, bool SPLIT = false, int MAXRT = 4>
-__global__ __launch_bounds__(NT) void k_ch_solve(ChSolveArgs s, const cplx<T>* __restrict__ R1,
-                                                 const cplx<T>* __restrict__ R2, const cplx<T>* __restrict__ R3x,
-                                                 const cplx<T>* __restrict__ R3z, cplx<T>* P, cplx<T>* B, cplx<T>* UX,
-                                                 cplx<T>* UZ, cplx<T>* taus) {
+template <typename T, int NT, bool REFINE, bool SPLIT, int MAXRT>
+__device__ inline void ch_solve_body(const ChSolveArgs& s, const cplx<T>* __restrict__ R1,
+                                     const cplx<T>* __restrict__ R2, const cplx<T>* __restrict__ R3x,
+                                     const cplx<T>* __restrict__ R3z, cplx<T>* P, cplx<T>* B, cplx<T>* UX,
+                                     cplx<T>* UZ, cplx<T>* taus, int wg) {
     extern __shared__ unsigned char smem_raw[];
     const int N = s.N, ld = s.ld;
-    const int wg = blockIdx.x, m = wg >> 1, g = wg & 1, h = g ^ 1;
+    const int m = wg >> 1, g = wg & 1, h = g ^ 1;
     const int PL = (N - 1) & 1;
     double* y = reinterpret_cast<double*>(smem_raw);  // [2][ld] right-hand sides
     double* x = y + 2 * ld;                             // [2][ld] solutions
@@ -1716,6 +1716,41 @@ __global__ __launch_bounds__(NT) void k_ch_solve(ChSolveArgs s, const cplx<T>* _
         }
         if (m != 0 && g == 0) tm[0] = cplx<T>{T(0), T(0)};
     }
+}
+
+template <typename T, int NT, bool REFINE = false, bool SPLIT = false, int MAXRT = 4>
+__global__ __launch_bounds__(NT) void k_ch_solve(ChSolveArgs s, const cplx<T>* __restrict__ R1,
+                                                 const cplx<T>* __restrict__ R2, const cplx<T>* __restrict__ R3x,
+                                                 const cplx<T>* __restrict__ R3z, cplx<T>* P, cplx<T>* B, cplx<T>* UX,
+                                                 cplx<T>* UZ, cplx<T>* taus) {
+    ch_solve_body<T, NT, REFINE, SPLIT, MAXRT>(s, R1, R2, R3x, R3z, P, B, UX, UZ, taus, (int)blockIdx.x);
+}
+
+// One problem of a group solve: its solve arguments, right-hand-side rows and outputs,
+// and the first workgroup of its 2 nxh.
+template <typename T>
+struct ChSolveItem {
+    ChSolveArgs s;
+    const cplx<T>* R[4];  // E1, E2, E3x, E3z rows
+    cplx<T>* out[5];      // p, b, ux, uz, taus
+    int wg0;
+};
+
+// The per-mode solves of several problems (uneven nx and nz, their own dt, diffusivities
+// and inverses) in one launch: workgroup w belongs to the last item with wg0 <= w (items
+// sorted by wg0); LDS sized for the largest system of the group.
+template <typename T, int NT, bool REFINE = false, bool SPLIT = false, int MAXRT = 4>
+__global__ __launch_bounds__(NT) void k_ch_solve_group(const ChSolveItem<T>* __restrict__ items, int nitems) {
+    const int w = (int)blockIdx.x;
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {  // workgroup-uniform binary search
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].wg0 <= w) lo = mid;
+        else hi = mid - 1;
+    }
+    const ChSolveItem<T>& it = items[lo];
+    ch_solve_body<T, NT, REFINE, SPLIT, MAXRT>(it.s, it.R[0], it.R[1], it.R[2], it.R[3], it.out[0], it.out[1],
+                                               it.out[2], it.out[3], it.out[4], w - it.wg0);
 }
 
 // reference layout (kx fastest, (nxh, N)) <-> mode-major (N contiguous per mode)

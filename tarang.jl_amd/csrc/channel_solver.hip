@@ -356,14 +356,17 @@ void ChannelSolver::stage_rhs_t(int st, bool final_update) {
     TG_HIP(hipGetLastError());
 }
 
+ChSolveArgs ChannelSolver::solve_args() const {
+    return ChSolveArgs{Ib_.ptr, Iu_.ptr, Iblo_.as<const signed char>(), Iulo_.as<const signed char>(),
+                       lo_quad_stride(nb_max_), lo_quad_stride(nu_max_), d_.nz, nxh_, (size_t)nb_max_ * nb_max_,
+                       (size_t)nu_max_ * nu_max_, nu_max_, factored_c_, bcv_.as<const double2>(), d_.refine_sweeps,
+                       2.0 * M_PI / d_.Lx, d_.kappa, d_.nu, 2.0 / d_.Lz, d_.Lx, d_.Lz};
+}
+
 template <typename T>
 void ChannelSolver::solve_t() {
     using V = cplx<T>;
-    const int N = d_.nz;
-    ChSolveArgs sa{Ib_.ptr, Iu_.ptr, Iblo_.as<const signed char>(), Iulo_.as<const signed char>(),
-                   lo_quad_stride(nb_max_), lo_quad_stride(nu_max_), N, nxh_, (size_t)nb_max_ * nb_max_, (size_t)nu_max_ * nu_max_, nu_max_,
-                   factored_c_, bcv_.as<const double2>(), d_.refine_sweeps, 2.0 * M_PI / d_.Lx, d_.kappa, d_.nu,
-                   2.0 / d_.Lz, d_.Lx, d_.Lz};
+    const ChSolveArgs sa = solve_args();
     constexpr int NT = 256;
     const size_t lds = ch_solve_lds_doubles(nu_max_, d_.refine_sweeps > 0) * sizeof(double);
     timed(4, [&] {
@@ -436,26 +439,23 @@ void ChannelSolver::zero_t(void* p, size_t n) {
     TG_HIP(hipGetLastError());
 }
 
-void ChannelSolver::launch_step_body() {
-    const bool f32 = d_.dtype == kF32;
+// The launches between implicit solves i - 1 and i of a step (i = nsolves(): after the last)
+//   multistep: [F_n, SBDF right-hand side] solve []
+//   RK:        [X0 copy, p = taus = 0, F_0, L x_0, stage-1 rhs] solve [F_1, L x_1, stage-2 rhs]
+//              solve ... [F_{s-1}, L x_{s-1}, final update, p = taus = 0]
+template <typename T>
+void ChannelSolver::segment_t(int i) {
+    using V = cplx<T>;
     const size_t n = field_elems();
     if (multistep()) {
         // F_n and M P(x_n) of the current state, rhs, one solve; p and the taus come from the solve
-        const bool first = first_multistep();
-        if (f32) {
-            rhs_t<float>(B_.ptr, UX_.ptr, UZ_.ptr, F_[0][0].ptr, F_[0][1].ptr, F_[0][2].ptr);
-            sbdf_t<float>(first);
-            solve_t<float>();
-        } else {
-            rhs_t<double>(B_.ptr, UX_.ptr, UZ_.ptr, F_[0][0].ptr, F_[0][1].ptr, F_[0][2].ptr);
-            sbdf_t<double>(first);
-            solve_t<double>();
+        if (i == 0) {
+            rhs_t<T>(B_.ptr, UX_.ptr, UZ_.ptr, F_[0][0].ptr, F_[0][1].ptr, F_[0][2].ptr);
+            sbdf_t<T>(first_multistep());
         }
         return;
     }
-    auto run = [&](auto tag) {
-        using T = decltype(tag);
-        using V = cplx<T>;
+    if (i == 0) {
         const unsigned nblk = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
         timed(5, [&] {
             hipLaunchKernelGGL((k_ch_copy<T>), dim3(nblk), dim3(256), 0, ctx_->stream, B_.as<const V>(),
@@ -466,19 +466,36 @@ void ChannelSolver::launch_step_body() {
         // stage 0: x_0 = M^+ M X0 keeps b, u and clears p and the taus
         zero_t<T>(P_.ptr, n);
         zero_t<T>(TAU_.ptr, (size_t)nxh_ * 8);
-        rhs_t<T>(B_.ptr, UX_.ptr, UZ_.ptr, F_[0][0].ptr, F_[0][1].ptr, F_[0][2].ptr);
-        lx_t<T>(0);
-        for (int st = 1; st < tab_.stages; ++st) {
-            stage_rhs_t<T>(st, false);
-            solve_t<T>();
-            rhs_t<T>(B_.ptr, UX_.ptr, UZ_.ptr, F_[st][0].ptr, F_[st][1].ptr, F_[st][2].ptr);
-            lx_t<T>(st);
-        }
+    }
+    rhs_t<T>(B_.ptr, UX_.ptr, UZ_.ptr, F_[i][0].ptr, F_[i][1].ptr, F_[i][2].ptr);
+    lx_t<T>(i);
+    if (i + 1 < tab_.stages) {
+        stage_rhs_t<T>(i + 1, false);
+    } else {
         stage_rhs_t<T>(0, true);
         zero_t<T>(P_.ptr, n);
         zero_t<T>(TAU_.ptr, (size_t)nxh_ * 8);
-    };
-    if (f32) run(float{}); else run(double{});
+    }
+}
+
+void ChannelSolver::segment(int i) {
+    if (d_.dtype == kF32) segment_t<float>(i);
+    else segment_t<double>(i);
+}
+
+void ChannelSolver::launch_step_body() {
+    for (int i = 0; i < nsolves(); ++i) {
+        segment(i);
+        if (d_.dtype == kF32) solve_t<float>();
+        else solve_t<double>();
+    }
+    segment(nsolves());
+}
+
+void ChannelSolver::advance_clock() {
+    dt_prev_ = d_.dt;
+    t_ += d_.dt;
+    ++iter_;
 }
 
 void ChannelSolver::step_once(bool allow_graph) {
@@ -508,9 +525,7 @@ void ChannelSolver::step_once(bool allow_graph) {
         }
         TG_HIP(hipGraphLaunch(graph_, ctx_->stream));
     }
-    dt_prev_ = d_.dt;
-    t_ += d_.dt;
-    ++iter_;
+    advance_clock();
 }
 
 void ChannelSolver::step(int nsteps) {
@@ -623,6 +638,139 @@ size_t footprint(const ChannelDesc& d) {
 void ChannelSolver::flag_nonfinite(int* dflag) const {
     for (const DevBuf* b : {&P_, &B_, &UX_, &UZ_})
         tarang::flag_nonfinite(ctx_, b->ptr, 2 * field_elems(), (DType)d_.dtype, dflag);
+}
+
+// ---- several problems stepped together ----
+
+ChannelGroup::ChannelGroup(const std::vector<ChannelSolver*>& members) : m_(members) {
+    TG_REQUIRE(!m_.empty(), kErrInvalid, "channel group: at least one member");
+    for (size_t i = 0; i < m_.size(); ++i) {
+        TG_REQUIRE(m_[i] != nullptr, kErrInvalid, "channel group: null member");
+        for (size_t j = 0; j < i; ++j)
+            TG_REQUIRE(m_[j] != m_[i], kErrInvalid, "channel group: a solver appears twice");
+        const ChannelDesc &a = m_[0]->d_, &d = m_[i]->d_;
+        TG_REQUIRE(m_[i]->ctx_ == m_[0]->ctx_, kErrInvalid, "channel group: members must share one context");
+        TG_REQUIRE(d.dtype == a.dtype && d.stepper == a.stepper && d.refine_sweeps == a.refine_sweeps &&
+                       d.split_inverse == a.split_inverse,
+                   kErrInvalid, "channel group: members must share dtype, timestepper, refine_sweeps and split_inverse");
+    }
+    ctx_ = m_[0]->ctx_;
+    for (ChannelSolver* s : m_) total_wg_ += 2 * s->nxh_;
+}
+
+ChannelGroup::~ChannelGroup() { invalidate_graph(); }
+
+template <typename T>
+void ChannelGroup::fill_items() {
+    using V = cplx<T>;
+    std::vector<ChSolveItem<T>> it(m_.size());
+    int wg0 = 0;
+    for (size_t i = 0; i < m_.size(); ++i) {
+        ChannelSolver* s = m_[i];
+        it[i].s = s->solve_args();
+        for (int r = 0; r < 4; ++r) it[i].R[r] = s->R_[r].as<const V>();
+        it[i].out[0] = s->P_.as<V>();
+        it[i].out[1] = s->B_.as<V>();
+        it[i].out[2] = s->UX_.as<V>();
+        it[i].out[3] = s->UZ_.as<V>();
+        it[i].out[4] = s->TAU_.as<V>();
+        it[i].wg0 = wg0;
+        wg0 += 2 * s->nxh_;
+    }
+    hitems_.resize(it.size() * sizeof(ChSolveItem<T>));
+    std::memcpy(hitems_.data(), it.data(), hitems_.size());
+}
+
+template <typename T>
+void ChannelGroup::solve_group_t() {
+    constexpr int NT = 256;
+    const ChannelDesc& d = m_[0]->d_;
+    int nu_max = 0;
+    for (ChannelSolver* s : m_) nu_max = std::max(nu_max, s->nu_max_);
+    const size_t lds = ch_solve_lds_doubles(nu_max, d.refine_sweeps > 0) * sizeof(double);
+    const ChSolveItem<T>* items = items_.as<const ChSolveItem<T>>();
+    const int n = (int)m_.size();
+    auto launch = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(total_wg_), dim3(NT), lds, ctx_->stream, items, n); };
+    auto by_size = [&](auto refine, auto split) {  // as ChannelSolver::solve_t, for the largest member
+        constexpr bool R = decltype(refine)::value, S = decltype(split)::value;
+        if (nu_max <= 2 * NT) launch(k_ch_solve_group<T, NT, R, S, 2>);
+        else launch(k_ch_solve_group<T, NT, R, S, 4>);
+    };
+    using yes = std::true_type;
+    using no = std::false_type;
+    bool done = false;
+    if constexpr (sizeof(T) == 4) {
+        if (d.split_inverse) {
+            if (d.refine_sweeps > 0) by_size(yes{}, yes{});
+            else by_size(no{}, yes{});
+            done = true;
+        }
+    }
+    if (!done) {
+        if (d.refine_sweeps > 0) by_size(yes{}, no{});
+        else by_size(no{}, no{});
+    }
+    TG_HIP(hipGetLastError());
+}
+
+void ChannelGroup::launch_body() {
+    const bool f32 = m_[0]->d_.dtype == kF32;
+    const int ns = m_[0]->nsolves();
+    for (int i = 0; i < ns; ++i) {
+        for (ChannelSolver* s : m_) s->segment(i);
+        if (f32) solve_group_t<float>();
+        else solve_group_t<double>();
+    }
+    for (ChannelSolver* s : m_) s->segment(ns);
+}
+
+void ChannelGroup::step_once() {
+    // each member factors for its own implicit coefficient (a no-op unless it changed); the
+    // item table holds every member's coefficient and is rebuilt, with the graph, when any
+    // member's dt, previous dt, first-step flag or factored coefficient changed
+    std::vector<double> key;
+    key.reserve(4 * m_.size());
+    for (ChannelSolver* s : m_) {
+        const bool first = s->first_multistep();
+        s->factor(s->implicit_c(first));
+        key.insert(key.end(), {s->d_.dt, s->dt_prev_, first ? 1.0 : 0.0, s->factored_c_});
+    }
+    if (key != graph_key_) {
+        invalidate_graph();
+        if (m_[0]->d_.dtype == kF32) fill_items<float>();
+        else fill_items<double>();
+        if (items_.bytes < hitems_.size()) items_ = DevBuf(ctx_, hitems_.size());
+        // stream-ordered after the previous step's launches, which read the old table
+        TG_HIP(hipMemcpyAsync(items_.ptr, hitems_.data(), hitems_.size(), hipMemcpyHostToDevice, ctx_->stream));
+        TG_HIP(hipStreamSynchronize(ctx_->stream));
+        graph_key_ = key;
+    }
+    if (!graph_on_) {
+        launch_body();
+    } else {
+        if (!graph_) {
+            hipGraph_t g = nullptr;
+            TG_HIP(hipStreamBeginCapture(ctx_->stream, hipStreamCaptureModeThreadLocal));
+            try {
+                launch_body();
+            } catch (...) {
+                (void)hipStreamEndCapture(ctx_->stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            TG_HIP(hipStreamEndCapture(ctx_->stream, &g));
+            TG_HIP(hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0));
+            TG_HIP(hipGraphDestroy(g));
+        }
+        TG_HIP(hipGraphLaunch(graph_, ctx_->stream));
+    }
+    for (ChannelSolver* s : m_) s->advance_clock();
+}
+
+void ChannelGroup::step(int nsteps) {
+    TG_REQUIRE(nsteps >= 0, kErrInvalid, "channel group: nsteps must be >= 0");
+    ctx_->activate();
+    for (int i = 0; i < nsteps; ++i) step_once();
 }
 
 }  // namespace tarang

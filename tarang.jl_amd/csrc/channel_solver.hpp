@@ -27,6 +27,9 @@ struct ChannelDesc {
 
 enum ChannelVar : int { kChP = 0, kChB = 1, kChUX = 2, kChUZ = 3 };
 
+struct ChSolveArgs;
+class ChannelGroup;
+
 class ChannelSolver {
   public:
     // the descriptor checks of the constructor, without allocating (throws tarang::Error)
@@ -61,6 +64,13 @@ class ChannelSolver {
     void kernel_bytes(double* out) const;  // algorithmic bytes per launch, same kinds
 
   private:
+    friend class ChannelGroup;
+    // implicit solves per step, and the launches before solve i (i = nsolves(): after the last)
+    int nsolves() const { return multistep() ? 1 : tab_.stages - 1; }
+    void segment(int i);
+    template <typename T> void segment_t(int i);
+    ChSolveArgs solve_args() const;
+    void advance_clock();
     template <typename T> void rhs_t(const void* b, const void* ux, const void* uz, void* F0, void* F1, void* F2);
     template <typename T> void lx_t(int s);
     template <typename T> void stage_rhs_t(int s, bool final_update);
@@ -109,6 +119,39 @@ class ChannelSolver {
     bool profiling_ = false;
     struct Mark { int kind; hipEvent_t a, b; };
     std::vector<Mark> marks_;
+};
+
+// Several 2-D channel problems stepped together: one context, and the same dtype,
+// timestepper and solve form (refine_sweeps, split_inverse); any nx, nz, dt, Lx, Lz,
+// diffusivities and walls.  Each implicit stage runs the per-mode solves of every member
+// as one launch (k_ch_solve_group, workgroups over all members' modes); the group step
+// is replayed as one hipGraph.  The members keep their own clocks and multistep history
+// and may still be stepped alone; they must outlive the group.
+class ChannelGroup {
+  public:
+    explicit ChannelGroup(const std::vector<ChannelSolver*>& members);
+    ~ChannelGroup();
+    void step(int nsteps);
+    int size() const { return (int)m_.size(); }
+    void enable_graph(bool on) { graph_on_ = on; }
+
+  private:
+    void step_once();
+    void launch_body();
+    template <typename T> void fill_items();
+    template <typename T> void solve_group_t();
+    void invalidate_graph() {
+        if (graph_) (void)hipGraphExecDestroy(graph_);
+        graph_ = nullptr;
+    }
+    std::vector<ChannelSolver*> m_;
+    Context* ctx_;
+    DevBuf items_;
+    std::vector<unsigned char> hitems_, uploaded_;  // item table built for this step / on the device
+    int total_wg_ = 0;
+    bool graph_on_ = true;
+    hipGraphExec_t graph_ = nullptr;
+    std::vector<double> graph_key_;  // per member: dt, previous dt, first multistep step
 };
 
 }  // namespace tarang

@@ -163,6 +163,10 @@ SIGNATURES = {
     "tarang_chan_rhs": [_P, _P, _P, _P],
     "tarang_chan_step": [_P, _I],
     "tarang_chan_set_graph": [_P, _I],
+    "tarang_chan_group_create": [_PP, _I, _PP],
+    "tarang_chan_group_step": [_P, _I],
+    "tarang_chan_group_set_graph": [_P, _I],
+    "tarang_chan_group_destroy": [_P],
     "tarang_chan_set_dt": [_P, _D],
     "tarang_chan_time": [_P, _PD, ctypes.POINTER(ctypes.c_long)],
     "tarang_chan_profile": [_P, _I, _PD, ctypes.POINTER(ctypes.c_long)],
