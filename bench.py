@@ -114,6 +114,8 @@ def main():
     ap.add_argument("--cfl-horizon", action="store_true",
                     help="channel: also run the configuration to its t = 20 horizon with CFL-controlled dt "
                          "(FP64 and FP32 split-inverse forms through the API)")
+    ap.add_argument("--no-group", action="store_true",
+                    help="channel: skip the group-stepping comparison (used for profiler runs)")
     ap.add_argument("--cfl-wall", type=float, default=480.0,
                     help="channel --cfl-horizon: wall-clock limit per form in seconds")
     ap.add_argument("--horizon", type=int, default=None,
@@ -482,7 +484,8 @@ def bench_channel(a):
         "kernels_f32": kernels,
         "cpu_baseline": cpu,
     }
-    line["group"] = channel_group_bench(a)
+    if not a.no_group:
+        line["group"] = channel_group_bench(a)
     if a.cfl_horizon:
         line["cfl_horizon"] = channel_cfl_horizon(a)
     return line

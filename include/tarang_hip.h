@@ -226,7 +226,7 @@ int tarang_fivp_kernel_bytes(tarang_fivp_t s, double* ybwd, double* xpass, doubl
  * FP64 whatever dtype is; dtype sets the state, the RHS and the stage arithmetic.
  * timestepper: 0 RK222, 1 RK443 (IMEX RK stage loop), 2 SBDF2, 4 SBDF1 (per-mode multistep,
  * step_subproblem_multistep.jl:197-420; SBDF2 starts with one SBDF1 step after create/set_field).
- * nx: power of 2 in 8..2048 (FP64: ..1024); nz: 4..512. */
+ * nx: 2^k in 8..2048, 3 2^k in 48..768 or 5 2^k in 80..640 (FP64: ..1024); nz: 4..512. */
 typedef struct {
     int nx, nz;
     double Lx, Lz;
@@ -305,7 +305,8 @@ int tarang_chan_group_destroy(tarang_chan_group_t g);
  * subproblem build and the per-mode multistep step!/run!
  * (src/core/timesteppers/step_subproblem_multistep.jl:197-420, SBDF2 coefficients
  * :544-552, SBDF1 start).  timestepper: 2 SBDF2, 4 SBDF1.
- * nx: power of 2, 8..1024 (FP64: ..512); ny: power of 2, 8..1024; nz: 4..256. */
+ * nx, ny: 2^k, 3 2^k (48..768) or 5 2^k (80..640); nx up to 1024 (FP64: 512), ny up to 1024;
+ * nz: 4..256. */
 typedef struct {
     int nx, ny, nz;
     double Lx, Ly, Lz;
@@ -351,9 +352,8 @@ int tarang_chan3_num_keys(tarang_chan3_t s, int* nkeys);
 int tarang_chan3_profile(tarang_chan3_t s, int nsteps, double* avg_ms8, long* counts8);
 int tarang_chan3_kernel_bytes(tarang_chan3_t s, double* bytes8);
 /* kernel variants picked by the autotuner at creation: [per-mode solve (0: 4 wavefronts per
- * workgroup, 1: 8, 2: 4 with a 3-deep ring of inverse loads), z-backward (0: the default mode
- * tile, 1: 16 modes, FP32 only)].  Every variant gives the same bits; TARANG_AUTOTUNE=0 keeps
- * [0, 0].  No reference counterpart. */
+ * workgroup, 1: 8), z-backward (0: the default mode tile, 1: 16 modes, FP32 only)].  Every
+ * variant gives the same bits; TARANG_AUTOTUNE=0 keeps [0, 0].  No reference counterpart. */
 int tarang_chan3_get_tuning(tarang_chan3_t s, int* variants2);
 int tarang_chan3_set_tuning(tarang_chan3_t s, const int* variants2);
 

@@ -40,7 +40,8 @@ void Channel3Solver::validate(const Channel3Desc& d) {
     const int xmax = d.dtype == kF64 ? 512 : 1024;
     TG_REQUIRE(size_ok(d.nx) && d.nx <= xmax && size_ok(d.ny) && d.ny <= 1024, kErrInvalid,
                "3-D channel solver: RealFourier sizes " + std::to_string(d.nx) + "x" + std::to_string(d.ny) +
-                   " unsupported (powers of 2; x 8.." + std::to_string(xmax) + ", y 8..1024)");
+                   " unsupported (2^k 8..2048, 3 2^k 48..768, 5 2^k 80..640; x up to " + std::to_string(xmax) +
+                   ", y up to 1024)");
     TG_REQUIRE(d.nz >= 4 && d.nz <= 256, kErrInvalid,
                "3-D channel solver: ChebyshevT size " + std::to_string(d.nz) + " unsupported (4..256)");
     TG_REQUIRE(d.refine_sweeps >= 0 && d.refine_sweeps <= 2, kErrInvalid,
@@ -213,7 +214,7 @@ Channel3Solver::Channel3Solver(Context* ctx, const Channel3Desc& d) : ctx_(ctx),
 }
 
 void Channel3Solver::set_tuning(const Chan3Tuning& t) {
-    TG_REQUIRE(t.solve >= 0 && t.solve <= 2, kErrInvalid, "3-D channel solver: solve variant is 0, 1 or 2");
+    TG_REQUIRE(t.solve == 0 || t.solve == 1, kErrInvalid, "3-D channel solver: solve variant is 0 or 1");
     TG_REQUIRE(t.zbwd == 0 || (t.zbwd == 1 && d_.dtype == kF32), kErrInvalid,
                "3-D channel solver: z-backward variant is 0, or 1 with an FP32 state");
     tune_ = t;
@@ -221,8 +222,8 @@ void Channel3Solver::set_tuning(const Chan3Tuning& t) {
 }
 
 // Times the variants of k3_solve and k3_zbwd on this solver's own buffers (HIP events,
-// median of 5 launches) at creation.  A variant replaces the default only when it is at
-// least 3 % faster, so timing noise does not flip the choice from run to run.  The
+// medians of 7 launches alternating with the default's) at creation.  A variant replaces the
+// default only when it is at least 3 % faster, so timing noise does not flip the choice.  The
 // choices are cached per (nx, ny, nz, dtype, solve form, |k_h|^2 keys) for the life of the process;
 // TARANG_AUTOTUNE=0 keeps the defaults.
 void Channel3Solver::autotune() {
@@ -244,44 +245,30 @@ void Channel3Solver::autotune() {
     hipEvent_t a, b;
     TG_HIP(hipEventCreate(&a));
     TG_HIP(hipEventCreate(&b));
-    auto measure = [&](auto&& launch) {
-        std::vector<float> v;
-        for (int r = 0; r < 6; ++r) {
-            TG_HIP(hipEventRecord(a, s));
-            launch();
-            TG_HIP(hipEventRecord(b, s));
-            TG_HIP(hipEventSynchronize(b));
-            float ms = 0;
-            TG_HIP(hipEventElapsedTime(&ms, a, b));
-            if (r > 0) v.push_back(ms);
-        }
-        std::sort(v.begin(), v.end());
-        return v[v.size() / 2];
+    // the two variants' launches alternate (a warm-up pair, then 7 timed pairs), so that
+    // clock and temperature drift during the measurement falls on both; medians compared
+    auto pick = [&](int& field, auto&& launch) {
+        std::vector<float> v[2];
+        for (int r = 0; r < 8; ++r)
+            for (int var = 0; var < 2; ++var) {
+                field = var;
+                TG_HIP(hipEventRecord(a, s));
+                launch();
+                TG_HIP(hipEventRecord(b, s));
+                TG_HIP(hipEventSynchronize(b));
+                float ms = 0;
+                TG_HIP(hipEventElapsedTime(&ms, a, b));
+                if (r > 0) v[var].push_back(ms);
+            }
+        for (auto& x : v) std::sort(x.begin(), x.end());
+        return v[1][v[1].size() / 2] < 0.97f * v[0][v[0].size() / 2] ? 1 : 0;
     };
     Chan3Tuning chosen;
     auto go = [&](auto tag) {
         using T = decltype(tag);
-        // k3_solve: the fastest variant replaces the default when 3 % faster (variant 2, the
-        // deeper load ring, applies to the small size class without refinement sweeps)
-        tune_.solve = 0;
-        const float t0 = measure([&] { solve_t<T>(); });
-        const int nvar = ch3_solve_small(nb_max_, nu_max_) && d_.refine_sweeps == 0 ? 3 : 2;
-        float best = t0;
-        for (int v = 1; v < nvar; ++v) {
-            tune_.solve = v;
-            const float tv = measure([&] { solve_t<T>(); });
-            if (tv < 0.97f * t0 && tv < best) {
-                best = tv;
-                chosen.solve = v;
-            }
-        }
-        if (sizeof(T) == 4) {
-            tune_.zbwd = 0;
-            const float z0 = measure([&] { rhs_t<T>(F_[0].ptr, F_[1].ptr, F_[2].ptr, F_[3].ptr); });
-            tune_.zbwd = 1;
-            const float z1 = measure([&] { rhs_t<T>(F_[0].ptr, F_[1].ptr, F_[2].ptr, F_[3].ptr); });
-            chosen.zbwd = z1 < 0.97f * z0 ? 1 : 0;
-        }
+        chosen.solve = pick(tune_.solve, [&] { solve_t<T>(); });
+        if (sizeof(T) == 4)
+            chosen.zbwd = pick(tune_.zbwd, [&] { rhs_t<T>(F_[0].ptr, F_[1].ptr, F_[2].ptr, F_[3].ptr); });
     };
     if (d_.dtype == kF32) go(float{});
     else go(double{});
@@ -561,7 +548,6 @@ void Channel3Solver::solve_t() {
             constexpr bool R = decltype(refine)::value, S = decltype(split)::value;
             const bool small = ch3_solve_small(nb_max_, nu_max_);
             if (tune_.solve == 1) launch(k3_solve<T, 512, R, S, true>, 512);  // 8 waves: small tiles per wave
-            else if (small && !R && tune_.solve == 2) launch(k3_solve<T, 256, R, S, true, 3>, 256);
             else if (small) launch(k3_solve<T, 256, R, S, true>, 256);
             else launch(k3_solve<T, 256, R, S, false>, 256);
         };

@@ -31,9 +31,7 @@ enum Channel3Var : int { kC3P = 0, kC3B = 1, kC3UX = 2, kC3UY = 3, kC3UZ = 4 };
 
 // kernel variants picked by the autotuner at creation; every variant gives the same bits
 struct Chan3Tuning {
-    // k3_solve: 0 four wavefronts, 1 eight, 2 four with a 3-deep inverse-load ring (small
-    // size class without refinement sweeps; otherwise as 0)
-    int solve = 0;
+    int solve = 0;  // k3_solve workgroup: 0 four wavefronts, 1 eight
     int zbwd = 0;   // k3_zbwd mode tile: 0 the type's default (32 FP32 / 16 FP64), 1 16 modes (FP32 only)
 };
 

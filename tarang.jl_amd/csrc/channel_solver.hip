@@ -38,7 +38,7 @@ void ChannelSolver::validate(const ChannelDesc& d) {
                kErrInvalid, "channel solver: timestepper must be RK222, RK443, SBDF2 or SBDF1");
     TG_REQUIRE(channel_nx_ok(d.nx, d.dtype == kF64), kErrInvalid,
                "channel solver: RealFourier size " + std::to_string(d.nx) +
-                   " unsupported (powers of 2, 8..2048; FP64 up to 1024)");
+                   " unsupported (2^k 8..2048, 3 2^k 48..768, 5 2^k 80..640; FP64 up to 1024)");
     TG_REQUIRE(d.nz >= 4 && d.nz <= 512, kErrInvalid,
                "channel solver: ChebyshevT size " + std::to_string(d.nz) + " unsupported (4..512)");
     TG_REQUIRE(d.refine_sweeps >= 0 && d.refine_sweeps <= 2, kErrInvalid,

@@ -19,9 +19,11 @@ inline bool dispatch_size(int n, F&& f, SizeList<N0, Ns...>) {
     return false;
 }
 
-// Fourier grid sizes with fused kernels (the 3/2-padded size must factor into 2, 3, 5)
-using Sizes1D = SizeList<8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096>;
-using Sizes2D = SizeList<8, 16, 32, 64, 128, 256, 512, 1024, 2048>;
+// Fourier grid sizes with fused kernels: powers of two, and three and five times powers of
+// two (every 3/2-padded length then factors into 2, 3 and 5: 72 = 8 9, 120 = 8 15, ...)
+using Sizes1D = SizeList<8, 16, 32, 48, 64, 80, 96, 128, 160, 192, 256, 320, 384, 512, 640, 768, 1024, 1536, 2048,
+                         4096>;
+using Sizes2D = SizeList<8, 16, 32, 48, 64, 80, 96, 128, 160, 192, 256, 320, 384, 512, 640, 768, 1024, 2048>;
 
 constexpr int pow2_floor(int x) {
     int p = 1;

@@ -83,11 +83,11 @@ void FourierSolver::validate(const FourierDesc& d) {
     const int nx = d.n[0], ny = d.ndim > 1 ? d.n[1] : 1;
     if (d.ndim == 1)
         TG_REQUIRE(fourier_size_ok_1d(nx), kErrInvalid,
-                   "RealFourier size " + std::to_string(nx) + " has no fused 1-D kernel (supported: 8..4096, powers of 2)");
+                   "RealFourier size " + std::to_string(nx) + " has no fused 1-D kernel (supported: 2^k 8..4096, 3 2^k 48..1536, 5 2^k 80..640)");
     else
         TG_REQUIRE(fourier_size_ok_2d(nx) && fourier_size_ok_2d(ny), kErrInvalid,
                    "RealFourier sizes " + std::to_string(nx) + "x" + std::to_string(ny) +
-                       " have no fused 2-D kernels (supported: 8..2048, powers of 2)");
+                       " have no fused 2-D kernels (supported: 2^k 8..2048, 3 2^k 48..768, 5 2^k 80..640)");
     TG_REQUIRE(!(d.ndim == 2 && d.dtype == kF64 && (nx > 1024 || ny > 1024)), kErrInvalid,
                "FP64 2-D fused kernels support sizes up to 1024");
 }
@@ -339,19 +339,14 @@ void FourierSolver::autotune() {
     hipEvent_t a, b;
     TG_HIP(hipEventCreate(&a));
     TG_HIP(hipEventCreate(&b));
-    auto measure = [&](auto&& launch) {
-        std::vector<float> v;
-        for (int r = 0; r < 6; ++r) {
-            TG_HIP(hipEventRecord(a, ctx_->stream));
-            launch();
-            TG_HIP(hipEventRecord(b, ctx_->stream));
-            TG_HIP(hipEventSynchronize(b));
-            float ms = 0;
-            TG_HIP(hipEventElapsedTime(&ms, a, b));
-            if (r > 0) v.push_back(ms);
-        }
-        std::sort(v.begin(), v.end());
-        return v[v.size() / 2];
+    auto time_one = [&](auto&& launch) {
+        TG_HIP(hipEventRecord(a, ctx_->stream));
+        launch();
+        TG_HIP(hipEventRecord(b, ctx_->stream));
+        TG_HIP(hipEventSynchronize(b));
+        float ms = 0;
+        TG_HIP(hipEventElapsedTime(&ms, a, b));
+        return ms;
     };
     auto run = [&](auto tag) {
         using T = decltype(tag);
@@ -368,14 +363,23 @@ void FourierSolver::autotune() {
         V* G = G_.as<V>();
         V* H = H_.as<V>();
         V* F = F_[0].as<V>();
-        // a variant replaces the current choice only when at least 3 % faster, so timing
-        // noise does not flip the choice from run to run
+        // the variants' launches alternate (a warm-up round, then 7 timed rounds) so that
+        // clock and temperature drift falls on all of them; a variant replaces the current
+        // choice only when its median is at least 3 % lower, so timing noise does not flip it
         auto pick = [&](int nvar, int& slot, auto&& run) {
+            std::vector<std::vector<float>> t(nvar);
+            for (int r = 0; r < 8; ++r)
+                for (int v = 0; v < nvar; ++v) {
+                    const float ms = time_one([&] { run(v); });
+                    if (r > 0) t[v].push_back(ms);
+                }
             slot = 0;
-            float best = measure([&] { run(0); });
-            for (int v = 1; v < nvar; ++v) {
-                const float t = measure([&] { run(v); });
-                if (t < 0.97f * best) { best = t; slot = v; }
+            float best = 0;
+            for (int v = 0; v < nvar; ++v) {
+                std::sort(t[v].begin(), t[v].end());
+                const float m = t[v][t[v].size() / 2];
+                if (v == 0) best = m;
+                else if (m < 0.97f * best) { best = m; slot = v; }
             }
         };
         pick(4, tune_.ybwd, [&](int v) { launch_ybwd<T, TurbSpec>(v, p, G, gstride); });

@@ -177,7 +177,7 @@ TransformPlan::TransformPlan(Context* ctx, const DomainDesc& d) : ctx_(ctx), d_(
         bool ok = d.kind[a] == kChebyshevT;
         if (!ok) dispatch_size(d.n[a], [&](auto) { ok = true; }, Sizes1D{});
         TG_REQUIRE(ok, kErrInvalid,
-                   "RealFourier size " + std::to_string(d.n[a]) + " not supported on the GPU (powers of 2, 8..4096)");
+                   "RealFourier size " + std::to_string(d.n[a]) + " not supported on the GPU (2^k 8..4096, 3 2^k 48..1536, 5 2^k 80..640)");
         if (d.kind[a] == kChebyshevT)
             TG_REQUIRE(d.n[a] <= 1024, kErrInvalid, "ChebyshevT size above 1024 is not supported");
         cshape_[a] = a == 0 ? d.n[a] / 2 + 1 : d.n[a];

@@ -114,13 +114,13 @@ def test_channel3d_graph_and_keys(gpu, oracle):
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_channel3d_tuning_variants_same_bits(gpu, oracle, dtype):
-    """Every kernel variant the autotuner can pick (k3_solve with 4 or 8 wavefronts or a
-    3-deep inverse-load ring, k3_zbwd with 32- or 16-mode tiles) gives the same bits, on a
-    32 x 32 x 24 state over 3 SBDF2 steps (multi-chunk keys, the split-inverse FP32 form)."""
+    """Every kernel variant the autotuner can pick (k3_solve with 4 or 8 wavefronts, k3_zbwd
+    with 32- or 16-mode tiles) gives the same bits, on a 32 x 32 x 24 state over 3 SBDF2 steps
+    (multi-chunk keys, the split-inverse FP32 form)."""
     nx, ny, N, L = 32, 32, 24, 4.0
     kappa, nu = kp()
     st, _ = random_state3(oracle, nx, ny, N, L, L, 1.0, seed=11)
-    variants = [(0, 0), (1, 0), (2, 0)] + ([(0, 1), (2, 1)] if dtype == np.float32 else [])
+    variants = [(0, 0), (1, 0)] + ([(0, 1), (1, 1)] if dtype == np.float32 else [])
     out = []
     for v in variants:
         print(f"channel3d tuning variant {v} {dtype.__name__}", flush=True)
@@ -142,7 +142,7 @@ def test_channel3d_tuning_variants_same_bits(gpu, oracle, dtype):
     with pytest.raises(_lib.TarangError):  # the 16-mode z tile is an FP32 variant
         _lib.call("tarang_chan3_set_tuning", s.h, (ctypes.c_int * 2)(0, 1))
     with pytest.raises(_lib.TarangError):
-        _lib.call("tarang_chan3_set_tuning", s.h, (ctypes.c_int * 2)(3, 0))
+        _lib.call("tarang_chan3_set_tuning", s.h, (ctypes.c_int * 2)(2, 0))
     s.close()
 
 
