@@ -41,7 +41,7 @@ template <typename T, int NY, int MY, int TK, int NT>
 __global__ __launch_bounds__(NT) void k3_ybwd(const cplx<T>* __restrict__ G, Ch3Geom geo, cplx<T>* __restrict__ Q,
                                               const cplx<T>* __restrict__ tw) {
     using V = cplx<T>;
-    constexpr int LS = line_stride(MY);
+    constexpr int LS = line_stride_lf<T, TK>(MY);  // the tile's TK lines side by side in every access
     constexpr bool TW_LDS = (size_t)(TK * LS + MY) * sizeof(V) <= 160 * 1024;
     __shared__ V buf[TK * LS];
     __shared__ V twl_lds[TW_LDS ? MY : 1];
@@ -75,7 +75,7 @@ __global__ __launch_bounds__(NT) void k3_ybwd(const cplx<T>* __restrict__ G, Ch3
         if (m == 0 || (xeven && m == geo.nx / 2)) v.y = T(0);  // real lines of the grid round trip
         out[(size_t)j * nxh + m] = v;
     };
-    fft_regs_io<T, MY, TK, NT, true, true, false>(buf, twl, load, store);  // radix 8/4/2/3: fewer registers
+    fft_regs_io<T, MY, TK, NT, true, true, false, true, LS>(buf, twl, load, store);  // radix 8/4/2/3: fewer registers
 }
 
 // ---------------------------------------------------------------------------
@@ -149,7 +149,7 @@ template <typename T, int NY, int MY, int TK, int NT>
 __global__ __launch_bounds__(NT) void k3_yfwd(const cplx<T>* __restrict__ H, Ch3Geom geo, cplx<T>* __restrict__ F,
                                               const cplx<T>* __restrict__ tw) {
     using V = cplx<T>;
-    constexpr int LS = line_stride(MY);
+    constexpr int LS = line_stride_lf<T, TK>(MY);  // the tile's TK lines side by side in every access
     constexpr bool TW_LDS = (size_t)(TK * LS + MY) * sizeof(V) <= 160 * 1024;
     __shared__ V buf[TK * LS];
     __shared__ V twl_lds[TW_LDS ? MY : 1];
@@ -164,7 +164,7 @@ __global__ __launch_bounds__(NT) void k3_yfwd(const cplx<T>* __restrict__ H, Ch3
         const int m = m0 + c;
         return m < nxh ? in[(size_t)j * nxh + m] : V{T(0), T(0)};
     };
-    fft_regs_io<T, MY, TK, NT, false, false>(buf, twl, load, [](int, int, V) {});
+    fft_regs_io<T, MY, TK, NT, false, false, true, true, LS>(buf, twl, load, [](int, int, V) {});
     // truncation (in place: entries n < NY/2 stay, n > NY/2 come from MY - NY + n, Nyquist folded)
     constexpr int NTR = TK * NY, PT = (NTR + NT - 1) / NT;
     V tr[PT];

@@ -57,6 +57,19 @@ __host__ __device__ constexpr int line_stride(int L) {
     const int b = pidx(L - 1) + 1;
     return b + (48 - b % 32) % 32;
 }
+// Line distance for batches whose lanes take NLINES lines side by side (LINES_FAST loaders
+// and storers, kx-fastest copies): L (mod 32) = W / NLINES complex values, W the complex
+// values per 256-byte row of LDS banks (32 in FP32, 16 in FP64), so that the lanes of one
+// 256-byte access, NLINES lines by W / NLINES neighbouring elements, fall on distinct
+// banks.  A bank-conflict model of every pass's index maps puts the 8-line FP32 batches
+// of length 384 at 1.7-2.0x the conflict-free LDS cycles, against 2.8-5.5x at line_stride.
+template <typename T, int NLINES>
+__host__ __device__ constexpr int line_stride_lf(int L) {
+    const int b = pidx(L - 1) + 1;
+    constexpr int W = 256 / (int)(2 * sizeof(T));
+    constexpr int res = NLINES >= W ? 1 : W / NLINES;
+    return b + ((res - b) % 32 + 32) % 32;
+}
 
 // copy the global twiddle table into LDS (caller synchronises before use)
 template <typename T, int L, int NT>
@@ -218,14 +231,15 @@ __device__ inline void dft_composite(cplx<T>* v) {
 }
 
 // One Stockham pass of radix R after NS points have been combined.
-template <typename T, int L, int R, int NS, int NLINES, int NT, bool INV>
+// LSX: the distance between lines (0: line_stride(L)), as in every batched FFT below
+template <typename T, int L, int R, int NS, int NLINES, int NT, bool INV, int LSX = 0>
 __device__ inline void stockham_pass(cplx<T>* buf, const cplx<T>* twl) {
     using V = cplx<T>;
     constexpr int NB = L / R;
     constexpr int TOT = NB * NLINES;
     constexpr int PER = (TOT + NT - 1) / NT;
     constexpr int TSTEP = L / (NS * R);
-    constexpr int LS = line_stride(L);
+    constexpr int LS = LSX ? LSX : line_stride(L);
     V v[PER][R];
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
@@ -264,22 +278,22 @@ __device__ inline void stockham_pass(cplx<T>* buf, const cplx<T>* twl) {
 // COMPOSITE selects the plan: composite radices halve the LDS round trips but leave
 // fewer butterflies per pass (a caller whose lines are short and few may do better
 // without them).
-template <typename T, int L, int NLINES, int NT, bool INV, bool COMPOSITE, int P = 0, int NS = 1>
+template <typename T, int L, int NLINES, int NT, bool INV, bool COMPOSITE, int P = 0, int NS = 1, int LSX = 0>
 __device__ inline void fft_passes(cplx<T>* buf, const cplx<T>* twl) {
     constexpr RadixPlan plan = radix_plan(L, COMPOSITE);
     static_assert(plan.n >= 0, "FFT length must factor into 2, 3 and 5");
     if constexpr (P < plan.n) {
         constexpr int R = plan.r[P];
-        stockham_pass<T, L, R, NS, NLINES, NT, INV>(buf, twl);
-        fft_passes<T, L, NLINES, NT, INV, COMPOSITE, P + 1, NS * R>(buf, twl);
+        stockham_pass<T, L, R, NS, NLINES, NT, INV, LSX>(buf, twl);
+        fft_passes<T, L, NLINES, NT, INV, COMPOSITE, P + 1, NS * R, LSX>(buf, twl);
     }
 }
 
 // In-LDS batched FFT of NLINES padded lines; twl is the LDS twiddle table.
 // The caller must have synchronised after filling buf and twl.
-template <typename T, int L, int NLINES, int NT, bool INV, bool COMPOSITE = true>
+template <typename T, int L, int NLINES, int NT, bool INV, bool COMPOSITE = true, int LSX = 0>
 __device__ inline void fft_lds(cplx<T>* buf, const cplx<T>* twl) {
-    if constexpr (L > 1) fft_passes<T, L, NLINES, NT, INV, COMPOSITE>(buf, twl);
+    if constexpr (L > 1) fft_passes<T, L, NLINES, NT, INV, COMPOSITE, 0, 1, LSX>(buf, twl);
 }
 
 // ---- passes fed from and drained to registers ----
@@ -292,23 +306,23 @@ __device__ inline void fft_lds(cplx<T>* buf, const cplx<T>* twl) {
 // neighbouring elements of one line (for lines that are rows).
 
 // runs passes [P, PEND) of the plan in LDS
-template <typename T, int L, int NLINES, int NT, bool INV, int P, int NS, int PEND, bool COMPOSITE = true>
+template <typename T, int L, int NLINES, int NT, bool INV, int P, int NS, int PEND, bool COMPOSITE = true, int LSX = 0>
 __device__ inline void fft_mid_passes(cplx<T>* buf, const cplx<T>* twl) {
     constexpr RadixPlan plan = radix_plan(L, COMPOSITE);
     if constexpr (P < PEND) {
         constexpr int R = plan.r[P];
-        stockham_pass<T, L, R, NS, NLINES, NT, INV>(buf, twl);
-        fft_mid_passes<T, L, NLINES, NT, INV, P + 1, NS * R, PEND, COMPOSITE>(buf, twl);
+        stockham_pass<T, L, R, NS, NLINES, NT, INV, LSX>(buf, twl);
+        fft_mid_passes<T, L, NLINES, NT, INV, P + 1, NS * R, PEND, COMPOSITE, LSX>(buf, twl);
     }
 }
 
 // first pass: loader -> DFT_R0 -> LDS, then a barrier (plans of two passes or more)
-template <typename T, int L, int NLINES, int NT, bool INV, bool COMPOSITE = true, bool LINES_FAST = true,
+template <typename T, int L, int NLINES, int NT, bool INV, bool COMPOSITE = true, bool LINES_FAST = true, int LSX = 0,
           typename Load>
 __device__ inline void fft_first_pass_regs(cplx<T>* buf, Load&& ld) {
     using V = cplx<T>;
     constexpr RadixPlan plan = radix_plan(L, COMPOSITE);
-    constexpr int LS = line_stride(L);
+    constexpr int LS = LSX ? LSX : line_stride(L);
     constexpr int R = plan.r[0];
     constexpr int NB = L / R;
     constexpr int TOT = NB * NLINES;
@@ -343,12 +357,12 @@ __device__ inline void fft_first_pass_regs(cplx<T>* buf, Load&& ld) {
 // pass also runs in LDS and the result is left in buf (after a barrier), st unused.
 // COMPOSITE selects the radix plan as for fft_lds.
 template <typename T, int L, int NLINES, int NT, bool INV, bool FULL, bool COMPOSITE = true, bool LINES_FAST = true,
-          typename Load, typename Store>
+          int LSX = 0, typename Load, typename Store>
 __device__ inline void fft_regs_io(cplx<T>* buf, const cplx<T>* twl, Load&& ld, Store&& st) {
     using V = cplx<T>;
     constexpr RadixPlan plan = radix_plan(L, COMPOSITE);
     static_assert(plan.n >= 1, "FFT length must factor into 2, 3 and 5");
-    constexpr int LS = line_stride(L);
+    constexpr int LS = LSX ? LSX : line_stride(L);
     if constexpr (plan.n == 1) {  // one butterfly per line: loader -> DFT_L -> storer
         constexpr int PER = (NLINES + NT - 1) / NT;
 #pragma unroll
@@ -368,13 +382,13 @@ __device__ inline void fft_regs_io(cplx<T>* buf, const cplx<T>* twl, Load&& ld, 
         }
         if constexpr (!FULL) __syncthreads();
     } else if constexpr (!FULL) {
-        fft_first_pass_regs<T, L, NLINES, NT, INV, COMPOSITE, LINES_FAST>(buf, ld);
-        fft_mid_passes<T, L, NLINES, NT, INV, 1, plan.r[0], plan.n, COMPOSITE>(buf, twl);
+        fft_first_pass_regs<T, L, NLINES, NT, INV, COMPOSITE, LINES_FAST, LSX>(buf, ld);
+        fft_mid_passes<T, L, NLINES, NT, INV, 1, plan.r[0], plan.n, COMPOSITE, LSX>(buf, twl);
     } else {
         constexpr int RL = plan.r[plan.n - 1];
         constexpr int NS = L / RL;  // points combined before the last pass
-        fft_first_pass_regs<T, L, NLINES, NT, INV, COMPOSITE, LINES_FAST>(buf, ld);
-        fft_mid_passes<T, L, NLINES, NT, INV, 1, plan.r[0], plan.n - 1, COMPOSITE>(buf, twl);
+        fft_first_pass_regs<T, L, NLINES, NT, INV, COMPOSITE, LINES_FAST, LSX>(buf, ld);
+        fft_mid_passes<T, L, NLINES, NT, INV, 1, plan.r[0], plan.n - 1, COMPOSITE, LSX>(buf, twl);
         // last pass: LDS -> twiddles, DFT_RL -> storer; butterfly j < NS writes j + r*NS
         constexpr int TOT = NS * NLINES;
         constexpr int PER = (TOT + NT - 1) / NT;
@@ -410,17 +424,18 @@ __device__ inline void fft_regs_io(cplx<T>* buf, const cplx<T>* twl, Load&& ld, 
 // As fft_regs_io with FULL = true, except that each thread runs last-pass butterfly j of
 // every line and st(i, v) receives element i of all NLINES lines at once (v[line]).  A
 // barrier separates the last pass's LDS reads from the st calls, so st may write buf.
-template <typename T, int L, int NLINES, int NT, bool INV, bool LINES_FAST = true, typename Load, typename Store>
+template <typename T, int L, int NLINES, int NT, bool INV, bool LINES_FAST = true, int LSX = 0, typename Load,
+          typename Store>
 __device__ inline void fft_regs_joint(cplx<T>* buf, const cplx<T>* twl, Load&& ld, Store&& st) {
     using V = cplx<T>;
     constexpr RadixPlan plan = radix_plan(L);
     static_assert(plan.n >= 1, "FFT length must factor into 2, 3 and 5");
-    constexpr int LS = line_stride(L);
+    constexpr int LS = LSX ? LSX : line_stride(L);
     constexpr int RL = plan.r[plan.n - 1];
     constexpr int NS = L / RL;  // points combined before the last pass (1: a single pass)
     if constexpr (plan.n >= 2) {
-        fft_first_pass_regs<T, L, NLINES, NT, INV, true, LINES_FAST>(buf, ld);
-        fft_mid_passes<T, L, NLINES, NT, INV, 1, plan.r[0], plan.n - 1>(buf, twl);
+        fft_first_pass_regs<T, L, NLINES, NT, INV, true, LINES_FAST, LSX>(buf, ld);
+        fft_mid_passes<T, L, NLINES, NT, INV, 1, plan.r[0], plan.n - 1, true, LSX>(buf, twl);
     }
     constexpr int PER = (NS + NT - 1) / NT;
     V v[PER][NLINES][RL];

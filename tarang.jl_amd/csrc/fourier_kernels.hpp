@@ -106,7 +106,7 @@ template <typename T, int NY, int MY, int TK, int NT, typename Spec>
 __global__ __launch_bounds__(NT) void k_ybwd(FourierParams<T> p, cplx<T>* __restrict__ g, size_t gstride,
                                              const cplx<T>* __restrict__ tw) {
     using V = cplx<T>;
-    constexpr int LS = line_stride(MY);
+    constexpr int LS = line_stride_lf<T, TK>(MY);  // the tile's TK lines side by side in every access
     constexpr bool EVEN = (NY % 2) == 0;
     constexpr int NN = NY - NY / 2 - 1;  // negative frequencies kept
     constexpr bool TW_LDS = twiddles_in_lds<T, MY, TK>();
@@ -146,7 +146,7 @@ __global__ __launch_bounds__(NT) void k_ybwd(FourierParams<T> p, cplx<T>* __rest
         const int kx = kx0 + c;
         if (kx < p.nxh) out[(size_t)d * p.nxh + kx] = v;
     };
-    fft_regs_io<T, MY, TK, NT, true, true>(buf, twl, load, store);
+    fft_regs_io<T, MY, TK, NT, true, true, true, true, LS>(buf, twl, load, store);
 }
 
 // ---------------------------------------------------------------------------
@@ -240,7 +240,7 @@ template <typename T, int NY, int MY, int TK, int NT>
 __global__ __launch_bounds__(NT) void k_yfwd(const cplx<T>* __restrict__ h, int nxh, int nx,
                                              cplx<T>* __restrict__ F, const cplx<T>* __restrict__ tw) {
     using V = cplx<T>;
-    constexpr int LS = line_stride(MY);
+    constexpr int LS = line_stride_lf<T, TK>(MY);  // the tile's TK lines side by side in every access
     constexpr bool TW_LDS = twiddles_in_lds<T, MY, TK>();
     __shared__ V buf[TK * LS];
     __shared__ V twl_lds[TW_LDS ? MY : 1];
@@ -252,7 +252,7 @@ __global__ __launch_bounds__(NT) void k_yfwd(const cplx<T>* __restrict__ h, int 
         const int kx = kx0 + c;
         return kx < nxh ? h[(size_t)d * nxh + kx] : V{T(0), T(0)};
     };
-    fft_regs_io<T, MY, TK, NT, false, false>(buf, twl, load, [](int, int, V) {});
+    fft_regs_io<T, MY, TK, NT, false, false, true, true, LS>(buf, twl, load, [](int, int, V) {});
     const bool xeven = (nx % 2) == 0;
     for (int idx = threadIdx.x; idx < TK * NY; idx += NT) {
         const int c = idx % TK, ky = idx / TK;

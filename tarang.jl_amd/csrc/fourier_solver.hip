@@ -219,8 +219,9 @@ struct YTiles {
     static constexpr int TK0 = tile_cols<T, M>();
     static constexpr int TK1 = TK0 > 1 ? TK0 / 2 : 1;
     static constexpr int TK2 = TK0 < 8 ? TK0 * 2 : 8;
-    static constexpr bool fits(int tk) {
-        return (size_t)(tk * line_stride(M) + M) * sizeof(cplx<T>) <= 160 * 1024;
+    template <int TK>
+    static constexpr bool fits() {
+        return (size_t)(TK * line_stride_lf<T, TK>(M) + M) * sizeof(cplx<T>) <= 160 * 1024;
     }
 };
 
@@ -245,7 +246,7 @@ void FourierSolver::launch_ybwd(int variant, const FourierParams<T>& p, cplx<T>*
         if constexpr (N >= 64) {
             if (variant == 1) return go(std::integral_constant<int, Y::TK1>{}, std::integral_constant<int, 256>{});
             if (variant == 2) {
-                if constexpr (Y::fits(Y::TK2))
+                if constexpr (Y::template fits<Y::TK2>())
                     return go(std::integral_constant<int, Y::TK2>{}, std::integral_constant<int, 512>{});
             }
             if (variant == 3) return go(std::integral_constant<int, Y::TK0>{}, std::integral_constant<int, 512>{});
@@ -303,7 +304,7 @@ void FourierSolver::launch_yfwd(int variant, const cplx<T>* H, cplx<T>* F) {
         if constexpr (N >= 64) {
             if (variant == 1) return go(std::integral_constant<int, Y::TK1>{}, std::integral_constant<int, 256>{});
             if (variant == 2) {
-                if constexpr (Y::fits(Y::TK2))
+                if constexpr (Y::template fits<Y::TK2>())
                     return go(std::integral_constant<int, Y::TK2>{}, std::integral_constant<int, 512>{});
             }
             if (variant == 3) return go(std::integral_constant<int, Y::TK0>{}, std::integral_constant<int, 512>{});

@@ -75,7 +75,7 @@ template <typename T, int N, int TK, int NT, bool INV>
 __global__ __launch_bounds__(NT) void k_c2c_axis(cplx<T>* __restrict__ a, size_t inner,
                                                  const cplx<T>* __restrict__ tw) {
     using V = cplx<T>;
-    constexpr int LS = line_stride(N);
+    constexpr int LS = line_stride_lf<T, TK>(N);  // the copies take the TK lines side by side
     __shared__ V buf[TK * LS];
     __shared__ V twl[N];
     const size_t i0 = (size_t)blockIdx.x * TK;
@@ -87,7 +87,7 @@ __global__ __launch_bounds__(NT) void k_c2c_axis(cplx<T>* __restrict__ a, size_t
         buf[c * LS + pidx(d)] = i < inner ? a[base + (size_t)d * inner + i] : V{T(0), T(0)};
     }
     __syncthreads();
-    fft_lds<T, N, TK, NT, INV>(buf, twl);
+    fft_lds<T, N, TK, NT, INV, true, LS>(buf, twl);
     const T s = INV ? T(1) / T(N) : T(1);
     for (int idx = threadIdx.x; idx < TK * N; idx += NT) {
         const int c = idx % TK, d = idx / TK;
