@@ -371,15 +371,27 @@ __global__ __launch_bounds__(NT, REFINE ? 1 : (SMALL || sizeof(T) == 8 ? 768 : 5
     // ---- b (parity h) and perp (parity g) systems ----
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nw = NT / 64, w0p = ((nb + 15) >> 4) % nw;
     const double kh = s.kh[key];
-    {
-        const size_t ob = (size_t)(2 * key + h) * s.sb, op = (size_t)(2 * key + g) * s.sb;
+    const size_t ob = (size_t)(2 * key + h) * s.sb, op = (size_t)(2 * key + g) * s.sb;
+    if constexpr (!REFINE) {
+        // the b system on the first half of the waves, the perp system on the second: each
+        // wave runs one load pipeline, with twice the tiles in flight
+        constexpr int MTH = 2 * MTB;
+        const int half = nw / 2;
+        const bool isb = wave < half;  // wave-uniform
+        dbl4 acc[MTH];
+        if (isb) ch3_apply<MTH, T, SPLIT, D>(s.Ib, s.Iblo, ob, nb, Yb, acc, wave, half, 0);
+        else ch3_apply<MTH, T, SPLIT, D>(s.Ip, s.Iplo, op, np, Yp, acc, wave - half, half, 0);
+        __syncthreads();  // the solutions overwrite the right-hand sides every wave of the group reads
+        if (isb) mfma_store<MTH>(acc, nb, Xb, wave, half, 0);
+        else mfma_store<MTH>(acc, np, Xp, wave - half, half, 0);
+    } else {
         dbl4 ab[MTB], ap[MTB];
         ch3_apply<MTB, T, SPLIT, D>(s.Ib, s.Iblo, ob, nb, Yb, ab, wave, nw, 0);
         ch3_apply<MTB, T, SPLIT, D>(s.Ip, s.Iplo, op, np, Yp, ap, wave, nw, w0p);
         __syncthreads();
         mfma_store<MTB>(ab, nb, Xb, wave, nw, 0);
         mfma_store<MTB>(ap, np, Xp, wave, nw, w0p);
-        for (int sw = 0; REFINE && sw < sweeps; ++sw) {
+        for (int sw = 0; sw < sweeps; ++sw) {
             __syncthreads();
             if (threadIdx.x < 16) chb_residual(N, h, kh, s.kappa, s.c, s.sz, Yb, Xb, kYS, 1, threadIdx.x);
             else if (threadIdx.x < 32) chb_residual(N, g, kh, s.nu, s.c, s.sz, Yp, Xp, kYS, 1, threadIdx.x - 16);
