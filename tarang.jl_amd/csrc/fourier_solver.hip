@@ -315,8 +315,9 @@ void FourierSolver::launch_yfwd(int variant, const cplx<T>* H, cplx<T>* F) {
 }
 
 // Times every variant of the three RHS kernels on this solver's own buffers (HIP
-// events, median of 5 launches) and keeps the fastest by a 3 % margin over the earlier
-// variants; results are cached per (problem, Nx, Ny, dtype) for the life of the process.
+// events, medians of 7 rounds of alternating launches) and keeps the fastest by a 3 %
+// margin over the earlier variants; results are cached per (problem, Nx, Ny, dtype) for
+// the life of the process.
 static std::map<std::array<int, 4>, FourierTuning>& tuning_cache() {
     static std::map<std::array<int, 4>, FourierTuning> m;
     return m;
