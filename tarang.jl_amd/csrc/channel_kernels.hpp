@@ -247,12 +247,12 @@ __device__ inline void zbwd_mfma(const cplx<T>* __restrict__ b, const cplx<T>* _
 // k_ch_zbwd: the 2-D z-backward transform, zbwd_mfma with planes 0 ux, 1 uz, 2 dx b,
 // 3 dz b, 4 dx ux, 5 dz ux, 6 dx uz, 7 dz uz (kx fastest, one N x nxh plane each).
 // grid (ceil(nxh / QT), 3 fields, z shares).
-template <typename T, int NT>
+template <typename T, int NT, int QT = ZbwdTile<T>::QT>
 __global__ __launch_bounds__(NT, 768 / NT) void k_ch_zbwd(const cplx<T>* __restrict__ b, const cplx<T>* __restrict__ ux,
                                                    const cplx<T>* __restrict__ uz, Ch3Geom geo,
                                                    const T* __restrict__ BT, const T* __restrict__ BDT,
                                                    cplx<T>* __restrict__ G) {
-    zbwd_mfma<T, NT, true>(b, ux, nullptr, uz, geo, BT, BDT, G);
+    zbwd_mfma<T, NT, true, QT>(b, ux, nullptr, uz, geo, BT, BDT, G);
 }
 
 // ---------------------------------------------------------------------------

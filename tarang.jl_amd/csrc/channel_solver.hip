@@ -270,17 +270,18 @@ void ChannelSolver::rhs_t(const void* b, const void* ux, const void* uz, void* F
     V* G = G_.as<V>();
     V* H = H_.as<V>();
     {
-        // Chebyshev evaluation on the matrix cores: tiles of QT modes x 3 fields, and the z
-        // tiles shared out over workgroups so that the few modes of a 2-D grid fill the GPU
-        constexpr int QT = ZbwdTile<T>::QT, NTZ = ZbwdTile<T>::NT;
+        // Chebyshev evaluation on the matrix cores: tiles of 16 modes x 3 fields (the few
+        // modes of a 2-D grid fill more CUs in small tiles), and the z tiles shared out over
+        // workgroups
+        constexpr int QT = 16, NTZ = ZbwdTile<T>::NT;
         constexpr int ZPASS = NTZ / 64;  // first-half z tiles of one pass of the waves
         const size_t blds = (size_t)((N + 15) & ~15) * zbwd_lds_stride(QT) * sizeof(T);
         const unsigned zsplit = (unsigned)std::max(1, (zbwd_half_tiles(N) + ZPASS - 1) / ZPASS);
         const Ch3Geom geo{d_.nx, 1, nxh_, N, nxh_, 2.0 * M_PI / d_.Lx, 0.0};
         timed(0, [&] {
-            hipLaunchKernelGGL((k_ch_zbwd<T, NTZ>), dim3((unsigned)((nxh_ + QT - 1) / QT), 3, zsplit), dim3(NTZ), blds,
-                               s, static_cast<const V*>(b), static_cast<const V*>(ux), static_cast<const V*>(uz), geo,
-                               BT_.as<const T>(), BDT_.as<const T>(), G);
+            hipLaunchKernelGGL((k_ch_zbwd<T, NTZ, QT>), dim3((unsigned)((nxh_ + QT - 1) / QT), 3, zsplit), dim3(NTZ),
+                               blds, s, static_cast<const V*>(b), static_cast<const V*>(ux), static_cast<const V*>(uz),
+                               geo, BT_.as<const T>(), BDT_.as<const T>(), G);
         });
     }
     TG_HIP(hipGetLastError());
