@@ -37,6 +37,21 @@ def _netcdf():
     return netcdf_file
 
 
+_HDF5_SIGNATURE = b"\x89HDF\r\n\x1a\n"
+
+
+def _open_read(path: str):
+    """NetCDF classic file for reading; a NetCDF-4 (HDF5) file is refused with the way to
+    convert it, since no HDF5 reader is available here."""
+    with open(path, "rb") as f:
+        head = f.read(8)
+    if head == _HDF5_SIGNATURE:
+        raise ValueError(f"{path} is a NetCDF-4 (HDF5) file; only NetCDF classic files can be read here. "
+                         f"Convert it with `nccopy -k classic {path} out.nc`, or write it with "
+                         f"mode = NC_CLASSIC_MODEL")
+    return _netcdf()(path, "r", mmap=False)
+
+
 def _target(path: str) -> str:
     stem = path[:-3] if path.endswith(".nc") else path
     if os.path.isdir(stem):
@@ -105,7 +120,7 @@ def save_field(field: ScalarField, filename: str, dataset_name: str = "field") -
 def load_field_(field: ScalarField, filename: str, dataset_name: str = "field") -> ScalarField:
     """Read variable `dataset_name` into the field's grid data (field_netcdf_io.jl:255-272)."""
     path = _source(filename)
-    with _netcdf()(path, "r", mmap=False) as nc:
+    with _open_read(path) as nc:
         _read_var(nc, dataset_name, field, path)
     return field
 
@@ -136,7 +151,7 @@ def load_state_(solver, path: str):
     if not fields:
         raise ValueError("load_state_: solver.state is empty; nothing to restore.")
     src = _source(path)
-    with _netcdf()(src, "r", mmap=False) as nc:
+    with _open_read(src) as nc:
         for f in fields:
             _read_var(nc, f.name, f, src)
         attrs = nc._attributes

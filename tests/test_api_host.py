@@ -173,3 +173,13 @@ def test_tau_field_signatures():
     assert tp.name == "tau_p" and tp.bases == () and tp.dtype == np.float64
     assert tb.bases == (xb,) and tb.name == "tau_b1"
     assert [c.name for c in tu.components] == ["tau_u1_x", "tau_u1_z"] and tu.components[0].bases == (xb,)
+
+
+def test_netcdf4_checkpoint_refused_with_conversion_hint(tmp_path):
+    """A NetCDF-4 (HDF5) file, the default of the reference's nccreate, is refused before any
+    field is touched, with the classic-format conversion named."""
+    from tarang_amd import checkpoint
+    p = tmp_path / "state.nc"
+    p.write_bytes(b"\x89HDF\r\n\x1a\n" + bytes(64))
+    with pytest.raises(ValueError, match="NetCDF-4.*nccopy -k classic"):
+        checkpoint.load_field_(None, str(p), "b")
