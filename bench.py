@@ -1020,6 +1020,56 @@ def bench_plugins(a):
         del grid, coeff, back
     torch.cuda.empty_cache()
 
+    # ---- ChebyshevT axis: dense matrix (method 1) against one FFT of length 2(n-1) per column
+    # (method 2), RealFourier^2 x ChebyshevT 256 x 256 x n at the n whose 2(n-1) the LDS FFT takes
+    for nzc in (17, 33, 65, 129, 257, 513, 769):
+        for tdt, ct, code, esz in ((torch.float64, torch.complex128, _lib.F64, 8),
+                                   (torch.float32, torch.complex64, _lib.F32, 4)):
+            d = _lib.DomainDesc()
+            d.ndim = 3
+            for i, (kind, n_, hi) in enumerate(((_lib.REALFOURIER, nx, 4.0), (_lib.REALFOURIER, ny, 4.0),
+                                                (_lib.CHEBYSHEVT, nzc, 1.0))):
+                d.kind[i], d.n[i], d.lo[i], d.hi[i], d.dealias[i] = kind, n_, 0.0, hi, 1.5
+            d.dtype = code
+            plan = ctypes.c_void_p()
+            _lib.call("tarang_plan_create", ctx, ctypes.byref(d), ctypes.byref(plan))
+            grid = torch.rand((nzc, ny, nx), dtype=torch.float64, device=dev, generator=g).to(tdt)
+            coeff = torch.empty((nzc, ny, nx // 2 + 1), dtype=ct, device=dev)
+            back = torch.empty_like(grid)
+            res = {}
+            for method, mname in ((1, "dense"), (2, "fft"), (0, "automatic")):
+                _lib.call("tarang_plan_set_cheb_method", plan, method)
+                fwd = lambda: _lib.call("tarang_forward_transform", plan, ctypes.c_void_p(grid.data_ptr()),  # noqa
+                                        ctypes.c_void_p(coeff.data_ptr()))
+                bwd = lambda: _lib.call("tarang_backward_transform", plan, ctypes.c_void_p(coeff.data_ptr()),  # noqa
+                                        ctypes.c_void_p(back.data_ptr()))
+                for _ in range(max(a.warmup, 3)):
+                    fwd()
+                    bwd()
+                torch.cuda.synchronize()
+                # alternate the two directions' timing rounds so that drift falls on both
+                tf, tb = [], []
+                for _ in range(3):
+                    tf.append(timed(fwd, reps))
+                    tb.append(timed(bwd, reps))
+                res[mname] = {"forward_ms": sorted(tf)[1], "backward_ms": sorted(tb)[1],
+                              "round_trip_rel_err": float(((back - grid).abs().max() / grid.abs().max()).item())}
+            nbytes = nx * ny * nzc * esz + (nx // 2 + 1) * ny * nzc * 2 * esz
+            print(json.dumps({
+                "metric": "ChebyshevT transform forms",
+                "value": nx * ny * nzc / (res["automatic"]["forward_ms"] * 1e-3),
+                "unit": "grid points/s (forward, automatic form)", "n_gpus": 1, "steps": reps,
+                "warmup": max(a.warmup, 3), "ms_per_step": res["automatic"]["forward_ms"], "higher_is_better": True,
+                "scaling": "weak", "vs_baseline": None, "dtype": "f64" if code == _lib.F64 else "f32",
+                "data": "synthetic uniform(0,1)",
+                "config": {"workload": f"tarang_forward/backward_transform, RealFourier^2 x ChebyshevT "
+                                       f"{nx}x{ny}x{nzc}: dense Chebyshev matrix vs FFT of length {2 * (nzc - 1)}"},
+                "forms": res, "algorithmic_bytes_per_transform": nbytes,
+                "roofline": None, "cpu_baseline": None}), flush=True)
+            _lib.call("tarang_plan_destroy", plan)
+            del grid, coeff, back
+    torch.cuda.empty_cache()
+
     # ---- batched complex LU, 2-D channel stage matrices: 513 modes, n = 1031 ----
     n, nm = 1031, 513
     log(f"batched LU: {nm} modes of n={n} ({nm * n * n * 16 / 1e9:.2f} GB per copy)")

@@ -254,6 +254,13 @@ int tarang_plan_coeff_shape(tarang_plan_t plan, int* shape3, int* is_complex) {
     });
 }
 
+int tarang_plan_set_cheb_method(tarang_plan_t plan, int method) {
+    return guard([&] {
+        NEED(plan, "plan");
+        plan->plan->set_cheb_method(method);
+    });
+}
+
 int tarang_forward_transform(tarang_plan_t plan, const void* grid, void* coeff) {
     return guard([&] {
         NEED(plan, "plan");

@@ -1,5 +1,6 @@
-// transforms.hip -- field transforms (RealFourier rfft/fft, ChebyshevT DCT-I as
-// an exact matrix along z) and the standalone dealiased product.
+// transforms.hip -- field transforms (RealFourier rfft/fft, ChebyshevT DCT-I as one FFT of
+// length 2(n-1) per column where the LDS FFT has that length and n >= 65, else as the exact
+// matrix on the matrix cores) and the standalone dealiased product.
 //
 // Conventions follow the reference CPU chain:
 //   forward: axis 0 rfft (unnormalised), later RealFourier axes complex FFT,
@@ -158,6 +159,54 @@ __global__ __launch_bounds__(NT) void k_cheb_axis(const T* __restrict__ in, T* _
     }
 }
 
+// ChebyshevT transform along a strided axis, (inner, n, outer) complex, through one FFT of
+// length M = 2(n - 1) per column of the column's even extension g (g_j = g_{M-j}): that is
+// REDFT00, and since a real even sequence has a real even DFT, the real and imaginary parts
+// of a complex column transform side by side in one complex FFT.
+//   forward:  c_k = (-1)^k w_k/(n-1) DFT(g)_k, g the extension of the grid values f_j
+//   backward: f_j = DFT(h)_j, h the extension of (-1)^k c_k halved at 0 < k < n-1
+// (the same sums as the dense matrices of cheb_matrix).  TK columns per workgroup, read
+// whole into LDS before any output is written, so out may equal in.
+template <typename T, int M, int TK, int NT>
+__global__ __launch_bounds__(NT) void k_cheb_fft_axis(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out,
+                                                      size_t inner, const cplx<T>* __restrict__ tw, bool fwd) {
+    using V = cplx<T>;
+    constexpr int n = M / 2 + 1;
+    constexpr int LS = line_stride_lf<T, TK>(M);
+    __shared__ V buf[TK * LS];
+    __shared__ V twl[M];
+    const size_t i0 = (size_t)blockIdx.x * TK;
+    const size_t base = (size_t)blockIdx.y * n * inner;
+    load_twiddles<T, M, NT>(twl, tw);
+    for (int idx = threadIdx.x; idx < TK * n; idx += NT) {
+        const int c = idx % TK, d = idx / TK;
+        const size_t i = i0 + c;
+        V v = i < inner ? in[base + (size_t)d * inner + i] : V{T(0), T(0)};
+        if (!fwd) {
+            const T sg = (d & 1) ? T(-1) : T(1);
+            const T h = (d == 0 || d == n - 1) ? sg : sg * T(0.5);
+            v = V{v.x * h, v.y * h};
+        }
+        buf[c * LS + pidx(d)] = v;
+        if (d > 0 && d < n - 1) buf[c * LS + pidx(M - d)] = v;
+    }
+    __syncthreads();
+    fft_lds<T, M, TK, NT, false, true, LS>(buf, twl);
+    const T inv = T(1) / T(n - 1);
+    for (int idx = threadIdx.x; idx < TK * n; idx += NT) {
+        const int c = idx % TK, k = idx / TK;
+        const size_t i = i0 + c;
+        if (i < inner) {
+            V v = buf[c * LS + pidx(k)];
+            if (fwd) {
+                const T w = ((k & 1) ? -inv : inv) * ((k == 0 || k == n - 1) ? T(0.5) : T(1));
+                v = V{v.x * w, v.y * w};
+            }
+            out[base + (size_t)k * inner + i] = v;
+        }
+    }
+}
+
 template <typename T>
 __global__ void k_pointwise_mul(const T* a, const T* b, T* out, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -262,12 +311,50 @@ void TransformPlan::axis_fft(void* data, int axis, bool inverse) {
     TG_HIP(hipGetLastError());
 }
 
+// the FFT form of a ChebyshevT axis exists where 2(n - 1) is one of the LDS FFT's lengths
+bool TransformPlan::cheb_fft_ok(int n) {
+    bool ok = false;
+    if (n >= 5) dispatch_size(2 * (n - 1), [&](auto) { ok = true; }, Sizes1D{});
+    return ok;
+}
+
+void TransformPlan::set_cheb_method(int method) {
+    TG_REQUIRE(method >= kChebAuto && method <= kChebFFT, kErrInvalid,
+               "Chebyshev method is 0 (automatic), 1 (dense matrix) or 2 (FFT of the even extension)");
+    if (method == kChebFFT)
+        for (int a = 0; a < d_.ndim; ++a)
+            TG_REQUIRE(d_.kind[a] != kChebyshevT || cheb_fft_ok(d_.n[a]), kErrInvalid,
+                       "Chebyshev FFT form needs 2(n-1) = 2^a 3^b 5^c between 8 and 4096 on every ChebyshevT axis");
+    cheb_method_ = method;
+}
+
+bool TransformPlan::cheb_uses_fft(int n) const {
+    if (cheb_method_ == kChebFFT) return true;
+    if (cheb_method_ == kChebDense) return false;
+    return cheb_fft_ok(n) && n >= kChebFFTMinN;
+}
+
 template <typename T>
 void TransformPlan::axis_cheb(const void* in, void* out, int axis, bool forward) {
     size_t inner = 1, outer = 1;
     for (int a = 0; a < axis; ++a) inner *= (size_t)cshape_[a];
     for (int a = axis + 1; a < d_.ndim; ++a) outer *= (size_t)cshape_[a];
     const int n = d_.n[axis];
+    if (cheb_uses_fft(n)) {
+        using V = cplx<T>;
+        const DType dt = std::is_same<T, float>::value ? kF32 : kF64;
+        constexpr int NT = 256;
+        dispatch_size(2 * (n - 1), [&](auto MC) {
+            constexpr int M = decltype(MC)::value;
+            constexpr int TK = tile_cols<T, M>();
+            const V* tw = static_cast<const V*>(ctx_->twiddle(M, dt));
+            hipLaunchKernelGGL((k_cheb_fft_axis<T, M, TK, NT>), dim3((unsigned)((inner + TK - 1) / TK), (unsigned)outer),
+                               dim3(NT), 0, ctx_->stream, static_cast<const V*>(in), static_cast<V*>(out), inner, tw,
+                               forward);
+        }, Sizes1D{});
+        TG_HIP(hipGetLastError());
+        return;
+    }
     const T* mt = static_cast<const T*>(cheb_matrix(n, forward));
     const size_t ncols = 2 * inner, KP = (size_t)((n + 15) & ~15);
     constexpr int NT = 256;

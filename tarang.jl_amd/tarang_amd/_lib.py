@@ -127,6 +127,7 @@ SIGNATURES = {
     "tarang_plan_create": [_P, ctypes.POINTER(DomainDesc), _PP],
     "tarang_plan_destroy": [_P],
     "tarang_plan_coeff_shape": [_P, _PI, _PI],
+    "tarang_plan_set_cheb_method": [_P, ctypes.c_int],
     "tarang_forward_transform": [_P, _P, _P],
     "tarang_backward_transform": [_P, _P, _P],
     "tarang_padded_multiply": [_P, _P, _P, _P],

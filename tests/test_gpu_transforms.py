@@ -38,6 +38,49 @@ def test_transform_parity(gpu, oracle, dtype, kinds, sizes, bounds):
     p.close()
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("kinds,sizes", [
+    ([RF, CH], [64, 5]),
+    ([RF, CH], [64, 17]),
+    ([RF, CH], [256, 65]),
+    ([RF, CH], [96, 129]),
+    ([RF, CH], [32, 769]),
+    ([RF, RF, CH], [32, 16, 33]),
+    ([RF, RF, CH], [64, 48, 257]),
+])
+def test_chebyshev_fft_form_parity(gpu, oracle, dtype, kinds, sizes):
+    """ChebyshevT axes through one FFT of length 2(n-1) per column (method 2) and through the
+    dense matrix (method 1), forward and backward, against the oracle's REDFT00 sums."""
+    from tarang_amd import _lib
+    print(f"chebyshev fft form {kinds} {sizes} {dtype.__name__}", flush=True)
+    r = np.random.default_rng(17)
+    g = r.standard_normal(sizes)
+    bounds = [(0.0, 4.0)] * (len(sizes) - 1) + [(0.0, 1.0)]
+    d = oracle.domain(kinds, sizes, bounds=bounds)
+    C_ref = oracle.forward_transform(d, g)
+    G_ref = oracle.backward_transform(d, C_ref)
+    p = gpu.plan(kinds, sizes, bounds=bounds, dtype=dtype)
+    for method in (2, 1):
+        _lib.call("tarang_plan_set_cheb_method", p.h, method)
+        C = p.forward(g)
+        ef, eb = relerr(C, C_ref), relerr(p.backward(C_ref), G_ref)
+        print(f"  method {method}: forward {ef:.2e} backward {eb:.2e}", flush=True)
+        assert ef < TOL[dtype] and eb < TOL[dtype]
+    p.close()
+
+
+def test_chebyshev_method_checks(gpu):
+    """The FFT form exists only where 2(n-1) is an LDS FFT length (n = 128: 254 = 2 127)."""
+    from tarang_amd import _lib
+    p = gpu.plan([RF, CH], [32, 128], bounds=[(0.0, 4.0), (0.0, 1.0)])
+    with pytest.raises(_lib.TarangError):
+        _lib.call("tarang_plan_set_cheb_method", p.h, 2)
+    with pytest.raises(_lib.TarangError):
+        _lib.call("tarang_plan_set_cheb_method", p.h, 3)
+    _lib.call("tarang_plan_set_cheb_method", p.h, 1)
+    p.close()
+
+
 def test_transform_mode_ordering_bit_exact(gpu):
     # a single Fourier mode lands on exactly one coefficient index (FFT order on y)
     nx, ny = 16, 8
