@@ -81,7 +81,7 @@ int tarang_plan_create(tarang_ctx_t ctx, const tarang_domain_desc* desc, tarang_
 int tarang_plan_destroy(tarang_plan_t plan);
 /* coefficient shape (n0/2+1, n1, n2) and whether it is complex (always 1 here) */
 int tarang_plan_coeff_shape(tarang_plan_t plan, int* shape3, int* is_complex);
-/* how ChebyshevT axes transform: 0 automatic (the FFT form where it exists and n >= 65, where
+/* how ChebyshevT axes transform: 0 automatic (the FFT form where it exists and n >= 17, where
  * it was measured faster; else the dense matrix), 1 the dense matrix on the matrix cores, 2 one FFT of length
  * 2(n-1) of each column's even extension (REDFT00, the reference's FFTW plan kind,
  * transform_chebyshev.jl:428-486); 2 needs 2(n-1) = 2^a 3^b 5^c in 8..4096.  The forms round

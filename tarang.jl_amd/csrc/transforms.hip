@@ -1,5 +1,5 @@
 // transforms.hip -- field transforms (RealFourier rfft/fft, ChebyshevT DCT-I as one FFT of
-// length 2(n-1) per column where the LDS FFT has that length and n >= 65, else as the exact
+// length 2(n-1) per column where the LDS FFT has that length and n >= 17, else as the exact
 // matrix on the matrix cores) and the standalone dealiased product.
 //
 // Conventions follow the reference CPU chain:
@@ -100,22 +100,28 @@ __global__ __launch_bounds__(NT) void k_c2c_axis(cplx<T>* __restrict__ a, size_t
     }
 }
 
-// Matrix along one axis on the matrix cores: out[k][c] = sum_j M[k][j] in[j][c] for each
-// outer index, the n x ncols real block of complex data (inner, n, outer) with its real
-// and imaginary parts as interleaved columns (ncols = 2 inner).  Workgroup = (tile of
-// 16 CT columns, outer index), every k of the tile: the tile's input is read whole into
-// LDS before any output is written, so out may equal in.  MT[j n + k] = M[k][j] (lanes
-// of one k-tile read consecutive entries); wave w takes k tiles w, w + NT/64, ...; the
-// loads of U reduction steps are issued before their products.
+// The Chebyshev matrices along one axis on the matrix cores, halved by the mirror symmetry
+// of the Gauss-Lobatto grid (T_k(-x) = (-1)^k T_k(x)): with h = ceil(n/2),
+//   forward:  c_2t = sum_{r<h} E[t][r] e_r,  c_2t+1 = sum_{r<n/2} O[t][r] o_r, where
+//             e_r = f_r + f_{n-1-r} and o_r = f_r - f_{n-1-r} (e = f at the middle of odd n)
+//   backward: with E_t = sum_{r<h} E[t][r] c_2r and O_t = sum_{r<n/2} O[t][r] c_2r+1,
+//             f_t = E_t + O_t and f_{n-1-t} = E_t - O_t (f = E at the middle of odd n)
+// for the n x ncols real block of complex data (inner, n, outer) with its real and imaginary
+// parts as interleaved columns (ncols = 2 inner).  Workgroup = (tile of 16 CT columns, outer
+// index), all outputs of the tile: the tile's input is read whole into LDS before any output
+// is written, so out may equal in.  AT = [E^T (h x h) | O^T (n/2 x h)], AT[r h + t]; wave w
+// takes t tiles w, w + NT/64, ...; the loads of U reduction steps are issued before their
+// products.
 template <typename T, int CT, int NT>
 __global__ __launch_bounds__(NT) void k_cheb_axis(const T* __restrict__ in, T* __restrict__ out, size_t ncols, int n,
-                                                  const T* __restrict__ MT) {
+                                                  const T* __restrict__ AT, bool fwd) {
     using Op = MfmaOp<T>;
     using acc4 = typename Op::acc4;
     constexpr int W = 16 * CT, S = W + 1, NWV = NT / 64, U = 4;
     extern __shared__ unsigned char smem_raw[];
     T* C = reinterpret_cast<T*>(smem_raw);  // [KP][S]
-    const int KP = (n + 15) & ~15, nkt = KP >> 4, ksteps = (n + 3) >> 2;
+    const int KP = (n + 15) & ~15, h = (n + 1) / 2, ho = n / 2, nkt = (h + 15) >> 4;
+    const T* AO = AT + (size_t)h * h;
     const size_t c0 = (size_t)blockIdx.x * W;
     const size_t base = (size_t)blockIdx.y * n * ncols;
     for (int idx = threadIdx.x; idx < KP * W; idx += NT) {
@@ -125,36 +131,63 @@ __global__ __launch_bounds__(NT) void k_cheb_axis(const T* __restrict__ in, T* _
         C[j * S + c] = v;
     }
     __syncthreads();
+    if (fwd) {  // e_r at row r, o_r at row n-1-r
+        for (int idx = threadIdx.x; idx < ho * W; idx += NT) {
+            const int r = idx / W, c = idx - r * W;
+            const T a = C[r * S + c], b = C[(n - 1 - r) * S + c];
+            C[r * S + c] = a + b;
+            C[(n - 1 - r) * S + c] = a - b;
+        }
+        __syncthreads();
+    }
     const int lane = threadIdx.x & 63, r16 = lane & 15, q4 = lane >> 4;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (int kt = wave; kt < nkt; kt += NWV) {
-        const int k = 16 * kt + r16;
-        acc4 acc[CT];
+        const int t = 16 * kt + r16;
+        acc4 ae[CT], ao[CT];
 #pragma unroll
-        for (int ct = 0; ct < CT; ++ct) acc[ct] = acc4{T(0), T(0), T(0), T(0)};
-        for (int s0 = 0; s0 < ksteps; s0 += U) {
-            T am[U];
+        for (int ct = 0; ct < CT; ++ct) ae[ct] = ao[ct] = acc4{T(0), T(0), T(0), T(0)};
+        // acc += A^T rows: reduction index r < R reads LDS row rb + rs r
+        auto gemm = [&](acc4(&acc)[CT], const T* A, int R, int rb, int rs) {
+            const int steps = (R + 3) >> 2;
+            for (int s0 = 0; s0 < steps; s0 += U) {
+                T am[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int j = 4 * (s0 + u) + q4;
-                const bool ok = j < n && k < n;
-                am[u] = MT[ok ? (size_t)j * n + k : 0];
-                am[u] = ok ? am[u] : T(0);
+                for (int u = 0; u < U; ++u) {
+                    const int r = 4 * (s0 + u) + q4;
+                    const bool ok = r < R && t < h;
+                    am[u] = A[ok ? (size_t)r * h + t : 0];
+                    am[u] = ok ? am[u] : T(0);
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    int row = rb + rs * (4 * (s0 + u) + q4);  // past R, a is zero
+                    row = row < 0 ? 0 : (row < KP ? row : KP - 1);
+#pragma unroll
+                    for (int ct = 0; ct < CT; ++ct) acc[ct] = Op::mma(am[u], C[row * S + 16 * ct + r16], acc[ct]);
+                }
             }
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int j = 4 * (s0 + u) + q4, jr = j < KP ? j : KP - 1;  // past n, a is zero
-#pragma unroll
-                for (int ct = 0; ct < CT; ++ct) acc[ct] = Op::mma(am[u], C[jr * S + 16 * ct + r16], acc[ct]);
-            }
-        }
+        };
+        // the even part (h terms), then the odd part (n/2 terms)
+        gemm(ae, AT, h, 0, fwd ? 1 : 2);
+        gemm(ao, AO, ho, fwd ? n - 1 : 1, fwd ? -1 : 2);
 #pragma unroll
         for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int kk = 16 * kt + Op::row(v, q4);
+                const int tt = 16 * kt + Op::row(v, q4);
                 const size_t c = c0 + 16 * ct + r16;
-                if (kk < n && c < ncols) out[base + (size_t)kk * ncols + c] = acc[ct][v];
+                if (tt >= h || c >= ncols) continue;
+                T* o = out + base + c;
+                if (fwd) {
+                    o[(size_t)(2 * tt) * ncols] = ae[ct][v];
+                    if (2 * tt + 1 < n) o[(size_t)(2 * tt + 1) * ncols] = ao[ct][v];
+                } else if (tt < ho) {
+                    o[(size_t)tt * ncols] = ae[ct][v] + ao[ct][v];
+                    o[(size_t)(n - 1 - tt) * ncols] = ae[ct][v] - ao[ct][v];
+                } else {
+                    o[(size_t)tt * ncols] = ae[ct][v];  // the middle point of odd n
+                }
             }
     }
 }
@@ -251,41 +284,38 @@ size_t TransformPlan::coeff_elems() const {
     return t;
 }
 
+// [E^T | O^T] of k_cheb_axis for the forward (grid -> coefficients) or backward matrix
 const void* TransformPlan::cheb_matrix(int n, bool forward) {
     const bool f32 = d_.dtype == kF32;
-    const std::string key = std::string("chebT_") + (forward ? "fwd_" : "bwd_") + std::to_string(n) + (f32 ? "_f32" : "_f64");
+    const std::string key = std::string("chebTps_") + (forward ? "fwd_" : "bwd_") + std::to_string(n) + (f32 ? "_f32" : "_f64");
     if (void* t = ctx_->find_table(key)) return t;  // built once per context: the host work is O(n^2)
-    std::vector<double> m((size_t)n * n);
     const long period = 2L * (n - 1);
     auto cospi = [&](long num) {  // cos(pi num / (n-1)) with exact range reduction
         const long r = ((num % period) + period) % period;
         return (double)cosl(3.14159265358979323846264338327950288L * (long double)r / (long double)(n - 1));
     };
-    for (int k = 0; k < n; ++k) {
-        for (int j = 0; j < n; ++j) {
-            if (forward) {
-                const double sgn = (k & 1) ? -1.0 : 1.0;
-                const double wk = (k == 0 || k == n - 1) ? 0.5 : 1.0;
-                double e;
-                if (j == 0) e = 1.0;
-                else if (j == n - 1) e = sgn;
-                else e = 2.0 * cospi((long)j * k);
-                m[(size_t)k * n + j] = sgn * wk / (n - 1) * e;
-            } else {  // row k = grid point, column j = Chebyshev degree: (-1)^j cos(pi jk/(n-1))
-                const double sgn = (j & 1) ? -1.0 : 1.0;
-                m[(size_t)k * n + j] = sgn * cospi((long)j * k);
-            }
-        }
+    // full matrices: forward F[k][j] = (-1)^k w_k/(n-1) (REDFT00 row k), backward B[j][k] =
+    // (-1)^k cos(pi jk/(n-1)) (grid point j, degree k)
+    auto F = [&](int k, int j) {
+        const double sgn = (k & 1) ? -1.0 : 1.0;
+        const double wk = (k == 0 || k == n - 1) ? 0.5 : 1.0;
+        const double e = j == 0 ? 1.0 : (j == n - 1 ? sgn : 2.0 * cospi((long)j * k));
+        return sgn * wk / (n - 1) * e;
+    };
+    auto B = [&](int j, int k) { return ((k & 1) ? -1.0 : 1.0) * cospi((long)j * k); };
+    const int h = (n + 1) / 2, ho = n / 2;
+    std::vector<double> at((size_t)(h + ho) * h, 0.0);
+    for (int t = 0; t < h; ++t) {
+        for (int r = 0; r < h; ++r)  // even part
+            at[(size_t)r * h + t] = forward ? F(2 * t, r) : B(t, 2 * r);
+        for (int r = 0; r < ho; ++r)  // odd part
+            if (forward ? 2 * t + 1 < n : true) at[(size_t)(h + r) * h + t] = forward ? F(2 * t + 1, r) : B(t, 2 * r + 1);
     }
-    // stored transposed, MT[j n + k] = M[k][j], for the matrix-core kernel's loads
-    std::vector<double> mt((size_t)n * n);
-    for (int k = 0; k < n; ++k)
-        for (int j = 0; j < n; ++j) mt[(size_t)j * n + k] = m[(size_t)k * n + j];
     if (f32) {
-        std::vector<float> h(mt.begin(), mt.end());
-        return ctx_->table(key, h.data(), h.size() * sizeof(float));
+        std::vector<float> hf(at.begin(), at.end());
+        return ctx_->table(key, hf.data(), hf.size() * sizeof(float));
     }
-    return ctx_->table(key, mt.data(), mt.size() * sizeof(double));
+    return ctx_->table(key, at.data(), at.size() * sizeof(double));
 }
 
 template <typename T>
@@ -364,7 +394,7 @@ void TransformPlan::axis_cheb(const void* in, void* out, int axis, bool forward)
         const size_t lds = KP * (16 * CT + 1) * sizeof(T);
         dim3 grid((unsigned)((ncols + 16 * CT - 1) / (16 * CT)), (unsigned)outer);
         hipLaunchKernelGGL((k_cheb_axis<T, CT, NT>), grid, dim3(NT), lds, ctx_->stream, static_cast<const T*>(in),
-                           static_cast<T*>(out), ncols, n, mt);
+                           static_cast<T*>(out), ncols, n, mt, forward);
     };
     if (KP * 65 * sizeof(T) <= 64 * 1024) launch(std::integral_constant<int, 4>{});
     else if (KP * 33 * sizeof(T) <= 80 * 1024) launch(std::integral_constant<int, 2>{});

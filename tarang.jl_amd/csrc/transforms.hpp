@@ -42,9 +42,9 @@ class TransformPlan {
     template <typename T> void axis_cheb(const void* in, void* out, int axis, bool forward);  // out may be in
     const void* cheb_matrix(int n, bool forward);
     bool cheb_uses_fft(int n) const;
-    // measured on 256 x 256 x n (k_cheb_axis against k_cheb_fft_axis): the FFT form is faster
-    // from n = 65 (FP64 21 against 40 us, FP32 17 against 23 us) and level or slower at 17 and 33
-    static constexpr int kChebFFTMinN = 65;
+    // measured on 256 x 256 x n (k_cheb_axis against k_cheb_fft_axis): the FFT form is level at
+    // n = 17 (FP64 12.2 against 12.4 us) and faster from n = 33 (16.2 against 20.9 us)
+    static constexpr int kChebFFTMinN = 17;
     int cheb_method_ = kChebAuto;
 
     Context* ctx_;
