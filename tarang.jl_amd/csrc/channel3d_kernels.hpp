@@ -519,7 +519,8 @@ __global__ void k3_sbdf(Ch3SbdfArgs<T> a, size_t n) {
 
 // L P(x) per mode (one workgroup per mode) for the RK stage loop; P is the grid round
 // trip (Hermitian part of the kx = 0 and kx = Nx/2 lines, through the partner mode
-// (m, -n)); tau_p has no basis and is not projected.
+// (m, -n)); tau_p has no basis and is not projected.  Dz and Dzz = Dz(Dz .) by
+// cheb_dz_wave, one wavefront per (field, real / imaginary part).
 //   E1  = i kx ux + i ky uy + Dz uz + [N-1] tau_u1z + [k=0, mode 0] tau_p
 //   E2  = kappa (kh^2 b - Dzz b) - kappa Dz[:,N-1] tau_b1 + [N-1] tau_b2
 //   E3a = nu (kh^2 ua - Dzz ua) - nu Dz[:,N-1] tau_u1a + i ka p + [N-1] tau_u2a   (a = x, y)
@@ -532,8 +533,10 @@ __global__ __launch_bounds__(NT) void k3_lx(const cplx<T>* __restrict__ p, const
                                             cplx<T>* E2, cplx<T>* E3x, cplx<T>* E3y, cplx<T>* E3z) {
     extern __shared__ unsigned char smem_raw[];
     const int N = geo.N, nxh = geo.nxh;
+    static_assert(NT % 64 == 0, "whole wavefronts");
     double2* v = reinterpret_cast<double2*>(smem_raw);  // [5][N]: p, b, ux, uy, uz
     double2* d1 = v + 5 * N;                             // Dz of each
+    double2* d2 = d1 + 5 * N;                            // Dzz of b, ux, uy, uz
     const int mode = blockIdx.x, m = mode % nxh, n = mode / nxh;
     const bool xline = m == 0 || ((geo.nx % 2) == 0 && m == geo.nx / 2);
     const int partner = m + nxh * ((geo.ny - n) % geo.ny);
@@ -549,16 +552,14 @@ __global__ __launch_bounds__(NT) void k3_lx(const cplx<T>* __restrict__ p, const
         v[idx] = w;
     }
     __syncthreads();
-    for (int idx = threadIdx.x; idx < 5 * N; idx += NT) {
-        const int f = idx / N, k = idx - f * N;
-        double2 acc{0, 0};
-        for (int j = k + 1; j < N; j += 2) {
-            const double c = cheb_dz(k, j, sz);
-            acc.x += c * v[f * N + j].x;
-            acc.y += c * v[f * N + j].y;
-        }
-        d1[idx] = acc;
-    }
+    // z derivatives by wavefront suffix scans, one wavefront per (field, real / imaginary part)
+    for (int q = threadIdx.x >> 6; q < 10; q += NT / 64)
+        cheb_dz_wave(reinterpret_cast<const double*>(v + (q >> 1) * N) + (q & 1),
+                     reinterpret_cast<double*>(d1 + (q >> 1) * N) + (q & 1), 2, N, sz);
+    __syncthreads();
+    for (int q = threadIdx.x >> 6; q < 8; q += NT / 64)
+        cheb_dz_wave(reinterpret_cast<const double*>(d1 + (1 + (q >> 1)) * N) + (q & 1),
+                     reinterpret_cast<double*>(d2 + (q >> 1) * N) + (q & 1), 2, N, sz);
     __syncthreads();
     double2 tau[kCh3TauStride];
     for (int t = 0; t < kCh3TauStride; ++t) {
@@ -572,15 +573,7 @@ __global__ __launch_bounds__(NT) void k3_lx(const cplx<T>* __restrict__ p, const
     }
     const double kx = geo.k0x * m, ky = ch3_ky(geo, n), kh2 = kx * kx + ky * ky;
     for (int k = threadIdx.x; k < N; k += NT) {
-        double2 dzz[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};  // Dzz of b, ux, uy, uz
-        for (int j = k + 1; j < N; j += 2) {
-            const double c = cheb_dz(k, j, sz);
-#pragma unroll
-            for (int f = 0; f < 4; ++f) {
-                dzz[f].x += c * d1[(f + 1) * N + j].x;
-                dzz[f].y += c * d1[(f + 1) * N + j].y;
-            }
-        }
+        const double2 dzz[4] = {d2[k], d2[N + k], d2[2 * N + k], d2[3 * N + k]};  // Dzz of b, ux, uy, uz
         const double dN = cheb_dz(k, N - 1, sz);
         const double2 P = v[k], Bv = v[N + k], Ux = v[2 * N + k], Uy = v[3 * N + k], Uz = v[4 * N + k];
         const double2 DP = d1[k], DUz = d1[4 * N + k];

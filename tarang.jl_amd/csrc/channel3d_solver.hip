@@ -573,7 +573,7 @@ void Channel3Solver::lx_t(int st) {
     constexpr int NT = 256;
     Ch3Geom geo{d_.nx, d_.ny, nxh_, N, nmodes_, 2.0 * M_PI / d_.Lx, 2.0 * M_PI / d_.Ly};
     timed(7, [&] {
-        hipLaunchKernelGGL((k3_lx<T, NT>), dim3(nmodes_), dim3(NT), (size_t)10 * N * sizeof(double2), ctx_->stream,
+        hipLaunchKernelGGL((k3_lx<T, NT>), dim3(nmodes_), dim3(NT), (size_t)14 * N * sizeof(double2), ctx_->stream,
                            P_.as<const V>(), B_.as<const V>(), UX_.as<const V>(), UY_.as<const V>(), UZ_.as<const V>(),
                            TAU_.as<const V>(), geo, 2.0 / d_.Lz, d_.kappa, d_.nu, LX_[st][0].as<V>(),
                            LX_[st][1].as<V>(), LX_[st][2].as<V>(), LX_[st][3].as<V>(), LX_[st][4].as<V>());

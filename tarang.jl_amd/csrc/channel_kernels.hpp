@@ -558,13 +558,46 @@ __global__ void k_ch_zero(cplx<T>* a, size_t n) {
         a[i] = cplx<T>{T(0), T(0)};
 }
 
+// (Dz f)_k = sum_{j > k, j - k odd} 2 j sz f_j, halved at k = 0 (the closed form of the
+// descending Chebyshev recurrence u_k = u_{k+2} + 2 (k+1) sz f_{k+1}), for one real sequence
+// f_j = in[j st], j < N, into out[k st], by one whole wavefront: each lane sums a run of
+// ceil(N/64) consecutive j by parity, a shuffle scan gives every lane the sums past its run,
+// and the lane then walks its run downwards.
+__device__ inline void cheb_dz_wave(const double* in, double* out, int st, int N, double sz) {
+    const int lane = threadIdx.x & 63, PL = (N + 63) / 64, j0 = lane * PL;
+    double se = 0.0, so = 0.0;  // this lane's sums of 2 j sz f_j over even / odd j
+    for (int i = 0; i < PL; ++i) {
+        const int j = j0 + i;
+        if (j < N) {
+            const double w = 2.0 * j * sz * in[(size_t)j * st];
+            if (j & 1) so += w;
+            else se += w;
+        }
+    }
+    double te = se, to = so;  // inclusive suffix sums over lanes >= lane
+    for (int d = 1; d < 64; d <<= 1) {
+        const double ne = __shfl_down(te, d), no = __shfl_down(to, d);
+        if (lane + d < 64) {
+            te += ne;
+            to += no;
+        }
+    }
+    double ae = te - se, ao = to - so;  // sums over j past this lane's run
+    for (int i = PL - 1; i >= 0; --i) {
+        const int k = j0 + i;
+        if (k < N) {
+            const double u = (k & 1) ? ae : ao;  // j > k of the other parity
+            out[(size_t)k * st] = k == 0 ? 0.5 * u : u;
+            const double w = 2.0 * k * sz * in[(size_t)k * st];
+            if (k & 1) ao += w;
+            else ae += w;
+        }
+    }
+}
+
 // L * P(x) per mode (one workgroup per mode), P = real part at kx = 0, Nx/2.
 // taus[m*8 + t]: 0 tau_p, 1 tau_b1, 2 tau_b2, 3 tau_u1x, 4 tau_u1z, 5 tau_u2x, 6 tau_u2z
-// The z derivatives are the suffix sums (Dz f)_k = sum_{j > k, j - k odd} 2 j sz f_j (halved
-// at k = 0), the closed form of the descending recurrence u_k = u_{k+2} + 2 (k+1) sz f_{k+1}:
-// one wavefront per (field, real / imaginary part), each lane summing a run of ceil(N/64)
-// consecutive j by parity, a suffix scan of the lane sums across the wavefront, then each lane
-// walking its run downwards; Dzz = Dz(Dz .).
+// The z derivatives come from cheb_dz_wave; Dzz = Dz(Dz .).
 template <typename T, int NT>
 __global__ __launch_bounds__(NT) void k_ch_lx(const cplx<T>* __restrict__ p, const cplx<T>* __restrict__ b,
                                               const cplx<T>* __restrict__ ux, const cplx<T>* __restrict__ uz,
@@ -587,41 +620,11 @@ __global__ __launch_bounds__(NT) void k_ch_lx(const cplx<T>* __restrict__ p, con
     }
     __syncthreads();
     // out[(f - f0) N + k] = (Dz in[f])_k for nf fields starting at f0 (real and imaginary parts
-    // interleaved)
+    // interleaved): one wavefront per (field, part)
     auto dz = [&](const double* in, double* out, int f0, int nf) {
-        const int lane = threadIdx.x & 63, PL = (N + 63) / 64, j0 = lane * PL;
         for (int q = threadIdx.x >> 6; q < 2 * nf; q += NT / 64) {
             const int f = f0 + (q >> 1), c = q & 1;
-            const double* fi = in + 2 * (size_t)f * N + c;
-            double* fo = out + 2 * (size_t)(f - f0) * N + c;
-            double se = 0.0, so = 0.0;  // this lane's sums of 2 j sz f_j over even / odd j
-            for (int i = 0; i < PL; ++i) {
-                const int j = j0 + i;
-                if (j < N) {
-                    const double w = 2.0 * j * sz * fi[2 * j];
-                    if (j & 1) so += w;
-                    else se += w;
-                }
-            }
-            double te = se, to = so;  // inclusive suffix sums over lanes >= lane
-            for (int d = 1; d < 64; d <<= 1) {
-                const double ne = __shfl_down(te, d), no = __shfl_down(to, d);
-                if (lane + d < 64) {
-                    te += ne;
-                    to += no;
-                }
-            }
-            double ae = te - se, ao = to - so;  // sums over j past this lane's run
-            for (int i = PL - 1; i >= 0; --i) {
-                const int k = j0 + i;
-                if (k < N) {
-                    const double u = (k & 1) ? ae : ao;  // j > k of the other parity
-                    fo[2 * k] = k == 0 ? 0.5 * u : u;
-                    const double w = 2.0 * k * sz * fi[2 * k];
-                    if (k & 1) ao += w;
-                    else ae += w;
-                }
-            }
+            cheb_dz_wave(in + 2 * (size_t)f * N + c, out + 2 * (size_t)(f - f0) * N + c, 2, N, sz);
         }
     };
     dz(v, d1, 0, 4);
