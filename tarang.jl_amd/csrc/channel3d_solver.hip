@@ -628,14 +628,13 @@ void Channel3Solver::launch_step_body() {
             const size_t n = field_elems();
             const unsigned nblk = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
             const DevBuf* X[4] = {&B_, &UX_, &UY_, &UZ_};
-            for (int r = 0; r < 4; r += 2)
+            for (int r = 0; r < 4; r += 2)  // the second pass also clears p and the taus
                 timed(5, [&] {
                     hipLaunchKernelGGL((k_ch_copy<T>), dim3(nblk), dim3(256), 0, ctx_->stream, X[r]->as<const V>(),
                                        X[r + 1]->as<const V>(), X[r + 1]->as<const V>(), X0_[r].as<V>(),
-                                       X0_[r + 1].as<V>(), X0_[r + 1].as<V>(), n);
+                                       X0_[r + 1].as<V>(), X0_[r + 1].as<V>(), n, r == 2 ? P_.as<V>() : nullptr,
+                                       r == 2 ? TAU_.as<V>() : nullptr, r == 2 ? (size_t)nmodes_ * kCh3TauStride : 0);
                 });
-            zero_t<T>(P_.ptr, n);
-            zero_t<T>(TAU_.ptr, (size_t)nmodes_ * kCh3TauStride);
             rhs_t<T>(Frk_[0][0].ptr, Frk_[0][1].ptr, Frk_[0][2].ptr, Frk_[0][3].ptr);
             lx_t<T>(0);
             for (int st = 1; st < tab_.stages; ++st) {

@@ -479,13 +479,20 @@ struct ChStageArgs {
     T dt;
     cplx<T>* out[4];           // rhs rows E1, E2, E3x, E3z  (or, final: b, ux, uz)
     int final_update;
+    cplx<T>* clr;              // final: p (n values) and the taus (ntau) cleared in the same pass
+    cplx<T>* taus;
+    size_t ntau;
 };
 
 // rhs = M x0 + dt sum_j (AE_j F_j - AI_j LX_j) on the four row blocks; the final
-// update applies M^+ (evolved rows only).
+// update applies M^+ (evolved rows only) and clears p and the taus.
 template <typename T>
 __global__ void k_ch_stage(ChStageArgs<T> a, size_t n) {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const size_t i0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x, step = (size_t)gridDim.x * blockDim.x;
+    if (a.taus)
+        for (size_t i = i0; i < a.ntau; i += step) a.taus[i] = cplx<T>{T(0), T(0)};
+    for (size_t i = i0; i < n; i += step) {
+        if (a.clr) a.clr[i] = cplx<T>{T(0), T(0)};
         for (int r = 0; r < 4; ++r) {
             if (a.final_update && r == 0) continue;
             cplx<T> v = r == 0 ? cplx<T>{T(0), T(0)} : a.X0[r - 1][i];
@@ -542,14 +549,20 @@ __global__ void k_ch_sbdf(ChSbdfArgs<T> a, size_t n) {
     }
 }
 
+// d_r = s_r (n values); with clr / taus given, also clears clr (n values) and taus (ntau)
 template <typename T>
 __global__ void k_ch_copy(const cplx<T>* __restrict__ s0, const cplx<T>* __restrict__ s1,
-                          const cplx<T>* __restrict__ s2, cplx<T>* d0, cplx<T>* d1, cplx<T>* d2, size_t n) {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+                          const cplx<T>* __restrict__ s2, cplx<T>* d0, cplx<T>* d1, cplx<T>* d2, size_t n,
+                          cplx<T>* __restrict__ clr = nullptr, cplx<T>* __restrict__ taus = nullptr, size_t ntau = 0) {
+    const size_t i0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x, step = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = i0; i < n; i += step) {
         d0[i] = s0[i];
         d1[i] = s1[i];
         d2[i] = s2[i];
+        if (clr) clr[i] = cplx<T>{T(0), T(0)};
     }
+    if (taus)
+        for (size_t i = i0; i < ntau; i += step) taus[i] = cplx<T>{T(0), T(0)};
 }
 
 template <typename T>

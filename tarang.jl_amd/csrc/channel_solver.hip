@@ -348,6 +348,9 @@ void ChannelSolver::stage_rhs_t(int st, bool final_update) {
         a.out[0] = B_.as<V>();
         a.out[1] = UX_.as<V>();
         a.out[2] = UZ_.as<V>();
+        a.clr = P_.as<V>();  // p = taus = 0 after the step
+        a.taus = TAU_.as<V>();
+        a.ntau = (size_t)nxh_ * 8;
     } else {
         for (int r = 0; r < 4; ++r) a.out[r] = R_[r].as<V>();
     }
@@ -459,23 +462,19 @@ void ChannelSolver::segment_t(int i) {
     if (i == 0) {
         const unsigned nblk = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
         timed(5, [&] {
+            // stage 0: x_0 = M^+ M X0 keeps b, u and clears p and the taus (same pass)
             hipLaunchKernelGGL((k_ch_copy<T>), dim3(nblk), dim3(256), 0, ctx_->stream, B_.as<const V>(),
                                UX_.as<const V>(), UZ_.as<const V>(), X0_[0].as<V>(), X0_[1].as<V>(), X0_[2].as<V>(),
-                               n);
+                               n, P_.as<V>(), TAU_.as<V>(), (size_t)nxh_ * 8);
         });
         TG_HIP(hipGetLastError());
-        // stage 0: x_0 = M^+ M X0 keeps b, u and clears p and the taus
-        zero_t<T>(P_.ptr, n);
-        zero_t<T>(TAU_.ptr, (size_t)nxh_ * 8);
     }
     rhs_t<T>(B_.ptr, UX_.ptr, UZ_.ptr, F_[i][0].ptr, F_[i][1].ptr, F_[i][2].ptr);
     lx_t<T>(i);
     if (i + 1 < tab_.stages) {
         stage_rhs_t<T>(i + 1, false);
     } else {
-        stage_rhs_t<T>(0, true);
-        zero_t<T>(P_.ptr, n);
-        zero_t<T>(TAU_.ptr, (size_t)nxh_ * 8);
+        stage_rhs_t<T>(0, true);  // also clears p and the taus
     }
 }
 
