@@ -58,6 +58,7 @@ static long line_base(const shp *s, int a, long line) {
 /* ---- per-line kernels ---- */
 static void cheb_forward_line(int n, double *x) {
     /* x: n reals in place.  REDFT00 -> /(n-1) -> halve ends -> flip odd */
+    if (n < 1) return;
     double *y = (double *)malloc(sizeof(double) * n);
     orc_dct1(n, x, y);
     const double norm = n > 1 ? 1.0 / (n - 1) : 1.0;
@@ -71,6 +72,7 @@ static void cheb_forward_line(int n, double *x) {
 
 static void cheb_backward_line(int n, double *x) {
     /* x: n coefficients in place (coeff_size == grid_size) */
+    if (n < 1) return;
     double *y = (double *)malloc(sizeof(double) * n);
     for (int k = 0; k < n; ++k) y[k] = (k & 1) ? -x[k] : x[k];
     y[0] *= 2.0;
