@@ -74,9 +74,10 @@ static void cheb_backward_line(int n, double *x) {
     /* x: n coefficients in place (coeff_size == grid_size) */
     if (n < 1) return;
     double *y = (double *)malloc(sizeof(double) * n);
-    for (int k = 0; k < n; ++k) y[k] = (k & 1) ? -x[k] : x[k];
-    y[0] *= 2.0;
-    if (n > 1) y[n - 1] *= 2.0;
+    for (int k = 0; k < n; ++k) {  /* flip odd degrees, double the end degrees */
+        const double v = (k & 1) ? -x[k] : x[k];
+        y[k] = (k == 0 || k == n - 1) ? 2.0 * v : v;
+    }
     orc_dct1(n, y, x);
     for (int k = 0; k < n; ++k) x[k] *= 0.5;
     free(y);
