@@ -66,8 +66,8 @@ def measured_traffic(name, kernel_label, dtype="float", variant=None):
 
 
 def host_threads():
-    """Threads for the CPU baseline: the cores this process may run on, at most 16 (a GPU box
-    gives each GPU a 16-core share of its host)."""
+    """Threads for the CPU baseline: the cores this process may run on, at most 16 (a host
+    shared by several GPUs gives each GPU a 16-core share)."""
     try:
         visible = len(os.sched_getaffinity(0))
     except AttributeError:

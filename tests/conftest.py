@@ -40,7 +40,7 @@ def gpu_available():
 
 @pytest.fixture(scope="session")
 def gpu():
-    """Skip-free on a GPU box: a gpu-marked test that finds no device fails."""
+    """Skip-free on a machine with a GPU: a gpu-marked test that finds no device fails."""
     import torch
     assert torch.cuda.is_available(), "gpu test run without a visible HIP device"
     from tests import gpu_helpers

@@ -10,7 +10,8 @@ What fixes each part independently of the oracle's own formulas:
   - stage loop + padded products: growth of the exact eigenvector at the rate
     exp(sigma T), second order in dt
   - the conduction state is steady; kx = 0 diffusion decays as exp(-kappa pi^2 t)
-  - the filtered system sizes (4N+7 at kx = 0, 4N+6 otherwise; a9 of the survey)
+  - the filtered system sizes (4N+7 at kx = 0, 4N+6 otherwise, the sizes of the reference's
+    per-mode subproblems)
 """
 import numpy as np
 import numpy.polynomial.chebyshev as C
