@@ -241,12 +241,15 @@ def bench_turbulence(a):
                             "algorithmic_bytes": nbytes[k],
                             "GBps": (nbytes[k] / (avg[k] * 1e-3) / 1e9) if nbytes[k] and avg[k] else None}
                for k in range(4)}
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic_turbulence_1024_f32.json")
-    if os.path.exists(tfile):
-        with open(tfile) as f:
-            tr = json.load(f)
-        traffic = tr.get(["k_ybwd", "k_xpass", "k_yfwd"][dom])
+    # the timed FP32 variant's template arguments: y kernels (tile width, workgroup size) by
+    # variant 0 (4, 256), 1 (2, 256), 2 (8, 512), 3 (4, 512); the x pass 256 or 512 threads
+    tv = tuning.get("f32", {})
+    yv = [(4, 256), (2, 256), (8, 512), (4, 512)]
+    M = 3 * N // 2
+    piece = [f"{N}, {M}, {yv[tv.get('ybwd', 0)][0]}, {yv[tv.get('ybwd', 0)][1]}",
+             f"{N}, {M}, {(256, 512)[tv.get('xpass', 0)]}",
+             f"{N}, {M}, {yv[tv.get('yfwd', 0)][0]}, {yv[tv.get('yfwd', 0)][1]}"][dom]
+    traffic, traffic_kernel = measured_traffic("traffic_turbulence_1024_f32.json", KERNELS[dom], "float", piece)
 
     for s in solvers.values():
         _lib.call("tarang_fivp_destroy", s)
@@ -291,7 +294,8 @@ def bench_turbulence(a):
                                  "after_steps": a.steps},
         "diagnostics": {"E0": E0, "Z0": Z0, "E_final_f64": Ef["f64"][0], "Z_final_f64": Ef["f64"][1]},
         "roofline": {"bound": "hbm", "kernel": KERNELS[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_kernel": traffic_kernel},
         "kernels_f32": kernels,
         "autotuned_variants": tuning,
         "cpu_baseline": cpu,
